@@ -1,0 +1,158 @@
+/*
+ * xnyz.h -- C ABI of libxnyz.so, the MI355X (gfx950) implementation of
+ * XnYZip's compression path.
+ *
+ * The reference exposes no FFI: its interface is the header-only C++ API in
+ * namespace XnYZip plus the CLI (source/main.cpp).  Each entry point below
+ * names the reference interface it replaces (paths relative to the reference
+ * repository root).  Plain pointers and sizes only; no exceptions cross the
+ * boundary: every call returns XNYZ_OK (0) or a negative XNYZ_E* code, and
+ * xnyz_last_error() returns the message of the calling thread's last error.
+ *
+ * Threading: one xnyz_ctx per (device, HIP stream); calls on distinct
+ * contexts may run concurrently from different threads.  A context owns its
+ * device and pinned host buffers and reuses them across calls.
+ */
+#ifndef XNYZ_H
+#define XNYZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XNYZ_ABI_VERSION 1
+
+enum xnyz_status {
+  XNYZ_OK = 0,
+  XNYZ_EINVAL = -1,       /* bad argument (N = 0, null pointer, bad enum)  */
+  XNYZ_EHIP = -2,         /* HIP runtime error                              */
+  XNYZ_ENOMEM = -3,       /* device or host allocation failed               */
+  XNYZ_EZSTD = -4,        /* libzstd error                                  */
+  XNYZ_EUNSUPPORTED = -5, /* input outside the implemented range (see msg) */
+  XNYZ_ECORRUPT = -6,     /* malformed compressed stream                    */
+  XNYZ_EINTERNAL = -7     /* device-side invariant violated                 */
+};
+
+/* QUANTIZER_TYPE (source/composer/utils.hpp:55-59); ADAPTIVE is not
+ * reachable from the reference CLI and is not offered. */
+enum xnyz_quantizer { XNYZ_CUBE = 0, XNYZ_TRUNCATED_OCTAHEDRON = 1 };
+
+/* How the assembled stream is wrapped.  XNYZ_FRAME_ZSTD3 is the reference's
+ * compress_u8_vector (source/composer/utils.hpp:35-53): one single-shot
+ * ZSTD_compress at level 3 -- byte-identical frames given the same libzstd.
+ * XNYZ_FRAME_ZSTD3_MT is the same level split over libzstd worker threads: a
+ * valid single frame with the content size, decodable by the reference's
+ * decompress_buffer, but not byte-identical to the single-shot frame.
+ * XNYZ_FRAME_NONE returns only the assembled (pre-zstd) stream. */
+enum xnyz_frame_mode { XNYZ_FRAME_ZSTD3 = 0, XNYZ_FRAME_ZSTD3_MT = 1, XNYZ_FRAME_NONE = 2 };
+
+/* Constructor arguments of BlockCompressor[RLE]<float>
+ * (source/composer/block_compressor.hpp:35, block_compressor_rle.hpp:34)
+ * plus the compress(points, is_hilbert) curve flag and the class choice. */
+typedef struct xnyz_opts {
+  int32_t quantizer;        /* enum xnyz_quantizer                          */
+  float l2_bound;           /* L2 error bound                               */
+  int32_t is_hilbert;       /* 0: Z-order (-z), 1: Hilbert (-h)             */
+  int32_t rle;              /* 0: BlockCompressor, 1: BlockCompressorRLE    */
+  int32_t direct_threshold; /* default 1024 (source/main.cpp:76)            */
+  int32_t frame_mode;       /* enum xnyz_frame_mode                         */
+  int32_t zstd_workers;     /* XNYZ_FRAME_ZSTD3_MT only; 0 = all host cores */
+} xnyz_opts;
+
+/* Library-owned byte buffer; release with xnyz_free(). */
+typedef struct xnyz_buf {
+  uint8_t* data;
+  uint64_t len;
+} xnyz_buf;
+
+/* What the reference prints from inside compress() (Encoding mode, Unique
+ * points, patches sizes, Max L2 error, ...), plus per-stage times. */
+typedef struct xnyz_stats {
+  uint64_t input_points;     /* N                                          */
+  uint64_t num_points;       /* N, or unique cells U in RLE mode           */
+  int32_t range[3];          /* max quantized x, y, z                      */
+  int32_t is_direct;         /* "Encoding mode: direct|block"             */
+  uint64_t blocks;           /* occupied 128^3 lattice blocks              */
+  uint64_t patches;          /* repos (or sfc-delta) PFD patches          */
+  uint64_t quad_patches;     /* quads PFD patches                          */
+  uint64_t wrapped_points;   /* points whose node z is -1                  */
+  int32_t counts_exp_golomb; /* RLE counts coded with Exp-Golomb           */
+  int32_t counts_eg_k;
+  double entropy_repos;      /* "entropy:" lines of HuffmanEncoder        */
+  double entropy_quads;
+  double entropy_counts;
+  double max_l2;             /* RLE: "Max L2 error:"                      */
+  double mse;                /* RLE: "MSE:"                                */
+  uint64_t prezstd_bytes;    /* assembled stream                           */
+  uint64_t frame_bytes;      /* returned frame                             */
+  double ms_device;          /* upload (host input) + kernels + readback   */
+  double ms_zstd;            /* frame stage on the host                    */
+  double ms_total;
+} xnyz_stats;
+
+typedef struct xnyz_ctx xnyz_ctx;
+
+/* Library version / ABI; no reference counterpart. */
+int xnyz_abi_version(void);
+const char* xnyz_last_error(void);
+
+/* Context on HIP device `device` with its own non-blocking stream.  No
+ * reference counterpart (the reference is single-threaded host code). */
+int xnyz_create(int device, xnyz_ctx** out);
+int xnyz_destroy(xnyz_ctx* ctx);
+
+/* Replaces BlockCompressor<float>::compress and BlockCompressorRLE<float>::
+ * compress (source/composer/block_compressor.hpp:39-222,
+ * block_compressor_rle.hpp:38-321) on points in HOST memory: n points of
+ * interleaved float x,y,z.  The input is const: the reference's in-place
+ * reordering of `points` is returned through `perm` instead (nullable;
+ * n entries; non-RLE only, else untouched).  `out_frame` receives the
+ * frame per opts->frame_mode; `out_prezstd` (nullable) the assembled
+ * stream.  `stats` is nullable.  N = 0 is XNYZ_EINVAL (the reference
+ * indexes an empty vector there). */
+int xnyz_compress(xnyz_ctx* ctx, const float* xyz, uint64_t n, const xnyz_opts* opts,
+                  xnyz_buf* out_frame, xnyz_buf* out_prezstd, uint32_t* perm,
+                  xnyz_stats* stats);
+
+/* Same, with the points already resident in device memory of ctx's device
+ * (d_xyz: n*3 floats, 4-byte aligned).  No reference counterpart beyond
+ * compress(); this is the entry point the throughput figures use. */
+int xnyz_compress_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
+                         const xnyz_opts* opts, xnyz_buf* out_frame,
+                         xnyz_buf* out_prezstd, uint32_t* perm, xnyz_stats* stats);
+
+/* Replaces TruncatedOctahedronQuantizer/CubeQuantizer::quantize after
+ * shift_points (source/quantizer/truncated_octahedron_quantizer.hpp:21-55,
+ * cube_quantizer.hpp:15-27, source/preprocessor/shifting.hpp:11-36) on
+ * device input: writes n int32 x,y,z lattice nodes to host `nodes` and the
+ * shift offset to host `offset[3]`. */
+int xnyz_quantize_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
+                         int32_t quantizer, float l2_bound, int32_t* nodes,
+                         float* offset);
+
+/* Replaces mve::mortonToHilbert3 (source/quantizer/hilbert-curves-cpp/
+ * hilbert.cpp:423-448) on device: out[i] = mortonToHilbert3(in[i], it),
+ * host arrays of n codes. */
+int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
+                           uint32_t iteration, uint64_t* out);
+
+/* Per-kernel device time (HIP events on the context's stream) accumulated
+ * over the calls made while enabled; enabling clears the totals.  The report
+ * is a JSON object {"kernel": {"calls": c, "ms": t}, ...} written to buf
+ * (NUL-terminated, truncated to cap); *needed gets the full size.  No
+ * reference counterpart (the reference's only timer is source/statistics/
+ * timer.hpp around the whole compress call). */
+int xnyz_profile_enable(xnyz_ctx* ctx, int on);
+int xnyz_profile_report(xnyz_ctx* ctx, char* buf, uint64_t cap, uint64_t* needed);
+
+/* Releases a buffer returned in an xnyz_buf (also for the reference's
+ * returned std::vector, which owned its bytes). */
+void xnyz_free(xnyz_buf* buf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

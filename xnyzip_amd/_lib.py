@@ -1,0 +1,136 @@
+"""ctypes binding of libxnyz.so (include/xnyz.h).
+
+The library is built in-tree by `python -m xnyzip_amd.build`.  There is no
+fallback: if the library or a HIP device is missing, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libxnyz.so")
+
+XNYZ_OK = 0
+CUBE, TRUNCATED_OCTAHEDRON = 0, 1
+FRAME_ZSTD3, FRAME_ZSTD3_MT, FRAME_NONE = 0, 1, 2
+
+
+class XnyzError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"xnyz error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("quantizer", C.c_int32), ("l2_bound", C.c_float), ("is_hilbert", C.c_int32),
+                ("rle", C.c_int32), ("direct_threshold", C.c_int32),
+                ("frame_mode", C.c_int32), ("zstd_workers", C.c_int32)]
+
+
+class Buf(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("len", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("input_points", C.c_uint64), ("num_points", C.c_uint64),
+                ("range", C.c_int32 * 3), ("is_direct", C.c_int32), ("blocks", C.c_uint64),
+                ("patches", C.c_uint64), ("quad_patches", C.c_uint64),
+                ("wrapped_points", C.c_uint64), ("counts_exp_golomb", C.c_int32),
+                ("counts_eg_k", C.c_int32), ("entropy_repos", C.c_double),
+                ("entropy_quads", C.c_double), ("entropy_counts", C.c_double),
+                ("max_l2", C.c_double), ("mse", C.c_double), ("prezstd_bytes", C.c_uint64),
+                ("frame_bytes", C.c_uint64), ("ms_device", C.c_double),
+                ("ms_zstd", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if name == "range" else v
+        return out
+
+
+_lib = None
+_lock = threading.Lock()
+
+EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
+           "xnyz_compress", "xnyz_compress_device", "xnyz_quantize_device",
+           "xnyz_morton_to_hilbert", "xnyz_profile_enable", "xnyz_profile_report",
+           "xnyz_free"]
+
+
+def lib() -> C.CDLL:
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m xnyzip_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        P, PP = C.c_void_p, C.POINTER(C.c_void_p)
+        L.xnyz_abi_version.restype = C.c_int
+        L.xnyz_last_error.restype = C.c_char_p
+        L.xnyz_create.argtypes = [C.c_int, PP]
+        L.xnyz_destroy.argtypes = [P]
+        comp = [P, P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf), C.POINTER(Buf), P,
+                C.POINTER(Stats)]
+        L.xnyz_compress.argtypes = comp
+        L.xnyz_compress_device.argtypes = comp
+        L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
+        L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
+        L.xnyz_profile_enable.argtypes = [P, C.c_int]
+        L.xnyz_profile_report.argtypes = [P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.xnyz_free.argtypes = [C.POINTER(Buf)]
+        _lib = L
+        return L
+
+
+def check(rc: int) -> None:
+    if rc != XNYZ_OK:
+        raise XnyzError(rc, lib().xnyz_last_error().decode())
+
+
+def take(buf: Buf) -> bytes:
+    if not buf.data:
+        return b""
+    out = C.string_at(buf.data, buf.len)
+    lib().xnyz_free(C.byref(buf))
+    return out
+
+
+class Context:
+    """One xnyz_ctx: a HIP device, its stream, and pooled buffers."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        check(lib().xnyz_create(device, C.byref(self._h)))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().xnyz_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def profile(self, on: bool) -> None:
+        """Enable (and clear) or disable per-kernel event timing."""
+        check(lib().xnyz_profile_enable(self._h, int(on)))
+
+    def profile_report(self) -> dict:
+        import json
+        need = C.c_uint64()
+        check(lib().xnyz_profile_report(self._h, None, 0, C.byref(need)))
+        buf = C.create_string_buffer(need.value + 16)
+        check(lib().xnyz_profile_report(self._h, buf, len(buf), C.byref(need)))
+        return json.loads(buf.value.decode())
