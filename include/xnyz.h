@@ -124,6 +124,18 @@ int xnyz_compress_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
                          const xnyz_opts* opts, xnyz_buf* out_frame,
                          xnyz_buf* out_prezstd, uint32_t* perm, xnyz_stats* stats);
 
+/* Many clouds in one launch set.  Replaces the MPI driver's per-block loop
+ * (source/mpi_driver.cpp:819-943: one compress() per block, blocks spread
+ * over ranks) on one GPU.  Cloud s is points [offsets[s], offsets[s+1]) of
+ * d_xyz (device memory, interleaved x,y,z floats), compressed with opts[s]
+ * -- each cloud its own quantizer, bound, curve, RLE flag and frame mode --
+ * into frames[s]; prezstd and stats are nullable arrays of `count` entries.
+ * Every cloud's stream is the one xnyz_compress_device would produce for it
+ * alone.  No permutation output. */
+int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                               uint32_t count, const xnyz_opts* opts, xnyz_buf* frames,
+                               xnyz_buf* prezstd, xnyz_stats* stats);
+
 /* Replaces TruncatedOctahedronQuantizer/CubeQuantizer::quantize after
  * shift_points (source/quantizer/truncated_octahedron_quantizer.hpp:21-55,
  * cube_quantizer.hpp:15-27, source/preprocessor/shifting.hpp:11-36) on
@@ -138,6 +150,17 @@ int xnyz_quantize_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
  * host arrays of n codes. */
 int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
                            uint32_t iteration, uint64_t* out);
+
+/* Replaces BlockDecompressor<float>::decompress and BlockDecompressorRLE<
+ * float>::decompress (source/composer/block_decompressor.hpp:31-192,
+ * block_decompressor_rle.hpp:32-207).  As there, the caller supplies what
+ * the stream does not record: opts->quantizer, l2_bound and rle;
+ * opts->frame_mode XNYZ_FRAME_NONE means `in` is an assembled (pre-zstd)
+ * stream, anything else a zstd frame.  `out_points` receives n*3 floats
+ * (len = 12*n bytes) in stream order.  Runs on the host: the entropy-coded
+ * streams have no restart points.  No device or context needed. */
+int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
+                    xnyz_buf* out_points);
 
 /* Per-kernel device time (HIP events on the context's stream) accumulated
  * over the calls made while enabled; enabling clears the totals.  The report

@@ -1,0 +1,100 @@
+"""Batched compression: many clouds of uneven size and mixed options in one
+launch set.  Every cloud's assembled stream and frame must equal the oracle's
+for that cloud compressed alone."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def X():
+    import xnyzip_amd
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    return xnyzip_amd
+
+
+def _uniform(seed, n, scale=100.0):
+    rng = np.random.default_rng(seed)
+    return (rng.random((n, 3)) * scale).astype(np.float32)
+
+
+def _clouds():
+    d = float(np.float32(np.float64(np.float32(1.0)) / np.sqrt(5.0)))
+    wrapped = _uniform(40, 3000, scale=3000 * d)
+    wrapped[:, 2] += 1.0
+    wrapped[5] = [d, d, 1.0]
+    wrapped[6] = [0.0, 0.0, 3.0]
+    dup = np.round(_uniform(41, 20000) / 5, 1).astype(np.float32)
+    cases = [
+        (_uniform(1, 20000), dict(quantizer=O.TO, l2_bound=0.01, is_hilbert=True, rle=True)),
+        (_uniform(2, 1), dict(quantizer=O.CUBE, l2_bound=0.01, is_hilbert=False, rle=False)),
+        (_uniform(3, 9000), dict(quantizer=O.CUBE, l2_bound=0.3, is_hilbert=True, rle=False,
+                                 direct_threshold=1 << 20)),
+        (dup, dict(quantizer=O.TO, l2_bound=0.02, is_hilbert=False, rle=True)),
+        (wrapped, dict(quantizer=O.TO, l2_bound=0.5, is_hilbert=True, rle=False,
+                       direct_threshold=0)),
+        (_uniform(4, 4096), dict(quantizer=O.TO, l2_bound=0.05, is_hilbert=False, rle=True,
+                                 direct_threshold=1 << 20)),
+        (_uniform(5, 70000, 20000.0), dict(quantizer=O.TO, l2_bound=0.001, is_hilbert=True,
+                                           rle=False)),
+        (_uniform(6, 12345), dict(quantizer=O.CUBE, l2_bound=0.01, is_hilbert=False, rle=True)),
+    ]
+    return cases
+
+
+def _check_batch(X, cases, frame_mode=0):
+    pts = np.concatenate([c for c, _ in cases])
+    offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in cases])]).tolist()
+    opts = [dict(o, frame_mode=frame_mode) for _, o in cases]
+    frames, pres, stats = X.compress_batch(torch.from_numpy(pts).cuda(), offs, opts,
+                                           want_prezstd=True)
+    for i, (c, o) in enumerate(cases):
+        ref = O.compress(c, o["quantizer"], o["l2_bound"], o["is_hilbert"], o["rle"],
+                         o.get("direct_threshold", 1024))
+        assert pres[i] == ref.pre, f"cloud {i}: assembled stream differs"
+        if frame_mode == 0:
+            assert frames[i] == ref.frame, f"cloud {i}: frame differs"
+        else:
+            back = O.decompress_frame(frames[i], o["quantizer"], o["l2_bound"], o["rle"])
+            assert np.array_equal(back, O.decompress_buffer(ref.pre, o["quantizer"],
+                                                            o["l2_bound"], o["rle"]))
+        assert stats[i].input_points == len(c)
+    return frames
+
+
+def test_mixed_batch(X):
+    _check_batch(X, _clouds())
+
+
+def test_batch_all_plain_and_all_rle(X):
+    base = _clouds()
+    _check_batch(X, [(c, dict(o, rle=False)) for c, o in base])
+    _check_batch(X, [(c, dict(o, rle=True)) for c, o in base])
+
+
+def test_many_small_clouds(X):
+    rng = np.random.default_rng(7)
+    cases = []
+    for i in range(300):
+        n = int(rng.integers(1, 3000))
+        cases.append((_uniform(100 + i, n, scale=float(rng.uniform(1, 200))),
+                      dict(quantizer=int(i % 2), l2_bound=[0.001, 0.01, 0.1][i % 3],
+                           is_hilbert=bool(i % 5 < 2), rle=bool(i % 7 < 4))))
+    _check_batch(X, cases)
+
+
+def test_chunked_frames_decode(X):
+    """The host-threaded frame mode writes one valid zstd frame per cloud."""
+    big = _uniform(9, 600000)
+    frame, pre, _, _ = X.compress(torch.from_numpy(big).cuda(), O.TO, 0.01, True, True,
+                                  frame_mode=1, want_prezstd=True)
+    ref = O.compress(big, O.TO, 0.01, True, True)
+    assert pre == ref.pre and len(pre) > 2 * (1 << 20)
+    assert np.array_equal(O.decompress_frame(frame, O.TO, 0.01, True),
+                          O.decompress_frame(ref.frame, O.TO, 0.01, True))
+    _check_batch(X, _clouds()[:3], frame_mode=1)

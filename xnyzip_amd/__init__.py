@@ -21,8 +21,9 @@ import numpy as np
 from . import _lib
 from ._lib import Context, XnyzError  # noqa: F401
 
-__all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "Context", "XnyzError",
-           "compress", "quantize", "morton_to_hilbert"]
+__all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
+           "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
+           "compress", "compress_batch", "quantize", "morton_to_hilbert"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -95,6 +96,41 @@ def compress(points, quantizer=QUANTIZER_TYPE.TRUNCATED_OCTAHEDRON, l2_bound=0.0
     return frame, prez, perm, st
 
 
+def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
+                   ctx: Context | None = None):
+    """Compress many clouds in one launch set.
+
+    points_device: contiguous float32 (N, 3) tensor on the GPU holding every
+    cloud back to back; offsets: N+1-style list of cloud starts (len = count+1);
+    opts: one dict per cloud with keys quantizer, l2_bound, is_hilbert, rle and
+    optionally direct_threshold, frame_mode.  Returns (frames, prezstd, stats)."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    count = len(offsets) - 1
+    if count <= 0 or len(opts) != count:
+        raise ValueError("need one opts entry per cloud")
+    ctx = ctx or default_context(t.device.index or 0)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    arr = (_lib.Opts * count)()
+    for i, o in enumerate(opts):
+        arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
+                       o.get("rle", False), o.get("direct_threshold", 1024),
+                       o.get("frame_mode", _lib.FRAME_ZSTD3), o.get("zstd_workers", 0))
+    frames = (_lib.Buf * count)()
+    pres = (_lib.Buf * count)() if want_prezstd else None
+    stats = (_lib.Stats * count)()
+    torch.cuda.synchronize(t.device)
+    _lib.check(_lib.lib().xnyz_compress_batch_device(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.cast(frames, C.c_void_p),
+        C.cast(pres, C.c_void_p) if pres is not None else None, C.cast(stats, C.c_void_p)))
+    out_frames = [_lib.take(f) for f in frames]
+    out_pres = [_lib.take(p) for p in pres] if pres is not None else None
+    return out_frames, out_pres, list(stats)
+
+
 class BlockCompressor:
     """BlockCompressor<float> (source/composer/block_compressor.hpp:30-222)."""
 
@@ -118,6 +154,38 @@ class BlockCompressor:
 
 class BlockCompressorRLE(BlockCompressor):
     """BlockCompressorRLE<float> (source/composer/block_compressor_rle.hpp:29-321)."""
+
+    rle = True
+
+
+def decompress(data: bytes, quantizer, l2_bound: float, rle: bool,
+               is_frame: bool = True) -> np.ndarray:
+    """BlockDecompressor[RLE]<float>::decompress: a zstd frame (or, with
+    is_frame=False, an assembled stream) -> (N, 3) float32 points."""
+    o = _opts(quantizer, l2_bound, False, rle, 1024,
+              _lib.FRAME_ZSTD3 if is_frame else _lib.FRAME_NONE, 0)
+    out = _lib.Buf()
+    src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    _lib.check(_lib.lib().xnyz_decompress(C.c_void_p(src.ctypes.data), len(data), C.byref(o),
+                                           C.byref(out)))
+    return np.frombuffer(_lib.take(out), dtype=np.float32).reshape(-1, 3).copy()
+
+
+class BlockDecompressor:
+    """BlockDecompressor<float> (source/composer/block_decompressor.hpp:24-192)."""
+
+    rle = False
+
+    def __init__(self, quantizer_type, l2_bound: float, is_debugging: bool = False):
+        self.quantizer_type = QUANTIZER_TYPE(quantizer_type)
+        self.l2_bound = float(np.float32(l2_bound))
+
+    def decompress(self, compressed: bytes) -> np.ndarray:
+        return decompress(compressed, self.quantizer_type, self.l2_bound, self.rle)
+
+
+class BlockDecompressorRLE(BlockDecompressor):
+    """BlockDecompressorRLE<float> (source/composer/block_decompressor_rle.hpp:25-207)."""
 
     rle = True
 

@@ -56,9 +56,10 @@ _lib = None
 _lock = threading.Lock()
 
 EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
-           "xnyz_compress", "xnyz_compress_device", "xnyz_quantize_device",
-           "xnyz_morton_to_hilbert", "xnyz_profile_enable", "xnyz_profile_report",
-           "xnyz_free"]
+           "xnyz_compress", "xnyz_compress_device", "xnyz_compress_batch_device",
+           "xnyz_quantize_device",
+           "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_profile_enable",
+           "xnyz_profile_report", "xnyz_free"]
 
 
 def lib() -> C.CDLL:
@@ -78,8 +79,10 @@ def lib() -> C.CDLL:
                 C.POINTER(Stats)]
         L.xnyz_compress.argtypes = comp
         L.xnyz_compress_device.argtypes = comp
+        L.xnyz_compress_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, P, P]
         L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
+        L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]
         L.xnyz_profile_enable.argtypes = [P, C.c_int]
         L.xnyz_profile_report.argtypes = [P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.xnyz_free.argtypes = [C.POINTER(Buf)]

@@ -1,28 +1,32 @@
-// api.cpp -- C ABI of libxnyz.so: context, the compression pipeline on one
-// HIP stream, stream assembly and the zstd frame stage.
+// api.cpp -- C ABI of libxnyz.so: context, the segmented compression pipeline
+// on one HIP stream, stream assembly and the zstd frame stage.
 //
-// Pipeline (reference: BlockCompressor[RLE]<float>::compress,
-// source/composer/block_compressor.hpp:39-222, block_compressor_rle.hpp:38-321):
+// One call compresses S clouds ("segments"; S = 1 for xnyz_compress) with one
+// launch set per stage.  Pipeline (reference: BlockCompressor[RLE]<float>::
+// compress, source/composer/block_compressor.hpp:39-222,
+// block_compressor_rle.hpp:38-321):
 //   device  bbox minimum -> offsets -> quantize + range (+ RLE diagnostic)
 //   host    sync 1: ranges decide block vs direct mode and the key layout
-//   device  keygen -> LSD radix sort -> [RLE cells + counts] -> block runs
-//           -> symbol histograms / first occurrences, patch count and sum
+//   device  keygen -> segmented LSD radix sort -> RLE cells + counts ->
+//           block runs -> symbol histograms / first occurrences, patches
 //   host    sync 2: Huffman / Exp-Golomb codebooks, Golomb-Rice k, widths
-//   device  per-tile bit counts of the variable-length sections
-//   host    sync 3: section sizes -> byte layout of the assembled stream
-//   device  zero fill, bit packers write every data section in place
-//   host    sync 4: copy back, write header / size fields / metas / trailer,
+//   device  per-tile bit counts of every section -> per-segment sizes
+//   host    sync 3: byte layout of every segment's assembled stream
+//   device  zero fill, every section written in place
+//   host    sync 4: copy back, header / size fields / metas / trailer,
 //           zstd level 3 (source/composer/utils.hpp:35-53)
 #include <hip/hip_runtime.h>
+#define ZSTD_STATIC_LINKING_ONLY  // block-level API (ZSTD_compressBlock)
 #include <zstd.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -32,9 +36,16 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+// Exported by libzstd 1.4 (zstd_compress.c) though absent from its public
+// header: makes the next block start without repeat offsets, so a chunk
+// compressed on its own may follow any other chunk in one frame (zstd's own
+// multithreaded compressor does the same between jobs).
+extern "C" void ZSTD_invalidateRepCodes(ZSTD_CCtx* cctx);
+
 namespace xnyz {
 HilbertSM build_hilbert_sm();
-}
+int decompress_impl(const uint8_t* frame, uint64_t len, const xnyz_opts* o, xnyz_buf* out);
+}  // namespace xnyz
 
 using namespace xnyz;
 
@@ -101,19 +112,41 @@ struct HBuf {  // pinned host memory
   }
 };
 
-// Codebook tables uploaded to the device after sync 2.
-struct LutBlock {
-  HufEntry sym[kSymBins];
-  HufEntry quad[kSymBins];
-  HufEntry cnt[kCntBins];
-};
-
 u32 bitlen(u64 v) { return v ? 64u - (u32)__builtin_clzll(v) : 0u; }
 
 double now_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
       .count();
+}
+
+int host_threads(int want) {
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  return want > 0 ? std::min(want, 256) : hw;
+}
+
+// Runs f(i) for i in [0, n) on up to `threads` host threads.
+void parallel_for(size_t n, int threads, const std::function<void(size_t)>& f) {
+  if (n == 0) return;
+  const size_t t = std::min<size_t>(n, (size_t)std::max(1, threads));
+  if (t == 1) {
+    for (size_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<size_t> next{0};
+  std::vector<std::exception_ptr> errs(t);
+  std::vector<std::thread> pool;
+  for (size_t w = 0; w < t; ++w)
+    pool.emplace_back([&, w] {
+      try {
+        for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+      } catch (...) {
+        errs[w] = std::current_exception();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
 }
 
 // Per-kernel device time from HIP events recorded on the library's stream.
@@ -170,9 +203,12 @@ struct xnyz_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   HilbertSM hs{};
-  DBuf ctl, keys[2], lo[2], pos, pos2, large, lut, bigv, bigc, out, arena, upload, tmp;
+  // device buffers
+  DBuf tiles, sp, sc, sh, lut, bigv, bigc, large, nlarge;
+  DBuf keys[2], lo[2], cells, cells_lo, pos, cnt, out, arena, upload, tmp;
   DBuf idx[2], lo64[2];  // permutation payload
-  HBuf h_ctl, h_out, h_lut;
+  // pinned host mirrors
+  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out;
   bool prof_on = false;
   EventProfiler prof;
 
@@ -186,50 +222,48 @@ struct xnyz_ctx {
   void copied() {  // after enqueueing a copy: the next kernel starts a new interval
     if (g_profiler) prof.reset(stream);
   }
-  void fetch_ctl(size_t bytes) {
-    check_hip(hipMemcpyAsync(h_ctl.p, ctl.p, bytes, hipMemcpyDeviceToHost, stream), "ctl D2H");
-    sync();
+  void h2d(DBuf& d, const HBuf& h, size_t bytes) {
+    check_hip(hipMemcpyAsync(d.p, h.p, bytes, hipMemcpyHostToDevice, stream), "H2D");
+    copied();
   }
-  const DevCtl& hc() const { return *h_ctl.as<DevCtl>(); }
+  void d2h(HBuf& h, const DBuf& d, size_t bytes) {
+    check_hip(hipMemcpyAsync(h.p, d.p, bytes, hipMemcpyDeviceToHost, stream), "D2H");
+    copied();
+  }
 };
 
 namespace {
 
-struct Section {  // one [u64 data_size][u64 meta_size][meta][data] record
-  std::vector<uint8_t> meta;
-  u64 data_bytes = 0;
-  u64 at = 0;       // byte offset of the size fields
-  u64 data_at = 0;  // byte offset of the data
+struct ProfGuard {  // installs the context's profiler for one call
+  xnyz_ctx* c;
+  explicit ProfGuard(xnyz_ctx* ctx) : c(ctx) {
+    if (c->prof_on) {
+      g_profiler = &c->prof;
+      c->prof.reset(c->stream);
+    }
+  }
+  ~ProfGuard() {
+    if (g_profiler == &c->prof) {
+      c->prof.marks.clear();
+      c->prof.used = 0;
+      g_profiler = nullptr;
+    }
+  }
 };
 
-void put_bytes(uint8_t* dst, const void* src, size_t n) { std::memcpy(dst, src, n); }
+// ---------------------------------------------------------------------------
+// Frame stage (source/composer/utils.hpp:35-53: ZSTD_compress at level 3).
 
-// zstd level 3 over the assembled stream (source/composer/utils.hpp:35-53).
-void zstd_frame(const uint8_t* src, u64 len, int mode, int workers, xnyz_buf* out) {
+uint8_t* xmalloc(size_t n) {
+  uint8_t* p = static_cast<uint8_t*>(std::malloc(n ? n : 1));
+  if (!p) throw Error(XNYZ_ENOMEM, "malloc failed");
+  return p;
+}
+
+void zstd_single(const uint8_t* src, u64 len, xnyz_buf* out) {
   const size_t bound = ZSTD_compressBound(len);
-  uint8_t* dst = static_cast<uint8_t*>(std::malloc(bound ? bound : 1));
-  if (!dst) throw Error(XNYZ_ENOMEM, "malloc of the frame buffer failed");
-  size_t r;
-  if (mode == XNYZ_FRAME_ZSTD3) {
-    r = ZSTD_compress(dst, bound, src, len, 3);
-  } else {
-    ZSTD_CCtx* cc = ZSTD_createCCtx();
-    if (!cc) {
-      std::free(dst);
-      throw Error(XNYZ_ENOMEM, "ZSTD_createCCtx failed");
-    }
-    int nw = workers > 0 ? workers : (int)std::max(1u, std::thread::hardware_concurrency());
-    ZSTD_CCtx_setParameter(cc, ZSTD_c_compressionLevel, 3);
-    ZSTD_CCtx_setParameter(cc, ZSTD_c_contentSizeFlag, 1);
-    size_t e = ZSTD_CCtx_setParameter(cc, ZSTD_c_nbWorkers, nw);
-    if (ZSTD_isError(e)) {
-      ZSTD_freeCCtx(cc);
-      std::free(dst);
-      throw Error(XNYZ_EZSTD, std::string("zstd workers unavailable: ") + ZSTD_getErrorName(e));
-    }
-    r = ZSTD_compress2(cc, dst, bound, src, len);
-    ZSTD_freeCCtx(cc);
-  }
+  uint8_t* dst = xmalloc(bound);
+  const size_t r = ZSTD_compress(dst, bound, src, len, 3);
   if (ZSTD_isError(r)) {
     std::free(dst);
     throw Error(XNYZ_EZSTD, ZSTD_getErrorName(r));
@@ -237,6 +271,100 @@ void zstd_frame(const uint8_t* src, u64 len, int mode, int workers, xnyz_buf* ou
   out->data = dst;
   out->len = r;
 }
+
+// One zstd frame, level 3, whose 1 MiB chunks are compressed on host threads
+// with the block-level API: single-segment frame header with the content
+// size, then each chunk's blocks (compressed, or raw when zstd declines).
+// Chunks after the first start without repeat offsets, so every chunk
+// decodes after any other.  Valid for every zstd decoder; not byte-identical
+// to the single-shot frame.
+void zstd_chunked(const uint8_t* src, u64 len, int threads, xnyz_buf* out) {
+  constexpr size_t kChunk = 1u << 20;
+  const size_t nchunks = (len + kChunk - 1) / kChunk;
+  if (nchunks <= 1 || threads <= 1) return zstd_single(src, len, out);
+  std::vector<std::vector<uint8_t>> parts(nchunks);
+  const int t = std::min<int>(threads, (int)nchunks);
+  std::vector<ZSTD_CCtx*> cctx(t, nullptr);
+  for (auto& c : cctx)
+    if (!(c = ZSTD_createCCtx())) throw Error(XNYZ_ENOMEM, "ZSTD_createCCtx failed");
+  std::atomic<size_t> next{0};
+  std::vector<std::string> err(t);
+  std::vector<std::thread> pool;
+  for (int w = 0; w < t; ++w)
+    pool.emplace_back([&, w] {
+      ZSTD_CCtx* cc = cctx[w];
+      for (size_t c; (c = next.fetch_add(1)) < nchunks;) {
+        const size_t off = c * kChunk, n = std::min<size_t>(kChunk, len - off);
+        size_t r = ZSTD_compressBegin(cc, 3);
+        if (ZSTD_isError(r)) {
+          err[w] = ZSTD_getErrorName(r);
+          return;
+        }
+        if (c) ZSTD_invalidateRepCodes(cc);
+        const size_t bs = ZSTD_getBlockSize(cc);
+        std::vector<uint8_t>& o = parts[c];
+        o.resize(ZSTD_compressBound(n) + 3 * (n / bs + 2));
+        size_t at = 0;
+        for (size_t b = 0; b < n; b += bs) {
+          const size_t m = std::min(bs, n - b);
+          const bool last = (c + 1 == nchunks) && (b + m == n);
+          r = ZSTD_compressBlock(cc, o.data() + at + 3, o.size() - at - 3, src + off + b, m);
+          if (ZSTD_isError(r)) {
+            err[w] = ZSTD_getErrorName(r);
+            return;
+          }
+          u32 type = 2, size = (u32)r;
+          if (r == 0 || r >= m) {  // stored as a raw block
+            type = 0;
+            size = (u32)m;
+            std::memcpy(o.data() + at + 3, src + off + b, m);
+          }
+          const u32 hdr = (last ? 1u : 0u) | (type << 1) | (size << 3);
+          o[at] = uint8_t(hdr);
+          o[at + 1] = uint8_t(hdr >> 8);
+          o[at + 2] = uint8_t(hdr >> 16);
+          at += 3 + size;
+        }
+        o.resize(at);
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto c : cctx) ZSTD_freeCCtx(c);
+  for (auto& e : err)
+    if (!e.empty()) throw Error(XNYZ_EZSTD, e);
+  // magic, frame header descriptor (8-byte content size, single segment),
+  // content size
+  size_t total = 13;
+  for (auto& p : parts) total += p.size();
+  uint8_t* dst = xmalloc(total);
+  const uint32_t magic = 0xFD2FB528u;
+  std::memcpy(dst, &magic, 4);
+  dst[4] = 0xE0;
+  const uint64_t fcs = len;
+  std::memcpy(dst + 5, &fcs, 8);
+  size_t at = 13;
+  for (auto& p : parts) {
+    std::memcpy(dst + at, p.data(), p.size());
+    at += p.size();
+  }
+  out->data = dst;
+  out->len = total;
+}
+
+void make_frame(const uint8_t* src, u64 len, const xnyz_opts& o, int threads, xnyz_buf* out) {
+  if (o.frame_mode == XNYZ_FRAME_NONE) {
+    out->data = xmalloc(len);
+    std::memcpy(out->data, src, len);
+    out->len = len;
+  } else if (o.frame_mode == XNYZ_FRAME_ZSTD3_MT) {
+    zstd_chunked(src, len, host_threads(o.zstd_workers > 0 ? o.zstd_workers : threads), out);
+  } else {
+    zstd_single(src, len, out);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Validation and per-segment setup
 
 void validate(const xnyz_opts* o, u64 n) {
   if (!o) throw Error(XNYZ_EINVAL, "opts is null");
@@ -277,385 +405,465 @@ void fill_lut(const Codebook<T>& cb, HufEntry* lut, u32 bins) {
   }
 }
 
-struct ProfGuard {  // installs the context's profiler for one call
-  xnyz_ctx* c;
-  explicit ProfGuard(xnyz_ctx* ctx) : c(ctx) {
-    if (c->prof_on) {
-      g_profiler = &c->prof;
-      c->prof.reset(c->stream);
-    }
-  }
-  ~ProfGuard() {
-    if (g_profiler == &c->prof) {
-      c->prof.marks.clear();
-      c->prof.used = 0;
-      g_profiler = nullptr;
-    }
-  }
+// Host-side state of one segment across the stages.
+struct SegHost {
+  u64 n = 0;
+  bool rle = false, hil = false, direct = false;
+  int rx = 0, ry = 0, rz = 0;
+  u32 W = 0;
+  float off[3] = {0, 0, 0};
+  u32 key_bits = 0, hi_bits = 0;
+  bool wide_needed = false;
+  double max_l2 = 0, mse = 0;
+  u64 ncoded = 0, B = 0, P = 0;
+  Codebook<uint64_t> cb_sym;
+  Codebook<uint8_t> cb_quad;
+  Codebook<int> cb_cnt;
+  std::vector<u32> big_vals;
+  std::vector<HufEntry> big_codes;
+  std::vector<std::pair<u32, u32>> large;  // (count, first cell index)
+  uint8_t grk = 0;
+  u32 w1 = 0, w2 = 0;
+  std::vector<std::vector<uint8_t>> metas;  // per section in stream order
+  u64 base = 0, total = 0;                  // stream bytes in the output buffer
 };
 
-int compress_impl(xnyz_ctx* ctx, const float* d_xyz, u64 n64, const xnyz_opts* o,
-                  xnyz_buf* out_frame, xnyz_buf* out_pre, uint32_t* perm, xnyz_stats* st,
-                  double t_start) {
-  validate(o, n64);
-  const u32 N = (u32)n64;
-  const bool rle = o->rle != 0;
-  const bool hil = o->is_hilbert != 0;
-  hipStream_t s = ctx->stream;
+// Layout of one stream: section records in stream order.
+struct Rec {
+  int kind;  // SecKind, or -1 for the always-empty quads patch section
+  u64 data_bytes;
+};
+
+std::vector<Rec> records(const SegHost& g, const SegCtl& c) {
+  std::vector<Rec> r;
+  auto bytes = [](u64 bits) { return (bits + 7) / 8; };
+  if (!g.direct) {
+    r.push_back({kSecBlk, bytes(c.sec_bits[kSecBlk])});
+    r.push_back({kSecCnt, bytes(c.sec_bits[kSecCnt])});
+  }
+  r.push_back({kSecGR, bytes(c.sec_bits[kSecGR])});
+  r.push_back({kSecHuf, bytes(c.sec_bits[kSecHuf])});
+  if (!g.direct) {
+    r.push_back({-1, 0});
+    r.push_back({kSecQuad, bytes(c.sec_bits[kSecQuad])});
+  }
+  if (g.rle) r.push_back({kSecRle, bytes(c.sec_bits[kSecRle])});
+  return r;
+}
+
+std::vector<uint8_t> le_meta(u64 count, uint8_t last) {  // BitPacker / GR meta
+  return {uint8_t(count), uint8_t(count >> 8), uint8_t(count >> 16), uint8_t(count >> 24), last};
+}
+
+std::vector<uint8_t> meta_of(const SegHost& g, int kind) {
+  switch (kind) {
+    case kSecBlk: return le_meta(g.B, (uint8_t)g.w1);
+    case kSecCnt: return le_meta(g.B, (uint8_t)g.w2);
+    case kSecGR: return g.P ? le_meta(g.P, g.grk) : std::vector<uint8_t>{};
+    case kSecHuf: return g.cb_sym.meta();
+    case kSecQuad: return g.cb_quad.meta();
+    case kSecRle: return g.cb_cnt.meta();
+    default: return {};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The pipeline
+
+struct CallOut {
+  xnyz_buf* frames;
+  xnyz_buf* pre;
+  xnyz_stats* stats;
+  uint32_t* perm;  // single segment, non-RLE only
+};
+
+int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off,
+                  const xnyz_opts* opts, const CallOut& co, double t_start) {
+  const u32 S = (u32)(off.size() - 1);
+  const u64 N64 = off[S];
+  if (N64 >= 0xFFFFFFFFull) throw Error(XNYZ_EUNSUPPORTED, "more than 2^32-2 points in one call");
+  for (u32 s = 0; s < S; ++s) validate(&opts[s], off[s + 1] - off[s]);
+  const u32 N = (u32)N64;
+  hipStream_t st = ctx->stream;
   ProfGuard prof_guard(ctx);
-  const QuantParams qp = quant_params(o);
+  const int threads = host_threads(0);
 
-  ctx->ctl.ensure(sizeof(DevCtl));
-  ctx->h_ctl.ensure(sizeof(DevCtl));
-  ctx->arena.ensure((1u << 20) + (size_t)N / 2 + 65536);
+  // ---- tiles and segment parameters ----------------------------------------
+  u32 ntiles = 0;
+  for (u32 s = 0; s < S; ++s) ntiles += (u32)((off[s + 1] - off[s] + kTile - 1) / kTile);
+  ctx->h_tiles.ensure(sizeof(Tile) * ntiles);
+  ctx->h_sp.ensure(sizeof(SegParam) * S);
+  ctx->h_sc.ensure(sizeof(SegCtl) * S);
+  ctx->h_sh.ensure(sizeof(SegHist) * S);
+  ctx->h_lut.ensure(sizeof(SegLut) * S);
+  Tile* tiles = ctx->h_tiles.as<Tile>();
+  SegParam* sp = ctx->h_sp.as<SegParam>();
+  SegCtl* sc = ctx->h_sc.as<SegCtl>();
+  SegHist* shh = ctx->h_sh.as<SegHist>();
+  SegLut* lut = ctx->h_lut.as<SegLut>();
+  std::vector<SegHost> g(S);
+  bool any_rle = false, want_perm = co.perm && S == 1 && !opts[0].rle;
+  {
+    u32 t = 0;
+    for (u32 s = 0; s < S; ++s) {
+      std::memset(&sp[s], 0, sizeof(SegParam));
+      const u32 n = (u32)(off[s + 1] - off[s]);
+      sp[s].p0 = (u32)off[s];
+      sp[s].n = n;
+      sp[s].tile0 = t;
+      sp[s].ntiles = (n + kTile - 1) / kTile;
+      sp[s].qp = quant_params(&opts[s]);
+      sp[s].rle = opts[s].rle != 0;
+      sp[s].diag = sp[s].rle;
+      for (u32 l = 0; l < sp[s].ntiles; ++l) tiles[t++] = Tile{s, l};
+      g[s].n = n;
+      g[s].rle = sp[s].rle;
+      g[s].hil = opts[s].is_hilbert != 0;
+      any_rle |= g[s].rle;
+    }
+  }
+  ctx->tiles.ensure(sizeof(Tile) * ntiles);
+  ctx->sp.ensure(sizeof(SegParam) * S);
+  ctx->sc.ensure(sizeof(SegCtl) * S);
+  ctx->sh.ensure(sizeof(SegHist) * S);
+  ctx->lut.ensure(sizeof(SegLut) * S);
+  const u32 large_cap = N / kCntBinsSeg + S + 2;
+  ctx->large.ensure((size_t)large_cap * 12);
+  ctx->nlarge.ensure(8);
+  ctx->arena.ensure((size_t)ntiles * (256 * 4 + kNumSec * 16 + 64) + (4u << 20));
   Arena ar{ctx->arena.as<char>(), ctx->arena.cap, 0};
-  DevCtl* dctl = ctx->ctl.as<DevCtl>();
+  ctx->h2d(ctx->tiles, ctx->h_tiles, sizeof(Tile) * ntiles);
+  ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
-  // ---- stage 1: bbox minimum, quantize, range scan ------------------------
-  launch_init_ctl(dctl, s);
-  launch_bbox(d_xyz, N, dctl, s);
-  launch_quant_stats(d_xyz, N, qp, rle, dctl, s);
-  ctx->fetch_ctl(offsetof(DevCtl, hist_sym));
-  const DevCtl& h = ctx->hc();
-  const int rx = h.range[0], ry = h.range[1], rz = h.range[2];
-  const u32 W = h.wrapped;
-  const float off[3] = {h.offset[0], h.offset[1], h.offset[2]};
-  const int max_range = std::max({rx, ry, rz});
-  const bool direct = max_range < o->direct_threshold;
-  double max_l2 = 0, mse = 0;
-  if (rle) {
-    double m2;
-    std::memcpy(&m2, &h.max_err2_bits, 8);
-    max_l2 = std::sqrt(m2);
-    mse = h.sum_err2 / N;
+  Batch bt{};
+  bt.S = S;
+  bt.ntiles = ntiles;
+  bt.N = N;
+  bt.tiles = ctx->tiles.as<Tile>();
+  bt.sp = ctx->sp.as<SegParam>();
+  bt.sc = ctx->sc.as<SegCtl>();
+  bt.sh = ctx->sh.as<SegHist>();
+  bt.lut = ctx->lut.as<SegLut>();
+  bt.large = ctx->large.as<u32>();
+  bt.n_large = ctx->nlarge.as<u32>();
+  bt.large_cap = large_cap;
+
+  // ---- stage 1: bbox minimum, quantize, range scan ----------------------------
+  launch_init(bt, st);
+  launch_bbox(bt, d_xyz, st);
+  launch_quant_stats(bt, d_xyz, st);
+  ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
+  ctx->sync();
+
+  // ---- key layout per segment (block_compressor.hpp:69-139) -------------------
+  bool wide = false, any_block = false, any_direct = false;
+  u32 narrow_bits = 0, hi_bits = 0;
+  for (u32 s = 0; s < S; ++s) {
+    SegHost& h = g[s];
+    const SegCtl& c = sc[s];
+    h.rx = c.range[0];
+    h.ry = c.range[1];
+    h.rz = c.range[2];
+    h.W = c.wrapped;
+    for (int a = 0; a < 3; ++a) h.off[a] = c.offset[a];
+    if (h.rle) {
+      double m2;
+      std::memcpy(&m2, &c.max_err2_bits, 8);
+      h.max_l2 = std::sqrt(m2);
+      h.mse = c.sum_err2 / h.n;
+    }
+    const int max_range = std::max({h.rx, h.ry, h.rz});
+    h.direct = max_range < opts[s].direct_threshold;
+    KeyParams& kp = sp[s].kp;
+    kp = KeyParams{};
+    kp.direct = h.direct;
+    kp.hilbert = h.hil;
+    if (h.direct) {
+      if (max_range >= (1 << 21) - 1)
+        throw Error(XNYZ_EUNSUPPORTED, "direct mode with a quantized range >= 2^21-1");
+      const u32 L = bitlen((u64)max_range);
+      kp.L = L ? L : 1;
+      h.key_bits = h.W ? 64 : 3 * kp.L;
+      h.hi_bits = h.key_bits;  // wide layout: the code sits in the high word
+      any_direct = true;
+    } else {
+      const u64 nx = (u64)h.rx / 128 + 1, ny = (u64)h.ry / 128 + 1, nz = (u64)h.rz / 128 + 1;
+      const unsigned __int128 nb = (unsigned __int128)nx * ny * nz;
+      if (nb > ((unsigned __int128)1 << 57))
+        throw Error(XNYZ_EUNSUPPORTED, "block grid larger than 2^57 blocks");
+      kp.nx = nx;
+      kp.nxny = nx * ny;
+      kp.C = ((1ull << 57) - 1) * kp.nxny;  // wraps mod 2^64 as the reference's size_t
+      kp.idb = bitlen((u64)nb - 1);
+      if (h.W && (u64)nb > kp.C)
+        throw Error(XNYZ_EUNSUPPORTED, "wrapped block ids overlap the regular ids");
+      h.hi_bits = kp.idb + (h.W ? 1 : 0);
+      h.key_bits = h.hi_bits + 21;
+      h.wide_needed = h.key_bits > 64;
+      any_block = true;
+    }
+    wide |= h.wide_needed;
+    narrow_bits = std::max(narrow_bits, h.key_bits);
+    hi_bits = std::max(hi_bits, h.hi_bits);
   }
+  bt.wide = wide;
+  ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
-  // ---- key layout ------------------------------------------------------------
-  KeyParams kp{};
-  kp.direct = direct;
-  kp.hilbert = hil;
-  u32 key_bits = 0, hi_bits = 0;
-  if (direct) {
-    if (max_range >= (1 << 21) - 1)
-      throw Error(XNYZ_EUNSUPPORTED, "direct mode with a quantized range >= 2^21-1");
-    u32 L = bitlen((u64)max_range);
-    kp.L = L ? L : 1;
-    key_bits = W ? 64 : 3 * kp.L;
-  } else {
-    const u64 nx = (u64)rx / 128 + 1, ny = (u64)ry / 128 + 1, nz = (u64)rz / 128 + 1;
-    const unsigned __int128 nb = (unsigned __int128)nx * ny * nz;
-    if (nb > ((unsigned __int128)1 << 57))
-      throw Error(XNYZ_EUNSUPPORTED, "block grid larger than 2^57 blocks");
-    kp.nx = nx;
-    kp.nxny = nx * ny;
-    kp.C = ((1ull << 57) - 1) * kp.nxny;  // wraps mod 2^64 as the reference's size_t
-    kp.idb = bitlen((u64)nb - 1);
-    if (W && (u64)nb > kp.C)
-      throw Error(XNYZ_EUNSUPPORTED, "wrapped block ids overlap the regular ids");
-    hi_bits = kp.idb + (W ? 1 : 0);
-    key_bits = hi_bits + 21;
-    kp.wide = key_bits > 64;
-  }
-
-  // ---- stage 2: keys, sort, runs, statistics -----------------------------------
+  // ---- stage 2: keys, sort, cells, runs, statistics ---------------------------
   for (int i = 0; i < 2; ++i) ctx->keys[i].ensure((size_t)N * 8);
-  if (kp.wide)
+  if (wide)
     for (int i = 0; i < 2; ++i) ctx->lo[i].ensure((size_t)N * 4);
   ctx->pos.ensure((size_t)N * 4 + 16);
-  ctx->pos2.ensure((size_t)N * 4 + 16);
-  const u32 large_cap = N / kCntBins + 2;
-  ctx->large.ensure((size_t)large_cap * 8);
   u64* ka = ctx->keys[0].as<u64>();
   u64* kb = ctx->keys[1].as<u64>();
-  u32* la = kp.wide ? ctx->lo[0].as<u32>() : nullptr;
-  u32* lb = kp.wide ? ctx->lo[1].as<u32>() : nullptr;
-
+  u32* la = wide ? ctx->lo[0].as<u32>() : nullptr;
+  u32* lb = wide ? ctx->lo[1].as<u32>() : nullptr;
   // The reference reorders the caller's points by the sort (ords,
   // block_compressor.hpp:141-144); here that order is an index payload of the
-  // sort, returned through `perm` (non-RLE only, as the reference's RLE
-  // reorder indexes N points with U indices).
-  const bool want_perm = perm && !rle;
-  u32* ia = nullptr;
-  u32* ib = nullptr;
-  u64* l64a = nullptr;
-  u64* l64b = nullptr;
+  // sort, returned through `perm`.
+  u32 *ia = nullptr, *ib = nullptr;
+  u64 *l64a = nullptr, *l64b = nullptr;
   if (want_perm) {
     for (int i = 0; i < 2; ++i) ctx->idx[i].ensure((size_t)N * 4);
     ia = ctx->idx[0].as<u32>();
     ib = ctx->idx[1].as<u32>();
-    if (kp.wide) {
+    if (wide) {
       for (int i = 0; i < 2; ++i) ctx->lo64[i].ensure((size_t)N * 8);
       l64a = ctx->lo64[0].as<u64>();
       l64b = ctx->lo64[1].as<u64>();
     }
   }
-  launch_keygen(d_xyz, N, qp, kp, ctx->hs, dctl, ka, l64a ? nullptr : la, ia, l64a, s);
+  launch_keygen(bt, d_xyz, ctx->hs, ka, l64a ? nullptr : la, ia, l64a, st);
   int cur;
   const u32* perm_dev = nullptr;
-  if (want_perm && kp.wide) {
-    cur = radix_sort_wide_v(ka, kb, l64a, l64b, N, hi_bits, ar, s);
-    launch_split_lo64(cur ? l64b : l64a, ctx->lo[cur].as<u32>(), ia, N, s);
+  if (want_perm && wide) {
+    cur = radix_sort_wide_v(bt, ka, kb, l64a, l64b, hi_bits, ar, st);
+    launch_split_lo64(cur ? l64b : l64a, ctx->lo[cur].as<u32>(), ia, N, st);
     perm_dev = ia;
-  } else if (want_perm) {
-    cur = radix_sort_narrow_v(ka, kb, ia, ib, N, key_bits, ar, s);
-    perm_dev = cur ? ib : ia;
-  } else if (kp.wide) {
-    cur = radix_sort_wide(ka, kb, la, lb, N, 21, hi_bits, ar, s);
+  } else if (wide) {
+    cur = radix_sort_wide(bt, ka, kb, la, lb, hi_bits, ar, st);
   } else {
-    cur = radix_sort_narrow(ka, kb, N, key_bits, ar, s);
+    cur = radix_sort(bt, ka, kb, ia, ib, narrow_bits, ar, st);
+    perm_dev = want_perm ? (cur ? ib : ia) : nullptr;
   }
   if (perm_dev) {
-    check_hip(hipMemcpyAsync(perm, perm_dev, (size_t)N * 4, hipMemcpyDeviceToHost, s),
+    check_hip(hipMemcpyAsync(co.perm, perm_dev, (size_t)N * 4, hipMemcpyDeviceToHost, st),
               "perm D2H");
     ctx->copied();
   }
-  KeyView kv{ctx->keys[cur].as<u64>(), kp.wide ? ctx->lo[cur].as<u32>() : nullptr, kp.idb, kp.C};
-  KeyView kfree{ctx->keys[cur ^ 1].as<u64>(), kp.wide ? ctx->lo[cur ^ 1].as<u32>() : nullptr,
-                kp.idb, kp.C};
-  const u32* n_dev = nullptr;  // element count after RLE (device)
-  u32* cnt = ctx->pos2.as<u32>();
-  if (rle) {
-    runs_unique(kv, kp.wide, N, ctx->pos.as<u32>(), &dctl->n_unique, ar, s);
-    launch_rle_finish(kv, kp.wide, ctx->pos.as<u32>(), &dctl->n_unique, N,
-                      ctx->keys[cur ^ 1].as<u64>(), kp.wide ? ctx->lo[cur ^ 1].as<u32>() : nullptr,
-                      cnt, dctl, ctx->large.as<u32>(), large_cap, s);
-    kv = kfree;
-    n_dev = &dctl->n_unique;
+  KeyView kv{ctx->keys[cur].as<u64>(), wide ? ctx->lo[cur].as<u32>() : nullptr};
+  u32* cnt = nullptr;
+  if (any_rle) {
+    ctx->cells.ensure((size_t)N * 8);
+    if (wide) ctx->cells_lo.ensure((size_t)N * 4);
+    ctx->cnt.ensure((size_t)N * 4 + 16);
+    cnt = ctx->cnt.as<u32>();
+    rle_cells(bt, kv, ctx->pos.as<u32>(), ctx->cells.as<u64>(),
+              wide ? ctx->cells_lo.as<u32>() : nullptr, cnt, ar, st);
+    kv = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
   }
-  if (direct) {
-    launch_stats_direct(kv.a, N, n_dev, dctl, s);
-  } else {
-    runs_blocks(kv, kp.wide, N, n_dev, ctx->pos.as<u32>(), &dctl->n_blocks, ar, s);
-    launch_stats_block(kv, kp.wide, ctx->pos.as<u32>(), &dctl->n_blocks, N, n_dev, dctl, s);
+  block_runs_and_stats(bt, kv, any_direct, any_block, ctx->pos.as<u32>(), ar, st);
+  ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
+  ctx->d2h(ctx->h_sh, ctx->sh, sizeof(SegHist) * S);
+  u32 n_large = 0;
+  check_hip(hipMemcpyAsync(&n_large, ctx->nlarge.p, 4, hipMemcpyDeviceToHost, st), "D2H");
+  ctx->sync();
+  if (n_large > large_cap) throw Error(XNYZ_EINTERNAL, "large-count list overflow");
+  if (n_large) {
+    std::vector<u32> raw((size_t)n_large * 3);
+    check_hip(hipMemcpyAsync(raw.data(), ctx->large.p, raw.size() * 4, hipMemcpyDeviceToHost, st),
+              "large D2H");
+    ctx->sync();
+    for (u32 i = 0; i < n_large; ++i) g[raw[3 * i]].large.emplace_back(raw[3 * i + 1], raw[3 * i + 2]);
   }
-  ctx->fetch_ctl(sizeof(DevCtl));
-  const u32 n = rle ? h.n_unique : N;  // elements of the coded streams
-  const u32 B = direct ? 0 : h.n_blocks;
-  if (h.n_qpatches) throw Error(XNYZ_EINTERNAL, "quads delta above 254");
-  if (rle && h.n_large > large_cap) throw Error(XNYZ_EINTERNAL, "large-count list overflow");
 
-  // ---- host: codebooks --------------------------------------------------------
-  ctx->h_lut.ensure(sizeof(LutBlock));
-  LutBlock& L = *ctx->h_lut.as<LutBlock>();
-  Codebook<uint64_t> cb_sym;
-  cb_sym.build(sym_counts(h.hist_sym, h.first_sym, kSymBins), n);
-  fill_lut(cb_sym, L.sym, kSymBins);
-  Codebook<uint8_t> cb_quad;
-  if (!direct) {
-    cb_quad.build(sym_counts(h.hist_quad, h.first_quad, kSymBins), n);
-    fill_lut(cb_quad, L.quad, kSymBins);
-  }
-  const u64 P = h.n_patches;
-  const uint8_t grk = golomb_rice_k(P, h.patch_sum, false);
-  Codebook<int> cb_cnt;
-  std::vector<u32> big_vals;
-  std::vector<HufEntry> big_codes;
-  if (rle) {
-    std::vector<SymCount> cs = sym_counts(h.hist_cnt, h.first_cnt, kCntBins);
-    if (h.n_large) {
-      std::vector<u32> raw((size_t)h.n_large * 2);
-      check_hip(hipMemcpyAsync(raw.data(), ctx->large.p, raw.size() * 4, hipMemcpyDeviceToHost, s),
-                "large D2H");
-      ctx->sync();
-      std::vector<std::pair<u32, u32>> lv;
-      for (u32 i = 0; i < h.n_large; ++i) lv.emplace_back(raw[2 * i], raw[2 * i + 1]);
-      std::sort(lv.begin(), lv.end());
-      for (size_t i = 0; i < lv.size();) {
+  // ---- host: codebooks per segment (in parallel) ----------------------------
+  parallel_for(S, threads, [&](size_t s) {
+    SegHost& h = g[s];
+    const SegCtl& c = sc[s];
+    const SegHist& hh = shh[s];
+    if (c.n_qpatches) throw Error(XNYZ_EINTERNAL, "quads delta above 254");
+    h.ncoded = c.ne;
+    h.B = h.direct ? 0 : c.nb;
+    h.P = c.n_patches;
+    h.cb_sym.build(sym_counts(hh.sym, hh.first_sym, kSymBins), h.ncoded);
+    fill_lut(h.cb_sym, lut[s].sym, kSymBins);
+    if (!h.direct) {
+      h.cb_quad.build(sym_counts(hh.quad, hh.first_quad, kSymBins), h.ncoded);
+      fill_lut(h.cb_quad, lut[s].quad, kSymBins);
+      h.w1 = c.max_blkd ? bitlen(c.max_blkd) : 1;
+      h.w2 = c.max_cntd ? bitlen(c.max_cntd) : 1;
+    }
+    h.grk = golomb_rice_k(h.P, c.patch_sum, false);
+    if (h.rle) {
+      std::vector<SymCount> cs = sym_counts(hh.cnt, hh.first_cnt, kCntBinsSeg);
+      std::sort(h.large.begin(), h.large.end());
+      for (size_t i = 0; i < h.large.size();) {
         size_t j = i;
-        while (j < lv.size() && lv[j].first == lv[i].first) ++j;
-        cs.push_back(SymCount{lv[i].first, (u64)(j - i), lv[i].second});
-        big_vals.push_back(lv[i].first);
+        u32 first = h.large[i].second;
+        while (j < h.large.size() && h.large[j].first == h.large[i].first) {
+          first = std::min(first, h.large[j].second);
+          ++j;
+        }
+        cs.push_back(SymCount{h.large[i].first, (u64)(j - i), first});
+        h.big_vals.push_back(h.large[i].first);
         i = j;
       }
-    }
-    // block_compressor_rle.hpp:290-303: build, compare the meta with a
-    // quarter of the data, then build again on the same object.
-    cb_cnt.build(cs, n);
-    const double meta1 = (double)cb_cnt.meta().size();
-    const double data1 = (double)((cb_cnt.data_bits() + 7) / 8);
-    if (meta1 > data1 * 0.25) cb_cnt.use_eg = true;
-    cb_cnt.build(cs, n);
-    if (!cb_cnt.use_eg) {
-      fill_lut(cb_cnt, L.cnt, kCntBins);
-      for (u32 v : big_vals) {
-        const auto& c = cb_cnt.code_table.at((int)v);
-        big_codes.push_back(HufEntry{c.bits, c.length});
+      // block_compressor_rle.hpp:290-303: build, compare the meta with a
+      // quarter of the data, then build again on the same object.
+      h.cb_cnt.build(cs, h.ncoded);
+      const double meta1 = (double)h.cb_cnt.meta().size();
+      const double data1 = (double)((h.cb_cnt.data_bits() + 7) / 8);
+      if (meta1 > data1 * 0.25) h.cb_cnt.use_eg = true;
+      h.cb_cnt.build(cs, h.ncoded);
+      if (!h.cb_cnt.use_eg) {
+        fill_lut(h.cb_cnt, lut[s].cnt, kCntBinsSeg);
+        for (u32 v : h.big_vals) {
+          const auto& cd = h.cb_cnt.code_table.at((int)v);
+          h.big_codes.push_back(HufEntry{cd.bits, cd.length});
+        }
       }
     }
+    if (h.cb_sym.max_length() > 56 || h.cb_quad.max_length() > 56 ||
+        h.cb_cnt.max_length() > 56)
+      throw Error(XNYZ_EUNSUPPORTED, "Huffman code longer than 56 bits");
+  });
+  std::vector<u32> big_vals;
+  std::vector<HufEntry> big_codes;
+  for (u32 s = 0; s < S; ++s) {
+    SegHost& h = g[s];
+    sp[s].grk = h.grk;
+    sp[s].cnt_eg = h.cb_cnt.use_eg;
+    sp[s].egk = h.cb_cnt.eg_k;
+    sp[s].w1 = h.w1;
+    sp[s].w2 = h.w2;
+    sp[s].big0 = (u32)big_vals.size();
+    sp[s].nbig = (u32)h.big_codes.size();
+    if (!h.big_codes.empty()) {
+      big_vals.insert(big_vals.end(), h.big_vals.begin(), h.big_vals.end());
+      big_codes.insert(big_codes.end(), h.big_codes.begin(), h.big_codes.end());
+    }
   }
-  if (cb_sym.max_length() > 56 || cb_quad.max_length() > 56 || cb_cnt.max_length() > 56)
-    throw Error(XNYZ_EUNSUPPORTED, "Huffman code longer than 56 bits");
-  ctx->lut.ensure(sizeof(LutBlock));
-  check_hip(hipMemcpyAsync(ctx->lut.p, &L, sizeof(LutBlock), hipMemcpyHostToDevice, s), "lut H2D");
+  ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
+  ctx->h2d(ctx->lut, ctx->h_lut, sizeof(SegLut) * S);
+  ctx->bigv.ensure(big_vals.size() * 4 + 16);
+  ctx->bigc.ensure(big_codes.size() * sizeof(HufEntry) + 16);
   if (!big_vals.empty()) {
-    ctx->bigv.ensure(big_vals.size() * 4);
-    ctx->bigc.ensure(big_codes.size() * sizeof(HufEntry) + 8);
     check_hip(hipMemcpyAsync(ctx->bigv.p, big_vals.data(), big_vals.size() * 4,
-                             hipMemcpyHostToDevice, s), "big H2D");
-    if (!big_codes.empty())
-      check_hip(hipMemcpyAsync(ctx->bigc.p, big_codes.data(), big_codes.size() * sizeof(HufEntry),
-                               hipMemcpyHostToDevice, s), "big H2D");
-    // pageable sources: finish before the vectors go out of scope
-    ctx->sync();
+                             hipMemcpyHostToDevice, st), "big H2D");
+    check_hip(hipMemcpyAsync(ctx->bigc.p, big_codes.data(), big_codes.size() * sizeof(HufEntry),
+                             hipMemcpyHostToDevice, st), "big H2D");
+    ctx->sync();  // pageable sources
   }
-  ctx->copied();
-  const LutBlock* dl = ctx->lut.as<LutBlock>();
+  bt.big_vals = ctx->bigv.as<u32>();
+  bt.big_codes = ctx->bigc.as<HufEntry>();
 
-  // ---- stage 3: variable-length section sizes ---------------------------------
-  CoderArgs ca{};
-  ca.kv = kv;
-  ca.wide = kp.wide;
-  ca.sfc = kv.a;
-  ca.cnt = cnt;
-  VlPlan plan_gr{}, plan_sym{}, plan_quad{}, plan_cnt{};
-  const CoderKind k_gr = direct ? kCoderGRDirect : kCoderGRBlock;
-  const CoderKind k_sym = direct ? kCoderHufDirect : kCoderHufBlock;
-  const CoderKind k_cnt = cb_cnt.use_eg ? kCoderCntEG : kCoderCntHuf;
-  CoderArgs a_gr = ca, a_sym = ca, a_quad = ca, a_cnt = ca;
-  a_gr.k = grk;
-  a_sym.lut = dl->sym;
-  a_quad.lut = dl->quad;
-  a_cnt.lut = dl->cnt;
-  a_cnt.k = cb_cnt.eg_k;
-  a_cnt.big_vals = ctx->bigv.as<u32>();
-  a_cnt.big_codes = ctx->bigc.as<HufEntry>();
-  a_cnt.n_big = (u32)big_vals.size();
-  if (P) {
-    plan_gr = vl_plan(n, ar);
-    vl_count(k_gr, a_gr, n, plan_gr, &dctl->sec_bits[kSecGR], ar, s);
-  }
-  plan_sym = vl_plan(n, ar);
-  vl_count(k_sym, a_sym, n, plan_sym, &dctl->sec_bits[kSecHuf], ar, s);
-  if (!direct) {
-    plan_quad = vl_plan(n, ar);
-    vl_count(kCoderQuadHuf, a_quad, n, plan_quad, &dctl->sec_bits[kSecQuadHuf], ar, s);
-  }
-  if (rle) {
-    plan_cnt = vl_plan(n, ar);
-    vl_count(k_cnt, a_cnt, n, plan_cnt, &dctl->sec_bits[kSecCnt], ar, s);
-  }
-  check_hip(hipMemcpyAsync(ctx->h_ctl.as<DevCtl>()->sec_bits, &dctl->sec_bits,
-                           sizeof(h.sec_bits), hipMemcpyDeviceToHost, s), "sec D2H");
+  // ---- stage 3: section sizes -------------------------------------------------
+  const SecPlan plan = plan_sections(bt, ar);
+  size_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ar, st);
+  ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
   ctx->sync();
-  const u64 gr_bits = P ? h.sec_bits[kSecGR] : 0;
-  if (h.sec_bits[kSecHuf] != cb_sym.data_bits() ||
-      (!direct && h.sec_bits[kSecQuadHuf] != cb_quad.data_bits()) ||
-      (rle && h.sec_bits[kSecCnt] != cb_cnt.data_bits()))
-    throw Error(XNYZ_EINTERNAL, "device and host disagree on a section size");
 
-  // ---- layout (Appendix: header, sections, trailer) ---------------------------
-  std::vector<Section> secs;
-  auto add = [&](std::vector<uint8_t> meta, u64 data_bytes) {
-    secs.push_back(Section{std::move(meta), data_bytes, 0, 0});
-    return secs.size() - 1;
-  };
-  auto le_meta = [](u32 count, uint8_t last) {  // BitPacker / GR meta: u32 LE + u8
-    return std::vector<uint8_t>{uint8_t(count), uint8_t(count >> 8), uint8_t(count >> 16),
-                                uint8_t(count >> 24), last};
-  };
-  size_t s1 = 0, s2 = 0, s3, s4, s6 = 0, s7 = 0;
-  u32 w1 = 0, w2 = 0;
-  if (!direct) {
-    w1 = h.max_blkd ? bitlen(h.max_blkd) : 1;
-    w2 = h.max_cntd ? bitlen(h.max_cntd) : 1;
-    s1 = add(le_meta(B, (uint8_t)w1), ((u64)B * w1 + 7) / 8);
-    s2 = add(le_meta(B, (uint8_t)w2), ((u64)B * w2 + 7) / 8);
+  // ---- layout: each segment's stream at a 256-byte aligned base ---------------
+  u64 out_total = 0;
+  for (u32 s = 0; s < S; ++s) {
+    SegHost& h = g[s];
+    const SegCtl& c = sc[s];
+    if (c.sec_bits[kSecHuf] != h.cb_sym.data_bits() ||
+        (!h.direct && c.sec_bits[kSecQuad] != h.cb_quad.data_bits()) ||
+        (h.rle && c.sec_bits[kSecRle] != h.cb_cnt.data_bits()) ||
+        (!h.direct && (c.sec_bits[kSecBlk] != h.B * h.w1 || c.sec_bits[kSecCnt] != h.B * h.w2)))
+      throw Error(XNYZ_EINTERNAL, "device and host disagree on a section size");
+    h.base = out_total;
+    u64 at = h.base + 30;
+    for (const Rec& r : records(h, c)) {
+      std::vector<uint8_t> m = meta_of(h, r.kind);
+      const u64 data_at = at + 16 + m.size();
+      if (r.kind >= 0) sp[s].sec_byte[r.kind] = data_at;
+      at = data_at + r.data_bytes;
+      h.metas.push_back(std::move(m));
+    }
+    h.total = at + 12 - h.base;
+    out_total = (h.base + h.total + 255) & ~u64(255);
   }
-  s3 = add(P ? le_meta((u32)P, grk) : std::vector<uint8_t>{}, (gr_bits + 7) / 8);
-  s4 = add(cb_sym.meta(), (cb_sym.data_bits() + 7) / 8);
-  if (!direct) {
-    add({}, 0);  // quads patches: none (deltas are at most 7)
-    s6 = add(cb_quad.meta(), (cb_quad.data_bits() + 7) / 8);
-  }
-  if (rle) s7 = add(cb_cnt.meta(), (cb_cnt.data_bits() + 7) / 8);
-  u64 at = 30;
-  for (auto& sc : secs) {
-    sc.at = at;
-    sc.data_at = at + 16 + sc.meta.size();
-    at = sc.data_at + sc.data_bytes;
-  }
-  const u64 total = at + 12;
+  ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
-  // ---- stage 4: write the data sections in place ------------------------------
-  ctx->out.ensure(total + 64);
+  // ---- stage 4: write every section in place ------------------------------------
+  ctx->out.ensure(out_total + 64);
   u32* ow = ctx->out.as<u32>();
-  launch_fill_u32(ow, (total + 64) / 4, 0, s);
-  if (!direct) {
-    pack_block_deltas(0, kv, kp.wide, ctx->pos.as<u32>(), B, n, w1, secs[s1].data_at, ow, s);
-    pack_block_deltas(1, kv, kp.wide, ctx->pos.as<u32>(), B, n, w2, secs[s2].data_at, ow, s);
-  }
-  if (P) vl_write(k_gr, a_gr, n, plan_gr, secs[s3].data_at, ow, s);
-  vl_write(k_sym, a_sym, n, plan_sym, secs[s4].data_at, ow, s);
-  if (!direct) vl_write(kCoderQuadHuf, a_quad, n, plan_quad, secs[s6].data_at, ow, s);
-  if (rle) vl_write(k_cnt, a_cnt, n, plan_cnt, secs[s7].data_at, ow, s);
-  ctx->h_out.ensure(total);
+  launch_fill_u32(ow, (out_total + 64) / 4, 0, st);
+  write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, st);
+  ctx->h_out.ensure(out_total + 64);
   uint8_t* hb = ctx->h_out.as<uint8_t>();
-  check_hip(hipMemcpyAsync(hb, ow, total, hipMemcpyDeviceToHost, s), "stream D2H");
+  check_hip(hipMemcpyAsync(hb, ow, out_total, hipMemcpyDeviceToHost, st), "stream D2H");
   ctx->sync();
 
+  // ---- host: header, size fields, metas, trailer; frames ----------------------
   // header (block_compressor.hpp:61-87): u64 points, u64 params.size() = 0,
   // i32 ranges, bool curve, bool direct; per section u64 data size, u64 meta
   // size, meta; trailer: the float offset.
-  const u64 np = n;
-  const u64 zero = 0;
-  put_bytes(hb, &np, 8);
-  put_bytes(hb + 8, &zero, 8);
-  const int32_t rr[3] = {rx, ry, rz};
-  put_bytes(hb + 16, rr, 12);
-  hb[28] = hil ? 1 : 0;
-  hb[29] = direct ? 1 : 0;
-  for (auto& sc : secs) {
-    const u64 ds = sc.data_bytes, ms = sc.meta.size();
-    put_bytes(hb + sc.at, &ds, 8);
-    put_bytes(hb + sc.at + 8, &ms, 8);
-    if (ms) put_bytes(hb + sc.at + 16, sc.meta.data(), ms);
-  }
-  put_bytes(hb + total - 12, off, 12);
-  const double t_dev = now_ms();
-
-  if (out_pre) {
-    out_pre->data = static_cast<uint8_t*>(std::malloc(total));
-    if (!out_pre->data) throw Error(XNYZ_ENOMEM, "malloc failed");
-    std::memcpy(out_pre->data, hb, total);
-    out_pre->len = total;
-  }
-  u64 frame_len = 0;
-  if (out_frame) {
-    if (o->frame_mode == XNYZ_FRAME_NONE) {
-      out_frame->data = static_cast<uint8_t*>(std::malloc(total));
-      if (!out_frame->data) throw Error(XNYZ_ENOMEM, "malloc failed");
-      std::memcpy(out_frame->data, hb, total);
-      out_frame->len = total;
-    } else {
-      zstd_frame(hb, total, o->frame_mode, o->zstd_workers, out_frame);
+  parallel_for(S, threads, [&](size_t s) {
+    SegHost& h = g[s];
+    uint8_t* b = hb + h.base;
+    const u64 np = h.ncoded, zero = 0;
+    std::memcpy(b, &np, 8);
+    std::memcpy(b + 8, &zero, 8);
+    const int32_t rr[3] = {h.rx, h.ry, h.rz};
+    std::memcpy(b + 16, rr, 12);
+    b[28] = h.hil ? 1 : 0;
+    b[29] = h.direct ? 1 : 0;
+    u64 at = 30;
+    size_t i = 0;
+    for (const Rec& r : records(h, sc[s])) {
+      const std::vector<uint8_t>& m = h.metas[i++];
+      const u64 ds = r.data_bytes, ms = m.size();
+      std::memcpy(b + at, &ds, 8);
+      std::memcpy(b + at + 8, &ms, 8);
+      if (ms) std::memcpy(b + at + 16, m.data(), ms);
+      at += 16 + ms + ds;
     }
-    frame_len = out_frame->len;
-  }
+    std::memcpy(b + at, h.off, 12);
+  });
+  const double t_dev = now_ms();
+  const bool per_seg_threads = S > 1;
+  parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
+    SegHost& h = g[s];
+    const uint8_t* b = hb + h.base;
+    if (co.pre) {
+      co.pre[s].data = xmalloc(h.total);
+      std::memcpy(co.pre[s].data, b, h.total);
+      co.pre[s].len = h.total;
+    }
+    if (co.frames) make_frame(b, h.total, opts[s], per_seg_threads ? 1 : threads, &co.frames[s]);
+  });
   const double t_end = now_ms();
-  if (st) {
-    std::memset(st, 0, sizeof(*st));
-    st->input_points = N;
-    st->num_points = n;
-    st->range[0] = rx;
-    st->range[1] = ry;
-    st->range[2] = rz;
-    st->is_direct = direct;
-    st->blocks = B;
-    st->patches = P;
-    st->quad_patches = 0;
-    st->wrapped_points = W;
-    st->counts_exp_golomb = rle ? cb_cnt.use_eg : 0;
-    st->counts_eg_k = rle ? cb_cnt.eg_k : 0;
-    st->entropy_repos = cb_sym.entropy;
-    st->entropy_quads = cb_quad.entropy;
-    st->entropy_counts = cb_cnt.entropy;
-    st->max_l2 = max_l2;
-    st->mse = mse;
-    st->prezstd_bytes = total;
-    st->frame_bytes = frame_len;
-    st->ms_device = t_dev - t_start;
-    st->ms_zstd = t_end - t_dev;
-    st->ms_total = t_end - t_start;
+  if (co.stats) {
+    for (u32 s = 0; s < S; ++s) {
+      const SegHost& h = g[s];
+      xnyz_stats& x = co.stats[s];
+      std::memset(&x, 0, sizeof(x));
+      x.input_points = h.n;
+      x.num_points = h.ncoded;
+      x.range[0] = h.rx;
+      x.range[1] = h.ry;
+      x.range[2] = h.rz;
+      x.is_direct = h.direct;
+      x.blocks = h.B;
+      x.patches = h.P;
+      x.wrapped_points = h.W;
+      x.counts_exp_golomb = h.rle ? h.cb_cnt.use_eg : 0;
+      x.counts_eg_k = h.rle ? h.cb_cnt.eg_k : 0;
+      x.entropy_repos = h.cb_sym.entropy;
+      x.entropy_quads = h.cb_quad.entropy;
+      x.entropy_counts = h.cb_cnt.entropy;
+      x.max_l2 = h.max_l2;
+      x.mse = h.mse;
+      x.prezstd_bytes = h.total;
+      x.frame_bytes = co.frames ? co.frames[s].len : 0;
+      x.ms_device = t_dev - t_start;
+      x.ms_zstd = t_end - t_dev;
+      x.ms_total = t_end - t_start;
+    }
   }
   return XNYZ_OK;
 }
@@ -709,14 +917,14 @@ int xnyz_destroy(xnyz_ctx* ctx) {
   return guarded([&] {
     use_device(ctx);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DBuf* b : {&ctx->ctl, &ctx->keys[0], &ctx->keys[1], &ctx->lo[0], &ctx->lo[1],
-                    &ctx->pos, &ctx->pos2, &ctx->large, &ctx->lut, &ctx->bigv, &ctx->bigc,
-                    &ctx->out, &ctx->arena, &ctx->upload, &ctx->tmp, &ctx->idx[0],
-                    &ctx->idx[1], &ctx->lo64[0], &ctx->lo64[1]})
+    for (DBuf* b : {&ctx->tiles, &ctx->sp, &ctx->sc, &ctx->sh, &ctx->lut, &ctx->bigv, &ctx->bigc,
+                    &ctx->large, &ctx->nlarge, &ctx->keys[0], &ctx->keys[1], &ctx->lo[0],
+                    &ctx->lo[1], &ctx->cells, &ctx->cells_lo, &ctx->pos, &ctx->cnt, &ctx->out,
+                    &ctx->arena, &ctx->upload, &ctx->tmp, &ctx->idx[0], &ctx->idx[1],
+                    &ctx->lo64[0], &ctx->lo64[1]})
       b->release();
-    ctx->h_ctl.release();
-    ctx->h_out.release();
-    ctx->h_lut.release();
+    for (HBuf* b : {&ctx->h_tiles, &ctx->h_sp, &ctx->h_sc, &ctx->h_sh, &ctx->h_lut, &ctx->h_out})
+      b->release();
     ctx->prof.destroy();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -729,8 +937,10 @@ int xnyz_compress_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xn
                          xnyz_stats* stats) {
   return guarded([&] {
     if (!ctx || !d_xyz) throw Error(XNYZ_EINVAL, "null context or input");
+    validate(opts, n);
     use_device(ctx);
-    return compress_impl(ctx, d_xyz, n, opts, out_frame, out_prezstd, perm, stats, now_ms());
+    return compress_impl(ctx, d_xyz, {0, n}, opts, CallOut{out_frame, out_prezstd, stats, perm},
+                         now_ms());
   });
 }
 
@@ -745,8 +955,27 @@ int xnyz_compress(xnyz_ctx* ctx, const float* xyz, uint64_t n, const xnyz_opts* 
     ctx->upload.ensure((size_t)n * 12);
     check_hip(hipMemcpyAsync(ctx->upload.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice,
                              ctx->stream), "points H2D");
-    return compress_impl(ctx, ctx->upload.as<float>(), n, opts, out_frame, out_prezstd, perm,
-                         stats, t0);
+    return compress_impl(ctx, ctx->upload.as<float>(), {0, n}, opts,
+                         CallOut{out_frame, out_prezstd, stats, perm}, t0);
+  });
+}
+
+int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                               uint32_t count, const xnyz_opts* opts, xnyz_buf* frames,
+                               xnyz_buf* prezstd, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off(offsets, offsets + count + 1);
+    for (u32 s = 0; s < count; ++s)
+      if (off[s + 1] < off[s]) throw Error(XNYZ_EINVAL, "offsets must not decrease");
+    if (off[0] != 0) {  // segments are addressed from the first cloud
+      for (auto& v : off) v -= offsets[0];
+      d_xyz += 3 * offsets[0];
+    }
+    use_device(ctx);
+    return compress_impl(ctx, d_xyz, off, opts, CallOut{frames, prezstd, stats, nullptr},
+                         now_ms());
   });
 }
 
@@ -760,19 +989,42 @@ int xnyz_quantize_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, int32_t 
     validate(&o, n);
     use_device(ctx);
     hipStream_t s = ctx->stream;
-    ctx->ctl.ensure(sizeof(DevCtl));
-    ctx->h_ctl.ensure(sizeof(DevCtl));
+    const u32 N = (u32)n, ntiles = (N + kTile - 1) / kTile;
+    ctx->h_tiles.ensure(sizeof(Tile) * ntiles);
+    ctx->h_sp.ensure(sizeof(SegParam));
+    ctx->h_sc.ensure(sizeof(SegCtl));
+    for (u32 l = 0; l < ntiles; ++l) ctx->h_tiles.as<Tile>()[l] = Tile{0, l};
+    SegParam& p = *ctx->h_sp.as<SegParam>();
+    std::memset(&p, 0, sizeof(p));
+    p.n = N;
+    p.ntiles = ntiles;
+    p.qp = quant_params(&o);
+    ctx->tiles.ensure(sizeof(Tile) * ntiles);
+    ctx->sp.ensure(sizeof(SegParam));
+    ctx->sc.ensure(sizeof(SegCtl));
+    ctx->sh.ensure(sizeof(SegHist));
+    ctx->nlarge.ensure(8);
     ctx->tmp.ensure((size_t)n * 12);
-    DevCtl* dctl = ctx->ctl.as<DevCtl>();
-    const QuantParams qp = quant_params(&o);
-    launch_init_ctl(dctl, s);
-    launch_bbox(d_xyz, (u32)n, dctl, s);
-    launch_quantize_nodes(d_xyz, (u32)n, qp, dctl, ctx->tmp.as<int32_t>(), s);
+    ctx->h2d(ctx->tiles, ctx->h_tiles, sizeof(Tile) * ntiles);
+    ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam));
+    Batch bt{};
+    bt.S = 1;
+    bt.ntiles = ntiles;
+    bt.N = N;
+    bt.tiles = ctx->tiles.as<Tile>();
+    bt.sp = ctx->sp.as<SegParam>();
+    bt.sc = ctx->sc.as<SegCtl>();
+    bt.sh = ctx->sh.as<SegHist>();
+    bt.n_large = ctx->nlarge.as<u32>();
+    launch_init(bt, s);
+    launch_bbox(bt, d_xyz, s);
+    launch_quantize_nodes(bt, d_xyz, ctx->tmp.as<int32_t>(), s);
     check_hip(hipMemcpyAsync(nodes, ctx->tmp.p, (size_t)n * 12, hipMemcpyDeviceToHost, s),
               "nodes D2H");
-    ctx->fetch_ctl(offsetof(DevCtl, hist_sym));
+    ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl));
+    ctx->sync();
     if (offset)
-      for (int a = 0; a < 3; ++a) offset[a] = ctx->hc().offset[a];
+      for (int a = 0; a < 3; ++a) offset[a] = ctx->h_sc.as<SegCtl>()->offset[a];
     return XNYZ_OK;
   });
 }
@@ -794,6 +1046,11 @@ int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n, uint32
     ctx->sync();
     return XNYZ_OK;
   });
+}
+
+int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
+                    xnyz_buf* out_points) {
+  return guarded([&] { return decompress_impl(in, len, opts, out_points); });
 }
 
 int xnyz_profile_enable(xnyz_ctx* ctx, int on) {

@@ -16,39 +16,7 @@ using i64 = long long;
 constexpr int kWave = 64;        // CDNA wavefront
 constexpr int kBlock = 256;      // threads per workgroup (4 wavefronts)
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr u32 kCntBins = 4096;   // direct-indexed bins of the RLE counts histogram
 constexpr u32 kSymBins = 256;    // PFD symbol alphabet (repos, quads, sfc deltas)
-
-// Device-resident control block: reductions and counters of one compress
-// call.  Zeroed (and the min/first fields set to all-ones) by k_init_ctl.
-struct DevCtl {
-  u64 minkey[3];        // per-axis bbox minimum: (ordered float << 32) | index
-  float offset[3];      // the shift: the minimum's exact float (FLT_MAX if none)
-  u32 pad1;
-  int range[3];         // max quantized x, y, z, starting from 0
-  u32 wrapped;          // nodes with z == -1
-  u64 max_err2_bits;    // RLE diagnostic: max squared error (double bits)
-  double sum_err2;      // RLE diagnostic: sum of squared errors
-  u32 n_unique;         // RLE: unique cells U
-  u32 n_blocks;         // occupied 128^3 blocks B
-  u32 n_patches;        // repos / sfc-delta values > 254
-  u32 n_qpatches;       // quads values > 254 (never, kept as a check)
-  u64 patch_sum;        // wrapping u64 sum of the patches (Golomb-Rice k)
-  u64 max_blkd;         // max block-id delta (BitPacker width)
-  u64 max_cntd;         // max block-count delta
-  u64 sec_bits[8];      // bit totals of the variable-length sections
-  u32 n_large;          // RLE counts >= kCntBins (listed separately)
-  u32 pad0;
-  u32 hist_sym[kSymBins];     // repos (block) or sfc-delta (direct) symbols
-  u32 first_sym[kSymBins];    // first index of each symbol
-  u32 hist_quad[kSymBins];
-  u32 first_quad[kSymBins];
-  u32 hist_cnt[kCntBins];
-  u32 first_cnt[kCntBins];
-};
-
-// Variable-length sections in DevCtl::sec_bits.
-enum Sec { kSecGR = 0, kSecHuf = 1, kSecQuadHuf = 2, kSecCnt = 3 };
 
 // One code word for the variable-length packers.  MSB-first streams
 // (Huffman, Exp-Golomb): `lead` zero bits, then `len` bits of `val`, most

@@ -318,13 +318,9 @@ std::vector<float> decode_buffer(const uint8_t* buf, uint64_t len, int quantizer
 }  // namespace
 }  // namespace xnyz
 
-extern "C" int xnyz_decompress(const uint8_t* frame, uint64_t len, const xnyz_opts* opts,
-                               float** xyz, uint64_t* n);
-
 namespace xnyz {
-int decompress_impl(const uint8_t* frame, uint64_t len, const xnyz_opts* o, float** xyz,
-                    uint64_t* n) {
-  if (!frame || !o || !xyz || !n) throw Error(XNYZ_EINVAL, "null argument");
+int decompress_impl(const uint8_t* frame, uint64_t len, const xnyz_opts* o, xnyz_buf* out) {
+  if (!frame || !o || !out) throw Error(XNYZ_EINVAL, "null argument");
   std::vector<uint8_t> raw;
   const uint8_t* buf = frame;
   uint64_t blen = len;
@@ -340,11 +336,12 @@ int decompress_impl(const uint8_t* frame, uint64_t len, const xnyz_opts* o, floa
     blen = raw.size();
   }
   std::vector<float> pts = decode_buffer(buf, blen, o->quantizer, o->l2_bound, o->rle != 0);
-  float* p = static_cast<float*>(std::malloc(pts.size() * sizeof(float) + 16));
+  const size_t bytes = pts.size() * sizeof(float);
+  uint8_t* p = static_cast<uint8_t*>(std::malloc(bytes + 16));
   if (!p) throw Error(XNYZ_ENOMEM, "malloc failed");
-  if (!pts.empty()) std::memcpy(p, pts.data(), pts.size() * sizeof(float));
-  *xyz = p;
-  *n = pts.size() / 3;
+  if (bytes) std::memcpy(p, pts.data(), bytes);
+  out->data = p;
+  out->len = bytes;
   return XNYZ_OK;
 }
 }  // namespace xnyz

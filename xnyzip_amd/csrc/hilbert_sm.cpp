@@ -108,6 +108,29 @@ struct Tables {
 
 }  // namespace
 
+// hilbertToMorton3 (hilbert.cpp:396-421), for the host decoder: undo the
+// lowest digit, then walk up, mapping each digit back and re-permuting the
+// digits below it.
+uint64_t hilbert_to_morton3(uint64_t h, unsigned iteration) {
+  static const Tables t;
+  const uint8_t lh = h & 7;
+  uint64_t r = (h ^ lh) | t.h2m[lh];
+  for (unsigned i = 1; i <= iteration; ++i) {
+    const unsigned sh = i * 3;
+    const uint8_t hd = (r >> sh) & 7, md = t.h2m[hd];
+    r ^= uint64_t{hd} << sh;
+    r |= uint64_t{md} << sh;
+    const P8& perm = t.h2m_perms[md];
+    for (unsigned j = i - 1; j < i; --j) {
+      const unsigned cs = j * 3;
+      const uint8_t c = (r >> cs) & 7;
+      r ^= uint64_t{c} << cs;
+      r |= uint64_t{perm[c]} << cs;
+    }
+  }
+  return r;
+}
+
 HilbertSM build_hilbert_sm() {
   const Tables t;
   std::vector<P8> states;

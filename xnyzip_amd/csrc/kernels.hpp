@@ -1,6 +1,10 @@
-// kernels.hpp -- launch wrappers of the gfx950 kernels (kernels.hip) and the
-// parameter blocks they take.  Wrappers enqueue on the given stream and throw
-// xnyz::Error on a launch failure.
+// kernels.hpp -- the gfx950 kernels (kernels.hip) as one segmented pipeline:
+// S clouds ("segments") of uneven size are processed together, every launch
+// covering all of them.  Work is split into tiles of kTile points; a tile
+// belongs to one segment, and the tile table maps a workgroup to its
+// segment and its index inside the segment.  Each stage then derives the
+// tile's range in the array it works on (points, RLE cells, or blocks).
+// Launch wrappers enqueue on the given stream and throw xnyz::Error.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,6 +49,8 @@ struct Arena {
   }
 };
 
+constexpr u32 kTile = 4096;  // points (cells, blocks) per tile, every stage
+
 // Morton -> Hilbert digit transform as a finite-state machine over the
 // rotation group: state s is a composition of the per-digit permutations of
 // hilbert.cpp; apply[s][d] is s applied to digit d, next[s][md] the state
@@ -60,114 +66,155 @@ struct HilbertSM {
 };
 
 struct QuantParams {
-  float div;       // quantizer divisor float(double(2b)/sqrt(5 or 3))
-  float b;         // L2 bound (recover, diagnostics)
-  float rec_div;   // float(sqrt(5)) or float(sqrt(3)) (recover)
-  int to;          // 1: truncated octahedron, 0: cube
+  float div;      // quantizer divisor float(double(2b)/sqrt(5 or 3))
+  float b;        // L2 bound (recover, diagnostics)
+  float rec_div;  // float(sqrt(5)) or float(sqrt(3)) (recover)
+  int to;         // 1: truncated octahedron, 0: cube
 };
 
 struct KeyParams {
-  u64 nx;          // blocks along x: range_x / 128 + 1
-  u64 nxny;        // nx * ny
-  u64 C;           // id offset of the z = -1 segment: (2^57-1)*nx*ny mod 2^64
-  u32 idb;         // bits of the block id field
-  u32 L;           // direct mode: Hilbert iteration (bit count of max range)
+  u64 nx;    // blocks along x: range_x / 128 + 1
+  u64 nxny;  // nx * ny
+  u64 C;     // id offset of the z = -1 segment: (2^57-1)*nx*ny mod 2^64
+  u32 idb;   // bits of the block id field
+  u32 L;     // direct mode: Hilbert iteration (bit count of max range)
   int direct;
   int hilbert;
-  int wide;        // key split into (u64 hi, u32 lo)
 };
 
-// Key view used after sorting (block mode): hi = seg<<idb | id-or-base,
-// lo = quad<<18 | sfc18.
+// Variable-length / fixed-width sections of one stream, in stream order.
+enum SecKind : u32 {
+  kSecBlk = 0,   // S1 BitPacker: block-id deltas            (block mode)
+  kSecCnt = 1,   // S2 BitPacker: block-count deltas         (block mode)
+  kSecGR = 2,    // S3 Golomb-Rice: repos / sfc-delta patches
+  kSecHuf = 3,   // S4 Huffman: repos / sfc-delta PFD symbols
+  kSecQuad = 4,  // S6 Huffman: quads PFD symbols            (block mode)
+  kSecRle = 5,   // S7 Huffman / Exp-Golomb: RLE counts      (RLE)
+  kNumSec = 6
+};
+
+// Per-segment parameters (host-written, uploaded before each stage).
+struct SegParam {
+  u32 p0, n;          // points [p0, p0 + n) of the concatenated input
+  u32 tile0, ntiles;  // the segment's tiles
+  QuantParams qp;
+  KeyParams kp;       // after the range scan
+  u32 rle, diag;
+  u32 grk;            // Golomb-Rice k                 (after statistics)
+  u32 cnt_eg, egk;    // RLE counts: Exp-Golomb and its k
+  u32 w1, w2;         // BitPacker widths
+  u32 big0, nbig;     // RLE counts >= kCntBinsSeg: range in the big tables
+  u32 pad;
+  u64 sec_byte[kNumSec];  // byte offset of each data section (after sizing)
+};
+
+constexpr u32 kCntBinsSeg = 256;  // direct-indexed RLE count bins per segment
+
+// Per-segment device reductions and counters.
+struct SegCtl {
+  u64 minkey[3];     // per-axis minimum: (ordered float << 32) | index
+  float offset[3];   // the shift (exact float of the minimum, FLT_MAX if none)
+  u32 wrapped;       // nodes with z == -1
+  int range[3];      // max quantized x, y, z, from 0
+  u32 pad0;
+  u64 max_err2_bits; // RLE diagnostic: max squared error (double bits)
+  double sum_err2;   // RLE diagnostic: sum of squared errors
+  u32 e0, ne;        // coded elements: RLE cells [e0, e0+ne) (or the points)
+  u32 b0, nb;        // blocks [b0, b0+nb)
+  u32 n_patches, n_qpatches;
+  u64 patch_sum;     // wrapping u64 sum of the patches
+  u64 max_blkd, max_cntd;
+  u64 sec_bits[kNumSec];   // bits of each section
+  u64 sec_bit0[kNumSec];   // scan offset of the segment's first tile
+};
+
+struct SegHist {  // per-segment symbol counts and first occurrences
+  u32 sym[kSymBins], first_sym[kSymBins];
+  u32 quad[kSymBins], first_quad[kSymBins];
+  u32 cnt[kCntBinsSeg], first_cnt[kCntBinsSeg];
+};
+
+struct SegLut {   // per-segment codebooks (host-built)
+  HufEntry sym[kSymBins];
+  HufEntry quad[kSymBins];
+  HufEntry cnt[kCntBinsSeg];
+};
+
+struct Tile {
+  u32 seg;
+  u32 local;
+};
+
+// Device pointers shared by the stages of one call.
+struct Batch {
+  u32 S;           // segments
+  u32 ntiles;      // tiles in the table
+  u32 N;           // points in all segments
+  const Tile* tiles;
+  const SegParam* sp;
+  SegCtl* sc;
+  SegHist* sh;
+  const SegLut* lut;
+  const u32* big_vals;        // sorted per segment
+  const HufEntry* big_codes;
+  u32* large;                 // (segment, count, cell) triples of large RLE counts
+  u32* n_large;
+  u32 large_cap;
+  bool wide;                  // keys split into (u64 hi, u32 lo)
+};
+
+// Sorted keys (block mode: hi = seg<<idb | id-or-base, lo = quad<<18 | sfc).
 struct KeyView {
   const u64* a;
   const u32* b;  // wide keys only
-  u32 idb;
-  u64 C;
 };
 
-void launch_init_ctl(DevCtl* ctl, hipStream_t s);
-void launch_bbox(const float* xyz, u32 n, DevCtl* ctl, hipStream_t s);
-void launch_quant_stats(const float* xyz, u32 n, const QuantParams& qp, bool diag,
-                        DevCtl* ctl, hipStream_t s);
-void launch_quantize_nodes(const float* xyz, u32 n, const QuantParams& qp, const DevCtl* ctl,
-                           int32_t* nodes, hipStream_t s);
+void launch_init(const Batch& bt, hipStream_t s);
+void launch_bbox(const Batch& bt, const float* xyz, hipStream_t s);
+void launch_quant_stats(const Batch& bt, const float* xyz, hipStream_t s);
+void launch_quantize_nodes(const Batch& bt, const float* xyz, int32_t* nodes, hipStream_t s);
 // idx (nullable): point index payload; lo64 (nullable, wide keys only):
 // index << 32 | low key word, written instead of lo.
-void launch_keygen(const float* xyz, u32 n, const QuantParams& qp, const KeyParams& kp,
-                   const HilbertSM& hs, const DevCtl* ctl, u64* key, u32* lo, u32* idx,
-                   u64* lo64, hipStream_t s);
+void launch_keygen(const Batch& bt, const float* xyz, const HilbertSM& hs, u64* key, u32* lo,
+                   u32* idx, u64* lo64, hipStream_t s);
 void launch_split_lo64(const u64* lo64, u32* lo, u32* idx, u32 n, hipStream_t s);
 void launch_m2h(const u64* in, u32 n, u32 iteration, const HilbertSM& hs, u64* out,
                 hipStream_t s);
 
-// Exclusive scan of n values; *total (device, nullable) gets the sum.
 void scan_excl_u32(const u32* in, u32* out, u64 n, u32* total, Arena& ar, hipStream_t s);
 void scan_excl_u64(const u64* in, u64* out, u64 n, u64* total, Arena& ar, hipStream_t s);
 
-// LSD radix sort of n keys over bits [0, bits).  Narrow: u64 keys only.
-// Wide: sorts (hi, lo) pairs by lo (lo_bits) then hi (hi_bits).  Results
-// end in the returned buffer index (0: a/b, 1: a2/b2).
-int radix_sort_narrow(u64* a, u64* a2, u32 n, u32 bits, Arena& ar, hipStream_t s);
-int radix_sort_wide(u64* hi, u64* hi2, u32* lo, u32* lo2, u32 n, u32 lo_bits, u32 hi_bits,
+// Segmented LSD radix sort of each segment's keys over bits [0, bits);
+// narrow: u64 keys (+ optional u32 payload); wide: (hi u64, lo u32) by lo then
+// hi (+ optional index payload folded into lo64).  Returns the buffer index
+// holding the result (0: first arrays, 1: second).
+int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, Arena& ar,
+               hipStream_t s);
+int radix_sort_wide(const Batch& bt, u64* hi, u64* hi2, u32* lo, u32* lo2, u32 hi_bits,
                     Arena& ar, hipStream_t s);
-// Same with a point-index payload: narrow keys carry v; wide keys carry the
-// index in the upper half of lo64 (index << 32 | 21-bit low key).
-int radix_sort_narrow_v(u64* a, u64* a2, u32* v, u32* v2, u32 n, u32 bits, Arena& ar,
-                        hipStream_t s);
-int radix_sort_wide_v(u64* hi, u64* hi2, u64* lo64, u64* lo64b, u32 n, u32 hi_bits, Arena& ar,
-                      hipStream_t s);
+int radix_sort_wide_v(const Batch& bt, u64* hi, u64* hi2, u64* lo64, u64* lo64b, u32 hi_bits,
+                      Arena& ar, hipStream_t s);
 
-// Element counts produced on the device (RLE cells, blocks) are passed as
-// device pointers (`*_dev`), with a host-side upper bound for grid sizing, so
-// no host round trip is needed between the sort and the statistics.
+// RLE: first position of every run of equal keys per segment, then the cells
+// (keys) and their counts, segment by segment, with count histograms.
+void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u64* ua, u32* ub, u32* cnt,
+               Arena& ar, hipStream_t s);
+// Block runs over the coded elements and all symbol statistics.
+void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, bool block_any,
+                          u32* bpos, Arena& ar, hipStream_t s);
 
-// Positions of the first key of every run of equal keys (RLE cells) or of
-// equal block ids; the count goes to *count (device).
-void runs_unique(const KeyView& kv, bool wide, u32 n, u32* pos, u32* count, Arena& ar,
-                 hipStream_t s);
-void runs_blocks(const KeyView& kv, bool wide, u32 n_bound, const u32* n_dev, u32* pos,
-                 u32* count, Arena& ar, hipStream_t s);
-
-// RLE: gather the unique keys and per-cell counts; histogram the counts.
-void launch_rle_finish(const KeyView& in, bool wide, const u32* upos, const u32* U_dev, u32 N,
-                       u64* ua, u32* ub, u32* cnt, DevCtl* ctl, u32* large, u32 large_cap,
-                       hipStream_t s);
-
-void launch_stats_block(const KeyView& kv, bool wide, const u32* bpos, const u32* B_dev,
-                        u32 n_bound, const u32* n_dev, DevCtl* ctl, hipStream_t s);
-void launch_stats_direct(const u64* sfc, u32 n_bound, const u32* n_dev, DevCtl* ctl,
-                         hipStream_t s);
-
-// Variable-length sections: phase A (per-tile bit totals, scanned into tile
-// offsets; section total into ctl->sec_bits[sec]) and phase B (write).
-struct VlPlan {
-  u64* tile_bits;
-  u64* tile_off;
-  u32 ntiles;
+// Phase A of every section: per-tile bit counts -> tile offsets; per-segment
+// section bits and first-tile offsets into SegCtl.
+struct SecPlan {
+  u64* tile_bits[kNumSec];
+  u64* tile_off[kNumSec];
 };
-enum CoderKind { kCoderGRBlock, kCoderGRDirect, kCoderHufBlock, kCoderHufDirect,
-                 kCoderQuadHuf, kCoderCntHuf, kCoderCntEG };
-struct CoderArgs {
-  KeyView kv;
-  bool wide;
-  const u64* sfc;         // direct mode keys
-  const u32* cnt;         // RLE counts
-  const HufEntry* lut;    // 256 entries (symbols) or kCntBins entries (counts)
-  const u32* big_vals;    // RLE counts >= kCntBins: sorted values ...
-  const HufEntry* big_codes;  // ... and their codes
-  u32 n_big;
-  u32 k;                  // Golomb-Rice or Exp-Golomb parameter
-};
-VlPlan vl_plan(u32 n, Arena& ar);
-void vl_count(CoderKind kind, const CoderArgs& a, u32 n, const VlPlan& p, u64* sec_total,
-              Arena& ar, hipStream_t s);
-void vl_write(CoderKind kind, const CoderArgs& a, u32 n, const VlPlan& p, u64 sec_byte0,
-              u32* out_words, hipStream_t s);
-
-// BitPacker sections: block-id deltas (which = 0) or block-count deltas (1).
-void pack_block_deltas(int which, const KeyView& kv, bool wide, const u32* bpos, u32 B,
-                       u32 n, u32 width, u64 sec_byte0, u32* out_words, hipStream_t s);
+SecPlan plan_sections(const Batch& bt, Arena& ar);
+void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
+                   const SecPlan& p, Arena& ar, hipStream_t s);
+// Phase B: write every section at SegParam::sec_byte.
+void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
+                    const SecPlan& p, u32* out_words, hipStream_t s);
 
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s);
 
