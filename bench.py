@@ -66,9 +66,8 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
         "k_flag_count": 8 * n_points,
         "k_flag_scatter": 8 * n_points + 4 * n_coded,
         "k_rle_finish": 4 * n_coded + 8 * n_coded + 4 * n_coded,
-        "k_stats_block": 8 * n_coded,
-        "k_vl_bits": 8 * n_coded,
-        "k_vl_write": 8 * n_coded + out_bytes / max(launches, 1),
+        "k_stats": 8 * n_coded,                       # read sorted cells
+        "k_sec_bits": 8 * n_coded,
     }
     return table.get(kernel)
 

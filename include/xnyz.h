@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define XNYZ_ABI_VERSION 1
+#define XNYZ_ABI_VERSION 2
 
 enum xnyz_status {
   XNYZ_OK = 0,
@@ -60,7 +60,37 @@ typedef struct xnyz_opts {
   int32_t direct_threshold; /* default 1024 (source/main.cpp:76)            */
   int32_t frame_mode;       /* enum xnyz_frame_mode                         */
   int32_t zstd_workers;     /* XNYZ_FRAME_ZSTD3_MT only; 0 = all host cores */
+  /* Mode overrides (no reference counterpart; zero = the reference's own
+   * choices).  Streams stay decodable by the reference's decoder: the
+   * Golomb-Rice k is stored in its meta (golomb_rice_encoder.hpp:135-139)
+   * and the decoder ignores the PFD limit (p_for_delta_encoder.hpp:45-57). */
+  int32_t mode_flags;       /* XNYZ_MODE_* bits                              */
+  int32_t gr_k;             /* XNYZ_MODE_GR_K: Golomb-Rice k, 0..62          */
+  int32_t pfd_width;        /* XNYZ_MODE_PFD_WIDTH: 1..8; repos / sfc-delta
+                               values above 2^w - 2 become patches (w = 8 is
+                               the reference's limit 254)                    */
 } xnyz_opts;
+
+enum xnyz_mode_flags { XNYZ_MODE_GR_K = 1, XNYZ_MODE_PFD_WIDTH = 2 };
+
+/* One coding mode of a chunk: curve, RLE, Golomb-Rice k, PFD width. */
+typedef struct xnyz_mode {
+  int32_t is_hilbert;
+  int32_t rle;
+  int32_t gr_k;       /* 0..62 */
+  int32_t pfd_width;  /* 1..8 */
+} xnyz_mode;
+
+/* Result of the exhaustive per-chunk search.  Sizes are exact assembled
+ * (pre-zstd) stream bytes. */
+typedef struct xnyz_search_result {
+  xnyz_mode best;              /* argmin over the whole space               */
+  uint64_t best_bytes;
+  uint64_t reference_bytes;    /* the opts' curve and RLE flag with the
+                                  reference's k rule and width 8            */
+  xnyz_mode variant[4];        /* [2*is_hilbert + rle]: best (k, width)     */
+  uint64_t variant_bytes[4];
+} xnyz_search_result;
 
 /* Library-owned byte buffer; release with xnyz_free(). */
 typedef struct xnyz_buf {
@@ -135,6 +165,18 @@ int xnyz_compress_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
 int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
                                uint32_t count, const xnyz_opts* opts, xnyz_buf* frames,
                                xnyz_buf* prezstd, xnyz_stats* stats);
+
+/* Exhaustive per-chunk mode search; no reference counterpart (the
+ * reference codes every chunk with the options it is given).  For each cloud
+ * of the batch (layout as xnyz_compress_batch_device; opts supply quantizer,
+ * l2_bound, direct_threshold and the reference curve / RLE flag), the device
+ * pipeline runs up to the symbol statistics once per curve x RLE variant
+ * (four sorts), and the Golomb-Rice k (0..31 and the reference's rule) and
+ * PFD width (1..8) are costed exactly from the device histograms and the
+ * patch bit populations.  results: `count` entries. */
+int xnyz_search_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                             uint32_t count, const xnyz_opts* opts,
+                             xnyz_search_result* results);
 
 /* Replaces TruncatedOctahedronQuantizer/CubeQuantizer::quantize after
  * shift_points (source/quantizer/truncated_octahedron_quantizer.hpp:21-55,

@@ -62,6 +62,9 @@ def lib() -> C.CDLL:
         L.oracle_compress.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int, C.c_int,
                                       C.c_int, _PP, _U64P, _PP, _U64P, _U64P,
                                       C.POINTER(OracleInfo)]
+        L.oracle_compress_mode.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int, C.c_int,
+                                           C.c_int, C.c_int, C.c_int, _PP, _U64P,
+                                           C.POINTER(OracleInfo)]
         L.oracle_compress_timed.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int,
                                             C.c_int, C.c_int, C.c_int, _U64P,
                                             C.POINTER(OracleInfo)]
@@ -156,6 +159,22 @@ def compress(points, quantizer: int, l2: float, hilbert: bool, rle: bool,
         raise RuntimeError(f"oracle_compress: {_err()}")
     f = lib().oracle_free
     return Compressed(_take(pre, pl.value, f), _take(fr, fl.value, f), perm, info)
+
+
+def compress_mode(points, quantizer: int, l2: float, hilbert: bool, rle: bool,
+                  direct_threshold: int = 1024, gr_k: int = -1, pfd_width: int = 0) -> bytes:
+    """Assembled stream with the mode overrides (gr_k >= 0 forces the
+    Golomb-Rice k of the repos / sfc-delta patches; pfd_width 1..8 sets the
+    PFD limit to 2^w - 2)."""
+    a = _f32(points)
+    pre, pl = C.c_void_p(), C.c_uint64()
+    info = OracleInfo()
+    rc = lib().oracle_compress_mode(a.ctypes.data, a.shape[0], quantizer, l2, int(hilbert),
+                                    int(rle), direct_threshold, int(gr_k), int(pfd_width),
+                                    C.byref(pre), C.byref(pl), C.byref(info))
+    if rc != 0:
+        raise RuntimeError(f"oracle_compress_mode: {_err()}")
+    return _take(pre, pl.value, lib().oracle_free)
 
 
 def compress_timed(points, quantizer: int, l2: float, hilbert: bool, rle: bool,

@@ -557,16 +557,24 @@ std::vector<uint64_t> bitunpack(const Packed& in) {
   return values;
 }
 
-// source/encoder/p_for_delta_encoder.hpp:27-57 (lim = 254)
+// source/encoder/p_for_delta_encoder.hpp:27-57 (lim = 254).  `lim` below 254
+// and a forced Golomb-Rice k are the mode overrides of the repos / sfc-delta
+// stream (oracle_compress_mode); the reference always uses 254 and its k rule.
+struct ModeOverride {
+  uint64_t lim = 254;
+  int gr_k = -1;
+};
+thread_local ModeOverride g_mode;
+
 template <typename T>
 struct PFD {
   std::vector<T> patches, deltas;
 };
 template <typename T>
-PFD<T> pfd_encode(const std::vector<T>& data) {
+PFD<T> pfd_encode(const std::vector<T>& data, uint64_t lim_in = 254) {
   PFD<T> r;
   r.deltas.reserve(data.size());
-  const T lim = 254;
+  const T lim = static_cast<T>(lim_in);
   for (const auto& v : data) {
     if (v > lim) {
       r.patches.push_back(v);
@@ -608,7 +616,7 @@ struct BitWriterLSB {
 };
 
 template <typename T>
-Packed gr_pack(const std::vector<T>& values) {
+Packed gr_pack(const std::vector<T>& values, int forced_k = -1) {
   if (values.empty()) return Packed{};
   uint64_t sum = 0;
   bool all_zero = true;
@@ -624,6 +632,7 @@ Packed gr_pack(const std::vector<T>& values) {
     k = (avg < 1.0) ? 0 : static_cast<uint8_t>(std::floor(std::log2(avg)));
   }
   if (k >= 63) k = 62;
+  if (forced_k >= 0) k = static_cast<uint8_t>(forced_k);
   Packed out;
   uint32_t n = static_cast<uint32_t>(values.size());
   out.meta.resize(5);
@@ -1046,9 +1055,9 @@ std::vector<size_t> encode_direct(const std::vector<V3i>& q, int max_range,
   std::vector<uint64_t> d(n);
   d[0] = e[0].first;
   for (size_t i = 1; i < n; i++) d[i] = e[i].first - e[i - 1].first;
-  auto pf = pfd_encode(d);
+  auto pf = pfd_encode(d, g_mode.lim);
   info.patches = pf.patches.size();
-  auto gp = gr_pack(pf.patches);
+  auto gp = gr_pack(pf.patches, g_mode.gr_k);
   buf.section(gp.meta, gp.data);
   Huffman<uint64_t> h;
   h.build(pf.deltas);
@@ -1071,9 +1080,9 @@ LCPMeta encode_block(const std::vector<V3i>& q, int rx, int ry, int rz,
   buf.section(p1.meta, p1.data);
   auto p2 = bitpack(cnt);
   buf.section(p2.meta, p2.data);
-  auto pr = pfd_encode(M.repos);
+  auto pr = pfd_encode(M.repos, g_mode.lim);
   info.patches = pr.patches.size();
-  auto g1 = gr_pack(pr.patches);
+  auto g1 = gr_pack(pr.patches, g_mode.gr_k);
   buf.section(g1.meta, g1.data);
   Huffman<uint64_t> h1;
   h1.build(pr.deltas);
@@ -1344,6 +1353,26 @@ int oracle_compress(const float* xyz, uint64_t n, int quantizer, float l2_bound,
     }
     if (perm && !rle)
       for (uint64_t i = 0; i < n; i++) perm[i] = R.perm[i];
+    if (info) *info = R.info;
+    return 0;
+  });
+}
+
+int oracle_compress_mode(const float* xyz, uint64_t n, int quantizer, float l2_bound,
+                         int is_hilbert, int rle, int direct_threshold, int gr_k, int pfd_width,
+                         uint8_t** pre, uint64_t* pre_len, oracle_info* info) {
+  return guarded([&] {
+    if (n == 0) return fail("empty input");
+    if (pfd_width < 0 || pfd_width > 8 || gr_k > 62) return fail("bad mode override");
+    g_mode.lim = pfd_width ? (1ull << pfd_width) - 2 : 254;
+    g_mode.gr_k = gr_k;
+    struct Reset {
+      ~Reset() { g_mode = ModeOverride{}; }
+    } reset;
+    auto R = compress(to_points(xyz, n), quantizer, l2_bound, is_hilbert != 0, rle != 0,
+                      direct_threshold);
+    *pre = dup_bytes(R.pre);
+    *pre_len = R.pre.size();
     if (info) *info = R.info;
     return 0;
   });

@@ -50,6 +50,14 @@ int oracle_compress(const float* xyz, uint64_t n, int quantizer, float l2_bound,
                     uint64_t* perm, oracle_info* info);
 
 /* Same pipeline, timed; `with_zstd` 0 stops at the assembled buffer. */
+/* oracle_compress with the mode overrides of the repos / sfc-delta stream
+ * (test infrastructure for the mode search): gr_k < 0 keeps the reference's
+ * Golomb-Rice k rule; pfd_width 0 keeps the PFD limit 254, 1..8 sets it to
+ * 2^w - 2.  Assembled stream only. */
+int oracle_compress_mode(const float* xyz, uint64_t n, int quantizer, float l2_bound,
+                         int is_hilbert, int rle, int direct_threshold, int gr_k, int pfd_width,
+                         uint8_t** pre, uint64_t* pre_len, oracle_info* info);
+
 int oracle_compress_timed(const float* xyz, uint64_t n, int quantizer,
                           float l2_bound, int is_hilbert, int rle,
                           int direct_threshold, int with_zstd,

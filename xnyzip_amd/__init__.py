@@ -23,7 +23,7 @@ from ._lib import Context, XnyzError  # noqa: F401
 
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
-           "compress", "compress_batch", "quantize", "morton_to_hilbert"]
+           "compress", "compress_batch", "search_batch", "quantize", "morton_to_hilbert"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -41,21 +41,28 @@ def default_context(device: int = 0) -> Context:
     return ctx
 
 
-def _opts(quantizer, l2_bound, is_hilbert, rle, direct_threshold, frame_mode, zstd_workers):
+def _opts(quantizer, l2_bound, is_hilbert, rle, direct_threshold, frame_mode, zstd_workers,
+          gr_k=None, pfd_width=None):
+    flags = (_lib.MODE_GR_K if gr_k is not None else 0) | \
+        (_lib.MODE_PFD_WIDTH if pfd_width is not None else 0)
     return _lib.Opts(int(quantizer), float(l2_bound), int(bool(is_hilbert)), int(bool(rle)),
-                     int(direct_threshold), int(frame_mode), int(zstd_workers))
+                     int(direct_threshold), int(frame_mode), int(zstd_workers), flags,
+                     int(gr_k or 0), int(pfd_width or 0))
 
 
 def compress(points, quantizer=QUANTIZER_TYPE.TRUNCATED_OCTAHEDRON, l2_bound=0.01,
              is_hilbert=False, rle=False, direct_threshold=1024,
              frame_mode=_lib.FRAME_ZSTD3, zstd_workers=0, want_prezstd=False,
-             want_perm=False, ctx: Context | None = None):
+             want_perm=False, ctx: Context | None = None, gr_k=None, pfd_width=None):
     """Compress one cloud.  Returns (frame, prezstd or None, perm or None, stats).
 
     A torch tensor on a HIP device is compressed in place (no copy); any other
-    input is converted to a contiguous float32 host array and uploaded."""
+    input is converted to a contiguous float32 host array and uploaded.
+    gr_k / pfd_width force the Golomb-Rice k and the PFD width of the repos
+    (or sfc-delta) stream; None keeps the reference's choice."""
     L = _lib.lib()
-    o = _opts(quantizer, l2_bound, is_hilbert, rle, direct_threshold, frame_mode, zstd_workers)
+    o = _opts(quantizer, l2_bound, is_hilbert, rle, direct_threshold, frame_mode, zstd_workers,
+              gr_k, pfd_width)
     fr, pre, st = _lib.Buf(), _lib.Buf(), _lib.Stats()
     dev_ptr, n, keep = None, 0, None
     try:
@@ -117,7 +124,8 @@ def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
     for i, o in enumerate(opts):
         arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
                        o.get("rle", False), o.get("direct_threshold", 1024),
-                       o.get("frame_mode", _lib.FRAME_ZSTD3), o.get("zstd_workers", 0))
+                       o.get("frame_mode", _lib.FRAME_ZSTD3), o.get("zstd_workers", 0),
+                       o.get("gr_k"), o.get("pfd_width"))
     frames = (_lib.Buf * count)()
     pres = (_lib.Buf * count)() if want_prezstd else None
     stats = (_lib.Stats * count)()
@@ -129,6 +137,39 @@ def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
     out_frames = [_lib.take(f) for f in frames]
     out_pres = [_lib.take(p) for p in pres] if pres is not None else None
     return out_frames, out_pres, list(stats)
+
+
+def search_batch(points_device, offsets, opts, ctx: Context | None = None):
+    """Exhaustive per-chunk mode search (curve x RLE x Golomb-Rice k x PFD
+    width) over the clouds of a batch; sizes are exact assembled-stream bytes.
+    opts: one dict per cloud (quantizer, l2_bound, optional direct_threshold,
+    and the reference's is_hilbert / rle for `reference_bytes`).  Returns one
+    dict per cloud: best (mode dict), best_bytes, reference_bytes, variants."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    count = len(offsets) - 1
+    if count <= 0 or len(opts) != count:
+        raise ValueError("need one opts entry per cloud")
+    ctx = ctx or default_context(t.device.index or 0)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    arr = (_lib.Opts * count)()
+    for i, o in enumerate(opts):
+        arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
+                       o.get("rle", False), o.get("direct_threshold", 1024), _lib.FRAME_NONE, 0)
+    res = (_lib.SearchResult * count)()
+    torch.cuda.synchronize(t.device)
+    _lib.check(_lib.lib().xnyz_search_batch_device(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.cast(res, C.c_void_p)))
+    out = []
+    for r in res:
+        out.append({"best": r.best.as_dict(), "best_bytes": r.best_bytes,
+                    "reference_bytes": r.reference_bytes,
+                    "variants": [dict(r.variant[v].as_dict(), bytes=r.variant_bytes[v])
+                                 for v in range(4)]})
+    return out
 
 
 class BlockCompressor:

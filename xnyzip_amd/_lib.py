@@ -26,7 +26,24 @@ class XnyzError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("quantizer", C.c_int32), ("l2_bound", C.c_float), ("is_hilbert", C.c_int32),
                 ("rle", C.c_int32), ("direct_threshold", C.c_int32),
-                ("frame_mode", C.c_int32), ("zstd_workers", C.c_int32)]
+                ("frame_mode", C.c_int32), ("zstd_workers", C.c_int32),
+                ("mode_flags", C.c_int32), ("gr_k", C.c_int32), ("pfd_width", C.c_int32)]
+
+
+MODE_GR_K, MODE_PFD_WIDTH = 1, 2
+
+
+class Mode(C.Structure):
+    _fields_ = [("is_hilbert", C.c_int32), ("rle", C.c_int32), ("gr_k", C.c_int32),
+                ("pfd_width", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SearchResult(C.Structure):
+    _fields_ = [("best", Mode), ("best_bytes", C.c_uint64), ("reference_bytes", C.c_uint64),
+                ("variant", Mode * 4), ("variant_bytes", C.c_uint64 * 4)]
 
 
 class Buf(C.Structure):
@@ -57,6 +74,7 @@ _lock = threading.Lock()
 
 EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress", "xnyz_compress_device", "xnyz_compress_batch_device",
+           "xnyz_search_batch_device",
            "xnyz_quantize_device",
            "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_profile_enable",
            "xnyz_profile_report", "xnyz_free"]
@@ -80,6 +98,7 @@ def lib() -> C.CDLL:
         L.xnyz_compress.argtypes = comp
         L.xnyz_compress_device.argtypes = comp
         L.xnyz_compress_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, P, P]
+        L.xnyz_search_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
         L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]

@@ -35,6 +35,7 @@
 #include "codebook.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "modes.hpp"
 
 // Exported by libzstd 1.4 (zstd_compress.c) though absent from its public
 // header: makes the next block start without repeat offsets, so a chunk
@@ -374,6 +375,12 @@ void validate(const xnyz_opts* o, u64 n) {
     throw Error(XNYZ_EINVAL, "l2_bound must be a positive finite float");
   if (o->frame_mode < XNYZ_FRAME_ZSTD3 || o->frame_mode > XNYZ_FRAME_NONE)
     throw Error(XNYZ_EINVAL, "bad frame_mode");
+  if (o->mode_flags & ~(XNYZ_MODE_GR_K | XNYZ_MODE_PFD_WIDTH))
+    throw Error(XNYZ_EINVAL, "unknown mode_flags bits");
+  if ((o->mode_flags & XNYZ_MODE_GR_K) && (o->gr_k < 0 || o->gr_k > 62))
+    throw Error(XNYZ_EINVAL, "gr_k must be in 0..62");
+  if ((o->mode_flags & XNYZ_MODE_PFD_WIDTH) && (o->pfd_width < 1 || o->pfd_width > 8))
+    throw Error(XNYZ_EINVAL, "pfd_width must be in 1..8");
   if (n == 0) throw Error(XNYZ_EINVAL, "empty input (the reference indexes an empty vector)");
   if (n >= 0xFFFFFFFFull) throw Error(XNYZ_EUNSUPPORTED, "more than 2^32-2 points in one call");
 }
@@ -475,6 +482,9 @@ struct CallOut {
   xnyz_buf* pre;
   xnyz_stats* stats;
   uint32_t* perm;  // single segment, non-RLE only
+  // Mode search: stop after the statistics and return each segment's size
+  // model instead of a stream.
+  std::vector<ModeStats>* search = nullptr;
 };
 
 int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off,
@@ -514,7 +524,9 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       sp[s].ntiles = (n + kTile - 1) / kTile;
       sp[s].qp = quant_params(&opts[s]);
       sp[s].rle = opts[s].rle != 0;
-      sp[s].diag = sp[s].rle;
+      sp[s].diag = sp[s].rle && !co.search;
+      sp[s].lim = (opts[s].mode_flags & XNYZ_MODE_PFD_WIDTH) ? pfd_limit(opts[s].pfd_width) : 254;
+      sp[s].search = co.search != nullptr;
       for (u32 l = 0; l < sp[s].ntiles; ++l) tiles[t++] = Tile{s, l};
       g[s].n = n;
       g[s].rle = sp[s].rle;
@@ -694,7 +706,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       h.w1 = c.max_blkd ? bitlen(c.max_blkd) : 1;
       h.w2 = c.max_cntd ? bitlen(c.max_cntd) : 1;
     }
-    h.grk = golomb_rice_k(h.P, c.patch_sum, false);
+    h.grk = (opts[s].mode_flags & XNYZ_MODE_GR_K) ? (uint8_t)opts[s].gr_k
+                                                   : golomb_rice_k(h.P, c.patch_sum, false);
     if (h.rle) {
       std::vector<SymCount> cs = sym_counts(hh.cnt, hh.first_cnt, kCntBinsSeg);
       std::sort(h.large.begin(), h.large.end());
@@ -728,6 +741,26 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
         h.cb_cnt.max_length() > 56)
       throw Error(XNYZ_EUNSUPPORTED, "Huffman code longer than 56 bits");
   });
+  if (co.search) {  // size model per segment (modes.hpp)
+    co.search->resize(S);
+    for (u32 s = 0; s < S; ++s) {
+      const SegHost& h = g[s];
+      const SegHist& hh = shh[s];
+      ModeStats& m = (*co.search)[s];
+      for (u32 i = 0; i < kSymBins; ++i) m.hist[i] = hh.sym[i];
+      for (u32 b = 0; b < 64; ++b) m.bitpop[b] = hh.bitpop[b];
+      m.patch_sum = sc[s].patch_sum;
+      auto sect = [](u64 meta, u64 bits) { return 16 + meta + (bits + 7) / 8; };
+      u64 fixed = 30 + 12;
+      if (!h.direct) {
+        fixed += sect(5, h.B * h.w1) + sect(5, h.B * h.w2) + 16 +
+                 sect(h.cb_quad.meta().size(), h.cb_quad.data_bits());
+      }
+      if (h.rle) fixed += sect(h.cb_cnt.meta().size(), h.cb_cnt.data_bits());
+      m.fixed_bytes = fixed;
+    }
+    return XNYZ_OK;
+  }
   std::vector<u32> big_vals;
   std::vector<HufEntry> big_codes;
   for (u32 s = 0; s < S; ++s) {
@@ -976,6 +1009,58 @@ int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t
     use_device(ctx);
     return compress_impl(ctx, d_xyz, off, opts, CallOut{frames, prezstd, stats, nullptr},
                          now_ms());
+  });
+}
+
+int xnyz_search_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                             uint32_t count, const xnyz_opts* opts,
+                             xnyz_search_result* results) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !results)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off(offsets, offsets + count + 1);
+    for (u32 s = 0; s < count; ++s)
+      if (off[s + 1] < off[s]) throw Error(XNYZ_EINVAL, "offsets must not decrease");
+    if (off[0] != 0) {
+      for (auto& v : off) v -= offsets[0];
+      d_xyz += 3 * offsets[0];
+    }
+    use_device(ctx);
+    for (u32 s = 0; s < count; ++s) {
+      std::memset(&results[s], 0, sizeof(results[s]));
+      results[s].best_bytes = ~0ull;
+    }
+    std::vector<xnyz_opts> vo(opts, opts + count);
+    for (int var = 0; var < 4; ++var) {  // 2 * is_hilbert + rle
+      for (u32 s = 0; s < count; ++s) {
+        vo[s] = opts[s];
+        vo[s].is_hilbert = var >> 1;
+        vo[s].rle = var & 1;
+        vo[s].mode_flags = 0;
+      }
+      std::vector<ModeStats> ms;
+      CallOut co{nullptr, nullptr, nullptr, nullptr};
+      co.search = &ms;
+      compress_impl(ctx, d_xyz, off, vo.data(), co, now_ms());
+      parallel_for(count, host_threads(0), [&](size_t s) {
+        ModeChoice ref{}, best{};
+        search_modes(ms[s], ref, best);
+        xnyz_search_result& r = results[s];
+        r.variant[var] = xnyz_mode{var >> 1, var & 1, best.k, best.w};
+        r.variant_bytes[var] = best.bytes;
+        if ((opts[s].is_hilbert != 0) == (var >> 1) && (opts[s].rle != 0) == (var & 1))
+          r.reference_bytes = ref.bytes;
+      });
+    }
+    for (u32 s = 0; s < count; ++s) {
+      xnyz_search_result& r = results[s];
+      for (int var = 0; var < 4; ++var)
+        if (r.variant_bytes[var] < r.best_bytes) {
+          r.best_bytes = r.variant_bytes[var];
+          r.best = r.variant[var];
+        }
+    }
+    return XNYZ_OK;
   });
 }
 

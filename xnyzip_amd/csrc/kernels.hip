@@ -71,6 +71,59 @@ __device__ __forceinline__ Range tile_range(u32 first, u32 count, u32 local) {
 
 enum Space { kPoints = 0, kCells = 1, kBlocks = 2 };
 
+// Reduction kernels run persistent workgroups: workgroup b walks the
+// contiguous tile span [b*T/G, (b+1)*T/G), accumulates in registers / LDS
+// while the segment stays the same, and flushes to the segment's counters
+// once per segment it touches -- a few thousand device atomics per launch
+// instead of one per wavefront and tile.
+constexpr u32 kPersistBlocks = 2048;  // 8 workgroups per CU on 256 CUs
+inline u32 persist_grid(u32 ntiles) { return ntiles < kPersistBlocks ? ntiles : kPersistBlocks; }
+
+__device__ __forceinline__ void block_tiles(u32 ntiles, u32& t0, u32& t1) {
+  t0 = (u32)((u64)blockIdx.x * ntiles / gridDim.x);
+  t1 = (u32)((u64)(blockIdx.x + 1) * ntiles / gridDim.x);
+}
+
+// Block-wide min / max; every thread gets the result.  `sh` holds
+// kWavesPerBlock entries and may be reused right after the call.
+template <typename T>
+__device__ __forceinline__ T block_min(T v, T* sh) {
+  v = wave_min(v);
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T r = sh[0];
+#pragma unroll
+  for (int i = 1; i < kWavesPerBlock; ++i) r = sh[i] < r ? sh[i] : r;
+  __syncthreads();
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T* sh) {
+  v = wave_max(v);
+  if (lane_id() == 0) sh[threadIdx.x / kWave] = v;
+  __syncthreads();
+  T r = sh[0];
+#pragma unroll
+  for (int i = 1; i < kWavesPerBlock; ++i) r = sh[i] > r ? sh[i] : r;
+  __syncthreads();
+  return r;
+}
+
+// Monotone device counters: skip the atomic when the (possibly stale) current
+// value already dominates -- a stale read only costs an extra atomic.
+__device__ __forceinline__ void min_to(u64* a, u64 v) {
+  if (v < __atomic_load_n(a, __ATOMIC_RELAXED)) atomicMin(a, v);
+}
+__device__ __forceinline__ void min_to(u32* a, u32 v) {
+  if (v < __atomic_load_n(a, __ATOMIC_RELAXED)) atomicMin(a, v);
+}
+__device__ __forceinline__ void max_to(u64* a, u64 v) {
+  if (v > __atomic_load_n(a, __ATOMIC_RELAXED)) atomicMax(a, v);
+}
+__device__ __forceinline__ void max_to(int* a, int v) {
+  if (v > __atomic_load_n(a, __ATOMIC_RELAXED)) atomicMax(a, v);
+}
+
 __device__ __forceinline__ Range stage_range(const SegParam& sp, const SegCtl& sc, u32 local,
                                              Space sp_kind) {
   if (sp_kind == kPoints) return tile_range(sp.p0, sp.n, local);
@@ -99,7 +152,8 @@ __global__ void k_init(Batch bt) {
                         w < offsetof(SegHist, quad) / 4) ||
                        (w >= offsetof(SegHist, first_quad) / 4 &&
                         w < offsetof(SegHist, cnt) / 4) ||
-                       (w >= offsetof(SegHist, first_cnt) / 4);
+                       (w >= offsetof(SegHist, first_cnt) / 4 &&
+                        w < offsetof(SegHist, bitpop) / 4);
     reinterpret_cast<u32*>(bt.sh)[i] = first ? 0xFFFFFFFFu : 0u;
   }
   if (tid == 0) *bt.n_large = 0;
@@ -135,32 +189,44 @@ __device__ __forceinline__ u64 min_key(float v, u32 i) {
   return ((u64)o << 32) | i;
 }
 
+__device__ __forceinline__ void flush_bbox(SegCtl& c, u64 (&b)[3], u64* sh) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const u64 m = block_min(b[a], sh);
+    if (threadIdx.x == 0 && m != ~0ull) min_to(&c.minkey[a], m);
+    b[a] = ~0ull;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_bbox(Batch bt, const F3* __restrict__ p) {
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  const Range r = tile_range(sp.p0, sp.n, t.local);
-  u64 bx = ~0ull, by = ~0ull, bz = ~0ull;
+  __shared__ u64 sh[kWavesPerBlock];
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
+  u64 b[3] = {~0ull, ~0ull, ~0ull};
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_bbox(bt.sc[cur], b, sh);
+      cur = t.seg;
+    }
+    const SegParam& sp = bt.sp[t.seg];
+    const Range r = tile_range(sp.p0, sp.n, t.local);
 #pragma unroll 4
-  for (u32 k = 0; k < kItems; ++k) {
-    const u32 o = k * kBlock + threadIdx.x;
-    if (o < r.count) {
-      const u32 i = r.first + o;
-      const F3 q = p[i];
-      const u64 kx = min_key(q.x, i), ky = min_key(q.y, i), kz = min_key(q.z, i);
-      bx = kx < bx ? kx : bx;
-      by = ky < by ? ky : by;
-      bz = kz < bz ? kz : bz;
+    for (u32 k = 0; k < kItems; ++k) {
+      const u32 o = k * kBlock + threadIdx.x;
+      if (o < r.count) {
+        const u32 i = r.first + o;
+        const F3 q = p[i];
+        const u64 kx = min_key(q.x, i), ky = min_key(q.y, i), kz = min_key(q.z, i);
+        b[0] = kx < b[0] ? kx : b[0];
+        b[1] = ky < b[1] ? ky : b[1];
+        b[2] = kz < b[2] ? kz : b[2];
+      }
     }
   }
-  bx = wave_min(bx);
-  by = wave_min(by);
-  bz = wave_min(bz);
-  if (lane_id() == 0) {
-    SegCtl& c = bt.sc[t.seg];
-    if (bx != ~0ull) atomicMin(&c.minkey[0], bx);
-    if (by != ~0ull) atomicMin(&c.minkey[1], by);
-    if (bz != ~0ull) atomicMin(&c.minkey[2], bz);
-  }
+  flush_bbox(bt.sc[cur], b, sh);
 }
 
 // The offset is the winner's exact float (keeps the sign of a zero).
@@ -243,56 +309,80 @@ __device__ __forceinline__ void recover_node(int X, int Y, int Z, const QuantPar
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_quant_stats(Batch bt, const F3* __restrict__ p) {
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  SegCtl& sc = bt.sc[t.seg];
-  const Range r = tile_range(sp.p0, sp.n, t.local);
-  const QuantParams qp = sp.qp;
-  const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
-  const bool diag = sp.diag != 0;
-  int mx = 0, my = 0, mz = 0;
-  u32 wr = 0;
-  double se = 0.0, me = 0.0;
-  for (u32 k = 0; k < kItems; ++k) {
-    const u32 o = k * kBlock + threadIdx.x;
-    if (o >= r.count) break;
-    float sx, sy, sz;
-    int X, Y, Z;
-    quant_point(p[r.first + o], off, qp, sx, sy, sz, X, Y, Z);
-    mx = X > mx ? X : mx;
-    my = Y > my ? Y : my;
-    mz = Z > mz ? Z : mz;
-    wr += (Z == -1);
-    if (diag) {
-      float rx, ry, rz;
-      recover_node(X, Y, Z, qp, rx, ry, rz);
-      const float a = sx - rx, b = sy - ry, c = sz - rz;
-      // block_compressor_rle.hpp:60-62: float products, left-to-right sum
-      const float m = __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
-      const double md = (double)m;
-      se += md;
-      me = md > me ? md : me;
-    }
+struct QStat {
+  int m[3];
+  u32 wr;
+  double se, me;
+};
+
+__device__ __forceinline__ void flush_qstat(SegCtl& c, QStat& q, bool diag, void* shv) {
+  int* shi = reinterpret_cast<int*>(shv);
+  u32* shu = reinterpret_cast<u32*>(shv);
+  double* shd = reinterpret_cast<double*>(shv);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int m = block_max(q.m[a], shi);
+    if (threadIdx.x == 0) max_to(&c.range[a], m);
+    q.m[a] = 0;
   }
-  mx = wave_max(mx);
-  my = wave_max(my);
-  mz = wave_max(mz);
-  wr = wave_sum(wr);
+  const u32 wr = block_sum(q.wr, shu);
+  if (threadIdx.x == 0 && wr) atomicAdd(&c.wrapped, wr);
+  q.wr = 0;
   if (diag) {
-    se = wave_sum(se);
-    me = wave_max(me);
+    const double se = block_sum(q.se, shd);
+    const double me = block_max(q.me, shd);
+    if (threadIdx.x == 0) {
+      atomicAdd(&c.sum_err2, se);
+      max_to(&c.max_err2_bits, (u64)__double_as_longlong(me));
+    }
+    q.se = q.me = 0.0;
   }
-  if (lane_id() == 0) {
-    atomicMax(&sc.range[0], mx);
-    atomicMax(&sc.range[1], my);
-    atomicMax(&sc.range[2], mz);
-    if (wr) atomicAdd(&sc.wrapped, wr);
-    if (diag) {
-      atomicAdd(&sc.sum_err2, se);
-      atomicMax(&sc.max_err2_bits, (u64)__double_as_longlong(me));
+}
+
+__global__ __launch_bounds__(kBlock) void k_quant_stats(Batch bt, const F3* __restrict__ p) {
+  __shared__ double sh[kWavesPerBlock];
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
+  QStat q{{0, 0, 0}, 0, 0.0, 0.0};
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_qstat(bt.sc[cur], q, bt.sp[cur].diag != 0, sh);
+      cur = t.seg;
+    }
+    const SegParam& sp = bt.sp[t.seg];
+    const SegCtl& sc = bt.sc[t.seg];
+    const Range r = tile_range(sp.p0, sp.n, t.local);
+    const QuantParams qp = sp.qp;
+    const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
+    const bool diag = sp.diag != 0;
+#pragma unroll 4
+    for (u32 k = 0; k < kItems; ++k) {
+      const u32 o = k * kBlock + threadIdx.x;
+      if (o < r.count) {
+        float sx, sy, sz;
+        int X, Y, Z;
+        quant_point(p[r.first + o], off, qp, sx, sy, sz, X, Y, Z);
+        q.m[0] = X > q.m[0] ? X : q.m[0];
+        q.m[1] = Y > q.m[1] ? Y : q.m[1];
+        q.m[2] = Z > q.m[2] ? Z : q.m[2];
+        q.wr += (Z == -1);
+        if (diag) {
+          float rx, ry, rz;
+          recover_node(X, Y, Z, qp, rx, ry, rz);
+          const float a = sx - rx, b = sy - ry, c = sz - rz;
+          // block_compressor_rle.hpp:60-62: float products, left-to-right sum
+          const float m = __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
+          const double md = (double)m;
+          q.se += md;
+          q.me = md > q.me ? md : q.me;
+        }
+      }
     }
   }
+  flush_qstat(bt.sc[cur], q, bt.sp[cur].diag != 0, sh);
 }
 
 __global__ __launch_bounds__(kBlock) void k_quantize_nodes(Batch bt, const F3* __restrict__ p,
@@ -302,16 +392,18 @@ __global__ __launch_bounds__(kBlock) void k_quantize_nodes(Batch bt, const F3* _
   const SegCtl& sc = bt.sc[t.seg];
   const Range r = tile_range(sp.p0, sp.n, t.local);
   const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
+#pragma unroll 4
   for (u32 k = 0; k < kItems; ++k) {
     const u32 o = k * kBlock + threadIdx.x;
-    if (o >= r.count) break;
-    const u32 i = r.first + o;
-    float sx, sy, sz;
-    int X, Y, Z;
-    quant_point(p[i], off, sp.qp, sx, sy, sz, X, Y, Z);
-    nodes[3ull * i] = X;
-    nodes[3ull * i + 1] = Y;
-    nodes[3ull * i + 2] = Z;
+    if (o < r.count) {
+      const u32 i = r.first + o;
+      float sx, sy, sz;
+      int X, Y, Z;
+      quant_point(p[i], off, sp.qp, sx, sy, sz, X, Y, Z);
+      nodes[3ull * i] = X;
+      nodes[3ull * i + 1] = Y;
+      nodes[3ull * i + 2] = Z;
+    }
   }
 }
 
@@ -403,45 +495,47 @@ __global__ __launch_bounds__(kBlock) void k_keygen(Batch bt, const F3* __restric
   const Range r = tile_range(sp.p0, sp.n, tl.local);
   const KeyParams kp = sp.kp;
   const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
+#pragma unroll 4
   for (u32 k = 0; k < kItems; ++k) {
     const u32 o = k * kBlock + threadIdx.x;
-    if (o >= r.count) break;
-    const u32 i = r.first + o;
-    float sx, sy, sz;
-    int X, Y, Z;
-    quant_point(p[i], off, sp.qp, sx, sy, sz, X, Y, Z);
-    if (idx) idx[i] = i;  // point index payload (permutation output)
-    u64 hi;
-    u32 k21;
-    if (kp.direct) {
-      const u64 m = split3_21((u32)X) | (split3_21((u32)Y) << 1) | (split3_21((u32)Z) << 2);
-      const u64 sfc = kp.hilbert ? hilbert_general(m, kp.L, t, hs.identity) : m;
-      if (!bt.wide) {
-        key[i] = sfc;
-        continue;
+    if (o < r.count) {
+      const u32 i = r.first + o;
+      float sx, sy, sz;
+      int X, Y, Z;
+      quant_point(p[i], off, sp.qp, sx, sy, sz, X, Y, Z);
+      if (idx) idx[i] = i;  // point index payload (permutation output)
+      u64 hi;
+      u32 k21;
+      if (kp.direct) {
+        const u64 m = split3_21((u32)X) | (split3_21((u32)Y) << 1) | (split3_21((u32)Z) << 2);
+        const u64 sfc = kp.hilbert ? hilbert_general(m, kp.L, t, hs.identity) : m;
+        if (!bt.wide) {
+          key[i] = sfc;
+          continue;
+        }
+        hi = sfc;
+        k21 = 0;
+      } else {
+        const u64 ux = (u64)(i64)X, uy = (u64)(i64)Y, uz = (u64)(i64)Z;
+        const u64 cx = ux >> 6, cy = uy >> 6, cz = uz >> 6;
+        const u32 m = split3_6((u32)(ux & 63)) | (split3_6((u32)(uy & 63)) << 1) |
+                      (split3_6((u32)(uz & 63)) << 2);
+        const u32 sfc = kp.hilbert ? hilbert18(m, t, hs.block_init) : m;
+        const u32 quad = (u32)((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2));
+        const u64 base = (cx >> 1) + (cy >> 1) * kp.nx;
+        const bool seg = (Z == -1);
+        const u64 v = seg ? base : base + (cz >> 1) * kp.nxny;
+        hi = ((u64)seg << kp.idb) | v;
+        k21 = (quad << 18) | sfc;
+        if (!bt.wide) {
+          key[i] = (hi << 21) | k21;
+          continue;
+        }
       }
-      hi = sfc;
-      k21 = 0;
-    } else {
-      const u64 ux = (u64)(i64)X, uy = (u64)(i64)Y, uz = (u64)(i64)Z;
-      const u64 cx = ux >> 6, cy = uy >> 6, cz = uz >> 6;
-      const u32 m = split3_6((u32)(ux & 63)) | (split3_6((u32)(uy & 63)) << 1) |
-                    (split3_6((u32)(uz & 63)) << 2);
-      const u32 sfc = kp.hilbert ? hilbert18(m, t, hs.block_init) : m;
-      const u32 quad = (u32)((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2));
-      const u64 base = (cx >> 1) + (cy >> 1) * kp.nx;
-      const bool seg = (Z == -1);
-      const u64 v = seg ? base : base + (cz >> 1) * kp.nxny;
-      hi = ((u64)seg << kp.idb) | v;
-      k21 = (quad << 18) | sfc;
-      if (!bt.wide) {
-        key[i] = (hi << 21) | k21;
-        continue;
-      }
+      key[i] = hi;
+      if (lo64) lo64[i] = ((u64)i << 32) | k21;  // wide keys + permutation
+      else lo[i] = k21;
     }
-    key[i] = hi;
-    if (lo64) lo64[i] = ((u64)i << 32) | k21;  // wide keys + permutation
-    else lo[i] = k21;
   }
 }
 
@@ -576,6 +670,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
                                                           const u32* __restrict__ offs) {
   __shared__ u32 wc[kWavesPerBlock][256];
   __shared__ u32 wb[kWavesPerBlock][256];
+  __shared__ u32 ds[256], gb[256], sc4[kWavesPerBlock];
+  __shared__ u64 stage[kTile];  // the tile's keys, then its values, in output order
+  static_assert(sizeof(K) <= 8 && sizeof(V) <= 8, "staging holds 64-bit items");
+  K* stage_k = reinterpret_cast<K*>(stage);
   for (u32 i = threadIdx.x; i < kWavesPerBlock * 256; i += kBlock) (&wc[0][0])[i] = 0;
   __syncthreads();
   const Tile t = bt.tiles[blockIdx.x];
@@ -584,6 +682,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
   const u32 w = threadIdx.x / kWave, l = lane_id();
   const u64 lt = (1ull << l) - 1;
   const u32 wstart = w * kWaveSub;
+  // The tile's global digit offsets are strided in memory; start that load
+  // now so its latency overlaps the key loads and the ranking.
+  const u32 my_off = offs[(u64)256 * sp.tile0 + (u64)threadIdx.x * sp.ntiles + t.local];
   K k[kItems];
   V v[kItems];
   u32 rk[kItems];
@@ -609,9 +710,19 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
     dg[s] = d;
   }
   __syncthreads();
+  // Tile-local order: digit-major, then (wave, slot, lane) = input order, so
+  // the permutation is stable.  wb[w][d] = tile-local start of wave w's run
+  // of digit d; gb[d] - ds[d] maps a tile-local slot of digit d to its
+  // global position.
   {
     const u32 d = threadIdx.x;
-    u32 acc = offs[(u64)256 * sp.tile0 + (u64)d * sp.ntiles + t.local];
+    u32 tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWavesPerBlock; ++ww) tot += wc[ww][d];
+    u32 all;
+    u32 acc = block_excl_scan(tot, sc4, all);
+    ds[d] = acc;
+    gb[d] = my_off;
 #pragma unroll
     for (int ww = 0; ww < kWavesPerBlock; ++ww) {
       wb[ww][d] = acc;
@@ -619,13 +730,37 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
     }
   }
   __syncthreads();
+  u32 lp[kItems];
 #pragma unroll
   for (u32 s = 0; s < kItems; ++s) {
-    const u32 o = wstart + s * kWave + l;
-    if (o < r.count) {
-      const u32 dst = wb[w][dg[s]] + rk[s];
-      kout[dst] = k[s];
-      if (kHasV) vout[dst] = v[s];
+    lp[s] = wb[w][dg[s]] + rk[s];
+    if (wstart + s * kWave + l < r.count) stage_k[lp[s]] = k[s];
+  }
+  __syncthreads();
+  // Slot i of the staged tile goes to gb[d] + (i - ds[d]): runs of equal
+  // digits are written by consecutive lanes.
+  u32 dst[kItems];
+#pragma unroll
+  for (u32 s = 0; s < kItems; ++s) {
+    const u32 i = s * kBlock + threadIdx.x;
+    if (i < r.count) {
+      const K key = stage_k[i];
+      const u32 d = (u32)(key >> shift) & 255u;
+      dst[s] = gb[d] + (i - ds[d]);
+      kout[dst[s]] = key;
+    }
+  }
+  if (kHasV) {
+    __syncthreads();
+    V* stage_v = reinterpret_cast<V*>(stage_k);
+#pragma unroll
+    for (u32 s = 0; s < kItems; ++s)
+      if (wstart + s * kWave + l < r.count) stage_v[lp[s]] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (u32 s = 0; s < kItems; ++s) {
+      const u32 i = s * kBlock + threadIdx.x;
+      if (i < r.count) vout[dst[s]] = stage_v[i];
     }
   }
 }
@@ -725,20 +860,31 @@ __global__ __launch_bounds__(kBlock) void k_flag_scatter(Batch bt, KV k, const u
   const Range r = flag_range<kKind>(bt, t);
   const u32 first = r.count ? r.first - (u32)((u64)t.local * kTile) : 0;
   const bool all = kKind == kFlagCell && !bt.sp[t.seg].rle;
-  const u32 base = threadIdx.x * kItems;
-  u32 bits = 0, c = 0;
+  // Wavefront w owns elements [w*kWaveSub, (w+1)*kWaveSub) of the tile,
+  // visited 64 at a time (coalesced); ballots give the in-order ranks.
+  const u32 w = threadIdx.x / kWave, l = lane_id();
+  const u64 lt = (1ull << l) - 1;
+  const u32 wstart = w * kWaveSub;
+  u64 m[kItems];
+  u32 c = 0;
 #pragma unroll
   for (u32 q = 0; q < kItems; ++q) {
-    const u32 o = base + q;
+    const u32 o = wstart + q * kWave + l;
     const bool fl = o < r.count && run_flag<kKind>(k, r.first + o, first, all);
-    bits |= (u32)fl << q;
-    c += fl;
+    m[q] = __ballot(fl);
+    c += (u32)__popcll(m[q]);
   }
-  u32 tot;
-  u32 run = block_excl_scan(c, sh, tot) + part[blockIdx.x];
+  if (l == 0) sh[w] = c;
+  __syncthreads();
+  u32 run = part[blockIdx.x];
 #pragma unroll
-  for (u32 q = 0; q < kItems; ++q)
-    if ((bits >> q) & 1u) pos[run++] = r.first + base + q;
+  for (u32 i = 0; i < (u32)kWavesPerBlock; ++i)
+    if (i < w) run += sh[i];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    if ((m[q] >> l) & 1ull) pos[run + (u32)__popcll(m[q] & lt)] = r.first + wstart + q * kWave + l;
+    run += (u32)__popcll(m[q]);
+  }
 }
 
 // Per-segment start and count of the compacted array from the scanned
@@ -777,110 +923,185 @@ void compact(const Batch& bt, const KV& k, u32* pos, int which, Arena& ar, hipSt
 // the stream only depends on the sorted cells and their counts).  Cells of
 // RLE segments only; counts below kCntBinsSeg go to the segment's histogram
 // (with first occurrences), larger ones to a (segment, count, cell) list.
+// LDS histogram with first occurrences -> the segment's device histogram.
+// Every thread of the block calls this (bin d = threadIdx.x); clears the LDS
+// copy for the next segment.
+__device__ __forceinline__ void flush_hist(u32* h, u32* f, u32* gh, u32* gf) {
+  __syncthreads();
+  const u32 d = threadIdx.x;
+  if (h[d]) {
+    atomicAdd(&gh[d], h[d]);
+    min_to(&gf[d], f[d]);
+  }
+  h[d] = 0;
+  f[d] = 0xFFFFFFFFu;
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u32* upos,
                                                        u64* __restrict__ ua,
                                                        u32* __restrict__ ub,
                                                        u32* __restrict__ cnt) {
   __shared__ u32 hs[kCntBinsSeg], fs[kCntBinsSeg];
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  const SegCtl& sc = bt.sc[t.seg];
+  static_assert(kCntBinsSeg == kBlock, "one count bin per thread");
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
   hs[threadIdx.x] = 0;
   fs[threadIdx.x] = 0xFFFFFFFFu;
   __syncthreads();
-  const Range r = tile_range(sc.e0, sc.ne, t.local);
-  const u32 end_cells = sc.e0 + sc.ne, end_pts = sp.p0 + sp.n;
-  for (u32 it = 0; it < kItems; ++it) {
-    const u32 o = it * kBlock + threadIdx.x;
-    if (o >= r.count) break;
-    const u32 u = r.first + o;
-    const u32 st = upos[u];
-    const u32 en = (u + 1 < end_cells) ? upos[u + 1] : end_pts;
-    const u32 c = en - st;
-    ua[u] = in.a[st];
-    if (in.wide) ub[u] = in.b[st];
-    cnt[u] = c;
-    const u32 rel = u - sc.e0;
-    if (c < kCntBinsSeg) {
-      atomicAdd(&hs[c], 1u);
-      if (rel < fs[c]) atomicMin(&fs[c], rel);
-    } else {
-      const u32 slot = atomicAdd(bt.n_large, 1u);
-      if (slot < bt.large_cap) {
-        bt.large[3 * slot] = t.seg;
-        bt.large[3 * slot + 1] = c;
-        bt.large[3 * slot + 2] = rel;
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_hist(hs, fs, bt.sh[cur].cnt, bt.sh[cur].first_cnt);
+      cur = t.seg;
+    }
+    const SegParam& sp = bt.sp[t.seg];
+    const SegCtl& sc = bt.sc[t.seg];
+    const Range r = tile_range(sc.e0, sc.ne, t.local);
+    const u32 end_cells = sc.e0 + sc.ne, end_pts = sp.p0 + sp.n;
+#pragma unroll 4
+    for (u32 it = 0; it < kItems; ++it) {
+      const u32 o = it * kBlock + threadIdx.x;
+      if (o < r.count) {
+        const u32 u = r.first + o;
+        const u32 st = upos[u];
+        const u32 en = (u + 1 < end_cells) ? upos[u + 1] : end_pts;
+        const u32 c = en - st;
+        ua[u] = in.a[st];
+        if (in.wide) ub[u] = in.b[st];
+        cnt[u] = c;
+        const u32 rel = u - sc.e0;
+        if (c < kCntBinsSeg) {
+          atomicAdd(&hs[c], 1u);
+          if (rel < fs[c]) atomicMin(&fs[c], rel);
+        } else {
+          const u32 slot = atomicAdd(bt.n_large, 1u);
+          if (slot < bt.large_cap) {
+            bt.large[3 * slot] = t.seg;
+            bt.large[3 * slot + 1] = c;
+            bt.large[3 * slot + 2] = rel;
+          }
+        }
       }
     }
   }
-  __syncthreads();
-  const u32 d = threadIdx.x;
-  if (hs[d]) {
-    atomicAdd(&bt.sh[t.seg].cnt[d], hs[d]);
-    atomicMin(&bt.sh[t.seg].first_cnt[d], fs[d]);
-  }
+  flush_hist(hs, fs, bt.sh[cur].cnt, bt.sh[cur].first_cnt);
 }
 
 // Symbol statistics over each segment's coded elements: PFD symbols of repos
 // (block mode) or of the SFC deltas (direct mode, block_compressor.hpp:110-117),
 // quads symbols, patch count and wrapping sum.  First occurrences are
 // relative to the segment's first element.
+struct StatLds {
+  u32 hs[256], fs[256], hq[256], fq[256];
+  u32 mag[kMagBins];
+  u64 bitpop[64];
+  u64 red[kWavesPerBlock];
+};
+
+__device__ __forceinline__ void flush_stats(StatLds& L, SegCtl& sc, SegHist& h, u32& np,
+                                            u32& nqp, u64& ps) {
+  const u32 a = block_sum(np, reinterpret_cast<u32*>(L.red));
+  const u32 b = block_sum(nqp, reinterpret_cast<u32*>(L.red));
+  const u64 c = block_sum(ps, L.red);
+  if (threadIdx.x == 0) {
+    if (a) atomicAdd(&sc.n_patches, a);
+    if (b) atomicAdd(&sc.n_qpatches, b);
+    if (c) atomicAdd(&sc.patch_sum, c);
+  }
+  np = nqp = 0;
+  ps = 0;
+  const u32 d = threadIdx.x;
+  if (d < kMagBins) {
+    if (L.mag[d]) atomicAdd(&h.mag[d], L.mag[d]);
+    L.mag[d] = 0;
+  }
+  if (d < 64) {
+    if (L.bitpop[d]) atomicAdd(&h.bitpop[d], L.bitpop[d]);
+    L.bitpop[d] = 0;
+  }
+  flush_hist(L.hq, L.fq, h.quad, h.first_quad);  // synchronises first
+  flush_hist(L.hs, L.fs, h.sym, h.first_sym);
+}
+
 __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
-  __shared__ u32 hs[256], fs[256], hq[256], fq[256];
-  hs[threadIdx.x] = 0;
-  hq[threadIdx.x] = 0;
-  fs[threadIdx.x] = 0xFFFFFFFFu;
-  fq[threadIdx.x] = 0xFFFFFFFFu;
+  __shared__ StatLds L;
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
+  const u32 d = threadIdx.x, l = lane_id();
+  L.hs[d] = L.hq[d] = 0;
+  L.fs[d] = L.fq[d] = 0xFFFFFFFFu;
+  if (d < kMagBins) L.mag[d] = 0;
+  if (d < 64) L.bitpop[d] = 0;
   __syncthreads();
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  SegCtl& sc = bt.sc[t.seg];
-  const Range r = tile_range(sc.e0, sc.ne, t.local);
-  const bool direct = sp.kp.direct != 0;
   u32 np = 0, nqp = 0;
   u64 ps = 0;
-  for (u32 it = 0; it < kItems; ++it) {
-    const u32 o = it * kBlock + threadIdx.x;
-    if (o >= r.count) break;
-    const u32 j = r.first + o, rel = j - sc.e0;
-    u64 v;
-    if (direct) {
-      v = (j == sc.e0) ? k.a[j] : k.a[j] - k.a[j - 1];
-    } else {
-      u32 qd;
-      lcp(k, j, sc.e0, v, qd);
-      const u32 qs = qd > 254 ? 0u : qd + 1u;
-      nqp += qd > 254;
-      atomicAdd(&hq[qs], 1u);
-      if (rel < fq[qs]) atomicMin(&fq[qs], rel);
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_stats(L, bt.sc[cur], bt.sh[cur], np, nqp, ps);
+      cur = t.seg;
     }
-    const u32 sym = v > 254 ? 0u : (u32)v + 1u;
-    atomicAdd(&hs[sym], 1u);
-    if (rel < fs[sym]) atomicMin(&fs[sym], rel);
-    if (v > 254) {
-      ++np;
-      ps += v;
+    const SegParam& sp = bt.sp[t.seg];
+    const SegCtl& sc = bt.sc[t.seg];
+    const Range r = tile_range(sc.e0, sc.ne, t.local);
+    const bool direct = sp.kp.direct != 0;
+    const u32 lim = sp.lim;
+    u64 bp = 0;  // this lane's share of bitpop: lane b counts bit b
+    // Loads first (all items in flight), then the LDS updates.
+    u64 v[kItems];
+    u32 qd[kItems];
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) {
+      const u32 o = it * kBlock + threadIdx.x;
+      v[it] = 0;
+      qd[it] = 0;
+      if (o < r.count) {
+        const u32 j = r.first + o;
+        if (direct) v[it] = (j == sc.e0) ? k.a[j] : k.a[j] - k.a[j - 1];
+        else lcp(k, j, sc.e0, v[it], qd[it]);
+      }
     }
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) {
+      const u32 o = it * kBlock + threadIdx.x;
+      if (o < r.count) {
+        const u32 rel = r.first + o - sc.e0;
+        if (!direct) {
+          const u32 qs = qd[it] > 254 ? 0u : qd[it] + 1u;
+          nqp += qd[it] > 254;
+          atomicAdd(&L.hq[qs], 1u);
+          if (rel < L.fq[qs]) atomicMin(&L.fq[qs], rel);
+        }
+        const u32 sym = v[it] > lim ? 0u : (u32)v[it] + 1u;
+        atomicAdd(&L.hs[sym], 1u);
+        if (rel < L.fs[sym]) atomicMin(&L.fs[sym], rel);
+        if (v[it] > lim) {
+          ++np;
+          ps += v[it];
+        }
+        atomicAdd(&L.mag[bitlen64(v[it])], 1u);
+      }
+    }
+    if (sp.search) {  // bit populations of the values > 254 (mode search only)
+      for (u32 it = 0; it < kItems; ++it) {
+        const bool big = it * kBlock + threadIdx.x < r.count && v[it] > 254;
+        if (__ballot(big)) {
+#pragma unroll 8
+          for (u32 b = 0; b < 64; ++b) {
+            const u32 c = (u32)__popcll(__ballot(big && ((v[it] >> b) & 1ull)));
+            if (l == b) bp += c;
+          }
+        }
+      }
+    }
+    if (bp) atomicAdd(&L.bitpop[l], bp);
   }
-  np = wave_sum(np);
-  nqp = wave_sum(nqp);
-  ps = wave_sum(ps);
-  if (lane_id() == 0) {
-    if (np) atomicAdd(&sc.n_patches, np);
-    if (nqp) atomicAdd(&sc.n_qpatches, nqp);
-    if (ps) atomicAdd(&sc.patch_sum, ps);
-  }
-  __syncthreads();
-  const u32 d = threadIdx.x;
-  SegHist& h = bt.sh[t.seg];
-  if (hs[d]) {
-    atomicAdd(&h.sym[d], hs[d]);
-    atomicMin(&h.first_sym[d], fs[d]);
-  }
-  if (hq[d]) {
-    atomicAdd(&h.quad[d], hq[d]);
-    atomicMin(&h.first_quad[d], fq[d]);
-  }
+  flush_stats(L, bt.sc[cur], bt.sh[cur], np, nqp, ps);
 }
 
 // Block-id and block-count deltas (block_compressor.hpp:146-147): u64
@@ -896,27 +1117,44 @@ __device__ __forceinline__ u64 block_delta(const KV& k, const u32* bpos, const S
   return b > sc.b0 ? c - (u64)(st - bpos[b - 1]) : c;
 }
 
+__device__ __forceinline__ void flush_block_stats(SegCtl& sc, u64& m0, u64& m1, u64* sh) {
+  const u64 a = block_max(m0, sh), b = block_max(m1, sh);
+  if (threadIdx.x == 0) {
+    if (a) max_to(&sc.max_blkd, a);
+    if (b) max_to(&sc.max_cntd, b);
+  }
+  m0 = m1 = 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_block_stats(Batch bt, KV k, const u32* bpos) {
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  if (sp.kp.direct) return;
-  SegCtl& sc = bt.sc[t.seg];
-  const Range r = tile_range(sc.b0, sc.nb, t.local);
+  __shared__ u64 sh[kWavesPerBlock];
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
   u64 m0 = 0, m1 = 0;
-  for (u32 it = 0; it < kItems; ++it) {
-    const u32 o = it * kBlock + threadIdx.x;
-    if (o >= r.count) break;
-    const u64 a = block_delta(k, bpos, sc, sp.kp, r.first + o, 0);
-    const u64 c = block_delta(k, bpos, sc, sp.kp, r.first + o, 1);
-    m0 = a > m0 ? a : m0;
-    m1 = c > m1 ? c : m1;
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_block_stats(bt.sc[cur], m0, m1, sh);
+      cur = t.seg;
+    }
+    const SegParam& sp = bt.sp[t.seg];
+    if (sp.kp.direct) continue;
+    const SegCtl& sc = bt.sc[t.seg];
+    const Range r = tile_range(sc.b0, sc.nb, t.local);
+#pragma unroll 4
+    for (u32 it = 0; it < kItems; ++it) {
+      const u32 o = it * kBlock + threadIdx.x;
+      if (o < r.count) {
+        const u64 a = block_delta(k, bpos, sc, sp.kp, r.first + o, 0);
+        const u64 c = block_delta(k, bpos, sc, sp.kp, r.first + o, 1);
+        m0 = a > m0 ? a : m0;
+        m1 = c > m1 ? c : m1;
+      }
+    }
   }
-  m0 = wave_max(m0);
-  m1 = wave_max(m1);
-  if (lane_id() == 0) {
-    if (m0) atomicMax(&sc.max_blkd, m0);
-    if (m1) atomicMax(&sc.max_cntd, m1);
-  }
+  flush_block_stats(bt.sc[cur], m0, m1, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -980,11 +1218,11 @@ __device__ __forceinline__ Code coder(const Batch& bt, const SecCtx& x, const Se
     lcp(k, j, sc.e0, v, qd);
   }
   if (kKind == kSecGR) {  // Golomb-Rice of the patches, LSB-first
-    if (v <= 254) return Code{0, 0, 0};
+    if (v <= sp.lim) return Code{0, 0, 0};
     return Code{(v & ((1ull << sp.grk) - 1)) << 1, sp.grk + 1, (u32)(v >> sp.grk)};
   }
   if (kKind == kSecHuf) {
-    const HufEntry e = lut.sym[v > 254 ? 0u : (u32)v + 1u];
+    const HufEntry e = lut.sym[v > sp.lim ? 0u : (u32)v + 1u];
     return Code{e.bits, e.len, 0};
   }
   const HufEntry e = lut.quad[qd > 254 ? 0u : qd + 1u];  // kSecQuad
@@ -993,36 +1231,69 @@ __device__ __forceinline__ Code coder(const Batch& bt, const SecCtx& x, const Se
 
 constexpr u32 kVlWin = 8448;  // LDS words: 4096 codes of 64 bits plus the edges
 
-template <u32 kKind>
+// Phase A: the bits of every section in one pass over the tile.  The
+// BitPacker sections are fixed-width (count x width); the cell-space sections
+// share one repos / quads derivation per element.  tile_bits is laid out
+// [section][tile], so one exclusive scan over it serves every section: a
+// section's in-segment offsets are differences of scan values, and the
+// constant base each section inherits from the ones before it cancels.
 __global__ __launch_bounds__(kBlock) void k_sec_bits(Batch bt, SecCtx x, u64* tile_bits) {
   __shared__ u64 sh[kWavesPerBlock];
   const Tile t = bt.tiles[blockIdx.x];
   const SegParam& sp = bt.sp[t.seg];
   const SegCtl& sc = bt.sc[t.seg];
-  u64 s = 0;
-  if (sec_active<kKind>(sp, sc)) {
-    const Range r = stage_range(sp, sc, t.local, SecTraits<kKind>::kSpace);
-    const SegLut& lut = bt.lut[t.seg];
-    for (u32 q = 0; q < kItems; ++q) {
-      const u32 o = threadIdx.x * kItems + q;
-      if (o < r.count) {
-        const Code cd = coder<kKind>(bt, x, sp, sc, lut, r.first + o);
-        s += (u64)cd.lead + cd.len;
+  const SegLut& lut = bt.lut[t.seg];
+  const KV& k = x.k;
+  const bool direct = sp.kp.direct != 0, rle = sp.rle != 0;
+  const u32 lim = sp.lim, grk = sp.grk;
+  u64 sg = 0, sh_ = 0, sq = 0, sr = 0;
+  const Range r = tile_range(sc.e0, sc.ne, t.local);
+#pragma unroll 4
+  for (u32 it = 0; it < kItems; ++it) {
+    const u32 o = it * kBlock + threadIdx.x;
+    if (o < r.count) {
+      const u32 j = r.first + o;
+      u64 v;
+      u32 qd = 0;
+      if (direct) {
+        v = (j == sc.e0) ? k.a[j] : k.a[j] - k.a[j - 1];
+      } else {
+        lcp(k, j, sc.e0, v, qd);
+        sq += lut.quad[qd > 254 ? 0u : qd + 1u].len;
+      }
+      sh_ += lut.sym[v > lim ? 0u : (u32)v + 1u].len;
+      if (v > lim) sg += (v >> grk) + grk + 1;
+      if (rle) {
+        const Code c = coder<kSecRle>(bt, x, sp, sc, lut, j);
+        sr += (u64)c.lead + c.len;
       }
     }
   }
-  const u64 tot = block_sum(s, sh);
-  if (threadIdx.x == 0) tile_bits[blockIdx.x] = tot;
+  u64 tot[kNumSec];
+  tot[kSecGR] = block_sum(sg, sh);
+  tot[kSecHuf] = block_sum(sh_, sh);
+  tot[kSecQuad] = block_sum(sq, sh);
+  tot[kSecRle] = block_sum(sr, sh);
+  if (threadIdx.x == 0) {
+    const u64 nb = direct ? 0 : tile_range(sc.b0, sc.nb, t.local).count;
+    tot[kSecBlk] = nb * sp.w1;
+    tot[kSecCnt] = nb * sp.w2;
+#pragma unroll
+    for (u32 q = 0; q < kNumSec; ++q) tile_bits[(u64)q * bt.ntiles + blockIdx.x] = tot[q];
+  }
 }
 
-// Per segment: the section's scan offset at its first tile and its size.
-__global__ void k_seg_sections(Batch bt, const u64* tile_off, const u64* total, u32 kind) {
-  for (u32 s = blockIdx.x * blockDim.x + threadIdx.x; s < bt.S; s += gridDim.x * blockDim.x) {
-    const SegParam& sp = bt.sp[s];
-    const u64 st = tile_off[sp.tile0];
-    const u64 en = (s + 1 < bt.S) ? tile_off[bt.sp[s + 1].tile0] : *total;
-    bt.sc[s].sec_bit0[kind] = st;
-    bt.sc[s].sec_bits[kind] = en - st;
+// Per segment and section: the scan offset at its first tile and its size.
+__global__ void k_seg_sections(Batch bt, const u64* tile_off, const u64* total) {
+  const u64 nt = bt.ntiles;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < bt.S * kNumSec;
+       i += gridDim.x * blockDim.x) {
+    const u32 s = i / kNumSec, q = i % kNumSec;
+    const u64* off = tile_off + q * nt;
+    const u64 st = off[bt.sp[s].tile0];
+    const u64 en = (s + 1 < bt.S) ? off[bt.sp[s + 1].tile0] : (q + 1 < kNumSec ? off[nt] : *total);
+    bt.sc[s].sec_bit0[q] = st;
+    bt.sc[s].sec_bits[q] = en - st;
   }
 }
 
@@ -1042,16 +1313,31 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
   if (!sec_active<kKind>(sp, sc)) return;
   const Range r = stage_range(sp, sc, t.local, SecTraits<kKind>::kSpace);
   const SegLut& lut = bt.lut[t.seg];
+  // Element order inside the tile: wavefront w owns [w*kWaveSub,
+  // (w+1)*kWaveSub), taken 64 consecutive elements per step, so every load
+  // is coalesced.  rel[q] = bit offset of the lane's q-th code in its wave.
+  const u32 w = threadIdx.x / kWave, l = lane_id();
+  const u32 wstart = w * kWaveSub;
   Code cd[kItems];
-  u64 s = 0;
+  u64 rel[kItems];
+  u64 carry = 0;
 #pragma unroll
   for (u32 q = 0; q < kItems; ++q) {
-    const u32 o = threadIdx.x * kItems + q;
+    const u32 o = wstart + q * kWave + l;
     cd[q] = o < r.count ? coder<kKind>(bt, x, sp, sc, lut, r.first + o) : Code{0, 0, 0};
-    s += (u64)cd[q].lead + cd[q].len;
+    const u64 b = (u64)cd[q].lead + cd[q].len;
+    const u64 inc = wave_incl_scan(b);
+    rel[q] = carry + inc - b;
+    carry += __shfl(inc, kWave - 1, kWave);
   }
-  u64 tot;
-  const u64 ex = block_excl_scan(s, sh, tot);
+  if (l == 0) sh[w] = carry;
+  __syncthreads();
+  u64 ex = 0, tot = 0;
+#pragma unroll
+  for (u32 i = 0; i < (u32)kWavesPerBlock; ++i) {
+    if (i < w) ex += sh[i];
+    tot += sh[i];
+  }
   if (tot == 0) return;
   const u64 abs0 = sp.sec_byte[kKind] * 8 + (tile_off[blockIdx.x] - sc.sec_bit0[kKind]);
   const u64 w0 = abs0 >> 5;
@@ -1060,12 +1346,8 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
   if (nwords <= kVlWin) {
     for (u32 i = threadIdx.x; i < nwords; i += kBlock) win[i] = 0;
     __syncthreads();
-    u64 p = o0 + ex;
 #pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      put_code<true, kMsb>(win, p, cd[q]);
-      p += (u64)cd[q].lead + cd[q].len;
-    }
+    for (u32 q = 0; q < kItems; ++q) put_code<true, kMsb>(win, o0 + ex + rel[q], cd[q]);
     __syncthreads();
     for (u32 i = threadIdx.x; i < nwords; i += kBlock) {
       u32 v = win[i];
@@ -1077,31 +1359,27 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
       }
     }
   } else {
-    u64 p = abs0 + ex;
 #pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      put_code<false, kMsb>(out, p, cd[q]);
-      p += (u64)cd[q].lead + cd[q].len;
-    }
+    for (u32 q = 0; q < kItems; ++q) put_code<false, kMsb>(out, abs0 + ex + rel[q], cd[q]);
   }
 }
 
+// Profiler / rocprof-visible names of the section writers.
 template <u32 kKind>
-void size_one(const Batch& bt, const SecCtx& x, const SecPlan& p, Arena& ar, hipStream_t s) {
-  const size_t mark = ar.off;
-  u64* total = ar.take<u64>(1);
-  k_sec_bits<kKind><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_bits[kKind]);
-  LAUNCH_CHECK("k_sec_bits");
-  scan_excl<u64>(p.tile_bits[kKind], p.tile_off[kKind], bt.ntiles, total, ar, s);
-  k_seg_sections<<<grid_for(bt.S, kBlock), kBlock, 0, s>>>(bt, p.tile_off[kKind], total, kKind);
-  LAUNCH_CHECK("k_seg_sections");
-  ar.off = mark;
+constexpr const char* sec_write_name() {
+  return kKind == kSecBlk    ? "k_sec_write<blk>"
+         : kKind == kSecCnt  ? "k_sec_write<cnt>"
+         : kKind == kSecGR   ? "k_sec_write<gr>"
+         : kKind == kSecHuf  ? "k_sec_write<huf>"
+         : kKind == kSecQuad ? "k_sec_write<quad>"
+                             : "k_sec_write<rle>";
 }
 
 template <u32 kKind>
 void write_one(const Batch& bt, const SecCtx& x, const SecPlan& p, u32* out, hipStream_t s) {
-  k_sec_write<kKind><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off[kKind], out);
-  LAUNCH_CHECK("k_sec_write");
+  k_sec_write<kKind><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off + (u64)kKind * bt.ntiles,
+                                                  out);
+  LAUNCH_CHECK(sec_write_name<kKind>());
 }
 
 }  // namespace
@@ -1127,14 +1405,14 @@ void launch_split_lo64(const u64* lo64, u32* lo, u32* idx, u32 n, hipStream_t s)
 }
 
 void launch_bbox(const Batch& bt, const float* xyz, hipStream_t s) {
-  k_bbox<<<bt.ntiles, kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
+  k_bbox<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
   LAUNCH_CHECK("k_bbox");
   k_offsets<<<grid_for((u64)bt.S * 3, kBlock), kBlock, 0, s>>>(bt, xyz);
   LAUNCH_CHECK("k_offsets");
 }
 
 void launch_quant_stats(const Batch& bt, const float* xyz, hipStream_t s) {
-  k_quant_stats<<<bt.ntiles, kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
+  k_quant_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
   LAUNCH_CHECK("k_quant_stats");
 }
 
@@ -1220,7 +1498,7 @@ void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u64* ua, u32* ub, 
                Arena& ar, hipStream_t s) {
   const KV k{in.a, in.b, bt.wide};
   compact<kFlagCell>(bt, k, upos, 0, ar, s);
-  k_rle_finish<<<bt.ntiles, kBlock, 0, s>>>(bt, k, upos, ua, ub, cnt);
+  k_rle_finish<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, k, upos, ua, ub, cnt);
   LAUNCH_CHECK("k_rle_finish");
 }
 
@@ -1229,32 +1507,32 @@ void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, b
   (void)direct_any;
   const KV k{kv.a, kv.b, bt.wide};
   if (block_any) compact<kFlagBlock>(bt, k, bpos, 1, ar, s);
-  k_stats<<<bt.ntiles, kBlock, 0, s>>>(bt, k);
+  k_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, k);
   LAUNCH_CHECK("k_stats");
   if (block_any) {
-    k_block_stats<<<bt.ntiles, kBlock, 0, s>>>(bt, k, bpos);
+    k_block_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, k, bpos);
     LAUNCH_CHECK("k_block_stats");
   }
 }
 
 SecPlan plan_sections(const Batch& bt, Arena& ar) {
   SecPlan p{};
-  for (u32 i = 0; i < kNumSec; ++i) {
-    p.tile_bits[i] = ar.take<u64>(bt.ntiles);
-    p.tile_off[i] = ar.take<u64>(bt.ntiles);
-  }
+  p.tile_bits = ar.take<u64>((u64)kNumSec * bt.ntiles);
+  p.tile_off = ar.take<u64>((u64)kNumSec * bt.ntiles);
   return p;
 }
 
 void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
                    const SecPlan& p, Arena& ar, hipStream_t s) {
   const SecCtx x{KV{kv.a, kv.b, bt.wide}, cnt, bpos};
-  size_one<kSecBlk>(bt, x, p, ar, s);
-  size_one<kSecCnt>(bt, x, p, ar, s);
-  size_one<kSecGR>(bt, x, p, ar, s);
-  size_one<kSecHuf>(bt, x, p, ar, s);
-  size_one<kSecQuad>(bt, x, p, ar, s);
-  size_one<kSecRle>(bt, x, p, ar, s);
+  const size_t mark = ar.off;
+  u64* total = ar.take<u64>(1);
+  k_sec_bits<<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_bits);
+  LAUNCH_CHECK("k_sec_bits");
+  scan_excl<u64>(p.tile_bits, p.tile_off, (u64)kNumSec * bt.ntiles, total, ar, s);
+  k_seg_sections<<<grid_for((u64)bt.S * kNumSec, kBlock), kBlock, 0, s>>>(bt, p.tile_off, total);
+  LAUNCH_CHECK("k_seg_sections");
+  ar.off = mark;
 }
 
 void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,

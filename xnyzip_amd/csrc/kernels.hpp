@@ -104,7 +104,10 @@ struct SegParam {
   u32 cnt_eg, egk;    // RLE counts: Exp-Golomb and its k
   u32 w1, w2;         // BitPacker widths
   u32 big0, nbig;     // RLE counts >= kCntBinsSeg: range in the big tables
-  u32 pad;
+  u32 lim;            // PFD patch threshold of repos / sfc deltas: v > lim is a
+                      // patch (254 = p_for_delta_encoder.hpp:31; lower values
+                      // only come from a mode override)
+  u32 search;         // also collect SegHist::bitpop (mode search)
   u64 sec_byte[kNumSec];  // byte offset of each data section (after sizing)
 };
 
@@ -128,10 +131,17 @@ struct SegCtl {
   u64 sec_bit0[kNumSec];   // scan offset of the segment's first tile
 };
 
+constexpr u32 kMagBins = 68;  // bitlen(v) in 0..64, padded
+
 struct SegHist {  // per-segment symbol counts and first occurrences
   u32 sym[kSymBins], first_sym[kSymBins];
   u32 quad[kSymBins], first_quad[kSymBins];
   u32 cnt[kCntBinsSeg], first_cnt[kCntBinsSeg];
+  // Over the repos (or sfc-delta) values v > 254: bitpop[b] = how many have
+  // bit b set, so sum(v >> k) = sum_{b >= k} bitpop[b] << (b - k) exactly,
+  // for every Golomb-Rice k.
+  u64 bitpop[64];
+  u32 mag[kMagBins];  // histogram of bitlen(v) over all repos values
 };
 
 struct SegLut {   // per-segment codebooks (host-built)
@@ -206,8 +216,8 @@ void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, b
 // Phase A of every section: per-tile bit counts -> tile offsets; per-segment
 // section bits and first-tile offsets into SegCtl.
 struct SecPlan {
-  u64* tile_bits[kNumSec];
-  u64* tile_off[kNumSec];
+  u64* tile_bits;  // [section][tile]
+  u64* tile_off;   // exclusive scan of tile_bits
 };
 SecPlan plan_sections(const Batch& bt, Arena& ar);
 void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
