@@ -178,6 +178,58 @@ int xnyz_search_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* 
                              uint32_t count, const xnyz_opts* opts,
                              xnyz_search_result* results);
 
+/* ---- Learned mode selector (no reference counterpart) -------------------
+ * An MLP 32 -> 64 -> 64 (ReLU) -> heads {Golomb-Rice k 0..31, PFD width
+ * 1..8, RLE off/on, curve Z/Hilbert}, FP16 weights with FP32 accumulation,
+ * trained and run by HIP kernels.  Parameter blob: XNYZ_MLP_PARAMS floats,
+ * layout feature mean[32], 1/std[32], W1[32][64], b1[64], W2[64][64], b2[64],
+ * W3[64][44], b3[44] (W[i][o]: input i -> output o; logits 0..31 k,
+ * 32..39 width 1..8, 40..41 RLE, 42..43 curve). */
+#define XNYZ_MLP_FEATURES 32
+#define XNYZ_MLP_PARAMS 9196
+
+/* Per cloud, XNYZ_MLP_FEATURES floats from one device pass after the range
+ * scan (no sort): log sizes and extents, HyperLogLog estimates of the
+ * distinct cells / 64^3 sub-blocks / 128^3 blocks, quantizer and bound, and
+ * input-order locality (consecutive points sharing a cell / sub-block /
+ * block, histogram of their Morton-code xor magnitudes).  features: host,
+ * count * XNYZ_MLP_FEATURES floats. */
+int xnyz_features_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                               uint32_t count, const xnyz_opts* opts, float* features);
+
+typedef struct xnyz_train_opts {
+  int32_t epochs;
+  int32_t batch;   /* samples per Adam step, <= 65536 */
+  float lr;        /* Adam learning rate */
+  int32_t init;    /* 1: fit the normaliser, He-initialise; 0: continue from params */
+  uint64_t seed;
+} xnyz_train_opts;
+
+/* Softmax cross-entropy on the four heads, Adam; labels from the exhaustive
+ * search (xnyz_search_result.best).  params: in/out, XNYZ_MLP_PARAMS floats
+ * (host).  final_loss (nullable): mean loss of the last epoch. */
+int xnyz_mlp_train(xnyz_ctx* ctx, const float* features, const xnyz_mode* labels, uint64_t n,
+                   const xnyz_train_opts* opts, float* params, float* final_loss);
+
+/* Predicted mode per sample and its confidence (the smallest, over the
+ * heads, of the top softmax probability); confidence nullable. */
+int xnyz_mlp_predict(xnyz_ctx* ctx, const float* params, const float* features, uint64_t n,
+                     xnyz_mode* modes, float* confidence);
+
+/* Batched compression with a per-cloud mode: the selector's prediction
+ * where its confidence is >= min_confidence, the exhaustive search's optimum
+ * otherwise (params NULL: search every cloud).  opts supply quantizer,
+ * l2_bound, direct_threshold and frame_mode; the chosen curve / RLE / k /
+ * width replace the rest.  chosen and searched (1 = fell back to the search)
+ * are nullable arrays of `count`.  The RLE flag is not recorded in the
+ * stream: decode each cloud with chosen[s].rle. */
+int xnyz_compress_selected_batch_device(xnyz_ctx* ctx, const float* d_xyz,
+                                        const uint64_t* offsets, uint32_t count,
+                                        const xnyz_opts* opts, const float* params,
+                                        float min_confidence, xnyz_buf* frames,
+                                        xnyz_buf* prezstd, xnyz_stats* stats,
+                                        xnyz_mode* chosen, int32_t* searched);
+
 /* Replaces TruncatedOctahedronQuantizer/CubeQuantizer::quantize after
  * shift_points (source/quantizer/truncated_octahedron_quantizer.hpp:21-55,
  * cube_quantizer.hpp:15-27, source/preprocessor/shifting.hpp:11-36) on

@@ -1,0 +1,152 @@
+"""Learned mode selector: features, the HIP MLP kernels (against a numpy
+FP32 model of the same FP16-operand arithmetic), training, and batched
+compression with selected / searched modes (byte-exact against the oracle
+for the chosen mode)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def X():
+    import xnyzip_amd
+    if not torch.cuda.is_available():
+        pytest.fail("no HIP device visible")
+    return xnyzip_amd
+
+
+def _uniform(seed, n, scale=100.0):
+    rng = np.random.default_rng(seed)
+    return (rng.random((n, 3)) * scale).astype(np.float32)
+
+
+def _f16(a):
+    return a.astype(np.float16).astype(np.float32)
+
+
+def _np_logits(P, F):
+    mean, istd = P[0:32], P[32:64]
+    o = 64
+    W1 = P[o:o + 2048].reshape(32, 64); o += 2048
+    b1 = P[o:o + 64]; o += 64
+    W2 = P[o:o + 4096].reshape(64, 64); o += 4096
+    b2 = P[o:o + 64]; o += 64
+    W3 = P[o:o + 2816].reshape(64, 44); o += 2816
+    b3 = P[o:o + 44]
+    xn = _f16((F - mean) * istd)
+    h1 = _f16(np.maximum(xn @ _f16(W1) + b1, 0))
+    h2 = _f16(np.maximum(h1 @ _f16(W2) + b2, 0))
+    return h2 @ _f16(W3) + b3
+
+
+HEADS = [(0, 32), (32, 40), (40, 42), (42, 44)]
+
+
+def test_features_match_exact_statistics(X):
+    clouds = [_uniform(1, 40000), np.round(_uniform(2, 30000) / 3, 1).astype(np.float32),
+              _uniform(3, 5000, 20.0)]
+    opts = [dict(quantizer=O.TO, l2_bound=0.01), dict(quantizer=O.CUBE, l2_bound=0.05),
+            dict(quantizer=O.TO, l2_bound=0.1)]
+    pts = np.concatenate(clouds)
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).tolist()
+    F = X.features_batch(torch.from_numpy(pts).cuda(), offs, opts)
+    assert F.shape == (3, 32) and np.isfinite(F).all()
+    for c, o, f in zip(clouds, opts, F):
+        ref = O.compress(c, o["quantizer"], o["l2_bound"], False, True)
+        assert f[0] == pytest.approx(np.log2(len(c)), abs=1e-5)
+        for a in range(3):
+            assert f[1 + a] == pytest.approx(np.log2(1 + ref.info.range[a]), abs=1e-5)
+        u_est = 2.0 ** f[5]
+        assert abs(u_est / ref.info.num_points - 1) < 0.2  # HyperLogLog, 256 registers
+        assert 0 <= f[15] <= 1 and 0 <= f[16] <= 1 and 0 <= f[17] <= 1
+
+
+def test_predict_matches_numpy_model(X):
+    rng = np.random.default_rng(5)
+    P = (rng.normal(size=9196) * 0.3).astype(np.float32)
+    P[0:32] = rng.normal(size=32).astype(np.float32)
+    P[32:64] = rng.uniform(0.5, 2, 32).astype(np.float32)
+    F = rng.normal(size=(3000, 32)).astype(np.float32)
+    modes, conf = X.mlp_predict(P, F)
+    Z = _np_logits(P, F)
+    keys = ["gr_k", "pfd_width", "rle", "is_hilbert"]
+    checked = 0
+    for i, m in enumerate(modes):
+        c = 1.0
+        for h, (a, b) in enumerate(HEADS):
+            z = Z[i, a:b]
+            top = np.sort(z)[::-1]
+            p = np.exp(z - z.max())
+            c = min(c, 1.0 / p.sum())
+            if top[0] - top[1] > 1e-2:
+                want = int(np.argmax(z)) + (1 if keys[h] == "pfd_width" else 0)
+                assert m[keys[h]] == want, (i, h)
+                checked += 1
+        assert conf[i] == pytest.approx(c, abs=2e-3)
+    assert checked > 4 * 3000 * 0.9
+
+
+def test_training_learns_a_rule(X):
+    rng = np.random.default_rng(6)
+    F = rng.normal(size=(8192, 32)).astype(np.float32) * 3 + 1
+    g = F > 1
+    modes = [dict(gr_k=int(4 * a + 2 * b + c), pfd_width=int(1 + d + 2 * e), rle=int(f),
+                  is_hilbert=int(gg))
+             for a, b, c, d, e, f, gg in zip(g[:, 0], g[:, 1], g[:, 2], g[:, 3], g[:, 4],
+                                             g[:, 5], g[:, 6] ^ g[:, 7])]
+    p1, l1 = X.mlp_train(F, modes, epochs=1, lr=3e-3, seed=1)
+    p2, l2 = X.mlp_train(F, modes, epochs=40, lr=3e-3, params=p1, seed=2)
+    assert l2 < 0.5 * l1
+    pred, _ = X.mlp_predict(p2, F)
+    for key in ["gr_k", "pfd_width", "rle", "is_hilbert"]:
+        acc = np.mean([p[key] == m[key] for p, m in zip(pred, modes)])
+        assert acc > 0.9, key
+
+
+def _clouds():
+    return [(_uniform(11, 20000), dict(quantizer=O.TO, l2_bound=0.01)),
+            (np.round(_uniform(12, 15000) / 4, 1).astype(np.float32),
+             dict(quantizer=O.CUBE, l2_bound=0.02)),
+            (_uniform(13, 6000, 20.0), dict(quantizer=O.TO, l2_bound=0.1)),
+            (_uniform(14, 9000, 3000.0), dict(quantizer=O.TO, l2_bound=0.05))]
+
+
+def test_selected_batch_streams(X):
+    cases = _clouds()
+    pts = torch.from_numpy(np.concatenate([c for c, _ in cases])).cuda()
+    offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in cases])]).tolist()
+    opts = [dict(o, frame_mode=0) for _, o in cases]
+    # search only (params None): the chosen modes are the search's optimum
+    fr, pre, st, modes, searched = X.compress_selected_batch(pts, offs, opts, want_prezstd=True)
+    res = X.search_batch(pts, offs, [o for _, o in cases])
+    assert list(searched) == [1] * len(cases)
+    for i, ((c, o), m, r) in enumerate(zip(cases, modes, res)):
+        assert m == r["best"], i
+        assert len(pre[i]) == r["best_bytes"], i
+        want = O.compress_mode(c, o["quantizer"], o["l2_bound"], bool(m["is_hilbert"]),
+                               bool(m["rle"]), gr_k=m["gr_k"], pfd_width=m["pfd_width"])
+        assert pre[i] == want, i
+        back = O.decompress_frame(fr[i], o["quantizer"], o["l2_bound"], bool(m["rle"]))
+        assert np.array_equal(back, O.decompress_buffer(want, o["quantizer"], o["l2_bound"],
+                                                        bool(m["rle"])))
+    # with a selector: confident clouds take the prediction, the rest the search
+    F = X.features_batch(pts, offs, [o for _, o in cases])
+    # np.repeat repeats rows in place, so each label repeats in place too
+    params, _ = X.mlp_train(np.repeat(F, 64, axis=0),
+                            [r["best"] for r in res for _ in range(64)], epochs=30, seed=3)
+    fr2, pre2, _, modes2, searched2 = X.compress_selected_batch(pts, offs, opts, params=params,
+                                                                min_confidence=0.5,
+                                                                want_prezstd=True)
+    pred, conf = X.mlp_predict(params, F)
+    for i, ((c, o), m) in enumerate(zip(cases, modes2)):
+        if searched2[i]:
+            assert conf[i] < 0.5 and m == res[i]["best"]
+        else:
+            assert conf[i] >= 0.5 and m == pred[i]
+        want = O.compress_mode(c, o["quantizer"], o["l2_bound"], bool(m["is_hilbert"]),
+                               bool(m["rle"]), gr_k=m["gr_k"], pfd_width=m["pfd_width"])
+        assert pre2[i] == want, i

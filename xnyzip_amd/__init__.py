@@ -23,7 +23,8 @@ from ._lib import Context, XnyzError  # noqa: F401
 
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
-           "compress", "compress_batch", "search_batch", "quantize", "morton_to_hilbert"]
+           "compress", "compress_batch", "search_batch", "features_batch", "mlp_train",
+           "mlp_predict", "compress_selected_batch", "quantize", "morton_to_hilbert"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -170,6 +171,99 @@ def search_batch(points_device, offsets, opts, ctx: Context | None = None):
                     "variants": [dict(r.variant[v].as_dict(), bytes=r.variant_bytes[v])
                                  for v in range(4)]})
     return out
+
+
+def _batch_args(points_device, offsets, opts, frame_default=_lib.FRAME_NONE):
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    count = len(offsets) - 1
+    if count <= 0 or len(opts) != count:
+        raise ValueError("need one opts entry per cloud")
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    arr = (_lib.Opts * count)()
+    for i, o in enumerate(opts):
+        arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
+                       o.get("rle", False), o.get("direct_threshold", 1024),
+                       o.get("frame_mode", frame_default), o.get("zstd_workers", 0))
+    torch.cuda.synchronize(t.device)
+    return t, count, off, arr
+
+
+def features_batch(points_device, offsets, opts, ctx: Context | None = None) -> np.ndarray:
+    """Selector features of every cloud: (count, 32) float32."""
+    t, count, off, arr = _batch_args(points_device, offsets, opts)
+    ctx = ctx or default_context(t.device.index or 0)
+    out = np.zeros((count, _lib.MLP_FEATURES), dtype=np.float32)
+    _lib.check(_lib.lib().xnyz_features_batch_device(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(out.ctypes.data)))
+    return out
+
+
+def _modes_array(modes):
+    arr = (_lib.Mode * len(modes))()
+    for i, m in enumerate(modes):
+        arr[i] = _lib.Mode(int(m["is_hilbert"]), int(m["rle"]), int(m["gr_k"]),
+                           int(m["pfd_width"]))
+    return arr
+
+
+def mlp_train(features, modes, epochs=30, batch=256, lr=3e-3, seed=0, params=None,
+              ctx: Context | None = None):
+    """Train the selector on (features, mode dicts); returns (params, final loss)."""
+    x = np.ascontiguousarray(features, dtype=np.float32)
+    n = x.shape[0]
+    p = np.zeros(_lib.MLP_PARAMS, np.float32) if params is None else \
+        np.array(params, dtype=np.float32)
+    to = _lib.TrainOpts(int(epochs), int(batch), float(lr), int(params is None), int(seed))
+    loss = C.c_float()
+    ctx = ctx or default_context(0)
+    _lib.check(_lib.lib().xnyz_mlp_train(ctx.handle, C.c_void_p(x.ctypes.data),
+                                          C.cast(_modes_array(modes), C.c_void_p), n,
+                                          C.byref(to), C.c_void_p(p.ctypes.data),
+                                          C.byref(loss)))
+    return p, loss.value
+
+
+def mlp_predict(params, features, ctx: Context | None = None):
+    """Predicted mode dicts and confidences."""
+    x = np.ascontiguousarray(features, dtype=np.float32)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    n = x.shape[0]
+    out = (_lib.Mode * n)()
+    conf = np.zeros(n, np.float32)
+    ctx = ctx or default_context(0)
+    _lib.check(_lib.lib().xnyz_mlp_predict(ctx.handle, C.c_void_p(p.ctypes.data),
+                                            C.c_void_p(x.ctypes.data), n,
+                                            C.cast(out, C.c_void_p),
+                                            C.c_void_p(conf.ctypes.data)))
+    return [m.as_dict() for m in out], conf
+
+
+def compress_selected_batch(points_device, offsets, opts, params=None, min_confidence=0.0,
+                            want_prezstd=False, ctx: Context | None = None):
+    """Batched compression with a mode per cloud: the selector's prediction
+    (params) where confident, the exhaustive search otherwise (params None:
+    search everything).  Returns (frames, prezstd, stats, modes, searched)."""
+    t, count, off, arr = _batch_args(points_device, offsets, opts, _lib.FRAME_ZSTD3)
+    ctx = ctx or default_context(t.device.index or 0)
+    p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+    frames = (_lib.Buf * count)()
+    pres = (_lib.Buf * count)() if want_prezstd else None
+    stats = (_lib.Stats * count)()
+    modes = (_lib.Mode * count)()
+    searched = np.zeros(count, np.int32)
+    _lib.check(_lib.lib().xnyz_compress_selected_batch_device(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(p.ctypes.data) if p is not None else None,
+        float(min_confidence), C.cast(frames, C.c_void_p),
+        C.cast(pres, C.c_void_p) if pres is not None else None, C.cast(stats, C.c_void_p),
+        C.cast(modes, C.c_void_p), C.c_void_p(searched.ctypes.data)))
+    out_frames = [_lib.take(f) for f in frames]
+    out_pres = [_lib.take(b) for b in pres] if pres is not None else None
+    return out_frames, out_pres, list(stats), [m.as_dict() for m in modes], searched
 
 
 class BlockCompressor:

@@ -41,6 +41,14 @@ class Mode(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+MLP_FEATURES, MLP_PARAMS = 32, 9196
+
+
+class TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float),
+                ("init", C.c_int32), ("seed", C.c_uint64)]
+
+
 class SearchResult(C.Structure):
     _fields_ = [("best", Mode), ("best_bytes", C.c_uint64), ("reference_bytes", C.c_uint64),
                 ("variant", Mode * 4), ("variant_bytes", C.c_uint64 * 4)]
@@ -74,7 +82,8 @@ _lock = threading.Lock()
 
 EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress", "xnyz_compress_device", "xnyz_compress_batch_device",
-           "xnyz_search_batch_device",
+           "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
+           "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_quantize_device",
            "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_profile_enable",
            "xnyz_profile_report", "xnyz_free"]
@@ -99,6 +108,12 @@ def lib() -> C.CDLL:
         L.xnyz_compress_device.argtypes = comp
         L.xnyz_compress_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, P, P]
         L.xnyz_search_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
+        L.xnyz_features_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
+        L.xnyz_mlp_train.argtypes = [P, P, P, C.c_uint64, C.POINTER(TrainOpts), P,
+                                     C.POINTER(C.c_float)]
+        L.xnyz_mlp_predict.argtypes = [P, P, P, C.c_uint64, P, P]
+        L.xnyz_compress_selected_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, C.c_float,
+                                                          P, P, P, P, P]
         L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]

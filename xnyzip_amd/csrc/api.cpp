@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <random>
 #include <string>
 #include <thread>
 #include <vector>
@@ -35,6 +36,7 @@
 #include "codebook.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "modes.hpp"
 
 // Exported by libzstd 1.4 (zstd_compress.c) though absent from its public
@@ -208,8 +210,11 @@ struct xnyz_ctx {
   DBuf tiles, sp, sc, sh, lut, bigv, bigc, large, nlarge;
   DBuf keys[2], lo[2], cells, cells_lo, pos, cnt, out, arena, upload, tmp;
   DBuf idx[2], lo64[2];  // permutation payload
+  DBuf feat;             // selector statistics
+  // selector: parameters, FP16 weights, optimiser state, samples
+  DBuf mP, mWh, mG, mM, mV, mX, mY, mIdx, mScratch, mLoss, mOut, mConf;
   // pinned host mirrors
-  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out;
+  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat;
   bool prof_on = false;
   EventProfiler prof;
 
@@ -485,7 +490,55 @@ struct CallOut {
   // Mode search: stop after the statistics and return each segment's size
   // model instead of a stream.
   std::vector<ModeStats>* search = nullptr;
+  // Selector features: stop after the range scan and return kFeat floats
+  // per segment.
+  std::vector<float>* features = nullptr;
 };
+
+// HyperLogLog estimate from 256 registers (small-range correction by linear
+// counting).
+double hll_estimate(const u32* reg) {
+  const double m = kHll, alpha = 0.7213 / (1.0 + 1.079 / m);
+  double sum = 0.0;
+  u32 zeros = 0;
+  for (u32 j = 0; j < kHll; ++j) {
+    sum += std::ldexp(1.0, -(int)reg[j]);
+    zeros += reg[j] == 0;
+  }
+  double e = alpha * m * m / sum;
+  if (e <= 2.5 * m && zeros) e = m * std::log(m / zeros);
+  return e;
+}
+
+// The selector's 32 inputs for one segment.
+void segment_features(const SegFeat& f, const SegCtl& c, const xnyz_opts& o, u64 n, float* x) {
+  const double nn = (double)n;
+  const double rx = c.range[0], ry = c.range[1], rz = c.range[2];
+  const double U = std::min(nn, std::max(1.0, hll_estimate(f.hll_cell)));
+  const double Sb = std::min(U, std::max(1.0, hll_estimate(f.hll_sub)));
+  const double B = std::min(Sb, std::max(1.0, hll_estimate(f.hll_blk)));
+  const double pairs = std::max(1.0, (double)f.pairs);
+  const int max_range = std::max({c.range[0], c.range[1], c.range[2]});
+  x[0] = (float)std::log2(nn);
+  x[1] = (float)std::log2(1 + rx);
+  x[2] = (float)std::log2(1 + ry);
+  x[3] = (float)std::log2(1 + rz);
+  x[4] = (float)(std::log2(nn) - std::log2((1 + rx) * (1 + ry) * (1 + rz)));
+  x[5] = (float)std::log2(U);
+  x[6] = (float)std::log2(nn / U);
+  x[7] = (float)std::log2(Sb);
+  x[8] = (float)std::log2(B);
+  x[9] = (float)std::log2(U / Sb);
+  x[10] = (float)std::log2(U / B);
+  x[11] = o.quantizer == XNYZ_TRUNCATED_OCTAHEDRON ? 1.f : 0.f;
+  x[12] = (float)std::log10((double)o.l2_bound);
+  x[13] = max_range < o.direct_threshold ? 1.f : 0.f;
+  x[14] = (float)(c.wrapped / nn);
+  x[15] = (float)(f.same_cell / pairs);
+  x[16] = (float)(f.same_sub / pairs);
+  x[17] = (float)(f.same_blk / pairs);
+  for (int b = 1; b <= 14; ++b) x[17 + b] = (float)(f.xor_hist[b] / pairs);
+}
 
 int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off,
                   const xnyz_opts* opts, const CallOut& co, double t_start) {
@@ -564,8 +617,21 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   launch_init(bt, st);
   launch_bbox(bt, d_xyz, st);
   launch_quant_stats(bt, d_xyz, st);
+  if (co.features) {
+    ctx->feat.ensure(sizeof(SegFeat) * S);
+    ctx->h_feat.ensure(sizeof(SegFeat) * S);
+    launch_features(bt, d_xyz, ctx->feat.as<SegFeat>(), st);
+    ctx->d2h(ctx->h_feat, ctx->feat, sizeof(SegFeat) * S);
+  }
   ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
   ctx->sync();
+  if (co.features) {
+    co.features->resize((size_t)S * kFeat);
+    const SegFeat* hf = ctx->h_feat.as<SegFeat>();
+    for (u32 s = 0; s < S; ++s)
+      segment_features(hf[s], sc[s], opts[s], g[s].n, co.features->data() + (size_t)s * kFeat);
+    return XNYZ_OK;
+  }
 
   // ---- key layout per segment (block_compressor.hpp:69-139) -------------------
   bool wide = false, any_block = false, any_direct = false;
@@ -916,6 +982,68 @@ int guarded(F&& f) {
 
 void use_device(xnyz_ctx* ctx) { check_hip(hipSetDevice(ctx->device), "hipSetDevice"); }
 
+// Batch offsets relative to the first cloud (advancing d_xyz to it).
+std::vector<u64> rebase(const uint64_t* offsets, uint32_t count, const float*& d_xyz) {
+  std::vector<u64> off(offsets, offsets + count + 1);
+  for (u32 s = 0; s < count; ++s)
+    if (off[s + 1] < off[s]) throw Error(XNYZ_EINVAL, "offsets must not decrease");
+  if (off[0] != 0) {
+    for (auto& v : off) v -= offsets[0];
+    d_xyz += 3 * offsets[0];
+  }
+  return off;
+}
+
+// Exhaustive mode search of every segment: one statistics pass per curve x
+// RLE variant, then the (k, width) grid from the size model.
+void search_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off,
+                 const xnyz_opts* opts, xnyz_search_result* results) {
+  const u32 count = (u32)(off.size() - 1);
+  for (u32 s = 0; s < count; ++s) {
+    std::memset(&results[s], 0, sizeof(results[s]));
+    results[s].best_bytes = ~0ull;
+  }
+  std::vector<xnyz_opts> vo(opts, opts + count);
+  for (int var = 0; var < 4; ++var) {  // 2 * is_hilbert + rle
+    for (u32 s = 0; s < count; ++s) {
+      vo[s] = opts[s];
+      vo[s].is_hilbert = var >> 1;
+      vo[s].rle = var & 1;
+      vo[s].mode_flags = 0;
+    }
+    std::vector<ModeStats> ms;
+    CallOut co{nullptr, nullptr, nullptr, nullptr};
+    co.search = &ms;
+    compress_impl(ctx, d_xyz, off, vo.data(), co, now_ms());
+    parallel_for(count, host_threads(0), [&](size_t s) {
+      ModeChoice ref{}, best{};
+      search_modes(ms[s], ref, best);
+      xnyz_search_result& r = results[s];
+      r.variant[var] = xnyz_mode{var >> 1, var & 1, best.k, best.w};
+      r.variant_bytes[var] = best.bytes;
+      if ((opts[s].is_hilbert != 0) == (var >> 1) && (opts[s].rle != 0) == (var & 1))
+        r.reference_bytes = ref.bytes;
+    });
+  }
+  for (u32 s = 0; s < count; ++s) {
+    xnyz_search_result& r = results[s];
+    for (int var = 0; var < 4; ++var)
+      if (r.variant_bytes[var] < r.best_bytes) {
+        r.best_bytes = r.variant_bytes[var];
+        r.best = r.variant[var];
+      }
+  }
+}
+
+xnyz_opts with_mode(xnyz_opts o, const xnyz_mode& m) {
+  o.is_hilbert = m.is_hilbert;
+  o.rle = m.rle;
+  o.gr_k = m.gr_k;
+  o.pfd_width = m.pfd_width;
+  o.mode_flags = XNYZ_MODE_GR_K | XNYZ_MODE_PFD_WIDTH;
+  return o;
+}
+
 }  // namespace
 
 extern "C" {
@@ -954,9 +1082,12 @@ int xnyz_destroy(xnyz_ctx* ctx) {
                     &ctx->large, &ctx->nlarge, &ctx->keys[0], &ctx->keys[1], &ctx->lo[0],
                     &ctx->lo[1], &ctx->cells, &ctx->cells_lo, &ctx->pos, &ctx->cnt, &ctx->out,
                     &ctx->arena, &ctx->upload, &ctx->tmp, &ctx->idx[0], &ctx->idx[1],
-                    &ctx->lo64[0], &ctx->lo64[1]})
+                    &ctx->lo64[0], &ctx->lo64[1], &ctx->feat, &ctx->mP, &ctx->mWh, &ctx->mG,
+                    &ctx->mM, &ctx->mV, &ctx->mX, &ctx->mY, &ctx->mIdx, &ctx->mScratch,
+                    &ctx->mLoss, &ctx->mOut, &ctx->mConf})
       b->release();
-    for (HBuf* b : {&ctx->h_tiles, &ctx->h_sp, &ctx->h_sc, &ctx->h_sh, &ctx->h_lut, &ctx->h_out})
+    for (HBuf* b : {&ctx->h_tiles, &ctx->h_sp, &ctx->h_sc, &ctx->h_sh, &ctx->h_lut, &ctx->h_out,
+                    &ctx->h_feat})
       b->release();
     ctx->prof.destroy();
     (void)hipStreamDestroy(ctx->stream);
@@ -1018,47 +1149,230 @@ int xnyz_search_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* 
   return guarded([&] {
     if (!ctx || !d_xyz || !offsets || !opts || !count || !results)
       throw Error(XNYZ_EINVAL, "null argument or empty batch");
-    std::vector<u64> off(offsets, offsets + count + 1);
-    for (u32 s = 0; s < count; ++s)
-      if (off[s + 1] < off[s]) throw Error(XNYZ_EINVAL, "offsets must not decrease");
-    if (off[0] != 0) {
-      for (auto& v : off) v -= offsets[0];
-      d_xyz += 3 * offsets[0];
-    }
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
     use_device(ctx);
-    for (u32 s = 0; s < count; ++s) {
-      std::memset(&results[s], 0, sizeof(results[s]));
-      results[s].best_bytes = ~0ull;
-    }
-    std::vector<xnyz_opts> vo(opts, opts + count);
-    for (int var = 0; var < 4; ++var) {  // 2 * is_hilbert + rle
-      for (u32 s = 0; s < count; ++s) {
-        vo[s] = opts[s];
-        vo[s].is_hilbert = var >> 1;
-        vo[s].rle = var & 1;
-        vo[s].mode_flags = 0;
-      }
-      std::vector<ModeStats> ms;
-      CallOut co{nullptr, nullptr, nullptr, nullptr};
-      co.search = &ms;
-      compress_impl(ctx, d_xyz, off, vo.data(), co, now_ms());
-      parallel_for(count, host_threads(0), [&](size_t s) {
-        ModeChoice ref{}, best{};
-        search_modes(ms[s], ref, best);
-        xnyz_search_result& r = results[s];
-        r.variant[var] = xnyz_mode{var >> 1, var & 1, best.k, best.w};
-        r.variant_bytes[var] = best.bytes;
-        if ((opts[s].is_hilbert != 0) == (var >> 1) && (opts[s].rle != 0) == (var & 1))
-          r.reference_bytes = ref.bytes;
-      });
-    }
-    for (u32 s = 0; s < count; ++s) {
-      xnyz_search_result& r = results[s];
-      for (int var = 0; var < 4; ++var)
-        if (r.variant_bytes[var] < r.best_bytes) {
-          r.best_bytes = r.variant_bytes[var];
-          r.best = r.variant[var];
+    search_impl(ctx, d_xyz, off, opts, results);
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_features_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                               uint32_t count, const xnyz_opts* opts, float* features) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !features)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    use_device(ctx);
+    std::vector<float> f;
+    CallOut co{nullptr, nullptr, nullptr, nullptr};
+    co.features = &f;
+    compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+    std::memcpy(features, f.data(), f.size() * sizeof(float));
+    return XNYZ_OK;
+  });
+}
+
+namespace {
+
+static_assert(kPTotal == XNYZ_MLP_PARAMS && kFeat == XNYZ_MLP_FEATURES,
+              "selector layout differs from include/xnyz.h");
+
+MlpLabel label_of(const xnyz_mode& m) {
+  return MlpLabel{std::min(std::max(m.gr_k, 0), 31), std::min(std::max(m.pfd_width, 1), 8) - 1,
+                  m.rle ? 1 : 0, m.is_hilbert ? 1 : 0};
+}
+
+void upload(DBuf& d, const void* src, size_t bytes, hipStream_t st) {
+  d.ensure(bytes + 16);
+  check_hip(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, st), "H2D");
+}
+
+void predict_impl(xnyz_ctx* ctx, const float* params, const float* features, u64 n,
+                  std::vector<MlpLabel>& out, std::vector<float>& conf) {
+  hipStream_t st = ctx->stream;
+  upload(ctx->mP, params, kPTotal * sizeof(float), st);
+  ctx->mWh.ensure(kHalfWeights * sizeof(__half) + 16);
+  mlp_refresh_half(ctx->mP.as<float>(), ctx->mWh.as<__half>(), st);
+  out.resize(n);
+  conf.resize(n);
+  constexpr u64 kChunk = 1 << 16;
+  ctx->mScratch.ensure(mlp_scratch_floats((u32)std::min(n, kChunk)) * sizeof(float));
+  ctx->mOut.ensure(std::min(n, kChunk) * sizeof(MlpLabel));
+  ctx->mConf.ensure(std::min(n, kChunk) * sizeof(float));
+  for (u64 a = 0; a < n; a += kChunk) {
+    const u32 m = (u32)std::min(kChunk, n - a);
+    upload(ctx->mX, features + a * kFeat, (size_t)m * kFeat * sizeof(float), st);
+    mlp_predict(ctx->mX.as<float>(), m, ctx->mP.as<float>(), ctx->mWh.as<__half>(),
+                ctx->mOut.as<MlpLabel>(), ctx->mConf.as<float>(), ctx->mScratch.as<float>(), st);
+    check_hip(hipMemcpyAsync(out.data() + a, ctx->mOut.p, m * sizeof(MlpLabel),
+                             hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipMemcpyAsync(conf.data() + a, ctx->mConf.p, m * sizeof(float),
+                             hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipStreamSynchronize(st), "sync");
+  }
+}
+
+}  // namespace
+
+int xnyz_mlp_train(xnyz_ctx* ctx, const float* features, const xnyz_mode* labels, uint64_t n,
+                   const xnyz_train_opts* to, float* params, float* final_loss) {
+  return guarded([&] {
+    if (!ctx || !features || !labels || !n || !to || !params)
+      throw Error(XNYZ_EINVAL, "null argument or no samples");
+    if (to->epochs <= 0 || to->batch <= 0 || to->batch > 65536 || !(to->lr > 0.f))
+      throw Error(XNYZ_EINVAL, "bad training options");
+    if (n > 0xFFFFFFFFull) throw Error(XNYZ_EUNSUPPORTED, "more than 2^32-1 samples");
+    use_device(ctx);
+    hipStream_t st = ctx->stream;
+    if (to->init) {  // normaliser from the data, He-initialised layers, zero biases
+      std::vector<double> mean(kFeat, 0.0), var(kFeat, 0.0);
+      for (u64 i = 0; i < n; ++i)
+        for (int f = 0; f < kFeat; ++f) mean[f] += features[i * kFeat + f];
+      for (int f = 0; f < kFeat; ++f) mean[f] /= (double)n;
+      for (u64 i = 0; i < n; ++i)
+        for (int f = 0; f < kFeat; ++f) {
+          const double d = features[i * kFeat + f] - mean[f];
+          var[f] += d * d;
         }
+      std::mt19937_64 rng(to->seed);
+      std::normal_distribution<float> nd(0.f, 1.f);
+      std::fill(params, params + kPTotal, 0.f);
+      for (int f = 0; f < kFeat; ++f) {
+        params[kPMean + f] = (float)mean[f];
+        const double sd = std::sqrt(var[f] / (double)n);
+        params[kPIstd + f] = sd > 1e-6 ? (float)(1.0 / sd) : 0.f;
+      }
+      auto he = [&](int off, int fan_in, int count) {
+        const float sc = std::sqrt(2.f / (float)fan_in);
+        for (int i = 0; i < count; ++i) params[off + i] = nd(rng) * sc;
+      };
+      he(kPW1, kFeat, kFeat * kHid);
+      he(kPW2, kHid, kHid * kHid);
+      he(kPW3, kHid, kHid * kOut);
+    }
+    std::vector<MlpLabel> y(n);
+    for (u64 i = 0; i < n; ++i) y[i] = label_of(labels[i]);
+    upload(ctx->mX, features, n * kFeat * sizeof(float), st);
+    upload(ctx->mY, y.data(), n * sizeof(MlpLabel), st);
+    upload(ctx->mP, params, kPTotal * sizeof(float), st);
+    ctx->mWh.ensure(kHalfWeights * sizeof(__half) + 16);
+    for (DBuf* b : {&ctx->mG, &ctx->mM, &ctx->mV}) {
+      b->ensure(kTrainable * sizeof(float) + 16);
+      check_hip(hipMemsetAsync(b->p, 0, kTrainable * sizeof(float), st), "memset");
+    }
+    const u32 nb = (u32)std::min<u64>(to->batch, n);
+    ctx->mScratch.ensure(mlp_scratch_floats(nb) * sizeof(float));
+    ctx->mLoss.ensure(4 * sizeof(float));
+    ctx->mIdx.ensure(n * 4 + 16);
+    mlp_refresh_half(ctx->mP.as<float>(), ctx->mWh.as<__half>(), st);
+    std::vector<u32> perm(n);
+    std::mt19937_64 rng(to->seed ^ 0x5DEECE66Dull);
+    int t = 0;
+    float loss = 0.f;
+    for (int e = 0; e < to->epochs; ++e) {
+      for (u64 i = 0; i < n; ++i) perm[i] = (u32)i;
+      std::shuffle(perm.begin(), perm.end(), rng);
+      check_hip(hipMemcpyAsync(ctx->mIdx.p, perm.data(), n * 4, hipMemcpyHostToDevice, st), "H2D");
+      check_hip(hipMemsetAsync(ctx->mLoss.p, 0, sizeof(float), st), "memset");
+      u32 steps = 0;
+      for (u64 a = 0; a + nb <= n; a += nb) {  // whole batches only
+        mlp_train_step(ctx->mX.as<float>(), ctx->mY.as<MlpLabel>(), ctx->mIdx.as<u32>() + a, nb,
+                       ctx->mP.as<float>(), ctx->mWh.as<__half>(), ctx->mScratch.as<float>(),
+                       ctx->mG.as<float>(), ctx->mM.as<float>(), ctx->mV.as<float>(), ++t,
+                       to->lr, ctx->mLoss.as<float>(), st);
+        ++steps;
+      }
+      float acc = 0.f;
+      check_hip(hipMemcpyAsync(&acc, ctx->mLoss.p, sizeof(float), hipMemcpyDeviceToHost, st),
+                "D2H");
+      check_hip(hipStreamSynchronize(st), "sync");  // also orders the next perm upload
+      loss = steps ? acc / (float)steps : 0.f;
+    }
+    check_hip(hipMemcpyAsync(params, ctx->mP.p, kPTotal * sizeof(float), hipMemcpyDeviceToHost,
+                             st), "D2H");
+    check_hip(hipStreamSynchronize(st), "sync");
+    if (final_loss) *final_loss = loss;
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_mlp_predict(xnyz_ctx* ctx, const float* params, const float* features, uint64_t n,
+                     xnyz_mode* modes, float* confidence) {
+  return guarded([&] {
+    if (!ctx || !params || !features || !n || !modes)
+      throw Error(XNYZ_EINVAL, "null argument or no samples");
+    use_device(ctx);
+    std::vector<MlpLabel> out;
+    std::vector<float> conf;
+    predict_impl(ctx, params, features, n, out, conf);
+    for (u64 i = 0; i < n; ++i) {
+      modes[i] = xnyz_mode{out[i].curve, out[i].rle, out[i].k, out[i].w + 1};
+      if (confidence) confidence[i] = conf[i];
+    }
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_compress_selected_batch_device(xnyz_ctx* ctx, const float* d_xyz,
+                                        const uint64_t* offsets, uint32_t count,
+                                        const xnyz_opts* opts, const float* params,
+                                        float min_confidence, xnyz_buf* frames,
+                                        xnyz_buf* prezstd, xnyz_stats* stats,
+                                        xnyz_mode* chosen, int32_t* searched) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !frames)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    use_device(ctx);
+    for (u32 s = 0; s < count; ++s) validate(&opts[s], off[s + 1] - off[s]);
+    std::vector<xnyz_mode> mode(count);
+    std::vector<char> fall(count, 1);
+    if (params) {  // features -> MLP
+      std::vector<float> f;
+      CallOut co{nullptr, nullptr, nullptr, nullptr};
+      co.features = &f;
+      compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+      std::vector<MlpLabel> out;
+      std::vector<float> conf;
+      predict_impl(ctx, params, f.data(), count, out, conf);
+      for (u32 s = 0; s < count; ++s) {
+        mode[s] = xnyz_mode{out[s].curve, out[s].rle, out[s].k, out[s].w + 1};
+        fall[s] = conf[s] < min_confidence;
+      }
+    }
+    std::vector<u32> fb;
+    for (u32 s = 0; s < count; ++s)
+      if (fall[s]) fb.push_back(s);
+    if (!fb.empty()) {  // exhaustive search of the fallback clouds, gathered
+      const bool all = fb.size() == count;
+      std::vector<u64> so{0};
+      std::vector<xnyz_opts> sopt;
+      const float* src = d_xyz;
+      if (!all) {
+        u64 tot = 0;
+        for (u32 s : fb) tot += off[s + 1] - off[s];
+        ctx->tmp.ensure((size_t)tot * 12);
+        for (u32 s : fb) {
+          const u64 n = off[s + 1] - off[s];
+          check_hip(hipMemcpyAsync(ctx->tmp.as<float>() + 3 * so.back(), d_xyz + 3 * off[s],
+                                   n * 12, hipMemcpyDeviceToDevice, ctx->stream), "gather");
+          so.push_back(so.back() + n);
+        }
+        src = ctx->tmp.as<float>();
+      } else {
+        so = off;
+      }
+      for (u32 s : fb) sopt.push_back(opts[s]);
+      std::vector<xnyz_search_result> r(fb.size());
+      search_impl(ctx, src, so, sopt.data(), r.data());
+      for (size_t i = 0; i < fb.size(); ++i) mode[fb[i]] = r[i].best;
+    }
+    std::vector<xnyz_opts> vo(count);
+    for (u32 s = 0; s < count; ++s) vo[s] = with_mode(opts[s], mode[s]);
+    compress_impl(ctx, d_xyz, off, vo.data(), CallOut{frames, prezstd, stats, nullptr}, now_ms());
+    for (u32 s = 0; s < count; ++s) {
+      if (chosen) chosen[s] = mode[s];
+      if (searched) searched[s] = fall[s];
     }
     return XNYZ_OK;
   });

@@ -539,6 +539,108 @@ __global__ __launch_bounds__(kBlock) void k_keygen(Batch bt, const F3* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Selector features (see SegFeat).  HyperLogLog: a 64-bit mix of the key
+// picks one of 256 registers by its top 8 bits; the register keeps the
+// largest 1 + (leading zeros of the remaining 56 bits).
+
+__device__ __forceinline__ u64 mix64(u64 z) {  // SplitMix64 finaliser
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ void hll_add(u32* reg, u64 key, u64 salt) {
+  const u64 h = mix64(key * 0x9E3779B97F4A7C15ull + salt);
+  const u32 r = (u32)__clzll((i64)((h << 8) | 0x80ull)) + 1u;
+  const u32 j = (u32)(h >> 56);
+  if (r > reg[j]) atomicMax(&reg[j], r);
+}
+
+struct FeatLds {
+  u32 cell[kHll], sub[kHll], blk[kHll];
+  u32 xh[16];
+  u32 red[kWavesPerBlock];
+};
+
+__device__ __forceinline__ void flush_feat(FeatLds& L, SegFeat& f, u32 (&c)[4]) {
+  u32 tot[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) tot[i] = block_sum(c[i], L.red);  // synchronises
+  const u32 d = threadIdx.x;
+  if (d < 4 && tot[d]) atomicAdd(d == 0 ? &f.same_cell : d == 1 ? &f.same_sub
+                                        : d == 2 ? &f.same_blk : &f.pairs, tot[d]);
+  if (L.cell[d]) atomicMax(&f.hll_cell[d], L.cell[d]);
+  if (L.sub[d]) atomicMax(&f.hll_sub[d], L.sub[d]);
+  if (L.blk[d]) atomicMax(&f.hll_blk[d], L.blk[d]);
+  if (d < 16 && L.xh[d]) atomicAdd(&f.xor_hist[d], L.xh[d]);
+  __syncthreads();
+  L.cell[d] = L.sub[d] = L.blk[d] = 0;
+  if (d < 16) L.xh[d] = 0;
+  c[0] = c[1] = c[2] = c[3] = 0;
+  __syncthreads();
+}
+
+__device__ __forceinline__ u64 node_morton(const F3& q, const float* off, const QuantParams& qp,
+                                           int& X, int& Y, int& Z) {
+  float sx, sy, sz;
+  quant_point(q, off, qp, sx, sy, sz, X, Y, Z);
+  return split3_21((u32)X) | (split3_21((u32)Y) << 1) | (split3_21((u32)Z) << 2);
+}
+
+__global__ __launch_bounds__(kBlock) void k_features(Batch bt, const F3* __restrict__ p,
+                                                     SegFeat* __restrict__ feat) {
+  static_assert(kHll == kBlock, "one register of each table per thread");
+  __shared__ FeatLds L;
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
+  const u32 d = threadIdx.x;
+  L.cell[d] = L.sub[d] = L.blk[d] = 0;
+  if (d < 16) L.xh[d] = 0;
+  __syncthreads();
+  u32 c[4] = {0, 0, 0, 0};  // same cell, same sub-block, same block, pairs
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_feat(L, feat[cur], c);
+      cur = t.seg;
+    }
+    const SegParam& sp = bt.sp[t.seg];
+    const SegCtl& sc = bt.sc[t.seg];
+    const Range r = tile_range(sp.p0, sp.n, t.local);
+    const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
+#pragma unroll 4
+    for (u32 k = 0; k < kItems; ++k) {
+      const u32 o = k * kBlock + threadIdx.x;
+      if (o < r.count) {
+        const u32 i = r.first + o;
+        int X, Y, Z;
+        const u64 m = node_morton(p[i], off, sp.qp, X, Y, Z);
+        const u64 cell = ((u64)(u32)X << 42) ^ ((u64)(u32)Y << 21) ^ (u64)(u32)Z;
+        const u64 sub = ((u64)(u32)(X >> 6) << 42) ^ ((u64)(u32)(Y >> 6) << 21) ^ (u32)(Z >> 6);
+        const u64 blk = ((u64)(u32)(X >> 7) << 42) ^ ((u64)(u32)(Y >> 7) << 21) ^ (u32)(Z >> 7);
+        hll_add(L.cell, cell, 1);
+        hll_add(L.sub, sub, 2);
+        hll_add(L.blk, blk, 3);
+        if (i > sp.p0) {
+          int PX, PY, PZ;
+          const u64 pm = node_morton(p[i - 1], off, sp.qp, PX, PY, PZ);
+          const u64 x = m ^ pm;
+          const u32 bl = bitlen64(x);
+          atomicAdd(&L.xh[bl / 4 < 15 ? bl / 4 : 15], 1u);
+          c[0] += x == 0;
+          c[1] += (X >> 6) == (PX >> 6) && (Y >> 6) == (PY >> 6) && (Z >> 6) == (PZ >> 6);
+          c[2] += (X >> 7) == (PX >> 7) && (Y >> 7) == (PY >> 7) && (Z >> 7) == (PZ >> 7);
+          c[3] += 1;
+        }
+      }
+    }
+  }
+  flush_feat(L, feat[cur], c);
+}
+
 __global__ __launch_bounds__(kBlock) void k_m2h(const u64* in, u32 n, u32 L, HilbertSM hs,
                                                 u64* out) {
   __shared__ SMTables t;
@@ -1414,6 +1516,13 @@ void launch_bbox(const Batch& bt, const float* xyz, hipStream_t s) {
 void launch_quant_stats(const Batch& bt, const float* xyz, hipStream_t s) {
   k_quant_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
   LAUNCH_CHECK("k_quant_stats");
+}
+
+void launch_features(const Batch& bt, const float* xyz, SegFeat* feat, hipStream_t s) {
+  launch_fill_u32(reinterpret_cast<u32*>(feat), (u64)bt.S * (sizeof(SegFeat) / 4), 0, s);
+  k_features<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz),
+                                                        feat);
+  LAUNCH_CHECK("k_features");
 }
 
 void launch_quantize_nodes(const Batch& bt, const float* xyz, int32_t* nodes, hipStream_t s) {

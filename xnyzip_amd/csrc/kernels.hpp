@@ -144,6 +144,19 @@ struct SegHist {  // per-segment symbol counts and first occurrences
   u32 mag[kMagBins];  // histogram of bitlen(v) over all repos values
 };
 
+// Per-segment statistics for the learned mode selector, from one pass over
+// the points in input order after the range scan: HyperLogLog registers
+// (256 each) of the lattice cells, the 64^3 sub-blocks and the 128^3 blocks,
+// and, over consecutive input points, a histogram of bitlen(m_i xor m_i-1)
+// of their 63-bit node Morton codes plus how often both share a cell,
+// sub-block or block.
+constexpr u32 kHll = 256;
+struct SegFeat {
+  u32 hll_cell[kHll], hll_sub[kHll], hll_blk[kHll];
+  u32 xor_hist[16];  // bin = min(15, bitlen / 4)
+  u32 same_cell, same_sub, same_blk, pairs;
+};
+
 struct SegLut {   // per-segment codebooks (host-built)
   HufEntry sym[kSymBins];
   HufEntry quad[kSymBins];
@@ -182,6 +195,8 @@ struct KeyView {
 void launch_init(const Batch& bt, hipStream_t s);
 void launch_bbox(const Batch& bt, const float* xyz, hipStream_t s);
 void launch_quant_stats(const Batch& bt, const float* xyz, hipStream_t s);
+// Selector statistics of every segment (after launch_quant_stats).
+void launch_features(const Batch& bt, const float* xyz, SegFeat* feat, hipStream_t s);
 void launch_quantize_nodes(const Batch& bt, const float* xyz, int32_t* nodes, hipStream_t s);
 // idx (nullable): point index payload; lo64 (nullable, wide keys only):
 // index << 32 | low key word, written instead of lo.
