@@ -42,6 +42,19 @@ WORKLOADS = {
                        rle=True),
 }
 
+# Batched workloads (one launch set per measurement; see run_batched).
+BATCHED = {
+    # vesicle-like shells cut into 2^20-point chunks, one stream per chunk
+    "shells-100M-chunked": dict(kind="chunked", n=100_000_000, chunk=1 << 20, quantizer=1,
+                                l2=0.05, hilbert=False, rle=False),
+    # independent clouds, log-uniform sizes in [1K, 4M], terrain (even i) or
+    # shells (odd i) with seed i, L2 bound {0.001, 0.01, 0.1}[i mod 3]
+    "batched-uneven": dict(kind="uneven", clouds=512, quantizer=1, hilbert=False, rle=True),
+    # half terrain, half shells, chunks of 2^U(16, 20) points: exhaustive
+    # mode search ("plain") vs the learned selector
+    "selector-64M": dict(kind="selector", n=64_000_000, quantizer=1),
+}
+
 
 def generate(w: dict, n: int | None = None):
     from xnyzip_amd import datasets
@@ -51,6 +64,27 @@ def generate(w: dict, n: int | None = None):
     if w["gen"] == "shells":
         return datasets.shells(n, seed=0)
     return datasets.uniform(n, seed=0)
+
+
+class OutBuffer:
+    """Host buffer the outputs are written into, reused across steps (grown
+    once if a call reports it too small)."""
+
+    def __init__(self, nbytes):
+        import numpy as np
+        self.buf = np.zeros(max(1 << 20, nbytes), dtype=np.uint8)
+
+    def compress(self, X, dev, offs, opts, ctx):
+        import numpy as np
+        from xnyzip_amd import _lib
+        try:
+            return X.compress_batch_into(dev, offs, opts, self.buf, ctx=ctx)
+        except _lib.XnyzError as e:
+            if e.code != _lib.XNYZ_ENOSPC:
+                raise
+            need = int(e.args[0].rsplit(":", 1)[1].split()[0])
+            self.buf = np.zeros(need + (1 << 20), dtype=np.uint8)
+            return X.compress_batch_into(dev, offs, opts, self.buf, ctx=ctx)
 
 
 def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launches: int,
@@ -77,7 +111,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="terrain-50M", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="terrain-50M",
+                    choices=sorted(WORKLOADS) + sorted(BATCHED))
+    ap.add_argument("--clouds", type=int, default=0, help="batched-uneven: cloud count")
     ap.add_argument("--points", type=int, default=0, help="override the point count")
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -96,6 +132,9 @@ def main() -> int:
     import xnyzip_amd as X
     from xnyzip_amd import _lib
 
+    if args.workload in BATCHED:
+        return run_batched(args)
+
     w = dict(WORKLOADS[args.workload])
     if args.points:
         w["n"] = args.points
@@ -109,10 +148,15 @@ def main() -> int:
     torch.cuda.synchronize()
     ctx = X.Context(0)
 
+    # Outputs land in one host buffer reused by every step (the caller-memory
+    # entry point), so the timed region holds no per-call allocation.
+    sink = OutBuffer(13 * n)
+    o1 = [dict(quantizer=w["quantizer"], l2_bound=w["l2"], is_hilbert=w["hilbert"],
+               rle=w["rle"])]
+
     def step(mode):
-        frame, _, _, st = X.compress(dev, w["quantizer"], w["l2"], w["hilbert"], w["rle"],
-                                     frame_mode=mode, ctx=ctx)
-        return len(frame), st
+        offs, st = sink.compress(X, dev, [0, n], [dict(o1[0], frame_mode=mode)], ctx)
+        return int(offs[1]), st[0]
 
     def timed(mode, steps, warmup):
         for _ in range(warmup):
@@ -200,6 +244,125 @@ def main() -> int:
         "kernel_ms_per_step": dev_ms,
         **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in extra.items()},
     }
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
+def _uneven_clouds(count):
+    import numpy as np
+
+    from xnyzip_amd import datasets
+    clouds, opts = [], []
+    for i in range(count):
+        rng = np.random.default_rng(i)
+        n = int(np.exp(rng.uniform(np.log(1e3), np.log(4e6))))
+        if i % 2 == 0:
+            c = datasets.terrain(n, seed=i, threads=4)
+        else:
+            c = datasets.shells(n, seed=i, count=max(1, min(2000, n // 1000)))
+        clouds.append(c)
+        opts.append(dict(quantizer=1, l2_bound=[0.001, 0.01, 0.1][i % 3]))
+    return clouds, opts
+
+
+def run_batched(args) -> int:
+    import numpy as np
+    import torch
+
+    import xnyzip_amd as X
+    from xnyzip_amd import _lib, datasets
+
+    w = dict(BATCHED[args.workload])
+    t_gen = time.perf_counter()
+    if w["kind"] == "chunked":
+        pts = datasets.shells(args.points or w["n"], seed=0)
+        n = len(pts)
+        offs = list(range(0, n, w["chunk"])) + [n]
+        opts = [dict(quantizer=w["quantizer"], l2_bound=w["l2"], is_hilbert=w["hilbert"],
+                     rle=w["rle"]) for _ in range(len(offs) - 1)]
+    elif w["kind"] == "uneven":
+        clouds, opts = _uneven_clouds(args.clouds or w["clouds"])
+        for o in opts:
+            o.update(is_hilbert=w["hilbert"], rle=w["rle"])
+        pts = np.concatenate(clouds)
+        offs = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).tolist()
+        del clouds
+    else:
+        half = (args.points or w["n"]) // 2
+        pts = np.concatenate([datasets.terrain(half, seed=0), datasets.shells(half, seed=1)])
+        rng = np.random.default_rng(0)
+        offs = [0]
+        while offs[-1] < len(pts):
+            offs.append(min(len(pts), offs[-1] + int(2.0 ** rng.uniform(16, 20))))
+        opts = [dict(quantizer=w["quantizer"], l2_bound=[0.001, 0.01, 0.1][i % 3])
+                for i in range(len(offs) - 1)]
+    n = len(pts)
+    print(f"[bench] workload {args.workload}: {n} points in {len(offs) - 1} clouds generated "
+          f"in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    dev = torch.from_numpy(pts).to("cuda:0")
+    torch.cuda.synchronize()
+    del pts
+    ctx = X.Context(0)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        for _ in range(steps):
+            out = fn()
+        return time.perf_counter() - t0, out
+
+    sink = OutBuffer(13 * n)
+    out = {"unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic",
+           "config": {"workload": args.workload, "points": n, "clouds": len(offs) - 1}}
+    if w["kind"] != "selector":
+        for mode, key in ((_lib.FRAME_ZSTD3, "frames"), (_lib.FRAME_NONE, "prezstd")):
+            o2 = [dict(o, frame_mode=mode) for o in opts]
+
+            def step():
+                res, st = sink.compress(X, dev, offs, o2, ctx)
+                return int(res[-1]), st
+            dt, (nbytes, _) = timed(step, args.steps, args.warmup)
+            gbps = 12.0 * n * args.steps / dt / 1e9
+            out[f"{key}_GBps"] = round(gbps, 3)
+            out[f"{key}_bytes"] = nbytes
+            out[f"{key}_ms_per_step"] = round(dt / args.steps * 1e3, 3)
+        out.update(metric="compress GB/s (fp32 xyz in), batched, zstd L3 frame per cloud",
+                   value=out["frames_GBps"], ms_per_step=out["frames_ms_per_step"],
+                   ratio=round(12.0 * n / out["frames_bytes"], 4))
+    else:
+        from xnyzip_amd import selector
+        params = selector.load_params()
+        res = {}
+        for label, p in (("plain_search", None), ("learned", params)):
+            for mode, key in ((_lib.FRAME_NONE, "prezstd"), (_lib.FRAME_ZSTD3, "frames")):
+                o2 = [dict(o, frame_mode=mode) for o in opts]
+
+                def step():
+                    fr, _, _, modes, searched = X.compress_selected_batch(
+                        dev, offs, o2, params=p, min_confidence=0.5, ctx=ctx)
+                    return sum(len(f) for f in fr), int(np.sum(searched))
+                dt, (nbytes, fell) = timed(step, args.steps, args.warmup)
+                res[f"{label}_{key}_GBps"] = round(12.0 * n * args.steps / dt / 1e9, 3)
+                res[f"{label}_{key}_bytes"] = nbytes
+                res[f"{label}_fell_back"] = fell
+        o2 = [dict(o, frame_mode=_lib.FRAME_NONE) for o in opts]
+        fr, _, _ = X.compress_batch(dev, offs, o2, ctx=ctx)
+        res["reference_mode_prezstd_bytes"] = sum(len(f) for f in fr)
+        res["learned_vs_search_bytes"] = round(
+            res["learned_prezstd_bytes"] / res["plain_search_prezstd_bytes"] - 1, 6)
+        res["speedup_prezstd"] = round(res["learned_prezstd_GBps"] /
+                                       res["plain_search_prezstd_GBps"], 3)
+        res["speedup_frames"] = round(res["learned_frames_GBps"] /
+                                      res["plain_search_frames_GBps"], 3)
+        out.update(metric="compress GB/s (fp32 xyz in), learned selector vs exhaustive search",
+                   value=res["learned_frames_GBps"], **res,
+                   ratio=round(12.0 * n / res["learned_frames_bytes"], 4))
     print(json.dumps(out))
     ctx.close()
     return 0

@@ -33,7 +33,8 @@ enum xnyz_status {
   XNYZ_EZSTD = -4,        /* libzstd error                                  */
   XNYZ_EUNSUPPORTED = -5, /* input outside the implemented range (see msg) */
   XNYZ_ECORRUPT = -6,     /* malformed compressed stream                    */
-  XNYZ_EINTERNAL = -7     /* device-side invariant violated                 */
+  XNYZ_EINTERNAL = -7,    /* device-side invariant violated                 */
+  XNYZ_ENOSPC = -8        /* caller's output buffer too small               */
 };
 
 /* QUANTIZER_TYPE (source/composer/utils.hpp:55-59); ADAPTIVE is not
@@ -166,6 +167,17 @@ int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t
                                uint32_t count, const xnyz_opts* opts, xnyz_buf* frames,
                                xnyz_buf* prezstd, xnyz_stats* stats);
 
+/* xnyz_compress_batch_device writing into caller memory instead of
+ * library-allocated buffers (no allocation or page faulting per call):
+ * cloud s's output, as opts[s].frame_mode selects, lands at out +
+ * out_offsets[s]; out_offsets (count + 1 entries) ends with the total.  If
+ * the total exceeds cap nothing is written, out_offsets[count] holds the
+ * size needed and the call returns XNYZ_ENOSPC.  No reference counterpart
+ * beyond compress(). */
+int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                                    uint32_t count, const xnyz_opts* opts, uint8_t* out,
+                                    uint64_t cap, uint64_t* out_offsets, xnyz_stats* stats);
+
 /* Exhaustive per-chunk mode search; no reference counterpart (the
  * reference codes every chunk with the options it is given).  For each cloud
  * of the batch (layout as xnyz_compress_batch_device; opts supply quantizer,
@@ -255,6 +267,12 @@ int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
  * streams have no restart points.  No device or context needed. */
 int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
                     xnyz_buf* out_points);
+
+/* Stage-2 launch sequences (key generation, radix passes, RLE cells, block
+ * runs, statistics: some 40 kernels) are captured into a hipGraph per launch
+ * shape and replayed while the shape repeats (same cloud sizes and options
+ * class); on by default, off while profiling.  No reference counterpart. */
+int xnyz_graphs_enable(xnyz_ctx* ctx, int on);
 
 /* Per-kernel device time (HIP events on the context's stream) accumulated
  * over the calls made while enabled; enabling clears the totals.  The report

@@ -98,3 +98,53 @@ def test_chunked_frames_decode(X):
     assert np.array_equal(O.decompress_frame(frame, O.TO, 0.01, True),
                           O.decompress_frame(ref.frame, O.TO, 0.01, True))
     _check_batch(X, _clouds()[:3], frame_mode=1)
+
+
+def test_batch_into_caller_buffer(X):
+    """The caller-memory entry point writes the same bytes as the allocating
+    one, back to back, and reports the size needed when the buffer is short."""
+    from xnyzip_amd import _lib
+    cases = _clouds()
+    pts = torch.from_numpy(np.concatenate([c for c, _ in cases])).cuda()
+    offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in cases])]).tolist()
+    for mode in (0, 1, 2):
+        opts = [dict(o, frame_mode=mode) for _, o in cases]
+        frames, _, _ = X.compress_batch(pts, offs, opts)
+        buf = np.zeros(sum(len(f) for f in frames) + 100, dtype=np.uint8)
+        res, stats = X.compress_batch_into(pts, offs, opts, buf)
+        for i, f in enumerate(frames):
+            assert bytes(buf[res[i]:res[i + 1]]) == f, (mode, i)
+            assert stats[i].frame_bytes == len(f)
+        small = np.zeros(10, dtype=np.uint8)
+        with pytest.raises(_lib.XnyzError) as e:
+            X.compress_batch_into(pts, offs, opts, small)
+        assert e.value.code == _lib.XNYZ_ENOSPC
+        assert "%d bytes needed" % res[-1] in str(e.value)
+
+
+def test_graph_replay_matches_eager(X):
+    """Stage 2 replayed from a captured hipGraph gives the eager bytes on
+    every replay, with other work (another launch shape, eager calls and
+    synchronisations) in between."""
+    ctx = X.Context(0)
+    cases = _clouds()
+    pts = torch.from_numpy(np.concatenate([c for c, _ in cases])).cuda()
+    offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in cases])]).tolist()
+    opts = [dict(o, frame_mode=2) for _, o in cases]
+    other = torch.from_numpy(_uniform(77, 50000)).cuda()
+    oopt = [dict(quantizer=O.TO, l2_bound=0.01, is_hilbert=True, rle=True, frame_mode=2)]
+    ctx.graphs(False)
+    eager, _, _ = X.compress_batch(pts, offs, opts, ctx=ctx)
+    eager_other, _, _ = X.compress_batch(other, [0, 50000], oopt, ctx=ctx)
+    ctx.graphs(True)
+    for rep in range(4):
+        got, _, _ = X.compress_batch(pts, offs, opts, ctx=ctx)
+        assert got == eager, rep
+        torch.cuda.synchronize()
+        got_o, _, _ = X.compress_batch(other, [0, 50000], oopt, ctx=ctx)  # recaptures
+        assert got_o == eager_other, rep
+        if rep == 1:
+            ctx.graphs(False)
+            assert X.compress_batch(pts, offs, opts, ctx=ctx)[0] == eager
+            ctx.graphs(True)
+    ctx.close()

@@ -23,7 +23,7 @@ from ._lib import Context, XnyzError  # noqa: F401
 
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
-           "compress", "compress_batch", "search_batch", "features_batch", "mlp_train",
+           "compress", "compress_batch", "compress_batch_into", "search_batch", "features_batch", "mlp_train",
            "mlp_predict", "compress_selected_batch", "quantize", "morton_to_hilbert"]
 
 
@@ -138,6 +138,41 @@ def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
     out_frames = [_lib.take(f) for f in frames]
     out_pres = [_lib.take(p) for p in pres] if pres is not None else None
     return out_frames, out_pres, list(stats)
+
+
+def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
+                        ctx: Context | None = None):
+    """compress_batch writing every cloud's output back to back into `out`
+    (a uint8 numpy array, reused across calls).  Returns (offsets of the
+    outputs, count + 1 entries, stats); raises XnyzError with code
+    XNYZ_ENOSPC if `out` is too small (offsets[-1] then holds the size
+    needed)."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    if out.dtype != np.uint8 or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint8 array")
+    count = len(offsets) - 1
+    if count <= 0 or len(opts) != count:
+        raise ValueError("need one opts entry per cloud")
+    ctx = ctx or default_context(t.device.index or 0)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    arr = (_lib.Opts * count)()
+    for i, o in enumerate(opts):
+        arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
+                       o.get("rle", False), o.get("direct_threshold", 1024),
+                       o.get("frame_mode", _lib.FRAME_ZSTD3), o.get("zstd_workers", 0),
+                       o.get("gr_k"), o.get("pfd_width"))
+    res = np.zeros(count + 1, dtype=np.uint64)
+    stats = (_lib.Stats * count)()
+    torch.cuda.synchronize(t.device)
+    rc = _lib.lib().xnyz_compress_batch_device_into(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(out.ctypes.data), out.size,
+        C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p))
+    _lib.check(rc)
+    return res, list(stats)
 
 
 def search_batch(points_device, offsets, opts, ctx: Context | None = None):

@@ -13,6 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libxnyz.so")
 
 XNYZ_OK = 0
+XNYZ_ENOSPC = -8
 CUBE, TRUNCATED_OCTAHEDRON = 0, 1
 FRAME_ZSTD3, FRAME_ZSTD3_MT, FRAME_NONE = 0, 1, 2
 
@@ -82,10 +83,11 @@ _lock = threading.Lock()
 
 EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress", "xnyz_compress_device", "xnyz_compress_batch_device",
+           "xnyz_compress_batch_device_into",
            "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_quantize_device",
-           "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_profile_enable",
+           "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_graphs_enable", "xnyz_profile_enable",
            "xnyz_profile_report", "xnyz_free"]
 
 
@@ -107,6 +109,8 @@ def lib() -> C.CDLL:
         L.xnyz_compress.argtypes = comp
         L.xnyz_compress_device.argtypes = comp
         L.xnyz_compress_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, P, P]
+        L.xnyz_compress_batch_device_into.argtypes = [P, P, P, C.c_uint32, P, P, C.c_uint64, P,
+                                                      P]
         L.xnyz_search_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
         L.xnyz_features_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
         L.xnyz_mlp_train.argtypes = [P, P, P, C.c_uint64, C.POINTER(TrainOpts), P,
@@ -118,6 +122,7 @@ def lib() -> C.CDLL:
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]
         L.xnyz_profile_enable.argtypes = [P, C.c_int]
+        L.xnyz_graphs_enable.argtypes = [P, C.c_int]
         L.xnyz_profile_report.argtypes = [P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.xnyz_free.argtypes = [C.POINTER(Buf)]
         _lib = L
@@ -159,6 +164,10 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    def graphs(self, on: bool) -> None:
+        """Replay the stage-2 launch sequence from a captured hipGraph (default on)."""
+        check(lib().xnyz_graphs_enable(self._h, int(on)))
 
     def profile(self, on: bool) -> None:
         """Enable (and clear) or disable per-kernel event timing."""

@@ -202,6 +202,61 @@ struct EventProfiler : Profiler {
 
 }  // namespace
 
+struct FrameState;
+
+// Bytes that determine a launch sequence.
+struct GraphKey {
+  std::vector<uint8_t> b;
+  template <typename T>
+  GraphKey& add(const T& v) {
+    const auto* p = reinterpret_cast<const uint8_t*>(&v);
+    b.insert(b.end(), p, p + sizeof(T));
+    return *this;
+  }
+  bool operator==(const GraphKey& o) const { return b == o.b; }
+};
+
+// One captured launch sequence, replayed while its key repeats and recaptured
+// when it changes.  `cur` / `perm` are host-side results of the enqueueing
+// code, stored with the graph.
+struct CachedGraph {
+  GraphKey key;
+  hipGraphExec_t exec = nullptr;
+  int cur = 0;
+  const uint32_t* perm = nullptr;
+  void clear() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.b.clear();
+  }
+  template <typename F>
+  void run(const GraphKey& k, hipStream_t st, F&& enqueue, int& cur_out, const uint32_t*& perm_out) {
+    if (!exec || !(key == k)) {
+      clear();
+      xnyz::check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "capture");
+      try {
+        enqueue();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hipGraph_t g = nullptr;
+      xnyz::check_hip(hipStreamEndCapture(st, &g), "end capture");
+      const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      xnyz::check_hip(e, "graph instantiate");
+      key = k;
+      cur = cur_out;
+      perm = perm_out;
+    }
+    xnyz::check_hip(hipGraphLaunch(exec, st), "graph launch");
+    cur_out = cur;
+    perm_out = perm;
+  }
+};
+
 struct xnyz_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -217,6 +272,9 @@ struct xnyz_ctx {
   HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat;
   bool prof_on = false;
   EventProfiler prof;
+  FrameState* fs = nullptr;  // frame stage (created on first use)
+  bool graphs = true;        // replay stage 2 from a captured graph
+  CachedGraph stage2;
 
   void sync() {
     check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -266,16 +324,85 @@ uint8_t* xmalloc(size_t n) {
   return p;
 }
 
-void zstd_single(const uint8_t* src, u64 len, xnyz_buf* out) {
-  const size_t bound = ZSTD_compressBound(len);
-  uint8_t* dst = xmalloc(bound);
-  const size_t r = ZSTD_compress(dst, bound, src, len, 3);
-  if (ZSTD_isError(r)) {
-    std::free(dst);
-    throw Error(XNYZ_EZSTD, ZSTD_getErrorName(r));
+// One output as byte pieces in order (a frame header and the chunks of a
+// host-threaded frame, or a single piece).
+struct Piece {
+  const uint8_t* p;
+  u64 n;
+};
+using Pieces = std::vector<Piece>;
+
+u64 pieces_size(const Pieces& v) {
+  u64 t = 0;
+  for (const Piece& x : v) t += x.n;
+  return t;
+}
+
+// Runs f(i, worker) for i in [0, n) on up to `threads` host threads.
+void parallel_for_w(size_t n, int threads, const std::function<void(size_t, size_t)>& f) {
+  if (n == 0) return;
+  const size_t t = std::min<size_t>(n, (size_t)std::max(1, threads));
+  if (t == 1) {
+    for (size_t i = 0; i < n; ++i) f(i, 0);
+    return;
   }
-  out->data = dst;
-  out->len = r;
+  std::atomic<size_t> next{0};
+  std::vector<std::exception_ptr> errs(t);
+  std::vector<std::thread> pool;
+  for (size_t w = 0; w < t; ++w)
+    pool.emplace_back([&, w] {
+      try {
+        for (size_t i; (i = next.fetch_add(1)) < n;) f(i, w);
+      } catch (...) {
+        errs[w] = std::current_exception();
+      }
+    });
+  for (auto& th : pool) th.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
+}
+
+constexpr size_t kZChunk = 1u << 20;  // host-threaded frames: bytes per chunk
+
+size_t zchunk_bound() { return ZSTD_compressBound(kZChunk) + 3 * (kZChunk / 1024 + 2); }
+
+// Scratch bytes the frame of `len` bytes needs in `mode` with `threads`.
+u64 frame_scratch(u64 len, int mode, int threads) {
+  if (mode == XNYZ_FRAME_NONE) return 0;
+  const u64 nchunks = (len + kZChunk - 1) / kZChunk;
+  if (mode == XNYZ_FRAME_ZSTD3_MT && nchunks > 1 && threads > 1) return 16 + nchunks * zchunk_bound();
+  return ZSTD_compressBound(len);
+}
+
+}  // namespace
+
+// Frame-stage state kept by a context across calls: one libzstd compression
+// context per host worker and the scratch the compressed bytes land in.
+struct FrameState {
+  std::vector<uint8_t> buf;
+  std::vector<ZSTD_CCtx*> cctx;
+  ZSTD_CCtx* get(size_t w) {
+    if (cctx.size() <= w) cctx.resize(w + 1, nullptr);
+    if (!cctx[w] && !(cctx[w] = ZSTD_createCCtx()))
+      throw xnyz::Error(XNYZ_ENOMEM, "ZSTD_createCCtx failed");
+    return cctx[w];
+  }
+  void ensure_workers(size_t n) {
+    for (size_t w = 0; w < n; ++w) get(w);
+  }
+  ~FrameState() {
+    for (auto c : cctx) ZSTD_freeCCtx(c);
+  }
+};
+
+namespace {
+
+// source/composer/utils.hpp:35-53: ZSTD_compress at level 3 (byte-identical:
+// ZSTD_compress is ZSTD_compressCCtx on a fresh context).
+Pieces zstd_single(ZSTD_CCtx* cc, const uint8_t* src, u64 len, uint8_t* dst, u64 cap) {
+  const size_t r = ZSTD_compressCCtx(cc, dst, cap, src, len, 3);
+  if (ZSTD_isError(r)) throw Error(XNYZ_EZSTD, ZSTD_getErrorName(r));
+  return {Piece{dst, r}};
 }
 
 // One zstd frame, level 3, whose 1 MiB chunks are compressed on host threads
@@ -284,89 +411,70 @@ void zstd_single(const uint8_t* src, u64 len, xnyz_buf* out) {
 // Chunks after the first start without repeat offsets, so every chunk
 // decodes after any other.  Valid for every zstd decoder; not byte-identical
 // to the single-shot frame.
-void zstd_chunked(const uint8_t* src, u64 len, int threads, xnyz_buf* out) {
-  constexpr size_t kChunk = 1u << 20;
-  const size_t nchunks = (len + kChunk - 1) / kChunk;
-  if (nchunks <= 1 || threads <= 1) return zstd_single(src, len, out);
-  std::vector<std::vector<uint8_t>> parts(nchunks);
+// `own` compresses the frame when it is not split (one chunk or one thread).
+Pieces zstd_chunked(FrameState& fs, ZSTD_CCtx* own, const uint8_t* src, u64 len, int threads,
+                    uint8_t* dst) {
+  const size_t nchunks = (len + kZChunk - 1) / kZChunk;
+  if (nchunks <= 1 || threads <= 1) return zstd_single(own, src, len, dst, ZSTD_compressBound(len));
   const int t = std::min<int>(threads, (int)nchunks);
-  std::vector<ZSTD_CCtx*> cctx(t, nullptr);
-  for (auto& c : cctx)
-    if (!(c = ZSTD_createCCtx())) throw Error(XNYZ_ENOMEM, "ZSTD_createCCtx failed");
-  std::atomic<size_t> next{0};
-  std::vector<std::string> err(t);
-  std::vector<std::thread> pool;
-  for (int w = 0; w < t; ++w)
-    pool.emplace_back([&, w] {
-      ZSTD_CCtx* cc = cctx[w];
-      for (size_t c; (c = next.fetch_add(1)) < nchunks;) {
-        const size_t off = c * kChunk, n = std::min<size_t>(kChunk, len - off);
-        size_t r = ZSTD_compressBegin(cc, 3);
-        if (ZSTD_isError(r)) {
-          err[w] = ZSTD_getErrorName(r);
-          return;
-        }
-        if (c) ZSTD_invalidateRepCodes(cc);
-        const size_t bs = ZSTD_getBlockSize(cc);
-        std::vector<uint8_t>& o = parts[c];
-        o.resize(ZSTD_compressBound(n) + 3 * (n / bs + 2));
-        size_t at = 0;
-        for (size_t b = 0; b < n; b += bs) {
-          const size_t m = std::min(bs, n - b);
-          const bool last = (c + 1 == nchunks) && (b + m == n);
-          r = ZSTD_compressBlock(cc, o.data() + at + 3, o.size() - at - 3, src + off + b, m);
-          if (ZSTD_isError(r)) {
-            err[w] = ZSTD_getErrorName(r);
-            return;
-          }
-          u32 type = 2, size = (u32)r;
-          if (r == 0 || r >= m) {  // stored as a raw block
-            type = 0;
-            size = (u32)m;
-            std::memcpy(o.data() + at + 3, src + off + b, m);
-          }
-          const u32 hdr = (last ? 1u : 0u) | (type << 1) | (size << 3);
-          o[at] = uint8_t(hdr);
-          o[at + 1] = uint8_t(hdr >> 8);
-          o[at + 2] = uint8_t(hdr >> 16);
-          at += 3 + size;
-        }
-        o.resize(at);
-      }
-    });
-  for (auto& th : pool) th.join();
-  for (auto c : cctx) ZSTD_freeCCtx(c);
-  for (auto& e : err)
-    if (!e.empty()) throw Error(XNYZ_EZSTD, e);
-  // magic, frame header descriptor (8-byte content size, single segment),
-  // content size
-  size_t total = 13;
-  for (auto& p : parts) total += p.size();
-  uint8_t* dst = xmalloc(total);
+  fs.ensure_workers(t);  // contexts are created before the workers start
+  const size_t cb = zchunk_bound();
+  uint8_t* hdr = dst;  // magic, descriptor (single segment, 8-byte content size), size
   const uint32_t magic = 0xFD2FB528u;
-  std::memcpy(dst, &magic, 4);
-  dst[4] = 0xE0;
+  std::memcpy(hdr, &magic, 4);
+  hdr[4] = 0xE0;
   const uint64_t fcs = len;
-  std::memcpy(dst + 5, &fcs, 8);
-  size_t at = 13;
-  for (auto& p : parts) {
-    std::memcpy(dst + at, p.data(), p.size());
-    at += p.size();
-  }
-  out->data = dst;
-  out->len = total;
+  std::memcpy(hdr + 5, &fcs, 8);
+  Pieces out(nchunks + 1);
+  out[0] = Piece{hdr, 13};
+  uint8_t* parts = dst + 16;
+  parallel_for_w(nchunks, t, [&](size_t c, size_t w) {
+    ZSTD_CCtx* cc = fs.cctx[w];
+    const size_t off = c * kZChunk, n = std::min<size_t>(kZChunk, len - off);
+    uint8_t* o = parts + c * cb;
+    size_t r = ZSTD_compressBegin(cc, 3);
+    if (ZSTD_isError(r)) throw Error(XNYZ_EZSTD, ZSTD_getErrorName(r));
+    if (c) ZSTD_invalidateRepCodes(cc);
+    const size_t bs = ZSTD_getBlockSize(cc);
+    size_t at = 0;
+    for (size_t b = 0; b < n; b += bs) {
+      const size_t m = std::min(bs, n - b);
+      const bool last = (c + 1 == nchunks) && (b + m == n);
+      r = ZSTD_compressBlock(cc, o + at + 3, cb - at - 3, src + off + b, m);
+      if (ZSTD_isError(r)) throw Error(XNYZ_EZSTD, ZSTD_getErrorName(r));
+      u32 type = 2, size = (u32)r;
+      if (r == 0 || r >= m) {  // stored as a raw block
+        type = 0;
+        size = (u32)m;
+        std::memcpy(o + at + 3, src + off + b, m);
+      }
+      const u32 h = (last ? 1u : 0u) | (type << 1) | (size << 3);
+      o[at] = uint8_t(h);
+      o[at + 1] = uint8_t(h >> 8);
+      o[at + 2] = uint8_t(h >> 16);
+      at += 3 + size;
+    }
+    out[c + 1] = Piece{o, at};
+  });
+  return out;
 }
 
-void make_frame(const uint8_t* src, u64 len, const xnyz_opts& o, int threads, xnyz_buf* out) {
-  if (o.frame_mode == XNYZ_FRAME_NONE) {
-    out->data = xmalloc(len);
-    std::memcpy(out->data, src, len);
-    out->len = len;
-  } else if (o.frame_mode == XNYZ_FRAME_ZSTD3_MT) {
-    zstd_chunked(src, len, host_threads(o.zstd_workers > 0 ? o.zstd_workers : threads), out);
-  } else {
-    zstd_single(src, len, out);
+// Copies pieces to dst, splitting large copies over host threads.
+void copy_pieces(const Pieces& v, uint8_t* dst, int threads) {
+  constexpr u64 kSlice = 8u << 20;
+  std::vector<std::pair<u64, u64>> jobs;  // (piece, offset in piece)
+  std::vector<u64> at(v.size());
+  u64 t = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
+    at[i] = t;
+    t += v[i].n;
+    for (u64 o = 0; o < v[i].n; o += kSlice) jobs.emplace_back(i, o);
   }
+  parallel_for(jobs.size(), t >= (32u << 20) ? threads : 1, [&](size_t j) {
+    const Piece& x = v[jobs[j].first];
+    const u64 o = jobs[j].second, n = std::min(kSlice, x.n - o);
+    std::memcpy(dst + at[jobs[j].first] + o, x.p + o, n);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -493,6 +601,11 @@ struct CallOut {
   // Selector features: stop after the range scan and return kFeat floats
   // per segment.
   std::vector<float>* features = nullptr;
+  // Outputs into caller memory instead of `frames`: segment s at
+  // dst + dst_off[s], dst_off[S] = total (also set when cap is too small).
+  uint8_t* dst = nullptr;
+  u64 cap = 0;
+  uint64_t* dst_off = nullptr;
 };
 
 // HyperLogLog estimate from 256 registers (small-range correction by linear
@@ -600,7 +713,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->h2d(ctx->tiles, ctx->h_tiles, sizeof(Tile) * ntiles);
   ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
-  Batch bt{};
+  Batch bt;
+  std::memset(&bt, 0, sizeof(bt));  // padding included: bt is part of a graph key
   bt.S = S;
   bt.ntiles = ntiles;
   bt.N = N;
@@ -688,10 +802,21 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
   // ---- stage 2: keys, sort, cells, runs, statistics ---------------------------
+  // Every buffer is sized before the launches, which may be captured.
   for (int i = 0; i < 2; ++i) ctx->keys[i].ensure((size_t)N * 8);
   if (wide)
     for (int i = 0; i < 2; ++i) ctx->lo[i].ensure((size_t)N * 4);
   ctx->pos.ensure((size_t)N * 4 + 16);
+  if (want_perm) {
+    for (int i = 0; i < 2; ++i) ctx->idx[i].ensure((size_t)N * 4);
+    if (wide)
+      for (int i = 0; i < 2; ++i) ctx->lo64[i].ensure((size_t)N * 8);
+  }
+  if (any_rle) {
+    ctx->cells.ensure((size_t)N * 8);
+    if (wide) ctx->cells_lo.ensure((size_t)N * 4);
+    ctx->cnt.ensure((size_t)N * 4 + 16);
+  }
   u64* ka = ctx->keys[0].as<u64>();
   u64* kb = ctx->keys[1].as<u64>();
   u32* la = wide ? ctx->lo[0].as<u32>() : nullptr;
@@ -702,27 +827,45 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   u32 *ia = nullptr, *ib = nullptr;
   u64 *l64a = nullptr, *l64b = nullptr;
   if (want_perm) {
-    for (int i = 0; i < 2; ++i) ctx->idx[i].ensure((size_t)N * 4);
     ia = ctx->idx[0].as<u32>();
     ib = ctx->idx[1].as<u32>();
     if (wide) {
-      for (int i = 0; i < 2; ++i) ctx->lo64[i].ensure((size_t)N * 8);
       l64a = ctx->lo64[0].as<u64>();
       l64b = ctx->lo64[1].as<u64>();
     }
   }
-  launch_keygen(bt, d_xyz, ctx->hs, ka, l64a ? nullptr : la, ia, l64a, st);
-  int cur;
+  u32* cnt = any_rle ? ctx->cnt.as<u32>() : nullptr;
+  int cur = 0;
   const u32* perm_dev = nullptr;
-  if (want_perm && wide) {
-    cur = radix_sort_wide_v(bt, ka, kb, l64a, l64b, hi_bits, ar, st);
-    launch_split_lo64(cur ? l64b : l64a, ctx->lo[cur].as<u32>(), ia, N, st);
-    perm_dev = ia;
-  } else if (wide) {
-    cur = radix_sort_wide(bt, ka, kb, la, lb, hi_bits, ar, st);
+  auto stage2 = [&] {
+    launch_keygen(bt, d_xyz, ctx->hs, ka, l64a ? nullptr : la, ia, l64a, st);
+    if (want_perm && wide) {
+      cur = radix_sort_wide_v(bt, ka, kb, l64a, l64b, hi_bits, ar, st);
+      launch_split_lo64(cur ? l64b : l64a, ctx->lo[cur].as<u32>(), ia, N, st);
+      perm_dev = ia;
+    } else if (wide) {
+      cur = radix_sort_wide(bt, ka, kb, la, lb, hi_bits, ar, st);
+    } else {
+      cur = radix_sort(bt, ka, kb, ia, ib, narrow_bits, ar, st);
+      perm_dev = want_perm ? (cur ? ib : ia) : nullptr;
+    }
+    KeyView k{ctx->keys[cur].as<u64>(), wide ? ctx->lo[cur].as<u32>() : nullptr};
+    if (any_rle)
+      rle_cells(bt, k, ctx->pos.as<u32>(), ctx->cells.as<u64>(),
+                wide ? ctx->cells_lo.as<u32>() : nullptr, cnt, ar, st);
+    if (any_rle) k = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
+    block_runs_and_stats(bt, k, any_direct, any_block, ctx->pos.as<u32>(), ar, st);
+  };
+  if (ctx->graphs && !g_profiler) {
+    // The launch sequence is a function of this signature alone (kernel
+    // parameters are device-resident), so an instantiated graph replays it.
+    GraphKey key;
+    key.add(d_xyz).add(bt).add(ka).add(kb).add(la).add(lb).add(ia).add(ib).add(l64a).add(l64b)
+        .add(cnt).add(ctx->pos.p).add(ctx->cells.p).add(ctx->cells_lo.p).add(ar.base).add(ar.off)
+        .add(narrow_bits).add(hi_bits).add(any_rle).add(any_block).add(any_direct).add(want_perm);
+    ctx->stage2.run(key, st, stage2, cur, perm_dev);
   } else {
-    cur = radix_sort(bt, ka, kb, ia, ib, narrow_bits, ar, st);
-    perm_dev = want_perm ? (cur ? ib : ia) : nullptr;
+    stage2();
   }
   if (perm_dev) {
     check_hip(hipMemcpyAsync(co.perm, perm_dev, (size_t)N * 4, hipMemcpyDeviceToHost, st),
@@ -730,17 +873,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     ctx->copied();
   }
   KeyView kv{ctx->keys[cur].as<u64>(), wide ? ctx->lo[cur].as<u32>() : nullptr};
-  u32* cnt = nullptr;
-  if (any_rle) {
-    ctx->cells.ensure((size_t)N * 8);
-    if (wide) ctx->cells_lo.ensure((size_t)N * 4);
-    ctx->cnt.ensure((size_t)N * 4 + 16);
-    cnt = ctx->cnt.as<u32>();
-    rle_cells(bt, kv, ctx->pos.as<u32>(), ctx->cells.as<u64>(),
-              wide ? ctx->cells_lo.as<u32>() : nullptr, cnt, ar, st);
-    kv = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
-  }
-  block_runs_and_stats(bt, kv, any_direct, any_block, ctx->pos.as<u32>(), ar, st);
+  if (any_rle) kv = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
   ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
   ctx->d2h(ctx->h_sh, ctx->sh, sizeof(SegHist) * S);
   u32 n_large = 0;
@@ -925,16 +1058,56 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   });
   const double t_dev = now_ms();
   const bool per_seg_threads = S > 1;
-  parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
-    SegHost& h = g[s];
-    const uint8_t* b = hb + h.base;
-    if (co.pre) {
-      co.pre[s].data = xmalloc(h.total);
-      std::memcpy(co.pre[s].data, b, h.total);
-      co.pre[s].len = h.total;
+  if (co.pre) {
+    parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
+      co.pre[s].data = xmalloc(g[s].total);
+      std::memcpy(co.pre[s].data, hb + g[s].base, g[s].total);
+      co.pre[s].len = g[s].total;
+    });
+  }
+  if (co.frames || co.dst) {
+    if (!ctx->fs) ctx->fs = new FrameState();
+    FrameState& fs = *ctx->fs;
+    std::vector<u64> zoff(S + 1, 0);
+    std::vector<int> zthreads(S);
+    for (u32 s = 0; s < S; ++s) {
+      const xnyz_opts& o = opts[s];
+      zthreads[s] = per_seg_threads ? 1 : host_threads(o.zstd_workers > 0 ? o.zstd_workers : threads);
+      zoff[s + 1] = zoff[s] + ((frame_scratch(g[s].total, o.frame_mode, zthreads[s]) + 63) & ~u64(63));
     }
-    if (co.frames) make_frame(b, h.total, opts[s], per_seg_threads ? 1 : threads, &co.frames[s]);
-  });
+    if (fs.buf.size() < zoff[S]) fs.buf.resize(zoff[S]);
+    // Every worker's context exists before the workers start (no resizing
+    // while they read the table).
+    fs.ensure_workers(per_seg_threads ? (size_t)std::min<u64>(S, (u64)threads) : 1);
+    std::vector<Pieces> outs(S);
+    parallel_for_w(S, per_seg_threads ? threads : 1, [&](size_t s, size_t w) {
+      const uint8_t* b = hb + g[s].base;
+      const u64 len = g[s].total;
+      uint8_t* z = fs.buf.data() + zoff[s];
+      switch (opts[s].frame_mode) {
+        case XNYZ_FRAME_NONE: outs[s] = {Piece{b, len}}; break;
+        case XNYZ_FRAME_ZSTD3_MT: outs[s] = zstd_chunked(fs, fs.cctx[w], b, len, zthreads[s], z); break;
+        default: outs[s] = zstd_single(fs.cctx[w], b, len, z, zoff[s + 1] - zoff[s]);
+      }
+    });
+    if (co.frames) {
+      parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
+        const u64 n = pieces_size(outs[s]);
+        co.frames[s].data = xmalloc(n);
+        co.frames[s].len = n;
+        copy_pieces(outs[s], co.frames[s].data, per_seg_threads ? 1 : threads);
+      });
+    } else {
+      co.dst_off[0] = 0;
+      for (u32 s = 0; s < S; ++s) co.dst_off[s + 1] = co.dst_off[s] + pieces_size(outs[s]);
+      if (co.dst_off[S] > co.cap)
+        throw Error(XNYZ_ENOSPC, "output buffer too small: " + std::to_string(co.dst_off[S]) +
+                                     " bytes needed");
+      parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
+        copy_pieces(outs[s], co.dst + co.dst_off[s], per_seg_threads ? 1 : threads);
+      });
+    }
+  }
   const double t_end = now_ms();
   if (co.stats) {
     for (u32 s = 0; s < S; ++s) {
@@ -958,7 +1131,9 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       x.max_l2 = h.max_l2;
       x.mse = h.mse;
       x.prezstd_bytes = h.total;
-      x.frame_bytes = co.frames ? co.frames[s].len : 0;
+      x.frame_bytes = co.frames ? co.frames[s].len
+                     : co.dst   ? co.dst_off[s + 1] - co.dst_off[s]
+                                : 0;
       x.ms_device = t_dev - t_start;
       x.ms_zstd = t_end - t_dev;
       x.ms_total = t_end - t_start;
@@ -1090,6 +1265,8 @@ int xnyz_destroy(xnyz_ctx* ctx) {
                     &ctx->h_feat})
       b->release();
     ctx->prof.destroy();
+    ctx->stage2.clear();
+    delete ctx->fs;
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return XNYZ_OK;
@@ -1140,6 +1317,22 @@ int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t
     use_device(ctx);
     return compress_impl(ctx, d_xyz, off, opts, CallOut{frames, prezstd, stats, nullptr},
                          now_ms());
+  });
+}
+
+int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
+                                    uint32_t count, const xnyz_opts* opts, uint8_t* out,
+                                    uint64_t cap, uint64_t* out_offsets, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !out_offsets || (!out && cap))
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    use_device(ctx);
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
   });
 }
 
@@ -1450,6 +1643,18 @@ int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n, uint32
 int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
                     xnyz_buf* out_points) {
   return guarded([&] { return decompress_impl(in, len, opts, out_points); });
+}
+
+int xnyz_graphs_enable(xnyz_ctx* ctx, int on) {
+  return guarded([&] {
+    if (!ctx) throw Error(XNYZ_EINVAL, "null context");
+    ctx->graphs = on != 0;
+    if (!ctx->graphs) {
+      use_device(ctx);
+      ctx->stage2.clear();
+    }
+    return XNYZ_OK;
+  });
 }
 
 int xnyz_profile_enable(xnyz_ctx* ctx, int on) {
