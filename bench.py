@@ -185,7 +185,10 @@ def main() -> int:
         k2 = max(1, min(3, args.steps))
         dt_e, (flen_e, _) = timed(_lib.FRAME_ZSTD3, k2, 0)
         dt_p, (plen, st_p) = timed(_lib.FRAME_NONE, k2, 1)
-        extra = {"exact_zstd_GBps": 12.0 * n * k2 / dt_e / 1e9,
+        dt_g, (glen, _) = timed(_lib.FRAME_ZSTD_GPU, k2, 1)
+        extra = {"gpu_frame_GBps": 12.0 * n * k2 / dt_g / 1e9, "gpu_frame_bytes": glen,
+                 "gpu_frame_ratio": 12.0 * n / glen,
+                 "exact_zstd_GBps": 12.0 * n * k2 / dt_e / 1e9,
                  "prezstd_GBps": 12.0 * n * k2 / dt_p / 1e9,
                  "device_ms": st_p.ms_device, "exact_frame_bytes": flen_e}
 

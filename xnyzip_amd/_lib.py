@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libxnyz.so")
 XNYZ_OK = 0
 XNYZ_ENOSPC = -8
 CUBE, TRUNCATED_OCTAHEDRON = 0, 1
-FRAME_ZSTD3, FRAME_ZSTD3_MT, FRAME_NONE = 0, 1, 2
+FRAME_ZSTD3, FRAME_ZSTD3_MT, FRAME_NONE, FRAME_ZSTD_GPU = 0, 1, 2, 3
 
 
 class XnyzError(RuntimeError):

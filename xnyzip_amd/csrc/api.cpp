@@ -38,6 +38,8 @@
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "modes.hpp"
+#include "zframe.hpp"
+#include "zframe_dev.hpp"
 
 // Exported by libzstd 1.4 (zstd_compress.c) though absent from its public
 // header: makes the next block start without repeat offsets, so a chunk
@@ -266,10 +268,12 @@ struct xnyz_ctx {
   DBuf keys[2], lo[2], cells, cells_lo, pos, cnt, out, arena, upload, tmp;
   DBuf idx[2], lo64[2];  // permutation payload
   DBuf feat;             // selector statistics
+  DBuf patch;            // stream header / meta bytes for k_patch
+  DBuf ztab, zscr, zout, zpos;  // device-built frames
   // selector: parameters, FP16 weights, optimiser state, samples
   DBuf mP, mWh, mG, mM, mV, mX, mY, mIdx, mScratch, mLoss, mOut, mConf;
   // pinned host mirrors
-  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat;
+  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat, h_zout;
   bool prof_on = false;
   EventProfiler prof;
   FrameState* fs = nullptr;  // frame stage (created on first use)
@@ -368,7 +372,7 @@ size_t zchunk_bound() { return ZSTD_compressBound(kZChunk) + 3 * (kZChunk / 1024
 
 // Scratch bytes the frame of `len` bytes needs in `mode` with `threads`.
 u64 frame_scratch(u64 len, int mode, int threads) {
-  if (mode == XNYZ_FRAME_NONE) return 0;
+  if (mode == XNYZ_FRAME_NONE || mode == XNYZ_FRAME_ZSTD_GPU) return 0;
   const u64 nchunks = (len + kZChunk - 1) / kZChunk;
   if (mode == XNYZ_FRAME_ZSTD3_MT && nchunks > 1 && threads > 1) return 16 + nchunks * zchunk_bound();
   return ZSTD_compressBound(len);
@@ -486,7 +490,7 @@ void validate(const xnyz_opts* o, u64 n) {
     throw Error(XNYZ_EINVAL, "quantizer must be XNYZ_CUBE or XNYZ_TRUNCATED_OCTAHEDRON");
   if (!(o->l2_bound > 0.0f) || !std::isfinite(o->l2_bound))
     throw Error(XNYZ_EINVAL, "l2_bound must be a positive finite float");
-  if (o->frame_mode < XNYZ_FRAME_ZSTD3 || o->frame_mode > XNYZ_FRAME_NONE)
+  if (o->frame_mode < XNYZ_FRAME_ZSTD3 || o->frame_mode > XNYZ_FRAME_ZSTD_GPU)
     throw Error(XNYZ_EINVAL, "bad frame_mode");
   if (o->mode_flags & ~(XNYZ_MODE_GR_K | XNYZ_MODE_PFD_WIDTH))
     throw Error(XNYZ_EINVAL, "unknown mode_flags bits");
@@ -1020,42 +1024,108 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   }
   ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
-  // ---- stage 4: write every section in place ------------------------------------
-  ctx->out.ensure(out_total + 64);
-  u32* ow = ctx->out.as<u32>();
-  launch_fill_u32(ow, (out_total + 64) / 4, 0, st);
-  write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, st);
-  ctx->h_out.ensure(out_total + 64);
-  uint8_t* hb = ctx->h_out.as<uint8_t>();
-  check_hip(hipMemcpyAsync(hb, ow, out_total, hipMemcpyDeviceToHost, st), "stream D2H");
-  ctx->sync();
-
-  // ---- host: header, size fields, metas, trailer; frames ----------------------
-  // header (block_compressor.hpp:61-87): u64 points, u64 params.size() = 0,
+  // ---- stage 4: the whole stream in device memory -----------------------------
+  // Header (block_compressor.hpp:61-87): u64 points, u64 params.size() = 0,
   // i32 ranges, bool curve, bool direct; per section u64 data size, u64 meta
-  // size, meta; trailer: the float offset.
-  parallel_for(S, threads, [&](size_t s) {
-    SegHost& h = g[s];
-    uint8_t* b = hb + h.base;
+  // size, meta; trailer: the float offset.  These bytes are copied in by
+  // k_patch before the sections are ORed in around them.
+  std::vector<uint8_t> pbytes;
+  std::vector<Patch> patches;
+  auto put = [&](u64 dst, const void* src, size_t len) {
+    if (!len) return;
+    patches.push_back(Patch{dst, (u32)pbytes.size(), (u32)len});
+    const auto* q = static_cast<const uint8_t*>(src);
+    pbytes.insert(pbytes.end(), q, q + len);
+  };
+  for (u32 s = 0; s < S; ++s) {
+    const SegHost& h = g[s];
+    uint8_t head[30];
     const u64 np = h.ncoded, zero = 0;
-    std::memcpy(b, &np, 8);
-    std::memcpy(b + 8, &zero, 8);
+    std::memcpy(head, &np, 8);
+    std::memcpy(head + 8, &zero, 8);
     const int32_t rr[3] = {h.rx, h.ry, h.rz};
-    std::memcpy(b + 16, rr, 12);
-    b[28] = h.hil ? 1 : 0;
-    b[29] = h.direct ? 1 : 0;
-    u64 at = 30;
+    std::memcpy(head + 16, rr, 12);
+    head[28] = h.hil ? 1 : 0;
+    head[29] = h.direct ? 1 : 0;
+    put(h.base, head, 30);
+    u64 at = h.base + 30;
     size_t i = 0;
     for (const Rec& r : records(h, sc[s])) {
       const std::vector<uint8_t>& m = h.metas[i++];
-      const u64 ds = r.data_bytes, ms = m.size();
-      std::memcpy(b + at, &ds, 8);
-      std::memcpy(b + at + 8, &ms, 8);
-      if (ms) std::memcpy(b + at + 16, m.data(), ms);
-      at += 16 + ms + ds;
+      const u64 sizes[2] = {r.data_bytes, (u64)m.size()};
+      put(at, sizes, 16);
+      put(at + 16, m.data(), m.size());
+      at += 16 + m.size() + r.data_bytes;
     }
-    std::memcpy(b + at, h.off, 12);
-  });
+    put(at, h.off, 12);
+  }
+  ctx->out.ensure(out_total + 64);
+  u32* ow = ctx->out.as<u32>();
+  launch_fill_u32(ow, (out_total + 64) / 4, 0, st);
+  const size_t prec = patches.size() * sizeof(Patch);
+  ctx->patch.ensure(prec + pbytes.size() + 16);
+  check_hip(hipMemcpyAsync(ctx->patch.p, patches.data(), prec, hipMemcpyHostToDevice, st), "H2D");
+  check_hip(hipMemcpyAsync(ctx->patch.as<uint8_t>() + prec, pbytes.data(), pbytes.size(),
+                           hipMemcpyHostToDevice, st), "H2D");
+  ctx->copied();
+  launch_patch(ctx->patch.as<Patch>(), (u32)patches.size(), ctx->patch.as<uint8_t>() + prec,
+               reinterpret_cast<uint8_t*>(ow), st);
+  write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, st);
+
+  // ---- frames built on the device (XNYZ_FRAME_ZSTD_GPU) ----------------------
+  const bool want_out = co.frames || co.dst;
+  bool host_pre = co.pre != nullptr;
+  std::vector<u32> gsegs;  // segments framed on the device, in order
+  for (u32 s = 0; s < S; ++s) {
+    if (want_out && opts[s].frame_mode == XNYZ_FRAME_ZSTD_GPU) gsegs.push_back(s);
+    else if (want_out) host_pre = true;
+  }
+  uint8_t* hb = nullptr;
+  if (host_pre) {
+    ctx->h_out.ensure(out_total + 64);
+    hb = ctx->h_out.as<uint8_t>();
+    check_hip(hipMemcpyAsync(hb, ow, out_total, hipMemcpyDeviceToHost, st), "stream D2H");
+    ctx->copied();
+  }
+  std::vector<ZFrame> zf;
+  std::vector<u64> zpos;
+  uint8_t* zb = nullptr;
+  if (!gsegs.empty()) {
+    std::vector<ZBlk> zk;
+    u64 bound = 0;
+    for (u32 f = 0; f < (u32)gsegs.size(); ++f) {
+      const SegHost& h = g[gsegs[f]];
+      const u32 first = (u32)zk.size();
+      for (u64 o = 0; o < h.total; o += xz::kBlock)
+        zk.push_back(ZBlk{h.base + o, (u32)std::min<u64>(xz::kBlock, h.total - o), f});
+      zf.push_back(ZFrame{h.total, first, (u32)zk.size() - first});
+      bound += 13 + h.total + 3 * (zk.size() - first);
+    }
+    const u32 nblk = (u32)zk.size();
+    const size_t tb = (zk.size() * sizeof(ZBlk) + 255) & ~size_t(255);
+    ctx->ztab.ensure(tb + zf.size() * sizeof(ZFrame) + 256);
+    check_hip(hipMemcpyAsync(ctx->ztab.p, zk.data(), zk.size() * sizeof(ZBlk),
+                             hipMemcpyHostToDevice, st), "H2D");
+    check_hip(hipMemcpyAsync(ctx->ztab.as<uint8_t>() + tb, zf.data(), zf.size() * sizeof(ZFrame),
+                             hipMemcpyHostToDevice, st), "H2D");
+    ctx->copied();
+    ctx->zscr.ensure(zframe_scratch_bytes(nblk));
+    ctx->zout.ensure(bound + 64);
+    ctx->zpos.ensure((nblk + 1) * sizeof(u64) + 64);
+    zframe_encode(reinterpret_cast<const uint8_t*>(ow), ctx->ztab.as<ZBlk>(), nblk,
+                  reinterpret_cast<const ZFrame*>(ctx->ztab.as<uint8_t>() + tb), ctx->zscr.p,
+                  ctx->zout.as<uint8_t>(), ctx->zpos.as<u64>(), ar, st);
+    zpos.resize(nblk + 1);
+    check_hip(hipMemcpyAsync(zpos.data(), ctx->zpos.p, (nblk + 1) * sizeof(u64),
+                             hipMemcpyDeviceToHost, st), "D2H");
+    ctx->sync();
+    const u64 total = zpos[nblk] + 13 * zf.size();
+    ctx->h_zout.ensure(total + 64);
+    zb = ctx->h_zout.as<uint8_t>();
+    check_hip(hipMemcpyAsync(zb, ctx->zout.p, total, hipMemcpyDeviceToHost, st), "frames D2H");
+  }
+  ctx->sync();
+
   const double t_dev = now_ms();
   const bool per_seg_threads = S > 1;
   if (co.pre) {
@@ -1081,10 +1151,17 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     fs.ensure_workers(per_seg_threads ? (size_t)std::min<u64>(S, (u64)threads) : 1);
     std::vector<Pieces> outs(S);
     parallel_for_w(S, per_seg_threads ? threads : 1, [&](size_t s, size_t w) {
-      const uint8_t* b = hb + g[s].base;
+      const uint8_t* b = hb ? hb + g[s].base : nullptr;
       const u64 len = g[s].total;
       uint8_t* z = fs.buf.data() + zoff[s];
       switch (opts[s].frame_mode) {
+        case XNYZ_FRAME_ZSTD_GPU: {
+          const u32 f = (u32)(std::lower_bound(gsegs.begin(), gsegs.end(), (u32)s) - gsegs.begin());
+          const u64 st0 = zpos[zf[f].first] + 13ull * f;
+          const u64 en = zpos[zf[f].first + zf[f].nblk] + 13ull * (f + 1);
+          outs[s] = {Piece{zb + st0, en - st0}};
+          break;
+        }
         case XNYZ_FRAME_NONE: outs[s] = {Piece{b, len}}; break;
         case XNYZ_FRAME_ZSTD3_MT: outs[s] = zstd_chunked(fs, fs.cctx[w], b, len, zthreads[s], z); break;
         default: outs[s] = zstd_single(fs.cctx[w], b, len, z, zoff[s + 1] - zoff[s]);
@@ -1259,10 +1336,11 @@ int xnyz_destroy(xnyz_ctx* ctx) {
                     &ctx->arena, &ctx->upload, &ctx->tmp, &ctx->idx[0], &ctx->idx[1],
                     &ctx->lo64[0], &ctx->lo64[1], &ctx->feat, &ctx->mP, &ctx->mWh, &ctx->mG,
                     &ctx->mM, &ctx->mV, &ctx->mX, &ctx->mY, &ctx->mIdx, &ctx->mScratch,
-                    &ctx->mLoss, &ctx->mOut, &ctx->mConf})
+                    &ctx->mLoss, &ctx->mOut, &ctx->mConf, &ctx->patch, &ctx->ztab,
+                    &ctx->zscr, &ctx->zout, &ctx->zpos})
       b->release();
     for (HBuf* b : {&ctx->h_tiles, &ctx->h_sp, &ctx->h_sc, &ctx->h_sh, &ctx->h_lut, &ctx->h_out,
-                    &ctx->h_feat})
+                    &ctx->h_feat, &ctx->h_zout})
       b->release();
     ctx->prof.destroy();
     ctx->stage2.clear();

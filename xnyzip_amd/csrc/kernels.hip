@@ -165,6 +165,14 @@ __global__ void k_fill_u32(u32* p, u64 count, u32 value) {
     p[i] = value;
 }
 
+__global__ void k_patch(const Patch* __restrict__ recs, u32 n, const uint8_t* __restrict__ bytes,
+                        uint8_t* __restrict__ out) {
+  for (u32 r = blockIdx.x; r < n; r += gridDim.x) {
+    const Patch q = recs[r];
+    for (u32 i = threadIdx.x; i < q.len; i += blockDim.x) out[q.dst + i] = bytes[q.src + i];
+  }
+}
+
 // (point index << 32 | quad << 18 | sfc) -> separate arrays
 __global__ void k_split_lo64(const u64* __restrict__ lo64, u32* __restrict__ lo,
                              u32* __restrict__ idx, u32 n) {
@@ -1499,6 +1507,12 @@ void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s) {
   if (!count) return;
   k_fill_u32<<<grid_for(count, kBlock * 8, 16384), kBlock, 0, s>>>(p, count, value);
   LAUNCH_CHECK("k_fill_u32");
+}
+
+void launch_patch(const Patch* recs, u32 n, const uint8_t* bytes, uint8_t* out, hipStream_t s) {
+  if (!n) return;
+  k_patch<<<n < 16384 ? n : 16384, 64, 0, s>>>(recs, n, bytes, out);
+  LAUNCH_CHECK("k_patch");
 }
 
 void launch_split_lo64(const u64* lo64, u32* lo, u32* idx, u32 n, hipStream_t s) {

@@ -243,4 +243,12 @@ void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u3
 
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s);
 
+// Byte runs copied into the stream buffer: out[dst, dst + len) =
+// bytes[src, src + len) per record.
+struct Patch {
+  u64 dst;
+  u32 src, len;
+};
+void launch_patch(const Patch* recs, u32 n, const uint8_t* bytes, uint8_t* out, hipStream_t s);
+
 }  // namespace xnyz
