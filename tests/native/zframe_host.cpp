@@ -1,7 +1,8 @@
 // Host driver of the zstd-format block encoder (xnyzip_amd/csrc/zframe.hpp),
 // test infrastructure only: runs the encoder's steps in the order the device
-// kernel does (earliest-position hash table, 256-byte segments parsed
-// independently, literal-length carry across segments), so the device
+// kernel does (earliest-position hash tables, one candidate byte per
+// position, segments parsed independently, literal-length carry across
+// segments), so the device
 // frames can be compared with these byte for byte and these checked against
 // libzstd's decoder.
 #include <cstdint>
@@ -22,13 +23,15 @@ static u32 encode_block(const u8* d, u32 n, u8* out) {
       if (p < f) f = p;
     }
   }
+  std::vector<u8> info(n);
+  for (u32 p = 0; p < n; ++p) info[p] = best_fixed(d, n, ht4.data(), ht8.data(), p);
   std::vector<Seq> seqs;
   std::vector<u8> lits;
   u32 carry = 0;  // literals pending from earlier segments
   for (u32 a = 0; a < n; a += kSeg) {
     const u32 b = a + kSeg < n ? a + kSeg : n;
     bool first = true;
-    auto best_at = [&](u32 p) { return best_fixed(d, n, ht4.data(), ht8.data(), p); };
+    auto best_at = [&](u32 p) { return cand_match(d, n, ht4.data(), ht8.data(), p, info[p]); };
     const u32 tail = parse_segment(d, a, b, best_at, [&](u32 lit0, u32 p, u32 len, u32 off) {
       for (u32 i = lit0; i < p; ++i) lits.push_back(d[i]);
       seqs.push_back(Seq{(p - lit0) + (first ? carry : 0), len, off});
@@ -42,8 +45,10 @@ static u32 encode_block(const u8* d, u32 n, u8* out) {
   for (u8 c : lits) hist[c]++;
   std::vector<SeqCodes> codes(seqs.size() + 1);
   HufTable h;
+  static Work W;
+  static SeqWork sw;
   return finish_block(out, d, n, lits.data(), (u32)lits.size(), hist, seqs.data(),
-                      (u32)seqs.size(), codes.data(), h);
+                      (u32)seqs.size(), codes.data(), h, W, sw);
 }
 
 extern "C" uint64_t zf_frame_bound(uint64_t n) { return 13 + n + 3 * (n / kBlock + 1) + 64; }

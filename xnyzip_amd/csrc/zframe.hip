@@ -1,16 +1,21 @@
 // zframe.hip -- zstd-format frames built on the device from the assembled
 // streams in HBM (format and parse rules in zframe.hpp).
 //
-//   k_zparse   one workgroup per 64 KiB block: the block and its two hash
-//              tables in LDS (128 KiB); the best fixed candidate of every
-//              position, evaluated by all 256 threads; then one thread per
-//              1 KiB segment runs the lazy parse over those candidates,
-//              writing its sequences and literals into per-segment slots
-//              and the literal histogram.
-//   k_zfinish  one lane per block: slots compacted in order (the first
+//   k_zparse   one workgroup per 16 KiB block: the block, its two hash
+//              tables and one candidate byte per position in LDS (64 KiB,
+//              two workgroups per CU); the candidates are evaluated by all
+//              256 threads, then one thread per 512-byte segment runs the
+//              lazy parse over them, writing its sequences and literals
+//              into per-segment slots and the literal histogram.
+//   k_zfinish  one wavefront per block: slots compacted in order (the first
 //              sequence of a segment takes the literals left pending by the
-//              segments before it), then the literals and sequences
-//              sections (finish_block) into the block's output slot.
+//              segments before it), then the block as zframe.hpp's
+//              finish_block writes it -- the serial steps (Huffman and FSE
+//              table builds, repeat-offset history, the FSE chain) executed
+//              uniformly by all 64 lanes on LDS working storage, with
+//              sequences fetched 64 at a time and broadcast by lane
+//              shuffles; Huffman streams bit-packed by all lanes at offsets
+//              from a wavefront scan.
 //   k_zgather  blocks copied behind their frame headers, positions from a
 //              device-wide scan of the block sizes.
 #include <hip/hip_runtime.h>
@@ -30,29 +35,208 @@ using xz::kSegs;
 using xz::kSlot;
 using xz::Seq;
 using xz::SeqCodes;
+using xz::u16;
 
 constexpr u32 kSegSeqs = kSeg / kMinMatch;  // most sequences one segment can hold
 
 struct BlkScratch {  // per block, in global memory
-  u32 cand[kBlock];
-  Seq seqs[kSegs * kSegSeqs];
-  SeqCodes codes[kSegs * kSegSeqs];
+  Seq seqs[kSegs * kSegSeqs];  // per-segment slots (k_zparse)
+  Seq cseq[kSegs * kSegSeqs];  // compacted (k_zfinish)
   u8 lits[kBlock];
+  u8 clit[kBlock];
   u32 hist[256];
   u32 nseq[kSegs], nlit[kSegs];
 };
 
+constexpr u32 kMaxSeqs = kBlock / kMinMatch;
+
+struct FinLds {
+  xz::Work W;
+  xz::SeqWork sw;
+  xz::HufTable h;
+  u32 hist[256];
+  u16 ov[kMaxSeqs];  // offset values (offset + 3 or a repeat code), < 2^15 in a 16 KiB block
+  u32 segn[kSegs], segl[kSegs], segll[kSegs], carry[kSegs], soff[kSegs + 1], loff[kSegs + 1];
+  u8 desc[160];
+  u32 sbits[4];
+};
+
+__device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
+  return Seq{(u32)__shfl((int)q.ll, (int)j, 64), (u32)__shfl((int)q.ml, (int)j, 64),
+             (u32)__shfl((int)q.off, (int)j, 64)};
+}
+
+// Repeat-offset history over the sequences (zframe.hpp seq_codes), one
+// step per sequence, into L.ov.
+__device__ void dev_offset_values(const Seq* cs, u32 ns, FinLds& L) {
+  const u32 lane = threadIdx.x;
+  u32 rep[3] = {0, 0, 0}, known = 0;
+  for (u32 base = 0; base < ns; base += 64) {
+    const Seq mine = base + lane < ns ? cs[base + lane] : Seq{0, 0, 0};
+    const u32 m = ns - base < 64 ? ns - base : 64;
+    for (u32 j = 0; j < m; ++j) {
+      const Seq q = shfl_seq(mine, j);
+      const u32 o = q.off, ll = q.ll;
+      u32 ov = 0;
+      if (ll) {
+        if (known >= 1 && o == rep[0]) {
+          ov = 1;
+        } else if (known >= 2 && o == rep[1]) {
+          ov = 2;
+          rep[1] = rep[0];
+          rep[0] = o;
+        } else if (known >= 3 && o == rep[2]) {
+          ov = 3;
+          rep[2] = rep[1];
+          rep[1] = rep[0];
+          rep[0] = o;
+        }
+      } else {
+        if (known >= 2 && o == rep[1]) {
+          ov = 1;
+          rep[1] = rep[0];
+          rep[0] = o;
+        } else if (known >= 3 && o == rep[2]) {
+          ov = 2;
+          rep[2] = rep[1];
+          rep[1] = rep[0];
+          rep[0] = o;
+        } else if (known >= 1 && rep[0] > 1 && o == rep[0] - 1) {
+          ov = 3;
+          rep[2] = rep[1];
+          rep[1] = rep[0];
+          rep[0] = o;
+          known = known < 3 ? known + 1 : 3;
+        }
+      }
+      if (!ov) {
+        ov = o + 3;
+        rep[2] = rep[1];
+        rep[1] = rep[0];
+        rep[0] = o;
+        known = known < 3 ? known + 1 : 3;
+      }
+      if (lane == 0) L.ov[base + j] = (u16)ov;
+    }
+  }
+}
+
+__device__ __forceinline__ SeqCodes codes_of(const Seq& q, u32 ov) {
+  SeqCodes c;
+  c.ll = (u8)xz::ll_code(q.ll);
+  c.ml = (u8)xz::ml_code(q.ml);
+  c.ov = ov;
+  c.of = (u8)xz::highbit(ov);
+  return c;
+}
+
+// Huffman-coded literals as write_huf_lits lays them out, written at their
+// final place: slot + base (slot 256-byte aligned; words are addressed from
+// the slot so that every 32-bit store and atomic is aligned).  Returns 0
+// when not representable or not smaller than raw.
+__device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, FinLds& L) {
+  u8* out = slot + base;
+  const u32 lane = threadIdx.x;
+  const bool four = n > 1023;
+  const u32 dl = xz::huf_write_desc(L.desc, L.h, L.W);  // uniform
+  if (!dl) return 0;
+  const u32 nst = four ? 4 : 1, seg = four ? (n + 3) / 4 : n;
+  u32 sbytes[4] = {0, 0, 0, 0};
+  for (u32 k = 0; k < nst; ++k) {
+    const u32 a = k * seg, e = k + 1 < nst ? a + seg : n;
+    u32 bits = 0;
+    for (u32 i = a + lane; i < e; i += 64) bits += L.h.nbits[lit[i]];
+    bits = wave_sum(bits);
+    sbytes[k] = (bits + 8) / 8;  // with the end mark
+    if (lane == 0) L.sbits[k] = bits;
+  }
+  u32 cs = dl + (four ? 6 : 0);
+  for (u32 k = 0; k < nst; ++k) cs += sbytes[k];
+  const u32 big = n > cs ? n : cs;
+  u32 hdr;
+  u64 v;
+  if (!four) {
+    if (big > 1023) return 0;
+    hdr = 3;
+    v = 2u | ((u64)n << 4) | ((u64)cs << 14);
+  } else if (big < 16384) {
+    hdr = 4;
+    v = 2u | (2u << 2) | ((u64)n << 4) | ((u64)cs << 18);
+  } else {
+    if (big >= (1u << 18)) return 0;
+    hdr = 5;
+    v = 2u | (3u << 2) | ((u64)n << 4) | ((u64)cs << 22);
+  }
+  const u32 total = hdr + cs;
+  if (total >= n + (n < 32 ? 1u : n < 4096 ? 2u : 3u)) return 0;
+  if (four)
+    for (u32 k = 0; k < 3; ++k)
+      if (sbytes[k] > 0xFFFF) return 0;
+  // header, description, jump table: byte stores; the streams' bytes zeroed
+  u32 at = hdr + dl + (four ? 6 : 0);
+  for (u32 i = lane; i < total - at; i += 64) out[at + i] = 0;
+  if (lane == 0) {
+    for (u32 i = 0; i < hdr; ++i) out[i] = (u8)(v >> (8 * i));
+    for (u32 i = 0; i < dl; ++i) out[hdr + i] = L.desc[i];
+    if (four)
+      for (u32 k = 0; k < 3; ++k) {
+        out[hdr + dl + 2 * k] = (u8)sbytes[k];
+        out[hdr + dl + 2 * k + 1] = (u8)(sbytes[k] >> 8);
+      }
+  }
+  __threadfence_block();
+  __syncthreads();
+  // streams: literal i of a stream sits after the codes of the literals
+  // following it; lane chunks of the reversed stream, offsets by scan
+  u32* w = reinterpret_cast<u32*>(slot);
+  for (u32 k = 0; k < nst; ++k) {
+    const u32 a = k * seg, e = k + 1 < nst ? a + seg : n, m = e - a;
+    const u32 per = (m + 63) / 64, r0 = lane * per, r1 = r0 + per < m ? r0 + per : m;
+    u32 bits = 0;
+    for (u32 r = r0; r < r1; ++r) bits += L.h.nbits[lit[e - 1 - r]];
+    const u32 inc = wave_incl_scan(bits);
+    u64 pos = (u64)(base + at) * 8 + (inc - bits);  // bit position from the slot start
+    u64 acc = 0;
+    u32 nb = (u32)(pos & 31);
+    u64 word = pos >> 5;
+    bool first = true;
+    for (u32 r = r0; r < r1; ++r) {
+      const u8 c = lit[e - 1 - r];
+      acc |= (u64)L.h.code[c] << nb;
+      nb += L.h.nbits[c];
+      if (nb >= 32) {
+        if (first) atomicOr(&w[word], (u32)acc);
+        else w[word] = (u32)acc;
+        first = false;
+        acc >>= 32;
+        nb -= 32;
+        ++word;
+      }
+    }
+    if (nb) atomicOr(&w[word], (u32)acc);
+    if (lane == 63) {  // the end mark after the stream's last code
+      const u64 endb = (u64)(base + at) * 8 + inc;
+      atomicOr(&w[endb >> 5], 1u << (endb & 31));
+    }
+    at += sbytes[k];
+  }
+  __threadfence_block();
+  __syncthreads();
+  return total;
+}
+
 __global__ __launch_bounds__(256) void k_zparse(const u8* __restrict__ pre,
                                                 const ZBlk* __restrict__ blks,
                                                 BlkScratch* __restrict__ scr) {
-  __shared__ u8 data[kBlock];
+  __shared__ __align__(16) u8 data[kBlock];
   __shared__ u32 ht4[1u << kHashLog], ht8[1u << kHashLog];
+  __shared__ u8 info[kBlock];
   __shared__ u32 hist[256];
   const ZBlk bk = blks[blockIdx.x];
   BlkScratch& S = scr[blockIdx.x];
   const u32 n = bk.n, tid = threadIdx.x;
   const u8* src = pre + bk.src;
-  for (u32 i = tid * 4; i + 4 <= n; i += kBlock / 64)  // 4-byte aligned source
+  for (u32 i = tid * 4; i + 4 <= n; i += 1024)  // 4-byte aligned source
     *reinterpret_cast<u32*>(data + i) = *reinterpret_cast<const u32*>(src + i);
   for (u32 i = (n & ~3u) + tid; i < n; i += 256) data[i] = src[i];
   for (u32 i = tid; i < (1u << kHashLog); i += 256) ht4[i] = ht8[i] = 0xFFFFFFFFu;
@@ -63,15 +247,15 @@ __global__ __launch_bounds__(256) void k_zparse(const u8* __restrict__ pre,
     if (p + 8 <= n) atomicMin(&ht8[xz::hash8(xz::read64(data + p))], p);
   }
   __syncthreads();
-  for (u32 p = tid; p < n; p += 256) S.cand[p] = xz::best_fixed(data, n, ht4, ht8, p);
-  __syncthreads();  // cand[] of every position visible to the parse threads
+  for (u32 p = tid; p < n; p += 256) info[p] = xz::best_fixed(data, n, ht4, ht8, p);
+  __syncthreads();
   const u32 nseg = (n + kSeg - 1) / kSeg;
   if (tid < nseg) {
     const u32 a = tid * kSeg, b = a + kSeg < n ? a + kSeg : n;
     Seq* sq = S.seqs + tid * kSegSeqs;
     u8* lit = S.lits + tid * kSeg;
     u32 ns = 0, nl = 0;
-    auto best_at = [&](u32 p) { return S.cand[p]; };
+    auto best_at = [&](u32 p) { return xz::cand_match(data, n, ht4, ht8, p, info[p]); };
     const u32 tail = xz::parse_segment(data, a, b, best_at, [&](u32 l0, u32 p, u32 len, u32 off) {
       for (u32 i = l0; i < p; ++i) {
         lit[nl++] = data[i];
@@ -91,35 +275,143 @@ __global__ __launch_bounds__(256) void k_zparse(const u8* __restrict__ pre,
 }
 
 __global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
-                                                const ZBlk* __restrict__ blks, u32 nblk,
+                                                const ZBlk* __restrict__ blks,
                                                 BlkScratch* __restrict__ scr,
                                                 u8* __restrict__ slots,
                                                 u64* __restrict__ slot_len) {
-  const u32 b = blockIdx.x * 64 + threadIdx.x;
-  if (b >= nblk) return;
+  __shared__ FinLds L;
+  const u32 b = blockIdx.x, lane = threadIdx.x;
   const ZBlk bk = blks[b];
   BlkScratch& S = scr[b];
   const u32 n = bk.n, nseg = (n + kSeg - 1) / kSeg;
-  // compact the per-segment slots in place (destinations never pass sources)
-  u32 ns = 0, nl = 0, carry = 0;
-  for (u32 t = 0; t < nseg; ++t) {
-    const Seq* sq = S.seqs + t * kSegSeqs;
-    u32 in_seqs = 0;
-    for (u32 k = 0; k < S.nseq[t]; ++k) {
-      Seq q = sq[k];
-      in_seqs += q.ll;
-      if (k == 0) q.ll += carry;
-      S.seqs[ns++] = q;
+  u8* out = slots + (u64)b * kSlot;
+  // 1. per-segment counts, offsets and literal carries
+  if (lane < kSegs) {
+    u32 ns = 0, nl = 0, llsum = 0;
+    if (lane < nseg) {
+      ns = S.nseq[lane];
+      nl = S.nlit[lane];
+      for (u32 k = 0; k < ns; ++k) llsum += S.seqs[lane * kSegSeqs + k].ll;
     }
-    const u8* lt = S.lits + t * kSeg;
-    const u32 m = S.nlit[t];
-    for (u32 i = 0; i < m; ++i) S.lits[nl + i] = lt[i];
-    nl += m;
-    carry = S.nseq[t] ? m - in_seqs : carry + m;
+    L.segn[lane] = ns;
+    L.segl[lane] = nl;
+    L.segll[lane] = llsum;
   }
-  xz::HufTable h;
-  slot_len[b] = xz::finish_block(slots + (u64)b * kSlot, pre + bk.src, n, S.lits, nl, S.hist,
-                                 S.seqs, ns, S.codes, h);
+  for (u32 i = lane; i < 256; i += 64) L.hist[i] = S.hist[i];
+  __syncthreads();
+  if (lane == 0) {
+    u32 so = 0, lo = 0, carry = 0;
+    for (u32 t = 0; t < nseg; ++t) {
+      L.soff[t] = so;
+      L.loff[t] = lo;
+      L.carry[t] = carry;
+      so += L.segn[t];
+      lo += L.segl[t];
+      carry = L.segn[t] ? L.segl[t] - L.segll[t] : carry + L.segl[t];
+    }
+    L.soff[nseg] = so;
+    L.loff[nseg] = lo;
+  }
+  __syncthreads();
+  const u32 ns = L.soff[nseg], nl = L.loff[nseg];
+  // 2. compaction (separate arrays: no overlap)
+  for (u32 t = 0; t < nseg; ++t) {
+    const u32 sn = L.segn[t], so = L.soff[t], ln = L.segl[t], lo = L.loff[t];
+    for (u32 k = lane; k < sn; k += 64) {
+      Seq q = S.seqs[t * kSegSeqs + k];
+      if (k == 0) q.ll += L.carry[t];
+      S.cseq[so + k] = q;
+    }
+    for (u32 i = lane; i < ln; i += 64) S.clit[lo + i] = S.lits[t * kSeg + i];
+  }
+  __threadfence_block();
+  __syncthreads();
+  // 3. literals section (finish_block's choice)
+  u8* body = out + 3;
+  u32 ls = 0;
+  bool rle = nl > 1;
+  for (u32 i = 0; rle && i < 256; ++i) rle = L.hist[i] == 0 || L.hist[i] == nl;
+  if (rle) {
+    if (lane == 0) ls = xz::write_raw_lits(body, S.clit, nl, 1);
+    ls = __shfl((int)ls, 0, 64);
+  } else {
+    if (nl >= 64 && xz::huf_build(L.hist, L.h, L.W)) {  // uniform
+      u64 bits = 0;
+      for (u32 c = 0; c < 256; ++c) bits += (u64)L.hist[c] * L.h.nbits[c];
+      if (bits / 8 + 16 < nl) ls = dev_huf_lits(out, 3, S.clit, nl, L);
+    }
+    if (!ls) {  // raw literals: header, then the bytes
+      const u32 h = nl < 32 ? 1 : nl < 4096 ? 2 : 3;
+      if (lane == 0) {
+        if (h == 1) {
+          body[0] = (u8)(nl << 3);
+        } else {
+          const u32 v = (nl << 4) | ((h == 2 ? 1u : 3u) << 2);
+          for (u32 i = 0; i < h; ++i) body[i] = (u8)(v >> (8 * i));
+        }
+      }
+      for (u32 i = lane; i < nl; i += 64) body[h + i] = S.clit[i];
+      ls = h + nl;
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  // 4. sequences section
+  u8* sq = body + ls;
+  u32 ss;
+  if (lane == 0) ss = xz::seq_count_header(sq, ns);
+  ss = __shfl((int)ss, 0, 64);
+  if (ns) {
+    dev_offset_values(S.cseq, ns, L);
+    for (u32 i = lane; i < 3 * 64; i += 64) (&L.sw.cnt[0][0])[i] = 0;
+    __syncthreads();
+    for (u32 i = lane; i < ns; i += 64) {
+      const SeqCodes c = codes_of(S.cseq[i], L.ov[i]);
+      atomicAdd(&L.sw.cnt[0][c.ll], 1u);
+      atomicAdd(&L.sw.cnt[1][c.of], 1u);
+      atomicAdd(&L.sw.cnt[2][c.ml], 1u);
+    }
+    __syncthreads();
+    ss += xz::seq_write_tables(sq + ss, ns, L.sw, L.W);  // uniform
+    xz::BitW bw(sq + ss);
+    xz::FseState st[3];
+    {
+      const Seq q = S.cseq[ns - 1];
+      xz::seq_encode_last(bw, st, L.sw, q, codes_of(q, L.ov[ns - 1]));
+    }
+    for (u32 top = ns - 1; top > 0;) {  // sequences top-1 down to 0, 64 at a time
+      const u32 m = top < 64 ? top : 64;
+      const u32 idx = top - 1 - lane;
+      const Seq mine = lane < m ? S.cseq[idx] : Seq{0, 0, 0};
+      const u32 myov = lane < m ? L.ov[idx] : 0u;
+      for (u32 j = 0; j < m; ++j) {
+        const Seq q = shfl_seq(mine, j);
+        const u32 ov = (u32)__shfl((int)myov, (int)j, 64);
+        xz::seq_encode_one(bw, st, L.sw, q, codes_of(q, ov));
+      }
+      top -= m;
+    }
+    ss += xz::seq_encode_close(bw, st, L.sw);
+  }
+  __threadfence_block();
+  __syncthreads();
+  // 5. the block: compressed, or raw when that is not smaller
+  const u32 size = ls + ss;
+  if (size >= n) {
+    for (u32 i = lane; i < n; i += 64) out[3 + i] = pre[bk.src + i];
+    if (lane == 0) {
+      out[0] = (u8)(n << 3);
+      out[1] = (u8)(n >> 5);
+      out[2] = (u8)(n >> 13);
+      slot_len[b] = 3 + n;
+    }
+  } else if (lane == 0) {
+    const u32 h = (2u << 1) | (size << 3);
+    out[0] = (u8)h;
+    out[1] = (u8)(h >> 8);
+    out[2] = (u8)(h >> 16);
+    slot_len[b] = 3 + size;
+  }
 }
 
 // Frame f at out + pos[frames[f].first] + 13 f: magic, descriptor (single
@@ -163,7 +455,7 @@ void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* fram
   k_zparse<<<nblk, 256, 0, s>>>(pre, blks, scr);
   check_hip(hipGetLastError(), "k_zparse");
   if (g_profiler) g_profiler->mark("k_zparse", s);
-  k_zfinish<<<(nblk + 63) / 64, 64, 0, s>>>(pre, blks, nblk, scr, slots, slot_len);
+  k_zfinish<<<nblk, 64, 0, s>>>(pre, blks, scr, slots, slot_len);
   check_hip(hipGetLastError(), "k_zfinish");
   if (g_profiler) g_profiler->mark("k_zfinish", s);
   scan_excl_u64(slot_len, pos, nblk, pos + nblk, ar, s);

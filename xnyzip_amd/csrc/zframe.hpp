@@ -4,11 +4,13 @@
 // byte-identical to ZSTD_compress's (the assembled stream inside is).
 //
 // Block format used (RFC 8878 section 3.1.1):
-//   * LZ: lazy parse of fixed 1 KiB segments of a 64 KiB block, candidates
-//     from two hash tables (4-byte and 8-byte keys) holding the EARLIEST
-//     position of each hash in the block (order-independent, so a parallel
-//     build gives the same tables), the segment's last three offsets and
-//     offsets 1, 8, 16, scored by length minus an offset cost.  Matches stay
+//   * LZ: lazy parse of fixed 512-byte segments of a 16 KiB block.  Every
+//     position's best fixed candidate -- offsets 1, 8, 16 and the hits of two
+//     hash tables (4- and 8-byte keys) holding the EARLIEST position of each
+//     hash in the block (order-independent, so a parallel build gives the
+//     same tables) -- is chosen with lengths capped at kCap and kept as one
+//     byte; the parse extends a capped winner and adds the segment's last
+//     three offsets, scoring length minus an offset cost.  Matches stay
 //     inside their segment and reference earlier bytes of the same block.
 //   * Literals: Huffman-coded (1 stream up to 1023 literals, else 4 streams),
 //     weights FSE-compressed (or direct 4-bit when that is smaller), code
@@ -37,10 +39,11 @@ using i16 = int16_t;
 using i32 = int32_t;
 using u64 = unsigned long long;
 
-constexpr u32 kBlock = 1u << 16;   // bytes per zstd block
-constexpr u32 kSeg = 1024;         // bytes per parse segment
+constexpr u32 kBlock = 1u << 14;   // bytes per zstd block
+constexpr u32 kSeg = 512;          // bytes per parse segment
 constexpr u32 kSegs = kBlock / kSeg;
-constexpr u32 kHashLog = 13;
+constexpr u32 kHashLog = 12;
+constexpr u32 kCap = 35;           // longest length a fixed candidate is measured to
 constexpr u32 kMinMatch = 4;
 constexpr u32 kMaxHufBits = 11;
 
@@ -95,11 +98,11 @@ struct FseTable {
   i32 dfs[kFseMaxSym];  // deltaFindState
 };
 
-XZ_HD inline void fse_build(FseTable& t, const i16* norm, u32 nsym, u32 log) {
+// spread: 2^log bytes, cumul: nsym + 1 words of working storage.
+XZ_HD inline void fse_build(FseTable& t, const i16* norm, u32 nsym, u32 log, u8* spread,
+                            u32* cumul) {
   const u32 size = 1u << log, mask = size - 1;
-  u8 spread[1u << kFseMaxLog];
   u32 high = size - 1;
-  u32 cumul[kFseMaxSym + 1];
   cumul[0] = 0;
   for (u32 s = 0; s < nsym; ++s) {
     if (norm[s] == -1) {
@@ -144,6 +147,25 @@ XZ_HD inline void fse_build(FseTable& t, const i16* norm, u32 nsym, u32 log) {
 
 struct FseState {
   u32 v;
+};
+
+// Working storage of the serial steps below (host: any memory; device: LDS,
+// with one wavefront executing the steps uniformly).
+struct Work {
+  u8 spread[1u << kFseMaxLog];
+  u32 cumul[kFseMaxSym + 1];
+  u32 cnt[256];
+  u16 sym[256], tmp[256];
+  u32 pos[257];
+  u64 w[512];
+  i32 parent[512];
+  u32 depth[512];
+  u8 wt[256];
+  u8 dtmp[160];
+  u32 wcount[16];
+  i16 norm[kFseMaxSym];
+  u8 ntmp[128];
+  FseTable wtab;
 };
 
 XZ_HD inline void fse_init(FseState& st, const FseTable& t, u32 s) {
@@ -256,8 +278,8 @@ struct HufTable {
 };
 
 // Returns false when fewer than two symbols are present.
-XZ_HD inline bool huf_build(const u32* count_in, HufTable& h) {
-  u32 cnt[256];
+XZ_HD inline bool huf_build(const u32* count_in, HufTable& h, Work& W) {
+  u32* cnt = W.cnt;
   u32 nsym = 0;
   for (u32 s = 0; s < 256; ++s) {
     cnt[s] = count_in[s];
@@ -269,21 +291,27 @@ XZ_HD inline bool huf_build(const u32* count_in, HufTable& h) {
   if (nsym < 2) return false;
   for (;;) {
     // Two-queue Huffman over the present symbols sorted by count.
-    u16 sym[256];
+    // present symbols sorted by (count, symbol): stable LSD radix sort over
+    // the count's bytes, symbols entered in increasing order
+    u16* sym = W.sym;
+    u16* tmp = W.tmp;
     u32 n = 0;
+    u32 maxc = 0;
     for (u32 s = 0; s < 256; ++s)
-      if (cnt[s]) sym[n++] = (u16)s;
-    for (u32 i = 1; i < n; ++i) {  // insertion sort by (count, symbol)
-      const u16 x = sym[i];
-      u32 j = i;
-      while (j > 0 && (cnt[sym[j - 1]] > cnt[x])) {
-        sym[j] = sym[j - 1];
-        --j;
+      if (cnt[s]) {
+        sym[n++] = (u16)s;
+        maxc = cnt[s] > maxc ? cnt[s] : maxc;
       }
-      sym[j] = x;
+    for (u32 sh = 0; sh < 32 && (maxc >> sh); sh += 8) {
+      u32* pos = W.pos;
+      for (u32 i = 0; i < 257; ++i) pos[i] = 0;
+      for (u32 i = 0; i < n; ++i) pos[((cnt[sym[i]] >> sh) & 255u) + 1]++;
+      for (u32 i = 1; i < 257; ++i) pos[i] += pos[i - 1];
+      for (u32 i = 0; i < n; ++i) tmp[pos[(cnt[sym[i]] >> sh) & 255u]++] = sym[i];
+      for (u32 i = 0; i < n; ++i) sym[i] = tmp[i];
     }
-    u64 w[512];
-    i32 parent[512];
+    u64* w = W.w;
+    i32* parent = W.parent;
     for (u32 i = 0; i < n; ++i) w[i] = cnt[sym[i]];
     u32 leaf = 0, node = n, inner = n;  // inner nodes created at [n, 2n-1)
     for (u32 k = 0; k + 1 < n; ++k) {
@@ -296,7 +324,7 @@ XZ_HD inline bool huf_build(const u32* count_in, HufTable& h) {
       parent[pick[0]] = parent[pick[1]] = (i32)node;
       ++node;
     }
-    u32 depth[512];
+    u32* depth = W.depth;
     depth[node - 1] = 0;
     u32 maxd = 0;
     for (i32 i = (i32)node - 2; i >= 0; --i) {
@@ -325,17 +353,18 @@ XZ_HD inline bool huf_build(const u32* count_in, HufTable& h) {
 // Huffman tree description (RFC 8878 4.2.1.1): weights of symbols
 // 0..last-1 (the last one is implied), FSE-compressed or direct.  Returns
 // bytes written, 0 if not representable.
-XZ_HD inline u32 huf_write_desc(u8* out, const HufTable& h) {
-  u8 wt[256];
+XZ_HD inline u32 huf_write_desc(u8* out, const HufTable& h, Work& W) {
+  u8* wt = W.wt;
   for (u32 s = 0; s < h.last; ++s) wt[s] = h.nbits[s] ? (u8)(h.maxbits + 1 - h.nbits[s]) : 0;
   const u32 nw = h.last;
   // direct: 4 bits per weight, at most 128 weights
   const u32 direct = nw <= 128 ? 1 + (nw + 1) / 2 : 0xFFFFFFFFu;
   // FSE: accuracy log 6, two interleaved states
   u32 fse = 0xFFFFFFFFu;
-  u8 tmp[160];
+  u8* tmp = W.dtmp;
   if (nw >= 2) {
-    u32 count[16] = {0};
+    u32* count = W.wcount;
+    for (u32 i = 0; i < 16; ++i) count[i] = 0;
     u32 maxw = 0;
     for (u32 i = 0; i < nw; ++i) {
       count[wt[i]]++;
@@ -345,11 +374,11 @@ XZ_HD inline u32 huf_write_desc(u8* out, const HufTable& h) {
     u32 present = 0;
     for (u32 i = 0; i < nsym; ++i) present += count[i] != 0;
     if (present >= 2) {
-      i16 norm[16];
+      i16* norm = W.norm;
       const u32 log = 6;
       fse_normalize(count, nsym, nw, log, norm);
-      FseTable t;
-      fse_build(t, norm, nsym, log);
+      FseTable& t = W.wtab;
+      fse_build(t, norm, nsym, log, W.spread, W.cumul);
       u32 hp = fse_write_ncount(tmp, norm, nsym, log);
       BitW bw(tmp + hp);
       FseState s1, s2;
@@ -538,7 +567,7 @@ XZ_HD inline void seq_codes(const Seq* sq, u32 n, SeqCodes* c) {
 // Per code stream: mode 0 predefined, 1 RLE, 2 FSE table written in the
 // block; fills the encoding table and writes its description (if any).
 XZ_HD inline u32 choose_table(int which, const u32* count, u32 n, FseTable& t, u8* desc,
-                              u32& mode) {
+                              u32& mode, Work& W) {
   const u32 ms = kMaxSym[which];
   u32 present = 0, last = 0, only = 0;
   for (u32 s = 0; s <= ms; ++s)
@@ -548,9 +577,10 @@ XZ_HD inline u32 choose_table(int which, const u32* count, u32 n, FseTable& t, u
       only = s;
     }
   if (present == 1 && n > 1) {  // RLE: one symbol, no bits
-    i16 norm[kFseMaxSym] = {0};
+    i16* norm = W.norm;
+    for (u32 s = 0; s <= only; ++s) norm[s] = 0;
     norm[only] = 1;
-    fse_build(t, norm, only + 1, 0);
+    fse_build(t, norm, only + 1, 0, W.spread, W.cumul);
     desc[0] = (u8)only;
     mode = 1;
     return 1;
@@ -570,78 +600,107 @@ XZ_HD inline u32 choose_table(int which, const u32* count, u32 n, FseTable& t, u
   // FSE table for this block
   u32 log = n > 1 ? highbit(n - 1) + 1 : 5;
   log = log < 5 ? 5 : log > kMaxLog[which] ? kMaxLog[which] : log;
-  i16 norm[kFseMaxSym];
+  i16* norm = W.norm;
   fse_normalize(count, last + 1, n, log, norm);
   u64 own = 0;
   for (u32 s = 0; s <= last; ++s)
     if (count[s]) own += (u64)count[s] * cost16((u32)norm[s], log);
-  u8 tmp[128];
+  u8* tmp = W.ntmp;
   const u32 hl = fse_write_ncount(tmp, norm, last + 1, log);
   own += 16ull * 8 * hl;
   if (pre_ok && pre <= own) {
-    fse_build(t, pn, kPreSyms[which], kPreLog[which]);
+    fse_build(t, pn, kPreSyms[which], kPreLog[which], W.spread, W.cumul);
     mode = 0;
     return 0;
   }
-  fse_build(t, norm, last + 1, log);
+  fse_build(t, norm, last + 1, log, W.spread, W.cumul);
   for (u32 i = 0; i < hl; ++i) desc[i] = tmp[i];
   mode = 2;
   return hl;
 }
 
+// The sequences section in steps the device can spread over a wavefront:
+// count header, code histograms, the three tables (mode byte + table
+// descriptions), the FSE bitstream.
+struct SeqWork {
+  FseTable t[3];  // literal lengths, offsets, match lengths
+  u32 cnt[3][64];
+};
+
+XZ_HD inline u32 seq_count_header(u8* out, u32 n) {
+  if (n < 128) {
+    out[0] = (u8)n;
+    return 1;
+  }
+  if (n < 0x7F00) {
+    out[0] = (u8)((n >> 8) + 128);
+    out[1] = (u8)n;
+    return 2;
+  }
+  out[0] = 0xFF;
+  out[1] = (u8)(n - 0x7F00);
+  out[2] = (u8)((n - 0x7F00) >> 8);
+  return 3;
+}
+
+// Mode byte and table descriptions from sw.cnt; fills sw.t.
+XZ_HD inline u32 seq_write_tables(u8* out, u32 n, SeqWork& sw, Work& W) {
+  u32 m[3];
+  u32 p = 1;
+  for (int k = 0; k < 3; ++k) p += choose_table(k, sw.cnt[k], n, sw.t[k], out + p, m[k], W);
+  out[0] = (u8)((m[0] << 6) | (m[1] << 4) | (m[2] << 2));
+  return p;
+}
+
+// One sequence's FSE symbols and extra bits (all but the last sequence).
+XZ_HD inline void seq_encode_one(BitW& bw, FseState* st, const SeqWork& sw, const Seq& q,
+                                 const SeqCodes& c) {
+  fse_encode(bw, st[1], sw.t[1], c.of);
+  fse_encode(bw, st[2], sw.t[2], c.ml);
+  fse_encode(bw, st[0], sw.t[0], c.ll);
+  bw.add(q.ll - ll_base(c.ll), ll_bits(c.ll));
+  bw.add(q.ml - ml_base(c.ml), ml_bits(c.ml));
+  bw.add(c.ov - (1u << c.of), c.of);
+}
+
+// The last sequence starts the states and writes its extra bits.
+XZ_HD inline void seq_encode_last(BitW& bw, FseState* st, const SeqWork& sw, const Seq& q,
+                                  const SeqCodes& c) {
+  fse_init(st[2], sw.t[2], c.ml);
+  fse_init(st[1], sw.t[1], c.of);
+  fse_init(st[0], sw.t[0], c.ll);
+  bw.add(q.ll - ll_base(c.ll), ll_bits(c.ll));
+  bw.add(q.ml - ml_base(c.ml), ml_bits(c.ml));
+  bw.add(c.ov - (1u << c.of), c.of);
+}
+
+XZ_HD inline u32 seq_encode_close(BitW& bw, const FseState* st, const SeqWork& sw) {
+  fse_flush(bw, st[2], sw.t[2]);
+  fse_flush(bw, st[1], sw.t[1]);
+  fse_flush(bw, st[0], sw.t[0]);
+  return bw.close();
+}
+
 // Sequences section (without the literals).  `codes`: scratch for n
 // entries.  Returns bytes written.
-XZ_HD inline u32 write_sequences(u8* out, const Seq* sq, u32 n, SeqCodes* codes) {
-  u32 p = 0;
-  if (n < 128) {
-    out[p++] = (u8)n;
-  } else if (n < 0x7F00) {
-    out[p++] = (u8)((n >> 8) + 128);
-    out[p++] = (u8)n;
-  } else {
-    out[p++] = 0xFF;
-    out[p++] = (u8)(n - 0x7F00);
-    out[p++] = (u8)((n - 0x7F00) >> 8);
-  }
+XZ_HD inline u32 write_sequences(u8* out, const Seq* sq, u32 n, SeqCodes* codes, SeqWork& sw,
+                                 Work& W) {
+  u32 p = seq_count_header(out, n);
   if (!n) return p;
   seq_codes(sq, n, codes);
-  u32 cnt[3][64];
   for (u32 k = 0; k < 3; ++k)
-    for (u32 s = 0; s < 64; ++s) cnt[k][s] = 0;
+    for (u32 s = 0; s < 64; ++s) sw.cnt[k][s] = 0;
   for (u32 i = 0; i < n; ++i) {
-    cnt[0][codes[i].ll]++;
-    cnt[1][codes[i].of]++;
-    cnt[2][codes[i].ml]++;
+    sw.cnt[0][codes[i].ll]++;
+    sw.cnt[1][codes[i].of]++;
+    sw.cnt[2][codes[i].ml]++;
   }
-  FseTable tll, tof, tml;
-  u32 mll, mof, mml;
-  u8* modes = out + p++;
-  p += choose_table(0, cnt[0], n, tll, out + p, mll);
-  p += choose_table(1, cnt[1], n, tof, out + p, mof);
-  p += choose_table(2, cnt[2], n, tml, out + p, mml);
-  *modes = (u8)((mll << 6) | (mof << 4) | (mml << 2));
+  p += seq_write_tables(out + p, n, sw, W);
   BitW bw(out + p);
-  FseState sll, sml, sof;
-  const SeqCodes& z = codes[n - 1];
-  fse_init(sml, tml, z.ml);
-  fse_init(sof, tof, z.of);
-  fse_init(sll, tll, z.ll);
-  bw.add(sq[n - 1].ll - ll_base(z.ll), ll_bits(z.ll));
-  bw.add(sq[n - 1].ml - ml_base(z.ml), ml_bits(z.ml));
-  bw.add(z.ov - (1u << z.of), z.of);
-  for (u32 i = n - 1; i-- > 0;) {
-    const SeqCodes& c = codes[i];
-    fse_encode(bw, sof, tof, c.of);
-    fse_encode(bw, sml, tml, c.ml);
-    fse_encode(bw, sll, tll, c.ll);
-    bw.add(sq[i].ll - ll_base(c.ll), ll_bits(c.ll));
-    bw.add(sq[i].ml - ml_base(c.ml), ml_bits(c.ml));
-    bw.add(c.ov - (1u << c.of), c.of);
-  }
-  fse_flush(bw, sml, tml);
-  fse_flush(bw, sof, tof);
-  fse_flush(bw, sll, tll);
-  return p + bw.close();
+  FseState st[3];
+  seq_encode_last(bw, st, sw, sq[n - 1], codes[n - 1]);
+  for (u32 i = n - 1; i-- > 0;) seq_encode_one(bw, st, sw, sq[i], codes[i]);
+  return p + seq_encode_close(bw, st, sw);
 }
 
 // ---------------------------------------------------------------------------
@@ -676,10 +735,10 @@ XZ_HD inline u32 write_raw_lits(u8* out, const u8* lit, u32 n, u32 type /*0 raw,
 // Huffman literals with table h built over `lit`.  The body is written at
 // out + 5 and moved down once the header size is known.  Returns 0 if the
 // encoding is not representable or not smaller than raw literals.
-XZ_HD inline u32 write_huf_lits(u8* out, const u8* lit, u32 n, const HufTable& h) {
+XZ_HD inline u32 write_huf_lits(u8* out, const u8* lit, u32 n, const HufTable& h, Work& W) {
   const bool four = n > 1023;
   u8* q = out + 5;
-  const u32 dl = huf_write_desc(q, h);
+  const u32 dl = huf_write_desc(q, h, W);
   if (!dl) return 0;
   u32 cs = dl;
   if (!four) {
@@ -745,38 +804,53 @@ XZ_HD inline u32 seg_end(u32 p, u32 n) {
   return e < n ? e : n;
 }
 
-// The best match at p among offsets 1, 8, 16 and the two hash candidates,
-// by length minus offset cost (no repeat discount; ties keep that order),
-// bounded by p's segment: (length << 16) | offset, or 0.  A function of p
-// alone, so every position can be evaluated in parallel.
-XZ_HD inline u32 best_fixed(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p) {
+// The best fixed candidate at p: offsets 1, 8, 16, then the 4-byte and the
+// 8-byte hash hits (candidate index 1..5), lengths measured up to kCap and
+// bounded by p's segment, scored by length minus offset cost (no repeat
+// discount; ties keep that order).  Returned as one byte, (index << 5) |
+// (length - kMinMatch), 0 if none -- a function of p alone, so every
+// position can be evaluated in parallel.
+XZ_HD inline u32 cand_offset(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p, u32 idx) {
+  switch (idx) {
+    case 1: return 1;
+    case 2: return 8;
+    case 3: return 16;
+    case 4: return p + 4 <= n ? p - ht4[hash4(read32(d + p))] : 0;
+    default: return p + 8 <= n ? p - ht8[hash8(read64(d + p))] : 0;
+  }
+}
+
+XZ_HD inline u8 best_fixed(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p) {
   const u32 b = seg_end(p, n);
   if (p + kMinMatch > b) return 0;
-  u32 best = 0, boff = 0;
+  const u32 cap = p + kCap < b ? p + kCap : b;
+  u32 best = 0, bidx = 0;
   i32 bs = -(1 << 30);
-  auto consider = [&](u32 off) {
-    if (!off || off > p) return;
-    const u32 l = match_len(d, p, p - off, b);
-    if (l < kMinMatch) return;
+  for (u32 idx = 1; idx <= 5; ++idx) {
+    if (idx == 4 && !(p + 4 <= n && ht4[hash4(read32(d + p))] < p)) continue;
+    if (idx == 5 && !(p + 8 <= n && ht8[hash8(read64(d + p))] < p)) continue;
+    const u32 off = cand_offset(d, n, ht4, ht8, p, idx);
+    if (!off || off > p) continue;
+    const u32 l = match_len(d, p, p - off, cap);
+    if (l < kMinMatch) continue;
     const i32 sc = (i32)l * kLenWeight - ((i32)highbit(off + 3) + kOffCost);
     if (sc > bs) {
       bs = sc;
       best = l;
-      boff = off;
+      bidx = idx;
     }
-  };
-  consider(1);
-  consider(8);
-  consider(16);
-  if (p + 4 <= n) {
-    const u32 q = ht4[hash4(read32(d + p))];
-    if (q < p) consider(p - q);
   }
-  if (p + 8 <= n) {
-    const u32 q = ht8[hash8(read64(d + p))];
-    if (q < p) consider(p - q);
-  }
-  return best ? (best << 16) | boff : 0;
+  return best ? (u8)((bidx << 5) | (best - kMinMatch)) : (u8)0;
+}
+
+// The match a candidate byte stands for: (length << 16) | offset, the
+// length extended past kCap to the segment end.
+XZ_HD inline u32 cand_match(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p, u8 info) {
+  if (!info) return 0;
+  const u32 off = cand_offset(d, n, ht4, ht8, p, info >> 5);
+  u32 len = (info & 31u) + kMinMatch;
+  if (len == kCap) len = match_len(d, p, p - off, seg_end(p, n));
+  return (len << 16) | off;
 }
 
 // Lazy parse of segment [a, b): at each position the precomputed best_at(p)
@@ -846,7 +920,7 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
 // bit 0 of out[0]).
 XZ_HD inline u32 finish_block(u8* out, const u8* raw, u32 n, const u8* lit, u32 nl,
                               const u32* hist, const Seq* seqs, u32 nseq, SeqCodes* codes,
-                              HufTable& h) {
+                              HufTable& h, Work& W, SeqWork& sw) {
   u8* body = out + 3;
   u32 ls = 0;
   bool rle = nl > 1;
@@ -854,14 +928,14 @@ XZ_HD inline u32 finish_block(u8* out, const u8* raw, u32 n, const u8* lit, u32 
   if (rle) {
     ls = write_raw_lits(body, lit, nl, 1);
   } else {
-    if (nl >= 64 && huf_build(hist, h)) {
+    if (nl >= 64 && huf_build(hist, h, W)) {
       u64 bits = 0;  // skip the encoding when the code cannot beat raw bytes
       for (u32 c = 0; c < 256; ++c) bits += (u64)hist[c] * h.nbits[c];
-      if (bits / 8 + 16 < nl) ls = write_huf_lits(body, lit, nl, h);
+      if (bits / 8 + 16 < nl) ls = write_huf_lits(body, lit, nl, h, W);
     }
     if (!ls) ls = write_raw_lits(body, lit, nl, 0);
   }
-  const u32 size = ls + write_sequences(body + ls, seqs, nseq, codes);
+  const u32 size = ls + write_sequences(body + ls, seqs, nseq, codes, sw, W);
   if (size >= n) {
     out[0] = (u8)(n << 3);
     out[1] = (u8)(n >> 5);
