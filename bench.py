@@ -15,8 +15,11 @@ oracle's is timed beside it ("exact_zstd_GBps"), as is the rate up to the
 assembled stream ("prezstd_GBps").
 
 The roofline entry is the kernel with the most device time, timed with HIP
-events on the library's own stream over the timed steps; its achieved GB/s
-uses the algorithmic bytes of one launch (see ALGO_BYTES).  cpu_baseline is
+events on the library's own stream over K more steps of the same work right
+after the timed ones (per-launch events keep stage 2 out of its captured
+graph, so they are not in the headline's steps); its achieved GB/s uses the
+algorithmic bytes of one launch (see ALGO_BYTES) and its traffic the HBM bytes
+rocprofv3's counters read for that kernel (see measured_traffic).  cpu_baseline is
 the CPU oracle (a single-threaded restatement of the reference) on a bounded
 sample of the same workload.
 """
@@ -88,8 +91,11 @@ class OutBuffer:
 
 
 def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launches: int,
-               out_bytes: int) -> float | None:
-    """Algorithmic bytes of ONE launch of `kernel` (reads + writes it must do)."""
+               out_bytes: int, dev_frame_bytes: int = 0) -> float | None:
+    """Algorithmic bytes of ONE launch of `kernel` (reads + writes it must do).
+    dev_frame_bytes: stream bytes whose frame blocks the device encodes (the
+    match finder must read each once; what it writes depends on the data and
+    is not counted, so its figure is a lower bound)."""
     per_pass_keys = n_points * key_bytes
     table = {
         "k_bbox": 12 * n_points,                      # read xyz
@@ -102,6 +108,7 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
         "k_rle_finish": 4 * n_coded + 8 * n_coded + 4 * n_coded,
         "k_stats": 8 * n_coded,                       # read sorted cells
         "k_sec_bits": 8 * n_coded,
+        "k_zparse": dev_frame_bytes,
     }
     return table.get(kernel)
 
@@ -118,6 +125,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary frame modes")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default run-time parameters")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
+                    help="end with one elementwise kernel of this size (counter scaling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -170,14 +181,27 @@ def main() -> int:
         return time.perf_counter() - t0, last
 
     head_mode = _lib.FRAME_ZSTD3_MT
-    for _ in range(args.warmup):
-        step(head_mode)
-    ctx.profile(True)
-    dt, (frame_len, st) = timed(head_mode, args.steps, 0)
-    prof = ctx.profile_report()
-    ctx.profile(False)
+    # Run-time parameters (persistent grid, host chunk size, the device's
+    # share of the frame, copy-back slice) tuned on this cloud first,
+    # outside the timed steps.
+    tuning = ctx.tuning(max(0, n.bit_length() - 1))
+    if not args.no_autotune:
+        t_at = time.perf_counter()
+        tuning, at_ms = X.autotune(dev, dict(o1[0], frame_mode=head_mode), reps=2, ctx=ctx)
+        print(f"[bench] autotune: {tuning} ({at_ms:.1f} ms/call) in "
+              f"{time.perf_counter() - t_at:.1f} s", file=sys.stderr, flush=True)
+    dt, (frame_len, st) = timed(head_mode, args.steps, args.warmup)
     value = 12.0 * n * args.steps / dt / 1e9
     print(f"[bench] headline: {value:.2f} GB/s ({dt / args.steps * 1e3:.1f} ms/step)",
+          file=sys.stderr, flush=True)
+    # Kernel times: the same K steps again with an event pair around every
+    # launch on the library's stream (stage 2 then runs eagerly, not from its
+    # captured graph, so each kernel is seen).
+    ctx.profile(True)
+    dt_prof, _ = timed(head_mode, args.steps, 0)
+    prof = ctx.profile_report()
+    ctx.profile(False)
+    print(f"[bench] profiled pass: {dt_prof / args.steps * 1e3:.1f} ms/step",
           file=sys.stderr, flush=True)
 
     extra = {}
@@ -200,13 +224,13 @@ def main() -> int:
         name, d = dom
         avg_ms = d["ms"] / d["calls"]
         ab = algo_bytes(name, n, st.num_points, key_bytes, d["calls"] // args.steps,
-                        st.prezstd_bytes)
+                        st.prezstd_bytes, dev_frame_bytes(st.prezstd_bytes, tuning))
         if ab is not None:
             ach = ab / (avg_ms * 1e-3) / 1e9
             roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
-                    "traffic": None, "avg_launch_ms": round(avg_ms, 4),
-                    "algo_bytes_per_launch": ab}
+                    "traffic": measured_traffic(name, args.workload, n),
+                    "avg_launch_ms": round(avg_ms, 4), "algo_bytes_per_launch": ab}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -238,7 +262,9 @@ def main() -> int:
         "config": {"workload": args.workload, "points": n,
                    "quantizer": "to" if w["quantizer"] == 1 else "cube", "l2_bound": w["l2"],
                    "curve": "hilbert" if w["hilbert"] else "z-order", "rle": w["rle"],
-                   "chunks": 1, "frame": "zstd L3, host worker threads"},
+                   "chunks": 1, "frame": "zstd L3, host worker threads",
+                   "tuning": {k: (round(v, 4) if isinstance(v, float) else v)
+                              for k, v in tuning.items()}},
         "ratio": round(12.0 * n / frame_len, 4),
         "prezstd_bytes": st.prezstd_bytes,
         "frame_bytes": frame_len,
@@ -249,7 +275,57 @@ def main() -> int:
     }
     print(json.dumps(out))
     ctx.close()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     return 0
+
+
+def dev_frame_bytes(total: int, tuning: dict) -> int:
+    """Stream bytes of a single host-threaded frame whose blocks the device
+    encodes under `tuning` (the split rule of the library's frame stage: the
+    device's share rounded up to whole chunks, at least 1 MiB, leaving the
+    host at least one chunk)."""
+    share = float(tuning["gpu_frame_share"])
+    chunk = 1 << int(tuning["zstd_chunk_log"])
+    if share <= 0:
+        return 0
+    import numpy as np
+    head = -(-int(total * float(np.float32(share))) // chunk) * chunk
+    if head < (1 << 20) or head + chunk > total:
+        return 0
+    return head
+
+
+def copy_probe(mib: int) -> None:
+    """One elementwise pass of known size (reads and writes `mib` MiB each):
+    the reference kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE to
+    bytes."""
+    import torch
+    a = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written by one neg kernel",
+          file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel: str, workload: str, n: int):
+    """HBM bytes per launch of `kernel` as measured with rocprofv3 counters
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy probe's reading) and
+    recorded in profiles/<kernel>_<workload>_traffic.md as a line
+    `traffic_bytes_per_launch: <kernel> <workload> <points> <bytes>`; None
+    when no such measurement exists for this kernel and size."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            p = line.strip().strip("`").split()
+            if (len(p) == 5 and p[0] == "traffic_bytes_per_launch:" and p[1] == kernel
+                    and p[2] == workload and int(p[3]) == n):
+                return int(p[4])
+    return None
 
 
 def _uneven_clouds(count):

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define XNYZ_ABI_VERSION 2
+#define XNYZ_ABI_VERSION 3
 
 enum xnyz_status {
   XNYZ_OK = 0,
@@ -34,7 +34,8 @@ enum xnyz_status {
   XNYZ_EUNSUPPORTED = -5, /* input outside the implemented range (see msg) */
   XNYZ_ECORRUPT = -6,     /* malformed compressed stream                    */
   XNYZ_EINTERNAL = -7,    /* device-side invariant violated                 */
-  XNYZ_ENOSPC = -8        /* caller's output buffer too small               */
+  XNYZ_ENOSPC = -8,       /* caller's output buffer too small               */
+  XNYZ_EIO = -9           /* a file could not be read or written            */
 };
 
 /* QUANTIZER_TYPE (source/composer/utils.hpp:55-59); ADAPTIVE is not
@@ -49,7 +50,7 @@ enum xnyz_quantizer { XNYZ_CUBE = 0, XNYZ_TRUNCATED_OCTAHEDRON = 1 };
  * decompress_buffer, but not byte-identical to the single-shot frame.
  * XNYZ_FRAME_NONE returns only the assembled (pre-zstd) stream.
  * XNYZ_FRAME_ZSTD_GPU builds the frame on the device with this library's
- * own zstd-format encoder (64 KiB blocks: LZ matches, Huffman literals, FSE
+ * own zstd-format encoder (16 KiB blocks: LZ matches, Huffman literals, FSE
  * sequences): a valid single frame with the content size for any zstd
  * decoder, about 2% larger than level 3 on point-cloud streams, and the
  * assembled stream never leaves device memory. */
@@ -283,6 +284,51 @@ int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
  * shape and replayed while the shape repeats (same cloud sizes and options
  * class); on by default, off while profiling.  No reference counterpart. */
 int xnyz_graphs_enable(xnyz_ctx* ctx, int on);
+
+/* Run-time parameters of a context, kept per size class (class c holds the
+ * calls with 2^c <= total points < 2^(c+1); c in [0, 32]).  No reference
+ * counterpart: the reference's comparable knobs are compile-time constants
+ * or absent (one ZSTD_compress call, source/composer/utils.hpp:35-53).
+ *   zstd_chunk_log   host-threaded frames: log2 bytes per chunk, 16..24 (20)
+ *   copy_slice_log   copy-back of the stream in slices of 2^x bytes, 20..30 (24)
+ *   gpu_frame_share  share of a single cloud's host-threaded frame whose
+ *                    blocks the device encodes (the frame's tail, while the
+ *                    host threads compress the head), 0..0.9 (0)
+ *   persist_blocks   workgroups of the persistent reduction kernels,
+ *                    64..16384 (2048)
+ *   copy_first       split frames: 1 = the head's copy-back runs before the
+ *                    device encoder on the context's stream, 0 = on a second
+ *                    stream beside it (1)
+ * The defaults are given in parentheses.  Every setting yields a valid frame
+ * of the same stream; only the frame bytes and the speed change. */
+typedef struct xnyz_tuning {
+  uint32_t zstd_chunk_log;
+  uint32_t copy_slice_log;
+  float gpu_frame_share;
+  uint32_t persist_blocks;
+  uint32_t copy_first;
+} xnyz_tuning;
+
+#define XNYZ_SIZE_CLASSES 33
+
+/* size_class -1 sets every class.  No reference counterpart. */
+int xnyz_tuning_set(xnyz_ctx* ctx, int size_class, const xnyz_tuning* t);
+int xnyz_tuning_get(xnyz_ctx* ctx, int size_class, xnyz_tuning* t);
+
+/* The table as text, one line per class ("class chunk_log slice_log share
+ * blocks copy_first"); load accepts any subset of the classes.  No reference
+ * counterpart. */
+int xnyz_tuning_save(xnyz_ctx* ctx, const char* path);
+int xnyz_tuning_load(xnyz_ctx* ctx, const char* path);
+
+/* Autotuner: times compressions of the device-resident cloud (n points,
+ * options `opts`, frames into scratch owned by the context) over candidate
+ * settings, one parameter at a time with the others held at their best so
+ * far, `reps` timed calls per candidate after one untimed call, and stores
+ * the fastest setting as the tuning of n's size class.  *best gets it,
+ * *best_ms its mean wall time per call.  No reference counterpart. */
+int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
+                  uint32_t reps, xnyz_tuning* best, double* best_ms);
 
 /* Per-kernel device time (HIP events on the context's stream) accumulated
  * over the calls made while enabled; enabling clears the totals.  The report

@@ -24,7 +24,7 @@ from ._lib import Context, XnyzError  # noqa: F401
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
            "compress", "compress_batch", "compress_batch_into", "search_batch", "features_batch", "mlp_train",
-           "mlp_predict", "compress_selected_batch", "quantize", "morton_to_hilbert"]
+           "mlp_predict", "compress_selected_batch", "quantize", "morton_to_hilbert", "autotune"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -173,6 +173,29 @@ def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
         C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p))
     _lib.check(rc)
     return res, list(stats)
+
+
+def autotune(points_device, opts: dict, reps: int = 2, ctx: Context | None = None):
+    """Time compressions of one device-resident cloud over candidate run-time
+    parameters (persistent grid; for FRAME_ZSTD3_MT also the host chunk size,
+    the device's share of the frame and the copy-back slice) and keep the
+    fastest as the context's tuning for the cloud's size class.  Returns
+    (tuning dict, mean ms per call)."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    ctx = ctx or default_context(t.device.index or 0)
+    o = _opts(opts["quantizer"], opts["l2_bound"], opts.get("is_hilbert", False),
+              opts.get("rle", False), opts.get("direct_threshold", 1024),
+              opts.get("frame_mode", _lib.FRAME_ZSTD3_MT), opts.get("zstd_workers", 0),
+              opts.get("gr_k"), opts.get("pfd_width"))
+    best = _lib.Tuning()
+    ms = C.c_double()
+    torch.cuda.synchronize(t.device)
+    _lib.check(_lib.lib().xnyz_autotune(ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0],
+                                        C.byref(o), int(reps), C.byref(best), C.byref(ms)))
+    return best.as_dict(), ms.value
 
 
 def search_batch(points_device, offsets, opts, ctx: Context | None = None):

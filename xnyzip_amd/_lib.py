@@ -13,7 +13,9 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "lib", "libxnyz.so")
 
 XNYZ_OK = 0
+XNYZ_EINVAL = -1
 XNYZ_ENOSPC = -8
+XNYZ_EIO = -9
 CUBE, TRUNCATED_OCTAHEDRON = 0, 1
 FRAME_ZSTD3, FRAME_ZSTD3_MT, FRAME_NONE, FRAME_ZSTD_GPU = 0, 1, 2, 3
 
@@ -55,6 +57,19 @@ class SearchResult(C.Structure):
                 ("variant", Mode * 4), ("variant_bytes", C.c_uint64 * 4)]
 
 
+class Tuning(C.Structure):
+    """Run-time parameters of a context for one size class (include/xnyz.h)."""
+    _fields_ = [("zstd_chunk_log", C.c_uint32), ("copy_slice_log", C.c_uint32),
+                ("gpu_frame_share", C.c_float), ("persist_blocks", C.c_uint32),
+                ("copy_first", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SIZE_CLASSES = 33
+
+
 class Buf(C.Structure):
     _fields_ = [("data", C.c_void_p), ("len", C.c_uint64)]
 
@@ -87,8 +102,9 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_quantize_device",
-           "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_graphs_enable", "xnyz_profile_enable",
-           "xnyz_profile_report", "xnyz_free"]
+           "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_graphs_enable",
+           "xnyz_tuning_set", "xnyz_tuning_get", "xnyz_tuning_save", "xnyz_tuning_load",
+           "xnyz_autotune", "xnyz_profile_enable", "xnyz_profile_report", "xnyz_free"]
 
 
 def lib() -> C.CDLL:
@@ -123,6 +139,12 @@ def lib() -> C.CDLL:
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]
         L.xnyz_profile_enable.argtypes = [P, C.c_int]
         L.xnyz_graphs_enable.argtypes = [P, C.c_int]
+        L.xnyz_tuning_set.argtypes = [P, C.c_int, C.POINTER(Tuning)]
+        L.xnyz_tuning_get.argtypes = [P, C.c_int, C.POINTER(Tuning)]
+        L.xnyz_tuning_save.argtypes = [P, C.c_char_p]
+        L.xnyz_tuning_load.argtypes = [P, C.c_char_p]
+        L.xnyz_autotune.argtypes = [P, P, C.c_uint64, C.POINTER(Opts), C.c_uint32,
+                                    C.POINTER(Tuning), C.POINTER(C.c_double)]
         L.xnyz_profile_report.argtypes = [P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.xnyz_free.argtypes = [C.POINTER(Buf)]
         _lib = L
@@ -168,6 +190,29 @@ class Context:
     def graphs(self, on: bool) -> None:
         """Replay the stage-2 launch sequence from a captured hipGraph (default on)."""
         check(lib().xnyz_graphs_enable(self._h, int(on)))
+
+    def tuning(self, size_class: int) -> dict:
+        """Run-time parameters used for calls of 2^c <= points < 2^(c+1)."""
+        t = Tuning()
+        check(lib().xnyz_tuning_get(self._h, int(size_class), C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, size_class: int = -1, **fields) -> None:
+        """Change fields of one size class's parameters (-1: every class,
+        starting from class 0's values)."""
+        t = Tuning()
+        check(lib().xnyz_tuning_get(self._h, max(0, int(size_class)), C.byref(t)))
+        for k, v in fields.items():
+            if k not in dict(Tuning._fields_):
+                raise ValueError(f"unknown tuning field {k}")
+            setattr(t, k, v)
+        check(lib().xnyz_tuning_set(self._h, int(size_class), C.byref(t)))
+
+    def save_tuning(self, path: str) -> None:
+        check(lib().xnyz_tuning_save(self._h, os.fsencode(path)))
+
+    def load_tuning(self, path: str) -> None:
+        check(lib().xnyz_tuning_load(self._h, os.fsencode(path)))
 
     def profile(self, on: bool) -> None:
         """Enable (and clear) or disable per-kernel event timing."""

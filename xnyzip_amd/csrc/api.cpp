@@ -273,11 +273,15 @@ struct xnyz_ctx {
   // selector: parameters, FP16 weights, optimiser state, samples
   DBuf mP, mWh, mG, mM, mV, mX, mY, mIdx, mScratch, mLoss, mOut, mConf;
   // pinned host mirrors
-  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat, h_zout;
+  HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat, h_zout, h_ztab, h_zpos;
   bool prof_on = false;
   EventProfiler prof;
   FrameState* fs = nullptr;  // frame stage (created on first use)
   bool graphs = true;        // replay stage 2 from a captured graph
+  std::vector<hipEvent_t> slice_ev;  // copy-back slices (frame overlap)
+  hipStream_t copy_stream = nullptr;  // copy-back beside device-built frame blocks
+  hipEvent_t ev_pipe = nullptr;       // end of the stream-building work
+  xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
 
   void sync() {
@@ -366,16 +370,47 @@ void parallel_for_w(size_t n, int threads, const std::function<void(size_t, size
     if (e) std::rethrow_exception(e);
 }
 
-constexpr size_t kZChunk = 1u << 20;  // host-threaded frames: bytes per chunk
-
-size_t zchunk_bound() { return ZSTD_compressBound(kZChunk) + 3 * (kZChunk / 1024 + 2); }
+// Host-threaded frames: bound of one compressed chunk of `chunk` bytes
+// (its blocks, each with a 3-byte header).
+size_t zchunk_bound(size_t chunk) { return ZSTD_compressBound(chunk) + 3 * (chunk / 1024 + 2); }
 
 // Scratch bytes the frame of `len` bytes needs in `mode` with `threads`.
-u64 frame_scratch(u64 len, int mode, int threads) {
+u64 frame_scratch(u64 len, int mode, int threads, u64 chunk) {
   if (mode == XNYZ_FRAME_NONE || mode == XNYZ_FRAME_ZSTD_GPU) return 0;
-  const u64 nchunks = (len + kZChunk - 1) / kZChunk;
-  if (mode == XNYZ_FRAME_ZSTD3_MT && nchunks > 1 && threads > 1) return 16 + nchunks * zchunk_bound();
+  const u64 nchunks = (len + chunk - 1) / chunk;
+  if (mode == XNYZ_FRAME_ZSTD3_MT && nchunks > 1 && threads > 1)
+    return 16 + nchunks * zchunk_bound(chunk);
   return ZSTD_compressBound(len);
+}
+
+// ---- run-time parameters (xnyz_tuning) ---------------------------------------
+
+xnyz_tuning default_tuning() { return xnyz_tuning{20, 24, 0.f, 2048, 1}; }
+
+// Size class of a call: floor(log2(total points)), 0 for 0 or 1 point.
+int size_class(u64 n) { return n <= 1 ? 0 : 63 - __builtin_clzll(n); }
+
+// Split frames: bytes of a stream of `total` bytes whose blocks the device
+// encodes (the head), or 0 when the stream is too short to split: the head
+// is the device's share rounded up to whole chunks, at least 1 MiB, and
+// leaves the host at least one chunk.
+u64 split_head(u64 total, const xnyz_tuning& t, u64 chunk) {
+  u64 h = (u64)((double)total * (double)t.gpu_frame_share);
+  h = (h + chunk - 1) / chunk * chunk;
+  if (h < (1u << 20) || h + chunk > total) return 0;
+  return h;
+}
+
+void check_tuning(const xnyz_tuning& t) {
+  if (t.zstd_chunk_log < 16 || t.zstd_chunk_log > 24)
+    throw Error(XNYZ_EINVAL, "zstd_chunk_log must be in [16, 24]");
+  if (t.copy_slice_log < 20 || t.copy_slice_log > 30)
+    throw Error(XNYZ_EINVAL, "copy_slice_log must be in [20, 30]");
+  if (!(t.gpu_frame_share >= 0.f && t.gpu_frame_share <= 0.9f))
+    throw Error(XNYZ_EINVAL, "gpu_frame_share must be in [0, 0.9]");
+  if (t.persist_blocks < 64 || t.persist_blocks > 16384)
+    throw Error(XNYZ_EINVAL, "persist_blocks must be in [64, 16384]");
+  if (t.copy_first > 1) throw Error(XNYZ_EINVAL, "copy_first must be 0 or 1");
 }
 
 }  // namespace
@@ -409,36 +444,46 @@ Pieces zstd_single(ZSTD_CCtx* cc, const uint8_t* src, u64 len, uint8_t* dst, u64
   return {Piece{dst, r}};
 }
 
-// One zstd frame, level 3, whose 1 MiB chunks are compressed on host threads
-// with the block-level API: single-segment frame header with the content
-// size, then each chunk's blocks (compressed, or raw when zstd declines).
-// Chunks after the first start without repeat offsets, so every chunk
-// decodes after any other.  Valid for every zstd decoder; not byte-identical
-// to the single-shot frame.
+// One zstd frame, level 3, whose chunks (`chunk` bytes each) are compressed
+// on host threads with the block-level API: single-segment frame header with
+// the content size `fcs`, then each chunk's blocks (compressed, or raw when
+// zstd declines).  Chunks after the first start without repeat offsets, so
+// every chunk decodes after any other.  Valid for every zstd decoder; not
+// byte-identical to the single-shot frame.
 // `own` compresses the frame when it is not split (one chunk or one thread).
+// `ready` (nullable) is called with a chunk's end offset before the chunk is
+// read: the source may still be arriving from the device.
+// With `header` false, only the blocks of [0, len) are written (the last one
+// flagged): the tail of a frame whose header and first blocks come from
+// elsewhere.
 Pieces zstd_chunked(FrameState& fs, ZSTD_CCtx* own, const uint8_t* src, u64 len, int threads,
-                    uint8_t* dst) {
-  const size_t nchunks = (len + kZChunk - 1) / kZChunk;
-  if (nchunks <= 1 || threads <= 1) return zstd_single(own, src, len, dst, ZSTD_compressBound(len));
-  const int t = std::min<int>(threads, (int)nchunks);
+                    uint8_t* dst, u64 chunk, const std::function<void(u64)>* ready, u64 fcs,
+                    bool header) {
+  const size_t nchunks = (len + chunk - 1) / chunk;
+  if (header && (nchunks <= 1 || threads <= 1)) {
+    if (ready) (*ready)(len);
+    return zstd_single(own, src, len, dst, ZSTD_compressBound(len));
+  }
+  const int t = std::max(1, std::min<int>(threads, (int)nchunks));
   fs.ensure_workers(t);  // contexts are created before the workers start
-  const size_t cb = zchunk_bound();
+  const size_t cb = zchunk_bound(chunk);
   uint8_t* hdr = dst;  // magic, descriptor (single segment, 8-byte content size), size
   const uint32_t magic = 0xFD2FB528u;
   std::memcpy(hdr, &magic, 4);
   hdr[4] = 0xE0;
-  const uint64_t fcs = len;
-  std::memcpy(hdr + 5, &fcs, 8);
+  const uint64_t fcs64 = fcs;
+  std::memcpy(hdr + 5, &fcs64, 8);
   Pieces out(nchunks + 1);
-  out[0] = Piece{hdr, 13};
+  out[0] = Piece{hdr, header ? 13u : 0u};
   uint8_t* parts = dst + 16;
   parallel_for_w(nchunks, t, [&](size_t c, size_t w) {
     ZSTD_CCtx* cc = fs.cctx[w];
-    const size_t off = c * kZChunk, n = std::min<size_t>(kZChunk, len - off);
+    const size_t off = c * chunk, n = std::min<size_t>(chunk, len - off);
+    if (ready) (*ready)(off + n);
     uint8_t* o = parts + c * cb;
     size_t r = ZSTD_compressBegin(cc, 3);
     if (ZSTD_isError(r)) throw Error(XNYZ_EZSTD, ZSTD_getErrorName(r));
-    if (c) ZSTD_invalidateRepCodes(cc);
+    if (c || !header) ZSTD_invalidateRepCodes(cc);
     const size_t bs = ZSTD_getBlockSize(cc);
     size_t at = 0;
     for (size_t b = 0; b < n; b += bs) {
@@ -667,6 +712,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   hipStream_t st = ctx->stream;
   ProfGuard prof_guard(ctx);
   const int threads = host_threads(0);
+  const xnyz_tuning tu = ctx->tuning[size_class(N64)];
 
   // ---- tiles and segment parameters ----------------------------------------
   u32 ntiles = 0;
@@ -730,6 +776,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   bt.large = ctx->large.as<u32>();
   bt.n_large = ctx->nlarge.as<u32>();
   bt.large_cap = large_cap;
+  bt.grid_cap = tu.persist_blocks;
 
   // ---- stage 1: bbox minimum, quantize, range scan ----------------------------
   launch_init(bt, st);
@@ -1072,7 +1119,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
                reinterpret_cast<uint8_t*>(ow), st);
   write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, st);
 
-  // ---- frames built on the device (XNYZ_FRAME_ZSTD_GPU) ----------------------
+  // ---- frame stage -------------------------------------------------------------
   const bool want_out = co.frames || co.dst;
   bool host_pre = co.pre != nullptr;
   std::vector<u32> gsegs;  // segments framed on the device, in order
@@ -1080,54 +1127,131 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     if (want_out && opts[s].frame_mode == XNYZ_FRAME_ZSTD_GPU) gsegs.push_back(s);
     else if (want_out) host_pre = true;
   }
+  const bool per_seg_threads = S > 1;
+  const u64 chunk = 1ull << tu.zstd_chunk_log;
+  // One cloud framed on host threads: its stream comes back in slices, each
+  // followed by an event, and a chunk is compressed as soon as its slice has
+  // landed -- the copy overlaps the frame stage.
+  const bool overlap = host_pre && S == 1 && !co.pre && opts[0].frame_mode == XNYZ_FRAME_ZSTD3_MT;
+  // Split frame: the device encodes the stream's head [0, head) -- the
+  // sections zstd spends the most time per byte on -- as the frame's first
+  // blocks while the host threads compress the tail; only the tail is copied
+  // back.
+  u64 head = 0;
+  if (overlap && tu.gpu_frame_share > 0.f) head = split_head(g[0].total, tu, chunk);
+  const bool split = head > 0;
+  hipStream_t cst = st;  // stream of the copy-back
+  if (split && !tu.copy_first) {
+    if (!ctx->copy_stream)
+      check_hip(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (!ctx->ev_pipe)
+      check_hip(hipEventCreateWithFlags(&ctx->ev_pipe, hipEventDisableTiming), "hipEventCreate");
+    cst = ctx->copy_stream;
+    check_hip(hipEventRecord(ctx->ev_pipe, st), "hipEventRecord");
+    check_hip(hipStreamWaitEvent(cst, ctx->ev_pipe, 0), "hipStreamWaitEvent");
+  }
+  struct Drain {  // an error in the frame stage leaves no copy or kernel in flight
+    hipStream_t a, b;
+    bool on;
+    ~Drain() {
+      if (!on) return;
+      (void)hipStreamSynchronize(a);
+      if (b != a) (void)hipStreamSynchronize(b);
+    }
+  } drain{st, cst, overlap};
   uint8_t* hb = nullptr;
+  std::function<void(u64)> ready;
   if (host_pre) {
     ctx->h_out.ensure(out_total + 64);
     hb = ctx->h_out.as<uint8_t>();
-    check_hip(hipMemcpyAsync(hb, ow, out_total, hipMemcpyDeviceToHost, st), "stream D2H");
+    if (overlap) {
+      // Copied: [c0, c1) of the stream buffer; a chunk ending at `end`
+      // (relative to c0) waits for the slice holding byte end - 1.
+      const u64 slice = 1ull << tu.copy_slice_log;
+      const u64 c0 = g[0].base + head, c1 = g[0].base + g[0].total;
+      const u64 nsl = (c1 - c0 + slice - 1) / slice;
+      while (ctx->slice_ev.size() < nsl) {
+        hipEvent_t e;
+        check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        ctx->slice_ev.push_back(e);
+      }
+      for (u64 i = 0; i < nsl; ++i) {
+        const u64 a = c0 + i * slice, m = std::min(slice, c1 - a);
+        check_hip(hipMemcpyAsync(hb + a, reinterpret_cast<const uint8_t*>(ow) + a, m,
+                                 hipMemcpyDeviceToHost, cst), "stream D2H");
+        check_hip(hipEventRecord(ctx->slice_ev[i], cst), "hipEventRecord");
+      }
+      ready = [ctx, slice](u64 end) {
+        check_hip(hipEventSynchronize(ctx->slice_ev[(end - 1) / slice]), "hipEventSynchronize");
+      };
+    } else {
+      check_hip(hipMemcpyAsync(hb, ow, out_total, hipMemcpyDeviceToHost, st), "stream D2H");
+    }
     ctx->copied();
   }
+
+  // Frames built on the device: whole XNYZ_FRAME_ZSTD_GPU segments, or the
+  // head of a split frame (its header and last block are patched when
+  // assembling).
+  struct DevFrame {
+    u64 src, len;
+  };
+  std::vector<DevFrame> dfr;
+  for (u32 s : gsegs) dfr.push_back({g[s].base, g[s].total});
+  if (split) dfr.push_back({g[0].base, head});
   std::vector<ZFrame> zf;
-  std::vector<u64> zpos;
+  const u64* zpos = nullptr;  // block positions in the device-built frames (pinned)
   uint8_t* zb = nullptr;
-  if (!gsegs.empty()) {
-    std::vector<ZBlk> zk;
+  u32 znblk = 0;
+  if (!dfr.empty()) {
+    // Block and frame tables staged in pinned memory: the copies stay
+    // asynchronous, so the host is not held until the stream drains.
+    u64 nb = 0;
+    for (const DevFrame& d : dfr) nb += (d.len + xz::kBlock - 1) / xz::kBlock;
+    znblk = (u32)nb;
+    const size_t tb = (nb * sizeof(ZBlk) + 255) & ~size_t(255);
+    const size_t tbytes = tb + dfr.size() * sizeof(ZFrame);
+    ctx->h_ztab.ensure(tbytes);
+    ZBlk* zk = ctx->h_ztab.as<ZBlk>();
     u64 bound = 0;
-    for (u32 f = 0; f < (u32)gsegs.size(); ++f) {
-      const SegHost& h = g[gsegs[f]];
-      const u32 first = (u32)zk.size();
-      for (u64 o = 0; o < h.total; o += xz::kBlock)
-        zk.push_back(ZBlk{h.base + o, (u32)std::min<u64>(xz::kBlock, h.total - o), f});
-      zf.push_back(ZFrame{h.total, first, (u32)zk.size() - first});
-      bound += 13 + h.total + 3 * (zk.size() - first);
+    u32 k = 0;
+    for (u32 f = 0; f < (u32)dfr.size(); ++f) {
+      const u32 first = k;
+      for (u64 o = 0; o < dfr[f].len; o += xz::kBlock)
+        zk[k++] = ZBlk{dfr[f].src + o, (u32)std::min<u64>(xz::kBlock, dfr[f].len - o), f};
+      zf.push_back(ZFrame{dfr[f].len, first, k - first});
+      bound += 13 + dfr[f].len + 3 * (k - first);
     }
-    const u32 nblk = (u32)zk.size();
-    const size_t tb = (zk.size() * sizeof(ZBlk) + 255) & ~size_t(255);
-    ctx->ztab.ensure(tb + zf.size() * sizeof(ZFrame) + 256);
-    check_hip(hipMemcpyAsync(ctx->ztab.p, zk.data(), zk.size() * sizeof(ZBlk),
-                             hipMemcpyHostToDevice, st), "H2D");
-    check_hip(hipMemcpyAsync(ctx->ztab.as<uint8_t>() + tb, zf.data(), zf.size() * sizeof(ZFrame),
-                             hipMemcpyHostToDevice, st), "H2D");
+    std::memcpy(ctx->h_ztab.as<uint8_t>() + tb, zf.data(), zf.size() * sizeof(ZFrame));
+    ctx->ztab.ensure(tbytes + 256);
+    check_hip(hipMemcpyAsync(ctx->ztab.p, ctx->h_ztab.p, tbytes, hipMemcpyHostToDevice, st),
+              "H2D");
     ctx->copied();
-    ctx->zscr.ensure(zframe_scratch_bytes(nblk));
+    ctx->zscr.ensure(zframe_scratch_bytes(znblk));
     ctx->zout.ensure(bound + 64);
-    ctx->zpos.ensure((nblk + 1) * sizeof(u64) + 64);
-    zframe_encode(reinterpret_cast<const uint8_t*>(ow), ctx->ztab.as<ZBlk>(), nblk,
+    ctx->zpos.ensure((znblk + 1) * sizeof(u64) + 64);
+    zframe_encode(reinterpret_cast<const uint8_t*>(ow), ctx->ztab.as<ZBlk>(), znblk,
                   reinterpret_cast<const ZFrame*>(ctx->ztab.as<uint8_t>() + tb), ctx->zscr.p,
                   ctx->zout.as<uint8_t>(), ctx->zpos.as<u64>(), ar, st);
-    zpos.resize(nblk + 1);
-    check_hip(hipMemcpyAsync(zpos.data(), ctx->zpos.p, (nblk + 1) * sizeof(u64),
+    ctx->h_zpos.ensure((znblk + 1) * sizeof(u64));
+    zpos = ctx->h_zpos.as<u64>();
+    check_hip(hipMemcpyAsync(ctx->h_zpos.p, ctx->zpos.p, (znblk + 1) * sizeof(u64),
                              hipMemcpyDeviceToHost, st), "D2H");
-    ctx->sync();
-    const u64 total = zpos[nblk] + 13 * zf.size();
+  }
+  // Brings the device-built frames to host memory (after the host frames
+  // when the frame is split, so the two run side by side).
+  auto fetch_dev_frames = [&] {
+    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const u64 total = zpos[znblk] + 13 * zf.size();
     ctx->h_zout.ensure(total + 64);
     zb = ctx->h_zout.as<uint8_t>();
     check_hip(hipMemcpyAsync(zb, ctx->zout.p, total, hipMemcpyDeviceToHost, st), "frames D2H");
-  }
-  ctx->sync();
+    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  };
+  if (!dfr.empty() && !split) fetch_dev_frames();
+  if (!overlap) ctx->sync();
 
   const double t_dev = now_ms();
-  const bool per_seg_threads = S > 1;
   if (co.pre) {
     parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
       co.pre[s].data = xmalloc(g[s].total);
@@ -1143,7 +1267,11 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     for (u32 s = 0; s < S; ++s) {
       const xnyz_opts& o = opts[s];
       zthreads[s] = per_seg_threads ? 1 : host_threads(o.zstd_workers > 0 ? o.zstd_workers : threads);
-      zoff[s + 1] = zoff[s] + ((frame_scratch(g[s].total, o.frame_mode, zthreads[s]) + 63) & ~u64(63));
+      // A split frame's tail always takes the chunked block path.
+      const u64 need = (split && s == 0)
+                           ? 16 + (g[s].total - head + chunk - 1) / chunk * zchunk_bound(chunk)
+                           : frame_scratch(g[s].total, o.frame_mode, zthreads[s], chunk);
+      zoff[s + 1] = zoff[s] + ((need + 63) & ~u64(63));
     }
     if (fs.buf.size() < zoff[S]) fs.buf.resize(zoff[S]);
     // Every worker's context exists before the workers start (no resizing
@@ -1163,10 +1291,22 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
           break;
         }
         case XNYZ_FRAME_NONE: outs[s] = {Piece{b, len}}; break;
-        case XNYZ_FRAME_ZSTD3_MT: outs[s] = zstd_chunked(fs, fs.cctx[w], b, len, zthreads[s], z); break;
+        case XNYZ_FRAME_ZSTD3_MT:
+          outs[s] = split ? zstd_chunked(fs, fs.cctx[w], b + head, len - head, zthreads[s], z,
+                                         chunk, &ready, 0, false)
+                          : zstd_chunked(fs, fs.cctx[w], b, len, zthreads[s], z, chunk,
+                                         overlap ? &ready : nullptr, len, true);
+          break;
         default: outs[s] = zstd_single(fs.cctx[w], b, len, z, zoff[s + 1] - zoff[s]);
       }
     });
+    if (split) {  // the device-built blocks open the frame
+      fetch_dev_frames();
+      const uint64_t fcs = g[0].total;  // the header covers the whole stream
+      std::memcpy(zb + 5, &fcs, 8);
+      zb[zpos[znblk - 1] + 13] &= 0xFE;  // the host's blocks follow the last one
+      outs[0].insert(outs[0].begin(), Piece{zb, 13 + zpos[znblk]});
+    }
     if (co.frames) {
       parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
         const u64 n = pieces_size(outs[s]);
@@ -1185,6 +1325,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       });
     }
   }
+  if (overlap && cst != st) check_hip(hipStreamSynchronize(cst), "hipStreamSynchronize");
+  if (overlap) ctx->sync();
   const double t_end = now_ms();
   if (co.stats) {
     for (u32 s = 0; s < S; ++s) {
@@ -1312,6 +1454,7 @@ int xnyz_create(int device, xnyz_ctx** out) {
     if (device < 0 || device >= count) throw Error(XNYZ_EINVAL, "no such HIP device");
     auto* c = new xnyz_ctx();
     c->device = device;
+    for (xnyz_tuning& t : c->tuning) t = default_tuning();
     try {
       use_device(c);
       check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1340,10 +1483,16 @@ int xnyz_destroy(xnyz_ctx* ctx) {
                     &ctx->zscr, &ctx->zout, &ctx->zpos})
       b->release();
     for (HBuf* b : {&ctx->h_tiles, &ctx->h_sp, &ctx->h_sc, &ctx->h_sh, &ctx->h_lut, &ctx->h_out,
-                    &ctx->h_feat, &ctx->h_zout})
+                    &ctx->h_feat, &ctx->h_zout, &ctx->h_ztab, &ctx->h_zpos})
       b->release();
     ctx->prof.destroy();
     ctx->stage2.clear();
+    for (hipEvent_t e : ctx->slice_ev) (void)hipEventDestroy(e);
+    if (ctx->ev_pipe) (void)hipEventDestroy(ctx->ev_pipe);
+    if (ctx->copy_stream) {
+      (void)hipStreamSynchronize(ctx->copy_stream);
+      (void)hipStreamDestroy(ctx->copy_stream);
+    }
     delete ctx->fs;
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1730,6 +1879,155 @@ int xnyz_graphs_enable(xnyz_ctx* ctx, int on) {
     if (!ctx->graphs) {
       use_device(ctx);
       ctx->stage2.clear();
+    }
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_tuning_set(xnyz_ctx* ctx, int size_class, const xnyz_tuning* t) {
+  return guarded([&] {
+    if (!ctx || !t) throw Error(XNYZ_EINVAL, "null argument");
+    if (size_class < -1 || size_class >= XNYZ_SIZE_CLASSES)
+      throw Error(XNYZ_EINVAL, "size class out of range");
+    check_tuning(*t);
+    if (size_class < 0) {
+      for (xnyz_tuning& x : ctx->tuning) x = *t;
+    } else {
+      ctx->tuning[size_class] = *t;
+    }
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_tuning_get(xnyz_ctx* ctx, int size_class, xnyz_tuning* t) {
+  return guarded([&] {
+    if (!ctx || !t) throw Error(XNYZ_EINVAL, "null argument");
+    if (size_class < 0 || size_class >= XNYZ_SIZE_CLASSES)
+      throw Error(XNYZ_EINVAL, "size class out of range");
+    *t = ctx->tuning[size_class];
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_tuning_save(xnyz_ctx* ctx, const char* path) {
+  return guarded([&] {
+    if (!ctx || !path) throw Error(XNYZ_EINVAL, "null argument");
+    FILE* f = std::fopen(path, "w");
+    if (!f) throw Error(XNYZ_EIO, std::string("cannot write ") + path);
+    std::fprintf(f,
+                 "# class zstd_chunk_log copy_slice_log gpu_frame_share persist_blocks copy_first\n");
+    for (int c = 0; c < XNYZ_SIZE_CLASSES; ++c) {
+      const xnyz_tuning& t = ctx->tuning[c];
+      std::fprintf(f, "%d %u %u %.4f %u %u\n", c, t.zstd_chunk_log, t.copy_slice_log,
+                   (double)t.gpu_frame_share, t.persist_blocks, t.copy_first);
+    }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) throw Error(XNYZ_EIO, std::string("cannot write ") + path);
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_tuning_load(xnyz_ctx* ctx, const char* path) {
+  return guarded([&] {
+    if (!ctx || !path) throw Error(XNYZ_EINVAL, "null argument");
+    FILE* f = std::fopen(path, "r");
+    if (!f) throw Error(XNYZ_EIO, std::string("cannot read ") + path);
+    std::vector<std::pair<int, xnyz_tuning>> rows;
+    char line[256];
+    int lineno = 0;
+    bool bad = false;
+    while (std::fgets(line, sizeof(line), f)) {
+      ++lineno;
+      const char* p = line;
+      while (*p == ' ' || *p == '\t') ++p;
+      if (*p == '#' || *p == '\n' || *p == 0) continue;
+      int c;
+      xnyz_tuning t;
+      if (std::sscanf(p, "%d %u %u %f %u %u", &c, &t.zstd_chunk_log, &t.copy_slice_log,
+                      &t.gpu_frame_share, &t.persist_blocks, &t.copy_first) != 6 ||
+          c < 0 || c >= XNYZ_SIZE_CLASSES) {
+        bad = true;
+        break;
+      }
+      rows.emplace_back(c, t);
+    }
+    std::fclose(f);
+    if (bad) throw Error(XNYZ_EINVAL, std::string(path) + ":" + std::to_string(lineno) +
+                                          ": expected 'class chunk_log slice_log share blocks copy_first'");
+    for (const auto& r : rows) check_tuning(r.second);  // all or nothing
+    for (const auto& r : rows) ctx->tuning[r.first] = r.second;
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
+                  uint32_t reps, xnyz_tuning* best, double* best_ms) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !opts || !n) throw Error(XNYZ_EINVAL, "null argument or empty cloud");
+    if (reps == 0 || reps > 1000) throw Error(XNYZ_EINVAL, "reps must be in [1, 1000]");
+    validate(opts, n);
+    use_device(ctx);
+    const int c = size_class(n);
+    const std::vector<u64> off{0, n};
+    std::vector<uint8_t> out(13 * (size_t)n + (1u << 20));
+    uint64_t out_off[2];
+    auto call = [&] {
+      for (int attempt = 0;; ++attempt) {
+        CallOut co{nullptr, nullptr, nullptr, nullptr};
+        co.dst = out.data();
+        co.cap = out.size();
+        co.dst_off = out_off;
+        try {
+          compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+          return;
+        } catch (const Error& e) {
+          if (e.code != XNYZ_ENOSPC || attempt) throw;
+          out.resize(out_off[1]);
+        }
+      }
+    };
+    const xnyz_tuning saved = ctx->tuning[c];
+    auto timed = [&](const xnyz_tuning& t) {
+      ctx->tuning[c] = t;
+      call();  // untimed: buffers, captured graphs and host pages settle
+      const double t0 = now_ms();
+      for (u32 r = 0; r < reps; ++r) call();
+      return (now_ms() - t0) / reps;
+    };
+    try {
+      xnyz_tuning cur = saved;
+      double cur_ms = timed(cur);
+      auto sweep = [&](auto field, std::initializer_list<double> values) {
+        for (double v : values) {
+          xnyz_tuning t = cur;
+          field(t, v);
+          if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
+          const double ms = timed(t);
+          if (ms < cur_ms) {
+            cur_ms = ms;
+            cur = t;
+          }
+        }
+      };
+      sweep([](xnyz_tuning& t, double v) { t.persist_blocks = (u32)v; },
+            {512, 1024, 2048, 4096, 8192});
+      if (opts->frame_mode == XNYZ_FRAME_ZSTD3_MT) {
+        sweep([](xnyz_tuning& t, double v) { t.zstd_chunk_log = (u32)v; }, {18, 19, 20, 21, 22});
+        sweep([](xnyz_tuning& t, double v) { t.gpu_frame_share = (float)v; },
+              {0.0, 0.1, 0.2, 0.3, 0.4, 0.5});
+        sweep([](xnyz_tuning& t, double v) { t.copy_first = (u32)v; }, {0, 1});
+        const double s0 = cur.gpu_frame_share;
+        if (s0 > 0)
+          sweep([](xnyz_tuning& t, double v) { t.gpu_frame_share = (float)v; },
+                {std::max(0.0, s0 - 0.05), std::min(0.9, s0 + 0.05)});
+        sweep([](xnyz_tuning& t, double v) { t.copy_slice_log = (u32)v; }, {22, 24, 26});
+      }
+      ctx->tuning[c] = cur;
+      if (best) *best = cur;
+      if (best_ms) *best_ms = cur_ms;
+    } catch (...) {
+      ctx->tuning[c] = saved;
+      throw;
     }
     return XNYZ_OK;
   });

@@ -76,8 +76,11 @@ enum Space { kPoints = 0, kCells = 1, kBlocks = 2 };
 // while the segment stays the same, and flushes to the segment's counters
 // once per segment it touches -- a few thousand device atomics per launch
 // instead of one per wavefront and tile.
-constexpr u32 kPersistBlocks = 2048;  // 8 workgroups per CU on 256 CUs
-inline u32 persist_grid(u32 ntiles) { return ntiles < kPersistBlocks ? ntiles : kPersistBlocks; }
+constexpr u32 kPersistBlocks = 2048;  // 8 workgroups per CU on 256 CUs (default)
+inline u32 persist_grid(const Batch& bt) {
+  const u32 cap = bt.grid_cap ? bt.grid_cap : kPersistBlocks;
+  return bt.ntiles < cap ? bt.ntiles : cap;
+}
 
 __device__ __forceinline__ void block_tiles(u32 ntiles, u32& t0, u32& t1) {
   t0 = (u32)((u64)blockIdx.x * ntiles / gridDim.x);
@@ -1521,20 +1524,20 @@ void launch_split_lo64(const u64* lo64, u32* lo, u32* idx, u32 n, hipStream_t s)
 }
 
 void launch_bbox(const Batch& bt, const float* xyz, hipStream_t s) {
-  k_bbox<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
+  k_bbox<<<persist_grid(bt), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
   LAUNCH_CHECK("k_bbox");
   k_offsets<<<grid_for((u64)bt.S * 3, kBlock), kBlock, 0, s>>>(bt, xyz);
   LAUNCH_CHECK("k_offsets");
 }
 
 void launch_quant_stats(const Batch& bt, const float* xyz, hipStream_t s) {
-  k_quant_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
+  k_quant_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz));
   LAUNCH_CHECK("k_quant_stats");
 }
 
 void launch_features(const Batch& bt, const float* xyz, SegFeat* feat, hipStream_t s) {
   launch_fill_u32(reinterpret_cast<u32*>(feat), (u64)bt.S * (sizeof(SegFeat) / 4), 0, s);
-  k_features<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz),
+  k_features<<<persist_grid(bt), kBlock, 0, s>>>(bt, reinterpret_cast<const F3*>(xyz),
                                                         feat);
   LAUNCH_CHECK("k_features");
 }
@@ -1621,7 +1624,7 @@ void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u64* ua, u32* ub, 
                Arena& ar, hipStream_t s) {
   const KV k{in.a, in.b, bt.wide};
   compact<kFlagCell>(bt, k, upos, 0, ar, s);
-  k_rle_finish<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, k, upos, ua, ub, cnt);
+  k_rle_finish<<<persist_grid(bt), kBlock, 0, s>>>(bt, k, upos, ua, ub, cnt);
   LAUNCH_CHECK("k_rle_finish");
 }
 
@@ -1630,10 +1633,10 @@ void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, b
   (void)direct_any;
   const KV k{kv.a, kv.b, bt.wide};
   if (block_any) compact<kFlagBlock>(bt, k, bpos, 1, ar, s);
-  k_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, k);
+  k_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, k);
   LAUNCH_CHECK("k_stats");
   if (block_any) {
-    k_block_stats<<<persist_grid(bt.ntiles), kBlock, 0, s>>>(bt, k, bpos);
+    k_block_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, k, bpos);
     LAUNCH_CHECK("k_block_stats");
   }
 }

@@ -183,6 +183,7 @@ struct Batch {
   u32* large;                 // (segment, count, cell) triples of large RLE counts
   u32* n_large;
   u32 large_cap;
+  u32 grid_cap;               // host side: workgroups of the persistent kernels (0: 2048)
   bool wide;                  // keys split into (u64 hi, u32 lo)
 };
 

@@ -1,12 +1,19 @@
 // zframe.hip -- zstd-format frames built on the device from the assembled
 // streams in HBM (format and parse rules in zframe.hpp).
 //
-//   k_zparse   one workgroup per 16 KiB block: the block, its two hash
-//              tables and one candidate byte per position in LDS (64 KiB,
-//              two workgroups per CU); the candidates are evaluated by all
-//              256 threads, then one thread per 512-byte segment runs the
-//              lazy parse over them, writing its sequences and literals
-//              into per-segment slots and the literal histogram.
+//   k_zcand    one workgroup per 16 KiB block: the block and its two hash
+//              tables in LDS (48 KiB); all 256 threads evaluate every
+//              position's best fixed candidate and write it, resolved to
+//              (capped length, offset), to global scratch.
+//   k_zparse   one wavefront per two blocks, one lane per 512-byte segment:
+//              only the blocks' bytes sit in LDS (33 KiB, so four such
+//              wavefronts share a CU), the candidates stream from scratch;
+//              each lane runs the lazy parse of its segment, writing its
+//              sequences and literals into per-segment slots and the
+//              block's literal histogram.
+//   Both read the block with aligned 32-bit LDS loads and byte funnel
+//   shifts (match lengths four bytes per step); results equal the
+//   byte-wise definitions in zframe.hpp.
 //   k_zfinish  one wavefront per block: slots compacted in order (the first
 //              sequence of a segment takes the literals left pending by the
 //              segments before it), then the block as zframe.hpp's
@@ -40,6 +47,7 @@ using xz::u16;
 constexpr u32 kSegSeqs = kSeg / kMinMatch;  // most sequences one segment can hold
 
 struct BlkScratch {  // per block, in global memory
+  u32 cand[kBlock];            // (capped length << 16) | offset, 0 = none (k_zcand)
   Seq seqs[kSegs * kSegSeqs];  // per-segment slots (k_zparse)
   Seq cseq[kSegs * kSegSeqs];  // compacted (k_zfinish)
   u8 lits[kBlock];
@@ -225,53 +233,121 @@ __device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, FinLds& L)
   return total;
 }
 
-__global__ __launch_bounds__(256) void k_zparse(const u8* __restrict__ pre,
-                                                const ZBlk* __restrict__ blks,
-                                                BlkScratch* __restrict__ scr) {
-  __shared__ __align__(16) u8 data[kBlock];
-  __shared__ u32 ht4[1u << kHashLog], ht8[1u << kHashLog];
-  __shared__ u8 info[kBlock];
-  __shared__ u32 hist[256];
-  const ZBlk bk = blks[blockIdx.x];
-  BlkScratch& S = scr[blockIdx.x];
-  const u32 n = bk.n, tid = threadIdx.x;
-  const u8* src = pre + bk.src;
-  for (u32 i = tid * 4; i + 4 <= n; i += 1024)  // 4-byte aligned source
+// Word-wise reader of a block held in LDS (4-byte aligned, padded by 8
+// bytes): the same values as xz::ByteReader.
+struct LdsReader {
+  __device__ static u32 r32(const u8* d, u32 p) {
+    const u32* w = reinterpret_cast<const u32*>(d);
+    return __builtin_amdgcn_alignbyte(w[(p >> 2) + 1], w[p >> 2], p & 3u);
+  }
+  __device__ static u32 len(const u8* d, u32 p, u32 q, u32 end) {
+    u32 l = 0;
+    while (p + l < end) {
+      const u32 x = r32(d, p + l) ^ r32(d, q + l);
+      const u32 r = end - (p + l);
+      if (x) {
+        const u32 e = (u32)__builtin_ctz(x) >> 3;
+        return l + (e < r ? e : r);
+      }
+      if (r <= 4) return l + r;
+      l += 4;
+    }
+    return l;
+  }
+};
+
+constexpr u32 kPad = 16;  // LDS block copies: readable bytes past the end
+
+// Copies block bytes [0, n) into LDS (4-byte aligned source and target);
+// the padding is zeroed so no load reads uninitialised memory.
+__device__ __forceinline__ void load_block(u8* data, const u8* src, u32 n, u32 tid, u32 nthr) {
+  for (u32 i = tid * 4; i + 4 <= n; i += nthr * 4)
     *reinterpret_cast<u32*>(data + i) = *reinterpret_cast<const u32*>(src + i);
-  for (u32 i = (n & ~3u) + tid; i < n; i += 256) data[i] = src[i];
+  for (u32 i = (n & ~3u) + tid; i < n + kPad; i += nthr) data[i] = i < n ? src[i] : 0;
+}
+
+__global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
+                                               const ZBlk* __restrict__ blks,
+                                               BlkScratch* __restrict__ scr) {
+  __shared__ __align__(16) u8 data[kBlock + kPad];
+  __shared__ u32 ht4[1u << kHashLog], ht8[1u << kHashLog];
+  const ZBlk bk = blks[blockIdx.x];
+  u32* cand = scr[blockIdx.x].cand;
+  const u32 n = bk.n, tid = threadIdx.x;
+  load_block(data, pre + bk.src, n, tid, 256);
   for (u32 i = tid; i < (1u << kHashLog); i += 256) ht4[i] = ht8[i] = 0xFFFFFFFFu;
-  hist[tid] = 0;
   __syncthreads();
   for (u32 p = tid; p + 4 <= n; p += 256) {
-    atomicMin(&ht4[xz::hash4(xz::read32(data + p))], p);
-    if (p + 8 <= n) atomicMin(&ht8[xz::hash8(xz::read64(data + p))], p);
+    const u32 v = LdsReader::r32(data, p);
+    atomicMin(&ht4[xz::hash4(v)], p);
+    if (p + 8 <= n)
+      atomicMin(&ht8[xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))], p);
   }
   __syncthreads();
-  for (u32 p = tid; p < n; p += 256) info[p] = xz::best_fixed(data, n, ht4, ht8, p);
+  for (u32 p = tid; p < n; p += 256) {
+    const u8 info = xz::best_fixed<LdsReader>(data, n, ht4, ht8, p);
+    u32 c = 0;
+    if (info)
+      c = (((info & 31u) + kMinMatch) << 16) |
+          xz::cand_offset<LdsReader>(data, n, ht4, ht8, p, info >> 5);
+    cand[p] = c;
+  }
+}
+
+constexpr u32 kParseBlocks = 2;  // blocks per k_zparse wavefront (32 segments each)
+
+__global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
+                                               const ZBlk* __restrict__ blks, u32 nblk,
+                                               BlkScratch* __restrict__ scr) {
+  __shared__ __align__(16) u8 data[kParseBlocks][kBlock + kPad];
+  __shared__ u32 hist[kParseBlocks][256];
+  const u32 lane = threadIdx.x;
+  const u32 b0 = blockIdx.x * kParseBlocks;
+  for (u32 k = 0; k < kParseBlocks; ++k) {
+    if (b0 + k < nblk) load_block(data[k], pre + blks[b0 + k].src, blks[b0 + k].n, lane, 64);
+    for (u32 i = lane; i < 256; i += 64) hist[k][i] = 0;
+  }
   __syncthreads();
-  const u32 nseg = (n + kSeg - 1) / kSeg;
-  if (tid < nseg) {
-    const u32 a = tid * kSeg, b = a + kSeg < n ? a + kSeg : n;
-    Seq* sq = S.seqs + tid * kSegSeqs;
-    u8* lit = S.lits + tid * kSeg;
-    u32 ns = 0, nl = 0;
-    auto best_at = [&](u32 p) { return xz::cand_match(data, n, ht4, ht8, p, info[p]); };
-    const u32 tail = xz::parse_segment(data, a, b, best_at, [&](u32 l0, u32 p, u32 len, u32 off) {
-      for (u32 i = l0; i < p; ++i) {
-        lit[nl++] = data[i];
-        atomicAdd(&hist[data[i]], 1u);
+  const u32 k = lane / kSegs, seg = lane % kSegs, b = b0 + k;
+  if (b < nblk) {
+    const u32 n = blks[b].n, nseg = (n + kSeg - 1) / kSeg;
+    BlkScratch& S = scr[b];
+    if (seg < nseg) {
+      const u8* d = data[k];
+      const u32* cand = S.cand;
+      const u32 a = seg * kSeg, e = a + kSeg < n ? a + kSeg : n;
+      Seq* sq = S.seqs + seg * kSegSeqs;
+      u8* lit = S.lits + seg * kSeg;
+      u32* h = hist[k];
+      u32 ns = 0, nl = 0;
+      // xz::cand_match on the resolved candidate: a capped length is
+      // extended to the segment end.
+      auto best_at = [&](u32 p) -> u32 {
+        const u32 c = cand[p];
+        u32 len = c >> 16;
+        if (len == xz::kCap) len = LdsReader::len(d, p, p - (c & 0xFFFFu), xz::seg_end(p, n));
+        return (len << 16) | (c & 0xFFFFu);
+      };
+      const u32 tail = xz::parse_segment<LdsReader>(d, a, e, best_at,
+                                                    [&](u32 l0, u32 p, u32 len, u32 off) {
+        for (u32 i = l0; i < p; ++i) {
+          lit[nl++] = d[i];
+          atomicAdd(&h[d[i]], 1u);
+        }
+        sq[ns++] = Seq{p - l0, len, off};
+      });
+      for (u32 i = tail; i < e; ++i) {
+        lit[nl++] = d[i];
+        atomicAdd(&h[d[i]], 1u);
       }
-      sq[ns++] = Seq{p - l0, len, off};
-    });
-    for (u32 i = tail; i < b; ++i) {
-      lit[nl++] = data[i];
-      atomicAdd(&hist[data[i]], 1u);
+      S.nseq[seg] = ns;
+      S.nlit[seg] = nl;
     }
-    S.nseq[tid] = ns;
-    S.nlit[tid] = nl;
   }
   __syncthreads();
-  S.hist[tid] = hist[tid];
+  for (u32 kk = 0; kk < kParseBlocks; ++kk)
+    if (b0 + kk < nblk)
+      for (u32 i = lane; i < 256; i += 64) scr[b0 + kk].hist[i] = hist[kk][i];
 }
 
 __global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
@@ -452,7 +528,10 @@ void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* fram
   auto* scr = reinterpret_cast<BlkScratch*>(scratch);
   u8* slots = reinterpret_cast<u8*>(scr + nblk);
   u64* slot_len = reinterpret_cast<u64*>(slots + (size_t)nblk * kSlot);
-  k_zparse<<<nblk, 256, 0, s>>>(pre, blks, scr);
+  k_zcand<<<nblk, 256, 0, s>>>(pre, blks, scr);
+  check_hip(hipGetLastError(), "k_zcand");
+  if (g_profiler) g_profiler->mark("k_zcand", s);
+  k_zparse<<<(nblk + kParseBlocks - 1) / kParseBlocks, 64, 0, s>>>(pre, blks, nblk, scr);
   check_hip(hipGetLastError(), "k_zparse");
   if (g_profiler) g_profiler->mark("k_zparse", s);
   k_zfinish<<<nblk, 64, 0, s>>>(pre, blks, scr, slots, slot_len);

@@ -795,6 +795,14 @@ XZ_HD inline u32 match_len(const u8* d, u32 p, u32 q, u32 end) {
   return l;
 }
 
+// How the match finder reads the block: 4-byte little-endian loads and
+// common-prefix lengths.  This byte-wise reader is the definition; a faster
+// reader (the device's word-wise one) must return the same values.
+struct ByteReader {
+  XZ_HD static u32 r32(const u8* d, u32 p) { return read32(d + p); }
+  XZ_HD static u32 len(const u8* d, u32 p, u32 q, u32 end) { return match_len(d, p, q, end); }
+};
+
 // Score of a match in 1/4-byte-ish units: length minus the offset's cost
 // (small for a repeat of one of the last three offsets).
 constexpr int kLenWeight = 4, kRepCost = 2, kOffCost = 4, kLitCost = 3;
@@ -810,16 +818,23 @@ XZ_HD inline u32 seg_end(u32 p, u32 n) {
 // discount; ties keep that order).  Returned as one byte, (index << 5) |
 // (length - kMinMatch), 0 if none -- a function of p alone, so every
 // position can be evaluated in parallel.
+template <typename R = ByteReader>
+XZ_HD inline u64 r64(const u8* d, u32 p) {
+  return (u64)R::r32(d, p) | ((u64)R::r32(d, p + 4) << 32);
+}
+
+template <typename R = ByteReader>
 XZ_HD inline u32 cand_offset(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p, u32 idx) {
   switch (idx) {
     case 1: return 1;
     case 2: return 8;
     case 3: return 16;
-    case 4: return p + 4 <= n ? p - ht4[hash4(read32(d + p))] : 0;
-    default: return p + 8 <= n ? p - ht8[hash8(read64(d + p))] : 0;
+    case 4: return p + 4 <= n ? p - ht4[hash4(R::r32(d, p))] : 0;
+    default: return p + 8 <= n ? p - ht8[hash8(r64<R>(d, p))] : 0;
   }
 }
 
+template <typename R = ByteReader>
 XZ_HD inline u8 best_fixed(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p) {
   const u32 b = seg_end(p, n);
   if (p + kMinMatch > b) return 0;
@@ -827,11 +842,11 @@ XZ_HD inline u8 best_fixed(const u8* d, u32 n, const u32* ht4, const u32* ht8, u
   u32 best = 0, bidx = 0;
   i32 bs = -(1 << 30);
   for (u32 idx = 1; idx <= 5; ++idx) {
-    if (idx == 4 && !(p + 4 <= n && ht4[hash4(read32(d + p))] < p)) continue;
-    if (idx == 5 && !(p + 8 <= n && ht8[hash8(read64(d + p))] < p)) continue;
-    const u32 off = cand_offset(d, n, ht4, ht8, p, idx);
+    if (idx == 4 && !(p + 4 <= n && ht4[hash4(R::r32(d, p))] < p)) continue;
+    if (idx == 5 && !(p + 8 <= n && ht8[hash8(r64<R>(d, p))] < p)) continue;
+    const u32 off = cand_offset<R>(d, n, ht4, ht8, p, idx);
     if (!off || off > p) continue;
-    const u32 l = match_len(d, p, p - off, cap);
+    const u32 l = R::len(d, p, p - off, cap);
     if (l < kMinMatch) continue;
     const i32 sc = (i32)l * kLenWeight - ((i32)highbit(off + 3) + kOffCost);
     if (sc > bs) {
@@ -845,11 +860,12 @@ XZ_HD inline u8 best_fixed(const u8* d, u32 n, const u32* ht4, const u32* ht8, u
 
 // The match a candidate byte stands for: (length << 16) | offset, the
 // length extended past kCap to the segment end.
+template <typename R = ByteReader>
 XZ_HD inline u32 cand_match(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p, u8 info) {
   if (!info) return 0;
-  const u32 off = cand_offset(d, n, ht4, ht8, p, info >> 5);
+  const u32 off = cand_offset<R>(d, n, ht4, ht8, p, info >> 5);
   u32 len = (info & 31u) + kMinMatch;
-  if (len == kCap) len = match_len(d, p, p - off, seg_end(p, n));
+  if (len == kCap) len = R::len(d, p, p - off, seg_end(p, n));
   return (len << 16) | off;
 }
 
@@ -857,7 +873,7 @@ XZ_HD inline u32 cand_match(const u8* d, u32 n, const u32* ht4, const u32* ht8, 
 // competes with the segment's last three offsets (scored with the repeat
 // discount); a literal is emitted first when the next position scores
 // better by more than a literal's cost.
-template <typename Best, typename Emit>
+template <typename R = ByteReader, typename Best, typename Emit>
 XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&& emit) {
   u32 rep[3] = {0, 0, 0};
   auto find = [&](u32 p, u32& best, u32& boff) -> i32 {
@@ -872,7 +888,7 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
     for (u32 c = 0; c < 3; ++c) {
       const u32 off = rep[c];
       if (!off || off > p || off == boff) continue;
-      const u32 l = match_len(d, p, p - off, b);
+      const u32 l = R::len(d, p, p - off, b);
       if (l < kMinMatch) continue;
       const i32 sc = sc_of(l, off);
       if (sc > bs) {
