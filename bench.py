@@ -94,8 +94,10 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
                out_bytes: int, dev_frame_bytes: int = 0) -> float | None:
     """Algorithmic bytes of ONE launch of `kernel` (reads + writes it must do).
     dev_frame_bytes: stream bytes whose frame blocks the device encodes (the
-    match finder must read each once; what it writes depends on the data and
-    is not counted, so its figure is a lower bound)."""
+    match finder, the parser and the entropy coder each pass over every such
+    byte once -- as block bytes, or as the literals and sequences standing
+    for them; what they write depends on the data and is not counted, so
+    these figures are lower bounds)."""
     per_pass_keys = n_points * key_bytes
     table = {
         "k_bbox": 12 * n_points,                      # read xyz
@@ -108,7 +110,9 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
         "k_rle_finish": 4 * n_coded + 8 * n_coded + 4 * n_coded,
         "k_stats": 8 * n_coded,                       # read sorted cells
         "k_sec_bits": 8 * n_coded,
+        "k_zcand": dev_frame_bytes,
         "k_zparse": dev_frame_bytes,
+        "k_zfinish": dev_frame_bytes,
     }
     return table.get(kernel)
 
@@ -125,8 +129,10 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary frame modes")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default run-time parameters")
+    ap.add_argument("--tuning", default=None, metavar="FILE",
+                    help="run-time tuning table (default: the packaged MI355X table)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the run-time parameters on this cloud first instead")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="end with one elementwise kernel of this size (counter scaling)")
     args = ap.parse_args()
@@ -182,14 +188,23 @@ def main() -> int:
 
     head_mode = _lib.FRAME_ZSTD3_MT
     # Run-time parameters (persistent grid, host chunk size, the device's
-    # share of the frame, copy-back slice) tuned on this cloud first,
-    # outside the timed steps.
-    tuning = ctx.tuning(max(0, n.bit_length() - 1))
-    if not args.no_autotune:
+    # share of the frame, copy-back slice): the table xnyzip_amd.tune
+    # measured, or tuned on this cloud first (outside the timed steps).
+    from xnyzip_amd import tune as AT
+    table = args.tuning or AT.DEFAULT_TABLE
+    if args.autotune:
         t_at = time.perf_counter()
-        tuning, at_ms = X.autotune(dev, dict(o1[0], frame_mode=head_mode), reps=2, ctx=ctx)
-        print(f"[bench] autotune: {tuning} ({at_ms:.1f} ms/call) in "
+        _, at_ms = X.autotune(dev, dict(o1[0], frame_mode=head_mode), reps=2, ctx=ctx)
+        print(f"[bench] autotune: {at_ms:.1f} ms/call, tuned in "
               f"{time.perf_counter() - t_at:.1f} s", file=sys.stderr, flush=True)
+        table = "autotuned on this cloud"
+    elif os.path.exists(table):
+        ctx.load_tuning(table)
+    else:
+        table = "defaults"
+    tuning = ctx.tuning(max(0, n.bit_length() - 1))
+    print(f"[bench] tuning ({os.path.relpath(table, ROOT) if os.path.exists(table) else table}):"
+          f" {tuning}", file=sys.stderr, flush=True)
     dt, (frame_len, st) = timed(head_mode, args.steps, args.warmup)
     value = 12.0 * n * args.steps / dt / 1e9
     print(f"[bench] headline: {value:.2f} GB/s ({dt / args.steps * 1e3:.1f} ms/step)",
@@ -313,18 +328,17 @@ def copy_probe(mib: int) -> None:
 def measured_traffic(kernel: str, workload: str, n: int):
     """HBM bytes per launch of `kernel` as measured with rocprofv3 counters
     (FETCH_SIZE + WRITE_SIZE, each scaled by the copy probe's reading) and
-    recorded in profiles/<kernel>_<workload>_traffic.md as a line
+    recorded in a profiles/*traffic*.md summary as a line
     `traffic_bytes_per_launch: <kernel> <workload> <points> <bytes>`; None
     when no such measurement exists for this kernel and size."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        for line in f:
-            p = line.strip().strip("`").split()
-            if (len(p) == 5 and p[0] == "traffic_bytes_per_launch:" and p[1] == kernel
-                    and p[2] == workload and int(p[3]) == n):
-                return int(p[4])
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.md"))):
+        with open(path) as f:
+            for line in f:
+                p = line.strip().strip("`").split()
+                if (len(p) == 5 and p[0] == "traffic_bytes_per_launch:" and p[1] == kernel
+                        and p[2] == workload and int(p[3]) == n):
+                    return int(p[4])
     return None
 
 

@@ -43,6 +43,8 @@ using xz::kSlot;
 using xz::Seq;
 using xz::SeqCodes;
 using xz::u16;
+using xz::i16;
+using xz::i32;
 
 constexpr u32 kSegSeqs = kSeg / kMinMatch;  // most sequences one segment can hold
 
@@ -50,6 +52,8 @@ struct BlkScratch {  // per block, in global memory
   u32 cand[kBlock];            // (capped length << 16) | offset, 0 = none (k_zcand)
   Seq seqs[kSegs * kSegSeqs];  // per-segment slots (k_zparse)
   Seq cseq[kSegs * kSegSeqs];  // compacted (k_zfinish)
+  u8 sym[3][kBlock / kMinMatch];   // LL, OF, ML codes per sequence (k_zfinish)
+  u16 fse[3][kBlock / kMinMatch];  // FSE bits per sequence and table: (count << 10) | value
   u8 lits[kBlock];
   u8 clit[kBlock];
   u32 hist[256];
@@ -67,6 +71,8 @@ struct FinLds {
   u32 segn[kSegs], segl[kSegs], segll[kSegs], carry[kSegs], soff[kSegs + 1], loff[kSegs + 1];
   u8 desc[160];
   u32 sbits[4];
+  u32 fin[3];  // FSE states after the first sequence (flushed at the end)
+  u32 ncum[xz::kFseMaxSym + 1];  // FSE table builds: cells of the symbols before
 };
 
 __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
@@ -76,57 +82,73 @@ __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
 
 // Repeat-offset history over the sequences (zframe.hpp seq_codes), one
 // step per sequence, into L.ov.
+// Offset_Values of a block's sequences in order (repeat codes against the
+// running three-offset history; RFC 8878 3.1.1.5), as xz::seq_codes
+// computes them: one lane walks the sequences with the history in registers,
+// over LDS entries (offset | 0x8000 when the literal length is 0) rewritten
+// in place.
 __device__ void dev_offset_values(const Seq* cs, u32 ns, FinLds& L) {
   const u32 lane = threadIdx.x;
-  u32 rep[3] = {0, 0, 0}, known = 0;
-  for (u32 base = 0; base < ns; base += 64) {
-    const Seq mine = base + lane < ns ? cs[base + lane] : Seq{0, 0, 0};
-    const u32 m = ns - base < 64 ? ns - base : 64;
-    for (u32 j = 0; j < m; ++j) {
-      const Seq q = shfl_seq(mine, j);
-      const u32 o = q.off, ll = q.ll;
-      u32 ov = 0;
-      if (ll) {
-        if (known >= 1 && o == rep[0]) {
-          ov = 1;
-        } else if (known >= 2 && o == rep[1]) {
-          ov = 2;
-          rep[1] = rep[0];
-          rep[0] = o;
-        } else if (known >= 3 && o == rep[2]) {
-          ov = 3;
-          rep[2] = rep[1];
-          rep[1] = rep[0];
-          rep[0] = o;
+  for (u32 i = lane; i < ns; i += 64) {
+    const Seq q = cs[i];
+    L.ov[i] = (u16)(q.off | (q.ll ? 0u : 0x8000u));
+  }
+  __syncthreads();
+  if (lane == 0) {
+    u32 r0 = 0, r1 = 0, r2 = 0, known = 0;
+    constexpr u32 kB = 8;  // entries loaded ahead of the dependent chain
+    for (u32 i0 = 0; i0 < ns; i0 += kB) {
+      u32 xb[kB];
+#pragma unroll
+      for (u32 q = 0; q < kB; ++q) xb[q] = i0 + q < ns ? L.ov[i0 + q] : 0u;
+#pragma unroll
+      for (u32 q = 0; q < kB; ++q) {
+        if (i0 + q >= ns) break;
+        const u32 x = xb[q], o = x & 0x7FFFu;
+        u32 ov = 0;
+        if (!(x & 0x8000u)) {
+          if (known >= 1 && o == r0) {
+            ov = 1;
+          } else if (known >= 2 && o == r1) {
+            ov = 2;
+            r1 = r0;
+            r0 = o;
+          } else if (known >= 3 && o == r2) {
+            ov = 3;
+            r2 = r1;
+            r1 = r0;
+            r0 = o;
+          }
+        } else {
+          if (known >= 2 && o == r1) {
+            ov = 1;
+            r1 = r0;
+            r0 = o;
+          } else if (known >= 3 && o == r2) {
+            ov = 2;
+            r2 = r1;
+            r1 = r0;
+            r0 = o;
+          } else if (known >= 1 && r0 > 1 && o == r0 - 1) {
+            ov = 3;
+            r2 = r1;
+            r1 = r0;
+            r0 = o;
+            known = known < 3 ? known + 1 : 3;
+          }
         }
-      } else {
-        if (known >= 2 && o == rep[1]) {
-          ov = 1;
-          rep[1] = rep[0];
-          rep[0] = o;
-        } else if (known >= 3 && o == rep[2]) {
-          ov = 2;
-          rep[2] = rep[1];
-          rep[1] = rep[0];
-          rep[0] = o;
-        } else if (known >= 1 && rep[0] > 1 && o == rep[0] - 1) {
-          ov = 3;
-          rep[2] = rep[1];
-          rep[1] = rep[0];
-          rep[0] = o;
+        if (!ov) {
+          ov = o + 3;
+          r2 = r1;
+          r1 = r0;
+          r0 = o;
           known = known < 3 ? known + 1 : 3;
         }
+        L.ov[i0 + q] = (u16)ov;
       }
-      if (!ov) {
-        ov = o + 3;
-        rep[2] = rep[1];
-        rep[1] = rep[0];
-        rep[0] = o;
-        known = known < 3 ? known + 1 : 3;
-      }
-      if (lane == 0) L.ov[base + j] = (u16)ov;
     }
   }
+  __syncthreads();
 }
 
 __device__ __forceinline__ SeqCodes codes_of(const Seq& q, u32 ov) {
@@ -136,6 +158,382 @@ __device__ __forceinline__ SeqCodes codes_of(const Seq& q, u32 ov) {
   c.ov = ov;
   c.of = (u8)xz::highbit(ov);
   return c;
+}
+
+// xz::huf_build on one wavefront, with the same tables: counts copied and
+// tested in parallel; present symbols ranked by (count, symbol) through
+// all-pairs comparison (the order the stable radix sort gives); the
+// two-queue merge by one lane; leaf depths by pointer jumping up the tree;
+// canonical codes from per-length wavefront scans.  Returns the same value
+// on every lane.
+__device__ bool dev_huf_build(const u32* count_in, xz::HufTable& h, xz::Work& W) {
+  const u32 lane = threadIdx.x;
+  u32* cnt = W.cnt;
+  u32 np = 0, last = 0;
+  for (u32 s = lane; s < 256; s += 64) {
+    const u32 c = count_in[s];
+    cnt[s] = c;
+    if (c) {
+      ++np;
+      last = s;
+    }
+  }
+  const u32 n = wave_sum(np);
+  const u32 top = wave_max(np ? last : 0u);
+  if (lane == 0 && n) h.last = top;
+  __syncthreads();
+  if (n < 2) return false;
+  u32* key = W.pos;
+  u16* sym = W.sym;
+  u64* w = W.w;
+  i32* anc = W.parent;
+  u32* dep = W.depth;
+  const u32 nodes = 2 * n - 1;
+  u32 maxd;
+  for (;;) {
+    for (u32 s = lane; s < 256; s += 64) key[s] = cnt[s] ? (cnt[s] << 8) | s : 0xFFFFFFFFu;
+    __syncthreads();
+    u32 mk[4], rk[4] = {0, 0, 0, 0};
+    for (u32 j = 0; j < 4; ++j) mk[j] = key[lane + 64 * j];
+    for (u32 t = 0; t < 256; ++t) {
+      const u32 kt = key[t];
+#pragma unroll
+      for (u32 j = 0; j < 4; ++j) rk[j] += kt < mk[j];
+    }
+    for (u32 j = 0; j < 4; ++j)
+      if (mk[j] != 0xFFFFFFFFu) sym[rk[j]] = (u16)(lane + 64 * j);
+    __syncthreads();
+    for (u32 i = lane; i < n; i += 64) w[i] = cnt[sym[i]];
+    __syncthreads();
+    if (lane == 0) {  // two queues: leaves by weight, inner nodes in creation order
+      u32 leaf = 0, node = n, inner = n;
+      for (u32 k = 0; k + 1 < n; ++k) {
+        u32 pick[2];
+        for (int c = 0; c < 2; ++c) {
+          if (leaf < n && (inner >= node || w[leaf] <= w[inner])) pick[c] = leaf++;
+          else pick[c] = inner++;
+        }
+        w[node] = w[pick[0]] + w[pick[1]];
+        anc[pick[0]] = anc[pick[1]] = (i32)node;
+        ++node;
+      }
+      anc[nodes - 1] = (i32)(nodes - 1);  // the root is its own ancestor
+    }
+    __syncthreads();
+    // distance to the root: d += d(anc), anc = anc(anc), 8 rounds (depth < 256)
+    constexpr u32 kPer = 8;  // nodes per lane (2 * 256 - 1 < 64 * 8)
+    u32 d[kPer], a[kPer];
+    for (u32 k = 0; k < kPer; ++k) {
+      const u32 i = lane + 64 * k;
+      a[k] = i < nodes ? (u32)anc[i] : 0u;
+      d[k] = i + 1 < nodes ? 1u : 0u;
+      if (i < nodes) dep[i] = d[k];
+    }
+    __syncthreads();
+    for (u32 r = 0; r < 8; ++r) {
+      for (u32 k = 0; k < kPer; ++k)
+        if (lane + 64 * k < nodes) {
+          d[k] += dep[a[k]];
+          a[k] = (u32)anc[a[k]];
+        }
+      __syncthreads();
+      for (u32 k = 0; k < kPer; ++k)
+        if (lane + 64 * k < nodes) {
+          dep[lane + 64 * k] = d[k];
+          anc[lane + 64 * k] = (i32)a[k];
+        }
+      __syncthreads();
+    }
+    u32 md = 0;
+    for (u32 k = 0; k < kPer; ++k)
+      if (lane + 64 * k < n) md = d[k] > md ? d[k] : md;
+    maxd = wave_max(md);
+    if (maxd <= xz::kMaxHufBits) {
+      for (u32 s = lane; s < 256; s += 64) h.nbits[s] = 0;
+      __syncthreads();
+      for (u32 k = 0; k < kPer; ++k)
+        if (lane + 64 * k < n) h.nbits[sym[lane + 64 * k]] = (u8)d[k];
+      if (lane == 0) h.maxbits = maxd;
+      __syncthreads();
+      break;
+    }
+    for (u32 s = lane; s < 256; s += 64)
+      if (cnt[s]) cnt[s] = (cnt[s] >> 1) | 1u;
+    __syncthreads();
+  }
+  // canonical codes: longest first from 0, by symbol; lane holds 4l .. 4l+3
+  u32 nb[4];
+  for (u32 j = 0; j < 4; ++j) nb[j] = h.nbits[4 * lane + j];
+  u32 base = 0;
+  for (u32 len = maxd; len >= 1; --len) {
+    u32 c = 0;
+    for (u32 j = 0; j < 4; ++j) c += nb[j] == len;
+    const u32 inc = wave_incl_scan(c);
+    const u32 tot = (u32)__shfl((int)inc, 63, 64);
+    u32 r = base + inc - c;
+    for (u32 j = 0; j < 4; ++j)
+      if (nb[j] == len) h.code[4 * lane + j] = (u16)r++;
+    base = (base + tot) >> 1;
+  }
+  __syncthreads();
+  return true;
+}
+
+// ---- sequence tables on one wavefront (lane s handles symbol s; <= 64) ----
+
+// xz::fse_normalize: same normalised counts.
+__device__ void dev_fse_normalize(const u32* count, u32 nsym, u32 total, u32 log, i16* norm) {
+  const u32 lane = threadIdx.x, size = 1u << log;
+  const u32 c = lane < nsym ? count[lane] : 0u;
+  i32 v = 0;
+  if (c) {
+    v = (i32)(((u64)c * size + total / 2) / total);
+    if (v < 1) v = 1;
+  }
+  // the first symbol with the largest count; sum fixed there or, when too
+  // large, on the first symbol with the largest value above 1, one at a time
+  const u32 bk = wave_max(c ? (c << 6) | (63u - lane) : 0u);
+  const u32 big = 63u - (bk & 63u);
+  i32 sum = wave_sum(v);
+  while (sum > (i32)size) {
+    const u32 m = wave_max(v > 1 ? ((u32)v << 6) | (63u - lane) : 0u);
+    if (!m) break;
+    if (lane == 63u - (m & 63u)) --v;
+    --sum;
+  }
+  if (sum < (i32)size && lane == big) v += (i32)size - sum;
+  if (lane < nsym) norm[lane] = (i16)v;
+  __syncthreads();
+}
+
+// xz::fse_build: same table.  The k-th step of the spread walk visits
+// position (k * step) mod size, so each lane takes a run of steps, counts
+// the positions at or below the "-1" region, and places the cells whose
+// ranks a wavefront scan gives it.
+__device__ void dev_fse_build(xz::FseTable& t, const i16* norm, u32 nsym, u32 log, xz::Work& W,
+                              FinLds& L) {
+  const u32 lane = threadIdx.x, size = 1u << log, mask = size - 1;
+  const i32 n = lane < nsym ? (i32)norm[lane] : 0;
+  const u32 m1 = n == -1 ? 1u : 0u, cn = n > 0 ? (u32)n : 0u;
+  const u32 cells = m1 + cn;
+  const u32 cum = wave_incl_scan(cells) - cells;  // xz::fse_build's cumul[s]
+  const u32 m1x = wave_incl_scan(m1) - m1;
+  const u32 nm1 = (u32)__shfl((int)(m1x + m1), 63, 64);
+  const u32 high = size - 1 - nm1;
+  const u32 ncx = wave_incl_scan(cn) - cn;
+  u8* spread = W.spread;
+  if (lane < nsym) L.ncum[lane] = ncx;
+  if (lane == 0) L.ncum[nsym] = high + 1;
+  if (m1) spread[size - 1 - m1x] = (u8)lane;
+  __syncthreads();
+  const u32 step = (size >> 1) + (size >> 3) + 3, per = (size + 63) / 64;
+  const u32 k0 = lane * per, k1 = k0 + per < size ? k0 + per : size;
+  u32 valid = 0;
+  for (u32 k = k0; k < k1; ++k) valid += ((k * step) & mask) <= high;
+  u32 rank = wave_incl_scan(valid) - valid;
+  if (valid) {
+    u32 lo = 0, hi = nsym - 1;  // the symbol whose cells cover `rank`
+    while (lo < hi) {
+      const u32 mid = (lo + hi + 1) / 2;
+      if (L.ncum[mid] <= rank) lo = mid;
+      else hi = mid - 1;
+    }
+    u32 sym = lo;
+    for (u32 k = k0; k < k1; ++k) {
+      const u32 pos = (k * step) & mask;
+      if (pos > high) continue;
+      while (L.ncum[sym + 1] <= rank) ++sym;
+      spread[pos] = (u8)sym;
+      ++rank;
+    }
+  }
+  __syncthreads();
+  // states: the cells of symbol s in position order take slots cum(s) ...
+  if (lane < nsym && cells) {
+    u32 slot = cum;
+    for (u32 u = 0; u < size; u += 4) {
+      const u32 q = *reinterpret_cast<const u32*>(spread + u);
+#pragma unroll
+      for (u32 b = 0; b < 4; ++b)
+        if (u + b < size && ((q >> (8 * b)) & 255u) == lane) t.state[slot++] = (u16)(size + u + b);
+    }
+  }
+  const u32 c1 = n == 0 ? 0u : (n == -1 || n == 1) ? 1u : (u32)n;
+  const u32 tot = wave_incl_scan(c1) - c1;
+  if (lane < nsym) {
+    if (n == 0) {
+      t.dnb[lane] = (i32)(((log + 1) << 16) - size);
+      t.dfs[lane] = 0;
+    } else if (n == -1 || n == 1) {
+      t.dnb[lane] = (i32)((log << 16) - size);
+      t.dfs[lane] = (i32)tot - 1;
+    } else {
+      const u32 maxb = log - xz::highbit((u32)n - 1);
+      t.dnb[lane] = (i32)((maxb << 16) - ((u32)n << maxb));
+      t.dfs[lane] = (i32)tot - n;
+    }
+  }
+  if (lane == 0) t.log = log;
+  __syncthreads();
+}
+
+// xz::choose_table: same choice, table and description.
+__device__ u32 dev_choose_table(int which, const u32* count, u32 n, xz::FseTable& t, u8* desc,
+                                u32& mode, xz::Work& W, FinLds& L) {
+  const u32 lane = threadIdx.x;
+  const u32 ms = xz::kMaxSym[which];
+  const u32 c = lane <= ms ? count[lane] : 0u;
+  const u32 present = wave_sum(c ? 1u : 0u);
+  const u32 last = wave_max(c ? lane : 0u);
+  i16* norm = W.norm;
+  if (present == 1 && n > 1) {  // RLE
+    if (lane <= last) norm[lane] = lane == last ? 1 : 0;
+    __syncthreads();
+    dev_fse_build(t, norm, last + 1, 0, W, L);
+    if (lane == 0) desc[0] = (u8)last;
+    mode = 1;
+    return 1;
+  }
+  const i16* pn = xz::pre_norm(which);
+  const bool bad = c && lane >= xz::kPreSyms[which];
+  const bool pre_ok = wave_max(bad ? 1u : 0u) == 0;
+  u64 pre = 0;
+  if (pre_ok && c) pre = (u64)c * xz::cost16(pn[lane] < 0 ? 1u : (u32)pn[lane], xz::kPreLog[which]);
+  pre = wave_sum(pre);
+  u32 log = n > 1 ? xz::highbit(n - 1) + 1 : 5;
+  log = log < 5 ? 5 : log > xz::kMaxLog[which] ? xz::kMaxLog[which] : log;
+  dev_fse_normalize(count, last + 1, n, log, norm);
+  u64 own = c ? (u64)c * xz::cost16((u32)norm[lane], log) : 0ull;
+  own = wave_sum(own);
+  u8* tmp = W.ntmp;
+  const u32 hl = xz::fse_write_ncount(tmp, norm, last + 1, log);  // uniform
+  own += 16ull * 8 * hl;
+  __syncthreads();
+  if (pre_ok && pre <= own) {
+    dev_fse_build(t, pn, xz::kPreSyms[which], xz::kPreLog[which], W, L);
+    mode = 0;
+    return 0;
+  }
+  dev_fse_build(t, norm, last + 1, log, W, L);
+  for (u32 i = lane; i < hl; i += 64) desc[i] = tmp[i];
+  mode = 2;
+  return hl;
+}
+
+// xz::seq_write_tables: mode byte and table descriptions from sw.cnt.
+__device__ u32 dev_seq_write_tables(u8* out, u32 n, xz::SeqWork& sw, xz::Work& W, FinLds& L) {
+  u32 m[3];
+  u32 p = 1;
+  for (int k = 0; k < 3; ++k) p += dev_choose_table(k, sw.cnt[k], n, sw.t[k], out + p, m[k], W, L);
+  if (threadIdx.x == 0) out[0] = (u8)((m[0] << 6) | (m[1] << 4) | (m[2] << 2));
+  return p;
+}
+
+// The sequences' FSE bitstream as xz::write_sequences writes it, at slot +
+// base (slot 256-byte aligned): the three state chains (literal lengths,
+// offsets, match lengths) are independent, so three lanes run them, from
+// the last sequence down, recording each step's emitted bits; then every
+// lane packs a run of sequences in stream order (last sequence first) at
+// offsets from a wavefront scan, and lane 0 appends the final states and
+// the end mark.  sym / ov / tables must be ready.  Returns bytes written.
+__device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, FinLds& L) {
+  const u32 lane = threadIdx.x;
+  if (lane < 3) {
+    const xz::FseTable& t = L.sw.t[lane];
+    const u8* sym = S.sym[lane];
+    u16* fo = S.fse[lane];
+    xz::FseState st;
+    xz::fse_init(st, t, sym[ns - 1]);
+    constexpr u32 kB = 16;  // codes (and their table entries) fetched ahead of the chain
+    for (u32 k0 = 0; k0 + 1 < ns; k0 += kB) {  // step k encodes sequence ns - 2 - k
+      u32 dnb[kB], dfs[kB];
+#pragma unroll
+      for (u32 q = 0; q < kB; ++q) {
+        const u32 c = k0 + q + 1 < ns ? sym[ns - 2 - k0 - q] : 0u;
+        dnb[q] = (u32)t.dnb[c];
+        dfs[q] = (u32)t.dfs[c];
+      }
+#pragma unroll
+      for (u32 q = 0; q < kB; ++q) {
+        if (k0 + q + 1 >= ns) break;
+        const u32 nbo = (st.v + dnb[q]) >> 16;
+        fo[ns - 2 - k0 - q] = (u16)((nbo << 10) | (st.v & ((1u << nbo) - 1u)));
+        st.v = t.state[(st.v >> nbo) + dfs[q]];
+      }
+    }
+    L.fin[lane] = st.v;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // stream order r = 0 .. ns - 1 is sequence ns - 1 - r
+  auto seq_bits = [&](u32 i) -> u32 {
+    const Seq q = S.cseq[i];
+    const u32 ov = L.ov[i];
+    u32 b = xz::ll_bits(xz::ll_code(q.ll)) + xz::ml_bits(xz::ml_code(q.ml)) + xz::highbit(ov);
+    if (i + 1 < ns) b += (S.fse[0][i] >> 10) + (S.fse[1][i] >> 10) + (S.fse[2][i] >> 10);
+    return b;
+  };
+  const u32 per = (ns + 63) / 64, r0 = lane * per < ns ? lane * per : ns;
+  const u32 r1 = r0 + per < ns ? r0 + per : ns;
+  u32 bits = 0;
+  for (u32 r = r0; r < r1; ++r) bits += seq_bits(ns - 1 - r);
+  const u32 inc = wave_incl_scan(bits);
+  const u32 total = __shfl((int)inc, 63, 64);
+  const u32 tl = (u32)L.sw.t[0].log, to = (u32)L.sw.t[1].log, tm = (u32)L.sw.t[2].log;
+  const u32 nbytes = (total + tm + to + tl + 1 + 7) / 8;
+  for (u32 i = lane; i < nbytes; i += 64) slot[base + i] = 0;
+  __threadfence_block();
+  __syncthreads();
+  u32* w = reinterpret_cast<u32*>(slot);
+  u64 pos = (u64)base * 8 + (inc - bits);
+  u64 acc = 0;
+  u32 nb = (u32)(pos & 31);
+  u64 word = pos >> 5;
+  bool first = true;
+  auto put = [&](u32 v, u32 n) {
+    acc |= (u64)(v & (n < 32 ? (1u << n) - 1u : 0xFFFFFFFFu)) << nb;
+    nb += n;
+    if (nb >= 32) {
+      if (first) atomicOr(&w[word], (u32)acc);
+      else w[word] = (u32)acc;
+      first = false;
+      acc >>= 32;
+      nb -= 32;
+      ++word;
+    }
+  };
+  for (u32 r = r0; r < r1; ++r) {
+    const u32 i = ns - 1 - r;
+    const Seq q = S.cseq[i];
+    const SeqCodes c = codes_of(q, L.ov[i]);
+    if (i + 1 < ns) {
+      const u32 fo = S.fse[1][i], fm = S.fse[2][i], fl = S.fse[0][i];
+      put(fo & 1023u, fo >> 10);
+      put(fm & 1023u, fm >> 10);
+      put(fl & 1023u, fl >> 10);
+    }
+    put(q.ll - xz::ll_base(c.ll), xz::ll_bits(c.ll));
+    put(q.ml - xz::ml_base(c.ml), xz::ml_bits(c.ml));
+    put(c.ov - (1u << c.of), c.of);
+  }
+  if (nb) atomicOr(&w[word], (u32)acc);
+  __threadfence_block();
+  __syncthreads();
+  if (lane == 0) {  // final states (match lengths, offsets, literal lengths), end mark
+    pos = (u64)base * 8 + total;
+    acc = 0;
+    nb = (u32)(pos & 31);
+    word = pos >> 5;
+    first = true;
+    put(L.fin[2], tm);
+    put(L.fin[1], to);
+    put(L.fin[0], tl);
+    put(1, 1);
+    if (nb) atomicOr(&w[word], (u32)acc);
+  }
+  __threadfence_block();
+  __syncthreads();
+  return nbytes;
 }
 
 // Huffman-coded literals as write_huf_lits lays them out, written at their
@@ -411,7 +809,7 @@ __global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
     if (lane == 0) ls = xz::write_raw_lits(body, S.clit, nl, 1);
     ls = __shfl((int)ls, 0, 64);
   } else {
-    if (nl >= 64 && xz::huf_build(L.hist, L.h, L.W)) {  // uniform
+    if (nl >= 64 && dev_huf_build(L.hist, L.h, L.W)) {
       u64 bits = 0;
       for (u32 c = 0; c < 256; ++c) bits += (u64)L.hist[c] * L.h.nbits[c];
       if (bits / 8 + 16 < nl) ls = dev_huf_lits(out, 3, S.clit, nl, L);
@@ -446,28 +844,16 @@ __global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
       atomicAdd(&L.sw.cnt[0][c.ll], 1u);
       atomicAdd(&L.sw.cnt[1][c.of], 1u);
       atomicAdd(&L.sw.cnt[2][c.ml], 1u);
+      S.sym[0][i] = c.ll;
+      S.sym[1][i] = c.of;
+      S.sym[2][i] = c.ml;
     }
+    __threadfence_block();
     __syncthreads();
-    ss += xz::seq_write_tables(sq + ss, ns, L.sw, L.W);  // uniform
-    xz::BitW bw(sq + ss);
-    xz::FseState st[3];
-    {
-      const Seq q = S.cseq[ns - 1];
-      xz::seq_encode_last(bw, st, L.sw, q, codes_of(q, L.ov[ns - 1]));
-    }
-    for (u32 top = ns - 1; top > 0;) {  // sequences top-1 down to 0, 64 at a time
-      const u32 m = top < 64 ? top : 64;
-      const u32 idx = top - 1 - lane;
-      const Seq mine = lane < m ? S.cseq[idx] : Seq{0, 0, 0};
-      const u32 myov = lane < m ? L.ov[idx] : 0u;
-      for (u32 j = 0; j < m; ++j) {
-        const Seq q = shfl_seq(mine, j);
-        const u32 ov = (u32)__shfl((int)myov, (int)j, 64);
-        xz::seq_encode_one(bw, st, L.sw, q, codes_of(q, ov));
-      }
-      top -= m;
-    }
-    ss += xz::seq_encode_close(bw, st, L.sw);
+    ss += dev_seq_write_tables(sq + ss, ns, L.sw, L.W, L);
+    __threadfence_block();
+    __syncthreads();
+    ss += dev_seq_bits(out, (u32)(sq - out) + ss, S, ns, L);
   }
   __threadfence_block();
   __syncthreads();
