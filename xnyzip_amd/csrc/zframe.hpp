@@ -15,9 +15,12 @@
 //   * Literals: Huffman-coded (1 stream up to 1023 literals, else 4 streams),
 //     weights FSE-compressed (or direct 4-bit when that is smaller), code
 //     lengths limited to 11 bits; raw or RLE literals when smaller.
-//   * Sequences: predefined FSE distributions for literal lengths, match
-//     lengths and offsets (Compression_Mode 0); offsets always coded as new
-//     offsets (Offset_Value = offset + 3), never as repeat offsets.
+//   * Sequences: per block and code kind (literal lengths, offsets, match
+//     lengths) the cheaper of the predefined distribution and the block's own
+//     FSE table by estimated bits (RLE mode for a single code); repeat
+//     offsets (RFC 8878 3.1.1.5) only where the repeated offset was coded
+//     earlier in the same block, so no block relies on the history left by
+//     the blocks before it.
 //   * A block whose encoding is not smaller than its bytes is stored raw.
 #pragma once
 
