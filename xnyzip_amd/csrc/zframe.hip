@@ -48,12 +48,13 @@ using xz::i32;
 
 constexpr u32 kSegSeqs = kSeg / kMinMatch;  // most sequences one segment can hold
 
-struct BlkScratch {  // per block, in global memory
+struct alignas(16) BlkScratch {  // per block, in global memory
   u32 cand[kBlock];            // (capped length << 16) | offset, 0 = none (k_zcand)
   Seq seqs[kSegs * kSegSeqs];  // per-segment slots (k_zparse)
   Seq cseq[kSegs * kSegSeqs];  // compacted (k_zfinish)
   u8 sym[3][kBlock / kMinMatch];   // LL, OF, ML codes per sequence (k_zfinish)
   u16 fse[3][kBlock / kMinMatch];  // FSE bits per sequence and table: (count << 10) | value
+  u32 ls, ss, raw;  // literals / sequences section bytes, block stored raw (k_zlits, k_zseqs)
   u8 lits[kBlock];
   u8 clit[kBlock];
   u32 hist[256];
@@ -62,17 +63,37 @@ struct BlkScratch {  // per block, in global memory
 
 constexpr u32 kMaxSeqs = kBlock / kMinMatch;
 
-struct FinLds {
-  xz::Work W;
-  xz::SeqWork sw;
+// A block's compressed form in its slot: header (3 bytes), the literals
+// section from byte 3 (k_zlits), the sequences section from kSeqOff
+// (k_zseqs); k_zgather joins them.  Literals take at most 3 + 3 + 16 KiB;
+// sequences at most 4 header bytes, 3 table descriptions of < 128 bytes and
+// < 80 bits per sequence (three FSE codes of <= 9 bits, extra bits <= 16 +
+// 16 + 14) plus the final states.
+constexpr u32 kSeqOff = kBlock + 512;
+static_assert(kSeqOff + 4 + 3 * 128 + kMaxSeqs * 10 + 8 <= kSlot, "slot too small");
+
+struct LitLds {  // k_zlits
+  xz::Work W;  // Huffman build and the weights' FSE table
   xz::HufTable h;
   u32 hist[256];
-  u16 ov[kMaxSeqs];  // offset values (offset + 3 or a repeat code), < 2^15 in a 16 KiB block
-  u32 segn[kSegs], segl[kSegs], segll[kSegs], carry[kSegs], soff[kSegs + 1], loff[kSegs + 1];
+  u32 segl[kSegs], loff[kSegs + 1];
   u8 desc[160];
   u32 sbits[4];
+};
+
+struct FseLds {  // FSE table builds
+  u8 spread[1u << xz::kFseMaxLog];
+  i16 norm[xz::kFseMaxSym];
+  u8 ntmp[128];
+  u32 ncum[xz::kFseMaxSym + 1];  // cells of the symbols before
+};
+
+struct SeqLds {  // k_zseqs
+  xz::SeqWork sw;
+  FseLds F;
+  u16 ov[kMaxSeqs];  // offset values (offset + 3 or a repeat code), < 2^15 in a 16 KiB block
+  u32 segn[kSegs], segl[kSegs], segll[kSegs], carry[kSegs], soff[kSegs + 1];
   u32 fin[3];  // FSE states after the first sequence (flushed at the end)
-  u32 ncum[xz::kFseMaxSym + 1];  // FSE table builds: cells of the symbols before
 };
 
 __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
@@ -87,7 +108,7 @@ __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
 // computes them: one lane walks the sequences with the history in registers,
 // over LDS entries (offset | 0x8000 when the literal length is 0) rewritten
 // in place.
-__device__ void dev_offset_values(const Seq* cs, u32 ns, FinLds& L) {
+__device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
   const u32 lane = threadIdx.x;
   for (u32 i = lane; i < ns; i += 64) {
     const Seq q = cs[i];
@@ -185,9 +206,9 @@ __device__ bool dev_huf_build(const u32* count_in, xz::HufTable& h, xz::Work& W)
   if (n < 2) return false;
   u32* key = W.pos;
   u16* sym = W.sym;
-  u64* w = W.w;
-  i32* anc = W.parent;
-  u32* dep = W.depth;
+  u32* w = W.w;
+  i16* anc = W.parent;
+  u16* dep = W.depth;
   const u32 nodes = 2 * n - 1;
   u32 maxd;
   for (;;) {
@@ -214,10 +235,10 @@ __device__ bool dev_huf_build(const u32* count_in, xz::HufTable& h, xz::Work& W)
           else pick[c] = inner++;
         }
         w[node] = w[pick[0]] + w[pick[1]];
-        anc[pick[0]] = anc[pick[1]] = (i32)node;
+        anc[pick[0]] = anc[pick[1]] = (i16)node;
         ++node;
       }
-      anc[nodes - 1] = (i32)(nodes - 1);  // the root is its own ancestor
+      anc[nodes - 1] = (i16)(nodes - 1);  // the root is its own ancestor
     }
     __syncthreads();
     // distance to the root: d += d(anc), anc = anc(anc), 8 rounds (depth < 256)
@@ -227,7 +248,7 @@ __device__ bool dev_huf_build(const u32* count_in, xz::HufTable& h, xz::Work& W)
       const u32 i = lane + 64 * k;
       a[k] = i < nodes ? (u32)anc[i] : 0u;
       d[k] = i + 1 < nodes ? 1u : 0u;
-      if (i < nodes) dep[i] = d[k];
+      if (i < nodes) dep[i] = (u16)d[k];
     }
     __syncthreads();
     for (u32 r = 0; r < 8; ++r) {
@@ -239,8 +260,8 @@ __device__ bool dev_huf_build(const u32* count_in, xz::HufTable& h, xz::Work& W)
       __syncthreads();
       for (u32 k = 0; k < kPer; ++k)
         if (lane + 64 * k < nodes) {
-          dep[lane + 64 * k] = d[k];
-          anc[lane + 64 * k] = (i32)a[k];
+          dep[lane + 64 * k] = (u16)d[k];
+          anc[lane + 64 * k] = (i16)a[k];
         }
       __syncthreads();
     }
@@ -310,8 +331,7 @@ __device__ void dev_fse_normalize(const u32* count, u32 nsym, u32 total, u32 log
 // position (k * step) mod size, so each lane takes a run of steps, counts
 // the positions at or below the "-1" region, and places the cells whose
 // ranks a wavefront scan gives it.
-__device__ void dev_fse_build(xz::FseTable& t, const i16* norm, u32 nsym, u32 log, xz::Work& W,
-                              FinLds& L) {
+__device__ void dev_fse_build(xz::FseTable& t, const i16* norm, u32 nsym, u32 log, FseLds& L) {
   const u32 lane = threadIdx.x, size = 1u << log, mask = size - 1;
   const i32 n = lane < nsym ? (i32)norm[lane] : 0;
   const u32 m1 = n == -1 ? 1u : 0u, cn = n > 0 ? (u32)n : 0u;
@@ -321,7 +341,7 @@ __device__ void dev_fse_build(xz::FseTable& t, const i16* norm, u32 nsym, u32 lo
   const u32 nm1 = (u32)__shfl((int)(m1x + m1), 63, 64);
   const u32 high = size - 1 - nm1;
   const u32 ncx = wave_incl_scan(cn) - cn;
-  u8* spread = W.spread;
+  u8* spread = L.spread;
   if (lane < nsym) L.ncum[lane] = ncx;
   if (lane == 0) L.ncum[nsym] = high + 1;
   if (m1) spread[size - 1 - m1x] = (u8)lane;
@@ -379,17 +399,17 @@ __device__ void dev_fse_build(xz::FseTable& t, const i16* norm, u32 nsym, u32 lo
 
 // xz::choose_table: same choice, table and description.
 __device__ u32 dev_choose_table(int which, const u32* count, u32 n, xz::FseTable& t, u8* desc,
-                                u32& mode, xz::Work& W, FinLds& L) {
+                                u32& mode, FseLds& L) {
   const u32 lane = threadIdx.x;
   const u32 ms = xz::kMaxSym[which];
   const u32 c = lane <= ms ? count[lane] : 0u;
   const u32 present = wave_sum(c ? 1u : 0u);
   const u32 last = wave_max(c ? lane : 0u);
-  i16* norm = W.norm;
+  i16* norm = L.norm;
   if (present == 1 && n > 1) {  // RLE
     if (lane <= last) norm[lane] = lane == last ? 1 : 0;
     __syncthreads();
-    dev_fse_build(t, norm, last + 1, 0, W, L);
+    dev_fse_build(t, norm, last + 1, 0, L);
     if (lane == 0) desc[0] = (u8)last;
     mode = 1;
     return 1;
@@ -405,26 +425,26 @@ __device__ u32 dev_choose_table(int which, const u32* count, u32 n, xz::FseTable
   dev_fse_normalize(count, last + 1, n, log, norm);
   u64 own = c ? (u64)c * xz::cost16((u32)norm[lane], log) : 0ull;
   own = wave_sum(own);
-  u8* tmp = W.ntmp;
+  u8* tmp = L.ntmp;
   const u32 hl = xz::fse_write_ncount(tmp, norm, last + 1, log);  // uniform
   own += 16ull * 8 * hl;
   __syncthreads();
   if (pre_ok && pre <= own) {
-    dev_fse_build(t, pn, xz::kPreSyms[which], xz::kPreLog[which], W, L);
+    dev_fse_build(t, pn, xz::kPreSyms[which], xz::kPreLog[which], L);
     mode = 0;
     return 0;
   }
-  dev_fse_build(t, norm, last + 1, log, W, L);
+  dev_fse_build(t, norm, last + 1, log, L);
   for (u32 i = lane; i < hl; i += 64) desc[i] = tmp[i];
   mode = 2;
   return hl;
 }
 
 // xz::seq_write_tables: mode byte and table descriptions from sw.cnt.
-__device__ u32 dev_seq_write_tables(u8* out, u32 n, xz::SeqWork& sw, xz::Work& W, FinLds& L) {
+__device__ u32 dev_seq_write_tables(u8* out, u32 n, xz::SeqWork& sw, FseLds& L) {
   u32 m[3];
   u32 p = 1;
-  for (int k = 0; k < 3; ++k) p += dev_choose_table(k, sw.cnt[k], n, sw.t[k], out + p, m[k], W, L);
+  for (int k = 0; k < 3; ++k) p += dev_choose_table(k, sw.cnt[k], n, sw.t[k], out + p, m[k], L);
   if (threadIdx.x == 0) out[0] = (u8)((m[0] << 6) | (m[1] << 4) | (m[2] << 2));
   return p;
 }
@@ -436,7 +456,7 @@ __device__ u32 dev_seq_write_tables(u8* out, u32 n, xz::SeqWork& sw, xz::Work& W
 // lane packs a run of sequences in stream order (last sequence first) at
 // offsets from a wavefront scan, and lane 0 appends the final states and
 // the end mark.  sym / ov / tables must be ready.  Returns bytes written.
-__device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, FinLds& L) {
+__device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L) {
   const u32 lane = threadIdx.x;
   if (lane < 3) {
     const xz::FseTable& t = L.sw.t[lane];
@@ -540,7 +560,7 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, FinLds& L
 // final place: slot + base (slot 256-byte aligned; words are addressed from
 // the slot so that every 32-bit store and atomic is aligned).  Returns 0
 // when not representable or not smaller than raw.
-__device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, FinLds& L) {
+__device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, LitLds& L) {
   u8* out = slot + base;
   const u32 lane = threadIdx.x;
   const bool four = n > 1023;
@@ -631,30 +651,44 @@ __device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, FinLds& L)
   return total;
 }
 
-// Word-wise reader of a block held in LDS (4-byte aligned, padded by 8
+// Word-wise reader of a block held in LDS (4-byte aligned, padded by kPad
 // bytes): the same values as xz::ByteReader.
 struct LdsReader {
   __device__ static u32 r32(const u8* d, u32 p) {
     const u32* w = reinterpret_cast<const u32*>(d);
     return __builtin_amdgcn_alignbyte(w[(p >> 2) + 1], w[p >> 2], p & 3u);
   }
+  // 16 bytes per step: five aligned words of each side loaded together
+  // (one LDS round trip), compared four bytes at a time.
   __device__ static u32 len(const u8* d, u32 p, u32 q, u32 end) {
+    const u32* w = reinterpret_cast<const u32*>(d);
     u32 l = 0;
     while (p + l < end) {
-      const u32 x = r32(d, p + l) ^ r32(d, q + l);
-      const u32 r = end - (p + l);
-      if (x) {
-        const u32 e = (u32)__builtin_ctz(x) >> 3;
-        return l + (e < r ? e : r);
+      const u32 pa = (p + l) >> 2, ps = (p + l) & 3u, qa = (q + l) >> 2, qs = (q + l) & 3u;
+      u32 a[5], b[5];
+#pragma unroll
+      for (u32 k = 0; k < 5; ++k) {
+        a[k] = w[pa + k];
+        b[k] = w[qa + k];
       }
-      if (r <= 4) return l + r;
-      l += 4;
+      const u32 r = end - (p + l);
+#pragma unroll
+      for (u32 k = 0; k < 4; ++k) {
+        const u32 x = __builtin_amdgcn_alignbyte(a[k + 1], a[k], ps) ^
+                      __builtin_amdgcn_alignbyte(b[k + 1], b[k], qs);
+        if (x) {
+          const u32 e = 4 * k + ((u32)__builtin_ctz(x) >> 3);
+          return l + (e < r ? e : r);
+        }
+        if (r <= 4 * (k + 1)) return l + r;
+      }
+      l += 16;
     }
     return l;
   }
 };
 
-constexpr u32 kPad = 16;  // LDS block copies: readable bytes past the end
+constexpr u32 kPad = 32;  // LDS block copies: readable bytes past the end (match loads read 20)
 
 // Copies block bytes [0, n) into LDS (4-byte aligned source and target);
 // the padding is zeroed so no load reads uninitialised memory.
@@ -664,41 +698,133 @@ __device__ __forceinline__ void load_block(u8* data, const u8* src, u32 n, u32 t
   for (u32 i = (n & ~3u) + tid; i < n + kPad; i += nthr) data[i] = i < n ? src[i] : 0;
 }
 
+// Byte-equality of a block with itself shifted by 1, 8 and 16 bytes, one bit
+// per position (bit x of offset o: d[x] == d[x - o]); the run of ones from
+// bit p is the match length of offset o at p.
+constexpr u32 kFixedOff[3] = {1, 8, 16};
+constexpr u32 kEqWords = kBlock / 32 + 4;
+
+// The 4-bit mask of the zero bytes of x (bit i: byte i of x is 0).
+__device__ __forceinline__ u32 zero_bytes(u32 x) {
+  const u32 z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+  return ((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u);
+}
+
+// Length of the run of ones of eq from bit p, at most cap (<= kCap <= 35).
+__device__ __forceinline__ u32 eq_run(const u32* eq, u32 p, u32 cap) {
+  const u32 w = p >> 5, s = p & 31u;
+  const u64 lo = ((u64)eq[w] | ((u64)eq[w + 1] << 32)) >> s;
+  const u64 x = ~lo;
+  u32 l = x ? (u32)__builtin_ctzll(x) : 64u;
+  if (l == 64u - s && l < cap) {
+    const u32 hi = ~eq[w + 2];
+    l += hi ? (u32)__builtin_ctz(hi) : 32u;
+  }
+  return l < cap ? l : cap;
+}
+
 __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
                                                const ZBlk* __restrict__ blks,
                                                BlkScratch* __restrict__ scr) {
   __shared__ __align__(16) u8 data[kBlock + kPad];
-  __shared__ u32 ht4[1u << kHashLog], ht8[1u << kHashLog];
+  // first the two hash tables (u32, built with atomicMin), then the same
+  // tables as u16 in the first half and the equality masks in the second
+  __shared__ __align__(16) u32 tab[2u << kHashLog];
+  static_assert(3 * kEqWords * 4 <= (4u << kHashLog), "masks fit the second half");
   const ZBlk bk = blks[blockIdx.x];
   u32* cand = scr[blockIdx.x].cand;
   const u32 n = bk.n, tid = threadIdx.x;
+  constexpr u32 kH = 1u << kHashLog;
   load_block(data, pre + bk.src, n, tid, 256);
-  for (u32 i = tid; i < (1u << kHashLog); i += 256) ht4[i] = ht8[i] = 0xFFFFFFFFu;
+  for (u32 i = tid; i < 2 * kH; i += 256) tab[i] = 0xFFFFFFFFu;
   __syncthreads();
   for (u32 p = tid; p + 4 <= n; p += 256) {
     const u32 v = LdsReader::r32(data, p);
-    atomicMin(&ht4[xz::hash4(v)], p);
+    atomicMin(&tab[xz::hash4(v)], p);
     if (p + 8 <= n)
-      atomicMin(&ht8[xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))], p);
+      atomicMin(&tab[kH + xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))], p);
   }
   __syncthreads();
+  {
+    u32 t[2 * kH / 256];
+#pragma unroll
+    for (u32 k = 0; k < 2 * kH / 256; ++k) t[k] = tab[tid + 256 * k];
+    __syncthreads();
+    u16* h16 = reinterpret_cast<u16*>(tab);
+#pragma unroll
+    for (u32 k = 0; k < 2 * kH / 256; ++k) h16[tid + 256 * k] = (u16)(t[k] < 0xFFFFu ? t[k] : 0xFFFFu);
+  }
+  u32* eq = tab + kH;  // [3][kEqWords]
+  {
+    const u32* dw = reinterpret_cast<const u32*>(data);
+    const u32 rot = (tid >> 3) & 7u;  // conflict-free LDS banks across the wavefront
+    for (u32 w = tid; w < kEqWords; w += 256) {
+      u32 m1 = 0, m8 = 0, m16 = 0;
+      if (32 * w < n) {
+#pragma unroll
+        for (u32 i = 0; i < 8; ++i) {
+          const u32 ii = (i + rot) & 7u, j = 8 * w + ii;
+          const u32 a = dw[j];
+          const u32 p1 = j >= 1 ? dw[j - 1] : 0u;
+          const u32 p2 = j >= 2 ? dw[j - 2] : 0u;
+          const u32 p4 = j >= 4 ? dw[j - 4] : 0u;
+          m1 |= zero_bytes(a ^ __builtin_amdgcn_alignbyte(a, p1, 3u)) << (4 * ii);
+          m8 |= zero_bytes(a ^ p2) << (4 * ii);
+          m16 |= zero_bytes(a ^ p4) << (4 * ii);
+        }
+      }
+      eq[w] = m1;
+      eq[kEqWords + w] = m8;
+      eq[2 * kEqWords + w] = m16;
+    }
+  }
+  __syncthreads();
+  // xz::best_fixed: offsets 1, 8, 16 from the masks, then the hash hits
+  const u16* h4 = reinterpret_cast<const u16*>(tab);
+  const u16* h8 = h4 + kH;
   for (u32 p = tid; p < n; p += 256) {
-    const u8 info = xz::best_fixed<LdsReader>(data, n, ht4, ht8, p);
+    const u32 e = xz::seg_end(p, n);
     u32 c = 0;
-    if (info)
-      c = (((info & 31u) + kMinMatch) << 16) |
-          xz::cand_offset<LdsReader>(data, n, ht4, ht8, p, info >> 5);
+    if (p + kMinMatch <= e) {
+      const u32 cap = p + xz::kCap < e ? xz::kCap : e - p;
+      u32 best = 0, boff = 0;
+      i32 bs = -(1 << 30);
+      auto consider = [&](u32 off, u32 l) {
+        if (l < kMinMatch) return;
+        const i32 sc = (i32)l * xz::kLenWeight - ((i32)xz::highbit(off + 3) + xz::kOffCost);
+        if (sc > bs) {
+          bs = sc;
+          best = l;
+          boff = off;
+        }
+      };
+#pragma unroll
+      for (u32 k = 0; k < 3; ++k)
+        if (kFixedOff[k] <= p) consider(kFixedOff[k], eq_run(eq + k * kEqWords, p, cap));
+      if (p + 4 <= n) {
+        const u32 v = LdsReader::r32(data, p);
+        const u32 q4 = h4[xz::hash4(v)];
+        if (q4 < p) consider(p - q4, LdsReader::len(data, p, q4, p + cap));
+        if (p + 8 <= n) {
+          const u32 q8 = h8[xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))];
+          if (q8 < p) consider(p - q8, LdsReader::len(data, p, q8, p + cap));
+        }
+      }
+      if (best) c = (best << 16) | boff;
+    }
     cand[p] = c;
   }
 }
 
 constexpr u32 kParseBlocks = 2;  // blocks per k_zparse wavefront (32 segments each)
+constexpr u32 kWin = 16;         // candidates per k_zparse window fetch
 
 __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
                                                const ZBlk* __restrict__ blks, u32 nblk,
                                                BlkScratch* __restrict__ scr) {
   __shared__ __align__(16) u8 data[kParseBlocks][kBlock + kPad];
   __shared__ u32 hist[kParseBlocks][256];
+  __shared__ u32 win[64][kWin + 1];  // per lane: candidates of positions [wb, wb + kWin)
   const u32 lane = threadIdx.x;
   const u32 b0 = blockIdx.x * kParseBlocks;
   for (u32 k = 0; k < kParseBlocks; ++k) {
@@ -719,9 +845,25 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
       u32* h = hist[k];
       u32 ns = 0, nl = 0;
       // xz::cand_match on the resolved candidate: a capped length is
-      // extended to the segment end.
+      // extended to the segment end.  Candidates are fetched kWin at a time
+      // into the lane's LDS window (one global latency per window, not per
+      // position).
+      u32 wb = 0x80000000u;  // p - wb >= kWin for every p: the first call fetches
+      u32* wr = win[lane];
       auto best_at = [&](u32 p) -> u32 {
-        const u32 c = cand[p];
+        if (p - wb >= kWin) {
+          wb = p & ~3u;
+          const uint4* src = reinterpret_cast<const uint4*>(cand + wb);
+#pragma unroll
+          for (u32 q = 0; q < kWin / 4; ++q) {
+            const uint4 v = src[q];
+            wr[4 * q] = v.x;
+            wr[4 * q + 1] = v.y;
+            wr[4 * q + 2] = v.z;
+            wr[4 * q + 3] = v.w;
+          }
+        }
+        const u32 c = wr[p - wb];
         u32 len = c >> 16;
         if (len == xz::kCap) len = LdsReader::len(d, p, p - (c & 0xFFFFu), xz::seg_end(p, n));
         return (len << 16) | (c & 0xFFFFu);
@@ -748,70 +890,46 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
       for (u32 i = lane; i < 256; i += 64) scr[b0 + kk].hist[i] = hist[kk][i];
 }
 
-__global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
-                                                const ZBlk* __restrict__ blks,
-                                                BlkScratch* __restrict__ scr,
-                                                u8* __restrict__ slots,
-                                                u64* __restrict__ slot_len) {
-  __shared__ FinLds L;
+__global__ __launch_bounds__(64) void k_zlits(const ZBlk* __restrict__ blks,
+                                              BlkScratch* __restrict__ scr,
+                                              u8* __restrict__ slots) {
+  __shared__ LitLds L;
   const u32 b = blockIdx.x, lane = threadIdx.x;
   const ZBlk bk = blks[b];
   BlkScratch& S = scr[b];
   const u32 n = bk.n, nseg = (n + kSeg - 1) / kSeg;
   u8* out = slots + (u64)b * kSlot;
-  // 1. per-segment counts, offsets and literal carries
-  if (lane < kSegs) {
-    u32 ns = 0, nl = 0, llsum = 0;
-    if (lane < nseg) {
-      ns = S.nseq[lane];
-      nl = S.nlit[lane];
-      for (u32 k = 0; k < ns; ++k) llsum += S.seqs[lane * kSegSeqs + k].ll;
-    }
-    L.segn[lane] = ns;
-    L.segl[lane] = nl;
-    L.segll[lane] = llsum;
-  }
+  // 1. literals compacted in segment order
+  if (lane < kSegs) L.segl[lane] = lane < nseg ? S.nlit[lane] : 0u;
   for (u32 i = lane; i < 256; i += 64) L.hist[i] = S.hist[i];
   __syncthreads();
-  if (lane == 0) {
-    u32 so = 0, lo = 0, carry = 0;
-    for (u32 t = 0; t < nseg; ++t) {
-      L.soff[t] = so;
-      L.loff[t] = lo;
-      L.carry[t] = carry;
-      so += L.segn[t];
-      lo += L.segl[t];
-      carry = L.segn[t] ? L.segl[t] - L.segll[t] : carry + L.segl[t];
-    }
-    L.soff[nseg] = so;
-    L.loff[nseg] = lo;
+  {
+    const u32 v = lane < kSegs ? L.segl[lane] : 0u;
+    const u32 inc = wave_incl_scan(v);
+    if (lane <= kSegs) L.loff[lane] = inc - v;
   }
   __syncthreads();
-  const u32 ns = L.soff[nseg], nl = L.loff[nseg];
-  // 2. compaction (separate arrays: no overlap)
+  const u32 nl = L.loff[nseg];
   for (u32 t = 0; t < nseg; ++t) {
-    const u32 sn = L.segn[t], so = L.soff[t], ln = L.segl[t], lo = L.loff[t];
-    for (u32 k = lane; k < sn; k += 64) {
-      Seq q = S.seqs[t * kSegSeqs + k];
-      if (k == 0) q.ll += L.carry[t];
-      S.cseq[so + k] = q;
-    }
+    const u32 ln = L.segl[t], lo = L.loff[t];
     for (u32 i = lane; i < ln; i += 64) S.clit[lo + i] = S.lits[t * kSeg + i];
   }
   __threadfence_block();
   __syncthreads();
-  // 3. literals section (finish_block's choice)
+  // 2. the literals section (xz::finish_block's choice)
   u8* body = out + 3;
   u32 ls = 0;
-  bool rle = nl > 1;
-  for (u32 i = 0; rle && i < 256; ++i) rle = L.hist[i] == 0 || L.hist[i] == nl;
+  u32 other = 0;  // a symbol count other than 0 and nl
+  for (u32 i = lane; i < 256; i += 64) other |= (L.hist[i] != 0 && L.hist[i] != nl) ? 1u : 0u;
+  const bool rle = nl > 1 && wave_max(other) == 0;
   if (rle) {
     if (lane == 0) ls = xz::write_raw_lits(body, S.clit, nl, 1);
     ls = __shfl((int)ls, 0, 64);
   } else {
     if (nl >= 64 && dev_huf_build(L.hist, L.h, L.W)) {
       u64 bits = 0;
-      for (u32 c = 0; c < 256; ++c) bits += (u64)L.hist[c] * L.h.nbits[c];
+      for (u32 c = lane; c < 256; c += 64) bits += (u64)L.hist[c] * L.h.nbits[c];
+      bits = wave_sum(bits);
       if (bits / 8 + 16 < nl) ls = dev_huf_lits(out, 3, S.clit, nl, L);
     }
     if (!ls) {  // raw literals: header, then the bytes
@@ -828,10 +946,57 @@ __global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
       ls = h + nl;
     }
   }
+  if (lane == 0) S.ls = ls;
+}
+
+__global__ __launch_bounds__(64) void k_zseqs(const ZBlk* __restrict__ blks,
+                                              BlkScratch* __restrict__ scr,
+                                              u8* __restrict__ slots,
+                                              u64* __restrict__ slot_len) {
+  __shared__ SeqLds L;
+  const u32 b = blockIdx.x, lane = threadIdx.x;
+  const ZBlk bk = blks[b];
+  BlkScratch& S = scr[b];
+  const u32 n = bk.n, nseg = (n + kSeg - 1) / kSeg;
+  u8* out = slots + (u64)b * kSlot;
+  // 1. per-segment counts, offsets and literal carries; sequences compacted
+  //    (a segment's first sequence takes the literals left pending before it)
+  if (lane < kSegs) {
+    u32 ns = 0, nl = 0, llsum = 0;
+    if (lane < nseg) {
+      ns = S.nseq[lane];
+      nl = S.nlit[lane];
+      for (u32 k = 0; k < ns; ++k) llsum += S.seqs[lane * kSegSeqs + k].ll;
+    }
+    L.segn[lane] = ns;
+    L.segl[lane] = nl;
+    L.segll[lane] = llsum;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    u32 so = 0, carry = 0;
+    for (u32 t = 0; t < nseg; ++t) {
+      L.soff[t] = so;
+      L.carry[t] = carry;
+      so += L.segn[t];
+      carry = L.segn[t] ? L.segl[t] - L.segll[t] : carry + L.segl[t];
+    }
+    L.soff[nseg] = so;
+  }
+  __syncthreads();
+  const u32 ns = L.soff[nseg];
+  for (u32 t = 0; t < nseg; ++t) {
+    const u32 sn = L.segn[t], so = L.soff[t];
+    for (u32 k = lane; k < sn; k += 64) {
+      Seq q = S.seqs[t * kSegSeqs + k];
+      if (k == 0) q.ll += L.carry[t];
+      S.cseq[so + k] = q;
+    }
+  }
   __threadfence_block();
   __syncthreads();
-  // 4. sequences section
-  u8* sq = body + ls;
+  // 2. the sequences section, at kSeqOff in the slot
+  u8* sq = out + kSeqOff;
   u32 ss;
   if (lane == 0) ss = xz::seq_count_header(sq, ns);
   ss = __shfl((int)ss, 0, 64);
@@ -850,36 +1015,25 @@ __global__ __launch_bounds__(64) void k_zfinish(const u8* __restrict__ pre,
     }
     __threadfence_block();
     __syncthreads();
-    ss += dev_seq_write_tables(sq + ss, ns, L.sw, L.W, L);
+    ss += dev_seq_write_tables(sq + ss, ns, L.sw, L.F);
     __threadfence_block();
     __syncthreads();
-    ss += dev_seq_bits(out, (u32)(sq - out) + ss, S, ns, L);
+    ss += dev_seq_bits(out, kSeqOff + ss, S, ns, L);
   }
-  __threadfence_block();
-  __syncthreads();
-  // 5. the block: compressed, or raw when that is not smaller
-  const u32 size = ls + ss;
-  if (size >= n) {
-    for (u32 i = lane; i < n; i += 64) out[3 + i] = pre[bk.src + i];
-    if (lane == 0) {
-      out[0] = (u8)(n << 3);
-      out[1] = (u8)(n >> 5);
-      out[2] = (u8)(n >> 13);
-      slot_len[b] = 3 + n;
-    }
-  } else if (lane == 0) {
-    const u32 h = (2u << 1) | (size << 3);
-    out[0] = (u8)h;
-    out[1] = (u8)(h >> 8);
-    out[2] = (u8)(h >> 16);
-    slot_len[b] = 3 + size;
+  // 3. the block: compressed, or raw when that is not smaller
+  if (lane == 0) {
+    const u32 size = S.ls + ss;
+    const bool raw = size >= n;
+    S.ss = ss;
+    S.raw = raw ? 1u : 0u;
+    slot_len[b] = 3 + (raw ? n : size);
   }
 }
 
-// Frame f at out + pos[frames[f].first] + 13 f: magic, descriptor (single
-// segment, 8-byte content size), content size, then its blocks.
-__global__ __launch_bounds__(256) void k_zgather(const ZBlk* __restrict__ blks,
+__global__ __launch_bounds__(256) void k_zgather(const u8* __restrict__ pre,
+                                                 const ZBlk* __restrict__ blks,
                                                  const ZFrame* __restrict__ frames,
+                                                 const BlkScratch* __restrict__ scr,
                                                  const u8* __restrict__ slots,
                                                  const u64* __restrict__ pos,
                                                  const u64* __restrict__ slot_len,
@@ -887,19 +1041,25 @@ __global__ __launch_bounds__(256) void k_zgather(const ZBlk* __restrict__ blks,
   const u32 b = blockIdx.x;
   const ZBlk bk = blks[b];
   const ZFrame fr = frames[bk.frame];
+  const BlkScratch& S = scr[b];
   u8* dst = out + pos[b] + 13ull * (bk.frame + 1);
   const u8* src = slots + (u64)b * kSlot;
-  const u32 len = (u32)slot_len[b];
+  const u32 len = (u32)slot_len[b], ls = S.ls, raw = S.raw;
+  const u32 last = b == fr.first + fr.nblk - 1 ? 1u : 0u;  // last block of the frame
+  const u32 h = raw ? (bk.n << 3) | last : (2u << 1) | ((len - 3) << 3) | last;
   for (u32 i = threadIdx.x; i < len; i += 256) {
-    u8 v = src[i];
-    if (i == 0 && b == fr.first + fr.nblk - 1) v |= 1;  // last block of the frame
+    u8 v;
+    if (i < 3) v = (u8)(h >> (8 * i));
+    else if (raw) v = pre[bk.src + i - 3];
+    else if (i - 3 < ls) v = src[i];
+    else v = src[kSeqOff + (i - 3 - ls)];
     dst[i] = v;
   }
   if (b == fr.first && threadIdx.x < 13) {
-    u8* h = out + pos[b] + 13ull * bk.frame;
+    u8* fh = out + pos[b] + 13ull * bk.frame;
     const u32 i = threadIdx.x;
     const u64 magic = 0xFD2FB528ull;
-    h[i] = i < 4 ? (u8)(magic >> (8 * i)) : i == 4 ? (u8)0xE0 : (u8)(fr.len >> (8 * (i - 5)));
+    fh[i] = i < 4 ? (u8)(magic >> (8 * i)) : i == 4 ? (u8)0xE0 : (u8)(fr.len >> (8 * (i - 5)));
   }
 }
 
@@ -920,11 +1080,14 @@ void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* fram
   k_zparse<<<(nblk + kParseBlocks - 1) / kParseBlocks, 64, 0, s>>>(pre, blks, nblk, scr);
   check_hip(hipGetLastError(), "k_zparse");
   if (g_profiler) g_profiler->mark("k_zparse", s);
-  k_zfinish<<<nblk, 64, 0, s>>>(pre, blks, scr, slots, slot_len);
-  check_hip(hipGetLastError(), "k_zfinish");
-  if (g_profiler) g_profiler->mark("k_zfinish", s);
+  k_zlits<<<nblk, 64, 0, s>>>(blks, scr, slots);
+  check_hip(hipGetLastError(), "k_zlits");
+  if (g_profiler) g_profiler->mark("k_zlits", s);
+  k_zseqs<<<nblk, 64, 0, s>>>(blks, scr, slots, slot_len);
+  check_hip(hipGetLastError(), "k_zseqs");
+  if (g_profiler) g_profiler->mark("k_zseqs", s);
   scan_excl_u64(slot_len, pos, nblk, pos + nblk, ar, s);
-  k_zgather<<<nblk, 256, 0, s>>>(blks, frames, slots, pos, slot_len, out);
+  k_zgather<<<nblk, 256, 0, s>>>(pre, blks, frames, scr, slots, pos, slot_len, out);
   check_hip(hipGetLastError(), "k_zgather");
   if (g_profiler) g_profiler->mark("k_zgather", s);
 }

@@ -160,9 +160,9 @@ struct Work {
   u32 cnt[256];
   u16 sym[256], tmp[256];
   u32 pos[257];
-  u64 w[512];
-  i32 parent[512];
-  u32 depth[512];
+  u32 w[512];       // tree weights: sums of one block's counts (< 2^15)
+  i16 parent[512];
+  u16 depth[512];
   u8 wt[256];
   u8 dtmp[160];
   u32 wcount[16];
@@ -313,8 +313,8 @@ XZ_HD inline bool huf_build(const u32* count_in, HufTable& h, Work& W) {
       for (u32 i = 0; i < n; ++i) tmp[pos[(cnt[sym[i]] >> sh) & 255u]++] = sym[i];
       for (u32 i = 0; i < n; ++i) sym[i] = tmp[i];
     }
-    u64* w = W.w;
-    i32* parent = W.parent;
+    u32* w = W.w;
+    i16* parent = W.parent;
     for (u32 i = 0; i < n; ++i) w[i] = cnt[sym[i]];
     u32 leaf = 0, node = n, inner = n;  // inner nodes created at [n, 2n-1)
     for (u32 k = 0; k + 1 < n; ++k) {
@@ -324,14 +324,14 @@ XZ_HD inline bool huf_build(const u32* count_in, HufTable& h, Work& W) {
         else pick[c] = inner++;
       }
       w[node] = w[pick[0]] + w[pick[1]];
-      parent[pick[0]] = parent[pick[1]] = (i32)node;
+      parent[pick[0]] = parent[pick[1]] = (i16)node;
       ++node;
     }
-    u32* depth = W.depth;
+    u16* depth = W.depth;
     depth[node - 1] = 0;
     u32 maxd = 0;
     for (i32 i = (i32)node - 2; i >= 0; --i) {
-      depth[i] = depth[parent[i]] + 1;
+      depth[i] = (u16)(depth[parent[i]] + 1);
       if ((u32)i < n && depth[i] > maxd) maxd = depth[i];
     }
     if (maxd <= kMaxHufBits) {
@@ -879,6 +879,11 @@ XZ_HD inline u32 cand_match(const u8* d, u32 n, const u32* ht4, const u32* ht8, 
 template <typename R = ByteReader, typename Best, typename Emit>
 XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&& emit) {
   u32 rep[3] = {0, 0, 0};
+  // Lengths of the repeat offsets at position `lp` (kNone: not measured): a
+  // match of length l >= 1 at p has length l - 1 at p + 1 under the same
+  // offset and bound, so the lazy look-ahead need not measure it again.
+  constexpr u32 kNone = 0xFFFFFFFFu;
+  u32 lp = kNone, rl[3] = {kNone, kNone, kNone};
   auto find = [&](u32 p, u32& best, u32& boff) -> i32 {
     const u32 v = best_at(p);
     best = v >> 16;
@@ -888,10 +893,14 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
       return (i32)l * kLenWeight - (r ? kRepCost : (i32)highbit(off + 3) + kOffCost);
     };
     i32 bs = best ? sc_of(best, boff) : -(1 << 30);
+    const bool next = lp != kNone && lp + 1 == p;
     for (u32 c = 0; c < 3; ++c) {
       const u32 off = rep[c];
+      const u32 known = next && rl[c] != kNone && rl[c] >= 1 ? rl[c] - 1 : kNone;
+      rl[c] = kNone;
       if (!off || off > p || off == boff) continue;
-      const u32 l = R::len(d, p, p - off, b);
+      const u32 l = known != kNone ? known : R::len(d, p, p - off, b);
+      rl[c] = l;
       if (l < kMinMatch) continue;
       const i32 sc = sc_of(l, off);
       if (sc > bs) {
@@ -900,6 +909,7 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
         boff = off;
       }
     }
+    lp = p;
     return bs;
   };
   u32 p = a, lit0 = a;
@@ -922,6 +932,7 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
     emit(lit0, p, best, boff);
     p += best;
     lit0 = p;
+    lp = kNone;  // the history may change
     if (boff != rep[0]) {
       rep[2] = rep[1];
       rep[1] = rep[0];
