@@ -115,16 +115,18 @@ __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
     L.ov[i] = (u16)(q.off | (q.ll ? 0u : 0x8000u));
   }
   __syncthreads();
-  if (lane == 0) {
+  // The whole wavefront walks the history on wavefront-uniform values
+  // (scalar registers and branches); lanes 0..kB-1 store a batch's results.
+  {
     u32 r0 = 0, r1 = 0, r2 = 0, known = 0;
-    constexpr u32 kB = 8;  // entries loaded ahead of the dependent chain
+    constexpr u32 kB = 8;
     for (u32 i0 = 0; i0 < ns; i0 += kB) {
-      u32 xb[kB];
+      u32 xb[kB], res[kB];
 #pragma unroll
-      for (u32 q = 0; q < kB; ++q) xb[q] = i0 + q < ns ? L.ov[i0 + q] : 0u;
+      for (u32 q = 0; q < kB; ++q)
+        xb[q] = __builtin_amdgcn_readfirstlane(i0 + q < ns ? (u32)L.ov[i0 + q] : 0u);
 #pragma unroll
       for (u32 q = 0; q < kB; ++q) {
-        if (i0 + q >= ns) break;
         const u32 x = xb[q], o = x & 0x7FFFu;
         u32 ov = 0;
         if (!(x & 0x8000u)) {
@@ -165,8 +167,12 @@ __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
           r0 = o;
           known = known < 3 ? known + 1 : 3;
         }
-        L.ov[i0 + q] = (u16)ov;
+        res[q] = (i0 + q < ns) ? ov : 0u;
       }
+      u32 mine = res[0];
+#pragma unroll
+      for (u32 q = 1; q < kB; ++q) mine = lane == q ? res[q] : mine;
+      if (lane < kB && i0 + lane < ns) L.ov[i0 + lane] = (u16)mine;
     }
   }
   __syncthreads();
@@ -464,22 +470,32 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
     u16* fo = S.fse[lane];
     xz::FseState st;
     xz::fse_init(st, t, sym[ns - 1]);
-    constexpr u32 kB = 16;  // codes (and their table entries) fetched ahead of the chain
-    for (u32 k0 = 0; k0 + 1 < ns; k0 += kB) {  // step k encodes sequence ns - 2 - k
-      u32 dnb[kB], dfs[kB];
+    // Step k encodes sequence ns - 2 - k.  The next batch's codes are
+    // loaded before this batch's records are stored, so waiting for those
+    // loads never waits for the stores.
+    constexpr u32 kB = 16;
+    auto code = [&](u32 k) -> u32 { return k + 1 < ns ? sym[ns - 2 - k] : 0u; };
+    u32 nxt[kB];
+#pragma unroll
+    for (u32 q = 0; q < kB; ++q) nxt[q] = code(q);
+    for (u32 k0 = 0; k0 + 1 < ns; k0 += kB) {
+      u32 dnb[kB], dfs[kB], rec[kB];
 #pragma unroll
       for (u32 q = 0; q < kB; ++q) {
-        const u32 c = k0 + q + 1 < ns ? sym[ns - 2 - k0 - q] : 0u;
-        dnb[q] = (u32)t.dnb[c];
-        dfs[q] = (u32)t.dfs[c];
+        dnb[q] = (u32)t.dnb[nxt[q]];
+        dfs[q] = (u32)t.dfs[nxt[q]];
       }
 #pragma unroll
+      for (u32 q = 0; q < kB; ++q) nxt[q] = code(k0 + kB + q);
+#pragma unroll
       for (u32 q = 0; q < kB; ++q) {
-        if (k0 + q + 1 >= ns) break;
         const u32 nbo = (st.v + dnb[q]) >> 16;
-        fo[ns - 2 - k0 - q] = (u16)((nbo << 10) | (st.v & ((1u << nbo) - 1u)));
-        st.v = t.state[(st.v >> nbo) + dfs[q]];
+        rec[q] = (nbo << 10) | (st.v & ((1u << nbo) - 1u));
+        if (k0 + q + 1 < ns) st.v = t.state[(st.v >> nbo) + dfs[q]];
       }
+#pragma unroll
+      for (u32 q = 0; q < kB; ++q)
+        if (k0 + q + 1 < ns) fo[ns - 2 - k0 - q] = (u16)rec[q];
     }
     L.fin[lane] = st.v;
   }
