@@ -1987,12 +1987,18 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
       }
     };
     const xnyz_tuning saved = ctx->tuning[c];
+    // The fastest of `reps` timed calls (after one untimed call): the
+    // minimum is the estimate least disturbed by other activity on the host.
     auto timed = [&](const xnyz_tuning& t) {
       ctx->tuning[c] = t;
       call();  // untimed: buffers, captured graphs and host pages settle
-      const double t0 = now_ms();
-      for (u32 r = 0; r < reps; ++r) call();
-      return (now_ms() - t0) / reps;
+      double best = 1e30;
+      for (u32 r = 0; r < reps; ++r) {
+        const double t0 = now_ms();
+        call();
+        best = std::min(best, now_ms() - t0);
+      }
+      return best;
     };
     try {
       xnyz_tuning cur = saved;
@@ -2019,8 +2025,10 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
         const double s0 = cur.gpu_frame_share;
         if (s0 > 0)
           sweep([](xnyz_tuning& t, double v) { t.gpu_frame_share = (float)v; },
-                {std::max(0.0, s0 - 0.05), std::min(0.9, s0 + 0.05)});
+                {std::max(0.0, s0 - 0.05), std::max(0.0, s0 - 0.025), std::min(0.9, s0 + 0.025),
+                 std::min(0.9, s0 + 0.05)});
         sweep([](xnyz_tuning& t, double v) { t.copy_slice_log = (u32)v; }, {22, 24, 26});
+        sweep([](xnyz_tuning& t, double v) { t.zstd_chunk_log = (u32)v; }, {18, 19, 20, 21});
       }
       ctx->tuning[c] = cur;
       if (best) *best = cur;
