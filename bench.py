@@ -112,7 +112,8 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
         "k_sec_bits": 8 * n_coded,
         "k_zcand": dev_frame_bytes,
         "k_zparse": dev_frame_bytes,
-        "k_zfinish": dev_frame_bytes,
+        "k_zlits": dev_frame_bytes,
+        "k_zseqs": dev_frame_bytes,
     }
     return table.get(kernel)
 

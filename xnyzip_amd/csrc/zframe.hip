@@ -2,29 +2,33 @@
 // streams in HBM (format and parse rules in zframe.hpp).
 //
 //   k_zcand    one workgroup per 16 KiB block: the block and its two hash
-//              tables in LDS (48 KiB); all 256 threads evaluate every
+//              tables in LDS (48 KiB; the tables are narrowed to 16 bits
+//              after the build so byte-equality masks for the offsets 1, 8
+//              and 16 fit beside them); all 256 threads evaluate every
 //              position's best fixed candidate and write it, resolved to
 //              (capped length, offset), to global scratch.
 //   k_zparse   one wavefront per two blocks, one lane per 512-byte segment:
-//              only the blocks' bytes sit in LDS (33 KiB, so four such
-//              wavefronts share a CU), the candidates stream from scratch;
-//              each lane runs the lazy parse of its segment, writing its
-//              sequences and literals into per-segment slots and the
-//              block's literal histogram.
+//              the blocks' bytes and a candidate window per lane in LDS
+//              (38 KiB, so four such wavefronts share a CU); each lane runs
+//              the lazy parse of its segment, writing its sequences and
+//              literals into per-segment slots and the block's literal
+//              histogram.
 //   Both read the block with aligned 32-bit LDS loads and byte funnel
-//   shifts (match lengths four bytes per step); results equal the
+//   shifts (match lengths 16 bytes per round trip); results equal the
 //   byte-wise definitions in zframe.hpp.
-//   k_zfinish  one wavefront per block: slots compacted in order (the first
-//              sequence of a segment takes the literals left pending by the
-//              segments before it), then the block as zframe.hpp's
-//              finish_block writes it -- the serial steps (Huffman and FSE
-//              table builds, repeat-offset history, the FSE chain) executed
-//              uniformly by all 64 lanes on LDS working storage, with
-//              sequences fetched 64 at a time and broadcast by lane
-//              shuffles; Huffman streams bit-packed by all lanes at offsets
-//              from a wavefront scan.
-//   k_zgather  blocks copied behind their frame headers, positions from a
-//              device-wide scan of the block sizes.
+//   k_zlits    one wavefront per block: literals compacted in order, then
+//              the literals section as zframe.hpp's finish_block chooses it
+//              (Huffman table built across the lanes, streams bit-packed by
+//              all lanes at offsets from a wavefront scan).
+//   k_zseqs    one wavefront per block: sequences compacted in order (the
+//              first sequence of a segment takes the literals left pending
+//              by the segments before it), repeat-offset codes, the three
+//              FSE tables built across the lanes, the FSE state chains on
+//              three lanes and the bitstream packed by all lanes; the block
+//              size (or raw) decided.
+//   k_zgather  each block's header and sections (or its raw bytes) copied
+//              behind its frame header, positions from a device-wide scan of
+//              the block sizes.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -51,12 +55,12 @@ constexpr u32 kSegSeqs = kSeg / kMinMatch;  // most sequences one segment can ho
 struct alignas(16) BlkScratch {  // per block, in global memory
   u32 cand[kBlock];            // (capped length << 16) | offset, 0 = none (k_zcand)
   Seq seqs[kSegs * kSegSeqs];  // per-segment slots (k_zparse)
-  Seq cseq[kSegs * kSegSeqs];  // compacted (k_zfinish)
-  u8 sym[3][kBlock / kMinMatch];   // LL, OF, ML codes per sequence (k_zfinish)
+  Seq cseq[kSegs * kSegSeqs];  // compacted (k_zseqs)
+  u8 sym[3][kBlock / kMinMatch];   // LL, OF, ML codes per sequence (k_zseqs)
   u16 fse[3][kBlock / kMinMatch];  // FSE bits per sequence and table: (count << 10) | value
   u32 ls, ss, raw;  // literals / sequences section bytes, block stored raw (k_zlits, k_zseqs)
   u8 lits[kBlock];
-  u8 clit[kBlock];
+  u8 clit[kBlock];  // compacted (k_zlits)
   u32 hist[256];
   u32 nseq[kSegs], nlit[kSegs];
 };
