@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define XNYZ_ABI_VERSION 3
+#define XNYZ_ABI_VERSION 4
 
 enum xnyz_status {
   XNYZ_OK = 0,
@@ -299,6 +299,8 @@ int xnyz_graphs_enable(xnyz_ctx* ctx, int on);
  *   copy_first       split frames: 1 = the head's copy-back runs before the
  *                    device encoder on the context's stream, 0 = on a second
  *                    stream beside it (1)
+ *   radix_bits       digit width of the key sort, 6..8 bits (8)
+ *   radix_tile       keys per radix-sort workgroup, 2048 or 4096 (4096)
  * The defaults are given in parentheses.  Every setting yields a valid frame
  * of the same stream; only the frame bytes and the speed change. */
 typedef struct xnyz_tuning {
@@ -307,6 +309,8 @@ typedef struct xnyz_tuning {
   float gpu_frame_share;
   uint32_t persist_blocks;
   uint32_t copy_first;
+  uint32_t radix_bits;
+  uint32_t radix_tile;
 } xnyz_tuning;
 
 #define XNYZ_SIZE_CLASSES 33
@@ -316,7 +320,7 @@ int xnyz_tuning_set(xnyz_ctx* ctx, int size_class, const xnyz_tuning* t);
 int xnyz_tuning_get(xnyz_ctx* ctx, int size_class, xnyz_tuning* t);
 
 /* The table as text, one line per class ("class chunk_log slice_log share
- * blocks copy_first"); load accepts any subset of the classes.  No reference
+ * blocks copy_first radix_bits radix_tile"); load accepts any subset of the classes.  No reference
  * counterpart. */
 int xnyz_tuning_save(xnyz_ctx* ctx, const char* path);
 int xnyz_tuning_load(xnyz_ctx* ctx, const char* path);

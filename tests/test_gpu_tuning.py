@@ -14,7 +14,7 @@ torch = pytest.importorskip("torch")
 
 MT, NONE = 1, 2
 DEFAULT = {"zstd_chunk_log": 20, "copy_slice_log": 24, "gpu_frame_share": 0.0,
-           "persist_blocks": 2048, "copy_first": 1}
+           "persist_blocks": 2048, "copy_first": 1, "radix_bits": 8, "radix_tile": 4096}
 
 
 @pytest.fixture(scope="module")
@@ -63,11 +63,11 @@ def test_table_set_get_save_load(X, ctx, tmp_path):
             assert abs(a["gpu_frame_share"] - b["gpu_frame_share"]) < 1e-4, c
         # A malformed file or an out-of-range row changes nothing.
         bad = tmp_path / "bad.txt"
-        bad.write_text("# comment\n3 20 24 0.1 2048 1\n4 20 24\n")
+        bad.write_text("# comment\n3 20 24 0.1 2048 1 8 4096\n4 20 24\n")
         with pytest.raises(_lib.XnyzError) as e:
             other.load_tuning(str(bad))
         assert e.value.code == _lib.XNYZ_EINVAL and other.tuning(3) == DEFAULT
-        bad.write_text("3 20 24 0.1 2048 0\n5 30 24 0.0 2048 1\n")
+        bad.write_text("3 20 24 0.1 2048 0 8 4096\n5 30 24 0.0 2048 1 8 4096\n")
         with pytest.raises(_lib.XnyzError) as e:
             other.load_tuning(str(bad))
         assert e.value.code == _lib.XNYZ_EINVAL and other.tuning(3) == DEFAULT
@@ -77,7 +77,8 @@ def test_table_set_get_save_load(X, ctx, tmp_path):
     finally:
         other.close()
     for field, value in (("zstd_chunk_log", 15), ("copy_slice_log", 31),
-                         ("gpu_frame_share", 0.95), ("persist_blocks", 63), ("copy_first", 2)):
+                         ("gpu_frame_share", 0.95), ("persist_blocks", 63), ("copy_first", 2),
+                         ("radix_bits", 9), ("radix_tile", 1024)):
         with pytest.raises(_lib.XnyzError) as e:
             ctx.set_tuning(7, **{field: value})
         assert e.value.code == _lib.XNYZ_EINVAL
@@ -159,6 +160,25 @@ def test_persist_blocks_same_stream(X, ctx, blocks):
         assert pre == O.compress(c, O.TO, 0.01, True, rle).pre, (blocks, rle)
 
 
+@pytest.mark.parametrize("bits,tile", [(6, 4096), (7, 2048), (8, 2048), (6, 2048)])
+def test_radix_shapes_same_stream(X, ctx, bits, tile):
+    """Every radix digit width and tile size sorts to the same order: the
+    streams equal the oracle's for narrow keys (a uniform cloud, a cloud
+    with duplicates), wide keys (~2^50 blocks: a u64 + u32 key) and a
+    non-RLE cloud whose permutation payload rides along."""
+    ctx.set_tuning(-1, radix_bits=bits, radix_tile=tile)
+    dup = np.repeat(_uniform(81, 5000), 7, axis=0)
+    cases = [(_uniform(82, 300_000), O.TO, 0.01, True, True),
+             (dup, O.CUBE, 0.02, False, True),
+             (_uniform(83, 20000, scale=20000.0), O.TO, 0.001, True, False),
+             (_uniform(84, 20000, scale=20000.0), O.TO, 0.001, True, True),
+             (_uniform(85, 123_457), O.CUBE, 0.05, True, False)]
+    for i, (c, q, l2, hil, rle) in enumerate(cases):
+        _, pre, _, _ = X.compress(torch.from_numpy(c).cuda(), q, l2, hil, rle,
+                                  frame_mode=NONE, want_prezstd=True, ctx=ctx)
+        assert pre == O.compress(c, q, l2, hil, rle).pre, (bits, tile, i)
+
+
 def test_autotune(X, ctx):
     n = 1_200_000
     c = _uniform(80, n)
@@ -167,6 +187,7 @@ def test_autotune(X, ctx):
     best, ms = X.autotune(dev, opts, reps=1, ctx=ctx)
     assert ms > 0
     got = ctx.tuning(_size_class(n))
+    assert got["radix_bits"] == best["radix_bits"] and got["radix_tile"] == best["radix_tile"]
     assert got["zstd_chunk_log"] == best["zstd_chunk_log"]
     assert got["persist_blocks"] == best["persist_blocks"]
     assert abs(got["gpu_frame_share"] - best["gpu_frame_share"]) < 1e-7

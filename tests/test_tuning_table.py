@@ -12,9 +12,11 @@ def test_packaged_table_rows():
         for line in f:
             if line.lstrip().startswith("#") or not line.strip():
                 continue
-            c, chunk, slice_log, share, blocks, first = line.split()
-            rows[int(c)] = (int(chunk), int(slice_log), float(share), int(blocks), int(first))
+            c, chunk, slice_log, share, blocks, first, rbits, rtile = line.split()
+            rows[int(c)] = (int(chunk), int(slice_log), float(share), int(blocks), int(first),
+                            int(rbits), int(rtile))
     assert sorted(rows) == list(range(33))
-    for c, (chunk, slice_log, share, blocks, first) in rows.items():
+    for c, (chunk, slice_log, share, blocks, first, rbits, rtile) in rows.items():
         assert 16 <= chunk <= 24 and 20 <= slice_log <= 30, c
         assert 0.0 <= share <= 0.9 and 64 <= blocks <= 16384 and first in (0, 1), c
+        assert 6 <= rbits <= 8 and rtile in (2048, 4096), c

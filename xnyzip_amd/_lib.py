@@ -61,7 +61,8 @@ class Tuning(C.Structure):
     """Run-time parameters of a context for one size class (include/xnyz.h)."""
     _fields_ = [("zstd_chunk_log", C.c_uint32), ("copy_slice_log", C.c_uint32),
                 ("gpu_frame_share", C.c_float), ("persist_blocks", C.c_uint32),
-                ("copy_first", C.c_uint32)]
+                ("copy_first", C.c_uint32), ("radix_bits", C.c_uint32),
+                ("radix_tile", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

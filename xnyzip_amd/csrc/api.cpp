@@ -385,7 +385,7 @@ u64 frame_scratch(u64 len, int mode, int threads, u64 chunk) {
 
 // ---- run-time parameters (xnyz_tuning) ---------------------------------------
 
-xnyz_tuning default_tuning() { return xnyz_tuning{20, 24, 0.f, 2048, 1}; }
+xnyz_tuning default_tuning() { return xnyz_tuning{20, 24, 0.f, 2048, 1, 8, 4096}; }
 
 // Size class of a call: floor(log2(total points)), 0 for 0 or 1 point.
 int size_class(u64 n) { return n <= 1 ? 0 : 63 - __builtin_clzll(n); }
@@ -411,6 +411,9 @@ void check_tuning(const xnyz_tuning& t) {
   if (t.persist_blocks < 64 || t.persist_blocks > 16384)
     throw Error(XNYZ_EINVAL, "persist_blocks must be in [64, 16384]");
   if (t.copy_first > 1) throw Error(XNYZ_EINVAL, "copy_first must be 0 or 1");
+  if (t.radix_bits < 6 || t.radix_bits > 8) throw Error(XNYZ_EINVAL, "radix_bits must be in [6, 8]");
+  if (t.radix_tile != 2048 && t.radix_tile != 4096)
+    throw Error(XNYZ_EINVAL, "radix_tile must be 2048 or 4096");
 }
 
 }  // namespace
@@ -713,6 +716,9 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ProfGuard prof_guard(ctx);
   const int threads = host_threads(0);
   const xnyz_tuning tu = ctx->tuning[size_class(N64)];
+  RadixShape rshape;
+  rshape.bits = tu.radix_bits;
+  rshape.sub = tu.radix_tile == 2048 ? 2 : 1;
 
   // ---- tiles and segment parameters ----------------------------------------
   u32 ntiles = 0;
@@ -758,7 +764,9 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   const u32 large_cap = N / kCntBinsSeg + S + 2;
   ctx->large.ensure((size_t)large_cap * 12);
   ctx->nlarge.ensure(8);
-  ctx->arena.ensure((size_t)ntiles * (256 * 4 + kNumSec * 16 + 64) + (4u << 20));
+  // radix digit counts (up to 256 digits x 2 radix tiles per tile), section
+  // sizing, scan partials
+  ctx->arena.ensure((size_t)ntiles * (512 * 4 + kNumSec * 16 + 64) + (4u << 20));
   Arena ar{ctx->arena.as<char>(), ctx->arena.cap, 0};
   ctx->h2d(ctx->tiles, ctx->h_tiles, sizeof(Tile) * ntiles);
   ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
@@ -891,13 +899,13 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   auto stage2 = [&] {
     launch_keygen(bt, d_xyz, ctx->hs, ka, l64a ? nullptr : la, ia, l64a, st);
     if (want_perm && wide) {
-      cur = radix_sort_wide_v(bt, ka, kb, l64a, l64b, hi_bits, ar, st);
+      cur = radix_sort_wide_v(bt, ka, kb, l64a, l64b, hi_bits, rshape, ar, st);
       launch_split_lo64(cur ? l64b : l64a, ctx->lo[cur].as<u32>(), ia, N, st);
       perm_dev = ia;
     } else if (wide) {
-      cur = radix_sort_wide(bt, ka, kb, la, lb, hi_bits, ar, st);
+      cur = radix_sort_wide(bt, ka, kb, la, lb, hi_bits, rshape, ar, st);
     } else {
-      cur = radix_sort(bt, ka, kb, ia, ib, narrow_bits, ar, st);
+      cur = radix_sort(bt, ka, kb, ia, ib, narrow_bits, rshape, ar, st);
       perm_dev = want_perm ? (cur ? ib : ia) : nullptr;
     }
     KeyView k{ctx->keys[cur].as<u64>(), wide ? ctx->lo[cur].as<u32>() : nullptr};
@@ -913,7 +921,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     GraphKey key;
     key.add(d_xyz).add(bt).add(ka).add(kb).add(la).add(lb).add(ia).add(ib).add(l64a).add(l64b)
         .add(cnt).add(ctx->pos.p).add(ctx->cells.p).add(ctx->cells_lo.p).add(ar.base).add(ar.off)
-        .add(narrow_bits).add(hi_bits).add(any_rle).add(any_block).add(any_direct).add(want_perm);
+        .add(narrow_bits).add(hi_bits).add(any_rle).add(any_block).add(any_direct).add(want_perm)
+        .add(rshape.bits).add(rshape.sub);
     ctx->stage2.run(key, st, stage2, cur, perm_dev);
   } else {
     stage2();
@@ -1915,11 +1924,13 @@ int xnyz_tuning_save(xnyz_ctx* ctx, const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) throw Error(XNYZ_EIO, std::string("cannot write ") + path);
     std::fprintf(f,
-                 "# class zstd_chunk_log copy_slice_log gpu_frame_share persist_blocks copy_first\n");
+                 "# class zstd_chunk_log copy_slice_log gpu_frame_share persist_blocks copy_first "
+                 "radix_bits radix_tile\n");
     for (int c = 0; c < XNYZ_SIZE_CLASSES; ++c) {
       const xnyz_tuning& t = ctx->tuning[c];
-      std::fprintf(f, "%d %u %u %.4f %u %u\n", c, t.zstd_chunk_log, t.copy_slice_log,
-                   (double)t.gpu_frame_share, t.persist_blocks, t.copy_first);
+      std::fprintf(f, "%d %u %u %.4f %u %u %u %u\n", c, t.zstd_chunk_log, t.copy_slice_log,
+                   (double)t.gpu_frame_share, t.persist_blocks, t.copy_first, t.radix_bits,
+                   t.radix_tile);
     }
     const bool ok = std::fclose(f) == 0;
     if (!ok) throw Error(XNYZ_EIO, std::string("cannot write ") + path);
@@ -1943,8 +1954,9 @@ int xnyz_tuning_load(xnyz_ctx* ctx, const char* path) {
       if (*p == '#' || *p == '\n' || *p == 0) continue;
       int c;
       xnyz_tuning t;
-      if (std::sscanf(p, "%d %u %u %f %u %u", &c, &t.zstd_chunk_log, &t.copy_slice_log,
-                      &t.gpu_frame_share, &t.persist_blocks, &t.copy_first) != 6 ||
+      if (std::sscanf(p, "%d %u %u %f %u %u %u %u", &c, &t.zstd_chunk_log, &t.copy_slice_log,
+                      &t.gpu_frame_share, &t.persist_blocks, &t.copy_first, &t.radix_bits,
+                      &t.radix_tile) != 8 ||
           c < 0 || c >= XNYZ_SIZE_CLASSES) {
         bad = true;
         break;
@@ -1953,7 +1965,8 @@ int xnyz_tuning_load(xnyz_ctx* ctx, const char* path) {
     }
     std::fclose(f);
     if (bad) throw Error(XNYZ_EINVAL, std::string(path) + ":" + std::to_string(lineno) +
-                                          ": expected 'class chunk_log slice_log share blocks copy_first'");
+                                          ": expected 'class chunk_log slice_log share blocks copy_first radix_bits "
+                                          "radix_tile'");
     for (const auto& r : rows) check_tuning(r.second);  // all or nothing
     for (const auto& r : rows) ctx->tuning[r.first] = r.second;
     return XNYZ_OK;
@@ -2017,6 +2030,8 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
       };
       sweep([](xnyz_tuning& t, double v) { t.persist_blocks = (u32)v; },
             {512, 1024, 2048, 4096, 8192});
+      sweep([](xnyz_tuning& t, double v) { t.radix_bits = (u32)v; }, {6, 7, 8});
+      sweep([](xnyz_tuning& t, double v) { t.radix_tile = (u32)v; }, {2048, 4096});
       if (opts->frame_mode == XNYZ_FRAME_ZSTD3_MT) {
         sweep([](xnyz_tuning& t, double v) { t.zstd_chunk_log = (u32)v; }, {18, 19, 20, 21, 22});
         sweep([](xnyz_tuning& t, double v) { t.gpu_frame_share = (float)v; },

@@ -26,6 +26,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "xnyz.h"
 
 #pragma clang fp contract(off)
 
@@ -743,75 +744,105 @@ void scan_excl(const T* in, T* out, u64 n, T* total, Arena& ar, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Segmented LSD radix sort, 8-bit digits.  The digit-count matrix is laid out
-// per segment ([segment][digit][tile in segment]), so one exclusive scan over
-// it yields, for every tile and digit, the output position inside the
-// tile's own segment range.  The scatter is stable: each wavefront ranks its
-// 64-key slots in order with ballot-based digit matching and running
-// per-wave counters in LDS, so equal digits keep their input order.
+// Segmented LSD radix sort, kBits-bit digits (6, 7 or 8), each tile of the
+// tile table cut into kSub radix tiles of kTile / kSub keys.  The digit-count
+// matrix is laid out per segment ([segment][digit][radix tile in segment]),
+// so one exclusive scan over it yields, for every radix tile and digit, the
+// output position inside the tile's own segment range.  The scatter is
+// stable: each wavefront ranks its 64-key slots in order with ballot-based
+// digit matching and running per-wave counters in LDS, so equal digits keep
+// their input order.  Digit width and radix tile size are run-time choices
+// (xnyz_tuning); every choice gives the same sorted order.
 
-constexpr u32 kWaveSub = kWave * kItems;  // keys per wavefront in a tile
+constexpr u32 kWaveSub = kWave * kItems;  // elements per wavefront in a tile
 
-template <typename K>
+template <u32 kBits, u32 kSub>
+struct RadixShapeT {
+  static constexpr u32 D = 1u << kBits;           // digits
+  static constexpr u32 kRT = kTile / kSub;        // keys per radix tile
+  static constexpr u32 kIt = kItems / kSub;       // keys per thread
+  static constexpr u32 kWSub = kWave * kIt;       // keys per wavefront
+  static_assert(D <= kBlock, "one digit per thread in the digit scans");
+};
+
+// Index of radix tile (t, sub) of a segment's digit d in the count matrix.
+__device__ __forceinline__ u64 count_index(const SegParam& sp, u32 local, u32 sub, u32 d, u32 D,
+                                           u32 nsub) {
+  return (u64)D * nsub * sp.tile0 + (u64)d * sp.ntiles * nsub + (u64)local * nsub + sub;
+}
+
+template <typename K, u32 kBits, u32 kSub>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(Batch bt, const K* __restrict__ keys,
                                                        u32 shift, u32* __restrict__ counts) {
-  __shared__ u32 h[kWavesPerBlock][256];
-  for (u32 i = threadIdx.x; i < kWavesPerBlock * 256; i += kBlock) (&h[0][0])[i] = 0;
+  using RS = RadixShapeT<kBits, kSub>;
+  __shared__ u32 h[kWavesPerBlock][RS::D];
+  for (u32 i = threadIdx.x; i < kWavesPerBlock * RS::D; i += kBlock) (&h[0][0])[i] = 0;
   __syncthreads();
-  const Tile t = bt.tiles[blockIdx.x];
+  const u32 tix = blockIdx.x / kSub, sub = blockIdx.x % kSub;
+  const Tile t = bt.tiles[tix];
   const SegParam& sp = bt.sp[t.seg];
   const Range r = tile_range(sp.p0, sp.n, t.local);
+  const u32 base = sub * RS::kRT;
   const u32 w = threadIdx.x / kWave;
 #pragma unroll 4
-  for (u32 it = 0; it < kItems; ++it) {
-    const u32 o = it * kBlock + threadIdx.x;
-    if (o < r.count) atomicAdd(&h[w][(u32)(keys[r.first + o] >> shift) & 255u], 1u);
+  for (u32 it = 0; it < RS::kIt; ++it) {
+    const u32 o = base + it * kBlock + threadIdx.x;
+    if (o < r.count) atomicAdd(&h[w][(u32)(keys[r.first + o] >> shift) & (RS::D - 1)], 1u);
   }
   __syncthreads();
   const u32 d = threadIdx.x;
-  u32 c = 0;
+  if (d < RS::D) {
+    u32 c = 0;
 #pragma unroll
-  for (int k = 0; k < kWavesPerBlock; ++k) c += h[k][d];
-  counts[(u64)256 * sp.tile0 + (u64)d * sp.ntiles + t.local] = c;
+    for (int k = 0; k < kWavesPerBlock; ++k) c += h[k][d];
+    counts[count_index(sp, t.local, sub, d, RS::D, kSub)] = c;
+  }
 }
 
-template <typename K, typename V, bool kHasV>
+template <typename K, typename V, bool kHasV, u32 kBits, u32 kSub>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __restrict__ kin,
                                                           K* __restrict__ kout,
                                                           const V* __restrict__ vin,
                                                           V* __restrict__ vout, u32 shift,
                                                           const u32* __restrict__ offs) {
-  __shared__ u32 wc[kWavesPerBlock][256];
-  __shared__ u32 wb[kWavesPerBlock][256];
-  __shared__ u32 ds[256], gb[256], sc4[kWavesPerBlock];
-  __shared__ u64 stage[kTile];  // the tile's keys, then its values, in output order
+  using RS = RadixShapeT<kBits, kSub>;
+  constexpr u32 D = RS::D, kIt = RS::kIt;
+  __shared__ u32 wc[kWavesPerBlock][D];
+  __shared__ u32 wb[kWavesPerBlock][D];
+  __shared__ u32 ds[D], gb[D], sc4[kWavesPerBlock];
+  __shared__ u64 stage[RS::kRT];  // the radix tile's keys, then its values, in output order
   static_assert(sizeof(K) <= 8 && sizeof(V) <= 8, "staging holds 64-bit items");
   K* stage_k = reinterpret_cast<K*>(stage);
-  for (u32 i = threadIdx.x; i < kWavesPerBlock * 256; i += kBlock) (&wc[0][0])[i] = 0;
+  for (u32 i = threadIdx.x; i < kWavesPerBlock * D; i += kBlock) (&wc[0][0])[i] = 0;
   __syncthreads();
-  const Tile t = bt.tiles[blockIdx.x];
+  const u32 tix = blockIdx.x / kSub, sub = blockIdx.x % kSub;
+  const Tile t = bt.tiles[tix];
   const SegParam& sp = bt.sp[t.seg];
-  const Range r = tile_range(sp.p0, sp.n, t.local);
+  const Range r0 = tile_range(sp.p0, sp.n, t.local);
+  const u32 base = sub * RS::kRT;
+  // this radix tile: keys [first, first + count) of the sorted range
+  const u32 count = r0.count > base ? (r0.count - base < RS::kRT ? r0.count - base : RS::kRT) : 0u;
+  const u32 first = r0.first + base;
   const u32 w = threadIdx.x / kWave, l = lane_id();
   const u64 lt = (1ull << l) - 1;
-  const u32 wstart = w * kWaveSub;
-  // The tile's global digit offsets are strided in memory; start that load
-  // now so its latency overlaps the key loads and the ranking.
-  const u32 my_off = offs[(u64)256 * sp.tile0 + (u64)threadIdx.x * sp.ntiles + t.local];
-  K k[kItems];
-  V v[kItems];
-  u32 rk[kItems];
-  u32 dg[kItems];
+  const u32 wstart = w * RS::kWSub;
+  // The radix tile's global digit offsets are strided in memory; start that
+  // load now so its latency overlaps the key loads and the ranking.
+  const u32 my_off = threadIdx.x < D ? offs[count_index(sp, t.local, sub, threadIdx.x, D, kSub)] : 0u;
+  K k[kIt];
+  V v[kIt];
+  u32 rk[kIt];
+  u32 dg[kIt];
 #pragma unroll
-  for (u32 s = 0; s < kItems; ++s) {
+  for (u32 s = 0; s < kIt; ++s) {
     const u32 o = wstart + s * kWave + l;
-    const bool valid = o < r.count;
-    k[s] = valid ? kin[r.first + o] : (K)0;
-    if (kHasV) v[s] = valid ? vin[r.first + o] : (V)0;
-    const u32 d = (u32)(k[s] >> shift) & 255u;
+    const bool valid = o < count;
+    k[s] = valid ? kin[first + o] : (K)0;
+    if (kHasV) v[s] = valid ? vin[first + o] : (V)0;
+    const u32 d = (u32)(k[s] >> shift) & (D - 1);
     u64 peers = __ballot(valid);
 #pragma unroll
-    for (u32 b = 0; b < 8; ++b) {
+    for (u32 b = 0; b < kBits; ++b) {
       const bool bit = (d >> b) & 1u;
       const u64 bb = __ballot(bit);
       peers &= bit ? bb : ~bb;
@@ -830,35 +861,39 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
   {
     const u32 d = threadIdx.x;
     u32 tot = 0;
+    if (d < D) {
 #pragma unroll
-    for (int ww = 0; ww < kWavesPerBlock; ++ww) tot += wc[ww][d];
+      for (int ww = 0; ww < kWavesPerBlock; ++ww) tot += wc[ww][d];
+    }
     u32 all;
     u32 acc = block_excl_scan(tot, sc4, all);
-    ds[d] = acc;
-    gb[d] = my_off;
+    if (d < D) {
+      ds[d] = acc;
+      gb[d] = my_off;
 #pragma unroll
-    for (int ww = 0; ww < kWavesPerBlock; ++ww) {
-      wb[ww][d] = acc;
-      acc += wc[ww][d];
+      for (int ww = 0; ww < kWavesPerBlock; ++ww) {
+        wb[ww][d] = acc;
+        acc += wc[ww][d];
+      }
     }
   }
   __syncthreads();
-  u32 lp[kItems];
+  u32 lp[kIt];
 #pragma unroll
-  for (u32 s = 0; s < kItems; ++s) {
+  for (u32 s = 0; s < kIt; ++s) {
     lp[s] = wb[w][dg[s]] + rk[s];
-    if (wstart + s * kWave + l < r.count) stage_k[lp[s]] = k[s];
+    if (wstart + s * kWave + l < count) stage_k[lp[s]] = k[s];
   }
   __syncthreads();
   // Slot i of the staged tile goes to gb[d] + (i - ds[d]): runs of equal
   // digits are written by consecutive lanes.
-  u32 dst[kItems];
+  u32 dst[kIt];
 #pragma unroll
-  for (u32 s = 0; s < kItems; ++s) {
+  for (u32 s = 0; s < kIt; ++s) {
     const u32 i = s * kBlock + threadIdx.x;
-    if (i < r.count) {
+    if (i < count) {
       const K key = stage_k[i];
-      const u32 d = (u32)(key >> shift) & 255u;
+      const u32 d = (u32)(key >> shift) & (D - 1);
       dst[s] = gb[d] + (i - ds[d]);
       kout[dst[s]] = key;
     }
@@ -867,29 +902,49 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
     __syncthreads();
     V* stage_v = reinterpret_cast<V*>(stage_k);
 #pragma unroll
-    for (u32 s = 0; s < kItems; ++s)
-      if (wstart + s * kWave + l < r.count) stage_v[lp[s]] = v[s];
+    for (u32 s = 0; s < kIt; ++s)
+      if (wstart + s * kWave + l < count) stage_v[lp[s]] = v[s];
     __syncthreads();
 #pragma unroll
-    for (u32 s = 0; s < kItems; ++s) {
+    for (u32 s = 0; s < kIt; ++s) {
       const u32 i = s * kBlock + threadIdx.x;
-      if (i < r.count) vout[dst[s]] = stage_v[i];
+      if (i < count) vout[dst[s]] = stage_v[i];
     }
   }
 }
 
-template <typename K, typename V, bool kHasV>
-void radix_pass(const Batch& bt, const K* kin, K* kout, const V* vin, V* vout, u32 shift,
-                Arena& ar, hipStream_t s) {
+template <typename K, typename V, bool kHasV, u32 kBits, u32 kSub>
+void radix_pass_t(const Batch& bt, const K* kin, K* kout, const V* vin, V* vout, u32 shift,
+                  Arena& ar, hipStream_t s) {
   const size_t mark = ar.off;
-  u32* counts = ar.take<u32>((u64)256 * bt.ntiles);
-  k_radix_hist<K><<<bt.ntiles, kBlock, 0, s>>>(bt, kin, shift, counts);
+  const u64 ncounts = (u64)(1u << kBits) * bt.ntiles * kSub;
+  u32* counts = ar.take<u32>(ncounts);
+  k_radix_hist<K, kBits, kSub><<<bt.ntiles * kSub, kBlock, 0, s>>>(bt, kin, shift, counts);
   LAUNCH_CHECK("k_radix_hist");
-  scan_excl<u32>(counts, counts, (u64)256 * bt.ntiles, nullptr, ar, s);
-  k_radix_scatter<K, V, kHasV><<<bt.ntiles, kBlock, 0, s>>>(bt, kin, kout, vin, vout, shift,
-                                                             counts);
+  scan_excl<u32>(counts, counts, ncounts, nullptr, ar, s);
+  k_radix_scatter<K, V, kHasV, kBits, kSub>
+      <<<bt.ntiles * kSub, kBlock, 0, s>>>(bt, kin, kout, vin, vout, shift, counts);
   LAUNCH_CHECK("k_radix_scatter");
   ar.off = mark;
+}
+
+// One pass of `shape.bits`-bit digits at `shift` (digits past the key's top
+// bit are zero).
+template <typename K, typename V, bool kHasV>
+void radix_pass(const Batch& bt, const K* kin, K* kout, const V* vin, V* vout, u32 shift,
+                RadixShape shape, Arena& ar, hipStream_t s) {
+  const bool half = shape.sub == 2;
+#define XNYZ_RADIX_CASE(B)                                                                 \
+  if (shape.bits == B) {                                                                   \
+    if (half) radix_pass_t<K, V, kHasV, B, 2>(bt, kin, kout, vin, vout, shift, ar, s);     \
+    else radix_pass_t<K, V, kHasV, B, 1>(bt, kin, kout, vin, vout, shift, ar, s);          \
+    return;                                                                                \
+  }
+  XNYZ_RADIX_CASE(6)
+  XNYZ_RADIX_CASE(7)
+  XNYZ_RADIX_CASE(8)
+#undef XNYZ_RADIX_CASE
+  throw Error(XNYZ_EINVAL, "radix digit width must be 6, 7 or 8 bits");
 }
 
 // ---------------------------------------------------------------------------
@@ -1571,48 +1626,46 @@ void scan_excl_u64(const u64* in, u64* out, u64 n, u64* total, Arena& ar, hipStr
   ar.off = mark;
 }
 
-int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, Arena& ar,
-               hipStream_t s) {
+int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, RadixShape shape,
+               Arena& ar, hipStream_t s) {
   int cur = 0;
-  for (u32 shift = 0; shift < bits; shift += 8) {
-    if (v) radix_pass<u64, u32, true>(bt, a, a2, v, v2, shift, ar, s);
-    else radix_pass<u64, u32, false>(bt, a, a2, nullptr, nullptr, shift, ar, s);
+  for (u32 shift = 0; shift < bits; shift += shape.bits) {
+    if (v) radix_pass<u64, u32, true>(bt, a, a2, v, v2, shift, shape, ar, s);
+    else radix_pass<u64, u32, false>(bt, a, a2, nullptr, nullptr, shift, shape, ar, s);
     std::swap(a, a2);
     std::swap(v, v2);
     cur ^= 1;
   }
   return cur;
 }
-
 int radix_sort_wide(const Batch& bt, u64* hi, u64* hi2, u32* lo, u32* lo2, u32 hi_bits,
-                    Arena& ar, hipStream_t s) {
+                    RadixShape shape, Arena& ar, hipStream_t s) {
   int cur = 0;
-  for (u32 shift = 0; shift < 21; shift += 8) {
-    radix_pass<u32, u64, true>(bt, lo, lo2, hi, hi2, shift, ar, s);
+  for (u32 shift = 0; shift < 21; shift += shape.bits) {
+    radix_pass<u32, u64, true>(bt, lo, lo2, hi, hi2, shift, shape, ar, s);
     std::swap(hi, hi2);
     std::swap(lo, lo2);
     cur ^= 1;
   }
-  for (u32 shift = 0; shift < hi_bits; shift += 8) {
-    radix_pass<u64, u32, true>(bt, hi, hi2, lo, lo2, shift, ar, s);
+  for (u32 shift = 0; shift < hi_bits; shift += shape.bits) {
+    radix_pass<u64, u32, true>(bt, hi, hi2, lo, lo2, shift, shape, ar, s);
     std::swap(hi, hi2);
     std::swap(lo, lo2);
     cur ^= 1;
   }
   return cur;
 }
-
 int radix_sort_wide_v(const Batch& bt, u64* hi, u64* hi2, u64* lo64, u64* lo64b, u32 hi_bits,
-                      Arena& ar, hipStream_t s) {
+                      RadixShape shape, Arena& ar, hipStream_t s) {
   int cur = 0;
-  for (u32 shift = 0; shift < 21; shift += 8) {
-    radix_pass<u64, u64, true>(bt, lo64, lo64b, hi, hi2, shift, ar, s);
+  for (u32 shift = 0; shift < 21; shift += shape.bits) {
+    radix_pass<u64, u64, true>(bt, lo64, lo64b, hi, hi2, shift, shape, ar, s);
     std::swap(hi, hi2);
     std::swap(lo64, lo64b);
     cur ^= 1;
   }
-  for (u32 shift = 0; shift < hi_bits; shift += 8) {
-    radix_pass<u64, u64, true>(bt, hi, hi2, lo64, lo64b, shift, ar, s);
+  for (u32 shift = 0; shift < hi_bits; shift += shape.bits) {
+    radix_pass<u64, u64, true>(bt, hi, hi2, lo64, lo64b, shift, shape, ar, s);
     std::swap(hi, hi2);
     std::swap(lo64, lo64b);
     cur ^= 1;

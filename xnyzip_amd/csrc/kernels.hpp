@@ -214,12 +214,19 @@ void scan_excl_u64(const u64* in, u64* out, u64 n, u64* total, Arena& ar, hipStr
 // narrow: u64 keys (+ optional u32 payload); wide: (hi u64, lo u32) by lo then
 // hi (+ optional index payload folded into lo64).  Returns the buffer index
 // holding the result (0: first arrays, 1: second).
-int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, Arena& ar,
-               hipStream_t s);
+// Radix sort shape: digit width (6, 7 or 8 bits) and radix tiles per tile of
+// the tile table (1 or 2).
+struct RadixShape {
+  u32 bits = 8;
+  u32 sub = 1;
+};
+
+int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, RadixShape shape,
+               Arena& ar, hipStream_t s);
 int radix_sort_wide(const Batch& bt, u64* hi, u64* hi2, u32* lo, u32* lo2, u32 hi_bits,
-                    Arena& ar, hipStream_t s);
+                    RadixShape shape, Arena& ar, hipStream_t s);
 int radix_sort_wide_v(const Batch& bt, u64* hi, u64* hi2, u64* lo64, u64* lo64b, u32 hi_bits,
-                      Arena& ar, hipStream_t s);
+                      RadixShape shape, Arena& ar, hipStream_t s);
 
 // RLE: first position of every run of equal keys per segment, then the cells
 // (keys) and their counts, segment by segment, with count histograms.
