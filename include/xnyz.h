@@ -327,11 +327,12 @@ int xnyz_tuning_load(xnyz_ctx* ctx, const char* path);
 
 /* Autotuner: times compressions of the device-resident cloud (n points,
  * options `opts`, frames into scratch owned by the context) over candidate
- * settings, one parameter at a time with the others held at their best so
- * far, `reps` timed calls per candidate after one untimed call (a candidate
- * scores its fastest call), and stores the fastest setting as the tuning of
- * n's size class.  *best gets it, *best_ms its score.  No reference
- * counterpart. */
+ * settings -- one parameter at a time with the others held at their best so
+ * far, the device share and copy order of a split frame as a grid, the
+ * kernel shapes once the share is set -- `reps` timed calls per candidate
+ * after one untimed call (a candidate scores its fastest call), and stores
+ * the fastest setting as the tuning of n's size class.  *best gets it,
+ * *best_ms its score.  No reference counterpart. */
 int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
                   uint32_t reps, xnyz_tuning* best, double* best_ms);
 

@@ -2028,15 +2028,24 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
           }
         }
       };
-      sweep([](xnyz_tuning& t, double v) { t.persist_blocks = (u32)v; },
-            {512, 1024, 2048, 4096, 8192});
-      sweep([](xnyz_tuning& t, double v) { t.radix_bits = (u32)v; }, {6, 7, 8});
-      sweep([](xnyz_tuning& t, double v) { t.radix_tile = (u32)v; }, {2048, 4096});
+      // Kernel shapes matter once the device is the slower side, so a split
+      // frame's share is found first, then the shapes, then the share again.
+      auto shapes = [&] {
+        sweep([](xnyz_tuning& t, double v) { t.persist_blocks = (u32)v; },
+              {512, 1024, 2048, 4096, 8192});
+        sweep([](xnyz_tuning& t, double v) { t.radix_bits = (u32)v; }, {6, 7, 8});
+        sweep([](xnyz_tuning& t, double v) { t.radix_tile = (u32)v; }, {2048, 4096});
+      };
       if (opts->frame_mode == XNYZ_FRAME_ZSTD3_MT) {
         sweep([](xnyz_tuning& t, double v) { t.zstd_chunk_log = (u32)v; }, {18, 19, 20, 21, 22});
-        sweep([](xnyz_tuning& t, double v) { t.gpu_frame_share = (float)v; },
-              {0.0, 0.1, 0.2, 0.3, 0.4, 0.5});
-        sweep([](xnyz_tuning& t, double v) { t.copy_first = (u32)v; }, {0, 1});
+        // device share and copy order together (they interact)
+        for (u32 first = 0; first < 2; ++first)
+          sweep([first](xnyz_tuning& t, double v) {
+                  t.copy_first = first;
+                  t.gpu_frame_share = (float)v;
+                },
+                {0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6});
+        shapes();
         const double s0 = cur.gpu_frame_share;
         if (s0 > 0)
           sweep([](xnyz_tuning& t, double v) { t.gpu_frame_share = (float)v; },
@@ -2044,6 +2053,8 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
                  std::min(0.9, s0 + 0.05)});
         sweep([](xnyz_tuning& t, double v) { t.copy_slice_log = (u32)v; }, {22, 24, 26});
         sweep([](xnyz_tuning& t, double v) { t.zstd_chunk_log = (u32)v; }, {18, 19, 20, 21});
+      } else {
+        shapes();
       }
       ctx->tuning[c] = cur;
       if (best) *best = cur;
