@@ -758,11 +758,19 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   load_block(data, pre + bk.src, n, tid, 256);
   for (u32 i = tid; i < 2 * kH; i += 256) tab[i] = 0xFFFFFFFFu;
   __syncthreads();
-  for (u32 p = tid; p + 4 <= n; p += 256) {
-    const u32 v = LdsReader::r32(data, p);
-    atomicMin(&tab[xz::hash4(v)], p);
-    if (p + 8 <= n)
-      atomicMin(&tab[kH + xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))], p);
+  // Runs of equal hashes on consecutive lanes (runs of equal bytes) leave
+  // the update to the run's first lane, whose position is the smallest:
+  // atomics on one address from a whole wavefront would serialise.
+  for (u32 p0 = 0; p0 < n; p0 += 256) {
+    const u32 p = p0 + tid;
+    const bool ok4 = p + 4 <= n, ok8 = p + 8 <= n;
+    const u32 v = ok4 ? LdsReader::r32(data, p) : 0u;
+    const u32 h4 = ok4 ? xz::hash4(v) : 0xFFFFFFFFu;
+    const u32 h8 = ok8 ? xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32)) : 0xFFFFFFFFu;
+    const u32 lane = tid & (kWave - 1);
+    const u32 up4 = (u32)__shfl_up((int)h4, 1, kWave), up8 = (u32)__shfl_up((int)h8, 1, kWave);
+    if (ok4 && (lane == 0 || up4 != h4)) atomicMin(&tab[h4], p);
+    if (ok8 && (lane == 0 || up8 != h8)) atomicMin(&tab[kH + h8], p);
   }
   __syncthreads();
   {
@@ -821,13 +829,21 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
 #pragma unroll
       for (u32 k = 0; k < 3; ++k)
         if (kFixedOff[k] <= p) consider(kFixedOff[k], eq_run(eq + k * kEqWords, p, cap));
+      // A hash hit is measured only if even a full-length match at its
+      // offset could beat the best so far (ties keep the earlier candidate,
+      // so skipping it cannot change the choice).
+      auto can_win = [&](u32 off) {
+        return (i32)cap * xz::kLenWeight - ((i32)xz::highbit(off + 3) + xz::kOffCost) > bs;
+      };
       if (p + 4 <= n) {
         const u32 v = LdsReader::r32(data, p);
         const u32 q4 = h4[xz::hash4(v)];
-        if (q4 < p) consider(p - q4, LdsReader::len(data, p, q4, p + cap));
+        if (q4 < p && can_win(p - q4)) consider(p - q4, LdsReader::len(data, p, q4, p + cap));
         if (p + 8 <= n) {
           const u32 q8 = h8[xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))];
-          if (q8 < p) consider(p - q8, LdsReader::len(data, p, q8, p + cap));
+          // the same earliest position as the 4-byte hit measures the same
+          if (q8 < p && q8 != q4 && can_win(p - q8))
+            consider(p - q8, LdsReader::len(data, p, q8, p + cap));
         }
       }
       if (best) c = (best << 16) | boff;
