@@ -281,6 +281,7 @@ struct xnyz_ctx {
   std::vector<hipEvent_t> slice_ev;  // copy-back slices (frame overlap)
   hipStream_t copy_stream = nullptr;  // copy-back beside device-built frame blocks
   hipEvent_t ev_pipe = nullptr;       // end of the stream-building work
+  hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
 
@@ -1239,13 +1240,16 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     ctx->zscr.ensure(zframe_scratch_bytes(znblk));
     ctx->zout.ensure(bound + 64);
     ctx->zpos.ensure((znblk + 1) * sizeof(u64) + 64);
-    zframe_encode(reinterpret_cast<const uint8_t*>(ow), ctx->ztab.as<ZBlk>(), znblk,
-                  reinterpret_cast<const ZFrame*>(ctx->ztab.as<uint8_t>() + tb), ctx->zscr.p,
-                  ctx->zout.as<uint8_t>(), ctx->zpos.as<u64>(), ar, st);
     ctx->h_zpos.ensure((znblk + 1) * sizeof(u64));
     zpos = ctx->h_zpos.as<u64>();
-    check_hip(hipMemcpyAsync(ctx->h_zpos.p, ctx->zpos.p, (znblk + 1) * sizeof(u64),
-                             hipMemcpyDeviceToHost, st), "D2H");
+    if (!ctx->ev_sizes)
+      check_hip(hipEventCreateWithFlags(&ctx->ev_sizes, hipEventDisableTiming), "hipEventCreate");
+    // block positions come back (and ev_sizes fires) before the blocks are
+    // gathered, so the host can place its own bytes meanwhile
+    zframe_encode(reinterpret_cast<const uint8_t*>(ow), ctx->ztab.as<ZBlk>(), znblk,
+                  reinterpret_cast<const ZFrame*>(ctx->ztab.as<uint8_t>() + tb), ctx->zscr.p,
+                  ctx->zout.as<uint8_t>(), ctx->zpos.as<u64>(), ar, st, ctx->h_zpos.as<u64>(),
+                  ctx->ev_sizes);
   }
   // Brings the device-built frames to host memory (after the host frames
   // when the frame is split, so the two run side by side).
@@ -1309,14 +1313,32 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
         default: outs[s] = zstd_single(fs.cctx[w], b, len, z, zoff[s + 1] - zoff[s]);
       }
     });
-    if (split) {  // the device-built blocks open the frame
+    if (split) {
+      // The device-built blocks open the frame.  Their total size is known
+      // before the blocks themselves arrive, so the host's blocks are put in
+      // place while the device gathers and copies its own.
+      check_hip(hipEventSynchronize(ctx->ev_sizes), "hipEventSynchronize");
+      const u64 head_bytes = 13 + zpos[znblk], n = head_bytes + pieces_size(outs[0]);
+      uint8_t* dst;
+      if (co.frames) {
+        dst = xmalloc(n);
+        co.frames[0].data = dst;
+        co.frames[0].len = n;
+      } else {
+        co.dst_off[0] = 0;
+        co.dst_off[1] = n;
+        if (n > co.cap)
+          throw Error(XNYZ_ENOSPC, "output buffer too small: " + std::to_string(n) +
+                                       " bytes needed");
+        dst = co.dst;
+      }
+      copy_pieces(outs[0], dst + head_bytes, threads);
       fetch_dev_frames();
       const uint64_t fcs = g[0].total;  // the header covers the whole stream
       std::memcpy(zb + 5, &fcs, 8);
       zb[zpos[znblk - 1] + 13] &= 0xFE;  // the host's blocks follow the last one
-      outs[0].insert(outs[0].begin(), Piece{zb, 13 + zpos[znblk]});
-    }
-    if (co.frames) {
+      copy_pieces({Piece{zb, head_bytes}}, dst, threads);
+    } else if (co.frames) {
       parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
         const u64 n = pieces_size(outs[s]);
         co.frames[s].data = xmalloc(n);
@@ -1498,6 +1520,7 @@ int xnyz_destroy(xnyz_ctx* ctx) {
     ctx->stage2.clear();
     for (hipEvent_t e : ctx->slice_ev) (void)hipEventDestroy(e);
     if (ctx->ev_pipe) (void)hipEventDestroy(ctx->ev_pipe);
+    if (ctx->ev_sizes) (void)hipEventDestroy(ctx->ev_sizes);
     if (ctx->copy_stream) {
       (void)hipStreamSynchronize(ctx->copy_stream);
       (void)hipStreamDestroy(ctx->copy_stream);

@@ -1106,7 +1106,8 @@ size_t zframe_scratch_bytes(u32 nblk) {
 }
 
 void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* frames,
-                   void* scratch, u8* out, u64* pos, Arena& ar, hipStream_t s) {
+                   void* scratch, u8* out, u64* pos, Arena& ar, hipStream_t s, u64* h_pos,
+                   hipEvent_t sizes_ready) {
   auto* scr = reinterpret_cast<BlkScratch*>(scratch);
   u8* slots = reinterpret_cast<u8*>(scr + nblk);
   u64* slot_len = reinterpret_cast<u64*>(slots + (size_t)nblk * kSlot);
@@ -1123,6 +1124,12 @@ void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* fram
   check_hip(hipGetLastError(), "k_zseqs");
   if (g_profiler) g_profiler->mark("k_zseqs", s);
   scan_excl_u64(slot_len, pos, nblk, pos + nblk, ar, s);
+  if (h_pos) {
+    check_hip(hipMemcpyAsync(h_pos, pos, (nblk + 1) * sizeof(u64), hipMemcpyDeviceToHost, s),
+              "block positions D2H");
+    if (sizes_ready) check_hip(hipEventRecord(sizes_ready, s), "hipEventRecord");
+    if (g_profiler) g_profiler->mark(nullptr, s);  // the copy is not k_zgather's time
+  }
   k_zgather<<<nblk, 256, 0, s>>>(pre, blks, frames, scr, slots, pos, slot_len, out);
   check_hip(hipGetLastError(), "k_zgather");
   if (g_profiler) g_profiler->mark("k_zgather", s);
