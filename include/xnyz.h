@@ -330,9 +330,10 @@ int xnyz_tuning_load(xnyz_ctx* ctx, const char* path);
  * settings -- one parameter at a time with the others held at their best so
  * far, the device share and copy order of a split frame as a grid, the
  * kernel shapes once the share is set -- `reps` timed calls per candidate
- * after one untimed call (a candidate scores its fastest call), and stores
- * the fastest setting as the tuning of n's size class.  *best gets it,
- * *best_ms its score.  No reference counterpart. */
+ * after one untimed call (a candidate scores its fastest call: wall time,
+ * or for the kernel shapes the device time of the stream-building kernels),
+ * and stores the best setting as the tuning of n's size class.  *best gets
+ * it, *best_ms its wall time per call.  No reference counterpart. */
 int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
                   uint32_t reps, xnyz_tuning* best, double* best_ms);
 

@@ -2051,13 +2051,51 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
           }
         }
       };
-      // Kernel shapes matter once the device is the slower side, so a split
-      // frame's share is found first, then the shapes, then the share again.
+      // Kernel shapes are scored by the device time of the stream-building
+      // kernels (HIP events around each launch): their differences are a
+      // fraction of a millisecond, below the noise of wall-clock calls.
+      auto dev_ms = [&](const xnyz_tuning& t) {
+        ctx->tuning[c] = t;
+        const bool was_on = ctx->prof_on;
+        ctx->prof_on = true;
+        double best = 1e30;
+        try {
+          call();
+          for (u32 r = 0; r < reps; ++r) {
+            ctx->prof.acc.clear();
+            call();
+            double ms = 0;
+            for (const auto& a : ctx->prof.acc)
+              if (a.first.rfind("k_z", 0) != 0) ms += a.second.second;
+            best = std::min(best, ms);
+          }
+        } catch (...) {
+          ctx->prof_on = was_on;
+          throw;
+        }
+        ctx->prof_on = was_on;
+        ctx->prof.acc.clear();
+        return best;
+      };
       auto shapes = [&] {
-        sweep([](xnyz_tuning& t, double v) { t.persist_blocks = (u32)v; },
-              {512, 1024, 2048, 4096, 8192});
-        sweep([](xnyz_tuning& t, double v) { t.radix_bits = (u32)v; }, {6, 7, 8});
-        sweep([](xnyz_tuning& t, double v) { t.radix_tile = (u32)v; }, {2048, 4096});
+        double base = dev_ms(cur);
+        auto sweep_dev = [&](auto field, std::initializer_list<double> values) {
+          for (double v : values) {
+            xnyz_tuning t = cur;
+            field(t, v);
+            if (std::memcmp(&t, &cur, sizeof(t)) == 0) continue;
+            const double ms = dev_ms(t);
+            if (ms < base) {
+              base = ms;
+              cur = t;
+            }
+          }
+        };
+        sweep_dev([](xnyz_tuning& t, double v) { t.persist_blocks = (u32)v; },
+                  {512, 1024, 2048, 4096, 8192});
+        sweep_dev([](xnyz_tuning& t, double v) { t.radix_bits = (u32)v; }, {6, 7, 8});
+        sweep_dev([](xnyz_tuning& t, double v) { t.radix_tile = (u32)v; }, {2048, 4096});
+        cur_ms = timed(cur);
       };
       if (opts->frame_mode == XNYZ_FRAME_ZSTD3_MT) {
         sweep([](xnyz_tuning& t, double v) { t.zstd_chunk_log = (u32)v; }, {18, 19, 20, 21, 22});

@@ -40,7 +40,7 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--sizes", default="20,22,24,25,26",
                     help="size classes (log2 of the point count) to tune")
-    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--reps", type=int, default=4)
     args = ap.parse_args(argv)
 
     import torch
