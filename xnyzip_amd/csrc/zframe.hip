@@ -706,6 +706,61 @@ struct LdsReader {
     }
     return l;
   }
+  // out[c] = len(d, p, p - off[c], end) for every c with bit c of `want`,
+  // in shared rounds: one load of p's side and one LDS round trip for all.
+  __device__ static void len3(const u8* d, u32 p, const u32* off, u32 want, u32 end, u32* out) {
+    const u32* w = reinterpret_cast<const u32*>(d);
+    const u32 q[3] = {p - off[0], p - off[1], p - off[2]};
+    for (u32 l = 0; want; l += 16) {
+      const u32 r = end - (p + l);  // > 0: a measurement ends once l reaches end - p
+      const u32 pa = (p + l) >> 2, ps = (p + l) & 3u;
+      u32 a[5];
+#pragma unroll
+      for (u32 k = 0; k < 5; ++k) a[k] = w[pa + k];
+#pragma unroll
+      for (u32 c = 0; c < 3; ++c) {
+        if (!(want >> c & 1u)) continue;
+        const u32 qa = (q[c] + l) >> 2, qs = (q[c] + l) & 3u;
+        u32 b[5];
+#pragma unroll
+        for (u32 k = 0; k < 5; ++k) b[k] = w[qa + k];
+        u32 e = 16;
+#pragma unroll
+        for (u32 k = 0; k < 4; ++k) {
+          const u32 x = __builtin_amdgcn_alignbyte(a[k + 1], a[k], ps) ^
+                        __builtin_amdgcn_alignbyte(b[k + 1], b[k], qs);
+          if (x && e == 16) e = 4 * k + ((u32)__builtin_ctz(x) >> 3);
+        }
+        if (e < 16 || r <= 16) {
+          out[c] = l + (e < r ? e : r);
+          want &= ~(1u << c);
+        }
+      }
+    }
+  }
+  // len(d, p, q, end) given v0, v1 = the words at p and p + 4: the first
+  // eight bytes cost three loads of q's side, and most hash hits end there
+  // (a 4- or 8-byte hash collision usually differs in the first byte).
+  __device__ static u32 len8(const u8* d, u32 p, u32 q, u32 end, u32 v0, u32 v1) {
+    const u32* w = reinterpret_cast<const u32*>(d);
+    const u32 qa = q >> 2, qs = q & 3u;
+    const u32 a0 = w[qa], a1 = w[qa + 1], a2 = w[qa + 2];
+    const u32 r = end - p;
+    u32 e;
+    u32 x = __builtin_amdgcn_alignbyte(a1, a0, qs) ^ v0;
+    if (x) {
+      e = (u32)__builtin_ctz(x) >> 3;
+    } else {
+      x = __builtin_amdgcn_alignbyte(a2, a1, qs) ^ v1;
+      if (x) {
+        e = 4 + ((u32)__builtin_ctz(x) >> 3);
+      } else {
+        if (r <= 8) return r;
+        return 8 + len(d, p + 8, q + 8, end);
+      }
+    }
+    return e < r ? e : r;
+  }
 };
 
 constexpr u32 kPad = 32;  // LDS block copies: readable bytes past the end (match loads read 20)
@@ -836,14 +891,16 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
         return (i32)cap * xz::kLenWeight - ((i32)xz::highbit(off + 3) + xz::kOffCost) > bs;
       };
       if (p + 4 <= n) {
-        const u32 v = LdsReader::r32(data, p);
+        // p + 8 may pass n: v1 then holds zero padding, never compared
+        // past the block end (len8 stops at p + cap <= n)
+        const u32 v = LdsReader::r32(data, p), v1 = LdsReader::r32(data, p + 4);
         const u32 q4 = h4[xz::hash4(v)];
-        if (q4 < p && can_win(p - q4)) consider(p - q4, LdsReader::len(data, p, q4, p + cap));
+        if (q4 < p && can_win(p - q4)) consider(p - q4, LdsReader::len8(data, p, q4, p + cap, v, v1));
         if (p + 8 <= n) {
-          const u32 q8 = h8[xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32))];
+          const u32 q8 = h8[xz::hash8((u64)v | ((u64)v1 << 32))];
           // the same earliest position as the 4-byte hit measures the same
           if (q8 < p && q8 != q4 && can_win(p - q8))
-            consider(p - q8, LdsReader::len(data, p, q8, p + cap));
+            consider(p - q8, LdsReader::len8(data, p, q8, p + cap, v, v1));
         }
       }
       if (best) c = (best << 16) | boff;
@@ -853,7 +910,12 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
 }
 
 constexpr u32 kParseBlocks = 2;  // blocks per k_zparse wavefront (32 segments each)
-constexpr u32 kWin = 16;         // candidates per k_zparse window fetch
+constexpr u32 kWin = 20;         // candidates per k_zparse window fetch
+constexpr u32 kRefill = 4;       // a wave-wide refill reloads lanes this far into their window
+// A window read starting at a position below kBlock stays inside the block's
+// scratch (cand is its first member; the words past it are never used).
+static_assert(offsetof(BlkScratch, cand) == 0 && sizeof(BlkScratch) >= sizeof(u32) * (kBlock + kWin),
+              "window past scratch");
 
 __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
                                                const ZBlk* __restrict__ blks, u32 nblk,
@@ -882,21 +944,26 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
       u32 ns = 0, nl = 0;
       // xz::cand_match on the resolved candidate: a capped length is
       // extended to the segment end.  Candidates are fetched kWin at a time
-      // into the lane's LDS window (one global latency per window, not per
-      // position).
+      // into the lane's LDS window.  Lanes move through their segments at
+      // different speeds, so some lane runs out of its window on most steps;
+      // the refill is therefore wave-wide: when any active lane has left its
+      // window, every active lane at least kRefill positions into its own
+      // reloads as well, and the wave waits on global memory once for all.
       u32 wb = 0x80000000u;  // p - wb >= kWin for every p: the first call fetches
       u32* wr = win[lane];
       auto best_at = [&](u32 p) -> u32 {
-        if (p - wb >= kWin) {
+        if (__any(p - wb >= kWin) && p - wb >= kRefill) {
           wb = p & ~3u;
           const uint4* src = reinterpret_cast<const uint4*>(cand + wb);
+          uint4 v[kWin / 4];
+#pragma unroll
+          for (u32 q = 0; q < kWin / 4; ++q) v[q] = src[q];
 #pragma unroll
           for (u32 q = 0; q < kWin / 4; ++q) {
-            const uint4 v = src[q];
-            wr[4 * q] = v.x;
-            wr[4 * q + 1] = v.y;
-            wr[4 * q + 2] = v.z;
-            wr[4 * q + 3] = v.w;
+            wr[4 * q] = v[q].x;
+            wr[4 * q + 1] = v[q].y;
+            wr[4 * q + 2] = v[q].z;
+            wr[4 * q + 3] = v[q].w;
           }
         }
         const u32 c = wr[p - wb];

@@ -804,6 +804,11 @@ XZ_HD inline u32 match_len(const u8* d, u32 p, u32 q, u32 end) {
 struct ByteReader {
   XZ_HD static u32 r32(const u8* d, u32 p) { return read32(d + p); }
   XZ_HD static u32 len(const u8* d, u32 p, u32 q, u32 end) { return match_len(d, p, q, end); }
+  // out[c] = len(d, p, p - off[c], end) for every c with bit c of `want`.
+  XZ_HD static void len3(const u8* d, u32 p, const u32* off, u32 want, u32 end, u32* out) {
+    for (u32 c = 0; c < 3; ++c)
+      if (want >> c & 1u) out[c] = match_len(d, p, p - off[c], end);
+  }
 };
 
 // Score of a match in 1/4-byte-ish units: length minus the offset's cost
@@ -894,49 +899,78 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
     };
     i32 bs = best ? sc_of(best, boff) : -(1 << 30);
     const bool next = lp != kNone && lp + 1 == p;
+    // The offsets to try are fixed by the candidate's offset (an offset
+    // equal to it or to an earlier winner scores no better), so all lengths
+    // still unknown are measured together, then scored in order.
+    u32 l[3], use = 0, measure = 0;
     for (u32 c = 0; c < 3; ++c) {
       const u32 off = rep[c];
-      const u32 known = next && rl[c] != kNone && rl[c] >= 1 ? rl[c] - 1 : kNone;
+      l[c] = next && rl[c] != kNone && rl[c] >= 1 ? rl[c] - 1 : kNone;
       rl[c] = kNone;
       if (!off || off > p || off == boff) continue;
-      const u32 l = known != kNone ? known : R::len(d, p, p - off, b);
-      rl[c] = l;
-      if (l < kMinMatch) continue;
-      const i32 sc = sc_of(l, off);
+      use |= 1u << c;
+      if (l[c] == kNone) measure |= 1u << c;
+    }
+    if (measure) R::len3(d, p, rep, measure, b, l);
+    for (u32 c = 0; c < 3; ++c) {
+      if (!(use >> c & 1u)) continue;
+      rl[c] = l[c];
+      if (l[c] < kMinMatch) continue;
+      const i32 sc = sc_of(l[c], rep[c]);
       if (sc > bs) {
         bs = sc;
-        best = l;
-        boff = off;
+        best = l[c];
+        boff = rep[c];
       }
     }
     lp = p;
     return bs;
   };
-  u32 p = a, lit0 = a;
-  while (p + kMinMatch <= b) {
-    u32 best, boff;
-    i32 sc = find(p, best, boff);
-    if (best < kMinMatch || sc <= 0) {
-      ++p;
-      continue;
+  // One loop, one find() per step: at p, or -- while holding the match
+  // found at p -- at p + 1 (lazy: a literal now if the next position matches
+  // better).  Lanes of a wavefront parsing different segments then run the
+  // same step code instead of diverging between two nested loops.
+  u32 p = a, lit0 = a, best = 0, boff = 0;
+  i32 sc = 0;
+  bool held = false;
+  for (;;) {
+    const u32 q = held ? p + 1 : p;
+    bool take = false;
+    if (q + kMinMatch > b) {
+      if (!held) break;
+      take = true;
+    } else {
+      u32 b1, o1;
+      const i32 sc1 = find(q, b1, o1);
+      if (!held) {
+        if (b1 >= kMinMatch && sc1 > 0) {
+          best = b1;
+          boff = o1;
+          sc = sc1;
+          held = true;
+        } else {
+          ++p;
+        }
+      } else if (b1 < kMinMatch || sc1 - kLitCost <= sc) {
+        take = true;
+      } else {
+        ++p;
+        best = b1;
+        boff = o1;
+        sc = sc1;
+      }
     }
-    u32 b1, o1;  // lazy: a literal now if the next position matches better
-    while (p + 1 + kMinMatch <= b) {
-      const i32 sc1 = find(p + 1, b1, o1);
-      if (b1 < kMinMatch || sc1 - kLitCost <= sc) break;
-      ++p;
-      best = b1;
-      boff = o1;
-      sc = sc1;
-    }
-    emit(lit0, p, best, boff);
-    p += best;
-    lit0 = p;
-    lp = kNone;  // the history may change
-    if (boff != rep[0]) {
-      rep[2] = rep[1];
-      rep[1] = rep[0];
-      rep[0] = boff;
+    if (take) {
+      emit(lit0, p, best, boff);
+      p += best;
+      lit0 = p;
+      lp = kNone;  // the history may change
+      held = false;
+      if (boff != rep[0]) {
+        rep[2] = rep[1];
+        rep[1] = rep[0];
+        rep[0] = boff;
+      }
     }
   }
   return lit0;
