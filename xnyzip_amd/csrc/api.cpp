@@ -1227,6 +1227,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     u32 k = 0;
     for (u32 f = 0; f < (u32)dfr.size(); ++f) {
       const u32 first = k;
+      // the encoder's kernels load blocks 16 bytes at a time
+      if (dfr[f].src & 15u) throw Error(XNYZ_EINTERNAL, "device frame source not 16-byte aligned");
       for (u64 o = 0; o < dfr[f].len; o += xz::kBlock)
         zk[k++] = ZBlk{dfr[f].src + o, (u32)std::min<u64>(xz::kBlock, dfr[f].len - o), f};
       zf.push_back(ZFrame{dfr[f].len, first, k - first});

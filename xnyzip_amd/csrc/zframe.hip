@@ -765,12 +765,25 @@ struct LdsReader {
 
 constexpr u32 kPad = 32;  // LDS block copies: readable bytes past the end (match loads read 20)
 
-// Copies block bytes [0, n) into LDS (4-byte aligned source and target);
-// the padding is zeroed so no load reads uninitialised memory.
-__device__ __forceinline__ void load_block(u8* data, const u8* src, u32 n, u32 tid, u32 nthr) {
-  for (u32 i = tid * 4; i + 4 <= n; i += nthr * 4)
-    *reinterpret_cast<u32*>(data + i) = *reinterpret_cast<const u32*>(src + i);
-  for (u32 i = (n & ~3u) + tid; i < n + kPad; i += nthr) data[i] = i < n ? src[i] : 0;
+// Copies block bytes [0, n) into LDS (16-byte aligned source and target:
+// blocks start at multiples of kBlock in 256-byte aligned streams); the
+// padding is zeroed so no load reads uninitialised memory.  Every lane issues
+// all its 16-byte loads before its first LDS store, so the wave waits on
+// global memory once per block, not once per load.
+template <u32 kThreads>
+__device__ __forceinline__ void load_block(u8* data, const u8* src, u32 n, u32 tid) {
+  constexpr u32 kPer = kBlock / 16 / kThreads;
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  uint4* t = reinterpret_cast<uint4*>(data);
+  const u32 nc = n / 16;
+  uint4 v[kPer];
+#pragma unroll
+  for (u32 k = 0; k < kPer; ++k)
+    if (tid + k * kThreads < nc) v[k] = s[tid + k * kThreads];
+#pragma unroll
+  for (u32 k = 0; k < kPer; ++k)
+    if (tid + k * kThreads < nc) t[tid + k * kThreads] = v[k];
+  for (u32 i = nc * 16 + tid; i < n + kPad; i += kThreads) data[i] = i < n ? src[i] : 0;
 }
 
 // Byte-equality of a block with itself shifted by 1, 8 and 16 bytes, one bit
@@ -810,7 +823,7 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   u32* cand = scr[blockIdx.x].cand;
   const u32 n = bk.n, tid = threadIdx.x;
   constexpr u32 kH = 1u << kHashLog;
-  load_block(data, pre + bk.src, n, tid, 256);
+  load_block<256>(data, pre + bk.src, n, tid);
   for (u32 i = tid; i < 2 * kH; i += 256) tab[i] = 0xFFFFFFFFu;
   __syncthreads();
   // Runs of equal hashes on consecutive lanes (runs of equal bytes) leave
@@ -926,7 +939,7 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
   const u32 lane = threadIdx.x;
   const u32 b0 = blockIdx.x * kParseBlocks;
   for (u32 k = 0; k < kParseBlocks; ++k) {
-    if (b0 + k < nblk) load_block(data[k], pre + blks[b0 + k].src, blks[b0 + k].n, lane, 64);
+    if (b0 + k < nblk) load_block<64>(data[k], pre + blks[b0 + k].src, blks[b0 + k].n, lane);
     for (u32 i = lane; i < 256; i += 64) hist[k][i] = 0;
   }
   __syncthreads();
