@@ -706,11 +706,12 @@ struct LdsReader {
     }
     return l;
   }
-  // out[c] = len(d, p, p - off[c], end) for every c with bit c of `want`,
-  // in shared rounds: one load of p's side and one LDS round trip for all.
-  __device__ static void len3(const u8* d, u32 p, const u32* off, u32 want, u32 end, u32* out) {
+  // out[c] = len(d, p, p - off[c], end) for every c (< 4) with bit c of
+  // `want`, in shared rounds: one load of p's side and one LDS round trip
+  // for all.
+  __device__ static void lens(const u8* d, u32 p, const u32* off, u32 want, u32 end, u32* out) {
     const u32* w = reinterpret_cast<const u32*>(d);
-    const u32 q[3] = {p - off[0], p - off[1], p - off[2]};
+    const u32 q[4] = {p - off[0], p - off[1], p - off[2], p - off[3]};
     for (u32 l = 0; want; l += 16) {
       const u32 r = end - (p + l);  // > 0: a measurement ends once l reaches end - p
       const u32 pa = (p + l) >> 2, ps = (p + l) & 3u;
@@ -718,7 +719,7 @@ struct LdsReader {
 #pragma unroll
       for (u32 k = 0; k < 5; ++k) a[k] = w[pa + k];
 #pragma unroll
-      for (u32 c = 0; c < 3; ++c) {
+      for (u32 c = 0; c < 4; ++c) {
         if (!(want >> c & 1u)) continue;
         const u32 qa = (q[c] + l) >> 2, qs = (q[c] + l) & 3u;
         u32 b[5];
@@ -955,8 +956,8 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
       u8* lit = S.lits + seg * kSeg;
       u32* h = hist[k];
       u32 ns = 0, nl = 0;
-      // xz::cand_match on the resolved candidate: a capped length is
-      // extended to the segment end.  Candidates are fetched kWin at a time
+      // xz::cand_match on the resolved candidate (the capped length;
+      // parse_segment extends it).  Candidates are fetched kWin at a time
       // into the lane's LDS window.  Lanes move through their segments at
       // different speeds, so some lane runs out of its window on most steps;
       // the refill is therefore wave-wide: when any active lane has left its
@@ -979,10 +980,7 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
             wr[4 * q + 3] = v[q].w;
           }
         }
-        const u32 c = wr[p - wb];
-        u32 len = c >> 16;
-        if (len == xz::kCap) len = LdsReader::len(d, p, p - (c & 0xFFFFu), xz::seg_end(p, n));
-        return (len << 16) | (c & 0xFFFFu);
+        return wr[p - wb];
       };
       const u32 tail = xz::parse_segment<LdsReader>(d, a, e, best_at,
                                                     [&](u32 l0, u32 p, u32 len, u32 off) {

@@ -804,9 +804,9 @@ XZ_HD inline u32 match_len(const u8* d, u32 p, u32 q, u32 end) {
 struct ByteReader {
   XZ_HD static u32 r32(const u8* d, u32 p) { return read32(d + p); }
   XZ_HD static u32 len(const u8* d, u32 p, u32 q, u32 end) { return match_len(d, p, q, end); }
-  // out[c] = len(d, p, p - off[c], end) for every c with bit c of `want`.
-  XZ_HD static void len3(const u8* d, u32 p, const u32* off, u32 want, u32 end, u32* out) {
-    for (u32 c = 0; c < 3; ++c)
+  // out[c] = len(d, p, p - off[c], end) for every c (< 4) with bit c of `want`.
+  XZ_HD static void lens(const u8* d, u32 p, const u32* off, u32 want, u32 end, u32* out) {
+    for (u32 c = 0; c < 4; ++c)
       if (want >> c & 1u) out[c] = match_len(d, p, p - off[c], end);
   }
 };
@@ -867,20 +867,20 @@ XZ_HD inline u8 best_fixed(const u8* d, u32 n, const u32* ht4, const u32* ht8, u
 }
 
 // The match a candidate byte stands for: (length << 16) | offset, the
-// length extended past kCap to the segment end.
+// length capped at kCap (parse_segment measures a capped one to the segment
+// end).
 template <typename R = ByteReader>
 XZ_HD inline u32 cand_match(const u8* d, u32 n, const u32* ht4, const u32* ht8, u32 p, u8 info) {
   if (!info) return 0;
   const u32 off = cand_offset<R>(d, n, ht4, ht8, p, info >> 5);
-  u32 len = (info & 31u) + kMinMatch;
-  if (len == kCap) len = R::len(d, p, p - off, seg_end(p, n));
-  return (len << 16) | off;
+  return (((info & 31u) + kMinMatch) << 16) | off;
 }
 
 // Lazy parse of segment [a, b): at each position the precomputed best_at(p)
-// competes with the segment's last three offsets (scored with the repeat
-// discount); a literal is emitted first when the next position scores
-// better by more than a literal's cost.
+// -- (length << 16) | offset, the length capped at kCap, 0 if none; a kCap
+// length is measured to the segment end here -- competes with the segment's
+// last three offsets (scored with the repeat discount); a literal is emitted
+// first when the next position scores better by more than a literal's cost.
 template <typename R = ByteReader, typename Best, typename Emit>
 XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&& emit) {
   u32 rep[3] = {0, 0, 0};
@@ -897,12 +897,13 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
       const bool r = off == rep[0] || off == rep[1] || off == rep[2];
       return (i32)l * kLenWeight - (r ? kRepCost : (i32)highbit(off + 3) + kOffCost);
     };
-    i32 bs = best ? sc_of(best, boff) : -(1 << 30);
     const bool next = lp != kNone && lp + 1 == p;
     // The offsets to try are fixed by the candidate's offset (an offset
     // equal to it or to an earlier winner scores no better), so all lengths
-    // still unknown are measured together, then scored in order.
-    u32 l[3], use = 0, measure = 0;
+    // still unknown -- the repeat offsets' and a capped candidate's -- are
+    // measured together, then scored in order.
+    const u32 offs[4] = {rep[0], rep[1], rep[2], boff};
+    u32 l[4], use = 0, measure = best == kCap ? 8u : 0u;
     for (u32 c = 0; c < 3; ++c) {
       const u32 off = rep[c];
       l[c] = next && rl[c] != kNone && rl[c] >= 1 ? rl[c] - 1 : kNone;
@@ -911,7 +912,9 @@ XZ_HD inline u32 parse_segment(const u8* d, u32 a, u32 b, Best&& best_at, Emit&&
       use |= 1u << c;
       if (l[c] == kNone) measure |= 1u << c;
     }
-    if (measure) R::len3(d, p, rep, measure, b, l);
+    if (measure) R::lens(d, p, offs, measure, b, l);
+    if (measure & 8u) best = l[3];
+    i32 bs = best ? sc_of(best, boff) : -(1 << 30);
     for (u32 c = 0; c < 3; ++c) {
       if (!(use >> c & 1u)) continue;
       rl[c] = l[c];
