@@ -105,13 +105,10 @@ __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
              (u32)__shfl((int)q.off, (int)j, 64)};
 }
 
-// Repeat-offset history over the sequences (zframe.hpp seq_codes), one
-// step per sequence, into L.ov.
 // Offset_Values of a block's sequences in order (repeat codes against the
 // running three-offset history; RFC 8878 3.1.1.5), as xz::seq_codes
-// computes them: one lane walks the sequences with the history in registers,
-// over LDS entries (offset | 0x8000 when the literal length is 0) rewritten
-// in place.
+// computes them, into L.ov: LDS entries (offset | 0x8000 when the literal
+// length is 0) rewritten in place.
 __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
   const u32 lane = threadIdx.x;
   for (u32 i = lane; i < ns; i += 64) {
@@ -119,65 +116,54 @@ __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
     L.ov[i] = (u16)(q.off | (q.ll ? 0u : 0x8000u));
   }
   __syncthreads();
-  // The whole wavefront walks the history on wavefront-uniform values
-  // (scalar registers and branches); lanes 0..kB-1 store a batch's results.
-  {
-    u32 r0 = 0, r1 = 0, r2 = 0, known = 0;
-    constexpr u32 kB = 8;
-    for (u32 i0 = 0; i0 < ns; i0 += kB) {
-      u32 xb[kB], res[kB];
-#pragma unroll
-      for (u32 q = 0; q < kB; ++q)
-        xb[q] = __builtin_amdgcn_readfirstlane(i0 + q < ns ? (u32)L.ov[i0 + q] : 0u);
-#pragma unroll
-      for (u32 q = 0; q < kB; ++q) {
-        const u32 x = xb[q], o = x & 0x7FFFu;
-        u32 ov = 0;
-        if (!(x & 0x8000u)) {
-          if (known >= 1 && o == r0) {
-            ov = 1;
-          } else if (known >= 2 && o == r1) {
-            ov = 2;
-            r1 = r0;
-            r0 = o;
-          } else if (known >= 3 && o == r2) {
-            ov = 3;
-            r2 = r1;
-            r1 = r0;
-            r0 = o;
-          }
-        } else {
-          if (known >= 2 && o == r1) {
-            ov = 1;
-            r1 = r0;
-            r0 = o;
-          } else if (known >= 3 && o == r2) {
-            ov = 2;
-            r2 = r1;
-            r1 = r0;
-            r0 = o;
-          } else if (known >= 1 && r0 > 1 && o == r0 - 1) {
-            ov = 3;
-            r2 = r1;
-            r1 = r0;
-            r0 = o;
-            known = known < 3 ? known + 1 : 3;
-          }
-        }
-        if (!ov) {
-          ov = o + 3;
-          r2 = r1;
-          r1 = r0;
-          r0 = o;
-          known = known < 3 ? known + 1 : 3;
-        }
-        res[q] = (i0 + q < ns) ? ov : 0u;
-      }
-      u32 mine = res[0];
-#pragma unroll
-      for (u32 q = 1; q < kB; ++q) mine = lane == q ? res[q] : mine;
-      if (lane < kB && i0 + lane < ns) L.ov[i0 + lane] = (u16)mine;
+  // Every step puts its offset in slot 0 of the history; a step that repeats
+  // slot 0 after literals ("stay") leaves the history as it is, and every
+  // other step moves the old slot 0 to slot 1.  So slot 0 before step i is
+  // o[i-1], and slot 1 is a copy-forward scan over the batch of 64 steps
+  // (one per lane).  Only slot 2 and the count of known slots need a serial
+  // walk: the wavefront runs it on uniform values, reading each step's
+  // packed inputs with readlane; lane q keeps step q's match flags, from
+  // which it forms its offset value.
+  constexpr u32 kNone = 0xFFFFFFFFu;
+  u32 prev_o = 0, slot1 = 0;  // o and slot 1 after the previous batch
+  u32 r2 = 0, known = 0;      // the serial walk's state
+  for (u32 i0 = 0; i0 < ns; i0 += 64) {
+    const u32 g = i0 + lane;
+    const u32 x = g < ns ? (u32)L.ov[g] : 0u;
+    const u32 o = x & 0x7FFFu;
+    const bool z = (x & 0x8000u) != 0;  // literal length 0
+    u32 po = (u32)__shfl_up((int)o, 1, 64);
+    if (lane == 0) po = prev_o;
+    const bool stay = g >= 1 && !z && o == po;
+    const bool em = g >= 1 && po > 1 && o == po - 1;
+    u32 s1 = stay ? kNone : po;  // slot 1 after step g
+    for (u32 d = 1; d < 64; d <<= 1) {
+      const u32 u = (u32)__shfl_up((int)s1, d, 64);
+      if (s1 == kNone && lane >= d) s1 = u;
     }
+    if (s1 == kNone) s1 = slot1;
+    u32 r1 = (u32)__shfl_up((int)s1, 1, 64);  // slot 1 before step g
+    if (lane == 0) r1 = slot1;
+    prev_o = (u32)__builtin_amdgcn_readlane((int)o, 63);
+    slot1 = (u32)__builtin_amdgcn_readlane((int)s1, 63);
+    const u32 pack = o | (r1 << 15) | (stay ? 1u << 30 : 0u);
+    const u32 m = ns - i0 < 64 ? ns - i0 : 64;
+    u32 flags = 0;
+    for (u32 q = 0; q < m; ++q) {
+      const u32 y = (u32)__builtin_amdgcn_readlane((int)pack, (int)q);
+      const u32 oq = y & 0x7FFFu, r1q = (y >> 15) & 0x7FFFu;
+      const bool e1 = known >= 2 && oq == r1q, e2 = known >= 3 && oq == r2;
+      flags = lane == q ? (e1 ? 1u : 0u) | (e2 ? 2u : 0u) : flags;
+      if (!(y >> 30)) {
+        r2 = e1 ? r2 : r1q;
+        known = !e1 && !e2 && known < 3 ? known + 1 : known;
+      }
+    }
+    const bool e1 = flags & 1u, e2 = flags & 2u;
+    u32 ov;
+    if (!z) ov = stay ? 1u : e1 ? 2u : e2 ? 3u : o + 3;
+    else ov = e1 ? 1u : e2 ? 2u : em ? 3u : o + 3;
+    if (g < ns) L.ov[g] = (u16)ov;
   }
   __syncthreads();
 }
