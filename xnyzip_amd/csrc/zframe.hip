@@ -763,13 +763,15 @@ __device__ __forceinline__ void load_block(u8* data, const u8* src, u32 n, u32 t
   const uint4* s = reinterpret_cast<const uint4*>(src);
   uint4* t = reinterpret_cast<uint4*>(data);
   const u32 nc = n / 16;
-  uint4 v[kPer];
+  if (nc) {
+    // unconditional loads (clamped to the last chunk) keep v in registers
+    uint4 v[kPer];
 #pragma unroll
-  for (u32 k = 0; k < kPer; ++k)
-    if (tid + k * kThreads < nc) v[k] = s[tid + k * kThreads];
+    for (u32 k = 0; k < kPer; ++k) v[k] = s[tid + k * kThreads < nc ? tid + k * kThreads : nc - 1];
 #pragma unroll
-  for (u32 k = 0; k < kPer; ++k)
-    if (tid + k * kThreads < nc) t[tid + k * kThreads] = v[k];
+    for (u32 k = 0; k < kPer; ++k)
+      if (tid + k * kThreads < nc) t[tid + k * kThreads] = v[k];
+  }
   for (u32 i = nc * 16 + tid; i < n + kPad; i += kThreads) data[i] = i < n ? src[i] : 0;
 }
 
@@ -801,43 +803,62 @@ __device__ __forceinline__ u32 eq_run(const u32* eq, u32 p, u32 cap) {
 __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
                                                const ZBlk* __restrict__ blks,
                                                BlkScratch* __restrict__ scr) {
-  __shared__ __align__(16) u8 data[kBlock + kPad];
-  // first the two hash tables (u32, built with atomicMin), then the same
-  // tables as u16 in the first half and the equality masks in the second
-  __shared__ __align__(16) u32 tab[2u << kHashLog];
-  static_assert(3 * kEqWords * 4 <= (4u << kHashLog), "masks fit the second half");
+  // LDS (40 KiB: four workgroups per CU): the block, the 4-byte hash table
+  // (u16), and a u32 area in which each table is built in turn with
+  // atomicMin; the 8-byte table ends as u16 in its first half, the equality
+  // masks take the second.  The block has no padding of its own: its copy's
+  // zero padding and every read past its end (all clamped to the block's
+  // bytes) fall in the 4-byte table.
+  constexpr u32 kH = 1u << kHashLog;
+  __shared__ __align__(16) u32 lds[(kBlock + 2 * kH + 4 * kH) / 4];
+  static_assert(kPad <= 2 * kH, "padding falls in the 4-byte table");
+  static_assert(3 * kEqWords * 4 <= 2 * kH, "masks fit the second half");
+  u8* const data = reinterpret_cast<u8*>(lds);
+  u16* const h4t = reinterpret_cast<u16*>(data + kBlock);
+  u32* const bld = reinterpret_cast<u32*>(data + kBlock + 2 * kH);
   const ZBlk bk = blks[blockIdx.x];
   u32* cand = scr[blockIdx.x].cand;
-  const u32 n = bk.n, tid = threadIdx.x;
-  constexpr u32 kH = 1u << kHashLog;
+  const u32 n = bk.n, tid = threadIdx.x, lane = tid & (kWave - 1);
   load_block<256>(data, pre + bk.src, n, tid);
-  for (u32 i = tid; i < 2 * kH; i += 256) tab[i] = 0xFFFFFFFFu;
+  for (u32 i = tid; i < kH; i += 256) bld[i] = 0xFFFFFFFFu;
   __syncthreads();
   // Runs of equal hashes on consecutive lanes (runs of equal bytes) leave
   // the update to the run's first lane, whose position is the smallest:
   // atomics on one address from a whole wavefront would serialise.
   for (u32 p0 = 0; p0 < n; p0 += 256) {
     const u32 p = p0 + tid;
-    const bool ok4 = p + 4 <= n, ok8 = p + 8 <= n;
-    const u32 v = ok4 ? LdsReader::r32(data, p) : 0u;
-    const u32 h4 = ok4 ? xz::hash4(v) : 0xFFFFFFFFu;
-    const u32 h8 = ok8 ? xz::hash8((u64)v | ((u64)LdsReader::r32(data, p + 4) << 32)) : 0xFFFFFFFFu;
-    const u32 lane = tid & (kWave - 1);
-    const u32 up4 = (u32)__shfl_up((int)h4, 1, kWave), up8 = (u32)__shfl_up((int)h8, 1, kWave);
-    if (ok4 && (lane == 0 || up4 != h4)) atomicMin(&tab[h4], p);
-    if (ok8 && (lane == 0 || up8 != h8)) atomicMin(&tab[kH + h8], p);
+    const bool ok = p + 4 <= n;
+    const u32 h = ok ? xz::hash4(LdsReader::r32(data, p)) : 0xFFFFFFFFu;
+    const u32 up = (u32)__shfl_up((int)h, 1, kWave);
+    if (ok && (lane == 0 || up != h)) atomicMin(&bld[h], p);
+  }
+  __syncthreads();
+#pragma unroll
+  for (u32 k = 0; k < kH / 256; ++k) {
+    const u32 t = bld[tid + 256 * k];
+    h4t[tid + 256 * k] = (u16)(t < 0xFFFFu ? t : 0xFFFFu);
+    bld[tid + 256 * k] = 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (u32 p0 = 0; p0 < n; p0 += 256) {
+    const u32 p = p0 + tid;
+    const bool ok = p + 8 <= n;
+    const u32 h = ok ? xz::hash8((u64)LdsReader::r32(data, p) | ((u64)LdsReader::r32(data, p + 4) << 32))
+                     : 0xFFFFFFFFu;
+    const u32 up = (u32)__shfl_up((int)h, 1, kWave);
+    if (ok && (lane == 0 || up != h)) atomicMin(&bld[h], p);
   }
   __syncthreads();
   {
-    u32 t[2 * kH / 256];
+    u32 t[kH / 256];
 #pragma unroll
-    for (u32 k = 0; k < 2 * kH / 256; ++k) t[k] = tab[tid + 256 * k];
+    for (u32 k = 0; k < kH / 256; ++k) t[k] = bld[tid + 256 * k];
     __syncthreads();
-    u16* h16 = reinterpret_cast<u16*>(tab);
+    u16* h16 = reinterpret_cast<u16*>(bld);
 #pragma unroll
-    for (u32 k = 0; k < 2 * kH / 256; ++k) h16[tid + 256 * k] = (u16)(t[k] < 0xFFFFu ? t[k] : 0xFFFFu);
+    for (u32 k = 0; k < kH / 256; ++k) h16[tid + 256 * k] = (u16)(t[k] < 0xFFFFu ? t[k] : 0xFFFFu);
   }
-  u32* eq = tab + kH;  // [3][kEqWords]
+  u32* eq = bld + kH / 2;  // [3][kEqWords]
   {
     const u32* dw = reinterpret_cast<const u32*>(data);
     const u32 rot = (tid >> 3) & 7u;  // conflict-free LDS banks across the wavefront
@@ -863,8 +884,8 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   }
   __syncthreads();
   // xz::best_fixed: offsets 1, 8, 16 from the masks, then the hash hits
-  const u16* h4 = reinterpret_cast<const u16*>(tab);
-  const u16* h8 = h4 + kH;
+  const u16* h4 = h4t;
+  const u16* h8 = reinterpret_cast<const u16*>(bld);
   for (u32 p = tid; p < n; p += 256) {
     const u32 e = xz::seg_end(p, n);
     u32 c = 0;
