@@ -2,15 +2,16 @@
 // streams in HBM (format and parse rules in zframe.hpp).
 //
 //   k_zcand    one workgroup per 16 KiB block: the block and its two hash
-//              tables in LDS (48 KiB; the tables are narrowed to 16 bits
-//              after the build so byte-equality masks for the offsets 1, 8
-//              and 16 fit beside them); all 256 threads evaluate every
-//              position's best fixed candidate and write it, resolved to
-//              (capped length, offset), to global scratch.
-//   k_zparse   one wavefront per two blocks, one lane per 512-byte segment:
-//              the blocks' bytes and a candidate window per lane in LDS
-//              (38 KiB, so four such wavefronts share a CU); each lane runs
-//              the lazy parse of its segment, writing its sequences and
+//              tables in LDS (40 KiB, four workgroups per CU: the tables are
+//              built one after the other in one 32-bit area and narrowed to
+//              16 bits, and byte-equality masks for the offsets 1, 8 and 16
+//              take the rest); all 256 threads evaluate every position's best
+//              fixed candidate and write it, resolved to (capped length,
+//              offset), to global scratch.
+//   k_zparse   one 32-lane workgroup per block, one lane per 512-byte
+//              segment: the block's bytes and a candidate window per lane in
+//              LDS (20 KiB, so two such wavefronts share a SIMD); each lane
+//              runs the lazy parse of its segment, writing its sequences and
 //              literals into per-segment slots and the block's literal
 //              histogram.
 //   Both read the block with aligned 32-bit LDS loads and byte funnel
@@ -22,10 +23,10 @@
 //              all lanes at offsets from a wavefront scan).
 //   k_zseqs    one wavefront per block: sequences compacted in order (the
 //              first sequence of a segment takes the literals left pending
-//              by the segments before it), repeat-offset codes, the three
-//              FSE tables built across the lanes, the FSE state chains on
-//              three lanes and the bitstream packed by all lanes; the block
-//              size (or raw) decided.
+//              by the segments before it), repeat-offset codes (a scan plus
+//              a short serial walk), the three FSE tables built across the
+//              lanes, the FSE state chains on three lanes and the bitstream
+//              packed by all lanes; the block size (or raw) decided.
 //   k_zgather  each block's header and sections (or its raw bytes) copied
 //              behind its frame header, positions from a device-wide scan of
 //              the block sizes.
@@ -759,18 +760,24 @@ constexpr u32 kPad = 32;  // LDS block copies: readable bytes past the end (matc
 // global memory once per block, not once per load.
 template <u32 kThreads>
 __device__ __forceinline__ void load_block(u8* data, const u8* src, u32 n, u32 tid) {
-  constexpr u32 kPer = kBlock / 16 / kThreads;
+  constexpr u32 kPer = kBlock / 16 / kThreads, kBat = kPer < 16 ? kPer : 16;
   const uint4* s = reinterpret_cast<const uint4*>(src);
   uint4* t = reinterpret_cast<uint4*>(data);
   const u32 nc = n / 16;
   if (nc) {
     // unconditional loads (clamped to the last chunk) keep v in registers
-    uint4 v[kPer];
 #pragma unroll
-    for (u32 k = 0; k < kPer; ++k) v[k] = s[tid + k * kThreads < nc ? tid + k * kThreads : nc - 1];
+    for (u32 k0 = 0; k0 < kPer; k0 += kBat) {
+      uint4 v[kBat];
 #pragma unroll
-    for (u32 k = 0; k < kPer; ++k)
-      if (tid + k * kThreads < nc) t[tid + k * kThreads] = v[k];
+      for (u32 k = 0; k < kBat; ++k) {
+        const u32 i = tid + (k0 + k) * kThreads;
+        v[k] = s[i < nc ? i : nc - 1];
+      }
+#pragma unroll
+      for (u32 k = 0; k < kBat; ++k)
+        if (tid + (k0 + k) * kThreads < nc) t[tid + (k0 + k) * kThreads] = v[k];
+    }
   }
   for (u32 i = nc * 16 + tid; i < n + kPad; i += kThreads) data[i] = i < n ? src[i] : 0;
 }
@@ -930,7 +937,8 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   }
 }
 
-constexpr u32 kParseBlocks = 2;  // blocks per k_zparse wavefront (32 segments each)
+constexpr u32 kParseBlocks = 1;  // blocks per k_zparse workgroup (32 segments each)
+constexpr u32 kParseLanes = kParseBlocks * kSegs;  // one lane per segment
 constexpr u32 kWin = 20;         // candidates per k_zparse window fetch
 constexpr u32 kRefill = 4;       // a wave-wide refill reloads lanes this far into their window
 // A window read starting at a position below kBlock stays inside the block's
@@ -938,17 +946,17 @@ constexpr u32 kRefill = 4;       // a wave-wide refill reloads lanes this far in
 static_assert(offsetof(BlkScratch, cand) == 0 && sizeof(BlkScratch) >= sizeof(u32) * (kBlock + kWin),
               "window past scratch");
 
-__global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
-                                               const ZBlk* __restrict__ blks, u32 nblk,
-                                               BlkScratch* __restrict__ scr) {
+__global__ __launch_bounds__(kParseLanes) void k_zparse(const u8* __restrict__ pre,
+                                                        const ZBlk* __restrict__ blks, u32 nblk,
+                                                        BlkScratch* __restrict__ scr) {
   __shared__ __align__(16) u8 data[kParseBlocks][kBlock + kPad];
   __shared__ u32 hist[kParseBlocks][256];
-  __shared__ u32 win[64][kWin + 1];  // per lane: candidates of positions [wb, wb + kWin)
+  __shared__ u32 win[kParseLanes][kWin + 1];  // per lane: candidates of positions [wb, wb + kWin)
   const u32 lane = threadIdx.x;
   const u32 b0 = blockIdx.x * kParseBlocks;
   for (u32 k = 0; k < kParseBlocks; ++k) {
-    if (b0 + k < nblk) load_block<64>(data[k], pre + blks[b0 + k].src, blks[b0 + k].n, lane);
-    for (u32 i = lane; i < 256; i += 64) hist[k][i] = 0;
+    if (b0 + k < nblk) load_block<kParseLanes>(data[k], pre + blks[b0 + k].src, blks[b0 + k].n, lane);
+    for (u32 i = lane; i < 256; i += kParseLanes) hist[k][i] = 0;
   }
   __syncthreads();
   const u32 k = lane / kSegs, seg = lane % kSegs, b = b0 + k;
@@ -1008,7 +1016,7 @@ __global__ __launch_bounds__(64) void k_zparse(const u8* __restrict__ pre,
   __syncthreads();
   for (u32 kk = 0; kk < kParseBlocks; ++kk)
     if (b0 + kk < nblk)
-      for (u32 i = lane; i < 256; i += 64) scr[b0 + kk].hist[i] = hist[kk][i];
+      for (u32 i = lane; i < 256; i += kParseLanes) scr[b0 + kk].hist[i] = hist[kk][i];
 }
 
 __global__ __launch_bounds__(64) void k_zlits(const ZBlk* __restrict__ blks,
@@ -1199,7 +1207,7 @@ void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* fram
   k_zcand<<<nblk, 256, 0, s>>>(pre, blks, scr);
   check_hip(hipGetLastError(), "k_zcand");
   if (g_profiler) g_profiler->mark("k_zcand", s);
-  k_zparse<<<(nblk + kParseBlocks - 1) / kParseBlocks, 64, 0, s>>>(pre, blks, nblk, scr);
+  k_zparse<<<(nblk + kParseBlocks - 1) / kParseBlocks, kParseLanes, 0, s>>>(pre, blks, nblk, scr);
   check_hip(hipGetLastError(), "k_zparse");
   if (g_profiler) g_profiler->mark("k_zparse", s);
   k_zlits<<<nblk, 64, 0, s>>>(blks, scr, slots);
