@@ -563,6 +563,8 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
   return nbytes;
 }
 
+constexpr u32 kLitBatch = 16;  // literal bytes each lane loads at once
+
 // Huffman-coded literals as write_huf_lits lays them out, written at their
 // final place: slot + base (slot 256-byte aligned; words are addressed from
 // the slot so that every 32-bit store and atomic is aligned).  Returns 0
@@ -578,7 +580,17 @@ __device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, LitLds& L)
   for (u32 k = 0; k < nst; ++k) {
     const u32 a = k * seg, e = k + 1 < nst ? a + seg : n;
     u32 bits = 0;
-    for (u32 i = a + lane; i < e; i += 64) bits += L.h.nbits[lit[i]];
+    for (u32 i0 = a; i0 < e; i0 += 64 * kLitBatch) {  // loads of a batch issued together
+      u32 c[kLitBatch];
+#pragma unroll
+      for (u32 q = 0; q < kLitBatch; ++q) {
+        const u32 i = i0 + q * 64 + lane;
+        c[q] = lit[i < e ? i : e - 1];
+      }
+#pragma unroll
+      for (u32 q = 0; q < kLitBatch; ++q)
+        if (i0 + q * 64 + lane < e) bits += L.h.nbits[c[q]];
+    }
     bits = wave_sum(bits);
     sbytes[k] = (bits + 8) / 8;  // with the end mark
     if (lane == 0) L.sbits[k] = bits;
@@ -624,26 +636,44 @@ __device__ u32 dev_huf_lits(u8* slot, u32 base, const u8* lit, u32 n, LitLds& L)
   u32* w = reinterpret_cast<u32*>(slot);
   for (u32 k = 0; k < nst; ++k) {
     const u32 a = k * seg, e = k + 1 < nst ? a + seg : n, m = e - a;
-    const u32 per = (m + 63) / 64, r0 = lane * per, r1 = r0 + per < m ? r0 + per : m;
+    const u32 per = (m + 63) / 64, r0 = lane * per < m ? lane * per : m;
+    const u32 r1 = r0 + per < m ? r0 + per : m;
+    // the lane's literals, read kLitBatch at a time (loads issued together)
+    auto batch = [&](u32 rb, u32* c) {
+#pragma unroll
+      for (u32 q = 0; q < kLitBatch; ++q) c[q] = lit[e - 1 - (rb + q < r1 ? rb + q : r1 - 1)];
+    };
     u32 bits = 0;
-    for (u32 r = r0; r < r1; ++r) bits += L.h.nbits[lit[e - 1 - r]];
+    for (u32 rb = r0; rb < r1; rb += kLitBatch) {
+      u32 c[kLitBatch];
+      batch(rb, c);
+#pragma unroll
+      for (u32 q = 0; q < kLitBatch; ++q)
+        if (rb + q < r1) bits += L.h.nbits[c[q]];
+    }
     const u32 inc = wave_incl_scan(bits);
     u64 pos = (u64)(base + at) * 8 + (inc - bits);  // bit position from the slot start
     u64 acc = 0;
     u32 nb = (u32)(pos & 31);
     u64 word = pos >> 5;
     bool first = true;
-    for (u32 r = r0; r < r1; ++r) {
-      const u8 c = lit[e - 1 - r];
-      acc |= (u64)L.h.code[c] << nb;
-      nb += L.h.nbits[c];
-      if (nb >= 32) {
-        if (first) atomicOr(&w[word], (u32)acc);
-        else w[word] = (u32)acc;
-        first = false;
-        acc >>= 32;
-        nb -= 32;
-        ++word;
+    for (u32 rb = r0; rb < r1; rb += kLitBatch) {
+      u32 cb[kLitBatch];
+      batch(rb, cb);
+#pragma unroll
+      for (u32 q = 0; q < kLitBatch; ++q) {
+        if (rb + q >= r1) break;
+        const u32 c = cb[q];
+        acc |= (u64)L.h.code[c] << nb;
+        nb += L.h.nbits[c];
+        if (nb >= 32) {
+          if (first) atomicOr(&w[word], (u32)acc);
+          else w[word] = (u32)acc;
+          first = false;
+          acc >>= 32;
+          nb -= 32;
+          ++word;
+        }
       }
     }
     if (nb) atomicOr(&w[word], (u32)acc);
@@ -1039,9 +1069,23 @@ __global__ __launch_bounds__(64) void k_zlits(const ZBlk* __restrict__ blks,
   }
   __syncthreads();
   const u32 nl = L.loff[nseg];
-  for (u32 t = 0; t < nseg; ++t) {
-    const u32 ln = L.segl[t], lo = L.loff[t];
-    for (u32 i = lane; i < ln; i += 64) S.clit[lo + i] = S.lits[t * kSeg + i];
+  // Four segments at a time, every lane's loads (up to 8 bytes of each)
+  // issued before its stores, so the wave waits on memory once per group.
+  for (u32 t0 = 0; t0 < nseg; t0 += 4) {
+    u32 v[4][kSeg / 64];
+#pragma unroll
+    for (u32 tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (u32 j = 0; j < kSeg / 64; ++j)  // segment slots are always readable
+        v[tt][j] = S.lits[(t0 + tt < kSegs ? t0 + tt : kSegs - 1) * kSeg + j * 64 + lane];
+#pragma unroll
+    for (u32 tt = 0; tt < 4; ++tt) {
+      const u32 t = t0 + tt;
+      const u32 ln = t < nseg ? L.segl[t] : 0u, lo = t < nseg ? L.loff[t] : 0u;
+#pragma unroll
+      for (u32 j = 0; j < kSeg / 64; ++j)
+        if (j * 64 + lane < ln) S.clit[lo + j * 64 + lane] = (u8)v[tt][j];
+    }
   }
   __threadfence_block();
   __syncthreads();
