@@ -64,6 +64,7 @@ struct alignas(16) BlkScratch {  // per block, in global memory
   u8 clit[kBlock];  // compacted (k_zlits)
   u32 hist[256];
   u32 nseq[kSegs], nlit[kSegs];
+  u32 ntail[kSegs];  // literals after a segment's last sequence (k_zparse)
 };
 
 constexpr u32 kMaxSeqs = kBlock / kMinMatch;
@@ -97,9 +98,11 @@ struct SeqLds {  // k_zseqs
   xz::SeqWork sw;
   FseLds F;
   u16 ov[kMaxSeqs];  // offset values (offset + 3 or a repeat code), < 2^15 in a 16 KiB block
-  u32 segn[kSegs], segl[kSegs], segll[kSegs], carry[kSegs], soff[kSegs + 1];
+  u32 segn[kSegs], segl[kSegs], segt[kSegs], carry[kSegs], soff[kSegs + 1];
   u32 fin[3];  // FSE states after the first sequence (flushed at the end)
 };
+
+constexpr u32 kSeqBatch = 4;  // sequences each lane loads at once
 
 __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
   return Seq{(u32)__shfl((int)q.ll, (int)j, 64), (u32)__shfl((int)q.ml, (int)j, 64),
@@ -112,9 +115,18 @@ __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
 // length is 0) rewritten in place.
 __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
   const u32 lane = threadIdx.x;
-  for (u32 i = lane; i < ns; i += 64) {
-    const Seq q = cs[i];
-    L.ov[i] = (u16)(q.off | (q.ll ? 0u : 0x8000u));
+  for (u32 i0 = 0; i0 < ns; i0 += 64 * kSeqBatch) {
+    Seq q[kSeqBatch];
+#pragma unroll
+    for (u32 b = 0; b < kSeqBatch; ++b) {
+      const u32 i = i0 + b * 64 + lane;
+      q[b] = cs[i < ns ? i : ns - 1];
+    }
+#pragma unroll
+    for (u32 b = 0; b < kSeqBatch; ++b) {
+      const u32 i = i0 + b * 64 + lane;
+      if (i < ns) L.ov[i] = (u16)(q[b].off | (q[b].ll ? 0u : 0x8000u));
+    }
   }
   __syncthreads();
   // Every step puts its offset in slot 0 of the history; a step that repeats
@@ -492,18 +504,36 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
   }
   __threadfence_block();
   __syncthreads();
-  // stream order r = 0 .. ns - 1 is sequence ns - 1 - r
-  auto seq_bits = [&](u32 i) -> u32 {
-    const Seq q = S.cseq[i];
-    const u32 ov = L.ov[i];
-    u32 b = xz::ll_bits(xz::ll_code(q.ll)) + xz::ml_bits(xz::ml_code(q.ml)) + xz::highbit(ov);
-    if (i + 1 < ns) b += (S.fse[0][i] >> 10) + (S.fse[1][i] >> 10) + (S.fse[2][i] >> 10);
-    return b;
-  };
+  // stream order r = 0 .. ns - 1 is sequence ns - 1 - r; each lane takes a
+  // run of it, its sequences and FSE records loaded kSeqBatch at a time
   const u32 per = (ns + 63) / 64, r0 = lane * per < ns ? lane * per : ns;
   const u32 r1 = r0 + per < ns ? r0 + per : ns;
+  struct SeqRec {
+    Seq q;
+    u32 f[3];  // FSE records (bits << 10 | value) of LL, OF, ML; unused for the last sequence
+  };
+  auto load = [&](u32 rb, SeqRec* v) {
+#pragma unroll
+    for (u32 b = 0; b < kSeqBatch; ++b) {
+      const u32 i = ns - 1 - (rb + b < r1 ? rb + b : r1 - 1);
+      v[b].q = S.cseq[i];
+#pragma unroll
+      for (u32 k = 0; k < 3; ++k) v[b].f[k] = S.fse[k][i];
+    }
+  };
   u32 bits = 0;
-  for (u32 r = r0; r < r1; ++r) bits += seq_bits(ns - 1 - r);
+  for (u32 rb = r0; rb < r1; rb += kSeqBatch) {
+    SeqRec v[kSeqBatch];
+    load(rb, v);
+#pragma unroll
+    for (u32 b = 0; b < kSeqBatch; ++b) {
+      if (rb + b >= r1) break;
+      const u32 i = ns - 1 - (rb + b);
+      const Seq& q = v[b].q;
+      bits += xz::ll_bits(xz::ll_code(q.ll)) + xz::ml_bits(xz::ml_code(q.ml)) + xz::highbit(L.ov[i]);
+      if (i + 1 < ns) bits += (v[b].f[0] >> 10) + (v[b].f[1] >> 10) + (v[b].f[2] >> 10);
+    }
+  }
   const u32 inc = wave_incl_scan(bits);
   const u32 total = __shfl((int)inc, 63, 64);
   const u32 tl = (u32)L.sw.t[0].log, to = (u32)L.sw.t[1].log, tm = (u32)L.sw.t[2].log;
@@ -529,19 +559,25 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
       ++word;
     }
   };
-  for (u32 r = r0; r < r1; ++r) {
-    const u32 i = ns - 1 - r;
-    const Seq q = S.cseq[i];
-    const SeqCodes c = codes_of(q, L.ov[i]);
-    if (i + 1 < ns) {
-      const u32 fo = S.fse[1][i], fm = S.fse[2][i], fl = S.fse[0][i];
-      put(fo & 1023u, fo >> 10);
-      put(fm & 1023u, fm >> 10);
-      put(fl & 1023u, fl >> 10);
+  for (u32 rb = r0; rb < r1; rb += kSeqBatch) {
+    SeqRec v[kSeqBatch];
+    load(rb, v);
+#pragma unroll
+    for (u32 b = 0; b < kSeqBatch; ++b) {
+      if (rb + b >= r1) break;
+      const u32 i = ns - 1 - (rb + b);
+      const Seq& q = v[b].q;
+      const SeqCodes c = codes_of(q, L.ov[i]);
+      if (i + 1 < ns) {
+        const u32 fo = v[b].f[1], fm = v[b].f[2], fl = v[b].f[0];
+        put(fo & 1023u, fo >> 10);
+        put(fm & 1023u, fm >> 10);
+        put(fl & 1023u, fl >> 10);
+      }
+      put(q.ll - xz::ll_base(c.ll), xz::ll_bits(c.ll));
+      put(q.ml - xz::ml_base(c.ml), xz::ml_bits(c.ml));
+      put(c.ov - (1u << c.of), c.of);
     }
-    put(q.ll - xz::ll_base(c.ll), xz::ll_bits(c.ll));
-    put(q.ml - xz::ml_base(c.ml), xz::ml_bits(c.ml));
-    put(c.ov - (1u << c.of), c.of);
   }
   if (nb) atomicOr(&w[word], (u32)acc);
   __threadfence_block();
@@ -1041,6 +1077,7 @@ __global__ __launch_bounds__(kParseLanes) void k_zparse(const u8* __restrict__ p
       }
       S.nseq[seg] = ns;
       S.nlit[seg] = nl;
+      S.ntail[seg] = e - tail;
     }
   }
   __syncthreads();
@@ -1135,15 +1172,10 @@ __global__ __launch_bounds__(64) void k_zseqs(const ZBlk* __restrict__ blks,
   // 1. per-segment counts, offsets and literal carries; sequences compacted
   //    (a segment's first sequence takes the literals left pending before it)
   if (lane < kSegs) {
-    u32 ns = 0, nl = 0, llsum = 0;
-    if (lane < nseg) {
-      ns = S.nseq[lane];
-      nl = S.nlit[lane];
-      for (u32 k = 0; k < ns; ++k) llsum += S.seqs[lane * kSegSeqs + k].ll;
-    }
-    L.segn[lane] = ns;
-    L.segl[lane] = nl;
-    L.segll[lane] = llsum;
+    const bool in = lane < nseg;
+    L.segn[lane] = in ? S.nseq[lane] : 0u;
+    L.segl[lane] = in ? S.nlit[lane] : 0u;
+    L.segt[lane] = in ? S.ntail[lane] : 0u;
   }
   __syncthreads();
   if (lane == 0) {
@@ -1152,18 +1184,33 @@ __global__ __launch_bounds__(64) void k_zseqs(const ZBlk* __restrict__ blks,
       L.soff[t] = so;
       L.carry[t] = carry;
       so += L.segn[t];
-      carry = L.segn[t] ? L.segl[t] - L.segll[t] : carry + L.segl[t];
+      carry = L.segn[t] ? L.segt[t] : carry + L.segl[t];
     }
     L.soff[nseg] = so;
   }
   __syncthreads();
   const u32 ns = L.soff[nseg];
-  for (u32 t = 0; t < nseg; ++t) {
-    const u32 sn = L.segn[t], so = L.soff[t];
-    for (u32 k = lane; k < sn; k += 64) {
-      Seq q = S.seqs[t * kSegSeqs + k];
-      if (k == 0) q.ll += L.carry[t];
-      S.cseq[so + k] = q;
+  // Four segments at a time, every lane's loads issued before its stores.
+  for (u32 t0 = 0; t0 < nseg; t0 += 4) {
+    Seq v[4][kSegSeqs / 64];
+#pragma unroll
+    for (u32 tt = 0; tt < 4; ++tt)
+#pragma unroll
+      for (u32 j = 0; j < kSegSeqs / 64; ++j)  // segment slots are always readable
+        v[tt][j] = S.seqs[(t0 + tt < kSegs ? t0 + tt : kSegs - 1) * kSegSeqs + j * 64 + lane];
+#pragma unroll
+    for (u32 tt = 0; tt < 4; ++tt) {
+      const u32 t = t0 + tt;
+      const u32 sn = t < nseg ? L.segn[t] : 0u, so = t < nseg ? L.soff[t] : 0u;
+#pragma unroll
+      for (u32 j = 0; j < kSegSeqs / 64; ++j) {
+        const u32 k = j * 64 + lane;
+        if (k < sn) {
+          Seq q = v[tt][j];
+          if (k == 0) q.ll += L.carry[t];
+          S.cseq[so + k] = q;
+        }
+      }
     }
   }
   __threadfence_block();
@@ -1177,14 +1224,25 @@ __global__ __launch_bounds__(64) void k_zseqs(const ZBlk* __restrict__ blks,
     dev_offset_values(S.cseq, ns, L);
     for (u32 i = lane; i < 3 * 64; i += 64) (&L.sw.cnt[0][0])[i] = 0;
     __syncthreads();
-    for (u32 i = lane; i < ns; i += 64) {
-      const SeqCodes c = codes_of(S.cseq[i], L.ov[i]);
-      atomicAdd(&L.sw.cnt[0][c.ll], 1u);
-      atomicAdd(&L.sw.cnt[1][c.of], 1u);
-      atomicAdd(&L.sw.cnt[2][c.ml], 1u);
-      S.sym[0][i] = c.ll;
-      S.sym[1][i] = c.of;
-      S.sym[2][i] = c.ml;
+    for (u32 i0 = 0; i0 < ns; i0 += 64 * kSeqBatch) {
+      Seq q[kSeqBatch];
+#pragma unroll
+      for (u32 b = 0; b < kSeqBatch; ++b) {
+        const u32 i = i0 + b * 64 + lane;
+        q[b] = S.cseq[i < ns ? i : ns - 1];
+      }
+#pragma unroll
+      for (u32 b = 0; b < kSeqBatch; ++b) {
+        const u32 i = i0 + b * 64 + lane;
+        if (i >= ns) break;
+        const SeqCodes c = codes_of(q[b], L.ov[i]);
+        atomicAdd(&L.sw.cnt[0][c.ll], 1u);
+        atomicAdd(&L.sw.cnt[1][c.of], 1u);
+        atomicAdd(&L.sw.cnt[2][c.ml], 1u);
+        S.sym[0][i] = c.ll;
+        S.sym[1][i] = c.of;
+        S.sym[2][i] = c.ml;
+      }
     }
     __threadfence_block();
     __syncthreads();
