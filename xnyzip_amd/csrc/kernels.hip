@@ -1465,6 +1465,130 @@ __global__ void k_seg_sections(Batch bt, const u64* tile_off, const u64* total) 
   }
 }
 
+// The codes of a thread's kItems elements (element o = wstart + q * kWave +
+// lane of the tile), as coder<kKind> gives them, computed in two steps:
+// every load an element needs (keys, counts, block positions) is issued for
+// all items first, with indices clamped into the range, and only then are the
+// codes formed -- one memory latency per step, not one per element.  `tab`
+// is the section's Huffman table, staged in LDS.
+template <u32 kKind>
+__device__ __forceinline__ void tile_codes(const Batch& bt, const SecCtx& x, const SegParam& sp,
+                                           const SegCtl& sc, const Range& r, u32 wstart, u32 l,
+                                           const HufEntry* tab, Code* cd) {
+  const KV& k = x.k;
+  const bool wide = k.wide;
+  const u32 last = r.count - 1;  // r.count >= 1
+  auto elem = [&](u32 q) { return wstart + q * kWave + l; };
+  auto idx = [&](u32 q) { return r.first + (elem(q) < r.count ? elem(q) : last); };
+  if (kKind == kSecBlk || kKind == kSecCnt) {
+    const u32 b0 = sc.b0, bend = sc.b0 + sc.nb, cend = sc.e0 + sc.ne;
+    u32 st[kItems], pv[kItems], nx[kItems];
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      const u32 b = idx(q);
+      st[q] = x.bpos[b];
+      pv[q] = x.bpos[b > b0 ? b - 1 : b];
+      if (kKind == kSecCnt) nx[q] = b + 1 < bend ? x.bpos[b + 1] : cend;
+    }
+    if (kKind == kSecCnt) {
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) {
+        const u32 b = idx(q);
+        const u64 c = (u64)(nx[q] - st[q]);
+        const u64 v = b > b0 ? c - (u64)(st[q] - pv[q]) : c;
+        cd[q] = elem(q) < r.count ? Code{v, sp.w2, 0} : Code{0, 0, 0};
+      }
+      return;
+    }
+    u64 h0[kItems], h1[kItems];
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      h0[q] = k.a[st[q]];
+      h1[q] = k.a[pv[q]];
+    }
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      const u32 b = idx(q);
+      const u64 id = id_of(wide ? h0[q] : h0[q] >> 21, sp.kp);
+      const u64 v = b > b0 ? id - id_of(wide ? h1[q] : h1[q] >> 21, sp.kp) : id;
+      cd[q] = elem(q) < r.count ? Code{v, sp.w1, 0} : Code{0, 0, 0};
+    }
+    return;
+  }
+  if (kKind == kSecRle) {
+    u32 c[kItems];
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) c[q] = x.cnt[idx(q)];
+    const bool eg = sp.cnt_eg != 0;
+    const u32 kk = sp.egk;
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      Code v;
+      if (eg) {  // Exp-Golomb of zigzag(int)
+        const u64 z = (u64)c[q] << 1;
+        const u64 n1 = (z >> kk) + 1;
+        const u32 nb = bitlen64(n1);
+        const u64 low = kk ? (__builtin_bitreverse64(z) >> (64 - kk)) : 0;
+        v = Code{(n1 << kk) | low, nb + kk, nb - 1};
+      } else {
+        const HufEntry e = c[q] < kCntBinsSeg ? tab[c[q]] : big_code(bt, sp, c[q]);
+        v = Code{e.bits, e.len, 0};
+      }
+      cd[q] = elem(q) < r.count ? v : Code{0, 0, 0};
+    }
+    return;
+  }
+  // repos / sfc deltas and quads (lcp), from the keys at j and j - 1
+  const bool direct = sp.kp.direct != 0;
+  const u32 e0 = sc.e0, lim = sp.lim, grk = sp.grk;
+  u64 a0[kItems], a1[kItems];
+  u32 w0[kItems], w1[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const u32 j = idx(q), jp = j > e0 ? j - 1 : j;
+    a0[q] = k.a[j];
+    a1[q] = k.a[jp];
+    if (wide && !direct) {
+      w0[q] = k.b[j];
+      w1[q] = k.b[jp];
+    }
+  }
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const u32 j = idx(q);
+    u64 v;
+    u32 qd = 0;
+    if (direct) {
+      v = j == e0 ? a0[q] : a0[q] - a1[q];
+    } else {  // lcp() on the loaded keys
+      const u64 hc = wide ? a0[q] : a0[q] >> 21, hp = wide ? a1[q] : a1[q] >> 21;
+      const u32 lc = wide ? w0[q] : (u32)(a0[q] & 0x1FFFFFu);
+      const u32 lp = wide ? w1[q] : (u32)(a1[q] & 0x1FFFFFu);
+      const u32 quad = lc >> 18, sfc = lc & 0x3FFFFu;
+      if (j == e0 || hp != hc) {
+        v = sfc;
+        qd = quad;
+      } else {
+        const u32 pq = lp >> 18;
+        qd = (quad - pq) & 0xFFu;
+        v = pq == quad ? (u64)sfc - (u64)(lp & 0x3FFFFu) : (u64)sfc;
+      }
+    }
+    Code c;
+    if (kKind == kSecGR) {  // Golomb-Rice of the patches, LSB-first
+      c = v <= lim ? Code{0, 0, 0}
+                   : Code{(v & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(v >> grk)};
+    } else if (kKind == kSecHuf) {
+      const HufEntry e = tab[v > lim ? 0u : (u32)v + 1u];
+      c = Code{e.bits, e.len, 0};
+    } else {  // kSecQuad
+      const HufEntry e = tab[qd > 254 ? 0u : qd + 1u];
+      c = Code{e.bits, e.len, 0};
+    }
+    cd[q] = elem(q) < r.count ? c : Code{0, 0, 0};
+  }
+}
+
 // Phase B: recompute the tile's codes, scan them inside the tile, assemble
 // the tile's bits in LDS (MSB-first streams in big-endian word order), then
 // store interior words and OR the two edge words, which neighbouring tiles
@@ -1473,29 +1597,48 @@ template <u32 kKind>
 __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const u64* tile_off,
                                                       u32* out) {
   constexpr bool kMsb = SecTraits<kKind>::kMsb;
+  constexpr bool kTab = kKind == kSecHuf || kKind == kSecQuad || kKind == kSecRle;
+  static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
   __shared__ u32 win[kVlWin];
   __shared__ u64 sh[kWavesPerBlock];
+  __shared__ HufEntry tab[kTab ? kBlock : 1];
   const Tile t = bt.tiles[blockIdx.x];
   const SegParam& sp = bt.sp[t.seg];
   const SegCtl& sc = bt.sc[t.seg];
   if (!sec_active<kKind>(sp, sc)) return;
   const Range r = stage_range(sp, sc, t.local, SecTraits<kKind>::kSpace);
-  const SegLut& lut = bt.lut[t.seg];
+  if (r.count == 0) return;
+  if (kTab) {
+    const SegLut& lut = bt.lut[t.seg];
+    tab[threadIdx.x] = kKind == kSecHuf ? lut.sym[threadIdx.x]
+                       : kKind == kSecQuad ? lut.quad[threadIdx.x] : lut.cnt[threadIdx.x];
+    __syncthreads();
+  }
   // Element order inside the tile: wavefront w owns [w*kWaveSub,
   // (w+1)*kWaveSub), taken 64 consecutive elements per step, so every load
-  // is coalesced.  rel[q] = bit offset of the lane's q-th code in its wave.
+  // is coalesced.  rel[q] = bit offset of the lane's q-th code in its wave:
+  // the kItems lane scans advance level by level, all items together.
   const u32 w = threadIdx.x / kWave, l = lane_id();
   const u32 wstart = w * kWaveSub;
   Code cd[kItems];
+  tile_codes<kKind>(bt, x, sp, sc, r, wstart, l, tab, cd);
   u64 rel[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) rel[q] = (u64)cd[q].lead + cd[q].len;
+#pragma unroll
+  for (u32 o = 1; o < kWave; o <<= 1) {
+    u64 up[kItems];
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) up[q] = __shfl_up(rel[q], o, kWave);
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q)
+      if (l >= o) rel[q] += up[q];
+  }
   u64 carry = 0;
 #pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    const u32 o = wstart + q * kWave + l;
-    cd[q] = o < r.count ? coder<kKind>(bt, x, sp, sc, lut, r.first + o) : Code{0, 0, 0};
-    const u64 b = (u64)cd[q].lead + cd[q].len;
-    const u64 inc = wave_incl_scan(b);
-    rel[q] = carry + inc - b;
+  for (u32 q = 0; q < kItems; ++q) {  // inclusive -> exclusive, items in order
+    const u64 inc = rel[q];
+    rel[q] = carry + inc - ((u64)cd[q].lead + cd[q].len);
     carry += __shfl(inc, kWave - 1, kWave);
   }
   if (l == 0) sh[w] = carry;
