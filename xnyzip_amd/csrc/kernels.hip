@@ -984,6 +984,52 @@ __device__ __forceinline__ void lcp(const KV& k, u32 j, u32 e0, u64& repos, u32&
   repos = (pq == quad) ? (u64)sfc - (u64)(pl & 0x3FFFFu) : (u64)sfc;
 }
 
+// lcp() (or the direct-mode delta) of kN elements first + o[q] of a range of
+// `count` elements whose segment starts at e0: the keys at j and j - 1 are
+// loaded for every item first, indices clamped into the range, so the
+// thread waits on memory once; items with o[q] >= count give v = qd = 0.
+template <u32 kN>
+__device__ __forceinline__ void lcp_items(const KV& k, bool direct, u32 e0, u32 first,
+                                          u32 count, const u32* o, u64* v, u32* qd) {
+  const bool wide = k.wide;
+  u64 a0[kN], a1[kN];
+  u32 w0[kN], w1[kN];
+#pragma unroll
+  for (u32 q = 0; q < kN; ++q) {
+    const u32 j = first + (o[q] < count ? o[q] : count - 1), jp = j > e0 ? j - 1 : j;
+    a0[q] = k.a[j];
+    a1[q] = k.a[jp];
+    if (wide && !direct) {
+      w0[q] = k.b[j];
+      w1[q] = k.b[jp];
+    }
+  }
+#pragma unroll
+  for (u32 q = 0; q < kN; ++q) {
+    const u32 j = first + (o[q] < count ? o[q] : count - 1);
+    u64 r;
+    u32 d = 0;
+    if (direct) {
+      r = j == e0 ? a0[q] : a0[q] - a1[q];
+    } else {
+      const u64 hc = wide ? a0[q] : a0[q] >> 21, hp = wide ? a1[q] : a1[q] >> 21;
+      const u32 lc = wide ? w0[q] : (u32)(a0[q] & 0x1FFFFFu);
+      const u32 lp = wide ? w1[q] : (u32)(a1[q] & 0x1FFFFFu);
+      const u32 quad = lc >> 18, sfc = lc & 0x3FFFFu;
+      if (j == e0 || hp != hc) {
+        r = sfc;
+        d = quad;
+      } else {
+        const u32 pq = lp >> 18;
+        d = (quad - pq) & 0xFFu;
+        r = pq == quad ? (u64)sfc - (u64)(lp & 0x3FFFFu) : (u64)sfc;
+      }
+    }
+    v[q] = o[q] < count ? r : 0;
+    qd[q] = o[q] < count ? d : 0;
+  }
+}
+
 enum FlagKind { kFlagCell = 0, kFlagBlock = 1 };
 
 // In a batch mixing RLE and plain segments, the plain segments' points all
@@ -1222,17 +1268,14 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
     u64 bp = 0;  // this lane's share of bitpop: lane b counts bit b
     // Loads first (all items in flight), then the LDS updates.
     u64 v[kItems];
-    u32 qd[kItems];
+    u32 qd[kItems], oo[kItems];
 #pragma unroll
-    for (u32 it = 0; it < kItems; ++it) {
-      const u32 o = it * kBlock + threadIdx.x;
-      v[it] = 0;
-      qd[it] = 0;
-      if (o < r.count) {
-        const u32 j = r.first + o;
-        if (direct) v[it] = (j == sc.e0) ? k.a[j] : k.a[j] - k.a[j - 1];
-        else lcp(k, j, sc.e0, v[it], qd[it]);
-      }
+    for (u32 it = 0; it < kItems; ++it) oo[it] = it * kBlock + threadIdx.x;
+    if (r.count) {
+      lcp_items<kItems>(k, direct, sc.e0, r.first, r.count, oo, v, qd);
+    } else {
+#pragma unroll
+      for (u32 it = 0; it < kItems; ++it) v[it] = qd[it] = 0;
     }
 #pragma unroll
     for (u32 it = 0; it < kItems; ++it) {
@@ -1406,34 +1449,46 @@ constexpr u32 kVlWin = 8448;  // LDS words: 4096 codes of 64 bits plus the edges
 // section's in-segment offsets are differences of scan values, and the
 // constant base each section inherits from the ones before it cancels.
 __global__ __launch_bounds__(kBlock) void k_sec_bits(Batch bt, SecCtx x, u64* tile_bits) {
+  static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
   __shared__ u64 sh[kWavesPerBlock];
+  __shared__ u32 len_sym[kBlock], len_quad[kBlock], len_cnt[kBlock];  // code lengths
   const Tile t = bt.tiles[blockIdx.x];
   const SegParam& sp = bt.sp[t.seg];
   const SegCtl& sc = bt.sc[t.seg];
   const SegLut& lut = bt.lut[t.seg];
   const KV& k = x.k;
-  const bool direct = sp.kp.direct != 0, rle = sp.rle != 0;
-  const u32 lim = sp.lim, grk = sp.grk;
+  const bool direct = sp.kp.direct != 0, rle = sp.rle != 0, eg = sp.cnt_eg != 0;
+  const u32 lim = sp.lim, grk = sp.grk, egk = sp.egk;
+  len_sym[threadIdx.x] = lut.sym[threadIdx.x].len;
+  len_quad[threadIdx.x] = lut.quad[threadIdx.x].len;
+  len_cnt[threadIdx.x] = lut.cnt[threadIdx.x].len;
+  __syncthreads();
   u64 sg = 0, sh_ = 0, sq = 0, sr = 0;
   const Range r = tile_range(sc.e0, sc.ne, t.local);
-#pragma unroll 4
-  for (u32 it = 0; it < kItems; ++it) {
-    const u32 o = it * kBlock + threadIdx.x;
-    if (o < r.count) {
-      const u32 j = r.first + o;
-      u64 v;
-      u32 qd = 0;
-      if (direct) {
-        v = (j == sc.e0) ? k.a[j] : k.a[j] - k.a[j - 1];
-      } else {
-        lcp(k, j, sc.e0, v, qd);
-        sq += lut.quad[qd > 254 ? 0u : qd + 1u].len;
-      }
-      sh_ += lut.sym[v > lim ? 0u : (u32)v + 1u].len;
-      if (v > lim) sg += (v >> grk) + grk + 1;
-      if (rle) {
-        const Code c = coder<kSecRle>(bt, x, sp, sc, lut, j);
-        sr += (u64)c.lead + c.len;
+  if (r.count) {
+    // every element's inputs loaded before any is used (see lcp_items)
+    u64 v[kItems];
+    u32 qd[kItems], oo[kItems], c[kItems];
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) oo[it] = it * kBlock + threadIdx.x;
+    if (rle) {
+#pragma unroll
+      for (u32 it = 0; it < kItems; ++it) c[it] = x.cnt[r.first + (oo[it] < r.count ? oo[it] : r.count - 1)];
+    }
+    lcp_items<kItems>(k, direct, sc.e0, r.first, r.count, oo, v, qd);
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) {
+      if (oo[it] >= r.count) continue;
+      if (!direct) sq += len_quad[qd[it] > 254 ? 0u : qd[it] + 1u];
+      sh_ += len_sym[v[it] > lim ? 0u : (u32)v[it] + 1u];
+      if (v[it] > lim) sg += (v[it] >> grk) + grk + 1;
+      if (rle) {  // coder<kSecRle>'s lead + len
+        if (eg) {
+          const u32 nb = bitlen64((((u64)c[it] << 1) >> egk) + 1);
+          sr += (u64)(nb - 1) + nb + egk;
+        } else {
+          sr += c[it] < kCntBinsSeg ? len_cnt[c[it]] : big_code(bt, sp, c[it]).len;
+        }
       }
     }
   }
@@ -1539,50 +1594,23 @@ __device__ __forceinline__ void tile_codes(const Batch& bt, const SecCtx& x, con
     return;
   }
   // repos / sfc deltas and quads (lcp), from the keys at j and j - 1
-  const bool direct = sp.kp.direct != 0;
-  const u32 e0 = sc.e0, lim = sp.lim, grk = sp.grk;
-  u64 a0[kItems], a1[kItems];
-  u32 w0[kItems], w1[kItems];
+  const u32 lim = sp.lim, grk = sp.grk;
+  u32 o[kItems], qd[kItems];
+  u64 v[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) o[q] = elem(q);
+  lcp_items<kItems>(k, sp.kp.direct != 0, sc.e0, r.first, r.count, o, v, qd);
 #pragma unroll
   for (u32 q = 0; q < kItems; ++q) {
-    const u32 j = idx(q), jp = j > e0 ? j - 1 : j;
-    a0[q] = k.a[j];
-    a1[q] = k.a[jp];
-    if (wide && !direct) {
-      w0[q] = k.b[j];
-      w1[q] = k.b[jp];
-    }
-  }
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    const u32 j = idx(q);
-    u64 v;
-    u32 qd = 0;
-    if (direct) {
-      v = j == e0 ? a0[q] : a0[q] - a1[q];
-    } else {  // lcp() on the loaded keys
-      const u64 hc = wide ? a0[q] : a0[q] >> 21, hp = wide ? a1[q] : a1[q] >> 21;
-      const u32 lc = wide ? w0[q] : (u32)(a0[q] & 0x1FFFFFu);
-      const u32 lp = wide ? w1[q] : (u32)(a1[q] & 0x1FFFFFu);
-      const u32 quad = lc >> 18, sfc = lc & 0x3FFFFu;
-      if (j == e0 || hp != hc) {
-        v = sfc;
-        qd = quad;
-      } else {
-        const u32 pq = lp >> 18;
-        qd = (quad - pq) & 0xFFu;
-        v = pq == quad ? (u64)sfc - (u64)(lp & 0x3FFFFu) : (u64)sfc;
-      }
-    }
     Code c;
     if (kKind == kSecGR) {  // Golomb-Rice of the patches, LSB-first
-      c = v <= lim ? Code{0, 0, 0}
-                   : Code{(v & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(v >> grk)};
+      c = v[q] <= lim ? Code{0, 0, 0}
+                      : Code{(v[q] & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(v[q] >> grk)};
     } else if (kKind == kSecHuf) {
-      const HufEntry e = tab[v > lim ? 0u : (u32)v + 1u];
+      const HufEntry e = tab[v[q] > lim ? 0u : (u32)v[q] + 1u];
       c = Code{e.bits, e.len, 0};
     } else {  // kSecQuad
-      const HufEntry e = tab[qd > 254 ? 0u : qd + 1u];
+      const HufEntry e = tab[qd[q] > 254 ? 0u : qd[q] + 1u];
       c = Code{e.bits, e.len, 0};
     }
     cd[q] = elem(q) < r.count ? c : Code{0, 0, 0};
