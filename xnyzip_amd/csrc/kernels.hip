@@ -1041,6 +1041,40 @@ __device__ __forceinline__ bool run_flag(const KV& k, u32 j, u32 first, bool all
   return k.hi(j) != k.hi(j - 1);
 }
 
+// run_flag() of kN elements r.first + o[q] (false past the range), the keys
+// at j and j - 1 loaded for every item before any is compared.
+template <int kKind, u32 kN>
+__device__ __forceinline__ void run_flags(const KV& k, const Range& r, u32 first, bool all,
+                                          const u32* o, bool* f) {
+  if (r.count == 0 || all) {
+#pragma unroll
+    for (u32 q = 0; q < kN; ++q) f[q] = o[q] < r.count;
+    return;
+  }
+  const bool wide = k.wide;
+  u64 a0[kN], a1[kN];
+  u32 w0[kN], w1[kN];
+#pragma unroll
+  for (u32 q = 0; q < kN; ++q) {
+    const u32 j = r.first + (o[q] < r.count ? o[q] : r.count - 1), jp = j > first ? j - 1 : j;
+    a0[q] = k.a[j];
+    a1[q] = k.a[jp];
+    if (kKind == kFlagCell && wide) {
+      w0[q] = k.b[j];
+      w1[q] = k.b[jp];
+    }
+  }
+#pragma unroll
+  for (u32 q = 0; q < kN; ++q) {
+    const u32 j = r.first + (o[q] < r.count ? o[q] : r.count - 1);
+    bool fl;
+    if (j == first) fl = true;
+    else if (kKind == kFlagCell) fl = !(a0[q] == a1[q] && (!wide || w0[q] == w1[q]));
+    else fl = (wide ? a0[q] : a0[q] >> 21) != (wide ? a1[q] : a1[q] >> 21);
+    f[q] = o[q] < r.count && fl;
+  }
+}
+
 template <int kKind>
 __device__ __forceinline__ Range flag_range(const Batch& bt, const Tile& t) {
   const SegParam& sp = bt.sp[t.seg];
@@ -1057,11 +1091,13 @@ __global__ __launch_bounds__(kBlock) void k_flag_count(Batch bt, KV k, u32* part
   const Range r = flag_range<kKind>(bt, t);
   const u32 first = r.count ? r.first - (u32)((u64)t.local * kTile) : 0;
   const bool all = kKind == kFlagCell && !bt.sp[t.seg].rle;
-  u32 c = 0;
-  for (u32 it = 0; it < kItems; ++it) {
-    const u32 o = it * kBlock + threadIdx.x;
-    if (o < r.count) c += run_flag<kKind>(k, r.first + o, first, all) ? 1u : 0u;
-  }
+  u32 o[kItems], c = 0;
+  bool f[kItems];
+#pragma unroll
+  for (u32 it = 0; it < kItems; ++it) o[it] = it * kBlock + threadIdx.x;
+  run_flags<kKind, kItems>(k, r, first, all, o, f);
+#pragma unroll
+  for (u32 it = 0; it < kItems; ++it) c += f[it] ? 1u : 0u;
   const u32 tot = block_sum(c, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
@@ -1080,12 +1116,14 @@ __global__ __launch_bounds__(kBlock) void k_flag_scatter(Batch bt, KV k, const u
   const u64 lt = (1ull << l) - 1;
   const u32 wstart = w * kWaveSub;
   u64 m[kItems];
-  u32 c = 0;
+  u32 c = 0, o[kItems];
+  bool f[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) o[q] = wstart + q * kWave + l;
+  run_flags<kKind, kItems>(k, r, first, all, o, f);
 #pragma unroll
   for (u32 q = 0; q < kItems; ++q) {
-    const u32 o = wstart + q * kWave + l;
-    const bool fl = o < r.count && run_flag<kKind>(k, r.first + o, first, all);
-    m[q] = __ballot(fl);
+    m[q] = __ballot(f[q]);
     c += (u32)__popcll(m[q]);
   }
   if (l == 0) sh[w] = c;
@@ -1175,16 +1213,31 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
     const SegCtl& sc = bt.sc[t.seg];
     const Range r = tile_range(sc.e0, sc.ne, t.local);
     const u32 end_cells = sc.e0 + sc.ne, end_pts = sp.p0 + sp.n;
-#pragma unroll 4
+    if (!r.count) continue;
+    // run starts of every item, then their keys, loaded before any is used
+    u32 st[kItems], en[kItems], kb[kItems];
+    u64 ka[kItems];
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) {
+      const u32 o = it * kBlock + threadIdx.x;
+      const u32 u = r.first + (o < r.count ? o : r.count - 1);
+      st[it] = upos[u];
+      const u32 nx = upos[u + 1 < end_cells ? u + 1 : u];
+      en[it] = u + 1 < end_cells ? nx : end_pts;
+    }
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) {
+      ka[it] = in.a[st[it]];
+      if (in.wide) kb[it] = in.b[st[it]];
+    }
+#pragma unroll
     for (u32 it = 0; it < kItems; ++it) {
       const u32 o = it * kBlock + threadIdx.x;
       if (o < r.count) {
         const u32 u = r.first + o;
-        const u32 st = upos[u];
-        const u32 en = (u + 1 < end_cells) ? upos[u + 1] : end_pts;
-        const u32 c = en - st;
-        ua[u] = in.a[st];
-        if (in.wide) ub[u] = in.b[st];
+        const u32 c = en[it] - st[it];
+        ua[u] = ka[it];
+        if (in.wide) ub[u] = kb[it];
         cnt[u] = c;
         const u32 rel = u - sc.e0;
         if (c < kCntBinsSeg) {
