@@ -1334,8 +1334,14 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
                                        " bytes needed");
         dst = co.dst;
       }
+      // the device-built blocks come back (behind the gather) while the
+      // host's blocks are copied into place
+      ctx->h_zout.ensure(head_bytes + 64);
+      zb = ctx->h_zout.as<uint8_t>();
+      check_hip(hipMemcpyAsync(zb, ctx->zout.p, head_bytes, hipMemcpyDeviceToHost, st), "frames D2H");
+      ctx->copied();
       copy_pieces(outs[0], dst + head_bytes, threads);
-      fetch_dev_frames();
+      check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
       const uint64_t fcs = g[0].total;  // the header covers the whole stream
       std::memcpy(zb + 5, &fcs, 8);
       zb[zpos[znblk - 1] + 13] &= 0xFE;  // the host's blocks follow the last one
