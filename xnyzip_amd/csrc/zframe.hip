@@ -1111,10 +1111,14 @@ __global__ __launch_bounds__(64) void k_zlits(const ZBlk* __restrict__ blks,
   for (u32 t0 = 0; t0 < nseg; t0 += 4) {
     u32 v[4][kSeg / 64];
 #pragma unroll
-    for (u32 tt = 0; tt < 4; ++tt)
+    for (u32 tt = 0; tt < 4; ++tt) {
+      // indices clamped to the segment's literals (slots are always readable)
+      const u32 t = t0 + tt < kSegs ? t0 + tt : kSegs - 1;
+      const u32 ln = t0 + tt < nseg ? L.segl[t] : 0u, top = ln ? ln - 1 : 0u;
 #pragma unroll
-      for (u32 j = 0; j < kSeg / 64; ++j)  // segment slots are always readable
-        v[tt][j] = S.lits[(t0 + tt < kSegs ? t0 + tt : kSegs - 1) * kSeg + j * 64 + lane];
+      for (u32 j = 0; j < kSeg / 64; ++j)
+        v[tt][j] = S.lits[t * kSeg + (j * 64 + lane < top ? j * 64 + lane : top)];
+    }
 #pragma unroll
     for (u32 tt = 0; tt < 4; ++tt) {
       const u32 t = t0 + tt;
@@ -1194,10 +1198,14 @@ __global__ __launch_bounds__(64) void k_zseqs(const ZBlk* __restrict__ blks,
   for (u32 t0 = 0; t0 < nseg; t0 += 4) {
     Seq v[4][kSegSeqs / 64];
 #pragma unroll
-    for (u32 tt = 0; tt < 4; ++tt)
+    for (u32 tt = 0; tt < 4; ++tt) {
+      // indices clamped to the segment's sequences (slots are always readable)
+      const u32 t = t0 + tt < kSegs ? t0 + tt : kSegs - 1;
+      const u32 sn = t0 + tt < nseg ? L.segn[t] : 0u, top = sn ? sn - 1 : 0u;
 #pragma unroll
-      for (u32 j = 0; j < kSegSeqs / 64; ++j)  // segment slots are always readable
-        v[tt][j] = S.seqs[(t0 + tt < kSegs ? t0 + tt : kSegs - 1) * kSegSeqs + j * 64 + lane];
+      for (u32 j = 0; j < kSegSeqs / 64; ++j)
+        v[tt][j] = S.seqs[t * kSegSeqs + (j * 64 + lane < top ? j * 64 + lane : top)];
+    }
 #pragma unroll
     for (u32 tt = 0; tt < 4; ++tt) {
       const u32 t = t0 + tt;
