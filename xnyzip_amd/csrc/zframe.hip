@@ -65,6 +65,7 @@ struct alignas(16) BlkScratch {  // per block, in global memory
   u32 hist[256];
   u32 nseq[kSegs], nlit[kSegs];
   u32 ntail[kSegs];  // literals after a segment's last sequence (k_zparse)
+  u16 ov[kBlock / kMinMatch];  // offset values (offset + 3 or a repeat code), < 2^15 (k_zseqs)
 };
 
 constexpr u32 kMaxSeqs = kBlock / kMinMatch;
@@ -97,7 +98,6 @@ struct FseLds {  // FSE table builds
 struct SeqLds {  // k_zseqs
   xz::SeqWork sw;
   FseLds F;
-  u16 ov[kMaxSeqs];  // offset values (offset + 3 or a repeat code), < 2^15 in a 16 KiB block
   u32 segn[kSegs], segl[kSegs], segt[kSegs], carry[kSegs], soff[kSegs + 1];
   u32 fin[3];  // FSE states after the first sequence (flushed at the end)
 };
@@ -111,9 +111,10 @@ __device__ __forceinline__ Seq shfl_seq(const Seq& q, u32 j) {
 
 // Offset_Values of a block's sequences in order (repeat codes against the
 // running three-offset history; RFC 8878 3.1.1.5), as xz::seq_codes
-// computes them, into L.ov: LDS entries (offset | 0x8000 when the literal
-// length is 0) rewritten in place.
-__device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
+// computes them, into ov (block scratch; kept out of LDS so more blocks
+// share a CU): entries (offset | 0x8000 when the literal length is 0)
+// rewritten in place.
+__device__ void dev_offset_values(const Seq* cs, u32 ns, u16* ov) {
   const u32 lane = threadIdx.x;
   for (u32 i0 = 0; i0 < ns; i0 += 64 * kSeqBatch) {
     Seq q[kSeqBatch];
@@ -125,9 +126,10 @@ __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
 #pragma unroll
     for (u32 b = 0; b < kSeqBatch; ++b) {
       const u32 i = i0 + b * 64 + lane;
-      if (i < ns) L.ov[i] = (u16)(q[b].off | (q[b].ll ? 0u : 0x8000u));
+      if (i < ns) ov[i] = (u16)(q[b].off | (q[b].ll ? 0u : 0x8000u));
     }
   }
+  __threadfence_block();
   __syncthreads();
   // Every step puts its offset in slot 0 of the history; a step that repeats
   // slot 0 after literals ("stay") leaves the history as it is, and every
@@ -142,7 +144,7 @@ __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
   u32 r2 = 0, known = 0;      // the serial walk's state
   for (u32 i0 = 0; i0 < ns; i0 += 64) {
     const u32 g = i0 + lane;
-    const u32 x = g < ns ? (u32)L.ov[g] : 0u;
+    const u32 x = g < ns ? (u32)ov[g] : 0u;
     const u32 o = x & 0x7FFFu;
     const bool z = (x & 0x8000u) != 0;  // literal length 0
     u32 po = (u32)__shfl_up((int)o, 1, 64);
@@ -173,11 +175,12 @@ __device__ void dev_offset_values(const Seq* cs, u32 ns, SeqLds& L) {
       }
     }
     const bool e1 = flags & 1u, e2 = flags & 2u;
-    u32 ov;
-    if (!z) ov = stay ? 1u : e1 ? 2u : e2 ? 3u : o + 3;
-    else ov = e1 ? 1u : e2 ? 2u : em ? 3u : o + 3;
-    if (g < ns) L.ov[g] = (u16)ov;
+    u32 v;
+    if (!z) v = stay ? 1u : e1 ? 2u : e2 ? 3u : o + 3;
+    else v = e1 ? 1u : e2 ? 2u : em ? 3u : o + 3;
+    if (g < ns) ov[g] = (u16)v;
   }
+  __threadfence_block();
   __syncthreads();
 }
 
@@ -510,6 +513,7 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
   const u32 r1 = r0 + per < ns ? r0 + per : ns;
   struct SeqRec {
     Seq q;
+    u32 ov;
     u32 f[3];  // FSE records (bits << 10 | value) of LL, OF, ML; unused for the last sequence
   };
   auto load = [&](u32 rb, SeqRec* v) {
@@ -517,6 +521,7 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
     for (u32 b = 0; b < kSeqBatch; ++b) {
       const u32 i = ns - 1 - (rb + b < r1 ? rb + b : r1 - 1);
       v[b].q = S.cseq[i];
+      v[b].ov = S.ov[i];
 #pragma unroll
       for (u32 k = 0; k < 3; ++k) v[b].f[k] = S.fse[k][i];
     }
@@ -530,7 +535,7 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
       if (rb + b >= r1) break;
       const u32 i = ns - 1 - (rb + b);
       const Seq& q = v[b].q;
-      bits += xz::ll_bits(xz::ll_code(q.ll)) + xz::ml_bits(xz::ml_code(q.ml)) + xz::highbit(L.ov[i]);
+      bits += xz::ll_bits(xz::ll_code(q.ll)) + xz::ml_bits(xz::ml_code(q.ml)) + xz::highbit(v[b].ov);
       if (i + 1 < ns) bits += (v[b].f[0] >> 10) + (v[b].f[1] >> 10) + (v[b].f[2] >> 10);
     }
   }
@@ -567,7 +572,7 @@ __device__ u32 dev_seq_bits(u8* slot, u32 base, BlkScratch& S, u32 ns, SeqLds& L
       if (rb + b >= r1) break;
       const u32 i = ns - 1 - (rb + b);
       const Seq& q = v[b].q;
-      const SeqCodes c = codes_of(q, L.ov[i]);
+      const SeqCodes c = codes_of(q, v[b].ov);
       if (i + 1 < ns) {
         const u32 fo = v[b].f[1], fm = v[b].f[2], fl = v[b].f[0];
         put(fo & 1023u, fo >> 10);
@@ -1229,21 +1234,23 @@ __global__ __launch_bounds__(64) void k_zseqs(const ZBlk* __restrict__ blks,
   if (lane == 0) ss = xz::seq_count_header(sq, ns);
   ss = __shfl((int)ss, 0, 64);
   if (ns) {
-    dev_offset_values(S.cseq, ns, L);
+    dev_offset_values(S.cseq, ns, S.ov);
     for (u32 i = lane; i < 3 * 64; i += 64) (&L.sw.cnt[0][0])[i] = 0;
     __syncthreads();
     for (u32 i0 = 0; i0 < ns; i0 += 64 * kSeqBatch) {
       Seq q[kSeqBatch];
+      u32 ovb[kSeqBatch];
 #pragma unroll
       for (u32 b = 0; b < kSeqBatch; ++b) {
         const u32 i = i0 + b * 64 + lane;
         q[b] = S.cseq[i < ns ? i : ns - 1];
+        ovb[b] = S.ov[i < ns ? i : ns - 1];
       }
 #pragma unroll
       for (u32 b = 0; b < kSeqBatch; ++b) {
         const u32 i = i0 + b * 64 + lane;
         if (i >= ns) break;
-        const SeqCodes c = codes_of(q[b], L.ov[i]);
+        const SeqCodes c = codes_of(q[b], ovb[b]);
         atomicAdd(&L.sw.cnt[0][c.ll], 1u);
         atomicAdd(&L.sw.cnt[1][c.of], 1u);
         atomicAdd(&L.sw.cnt[2][c.ml], 1u);
