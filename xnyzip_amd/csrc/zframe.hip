@@ -1,11 +1,11 @@
 // zframe.hip -- zstd-format frames built on the device from the assembled
 // streams in HBM (format and parse rules in zframe.hpp).
 //
-//   k_zcand    one workgroup per 16 KiB block: the block and its two hash
-//              tables in LDS (40 KiB, four workgroups per CU: the tables are
+//   k_zcand    one 512-thread workgroup per 16 KiB block: the block and its
+//              two hash tables in LDS (40 KiB, four workgroups per CU: the tables are
 //              built one after the other in one 32-bit area and narrowed to
 //              16 bits, and byte-equality masks for the offsets 1, 8 and 16
-//              take the rest); all 256 threads evaluate every position's best
+//              take the rest); all threads evaluate every position's best
 //              fixed candidate and write it, resolved to (capped length,
 //              offset), to global scratch.
 //   k_zparse   one 32-lane workgroup per block, one lane per 512-byte
@@ -878,7 +878,9 @@ __device__ __forceinline__ u32 eq_run(const u32* eq, u32 p, u32 cap) {
   return l < cap ? l : cap;
 }
 
-__global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
+constexpr u32 kCandThreads = 512;  // k_zcand workgroup: eight wavefronts share a block
+
+__global__ __launch_bounds__(kCandThreads) void k_zcand(const u8* __restrict__ pre,
                                                const ZBlk* __restrict__ blks,
                                                BlkScratch* __restrict__ scr) {
   // LDS (40 KiB: four workgroups per CU): the block, the 4-byte hash table
@@ -897,13 +899,13 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   const ZBlk bk = blks[blockIdx.x];
   u32* cand = scr[blockIdx.x].cand;
   const u32 n = bk.n, tid = threadIdx.x, lane = tid & (kWave - 1);
-  load_block<256>(data, pre + bk.src, n, tid);
-  for (u32 i = tid; i < kH; i += 256) bld[i] = 0xFFFFFFFFu;
+  load_block<kCandThreads>(data, pre + bk.src, n, tid);
+  for (u32 i = tid; i < kH; i += kCandThreads) bld[i] = 0xFFFFFFFFu;
   __syncthreads();
   // Runs of equal hashes on consecutive lanes (runs of equal bytes) leave
   // the update to the run's first lane, whose position is the smallest:
   // atomics on one address from a whole wavefront would serialise.
-  for (u32 p0 = 0; p0 < n; p0 += 256) {
+  for (u32 p0 = 0; p0 < n; p0 += kCandThreads) {
     const u32 p = p0 + tid;
     const bool ok = p + 4 <= n;
     const u32 h = ok ? xz::hash4(LdsReader::r32(data, p)) : 0xFFFFFFFFu;
@@ -912,13 +914,13 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   }
   __syncthreads();
 #pragma unroll
-  for (u32 k = 0; k < kH / 256; ++k) {
-    const u32 t = bld[tid + 256 * k];
-    h4t[tid + 256 * k] = (u16)(t < 0xFFFFu ? t : 0xFFFFu);
-    bld[tid + 256 * k] = 0xFFFFFFFFu;
+  for (u32 k = 0; k < kH / kCandThreads; ++k) {
+    const u32 t = bld[tid + kCandThreads * k];
+    h4t[tid + kCandThreads * k] = (u16)(t < 0xFFFFu ? t : 0xFFFFu);
+    bld[tid + kCandThreads * k] = 0xFFFFFFFFu;
   }
   __syncthreads();
-  for (u32 p0 = 0; p0 < n; p0 += 256) {
+  for (u32 p0 = 0; p0 < n; p0 += kCandThreads) {
     const u32 p = p0 + tid;
     const bool ok = p + 8 <= n;
     const u32 h = ok ? xz::hash8((u64)LdsReader::r32(data, p) | ((u64)LdsReader::r32(data, p + 4) << 32))
@@ -928,19 +930,19 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   }
   __syncthreads();
   {
-    u32 t[kH / 256];
+    u32 t[kH / kCandThreads];
 #pragma unroll
-    for (u32 k = 0; k < kH / 256; ++k) t[k] = bld[tid + 256 * k];
+    for (u32 k = 0; k < kH / kCandThreads; ++k) t[k] = bld[tid + kCandThreads * k];
     __syncthreads();
     u16* h16 = reinterpret_cast<u16*>(bld);
 #pragma unroll
-    for (u32 k = 0; k < kH / 256; ++k) h16[tid + 256 * k] = (u16)(t[k] < 0xFFFFu ? t[k] : 0xFFFFu);
+    for (u32 k = 0; k < kH / kCandThreads; ++k) h16[tid + kCandThreads * k] = (u16)(t[k] < 0xFFFFu ? t[k] : 0xFFFFu);
   }
   u32* eq = bld + kH / 2;  // [3][kEqWords]
   {
     const u32* dw = reinterpret_cast<const u32*>(data);
     const u32 rot = (tid >> 3) & 7u;  // conflict-free LDS banks across the wavefront
-    for (u32 w = tid; w < kEqWords; w += 256) {
+    for (u32 w = tid; w < kEqWords; w += kCandThreads) {
       u32 m1 = 0, m8 = 0, m16 = 0;
       if (32 * w < n) {
 #pragma unroll
@@ -964,7 +966,7 @@ __global__ __launch_bounds__(256) void k_zcand(const u8* __restrict__ pre,
   // xz::best_fixed: offsets 1, 8, 16 from the masks, then the hash hits
   const u16* h4 = h4t;
   const u16* h8 = reinterpret_cast<const u16*>(bld);
-  for (u32 p = tid; p < n; p += 256) {
+  for (u32 p = tid; p < n; p += kCandThreads) {
     const u32 e = xz::seg_end(p, n);
     u32 c = 0;
     if (p + kMinMatch <= e) {
@@ -1321,7 +1323,7 @@ void zframe_encode(const u8* pre, const ZBlk* blks, u32 nblk, const ZFrame* fram
   auto* scr = reinterpret_cast<BlkScratch*>(scratch);
   u8* slots = reinterpret_cast<u8*>(scr + nblk);
   u64* slot_len = reinterpret_cast<u64*>(slots + (size_t)nblk * kSlot);
-  k_zcand<<<nblk, 256, 0, s>>>(pre, blks, scr);
+  k_zcand<<<nblk, kCandThreads, 0, s>>>(pre, blks, scr);
   check_hip(hipGetLastError(), "k_zcand");
   if (g_profiler) g_profiler->mark("k_zcand", s);
   k_zparse<<<(nblk + kParseBlocks - 1) / kParseBlocks, kParseLanes, 0, s>>>(pre, blks, nblk, scr);
