@@ -351,6 +351,20 @@ __device__ __forceinline__ void flush_qstat(SegCtl& c, QStat& q, bool diag, void
   }
 }
 
+// The points of kGroup of a thread's elements (element k * kBlock + thread
+// of the range, k = k0 .. k0 + kGroup - 1), loaded together before any is
+// used, indices clamped into the range (r.count >= 1): the thread waits on
+// memory once per group.
+constexpr u32 kGroup = 8;
+__device__ __forceinline__ void load_points(const F3* __restrict__ p, const Range& r, u32 k0,
+                                            F3* v) {
+#pragma unroll
+  for (u32 k = 0; k < kGroup; ++k) {
+    const u32 o = (k0 + k) * kBlock + threadIdx.x;
+    v[k] = p[r.first + (o < r.count ? o : r.count - 1)];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_quant_stats(Batch bt, const F3* __restrict__ p) {
   __shared__ double sh[kWavesPerBlock];
   u32 t0, t1;
@@ -370,26 +384,31 @@ __global__ __launch_bounds__(kBlock) void k_quant_stats(Batch bt, const F3* __re
     const QuantParams qp = sp.qp;
     const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
     const bool diag = sp.diag != 0;
-#pragma unroll 4
-    for (u32 k = 0; k < kItems; ++k) {
-      const u32 o = k * kBlock + threadIdx.x;
-      if (o < r.count) {
-        float sx, sy, sz;
-        int X, Y, Z;
-        quant_point(p[r.first + o], off, qp, sx, sy, sz, X, Y, Z);
-        q.m[0] = X > q.m[0] ? X : q.m[0];
-        q.m[1] = Y > q.m[1] ? Y : q.m[1];
-        q.m[2] = Z > q.m[2] ? Z : q.m[2];
-        q.wr += (Z == -1);
-        if (diag) {
-          float rx, ry, rz;
-          recover_node(X, Y, Z, qp, rx, ry, rz);
-          const float a = sx - rx, b = sy - ry, c = sz - rz;
-          // block_compressor_rle.hpp:60-62: float products, left-to-right sum
-          const float m = __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
-          const double md = (double)m;
-          q.se += md;
-          q.me = md > q.me ? md : q.me;
+    if (!r.count) continue;
+    for (u32 k0 = 0; k0 < kItems; k0 += kGroup) {
+      F3 pt[kGroup];
+      load_points(p, r, k0, pt);
+#pragma unroll
+      for (u32 kk = 0; kk < kGroup; ++kk) {
+        const u32 o = (k0 + kk) * kBlock + threadIdx.x;
+        if (o < r.count) {
+          float sx, sy, sz;
+          int X, Y, Z;
+          quant_point(pt[kk], off, qp, sx, sy, sz, X, Y, Z);
+          q.m[0] = X > q.m[0] ? X : q.m[0];
+          q.m[1] = Y > q.m[1] ? Y : q.m[1];
+          q.m[2] = Z > q.m[2] ? Z : q.m[2];
+          q.wr += (Z == -1);
+          if (diag) {
+            float rx, ry, rz;
+            recover_node(X, Y, Z, qp, rx, ry, rz);
+            const float a = sx - rx, b = sy - ry, c = sz - rz;
+            // block_compressor_rle.hpp:60-62: float products, left-to-right sum
+            const float m = __fadd_rn(__fadd_rn(__fmul_rn(a, a), __fmul_rn(b, b)), __fmul_rn(c, c));
+            const double md = (double)m;
+            q.se += md;
+            q.me = md > q.me ? md : q.me;
+          }
         }
       }
     }
@@ -507,46 +526,51 @@ __global__ __launch_bounds__(kBlock) void k_keygen(Batch bt, const F3* __restric
   const Range r = tile_range(sp.p0, sp.n, tl.local);
   const KeyParams kp = sp.kp;
   const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
-#pragma unroll 4
-  for (u32 k = 0; k < kItems; ++k) {
-    const u32 o = k * kBlock + threadIdx.x;
-    if (o < r.count) {
-      const u32 i = r.first + o;
-      float sx, sy, sz;
-      int X, Y, Z;
-      quant_point(p[i], off, sp.qp, sx, sy, sz, X, Y, Z);
-      if (idx) idx[i] = i;  // point index payload (permutation output)
-      u64 hi;
-      u32 k21;
-      if (kp.direct) {
-        const u64 m = split3_21((u32)X) | (split3_21((u32)Y) << 1) | (split3_21((u32)Z) << 2);
-        const u64 sfc = kp.hilbert ? hilbert_general(m, kp.L, t, hs.identity) : m;
-        if (!bt.wide) {
-          key[i] = sfc;
-          continue;
+  if (!r.count) return;
+  for (u32 k0 = 0; k0 < kItems; k0 += kGroup) {
+    F3 pt[kGroup];
+    load_points(p, r, k0, pt);
+#pragma unroll
+    for (u32 kk = 0; kk < kGroup; ++kk) {
+      const u32 o = (k0 + kk) * kBlock + threadIdx.x;
+      if (o < r.count) {
+        const u32 i = r.first + o;
+        float sx, sy, sz;
+        int X, Y, Z;
+        quant_point(pt[kk], off, sp.qp, sx, sy, sz, X, Y, Z);
+        if (idx) idx[i] = i;  // point index payload (permutation output)
+        u64 hi;
+        u32 k21;
+        if (kp.direct) {
+          const u64 m = split3_21((u32)X) | (split3_21((u32)Y) << 1) | (split3_21((u32)Z) << 2);
+          const u64 sfc = kp.hilbert ? hilbert_general(m, kp.L, t, hs.identity) : m;
+          if (!bt.wide) {
+            key[i] = sfc;
+            continue;
+          }
+          hi = sfc;
+          k21 = 0;
+        } else {
+          const u64 ux = (u64)(i64)X, uy = (u64)(i64)Y, uz = (u64)(i64)Z;
+          const u64 cx = ux >> 6, cy = uy >> 6, cz = uz >> 6;
+          const u32 m = split3_6((u32)(ux & 63)) | (split3_6((u32)(uy & 63)) << 1) |
+                        (split3_6((u32)(uz & 63)) << 2);
+          const u32 sfc = kp.hilbert ? hilbert18(m, t, hs.block_init) : m;
+          const u32 quad = (u32)((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2));
+          const u64 base = (cx >> 1) + (cy >> 1) * kp.nx;
+          const bool seg = (Z == -1);
+          const u64 v = seg ? base : base + (cz >> 1) * kp.nxny;
+          hi = ((u64)seg << kp.idb) | v;
+          k21 = (quad << 18) | sfc;
+          if (!bt.wide) {
+            key[i] = (hi << 21) | k21;
+            continue;
+          }
         }
-        hi = sfc;
-        k21 = 0;
-      } else {
-        const u64 ux = (u64)(i64)X, uy = (u64)(i64)Y, uz = (u64)(i64)Z;
-        const u64 cx = ux >> 6, cy = uy >> 6, cz = uz >> 6;
-        const u32 m = split3_6((u32)(ux & 63)) | (split3_6((u32)(uy & 63)) << 1) |
-                      (split3_6((u32)(uz & 63)) << 2);
-        const u32 sfc = kp.hilbert ? hilbert18(m, t, hs.block_init) : m;
-        const u32 quad = (u32)((cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2));
-        const u64 base = (cx >> 1) + (cy >> 1) * kp.nx;
-        const bool seg = (Z == -1);
-        const u64 v = seg ? base : base + (cz >> 1) * kp.nxny;
-        hi = ((u64)seg << kp.idb) | v;
-        k21 = (quad << 18) | sfc;
-        if (!bt.wide) {
-          key[i] = (hi << 21) | k21;
-          continue;
-        }
+        key[i] = hi;
+        if (lo64) lo64[i] = ((u64)i << 32) | k21;  // wide keys + permutation
+        else lo[i] = k21;
       }
-      key[i] = hi;
-      if (lo64) lo64[i] = ((u64)i << 32) | k21;  // wide keys + permutation
-      else lo[i] = k21;
     }
   }
 }
@@ -1407,12 +1431,36 @@ __global__ __launch_bounds__(kBlock) void k_block_stats(Batch bt, KV k, const u3
     if (sp.kp.direct) continue;
     const SegCtl& sc = bt.sc[t.seg];
     const Range r = tile_range(sc.b0, sc.nb, t.local);
-#pragma unroll 4
-    for (u32 it = 0; it < kItems; ++it) {
-      const u32 o = it * kBlock + threadIdx.x;
-      if (o < r.count) {
-        const u64 a = block_delta(k, bpos, sc, sp.kp, r.first + o, 0);
-        const u64 c = block_delta(k, bpos, sc, sp.kp, r.first + o, 1);
+    if (!r.count) continue;
+    // block_delta() of kGroup items at a time: block positions first, then
+    // their keys, each step's loads issued together
+    const u32 b0 = sc.b0, bend = sc.b0 + sc.nb, cend = sc.e0 + sc.ne;
+    for (u32 g0 = 0; g0 < kItems; g0 += kGroup) {
+      u32 st[kGroup], pv[kGroup], nx[kGroup];
+      u64 h0[kGroup], h1[kGroup];
+#pragma unroll
+      for (u32 it = 0; it < kGroup; ++it) {
+        const u32 o = (g0 + it) * kBlock + threadIdx.x;
+        const u32 b = r.first + (o < r.count ? o : r.count - 1);
+        st[it] = bpos[b];
+        pv[it] = bpos[b > b0 ? b - 1 : b];
+        const u32 after = bpos[b + 1 < bend ? b + 1 : b];
+        nx[it] = b + 1 < bend ? after : cend;
+      }
+#pragma unroll
+      for (u32 it = 0; it < kGroup; ++it) {
+        h0[it] = k.a[st[it]];
+        h1[it] = k.a[pv[it]];
+      }
+#pragma unroll
+      for (u32 it = 0; it < kGroup; ++it) {
+        const u32 o = (g0 + it) * kBlock + threadIdx.x;
+        if (o >= r.count) continue;
+        const u32 b = r.first + o;
+        const u64 id = id_of(k.wide ? h0[it] : h0[it] >> 21, sp.kp);
+        const u64 a = b > b0 ? id - id_of(k.wide ? h1[it] : h1[it] >> 21, sp.kp) : id;
+        const u64 cc = (u64)(nx[it] - st[it]);
+        const u64 c = b > b0 ? cc - (u64)(st[it] - pv[it]) : cc;
         m0 = a > m0 ? a : m0;
         m1 = c > m1 ? c : m1;
       }
@@ -1596,7 +1644,10 @@ __device__ __forceinline__ void tile_codes(const Batch& bt, const SecCtx& x, con
       const u32 b = idx(q);
       st[q] = x.bpos[b];
       pv[q] = x.bpos[b > b0 ? b - 1 : b];
-      if (kKind == kSecCnt) nx[q] = b + 1 < bend ? x.bpos[b + 1] : cend;
+      if (kKind == kSecCnt) {
+        const u32 after = x.bpos[b + 1 < bend ? b + 1 : b];
+        nx[q] = b + 1 < bend ? after : cend;
+      }
     }
     if (kKind == kSecCnt) {
 #pragma unroll
