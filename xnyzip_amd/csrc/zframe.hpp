@@ -153,22 +153,31 @@ struct FseState {
 };
 
 // Working storage of the serial steps below (host: any memory; device: LDS,
-// with one wavefront executing the steps uniformly).
+// with one wavefront executing the steps uniformly).  The Huffman build's
+// arrays are dead once it returns (its result is the HufTable), so the
+// weight description and the FSE table builds that follow share their
+// space.
 struct Work {
-  u8 spread[1u << kFseMaxLog];
-  u32 cumul[kFseMaxSym + 1];
-  u32 cnt[256];
-  u16 sym[256], tmp[256];
-  u32 pos[257];
-  u32 w[512];       // tree weights: sums of one block's counts (< 2^15)
-  i16 parent[512];
-  u16 depth[512];
-  u8 wt[256];
-  u8 dtmp[160];
-  u32 wcount[16];
-  i16 norm[kFseMaxSym];
-  u8 ntmp[128];
-  FseTable wtab;
+  union {
+    struct {  // huf_build
+      u32 cnt[256];
+      u16 sym[256], tmp[256];
+      u32 pos[257];
+      u32 w[512];  // tree weights: sums of one block's counts (< 2^15)
+      i16 parent[512];
+      u16 depth[512];
+    };
+    struct {  // huf_write_desc, the sequence tables
+      u8 spread[1u << kFseMaxLog];
+      u32 cumul[kFseMaxSym + 1];
+      u8 wt[256];
+      u8 dtmp[160];
+      u32 wcount[16];
+      i16 norm[kFseMaxSym];
+      u8 ntmp[128];
+      FseTable wtab;
+    };
+  };
 };
 
 XZ_HD inline void fse_init(FseState& st, const FseTable& t, u32 s) {
