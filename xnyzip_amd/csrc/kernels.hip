@@ -1721,39 +1721,20 @@ __device__ __forceinline__ void tile_codes(const Batch& bt, const SecCtx& x, con
   }
 }
 
-// Phase B: recompute the tile's codes, scan them inside the tile, assemble
-// the tile's bits in LDS (MSB-first streams in big-endian word order), then
-// store interior words and OR the two edge words, which neighbouring tiles
-// share.  A tile whose span exceeds the LDS window writes to global memory.
+// Phase B: scan one tile's codes of section kKind (cd: the thread's kItems
+// codes in wave-striped order) inside the tile, assemble the tile's bits in
+// LDS (MSB-first streams in big-endian word order), then store interior
+// words and OR the two edge words, which neighbouring tiles share.  A tile
+// whose span exceeds the LDS window writes to global memory.  Starts with a
+// barrier, so calls for several sections may follow one another on the same
+// window.
 template <u32 kKind>
-__global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const u64* tile_off,
-                                                      u32* out) {
+__device__ __forceinline__ void pack_tile(const SegParam& sp, const SegCtl& sc, u64 toff,
+                                          const Code* cd, u32* win, u64* sh, u32* out) {
   constexpr bool kMsb = SecTraits<kKind>::kMsb;
-  constexpr bool kTab = kKind == kSecHuf || kKind == kSecQuad || kKind == kSecRle;
-  static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
-  __shared__ u32 win[kVlWin];
-  __shared__ u64 sh[kWavesPerBlock];
-  __shared__ HufEntry tab[kTab ? kBlock : 1];
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  const SegCtl& sc = bt.sc[t.seg];
-  if (!sec_active<kKind>(sp, sc)) return;
-  const Range r = stage_range(sp, sc, t.local, SecTraits<kKind>::kSpace);
-  if (r.count == 0) return;
-  if (kTab) {
-    const SegLut& lut = bt.lut[t.seg];
-    tab[threadIdx.x] = kKind == kSecHuf ? lut.sym[threadIdx.x]
-                       : kKind == kSecQuad ? lut.quad[threadIdx.x] : lut.cnt[threadIdx.x];
-    __syncthreads();
-  }
-  // Element order inside the tile: wavefront w owns [w*kWaveSub,
-  // (w+1)*kWaveSub), taken 64 consecutive elements per step, so every load
-  // is coalesced.  rel[q] = bit offset of the lane's q-th code in its wave:
-  // the kItems lane scans advance level by level, all items together.
   const u32 w = threadIdx.x / kWave, l = lane_id();
-  const u32 wstart = w * kWaveSub;
-  Code cd[kItems];
-  tile_codes<kKind>(bt, x, sp, sc, r, wstart, l, tab, cd);
+  // rel[q] = bit offset of the lane's q-th code in its wave: the kItems lane
+  // scans advance level by level, all items together
   u64 rel[kItems];
 #pragma unroll
   for (u32 q = 0; q < kItems; ++q) rel[q] = (u64)cd[q].lead + cd[q].len;
@@ -1773,6 +1754,7 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
     rel[q] = carry + inc - ((u64)cd[q].lead + cd[q].len);
     carry += __shfl(inc, kWave - 1, kWave);
   }
+  __syncthreads();  // the window and sh of a previous section are free
   if (l == 0) sh[w] = carry;
   __syncthreads();
   u64 ex = 0, tot = 0;
@@ -1782,7 +1764,7 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
     tot += sh[i];
   }
   if (tot == 0) return;
-  const u64 abs0 = sp.sec_byte[kKind] * 8 + (tile_off[blockIdx.x] - sc.sec_bit0[kKind]);
+  const u64 abs0 = sp.sec_byte[kKind] * 8 + (toff - sc.sec_bit0[kKind]);
   const u64 w0 = abs0 >> 5;
   const u32 o0 = (u32)(abs0 & 31);
   const u64 nwords = (o0 + tot + 31) >> 5;
@@ -1804,6 +1786,114 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
   } else {
 #pragma unroll
     for (u32 q = 0; q < kItems; ++q) put_code<false, kMsb>(out, abs0 + ex + rel[q], cd[q]);
+  }
+}
+
+// One section's tile: its codes (tile_codes), then pack_tile.  Element order
+// inside the tile: wavefront w owns [w*kWaveSub, (w+1)*kWaveSub), taken 64
+// consecutive elements per step, so every load is coalesced.
+template <u32 kKind>
+__global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const u64* tile_off,
+                                                      u32* out) {
+  constexpr bool kTab = kKind == kSecHuf || kKind == kSecQuad || kKind == kSecRle;
+  static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
+  __shared__ u32 win[kVlWin];
+  __shared__ u64 sh[kWavesPerBlock];
+  __shared__ HufEntry tab[kTab ? kBlock : 1];
+  const Tile t = bt.tiles[blockIdx.x];
+  const SegParam& sp = bt.sp[t.seg];
+  const SegCtl& sc = bt.sc[t.seg];
+  if (!sec_active<kKind>(sp, sc)) return;
+  const Range r = stage_range(sp, sc, t.local, SecTraits<kKind>::kSpace);
+  if (r.count == 0) return;
+  if (kTab) {
+    const SegLut& lut = bt.lut[t.seg];
+    tab[threadIdx.x] = kKind == kSecHuf ? lut.sym[threadIdx.x]
+                       : kKind == kSecQuad ? lut.quad[threadIdx.x] : lut.cnt[threadIdx.x];
+    __syncthreads();
+  }
+  const u32 w = threadIdx.x / kWave, l = lane_id();
+  Code cd[kItems];
+  tile_codes<kKind>(bt, x, sp, sc, r, w * kWaveSub, l, tab, cd);
+  pack_tile<kKind>(sp, sc, tile_off[blockIdx.x], cd, win, sh, out);
+}
+
+// The four cell-space sections (Golomb-Rice patches, Huffman symbols,
+// quads, RLE counts) of one tile in one pass: the keys (and counts) are
+// loaded and the repos / quads derived once, then each active section's
+// codes are formed and packed in turn on the same LDS window.
+__global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
+                                                            const u64* tile_off, u32* out) {
+  static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
+  __shared__ u32 win[kVlWin];
+  __shared__ u64 sh[kWavesPerBlock];
+  __shared__ HufEntry tsym[kBlock], tquad[kBlock], tcnt[kBlock];
+  const Tile t = bt.tiles[blockIdx.x];
+  const SegParam& sp = bt.sp[t.seg];
+  const SegCtl& sc = bt.sc[t.seg];
+  const Range r = tile_range(sc.e0, sc.ne, t.local);
+  if (r.count == 0) return;
+  const bool gr = sec_active<kSecGR>(sp, sc), quad = sec_active<kSecQuad>(sp, sc);
+  const bool rle = sec_active<kSecRle>(sp, sc);
+  const SegLut& lut = bt.lut[t.seg];
+  tsym[threadIdx.x] = lut.sym[threadIdx.x];
+  if (quad) tquad[threadIdx.x] = lut.quad[threadIdx.x];
+  if (rle) tcnt[threadIdx.x] = lut.cnt[threadIdx.x];
+  __syncthreads();
+  const u32 w = threadIdx.x / kWave, l = lane_id(), wstart = w * kWaveSub;
+  const u32 lim = sp.lim, grk = sp.grk;
+  const u64 nt = bt.ntiles, b = blockIdx.x;
+  u32 o[kItems], qd[kItems], c[kItems];
+  u64 v[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) o[q] = wstart + q * kWave + l;
+  if (rle) {
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) c[q] = x.cnt[r.first + (o[q] < r.count ? o[q] : r.count - 1)];
+  }
+  lcp_items<kItems>(x.k, sp.kp.direct != 0, sc.e0, r.first, r.count, o, v, qd);
+  Code cd[kItems];
+  if (gr) {  // Golomb-Rice of the patches, LSB-first
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q)
+      cd[q] = o[q] < r.count && v[q] > lim
+                  ? Code{(v[q] & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(v[q] >> grk)}
+                  : Code{0, 0, 0};
+    pack_tile<kSecGR>(sp, sc, tile_off[kSecGR * nt + b], cd, win, sh, out);
+  }
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const HufEntry e = tsym[v[q] > lim ? 0u : (u32)v[q] + 1u];
+    cd[q] = o[q] < r.count ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
+  }
+  pack_tile<kSecHuf>(sp, sc, tile_off[kSecHuf * nt + b], cd, win, sh, out);
+  if (quad) {
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      const HufEntry e = tquad[qd[q] > 254 ? 0u : qd[q] + 1u];
+      cd[q] = o[q] < r.count ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
+    }
+    pack_tile<kSecQuad>(sp, sc, tile_off[kSecQuad * nt + b], cd, win, sh, out);
+  }
+  if (rle) {
+    const bool eg = sp.cnt_eg != 0;
+    const u32 kk = sp.egk;
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      Code e;
+      if (eg) {  // Exp-Golomb of zigzag(int)
+        const u64 z = (u64)c[q] << 1;
+        const u64 n1 = (z >> kk) + 1;
+        const u32 nb = bitlen64(n1);
+        const u64 low = kk ? (__builtin_bitreverse64(z) >> (64 - kk)) : 0;
+        e = Code{(n1 << kk) | low, nb + kk, nb - 1};
+      } else {
+        const HufEntry h = c[q] < kCntBinsSeg ? tcnt[c[q]] : big_code(bt, sp, c[q]);
+        e = Code{h.bits, h.len, 0};
+      }
+      cd[q] = o[q] < r.count ? e : Code{0, 0, 0};
+    }
+    pack_tile<kSecRle>(sp, sc, tile_off[kSecRle * nt + b], cd, win, sh, out);
   }
 }
 
@@ -1994,10 +2084,8 @@ void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u3
   const SecCtx x{KV{kv.a, kv.b, bt.wide}, cnt, bpos};
   write_one<kSecBlk>(bt, x, p, out_words, s);
   write_one<kSecCnt>(bt, x, p, out_words, s);
-  write_one<kSecGR>(bt, x, p, out_words, s);
-  write_one<kSecHuf>(bt, x, p, out_words, s);
-  write_one<kSecQuad>(bt, x, p, out_words, s);
-  write_one<kSecRle>(bt, x, p, out_words, s);
+  k_sec_write_cells<<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
+  LAUNCH_CHECK("k_sec_write<cells>");
 }
 
 }  // namespace xnyz
