@@ -2113,7 +2113,7 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
                   t.copy_first = first;
                   t.gpu_frame_share = (float)v;
                 },
-                {0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6});
+                {0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7});
         shapes();
         const double s0 = cur.gpu_frame_share;
         if (s0 > 0)
