@@ -350,6 +350,16 @@ int xnyz_profile_report(xnyz_ctx* ctx, char* buf, uint64_t cap, uint64_t* needed
  * returned std::vector, which owned its bytes). */
 void xnyz_free(xnyz_buf* buf);
 
+/* Pinned (page-locked) host memory for the `out` of
+ * xnyz_compress_batch_device_into: when the output lies inside one such
+ * buffer, the device-built blocks of a split frame are copied there by DMA
+ * directly instead of through a staging buffer.  No reference counterpart
+ * (the reference returns a std::vector).  xnyz_host_free takes only
+ * pointers from xnyz_host_alloc (XNYZ_EINVAL otherwise); free a buffer only
+ * when no call writing into it is in flight. */
+int xnyz_host_alloc(uint64_t bytes, void** out);
+int xnyz_host_free(void* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,32 @@ def test_split_frame(X, ctx, share, chunk_log, rle, first):
     assert again == frame
 
 
+def test_split_frame_pinned_out(X, ctx):
+    """Output in a pinned buffer from xnyz_host_alloc takes the device's
+    blocks by DMA: the bytes equal those written into pageable memory, at
+    an offset inside the buffer too; a buffer too small is refused."""
+    n = 1_500_000
+    c = _uniform(86, n)
+    ctx.set_tuning(_size_class(n), zstd_chunk_log=17, gpu_frame_share=0.5, copy_slice_log=20)
+    dev = torch.from_numpy(c).cuda()
+    opts = [dict(quantizer=O.TO, l2_bound=0.01, is_hilbert=True, rle=True, frame_mode=MT)]
+    page = np.zeros(13 * n, dtype=np.uint8)
+    offs, _ = X.compress_batch_into(dev, [0, n], opts, page, ctx=ctx)
+    want = bytes(page[:int(offs[1])])
+    assert unzstd(want) == O.compress(c, O.TO, 0.01, True, True).pre
+    pinned = X.host_buffer(13 * n + 4096)
+    for at in (0, 4096):
+        pinned[:] = 0xA5
+        offs2, _ = X.compress_batch_into(dev, [0, n], opts, pinned[at:], ctx=ctx)
+        assert bytes(pinned[at:at + int(offs2[1])]) == want, at
+        assert not np.any(pinned[at + int(offs2[1]):] != 0xA5)
+    from xnyzip_amd import _lib
+    with pytest.raises(_lib.XnyzError) as e:
+        X.compress_batch_into(dev, [0, n], opts, pinned[:len(want) - 1], ctx=ctx)
+    assert e.value.code == _lib.XNYZ_ENOSPC
+    del pinned
+
+
 def test_split_needs_room(X, ctx):
     """A stream too short to split (device share under 1 MiB) takes the host
     path."""

@@ -42,3 +42,8 @@ def test_argument_errors_without_gpu():
     assert L.xnyz_compress(None, None, 0, None, None, None, None, None) == -1
     assert b"null" in L.xnyz_last_error()
     L.xnyz_free(None)                               # no-op
+    assert L.xnyz_host_alloc(0, None) == -1         # null out, zero size
+    assert L.xnyz_host_free(None) == 0              # no-op
+    scratch = ctypes.create_string_buffer(64)       # not from xnyz_host_alloc
+    assert L.xnyz_host_free(ctypes.addressof(scratch)) == -1
+    assert b"xnyz_host_alloc" in L.xnyz_last_error()

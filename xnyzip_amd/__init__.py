@@ -23,7 +23,7 @@ from ._lib import Context, XnyzError  # noqa: F401
 
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
-           "compress", "compress_batch", "compress_batch_into", "search_batch", "features_batch", "mlp_train",
+           "compress", "compress_batch", "compress_batch_into", "host_buffer", "search_batch", "features_batch", "mlp_train",
            "mlp_predict", "compress_selected_batch", "quantize", "morton_to_hilbert", "autotune"]
 
 
@@ -173,6 +173,33 @@ def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
         C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p))
     _lib.check(rc)
     return res, list(stats)
+
+
+class _Pinned:
+    """Owner of one xnyz_host_alloc buffer (freed with the last array
+    viewing it)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        _lib.check(_lib.lib().xnyz_host_alloc(nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def __del__(self):
+        if self.ptr:
+            _lib.lib().xnyz_host_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def host_buffer(nbytes: int) -> np.ndarray:
+    """A uint8 array of `nbytes` in pinned host memory (xnyz_host_alloc): as
+    the `out` of compress_batch_into it takes the device-built blocks of a
+    split frame by DMA, with no staging copy."""
+    if nbytes <= 0:
+        raise ValueError("nbytes must be positive")
+    owner = _Pinned(int(nbytes))
+    view = (C.c_uint8 * int(nbytes)).from_address(owner.ptr)
+    view._owner = owner
+    return np.frombuffer(view, dtype=np.uint8)
 
 
 def autotune(points_device, opts: dict, reps: int = 2, ctx: Context | None = None):

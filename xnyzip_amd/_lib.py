@@ -105,7 +105,8 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_quantize_device",
            "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_graphs_enable",
            "xnyz_tuning_set", "xnyz_tuning_get", "xnyz_tuning_save", "xnyz_tuning_load",
-           "xnyz_autotune", "xnyz_profile_enable", "xnyz_profile_report", "xnyz_free"]
+           "xnyz_autotune", "xnyz_profile_enable", "xnyz_profile_report", "xnyz_free",
+           "xnyz_host_alloc", "xnyz_host_free"]
 
 
 def lib() -> C.CDLL:
@@ -148,6 +149,8 @@ def lib() -> C.CDLL:
                                     C.POINTER(Tuning), C.POINTER(C.c_double)]
         L.xnyz_profile_report.argtypes = [P, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.xnyz_free.argtypes = [C.POINTER(Buf)]
+        L.xnyz_host_alloc.argtypes = [C.c_uint64, PP]
+        L.xnyz_host_free.argtypes = [P]
         _lib = L
         return L
 

@@ -27,6 +27,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
@@ -116,6 +118,41 @@ struct HBuf {  // pinned host memory
     return static_cast<T*>(p);
   }
 };
+
+// Pinned host buffers handed out by xnyz_host_alloc: base -> size.  Output
+// inside one of them takes device-built frame bytes straight from the DMA
+// engine, with no staging copy.
+std::mutex g_pinned_mu;
+std::map<uintptr_t, u64> g_pinned;
+
+bool pinned_range(const void* p, u64 n) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> lk(g_pinned_mu);
+  auto it = g_pinned.upper_bound(a);
+  if (it == g_pinned.begin()) return false;
+  --it;
+  return a - it->first <= it->second && n <= it->second - (a - it->first);
+}
+
+void* pinned_alloc(u64 bytes) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error(XNYZ_ENOMEM, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  std::lock_guard<std::mutex> lk(g_pinned_mu);
+  g_pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+  return p;
+}
+
+void pinned_free(void* p) {
+  {
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    if (!g_pinned.erase(reinterpret_cast<uintptr_t>(p)))
+      throw Error(XNYZ_EINVAL, "pointer did not come from xnyz_host_alloc");
+  }
+  check_hip(hipHostFree(p), "hipHostFree");
+}
 
 u32 bitlen(u64 v) { return v ? 64u - (u32)__builtin_clzll(v) : 0u; }
 
@@ -1335,9 +1372,11 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
         dst = co.dst;
       }
       // the device-built blocks come back (behind the gather) while the
-      // host's blocks are copied into place
-      ctx->h_zout.ensure(head_bytes + 64);
-      zb = ctx->h_zout.as<uint8_t>();
+      // host's blocks are copied into place: straight into the output when
+      // it is pinned, else through the context's staging buffer
+      const bool direct = co.dst && pinned_range(dst, n);
+      if (!direct) ctx->h_zout.ensure(head_bytes + 64);
+      zb = direct ? dst : ctx->h_zout.as<uint8_t>();
       check_hip(hipMemcpyAsync(zb, ctx->zout.p, head_bytes, hipMemcpyDeviceToHost, st), "frames D2H");
       ctx->copied();
       copy_pieces(outs[0], dst + head_bytes, threads);
@@ -1345,7 +1384,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       const uint64_t fcs = g[0].total;  // the header covers the whole stream
       std::memcpy(zb + 5, &fcs, 8);
       zb[zpos[znblk - 1] + 13] &= 0xFE;  // the host's blocks follow the last one
-      copy_pieces({Piece{zb, head_bytes}}, dst, threads);
+      if (!direct) copy_pieces({Piece{zb, head_bytes}}, dst, threads);
     } else if (co.frames) {
       parallel_for(S, per_seg_threads ? threads : 1, [&](size_t s) {
         const u64 n = pieces_size(outs[s]);
@@ -2013,20 +2052,41 @@ int xnyz_autotune(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts
     use_device(ctx);
     const int c = size_class(n);
     const std::vector<u64> off{0, n};
-    std::vector<uint8_t> out(13 * (size_t)n + (1u << 20));
+    // A pinned output, as a serving loop reusing one would have (a split
+    // frame's device blocks are copied straight into it).
+    struct Out {
+      uint8_t* p = nullptr;
+      u64 n = 0;
+      void alloc(u64 bytes) {
+        release();
+        p = static_cast<uint8_t*>(pinned_alloc(bytes));
+        n = bytes;
+      }
+      void release() {
+        if (p) pinned_free(p);
+        p = nullptr;
+      }
+      ~Out() {
+        try {
+          release();
+        } catch (...) {
+        }
+      }
+    } out;
+    out.alloc(13 * n + (1u << 20));
     uint64_t out_off[2];
     auto call = [&] {
       for (int attempt = 0;; ++attempt) {
         CallOut co{nullptr, nullptr, nullptr, nullptr};
-        co.dst = out.data();
-        co.cap = out.size();
+        co.dst = out.p;
+        co.cap = out.n;
         co.dst_off = out_off;
         try {
           compress_impl(ctx, d_xyz, off, opts, co, now_ms());
           return;
         } catch (const Error& e) {
           if (e.code != XNYZ_ENOSPC || attempt) throw;
-          out.resize(out_off[1]);
+          out.alloc(out_off[1]);
         }
       }
     };
@@ -2163,6 +2223,23 @@ int xnyz_profile_report(xnyz_ctx* ctx, char* buf, uint64_t cap, uint64_t* needed
       std::memcpy(buf, js.data(), n);
       buf[n] = 0;
     }
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_host_alloc(uint64_t bytes, void** out) {
+  return guarded([&] {
+    if (!out || !bytes) throw Error(XNYZ_EINVAL, "null out or zero size");
+    *out = nullptr;
+    *out = pinned_alloc(bytes);
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_host_free(void* p) {
+  if (!p) return XNYZ_OK;
+  return guarded([&] {
+    pinned_free(p);
     return XNYZ_OK;
   });
 }
