@@ -186,7 +186,7 @@ def main() -> int:
 
     def step(mode, out=None):
         offs, st = (out or sink).compress(X, dev, [0, n], [dict(o1[0], frame_mode=mode)], ctx)
-        return int(offs[1]), st[0]
+        return int(offs[1] - offs[0]), st[0]
 
     def timed(mode, steps, warmup, out=None):
         for _ in range(warmup):

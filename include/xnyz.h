@@ -180,11 +180,15 @@ int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t
 
 /* xnyz_compress_batch_device writing into caller memory instead of
  * library-allocated buffers (no allocation or page faulting per call):
- * cloud s's output, as opts[s].frame_mode selects, lands at out +
- * out_offsets[s]; out_offsets (count + 1 entries) ends with the total.  If
- * the total exceeds cap nothing is written, out_offsets[count] holds the
- * size needed and the call returns XNYZ_ENOSPC.  No reference counterpart
- * beyond compress(). */
+ * cloud s's output, as opts[s].frame_mode selects, is out[out_offsets[s],
+ * out_offsets[s + 1]) (count + 1 entries, outputs back to back).
+ * out_offsets[0] is 0, except for a frame split between the device and the
+ * host (a single XNYZ_FRAME_ZSTD3_MT cloud with a device share) when cap
+ * also holds the bound of both parts: then the host's blocks are placed at
+ * a fixed offset while they are compressed, the device's part just before
+ * them, and the frame starts at out_offsets[0] > 0.  If the output does not
+ * fit, nothing is written, out_offsets[count] holds the size needed and the
+ * call returns XNYZ_ENOSPC.  No reference counterpart beyond compress(). */
 int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
                                     uint32_t count, const xnyz_opts* opts, uint8_t* out,
                                     uint64_t cap, uint64_t* out_offsets, xnyz_stats* stats);

@@ -136,10 +136,14 @@ def test_split_frame(X, ctx, share, chunk_log, rle, first):
     assert again == frame
 
 
-def test_split_frame_pinned_out(X, ctx):
-    """Output in a pinned buffer from xnyz_host_alloc takes the device's
-    blocks by DMA: the bytes equal those written into pageable memory, at
-    an offset inside the buffer too; a buffer too small is refused."""
+def test_split_frame_into(X, ctx):
+    """Split frames into caller memory.  With room for both parts' bounds the
+    layout is anchored: the host's blocks sit at the device part's bound
+    and the frame starts past offset 0; in a tight buffer it starts at 0.
+    Pinned buffers from xnyz_host_alloc take the device's blocks by DMA.
+    Every layout gives the same bytes, nothing outside the frame is
+    written, and a buffer one byte short is refused."""
+    from xnyzip_amd import _lib
     n = 1_500_000
     c = _uniform(86, n)
     ctx.set_tuning(_size_class(n), zstd_chunk_log=17, gpu_frame_share=0.5, copy_slice_log=20)
@@ -147,19 +151,29 @@ def test_split_frame_pinned_out(X, ctx):
     opts = [dict(quantizer=O.TO, l2_bound=0.01, is_hilbert=True, rle=True, frame_mode=MT)]
     page = np.zeros(13 * n, dtype=np.uint8)
     offs, _ = X.compress_batch_into(dev, [0, n], opts, page, ctx=ctx)
-    want = bytes(page[:int(offs[1])])
+    assert offs[0] > 0, "13 bytes per point leave room for the anchored layout"
+    want = bytes(page[int(offs[0]):int(offs[1])])
     assert unzstd(want) == O.compress(c, O.TO, 0.01, True, True).pre
+    frame, _, _, _ = X.compress(dev, O.TO, 0.01, True, True, frame_mode=MT, ctx=ctx)
+    assert frame == want
+    size = len(want)
     pinned = X.host_buffer(13 * n + 4096)
-    for at in (0, 4096):
+    for at, cap in ((0, 13 * n), (4096, 13 * n), (8, size + 100), (0, size)):
         pinned[:] = 0xA5
-        offs2, _ = X.compress_batch_into(dev, [0, n], opts, pinned[at:], ctx=ctx)
-        assert bytes(pinned[at:at + int(offs2[1])]) == want, at
-        assert not np.any(pinned[at + int(offs2[1]):] != 0xA5)
-    from xnyzip_amd import _lib
+        view = pinned[at:at + cap]
+        o2, _ = X.compress_batch_into(dev, [0, n], opts, view, ctx=ctx)
+        a, b = int(o2[0]), int(o2[1])
+        assert b - a == size and bytes(view[a:b]) == want, (at, cap)
+        assert (a > 0) == (cap == 13 * n), (at, cap)
+        assert not np.any(pinned[:at + a] != 0xA5) and not np.any(pinned[at + b:] != 0xA5)
+    tight = np.zeros(size + 100, dtype=np.uint8)
+    o3, _ = X.compress_batch_into(dev, [0, n], opts, tight, ctx=ctx)
+    assert o3[0] == 0 and bytes(tight[:size]) == want
+    pinned[:] = 0xA5
     with pytest.raises(_lib.XnyzError) as e:
-        X.compress_batch_into(dev, [0, n], opts, pinned[:len(want) - 1], ctx=ctx)
-    assert e.value.code == _lib.XNYZ_ENOSPC
-    del pinned
+        X.compress_batch_into(dev, [0, n], opts, pinned[:size - 1], ctx=ctx)
+    assert e.value.code == _lib.XNYZ_ENOSPC and not np.any(pinned[:size - 1] != 0xA5)
+    del view, pinned
 
 
 def test_split_needs_room(X, ctx):

@@ -143,10 +143,12 @@ def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
 def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
                         ctx: Context | None = None):
     """compress_batch writing every cloud's output back to back into `out`
-    (a uint8 numpy array, reused across calls).  Returns (offsets of the
-    outputs, count + 1 entries, stats); raises XnyzError with code
-    XNYZ_ENOSPC if `out` is too small (offsets[-1] then holds the size
-    needed)."""
+    (a uint8 numpy array, reused across calls).  Returns (offsets, stats):
+    cloud s's output is out[offsets[s]:offsets[s + 1]]; offsets[0] is 0
+    except for a split frame with room for its anchored layout (the host's
+    blocks placed while compressed, the device's part in front of them).
+    Raises XnyzError with code XNYZ_ENOSPC if `out` is too small
+    (offsets[-1] then holds the size needed)."""
     import torch
     t = points_device.reshape(-1, 3)
     if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:

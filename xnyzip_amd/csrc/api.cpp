@@ -496,11 +496,14 @@ Pieces zstd_single(ZSTD_CCtx* cc, const uint8_t* src, u64 len, uint8_t* dst, u64
 // read: the source may still be arriving from the device.
 // With `header` false, only the blocks of [0, len) are written (the last one
 // flagged): the tail of a frame whose header and first blocks come from
-// elsewhere.
+// elsewhere.  `place` (nullable, with `header` false) is where those blocks
+// go in the end: the workers copy every chunk there as soon as all chunks
+// before it are done, while later chunks are still being compressed.
 Pieces zstd_chunked(FrameState& fs, ZSTD_CCtx* own, const uint8_t* src, u64 len, int threads,
                     uint8_t* dst, u64 chunk, const std::function<void(u64)>* ready, u64 fcs,
-                    bool header) {
+                    bool header, uint8_t* place = nullptr) {
   const size_t nchunks = (len + chunk - 1) / chunk;
+  if (place && header) throw Error(XNYZ_EINTERNAL, "placed chunks carry no frame header");
   if (header && (nchunks <= 1 || threads <= 1)) {
     if (ready) (*ready)(len);
     return zstd_single(own, src, len, dst, ZSTD_compressBound(len));
@@ -517,6 +520,11 @@ Pieces zstd_chunked(FrameState& fs, ZSTD_CCtx* own, const uint8_t* src, u64 len,
   Pieces out(nchunks + 1);
   out[0] = Piece{hdr, header ? 13u : 0u};
   uint8_t* parts = dst + 16;
+  // in-order placement: chunks [0, placed) have their final offsets
+  std::mutex mu;
+  std::vector<char> done(place ? nchunks : 0, 0);
+  size_t placed = 0;
+  u64 place_at = 0;
   parallel_for_w(nchunks, t, [&](size_t c, size_t w) {
     ZSTD_CCtx* cc = fs.cctx[w];
     const size_t off = c * chunk, n = std::min<size_t>(chunk, len - off);
@@ -545,6 +553,22 @@ Pieces zstd_chunked(FrameState& fs, ZSTD_CCtx* own, const uint8_t* src, u64 len,
       at += 3 + size;
     }
     out[c + 1] = Piece{o, at};
+    if (place) {
+      size_t first, last;
+      u64 to;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        done[c] = 1;
+        first = placed;
+        to = place_at;
+        while (placed < nchunks && done[placed]) place_at += out[++placed].n;
+        last = placed;
+      }
+      for (size_t k = first; k < last; ++k) {  // this worker's share of the prefix
+        std::memcpy(place + to, out[k + 1].p, out[k + 1].n);
+        to += out[k + 1].n;
+      }
+    }
   });
   return out;
 }
@@ -691,8 +715,10 @@ struct CallOut {
   // Selector features: stop after the range scan and return kFeat floats
   // per segment.
   std::vector<float>* features = nullptr;
-  // Outputs into caller memory instead of `frames`: segment s at
-  // dst + dst_off[s], dst_off[S] = total (also set when cap is too small).
+  // Outputs into caller memory instead of `frames`: segment s in
+  // [dst + dst_off[s], dst + dst_off[s + 1]), back to back; dst_off[0] is 0
+  // except for an anchored split frame.  When cap is too small, dst_off[S]
+  // is the size needed.
   uint8_t* dst = nullptr;
   u64 cap = 0;
   uint64_t* dst_off = nullptr;
@@ -1326,6 +1352,17 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       zoff[s + 1] = zoff[s] + ((need + 63) & ~u64(63));
     }
     if (fs.buf.size() < zoff[S]) fs.buf.resize(zoff[S]);
+    // Anchored split frame (caller memory with room for it): the host's
+    // blocks are placed at out + anchor while they are compressed, anchor
+    // being the bound of the device-built part, and that part lands right
+    // in front of them once its size is known -- the frame then starts
+    // past `out`, and no host block is copied after the device is done.
+    u64 anchor = 0;
+    if (split && co.dst) {
+      const u64 dev_bound = 13 + head + 3 * ((head + xz::kBlock - 1) / xz::kBlock);
+      const u64 host_bound = (g[0].total - head + chunk - 1) / chunk * zchunk_bound(chunk);
+      if (co.cap >= dev_bound + host_bound) anchor = dev_bound;
+    }
     // Every worker's context exists before the workers start (no resizing
     // while they read the table).
     fs.ensure_workers(per_seg_threads ? (size_t)std::min<u64>(S, (u64)threads) : 1);
@@ -1345,7 +1382,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
         case XNYZ_FRAME_NONE: outs[s] = {Piece{b, len}}; break;
         case XNYZ_FRAME_ZSTD3_MT:
           outs[s] = split ? zstd_chunked(fs, fs.cctx[w], b + head, len - head, zthreads[s], z,
-                                         chunk, &ready, 0, false)
+                                         chunk, &ready, 0, false,
+                                         anchor ? co.dst + anchor : nullptr)
                           : zstd_chunked(fs, fs.cctx[w], b, len, zthreads[s], z, chunk,
                                          overlap ? &ready : nullptr, len, true);
           break;
@@ -1357,12 +1395,18 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       // before the blocks themselves arrive, so the host's blocks are put in
       // place while the device gathers and copies its own.
       check_hip(hipEventSynchronize(ctx->ev_sizes), "hipEventSynchronize");
-      const u64 head_bytes = 13 + zpos[znblk], n = head_bytes + pieces_size(outs[0]);
+      const u64 head_bytes = 13 + zpos[znblk], host_bytes = pieces_size(outs[0]);
+      const u64 n = head_bytes + host_bytes;
       uint8_t* dst;
       if (co.frames) {
         dst = xmalloc(n);
         co.frames[0].data = dst;
         co.frames[0].len = n;
+      } else if (anchor) {
+        if (head_bytes > anchor) throw Error(XNYZ_EINTERNAL, "device frame exceeds its bound");
+        co.dst_off[0] = anchor - head_bytes;
+        co.dst_off[1] = anchor + host_bytes;
+        dst = co.dst + co.dst_off[0];
       } else {
         co.dst_off[0] = 0;
         co.dst_off[1] = n;
@@ -1379,7 +1423,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       zb = direct ? dst : ctx->h_zout.as<uint8_t>();
       check_hip(hipMemcpyAsync(zb, ctx->zout.p, head_bytes, hipMemcpyDeviceToHost, st), "frames D2H");
       ctx->copied();
-      copy_pieces(outs[0], dst + head_bytes, threads);
+      if (!anchor) copy_pieces(outs[0], dst + head_bytes, threads);
       check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
       const uint64_t fcs = g[0].total;  // the header covers the whole stream
       std::memcpy(zb + 5, &fcs, 8);
