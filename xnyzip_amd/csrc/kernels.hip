@@ -808,10 +808,20 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(Batch bt, const K* __rest
   const Range r = tile_range(sp.p0, sp.n, t.local);
   const u32 base = sub * RS::kRT;
   const u32 w = threadIdx.x / kWave;
-#pragma unroll 4
-  for (u32 it = 0; it < RS::kIt; ++it) {
-    const u32 o = base + it * kBlock + threadIdx.x;
-    if (o < r.count) atomicAdd(&h[w][(u32)(keys[r.first + o] >> shift) & (RS::D - 1)], 1u);
+  if (r.count > base) {  // uniform; an empty radix tile still writes its zero counts
+    // every key in flight before the first count: loads clamped to the
+    // tile's last key, so none is conditional
+    const u32 last = r.count - 1;
+    K kk[RS::kIt];
+#pragma unroll
+    for (u32 it = 0; it < RS::kIt; ++it) {
+      const u32 o = base + it * kBlock + threadIdx.x;
+      kk[it] = keys[r.first + (o < r.count ? o : last)];
+    }
+#pragma unroll
+    for (u32 it = 0; it < RS::kIt; ++it)
+      if (base + it * kBlock + threadIdx.x < r.count)
+        atomicAdd(&h[w][(u32)(kk[it] >> shift) & (RS::D - 1)], 1u);
   }
   __syncthreads();
   const u32 d = threadIdx.x;
@@ -853,16 +863,24 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
   // The radix tile's global digit offsets are strided in memory; start that
   // load now so its latency overlaps the key loads and the ranking.
   const u32 my_off = threadIdx.x < D ? offs[count_index(sp, t.local, sub, threadIdx.x, D, kSub)] : 0u;
+  if (count == 0) return;  // uniform: an empty radix tile moves nothing
   K k[kIt];
   V v[kIt];
   u32 rk[kIt];
   u32 dg[kIt];
+  // every key (and value) in flight before the ranking: loads clamped to
+  // the tile's last item, so none is conditional
+#pragma unroll
+  for (u32 s = 0; s < kIt; ++s) {
+    const u32 o = wstart + s * kWave + l;
+    const u32 oc = o < count ? o : count - 1;
+    k[s] = kin[first + oc];
+    if (kHasV) v[s] = vin[first + oc];
+  }
 #pragma unroll
   for (u32 s = 0; s < kIt; ++s) {
     const u32 o = wstart + s * kWave + l;
     const bool valid = o < count;
-    k[s] = valid ? kin[first + o] : (K)0;
-    if (kHasV) v[s] = valid ? vin[first + o] : (V)0;
     const u32 d = (u32)(k[s] >> shift) & (D - 1);
     u64 peers = __ballot(valid);
 #pragma unroll
