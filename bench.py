@@ -416,6 +416,11 @@ def run_batched(args) -> int:
     del pts
     ctx = X.Context(0)
 
+    # Steady state: the first calls of a process fault in the pages of the
+    # outputs' fresh host allocations (one buffer per cloud) until the
+    # allocator reuses them, so every form gets at least three warmup calls.
+    wu = max(args.warmup, 3)
+
     def timed(fn, steps, warmup):
         for _ in range(warmup):
             fn()
@@ -427,7 +432,7 @@ def run_batched(args) -> int:
         return time.perf_counter() - t0, out
 
     sink = OutBuffer(13 * n, pinned=not args.pageable_out)
-    out = {"unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+    out = {"unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": wu,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic",
            "config": {"workload": args.workload, "points": n, "clouds": len(offs) - 1}}
@@ -439,7 +444,7 @@ def run_batched(args) -> int:
             def step():
                 res, st = sink.compress(X, dev, offs, o2, ctx)
                 return int(res[-1]), st
-            dt, (nbytes, _) = timed(step, args.steps, args.warmup)
+            dt, (nbytes, _) = timed(step, args.steps, wu)
             gbps = 12.0 * n * args.steps / dt / 1e9
             out[f"{key}_GBps"] = round(gbps, 3)
             out[f"{key}_bytes"] = nbytes
@@ -466,7 +471,7 @@ def run_batched(args) -> int:
                     fr, _, _, modes, searched = X.compress_selected_batch(
                         dev, offs, o2, params=p, min_confidence=0.5, ctx=ctx)
                     return sum(len(f) for f in fr), int(np.sum(searched))
-                dt, (nbytes, fell) = timed(step, args.steps, args.warmup)
+                dt, (nbytes, fell) = timed(step, args.steps, wu)
                 res[f"{label}_{key}_GBps"] = round(12.0 * n * args.steps / dt / 1e9, 3)
                 res[f"{label}_{key}_bytes"] = nbytes
                 res[f"{label}_fell_back"] = fell
