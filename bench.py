@@ -361,20 +361,9 @@ def measured_traffic(kernel: str, workload: str, n: int):
 
 
 def _uneven_clouds(count):
-    import numpy as np
-
     from xnyzip_amd import datasets
-    clouds, opts = [], []
-    for i in range(count):
-        rng = np.random.default_rng(i)
-        n = int(np.exp(rng.uniform(np.log(1e3), np.log(4e6))))
-        if i % 2 == 0:
-            c = datasets.terrain(n, seed=i, threads=4)
-        else:
-            c = datasets.shells(n, seed=i, count=max(1, min(2000, n // 1000)))
-        clouds.append(c)
-        opts.append(dict(quantizer=1, l2_bound=[0.001, 0.01, 0.1][i % 3]))
-    return clouds, opts
+    return ([datasets.uneven_cloud(i) for i in range(count)],
+            [datasets.uneven_opts(i) for i in range(count)])
 
 
 def run_batched(args) -> int:
@@ -400,14 +389,7 @@ def run_batched(args) -> int:
         offs = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).tolist()
         del clouds
     else:
-        half = (args.points or w["n"]) // 2
-        pts = np.concatenate([datasets.terrain(half, seed=0), datasets.shells(half, seed=1)])
-        rng = np.random.default_rng(0)
-        offs = [0]
-        while offs[-1] < len(pts):
-            offs.append(min(len(pts), offs[-1] + int(2.0 ** rng.uniform(16, 20))))
-        opts = [dict(quantizer=w["quantizer"], l2_bound=[0.001, 0.01, 0.1][i % 3])
-                for i in range(len(offs) - 1)]
+        pts, offs, opts = datasets.selector_workload(args.points or w["n"])
     n = len(pts)
     print(f"[bench] workload {args.workload}: {n} points in {len(offs) - 1} clouds generated "
           f"in {time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
@@ -470,7 +452,7 @@ def run_batched(args) -> int:
                 def step():
                     fr, _, _, modes, searched = X.compress_selected_batch(
                         dev, offs, o2, params=p, min_confidence=0.5, ctx=ctx)
-                    return sum(len(f) for f in fr), int(np.sum(searched))
+                    return sum(len(f) for f in fr), int(np.sum(searched & 1))
                 dt, (nbytes, fell) = timed(step, args.steps, wu)
                 res[f"{label}_{key}_GBps"] = round(12.0 * n * args.steps / dt / 1e9, 3)
                 res[f"{label}_{key}_bytes"] = nbytes

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define XNYZ_ABI_VERSION 4
+#define XNYZ_ABI_VERSION 5
 
 enum xnyz_status {
   XNYZ_OK = 0,
@@ -247,15 +247,32 @@ int xnyz_mlp_predict(xnyz_ctx* ctx, const float* params, const float* features, 
  * where its confidence is >= min_confidence, the exhaustive search's optimum
  * otherwise (params NULL: search every cloud).  opts supply quantizer,
  * l2_bound, direct_threshold and frame_mode; the chosen curve / RLE / k /
- * width replace the rest.  chosen and searched (1 = fell back to the search)
- * are nullable arrays of `count`.  The RLE flag is not recorded in the
- * stream: decode each cloud with chosen[s].rle. */
+ * width replace the rest.  The compressing pass then settles each cloud's
+ * Golomb-Rice k and PFD width exactly from its own statistics (the size
+ * model of the search, at no extra device pass): a predicted k / width
+ * stays where it is optimal and is replaced by the optimum where it costs
+ * more.  chosen (nullable, `count` entries) gets the modes used; searched
+ * (nullable) gets bit 0 = the cloud fell back to the search, bit 1 = the
+ * predicted k / width were replaced.  The RLE flag is not recorded in the
+ * stream: decode each cloud with chosen[s].rle (or write the outputs into a
+ * container, xnyz_container_*). */
 int xnyz_compress_selected_batch_device(xnyz_ctx* ctx, const float* d_xyz,
                                         const uint64_t* offsets, uint32_t count,
                                         const xnyz_opts* opts, const float* params,
                                         float min_confidence, xnyz_buf* frames,
                                         xnyz_buf* prezstd, xnyz_stats* stats,
                                         xnyz_mode* chosen, int32_t* searched);
+
+/* xnyz_compress_selected_batch_device writing into caller memory, laid out
+ * as by xnyz_compress_batch_device_into (out_offsets: count + 1 entries;
+ * XNYZ_ENOSPC with the size needed in out_offsets[count] when cap is too
+ * small).  No reference counterpart beyond compress(). */
+int xnyz_compress_selected_batch_device_into(xnyz_ctx* ctx, const float* d_xyz,
+                                             const uint64_t* offsets, uint32_t count,
+                                             const xnyz_opts* opts, const float* params,
+                                             float min_confidence, uint8_t* out, uint64_t cap,
+                                             uint64_t* out_offsets, xnyz_stats* stats,
+                                             xnyz_mode* chosen, int32_t* searched);
 
 /* Replaces TruncatedOctahedronQuantizer/CubeQuantizer::quantize after
  * shift_points (source/quantizer/truncated_octahedron_quantizer.hpp:21-55,

@@ -143,10 +143,75 @@ def test_selected_batch_streams(X):
                                                                 want_prezstd=True)
     pred, conf = X.mlp_predict(params, F)
     for i, ((c, o), m) in enumerate(zip(cases, modes2)):
-        if searched2[i]:
+        var = 2 * m["is_hilbert"] + m["rle"]
+        if searched2[i] & 1:
             assert conf[i] < 0.5 and m == res[i]["best"]
         else:
-            assert conf[i] >= 0.5 and m == pred[i]
+            assert conf[i] >= 0.5
+            assert (m["is_hilbert"], m["rle"]) == (pred[i]["is_hilbert"], pred[i]["rle"])
+            if not searched2[i] & 2:
+                assert (m["gr_k"], m["pfd_width"]) == (pred[i]["gr_k"], pred[i]["pfd_width"])
+        # k and width are the exact optimum of the chosen curve x RLE variant
+        assert len(pre2[i]) == res[i]["variants"][var]["bytes"], i
         want = O.compress_mode(c, o["quantizer"], o["l2_bound"], bool(m["is_hilbert"]),
                                bool(m["rle"]), gr_k=m["gr_k"], pfd_width=m["pfd_width"])
         assert pre2[i] == want, i
+
+
+def _mixed_chunks():
+    """Terrain and shell chunks, a small direct-mode shell, and a TO cloud in
+    direct mode whose minimum-z point takes the wrapped z = -1 node (its
+    sfc delta is near 2^63: a Golomb-Rice k below ~40 would cost more than
+    2^23 bytes)."""
+    from xnyzip_amd import datasets as D
+    ter = D.terrain(1 << 18, seed=5)
+    she = D.shells(1 << 18, seed=6, count=40)
+    d = float(np.float32(np.float64(np.float32(1.0)) / np.sqrt(5.0)))
+    w = _uniform(21, 3000, scale=100 * d)
+    w[17] = [d, d, 0.0]
+    w[:, 2] += 1.0
+    w[17, 2] = 1.0
+    w[200] = [0.0, 0.0, 3.0]
+    return [(ter[:100000], dict(quantizer=O.TO, l2_bound=0.01)),
+            (ter[100000:160000], dict(quantizer=O.CUBE, l2_bound=0.1)),
+            (she[:70000], dict(quantizer=O.TO, l2_bound=0.05)),
+            (she[70000:72000], dict(quantizer=O.TO, l2_bound=0.1)),
+            (_uniform(13, 6000, 20.0), dict(quantizer=O.TO, l2_bound=0.1)),
+            (w, dict(quantizer=O.TO, l2_bound=0.5, direct_threshold=1 << 20))]
+
+
+def test_packaged_selector_streams(X):
+    """The packaged selector parameters (as used by the selector benchmark) on
+    mixed chunks, through the caller-memory entry point reusing one context
+    after search-only calls: every stream is the oracle's for the chosen mode,
+    k / width are the variant's exact optimum, and no cloud blows up."""
+    from xnyzip_amd import selector
+    params = selector.load_params()
+    cases = _mixed_chunks()
+    pts = torch.from_numpy(np.concatenate([c for c, _ in cases])).cuda()
+    offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in cases])]).tolist()
+    sopts = [dict(o) for _, o in cases]
+    opts = [dict(o, frame_mode=2) for _, o in cases]
+    ctx = X.Context(0)
+    res = X.search_batch(pts, offs, sopts, ctx=ctx)
+    _, _, _, _, s0 = X.compress_selected_batch(pts, offs, opts, ctx=ctx)
+    assert list(s0) == [1] * len(cases)
+    out = np.zeros(64 << 20, np.uint8)
+    for rep in range(2):
+        ro, st, modes, searched = X.compress_selected_batch_into(
+            pts, offs, opts, out, params=params, min_confidence=0.5, ctx=ctx)
+        total = best = 0
+        for i, ((c, o), m) in enumerate(zip(cases, modes)):
+            pre = bytes(out[int(ro[i]):int(ro[i + 1])])
+            var = 2 * m["is_hilbert"] + m["rle"]
+            assert len(pre) == res[i]["variants"][var]["bytes"], (rep, i)
+            assert len(pre) <= 1.02 * res[i]["reference_bytes"], (rep, i, m)
+            want = O.compress_mode(c, o["quantizer"], o["l2_bound"], bool(m["is_hilbert"]),
+                                   bool(m["rle"]), o.get("direct_threshold", 1024),
+                                   gr_k=m["gr_k"], pfd_width=m["pfd_width"])
+            assert pre == want, (rep, i)
+            total += len(pre)
+            best += res[i]["best_bytes"]
+        print(f"selected {total} B vs search optimum {best} B "
+              f"({100.0 * (total / best - 1):+.3f}%), searched {list(searched)}")
+    assert st[-1].is_direct and st[-1].wrapped_points == 1

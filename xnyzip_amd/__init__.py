@@ -24,7 +24,7 @@ from ._lib import Context, XnyzError  # noqa: F401
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
            "compress", "compress_batch", "compress_batch_into", "host_buffer", "search_batch", "features_batch", "mlp_train",
-           "mlp_predict", "compress_selected_batch", "quantize", "morton_to_hilbert", "autotune"]
+           "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -351,6 +351,29 @@ def compress_selected_batch(points_device, offsets, opts, params=None, min_confi
     out_frames = [_lib.take(f) for f in frames]
     out_pres = [_lib.take(b) for b in pres] if pres is not None else None
     return out_frames, out_pres, list(stats), [m.as_dict() for m in modes], searched
+
+
+def compress_selected_batch_into(points_device, offsets, opts, out: np.ndarray, params=None,
+                                 min_confidence=0.0, ctx: Context | None = None):
+    """compress_selected_batch writing the outputs back to back into `out`
+    (a contiguous uint8 array, reused across calls; layout as
+    compress_batch_into).  Returns (offsets, stats, modes, searched)."""
+    t, count, off, arr = _batch_args(points_device, offsets, opts, _lib.FRAME_ZSTD3)
+    if out.dtype != np.uint8 or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint8 array")
+    ctx = ctx or default_context(t.device.index or 0)
+    p = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+    res = np.zeros(count + 1, dtype=np.uint64)
+    stats = (_lib.Stats * count)()
+    modes = (_lib.Mode * count)()
+    searched = np.zeros(count, np.int32)
+    _lib.check(_lib.lib().xnyz_compress_selected_batch_device_into(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(p.ctypes.data) if p is not None else None,
+        float(min_confidence), C.c_void_p(out.ctypes.data), out.size,
+        C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p), C.cast(modes, C.c_void_p),
+        C.c_void_p(searched.ctypes.data)))
+    return res, list(stats), [m.as_dict() for m in modes], searched
 
 
 class BlockCompressor:

@@ -102,6 +102,7 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress_batch_device_into",
            "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
+           "xnyz_compress_selected_batch_device_into",
            "xnyz_quantize_device",
            "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_graphs_enable",
            "xnyz_tuning_set", "xnyz_tuning_get", "xnyz_tuning_save", "xnyz_tuning_load",
@@ -136,6 +137,9 @@ def lib() -> C.CDLL:
         L.xnyz_mlp_predict.argtypes = [P, P, P, C.c_uint64, P, P]
         L.xnyz_compress_selected_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, C.c_float,
                                                           P, P, P, P, P]
+        L.xnyz_compress_selected_batch_device_into.argtypes = [P, P, P, C.c_uint32, P, P,
+                                                               C.c_float, P, C.c_uint64, P, P,
+                                                               P, P]
         L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]

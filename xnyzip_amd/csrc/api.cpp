@@ -68,6 +68,7 @@ int set_error(int code, const std::string& msg) {
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
+  const char* name = "device buffer";
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
     if (p) check_hip(hipFree(p), "hipFree");
@@ -77,7 +78,9 @@ struct DBuf {
     if (hipMalloc(&p, want) != hipSuccess) {
       (void)hipGetLastError();
       p = nullptr;
-      throw Error(XNYZ_ENOMEM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+      throw Error(XNYZ_ENOMEM, std::string("hipMalloc of ") + std::to_string(want) +
+                                   " bytes for " + name + " (" + std::to_string(bytes) +
+                                   " requested) failed");
     }
     cap = want;
   }
@@ -95,6 +98,7 @@ struct DBuf {
 struct HBuf {  // pinned host memory
   void* p = nullptr;
   size_t cap = 0;
+  const char* name = "host buffer";
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
     if (p) check_hip(hipHostFree(p), "hipHostFree");
@@ -104,7 +108,9 @@ struct HBuf {  // pinned host memory
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
       (void)hipGetLastError();
       p = nullptr;
-      throw Error(XNYZ_ENOMEM, "hipHostMalloc of " + std::to_string(want) + " bytes failed");
+      throw Error(XNYZ_ENOMEM, std::string("hipHostMalloc of ") + std::to_string(want) +
+                                   " bytes for " + name + " (" + std::to_string(bytes) +
+                                   " requested) failed");
     }
     cap = want;
   }
@@ -321,6 +327,23 @@ struct xnyz_ctx {
   hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
+
+  // Every buffer's member name, for allocation errors.
+  void name_buffers() {
+#define XNYZ_NAME(b) b.name = #b
+    XNYZ_NAME(tiles); XNYZ_NAME(sp); XNYZ_NAME(sc); XNYZ_NAME(sh); XNYZ_NAME(lut);
+    XNYZ_NAME(bigv); XNYZ_NAME(bigc); XNYZ_NAME(large); XNYZ_NAME(nlarge);
+    XNYZ_NAME(keys[0]); XNYZ_NAME(keys[1]); XNYZ_NAME(lo[0]); XNYZ_NAME(lo[1]);
+    XNYZ_NAME(cells); XNYZ_NAME(cells_lo); XNYZ_NAME(pos); XNYZ_NAME(cnt); XNYZ_NAME(out);
+    XNYZ_NAME(arena); XNYZ_NAME(upload); XNYZ_NAME(tmp); XNYZ_NAME(idx[0]); XNYZ_NAME(idx[1]);
+    XNYZ_NAME(lo64[0]); XNYZ_NAME(lo64[1]); XNYZ_NAME(feat); XNYZ_NAME(patch); XNYZ_NAME(ztab);
+    XNYZ_NAME(zscr); XNYZ_NAME(zout); XNYZ_NAME(zpos); XNYZ_NAME(mP); XNYZ_NAME(mWh);
+    XNYZ_NAME(mG); XNYZ_NAME(mM); XNYZ_NAME(mV); XNYZ_NAME(mX); XNYZ_NAME(mY);
+    XNYZ_NAME(mIdx); XNYZ_NAME(mScratch); XNYZ_NAME(mLoss); XNYZ_NAME(mOut); XNYZ_NAME(mConf);
+    XNYZ_NAME(h_tiles); XNYZ_NAME(h_sp); XNYZ_NAME(h_sc); XNYZ_NAME(h_sh); XNYZ_NAME(h_lut);
+    XNYZ_NAME(h_out); XNYZ_NAME(h_feat); XNYZ_NAME(h_zout); XNYZ_NAME(h_ztab); XNYZ_NAME(h_zpos);
+#undef XNYZ_NAME
+  }
 
   void sync() {
     check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -722,6 +745,16 @@ struct CallOut {
   uint8_t* dst = nullptr;
   u64 cap = 0;
   uint64_t* dst_off = nullptr;
+  // Per segment (nullable): settle the Golomb-Rice k and PFD width of the
+  // segment's curve x RLE variant from its own statistics with the exact
+  // size model (modes.hpp) -- the statistics pass runs at the reference limit
+  // 254 and also collects the patch bit populations.  The opts' k and width
+  // stay when they cost no more than the optimum, else the optimum replaces
+  // them.  The mode used lands in modes_out[s] (nullable), and
+  // corrected[s] (nullable) says whether the opts' k / width were replaced.
+  const char* refine = nullptr;
+  xnyz_mode* modes_out = nullptr;
+  char* corrected = nullptr;
 };
 
 // HyperLogLog estimate from 256 registers (small-range correction by linear
@@ -811,8 +844,11 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       sp[s].qp = quant_params(&opts[s]);
       sp[s].rle = opts[s].rle != 0;
       sp[s].diag = sp[s].rle && !co.search;
-      sp[s].lim = (opts[s].mode_flags & XNYZ_MODE_PFD_WIDTH) ? pfd_limit(opts[s].pfd_width) : 254;
-      sp[s].search = co.search != nullptr;
+      const bool refine = co.refine && co.refine[s];
+      sp[s].lim = (opts[s].mode_flags & XNYZ_MODE_PFD_WIDTH) && !refine
+                      ? pfd_limit(opts[s].pfd_width)
+                      : 254;
+      sp[s].search = co.search != nullptr || refine;
       for (u32 l = 0; l < sp[s].ntiles; ++l) tiles[t++] = Tile{s, l};
       g[s].n = n;
       g[s].rle = sp[s].rle;
@@ -1013,6 +1049,20 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   }
 
   // ---- host: codebooks per segment (in parallel) ----------------------------
+  // Every stream byte outside the repos / sfc-delta sections (header, trailer,
+  // block, quads and RLE sections with their size fields), once the other
+  // codebooks are built.
+  auto fixed_bytes = [](const SegHost& h) {
+    auto sect = [](u64 meta, u64 bits) { return 16 + meta + (bits + 7) / 8; };
+    u64 fixed = 30 + 12;
+    if (!h.direct) {
+      fixed += sect(5, h.B * h.w1) + sect(5, h.B * h.w2) + 16 +
+               sect(h.cb_quad.meta().size(), h.cb_quad.data_bits());
+    }
+    if (h.rle) fixed += sect(h.cb_cnt.meta().size(), h.cb_cnt.data_bits());
+    return fixed;
+  };
+  std::vector<xnyz_mode> used(S);
   parallel_for(S, threads, [&](size_t s) {
     SegHost& h = g[s];
     const SegCtl& c = sc[s];
@@ -1021,16 +1071,12 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     h.ncoded = c.ne;
     h.B = h.direct ? 0 : c.nb;
     h.P = c.n_patches;
-    h.cb_sym.build(sym_counts(hh.sym, hh.first_sym, kSymBins), h.ncoded);
-    fill_lut(h.cb_sym, lut[s].sym, kSymBins);
     if (!h.direct) {
       h.cb_quad.build(sym_counts(hh.quad, hh.first_quad, kSymBins), h.ncoded);
       fill_lut(h.cb_quad, lut[s].quad, kSymBins);
       h.w1 = c.max_blkd ? bitlen(c.max_blkd) : 1;
       h.w2 = c.max_cntd ? bitlen(c.max_cntd) : 1;
     }
-    h.grk = (opts[s].mode_flags & XNYZ_MODE_GR_K) ? (uint8_t)opts[s].gr_k
-                                                   : golomb_rice_k(h.P, c.patch_sum, false);
     if (h.rle) {
       std::vector<SymCount> cs = sym_counts(hh.cnt, hh.first_cnt, kCntBinsSeg);
       std::sort(h.large.begin(), h.large.end());
@@ -1060,27 +1106,64 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
         }
       }
     }
+    // The repos / sfc-delta symbols and patches (p_for_delta_encoder.hpp:27-43).
+    std::vector<SymCount> syms = sym_counts(hh.sym, hh.first_sym, kSymBins);
+    h.grk = (opts[s].mode_flags & XNYZ_MODE_GR_K) ? (uint8_t)opts[s].gr_k
+                                                   : golomb_rice_k(h.P, c.patch_sum, false);
+    used[s] = xnyz_mode{h.hil, h.rle, h.grk, (opts[s].mode_flags & XNYZ_MODE_PFD_WIDTH)
+                                                  ? opts[s].pfd_width : 8};
+    if (co.refine && co.refine[s]) {
+      // Statistics at limit 254: cost the asked (k, width) and the optimum
+      // exactly, keep the asked one unless it costs more.
+      ModeStats m;
+      for (u32 i = 0; i < kSymBins; ++i) m.hist[i] = hh.sym[i];
+      for (u32 b = 0; b < 64; ++b) m.bitpop[b] = hh.bitpop[b];
+      m.patch_sum = c.patch_sum;
+      m.fixed_bytes = fixed_bytes(h);
+      ModeChoice ref{}, best{};
+      search_modes(m, ref, best);
+      const int ak = used[s].gr_k, aw = used[s].pfd_width;
+      const bool keep = mode_cost(m, ak, aw).bytes <= best.bytes;
+      const int k = keep ? ak : best.k, w = keep ? aw : best.w;
+      if (co.corrected) co.corrected[s] = !keep;
+      // Fold the values in (lim(w), 254] into the patch symbol 0: they become
+      // patches, and symbol 0 first occurs where the first of them does.
+      const u32 lim = pfd_limit(w);
+      std::vector<SymCount> folded;
+      SymCount zero{0, 0, ~0ull};
+      for (const SymCount& x : syms) {
+        if (x.sym == 0 || x.sym - 1 > lim) {
+          zero.freq += x.freq;
+          zero.first = std::min(zero.first, x.first);
+          if (x.sym) h.P += x.freq;
+        } else {
+          folded.push_back(x);
+        }
+      }
+      if (zero.freq) folded.insert(folded.begin(), zero);
+      syms.swap(folded);
+      sp[s].lim = lim;
+      h.grk = (uint8_t)k;
+      used[s].gr_k = k;
+      used[s].pfd_width = w;
+    }
+    h.cb_sym.build(syms, h.ncoded);
+    fill_lut(h.cb_sym, lut[s].sym, kSymBins);
     if (h.cb_sym.max_length() > 56 || h.cb_quad.max_length() > 56 ||
         h.cb_cnt.max_length() > 56)
       throw Error(XNYZ_EUNSUPPORTED, "Huffman code longer than 56 bits");
   });
+  if (co.modes_out)
+    for (u32 s = 0; s < S; ++s) co.modes_out[s] = used[s];
   if (co.search) {  // size model per segment (modes.hpp)
     co.search->resize(S);
     for (u32 s = 0; s < S; ++s) {
-      const SegHost& h = g[s];
       const SegHist& hh = shh[s];
       ModeStats& m = (*co.search)[s];
       for (u32 i = 0; i < kSymBins; ++i) m.hist[i] = hh.sym[i];
       for (u32 b = 0; b < 64; ++b) m.bitpop[b] = hh.bitpop[b];
       m.patch_sum = sc[s].patch_sum;
-      auto sect = [](u64 meta, u64 bits) { return 16 + meta + (bits + 7) / 8; };
-      u64 fixed = 30 + 12;
-      if (!h.direct) {
-        fixed += sect(5, h.B * h.w1) + sect(5, h.B * h.w2) + 16 +
-                 sect(h.cb_quad.meta().size(), h.cb_quad.data_bits());
-      }
-      if (h.rle) fixed += sect(h.cb_cnt.meta().size(), h.cb_cnt.data_bits());
-      m.fixed_bytes = fixed;
+      m.fixed_bytes = fixed_bytes(g[s]);
     }
     return XNYZ_OK;
   }
@@ -1576,6 +1659,7 @@ int xnyz_create(int device, xnyz_ctx** out) {
     if (device < 0 || device >= count) throw Error(XNYZ_EINVAL, "no such HIP device");
     auto* c = new xnyz_ctx();
     c->device = device;
+    c->name_buffers();
     for (xnyz_tuning& t : c->tuning) t = default_tuning();
     try {
       use_device(c);
@@ -1755,6 +1839,72 @@ void predict_impl(xnyz_ctx* ctx, const float* params, const float* features, u64
   }
 }
 
+// Batched compression with a mode per cloud: the selector's curve x RLE
+// where it is confident, the exhaustive search's optimum elsewhere.  Every
+// cloud's Golomb-Rice k and PFD width are then settled exactly from the
+// statistics of the compressing pass itself (CallOut::refine): the
+// selector's k / width stay where the size model finds them optimal.
+void selected_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off,
+                   const xnyz_opts* opts, const float* params, float min_confidence,
+                   CallOut co, xnyz_mode* chosen, int32_t* searched) {
+  const u32 count = (u32)(off.size() - 1);
+  for (u32 s = 0; s < count; ++s) validate(&opts[s], off[s + 1] - off[s]);
+  std::vector<xnyz_mode> mode(count);
+  std::vector<char> fall(count, 1);
+  if (params) {  // features -> MLP
+    std::vector<float> f;
+    CallOut fo{nullptr, nullptr, nullptr, nullptr};
+    fo.features = &f;
+    compress_impl(ctx, d_xyz, off, opts, fo, now_ms());
+    std::vector<MlpLabel> out;
+    std::vector<float> conf;
+    predict_impl(ctx, params, f.data(), count, out, conf);
+    for (u32 s = 0; s < count; ++s) {
+      mode[s] = xnyz_mode{out[s].curve, out[s].rle, out[s].k, out[s].w + 1};
+      fall[s] = conf[s] < min_confidence;
+    }
+  }
+  std::vector<u32> fb;
+  for (u32 s = 0; s < count; ++s)
+    if (fall[s]) fb.push_back(s);
+  if (!fb.empty()) {  // exhaustive search of the fallback clouds, gathered
+    const bool all = fb.size() == count;
+    std::vector<u64> so{0};
+    std::vector<xnyz_opts> sopt;
+    const float* src = d_xyz;
+    if (!all) {
+      u64 tot = 0;
+      for (u32 s : fb) tot += off[s + 1] - off[s];
+      ctx->tmp.ensure((size_t)tot * 12);
+      for (u32 s : fb) {
+        const u64 n = off[s + 1] - off[s];
+        check_hip(hipMemcpyAsync(ctx->tmp.as<float>() + 3 * so.back(), d_xyz + 3 * off[s],
+                                 n * 12, hipMemcpyDeviceToDevice, ctx->stream), "gather");
+        so.push_back(so.back() + n);
+      }
+      src = ctx->tmp.as<float>();
+    } else {
+      so = off;
+    }
+    for (u32 s : fb) sopt.push_back(opts[s]);
+    std::vector<xnyz_search_result> r(fb.size());
+    search_impl(ctx, src, so, sopt.data(), r.data());
+    for (size_t i = 0; i < fb.size(); ++i) mode[fb[i]] = r[i].best;
+  }
+  std::vector<xnyz_opts> vo(count);
+  for (u32 s = 0; s < count; ++s) vo[s] = with_mode(opts[s], mode[s]);
+  std::vector<char> refine(count, 1), corrected(count, 0);
+  std::vector<xnyz_mode> used(count);
+  co.refine = refine.data();
+  co.modes_out = used.data();
+  co.corrected = corrected.data();
+  compress_impl(ctx, d_xyz, off, vo.data(), co, now_ms());
+  for (u32 s = 0; s < count; ++s) {
+    if (chosen) chosen[s] = used[s];
+    if (searched) searched[s] = (fall[s] ? 1 : 0) | (corrected[s] ? 2 : 0);
+  }
+}
+
 }  // namespace
 
 int xnyz_mlp_train(xnyz_ctx* ctx, const float* features, const xnyz_mode* labels, uint64_t n,
@@ -1867,56 +2017,28 @@ int xnyz_compress_selected_batch_device(xnyz_ctx* ctx, const float* d_xyz,
       throw Error(XNYZ_EINVAL, "null argument or empty batch");
     std::vector<u64> off = rebase(offsets, count, d_xyz);
     use_device(ctx);
-    for (u32 s = 0; s < count; ++s) validate(&opts[s], off[s + 1] - off[s]);
-    std::vector<xnyz_mode> mode(count);
-    std::vector<char> fall(count, 1);
-    if (params) {  // features -> MLP
-      std::vector<float> f;
-      CallOut co{nullptr, nullptr, nullptr, nullptr};
-      co.features = &f;
-      compress_impl(ctx, d_xyz, off, opts, co, now_ms());
-      std::vector<MlpLabel> out;
-      std::vector<float> conf;
-      predict_impl(ctx, params, f.data(), count, out, conf);
-      for (u32 s = 0; s < count; ++s) {
-        mode[s] = xnyz_mode{out[s].curve, out[s].rle, out[s].k, out[s].w + 1};
-        fall[s] = conf[s] < min_confidence;
-      }
-    }
-    std::vector<u32> fb;
-    for (u32 s = 0; s < count; ++s)
-      if (fall[s]) fb.push_back(s);
-    if (!fb.empty()) {  // exhaustive search of the fallback clouds, gathered
-      const bool all = fb.size() == count;
-      std::vector<u64> so{0};
-      std::vector<xnyz_opts> sopt;
-      const float* src = d_xyz;
-      if (!all) {
-        u64 tot = 0;
-        for (u32 s : fb) tot += off[s + 1] - off[s];
-        ctx->tmp.ensure((size_t)tot * 12);
-        for (u32 s : fb) {
-          const u64 n = off[s + 1] - off[s];
-          check_hip(hipMemcpyAsync(ctx->tmp.as<float>() + 3 * so.back(), d_xyz + 3 * off[s],
-                                   n * 12, hipMemcpyDeviceToDevice, ctx->stream), "gather");
-          so.push_back(so.back() + n);
-        }
-        src = ctx->tmp.as<float>();
-      } else {
-        so = off;
-      }
-      for (u32 s : fb) sopt.push_back(opts[s]);
-      std::vector<xnyz_search_result> r(fb.size());
-      search_impl(ctx, src, so, sopt.data(), r.data());
-      for (size_t i = 0; i < fb.size(); ++i) mode[fb[i]] = r[i].best;
-    }
-    std::vector<xnyz_opts> vo(count);
-    for (u32 s = 0; s < count; ++s) vo[s] = with_mode(opts[s], mode[s]);
-    compress_impl(ctx, d_xyz, off, vo.data(), CallOut{frames, prezstd, stats, nullptr}, now_ms());
-    for (u32 s = 0; s < count; ++s) {
-      if (chosen) chosen[s] = mode[s];
-      if (searched) searched[s] = fall[s];
-    }
+    selected_impl(ctx, d_xyz, off, opts, params, min_confidence,
+                  CallOut{frames, prezstd, stats, nullptr}, chosen, searched);
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_compress_selected_batch_device_into(xnyz_ctx* ctx, const float* d_xyz,
+                                             const uint64_t* offsets, uint32_t count,
+                                             const xnyz_opts* opts, const float* params,
+                                             float min_confidence, uint8_t* out, uint64_t cap,
+                                             uint64_t* out_offsets, xnyz_stats* stats,
+                                             xnyz_mode* chosen, int32_t* searched) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !out_offsets || (!out && cap))
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    use_device(ctx);
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    selected_impl(ctx, d_xyz, off, opts, params, min_confidence, co, chosen, searched);
     return XNYZ_OK;
   });
 }

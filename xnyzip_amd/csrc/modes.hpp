@@ -93,15 +93,23 @@ inline WidthCost width_cost(const ModeStats& m, int w) {
                    psum};
 }
 
-// The Golomb-Rice section (with its two size fields) at width w and k.
+// Sizes at or above this are out of reach (no stream of that size can be
+// built); the model saturates there instead of wrapping.
+constexpr uint64_t kHugeBytes = 1ull << 62;
+
+// The Golomb-Rice section (with its two size fields) at width w and k.  The
+// unary part is summed in 128 bits and saturates at kHugeBytes: a patch near
+// 2^63 (the direct-mode delta of a wrapped z = -1 node) costs ~2^63 bits at
+// k = 0, which a 64-bit sum would wrap into a small, wrong size.
 inline uint64_t gr_bytes(const ModeStats& m, const WidthCost& wc, int w, int k) {
   if (!wc.patches) return 16;
   const uint32_t lim = pfd_limit(w);
-  uint64_t shifted = 0;
-  for (int b = k; b < 64; ++b) shifted += m.bitpop[b] << (b - k);
-  for (uint32_t v = lim + 1; v <= 254; ++v) shifted += m.hist[v + 1] * (uint64_t)(v >> k);
-  const uint64_t bits = wc.patches * (uint64_t)(k + 1) + shifted;
-  return 16 + 5 + (bits + 7) / 8;
+  unsigned __int128 shifted = 0;
+  for (int b = k; b < 64; ++b) shifted += (unsigned __int128)m.bitpop[b] << (b - k);
+  for (uint32_t v = lim + 1; v <= 254; ++v) shifted += (unsigned __int128)m.hist[v + 1] * (v >> k);
+  const unsigned __int128 bits = (unsigned __int128)wc.patches * (uint64_t)(k + 1) + shifted;
+  const unsigned __int128 bytes = 16 + 5 + (bits + 7) / 8;
+  return bytes >= kHugeBytes ? kHugeBytes : (uint64_t)bytes;
 }
 
 // Stream bytes with Golomb-Rice k and PFD width w.
@@ -117,8 +125,9 @@ struct ModeChoice {
 };
 
 // The reference's choice (k by its rule, w = 8) and the exhaustive optimum
-// over w in 1..8 and k in 0..31 plus each width's rule k; ties keep the
-// smaller w, then the smaller k.
+// over w in 1..8 and k in 0..62 (every k the Golomb-Rice meta can carry,
+// golomb_rice_encoder.hpp:121-123 clamps to 62); ties keep the smaller w,
+// then the smaller k.
 inline void search_modes(const ModeStats& m, ModeChoice& ref, ModeChoice& best) {
   best = ModeChoice{0, 0, ~0ull};
   for (int w = 1; w <= 8; ++w) {
@@ -133,8 +142,7 @@ inline void search_modes(const ModeStats& m, ModeChoice& ref, ModeChoice& best) 
       consider(rule_k);
       continue;
     }
-    for (int k = 0; k < 32; ++k) consider(k);
-    if (rule_k >= 32) consider(rule_k);
+    for (int k = 0; k <= 62; ++k) consider(k);
   }
 }
 

@@ -11,6 +11,11 @@ float32 arrays of interleaved x, y, z, and run on the host.
 * shells(n, seed, count): `count` spherical shells, centres U[0, 2000)^3,
   radii U(20, 60), points uniform on each surface with N(0, 1) radial
   thickness; each shell's points contiguous, shells in random order.
+* uneven_cloud(i) / uneven_opts(i): cloud i of the batched uneven workload
+  (size log-uniform in [1K, 4M] from seed i; terrain for even i, shells for
+  odd i; L2 bound {0.001, 0.01, 0.1}[i mod 3]).
+* selector_workload(n): half terrain, half shells, cut into chunks of
+  2^U(16, 20) points with L2 bounds {0.001, 0.01, 0.1} in turn.
 """
 from __future__ import annotations
 
@@ -98,3 +103,32 @@ def shells(n: int, seed: int = 0, count: int = 2000, box: float = 2000.0) -> np.
         out[pos:pos + m] = (centres[k] + v * r[:, None]).astype(np.float32)
         pos += m
     return out
+
+
+def uneven_size(i: int) -> int:
+    rng = np.random.default_rng(i)
+    return int(np.exp(rng.uniform(np.log(1e3), np.log(4e6))))
+
+
+def uneven_cloud(i: int, threads: int = 4) -> np.ndarray:
+    n = uneven_size(i)
+    if i % 2 == 0:
+        return terrain(n, seed=i, threads=threads)
+    return shells(n, seed=i, count=max(1, min(2000, n // 1000)))
+
+
+def uneven_opts(i: int) -> dict:
+    return dict(quantizer=1, l2_bound=[0.001, 0.01, 0.1][i % 3])
+
+
+def selector_workload(n: int):
+    """(points, chunk offsets, per-chunk opts) of the selector workload."""
+    half = n // 2
+    pts = np.concatenate([terrain(half, seed=0), shells(half, seed=1)])
+    rng = np.random.default_rng(0)
+    offs = [0]
+    while offs[-1] < len(pts):
+        offs.append(min(len(pts), offs[-1] + int(2.0 ** rng.uniform(16, 20))))
+    opts = [dict(quantizer=1, l2_bound=[0.001, 0.01, 0.1][i % 3])
+            for i in range(len(offs) - 1)]
+    return pts, offs, opts

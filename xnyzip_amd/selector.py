@@ -151,7 +151,7 @@ def main() -> int:
                                                                 want_prezstd=True)
             sel += sum(len(p) for p in pres)
             best += sum(r["best_bytes"] for _, _, r in b)
-            fell += int(np.sum(searched))
+            fell += int(np.sum(searched & 1))
         log(f"min_confidence {mc}: stream bytes vs exhaustive optimum {sel / best - 1:+.4%}, "
             f"fell back on {fell}/{len(echunks)} chunks")
     return 0
