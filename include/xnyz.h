@@ -193,6 +193,30 @@ int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uin
                                     uint32_t count, const xnyz_opts* opts, uint8_t* out,
                                     uint64_t cap, uint64_t* out_offsets, xnyz_stats* stats);
 
+/* xnyz_compress_batch_device_into without allocating: no hipMalloc,
+ * hipFree, hipHostMalloc or graph instantiation during the call.  No
+ * reference counterpart; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Every device buffer of the pipeline is carved from one file-scope device
+ * array of XNYZ_NOALLOC_SCRATCH_BYTES per GPU (held from the library's load
+ * until the process ends; its address is looked up by xnyz_create), and the
+ * host-side tables from a pinned block xnyz_create sets up.  Limits: at most
+ * XNYZ_NOALLOC_MAX_CLOUDS clouds and XNYZ_NOALLOC_MAX_POINTS points per call,
+ * opts[s].frame_mode XNYZ_FRAME_NONE (the assembled streams, back to back in
+ * `out` as out_offsets says); beyond them, or when a call's buffers (keys,
+ * cells, stream) outgrow the array, it returns XNYZ_EUNSUPPORTED and
+ * launches nothing.  XNYZ_ENOSPC (size needed in out_offsets[count]) as for
+ * the _into form.  All calls on one GPU share its array: calls that overlap
+ * on one GPU (from any context) race and must be serialised by the caller.
+ * Streams are the same bytes xnyz_compress_batch_device_into returns. */
+#define XNYZ_NOALLOC_SCRATCH_BYTES (256ull << 20)
+#define XNYZ_NOALLOC_MAX_POINTS (1ull << 22)
+#define XNYZ_NOALLOC_MAX_CLOUDS 256
+int xnyz_compress_batch_device_noalloc(xnyz_ctx* ctx, const float* d_xyz,
+                                       const uint64_t* offsets, uint32_t count,
+                                       const xnyz_opts* opts, uint8_t* out, uint64_t cap,
+                                       uint64_t* out_offsets, xnyz_stats* stats);
+
 /* Exhaustive per-chunk mode search; no reference counterpart (the
  * reference codes every chunk with the options it is given).  For each cloud
  * of the batch (layout as xnyz_compress_batch_device; opts supply quantizer,

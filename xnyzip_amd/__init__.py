@@ -23,7 +23,7 @@ from ._lib import Context, XnyzError  # noqa: F401
 
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
-           "compress", "compress_batch", "compress_batch_into", "host_buffer", "search_batch", "features_batch", "mlp_train",
+           "compress", "compress_batch", "compress_batch_into", "compress_batch_noalloc", "host_buffer", "search_batch", "features_batch", "mlp_train",
            "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune"]
 
 
@@ -174,6 +174,24 @@ def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
         C.cast(arr, C.c_void_p), C.c_void_p(out.ctypes.data), out.size,
         C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p))
     _lib.check(rc)
+    return res, list(stats)
+
+
+def compress_batch_noalloc(points_device, offsets, opts, out: np.ndarray,
+                           ctx: Context | None = None):
+    """compress_batch_into without any allocation by the library during the
+    call (assembled streams only; at most 2^22 points and 256 clouds; calls
+    on one GPU must not overlap).  Returns (offsets, stats)."""
+    t, count, off, arr = _batch_args(points_device, offsets, opts, _lib.FRAME_NONE)
+    if out.dtype != np.uint8 or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint8 array")
+    ctx = ctx or default_context(t.device.index or 0)
+    res = np.zeros(count + 1, dtype=np.uint64)
+    stats = (_lib.Stats * count)()
+    _lib.check(_lib.lib().xnyz_compress_batch_device_noalloc(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(out.ctypes.data), out.size,
+        C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p)))
     return res, list(stats)
 
 

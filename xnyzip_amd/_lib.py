@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libxnyz.so")
 
 XNYZ_OK = 0
 XNYZ_EINVAL = -1
+XNYZ_EUNSUPPORTED = -5
 XNYZ_ENOSPC = -8
 XNYZ_EIO = -9
 CUBE, TRUNCATED_OCTAHEDRON = 0, 1
@@ -99,7 +100,7 @@ _lock = threading.Lock()
 
 EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress", "xnyz_compress_device", "xnyz_compress_batch_device",
-           "xnyz_compress_batch_device_into",
+           "xnyz_compress_batch_device_into", "xnyz_compress_batch_device_noalloc",
            "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_compress_selected_batch_device_into",
@@ -130,6 +131,8 @@ def lib() -> C.CDLL:
         L.xnyz_compress_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, P, P]
         L.xnyz_compress_batch_device_into.argtypes = [P, P, P, C.c_uint32, P, P, C.c_uint64, P,
                                                       P]
+        L.xnyz_compress_batch_device_noalloc.argtypes = [P, P, P, C.c_uint32, P, P, C.c_uint64,
+                                                         P, P]
         L.xnyz_search_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
         L.xnyz_features_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
         L.xnyz_mlp_train.argtypes = [P, P, P, C.c_uint64, C.POINTER(TrainOpts), P,

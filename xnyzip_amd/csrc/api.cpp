@@ -58,6 +58,10 @@ using namespace xnyz;
 
 namespace {
 
+// Host staging of a no-allocation call (per-segment parameters, statistics
+// and codebook tables for up to XNYZ_NOALLOC_MAX_CLOUDS clouds).
+constexpr size_t kNoAllocHostBytes = size_t(4) << 20;
+
 thread_local std::string g_last_error;
 
 int set_error(int code, const std::string& msg) {
@@ -65,12 +69,41 @@ int set_error(int code, const std::string& msg) {
   return code;
 }
 
+// Fixed regions that DBuf / HBuf carve from instead of allocating, while a
+// no-allocation call runs on this thread (xnyz_compress_batch_device_noalloc):
+// the device scratch array and the context's host staging block.  Each
+// buffer the call asks for gets a fresh 256-byte aligned slice; running out
+// is an error, not an allocation.
+struct Bump {
+  char* dev;
+  size_t dcap, doff;
+  char* host;
+  size_t hcap, hoff;
+  static void* take(char* base, size_t cap, size_t& off, size_t bytes, const char* name,
+                    const char* what) {
+    const size_t b = (bytes + 255) & ~size_t(255);
+    if (off + b > cap)
+      throw Error(XNYZ_EUNSUPPORTED, std::string("no-allocation ") + what + " of " +
+                                         std::to_string(cap >> 20) + " MiB exhausted by " +
+                                         name + " (" + std::to_string(bytes) + " bytes)");
+    void* p = base + off;
+    off += b;
+    return p;
+  }
+};
+thread_local Bump* g_bump = nullptr;
+
 struct DBuf {
   void* p = nullptr;
   size_t cap = 0;
   const char* name = "device buffer";
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
+    if (g_bump) {
+      p = Bump::take(g_bump->dev, g_bump->dcap, g_bump->doff, bytes, name, "device scratch");
+      cap = bytes;
+      return;
+    }
     if (p) check_hip(hipFree(p), "hipFree");
     p = nullptr;
     cap = 0;
@@ -101,6 +134,11 @@ struct HBuf {  // pinned host memory
   const char* name = "host buffer";
   void ensure(size_t bytes) {
     if (bytes <= cap) return;
+    if (g_bump) {
+      p = Bump::take(g_bump->host, g_bump->hcap, g_bump->hoff, bytes, name, "host staging");
+      cap = bytes;
+      return;
+    }
     if (p) check_hip(hipHostFree(p), "hipHostFree");
     p = nullptr;
     cap = 0;
@@ -327,6 +365,20 @@ struct xnyz_ctx {
   hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
+  // No-allocation calls: the device address of this GPU's scratch array and a
+  // pinned host staging block, both set up by xnyz_create.
+  void* na_dev = nullptr;
+  HBuf na_host;
+
+  std::vector<DBuf*> dbufs() {
+    return {&tiles, &sp, &sc, &sh, &lut, &bigv, &bigc, &large, &nlarge, &keys[0], &keys[1],
+            &lo[0], &lo[1], &cells, &cells_lo, &pos, &cnt, &out, &arena, &upload, &tmp,
+            &idx[0], &idx[1], &lo64[0], &lo64[1], &feat, &patch, &ztab, &zscr, &zout, &zpos,
+            &mP, &mWh, &mG, &mM, &mV, &mX, &mY, &mIdx, &mScratch, &mLoss, &mOut, &mConf};
+  }
+  std::vector<HBuf*> hbufs() {
+    return {&h_tiles, &h_sp, &h_sc, &h_sh, &h_lut, &h_out, &h_feat, &h_zout, &h_ztab, &h_zpos};
+  }
 
   // Every buffer's member name, for allocation errors.
   void name_buffers() {
@@ -342,6 +394,7 @@ struct xnyz_ctx {
     XNYZ_NAME(mIdx); XNYZ_NAME(mScratch); XNYZ_NAME(mLoss); XNYZ_NAME(mOut); XNYZ_NAME(mConf);
     XNYZ_NAME(h_tiles); XNYZ_NAME(h_sp); XNYZ_NAME(h_sc); XNYZ_NAME(h_sh); XNYZ_NAME(h_lut);
     XNYZ_NAME(h_out); XNYZ_NAME(h_feat); XNYZ_NAME(h_zout); XNYZ_NAME(h_ztab); XNYZ_NAME(h_zpos);
+    XNYZ_NAME(na_host);
 #undef XNYZ_NAME
   }
 
@@ -755,6 +808,10 @@ struct CallOut {
   const char* refine = nullptr;
   xnyz_mode* modes_out = nullptr;
   char* corrected = nullptr;
+  // No-allocation call: every buffer comes from the fixed regions (g_bump),
+  // stage 2 runs without a graph, and each segment's stream is copied from
+  // device memory straight into dst (XNYZ_FRAME_NONE only).
+  bool noalloc = false;
 };
 
 // HyperLogLog estimate from 256 registers (small-range correction by linear
@@ -1015,7 +1072,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     if (any_rle) k = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
     block_runs_and_stats(bt, k, any_direct, any_block, ctx->pos.as<u32>(), ar, st);
   };
-  if (ctx->graphs && !g_profiler) {
+  if (ctx->graphs && !g_profiler && !co.noalloc) {
     // The launch sequence is a function of this signature alone (kernel
     // parameters are device-resident), so an instantiated graph replays it.
     GraphKey key;
@@ -1275,6 +1332,57 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
                reinterpret_cast<uint8_t*>(ow), st);
   write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, st);
 
+  // Fills co.stats (frame bytes from co.frames / co.dst_off) once the outputs
+  // are in place.
+  auto fill_stats = [&](double t_dev, double t_end) {
+    if (!co.stats) return;
+    for (u32 s = 0; s < S; ++s) {
+      const SegHost& h = g[s];
+      xnyz_stats& x = co.stats[s];
+      std::memset(&x, 0, sizeof(x));
+      x.input_points = h.n;
+      x.num_points = h.ncoded;
+      x.range[0] = h.rx;
+      x.range[1] = h.ry;
+      x.range[2] = h.rz;
+      x.is_direct = h.direct;
+      x.blocks = h.B;
+      x.patches = h.P;
+      x.wrapped_points = h.W;
+      x.counts_exp_golomb = h.rle ? h.cb_cnt.use_eg : 0;
+      x.counts_eg_k = h.rle ? h.cb_cnt.eg_k : 0;
+      x.entropy_repos = h.cb_sym.entropy;
+      x.entropy_quads = h.cb_quad.entropy;
+      x.entropy_counts = h.cb_cnt.entropy;
+      x.max_l2 = h.max_l2;
+      x.mse = h.mse;
+      x.prezstd_bytes = h.total;
+      x.frame_bytes = co.frames ? co.frames[s].len
+                     : co.dst   ? co.dst_off[s + 1] - co.dst_off[s]
+                                : 0;
+      x.ms_device = t_dev - t_start;
+      x.ms_zstd = t_end - t_dev;
+      x.ms_total = t_end - t_start;
+    }
+  };
+
+  if (co.noalloc) {  // assembled streams, device -> caller memory, back to back
+    co.dst_off[0] = 0;
+    for (u32 s = 0; s < S; ++s) co.dst_off[s + 1] = co.dst_off[s] + g[s].total;
+    if (co.dst_off[S] > co.cap)
+      throw Error(XNYZ_ENOSPC, "output buffer too small: " + std::to_string(co.dst_off[S]) +
+                                   " bytes needed");
+    for (u32 s = 0; s < S; ++s)
+      check_hip(hipMemcpyAsync(co.dst + co.dst_off[s],
+                               reinterpret_cast<const uint8_t*>(ow) + g[s].base, g[s].total,
+                               hipMemcpyDeviceToHost, st), "stream D2H");
+    ctx->copied();
+    ctx->sync();
+    const double t = now_ms();
+    fill_stats(t, t);
+    return XNYZ_OK;
+  }
+
   // ---- frame stage -------------------------------------------------------------
   const bool want_out = co.frames || co.dst;
   bool host_pre = co.pre != nullptr;
@@ -1533,36 +1641,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   if (overlap && cst != st) check_hip(hipStreamSynchronize(cst), "hipStreamSynchronize");
   if (overlap) ctx->sync();
   const double t_end = now_ms();
-  if (co.stats) {
-    for (u32 s = 0; s < S; ++s) {
-      const SegHost& h = g[s];
-      xnyz_stats& x = co.stats[s];
-      std::memset(&x, 0, sizeof(x));
-      x.input_points = h.n;
-      x.num_points = h.ncoded;
-      x.range[0] = h.rx;
-      x.range[1] = h.ry;
-      x.range[2] = h.rz;
-      x.is_direct = h.direct;
-      x.blocks = h.B;
-      x.patches = h.P;
-      x.wrapped_points = h.W;
-      x.counts_exp_golomb = h.rle ? h.cb_cnt.use_eg : 0;
-      x.counts_eg_k = h.rle ? h.cb_cnt.eg_k : 0;
-      x.entropy_repos = h.cb_sym.entropy;
-      x.entropy_quads = h.cb_quad.entropy;
-      x.entropy_counts = h.cb_cnt.entropy;
-      x.max_l2 = h.max_l2;
-      x.mse = h.mse;
-      x.prezstd_bytes = h.total;
-      x.frame_bytes = co.frames ? co.frames[s].len
-                     : co.dst   ? co.dst_off[s + 1] - co.dst_off[s]
-                                : 0;
-      x.ms_device = t_dev - t_start;
-      x.ms_zstd = t_end - t_dev;
-      x.ms_total = t_end - t_start;
-    }
-  }
+  fill_stats(t_dev, t_end);
   return XNYZ_OK;
 }
 
@@ -1665,6 +1744,8 @@ int xnyz_create(int device, xnyz_ctx** out) {
       use_device(c);
       check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
       c->hs = build_hilbert_sm();
+      c->na_dev = noalloc_scratch(device);
+      c->na_host.ensure(kNoAllocHostBytes);
     } catch (...) {
       delete c;
       throw;
@@ -1679,18 +1760,9 @@ int xnyz_destroy(xnyz_ctx* ctx) {
   return guarded([&] {
     use_device(ctx);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DBuf* b : {&ctx->tiles, &ctx->sp, &ctx->sc, &ctx->sh, &ctx->lut, &ctx->bigv, &ctx->bigc,
-                    &ctx->large, &ctx->nlarge, &ctx->keys[0], &ctx->keys[1], &ctx->lo[0],
-                    &ctx->lo[1], &ctx->cells, &ctx->cells_lo, &ctx->pos, &ctx->cnt, &ctx->out,
-                    &ctx->arena, &ctx->upload, &ctx->tmp, &ctx->idx[0], &ctx->idx[1],
-                    &ctx->lo64[0], &ctx->lo64[1], &ctx->feat, &ctx->mP, &ctx->mWh, &ctx->mG,
-                    &ctx->mM, &ctx->mV, &ctx->mX, &ctx->mY, &ctx->mIdx, &ctx->mScratch,
-                    &ctx->mLoss, &ctx->mOut, &ctx->mConf, &ctx->patch, &ctx->ztab,
-                    &ctx->zscr, &ctx->zout, &ctx->zpos})
-      b->release();
-    for (HBuf* b : {&ctx->h_tiles, &ctx->h_sp, &ctx->h_sc, &ctx->h_sh, &ctx->h_lut, &ctx->h_out,
-                    &ctx->h_feat, &ctx->h_zout, &ctx->h_ztab, &ctx->h_zpos})
-      b->release();
+    for (DBuf* b : ctx->dbufs()) b->release();
+    for (HBuf* b : ctx->hbufs()) b->release();
+    ctx->na_host.release();
     ctx->prof.destroy();
     ctx->stage2.clear();
     for (hipEvent_t e : ctx->slice_ev) (void)hipEventDestroy(e);
@@ -1770,6 +1842,68 @@ int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uin
   });
 }
 
+int xnyz_compress_batch_device_noalloc(xnyz_ctx* ctx, const float* d_xyz,
+                                       const uint64_t* offsets, uint32_t count,
+                                       const xnyz_opts* opts, uint8_t* out, uint64_t cap,
+                                       uint64_t* out_offsets, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !out_offsets || (!out && cap))
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    if (count > XNYZ_NOALLOC_MAX_CLOUDS)
+      throw Error(XNYZ_EUNSUPPORTED, "no-allocation calls take at most " +
+                                         std::to_string(XNYZ_NOALLOC_MAX_CLOUDS) + " clouds");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    if (off[count] > XNYZ_NOALLOC_MAX_POINTS)
+      throw Error(XNYZ_EUNSUPPORTED, "no-allocation calls take at most " +
+                                         std::to_string(XNYZ_NOALLOC_MAX_POINTS) + " points");
+    for (u32 s = 0; s < count; ++s)
+      if (opts[s].frame_mode != XNYZ_FRAME_NONE)
+        throw Error(XNYZ_EUNSUPPORTED, "no-allocation calls return assembled streams only "
+                                       "(frame_mode XNYZ_FRAME_NONE)");
+    use_device(ctx);
+    // Every pooled buffer of the context is set aside for the call, so the
+    // pipeline's requests all carve from the fixed regions; the context's own
+    // buffers come back untouched afterwards, error or not.
+    struct Swap {
+      xnyz_ctx* c;
+      std::vector<DBuf> d;
+      std::vector<HBuf> h;
+      Bump bump;
+      explicit Swap(xnyz_ctx* ctx) : c(ctx) {
+        for (DBuf* b : c->dbufs()) {
+          d.push_back(*b);
+          b->p = nullptr;
+          b->cap = 0;
+        }
+        for (HBuf* b : c->hbufs()) {
+          h.push_back(*b);
+          b->p = nullptr;
+          b->cap = 0;
+        }
+        // the array's last 64 KiB stay unused: loads that run a few words
+        // past a buffer's end (as they may into an allocation's growth
+        // margin) stay inside the array
+        bump = Bump{static_cast<char*>(c->na_dev), kNoAllocScratchBytes - (64u << 10), 0,
+                    c->na_host.as<char>(), kNoAllocHostBytes, 0};
+        g_bump = &bump;
+      }
+      ~Swap() {
+        g_bump = nullptr;
+        size_t i = 0;
+        for (DBuf* b : c->dbufs()) *b = d[i++];
+        i = 0;
+        for (HBuf* b : c->hbufs()) *b = h[i++];
+      }
+    } swap(ctx);
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    co.noalloc = true;
+    return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+  });
+}
+
 int xnyz_search_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
                              uint32_t count, const xnyz_opts* opts,
                              xnyz_search_result* results) {
@@ -1801,6 +1935,8 @@ int xnyz_features_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t
 
 namespace {
 
+static_assert(kNoAllocScratchBytes == XNYZ_NOALLOC_SCRATCH_BYTES,
+              "no-allocation scratch size differs from include/xnyz.h");
 static_assert(kPTotal == XNYZ_MLP_PARAMS && kFeat == XNYZ_MLP_FEATURES,
               "selector layout differs from include/xnyz.h");
 

@@ -51,6 +51,13 @@ struct Arena {
 
 constexpr u32 kTile = 4096;  // points (cells, blocks) per tile, every stage
 
+// The no-allocation entry point's device scratch (noalloc.hip): one array of
+// this many bytes per GPU, held from the library's load until the process
+// ends.  noalloc_scratch(device) is its device address on `device`, looked up
+// once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+void* noalloc_scratch(int device);
+
 // Morton -> Hilbert digit transform as a finite-state machine over the
 // rotation group: state s is a composition of the per-digit permutations of
 // hilbert.cpp; apply[s][d] is s applied to digit d, next[s][md] the state
