@@ -186,3 +186,25 @@ def test_repeat_calls_identical(X):
     pts = torch.from_numpy(_uniform(13, 200000)).cuda()
     outs = [X.compress(pts, O.TO, 0.01, True, True, want_prezstd=True)[1] for _ in range(3)]
     assert outs[0] == outs[1] == outs[2]
+
+
+@pytest.mark.parametrize("name", sorted(__import__("hand_streams").CASES))
+def test_hand_written_streams(X, name):
+    """The byte images written out by hand from the format (tests/hand_streams.py)."""
+    import hand_streams as H
+    pts, rle, thr, want = H.CASES[name]
+    _, pre, _, _ = X.compress(torch.from_numpy(pts).cuda(), O.TO, H.B1, False, rle, thr,
+                              want_prezstd=True)
+    assert pre == want
+
+
+def test_quantizer_known_answers(X):
+    """Half-way, tie, q >= 2^23 and floor-below-integer points, derived by hand."""
+    import hand_streams as H
+    for q, b, rows in ((O.TO, H.B1, H.TO_KNOWN), (O.CUBE, H.B1_CUBE, H.CUBE_KNOWN)):
+        pts = torch.tensor([p for p, _ in rows], dtype=torch.float32).cuda()
+        nodes, off = X.quantize(pts, q, b)
+        assert (off == 0).all()
+        assert nodes.tolist() == [n for _, n in rows]
+        # the compression pipeline's own quantizer on the same points
+        _check(X, np.array([p for p, _ in rows], np.float32), q, b, False, False, thr=0)

@@ -1366,7 +1366,15 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     }
   };
 
-  if (co.noalloc) {  // assembled streams, device -> caller memory, back to back
+  // Assembled streams into caller memory that the DMA engine can write
+  // (pinned, or any memory for a no-allocation call): each stream is copied
+  // straight from device memory to its place, with no staging buffer (one
+  // copy per cloud, so only for batches of a few clouds; larger batches take
+  // the staged path below).
+  bool all_none = true;
+  for (u32 s = 0; s < S; ++s) all_none &= opts[s].frame_mode == XNYZ_FRAME_NONE;
+  if (co.noalloc || (all_none && S <= 64 && co.dst && !co.pre && co.cap &&
+                     pinned_range(co.dst, co.cap))) {
     co.dst_off[0] = 0;
     for (u32 s = 0; s < S; ++s) co.dst_off[s + 1] = co.dst_off[s] + g[s].total;
     if (co.dst_off[S] > co.cap)
