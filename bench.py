@@ -4,24 +4,28 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload terrain-50M]
 
 A step compresses the whole cloud once: points already resident in HBM ->
-assembled stream -> zstd level-3 frame in host memory (the reference's
-compress(), source/composer/block_compressor_rle.hpp:38-321).  `value` is
-GB/s of fp32 xyz input (12 bytes per point) over the K timed steps.
+assembled stream in pinned host memory.  The assembled stream is the buffer
+the reference's compress() hands to ZSTD_compress (source/composer/
+block_compressor_rle.hpp:38-321, source/composer/utils.hpp:35-53), byte for
+byte; the last timed stream is checked against the oracle's afterwards.
+`value` is GB/s of fp32 xyz input (12 bytes per point) over the K timed
+steps.
 
-Frame stage: the headline uses libzstd level 3 split over host worker threads
-(a valid single frame with the content size, decodable by the reference's
-decompress_buffer); the single-shot frame that is byte-identical to the
-oracle's is timed beside it ("exact_zstd_GBps"), as is the rate up to the
-assembled stream ("prezstd_GBps").
+The zstd frame stage is reported beside it over the same K steps
+("framed": level 3 split between device-built head blocks and host worker
+threads -- one valid frame, not byte-identical to the single-shot frame), as
+are the single-shot level-3 frame that is byte-identical to the oracle's
+("exact_zstd_GBps") and the frame built wholly on the device.
 
-The roofline entry is the kernel with the most device time, timed with HIP
-events on the library's own stream over K more steps of the same work right
-after the timed ones (per-launch events keep stage 2 out of its captured
+The roofline entry is the kernel with the most device time per step of the
+headline path, timed with HIP events on the library's own stream over K more
+steps of the same work (per-launch events keep stage 2 out of its captured
 graph, so they are not in the headline's steps); its achieved GB/s uses the
-algorithmic bytes of one launch (see ALGO_BYTES) and its traffic the HBM bytes
-rocprofv3's counters read for that kernel (see measured_traffic).  cpu_baseline is
-the CPU oracle (a single-threaded restatement of the reference) on a bounded
-sample of the same workload.
+algorithmic bytes of one launch (see algo_bytes) and its traffic the HBM
+bytes rocprofv3's counters read for that kernel (see measured_traffic).
+"roofline_frame" is the same for the device-built frame blocks of the split
+frame.  cpu_baseline is the CPU oracle (a single-threaded restatement of the
+reference) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -120,6 +124,10 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
         "k_rle_finish": 4 * n_coded + 8 * n_coded + 4 * n_coded,
         "k_stats": 8 * n_coded,                       # read sorted cells
         "k_sec_bits": 8 * n_coded,
+        # read the sorted cells, write the repos / quads / counts sections
+        # (at most the whole stream: a lower bound on the time's worth)
+        "k_sec_write_cells": 8 * n_coded + out_bytes,
+        "k_block_stats": 8 * n_coded,
         "k_zcand": dev_frame_bytes,
         "k_zparse": dev_frame_bytes,
         "k_zlits": dev_frame_bytes,
@@ -186,7 +194,7 @@ def main() -> int:
 
     def step(mode, out=None):
         offs, st = (out or sink).compress(X, dev, [0, n], [dict(o1[0], frame_mode=mode)], ctx)
-        return int(offs[1] - offs[0]), st[0]
+        return int(offs[0]), int(offs[1] - offs[0]), st[0]
 
     def timed(mode, steps, warmup, out=None):
         for _ in range(warmup):
@@ -199,15 +207,16 @@ def main() -> int:
         torch.cuda.synchronize()
         return time.perf_counter() - t0, last
 
-    head_mode = _lib.FRAME_ZSTD3_MT
-    # Run-time parameters (persistent grid, host chunk size, the device's
-    # share of the frame, copy-back slice): the table xnyzip_amd.tune
-    # measured, or tuned on this cloud first (outside the timed steps).
+    stream_mode, frame_mode = _lib.FRAME_NONE, _lib.FRAME_ZSTD3_MT
+    # Run-time parameters (persistent grid, radix shape, host chunk size, the
+    # device's share of a split frame, copy-back slice): the table
+    # xnyzip_amd.tune measured, or tuned on this cloud first (outside the
+    # timed steps).
     from xnyzip_amd import tune as AT
     table = args.tuning or AT.DEFAULT_TABLE
     if args.autotune:
         t_at = time.perf_counter()
-        _, at_ms = X.autotune(dev, dict(o1[0], frame_mode=head_mode), reps=2, ctx=ctx)
+        _, at_ms = X.autotune(dev, dict(o1[0], frame_mode=frame_mode), reps=2, ctx=ctx)
         print(f"[bench] autotune: {at_ms:.1f} ms/call, tuned in "
               f"{time.perf_counter() - t_at:.1f} s", file=sys.stderr, flush=True)
         table = "autotuned on this cloud"
@@ -218,52 +227,77 @@ def main() -> int:
     tuning = ctx.tuning(max(0, n.bit_length() - 1))
     print(f"[bench] tuning ({os.path.relpath(table, ROOT) if os.path.exists(table) else table}):"
           f" {tuning}", file=sys.stderr, flush=True)
-    dt, (frame_len, st) = timed(head_mode, args.steps, args.warmup)
+
+    # Headline: the assembled stream -- byte for byte the buffer the
+    # reference hands to ZSTD_compress -- in host memory.
+    dt, (s_at, stream_len, st) = timed(stream_mode, args.steps, args.warmup)
     value = 12.0 * n * args.steps / dt / 1e9
-    print(f"[bench] headline: {value:.2f} GB/s ({dt / args.steps * 1e3:.1f} ms/step)",
-          file=sys.stderr, flush=True)
-    # Kernel times: the same K steps again with an event pair around every
-    # launch on the library's stream (stage 2 then runs eagerly, not from its
-    # captured graph, so each kernel is seen).
-    ctx.profile(True)
-    dt_prof, _ = timed(head_mode, args.steps, 0)
-    prof = ctx.profile_report()
-    ctx.profile(False)
+    print(f"[bench] headline (assembled stream): {value:.2f} GB/s "
+          f"({dt / args.steps * 1e3:.1f} ms/step)", file=sys.stderr, flush=True)
+    verified = verify_stream(args.workload, w, n, bytes(sink.buf[s_at:s_at + stream_len]))
+    print(f"[bench] last timed stream: {verified}", file=sys.stderr, flush=True)
+
+    # The zstd frame stage over the same number of steps: level 3 split over
+    # the device (head blocks) and host worker threads (tail chunks) -- one
+    # valid frame, not byte-identical to the single-shot level-3 frame.
+    dt_f, (_, frame_len, _) = timed(frame_mode, args.steps, 1)
+    framed = {"GBps": round(12.0 * n * args.steps / dt_f / 1e9, 3),
+              "ms_per_step": round(dt_f / args.steps * 1e3, 3),
+              "frame_bytes": frame_len, "ratio": round(12.0 * n / frame_len, 4),
+              "byte_identical_to_single_shot_L3": False}
+    print(f"[bench] split frame: {framed['GBps']:.2f} GB/s", file=sys.stderr, flush=True)
+
+    # Kernel times: K more steps of each path with an event pair around every
+    # launch on the library's stream (stage 2 then runs eagerly, not from
+    # its captured graph, so each kernel is seen).
+    def profiled(mode):
+        ctx.profile(True)
+        dtp, _ = timed(mode, args.steps, 0)
+        rep = ctx.profile_report()
+        ctx.profile(False)
+        return dtp, rep
+    dt_prof, prof = profiled(stream_mode)
+    _, prof_f = profiled(frame_mode)
     print(f"[bench] profiled pass: {dt_prof / args.steps * 1e3:.1f} ms/step",
           file=sys.stderr, flush=True)
 
     extra = {}
     if not args.no_extra:
         k2 = max(1, min(3, args.steps))
-        dt_e, (flen_e, _) = timed(_lib.FRAME_ZSTD3, k2, 0)
-        dt_p, (plen, st_p) = timed(_lib.FRAME_NONE, k2, 1)
-        dt_g, (glen, _) = timed(_lib.FRAME_ZSTD_GPU, k2, 1)
+        dt_e, (_, flen_e, _) = timed(_lib.FRAME_ZSTD3, k2, 0)
+        dt_g, (_, glen, _) = timed(_lib.FRAME_ZSTD_GPU, k2, 1)
         other = OutBuffer(13 * n, pinned=args.pageable_out)
-        dt_o, _ = timed(head_mode, k2, 1, other)
+        dt_o, _ = timed(stream_mode, k2, 1, other)
         del other
-        extra = {("pinned_out_GBps" if args.pageable_out else "pageable_out_GBps"):
+        extra = {("pinned_out_stream_GBps" if args.pageable_out else "pageable_out_stream_GBps"):
                  12.0 * n * k2 / dt_o / 1e9,
                  "gpu_frame_GBps": 12.0 * n * k2 / dt_g / 1e9, "gpu_frame_bytes": glen,
                  "gpu_frame_ratio": 12.0 * n / glen,
                  "exact_zstd_GBps": 12.0 * n * k2 / dt_e / 1e9,
-                 "prezstd_GBps": 12.0 * n * k2 / dt_p / 1e9,
-                 "device_ms": st_p.ms_device, "exact_frame_bytes": flen_e}
+                 "exact_frame_bytes": flen_e, "exact_frame_ratio": 12.0 * n / flen_e}
+        framed["bytes_vs_single_shot_L3"] = round(frame_len / flen_e - 1, 5)
 
-    # roofline: the kernel with the most device time in the timed region
     key_bytes = 8
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"]) if prof else None
-    roof = None
-    if dom:
-        name, d = dom
+
+    def roofline(rep, dev_bytes):
+        """The kernel with the most device time per step in `rep`."""
+        if not rep:
+            return None
+        name, d = max(rep.items(), key=lambda kv: kv[1]["ms"])
         avg_ms = d["ms"] / d["calls"]
         ab = algo_bytes(name, n, st.num_points, key_bytes, d["calls"] // args.steps,
-                        st.prezstd_bytes, dev_frame_bytes(st.prezstd_bytes, tuning))
-        if ab is not None:
-            ach = ab / (avg_ms * 1e-3) / 1e9
-            roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
-                    "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
-                    "traffic": measured_traffic(name, args.workload, n),
-                    "avg_launch_ms": round(avg_ms, 4), "algo_bytes_per_launch": ab}
+                        st.prezstd_bytes, dev_bytes)
+        if ab is None:
+            return {"kernel": name, "avg_launch_ms": round(avg_ms, 4)}
+        ach = ab / (avg_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "kernel": name, "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                "traffic": measured_traffic(name, args.workload, n),
+                "avg_launch_ms": round(avg_ms, 4), "algo_bytes_per_launch": ab,
+                "launches_per_step": d["calls"] // args.steps}
+    roof = roofline(prof, 0)
+    roof_f = roofline({k: v for k, v in prof_f.items() if k.startswith("k_z")},
+                      dev_frame_bytes(st.prezstd_bytes, tuning))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -271,11 +305,12 @@ def main() -> int:
         m = min(args.cpu_sample, n)
         sample = pts[:m]
         nbytes, info = O.compress_timed(sample, O.TO if w["quantizer"] == 1 else O.CUBE, w["l2"],
-                                        w["hilbert"], w["rle"], 1024, True)
+                                        w["hilbert"], w["rle"], 1024, False)
         cpu = {"value": round(12.0 * m / info.seconds_total / 1e9, 5), "unit": "GB/s",
                "cores": 1, "kind": "port",
-               "sample": f"first {m} points of the same workload, one compress() incl. zstd L3, "
-                         f"{info.seconds_total:.2f} s"}
+               "sample": f"first {m} points of the same workload, one compress() up to the "
+                         f"assembled stream (the headline's boundary; the RLE diagnostic "
+                         f"included, as in the reference), {info.seconds_total:.2f} s"}
 
     dev_ms = {k: round(v["ms"] / args.steps, 3) for k, v in
               sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
@@ -295,13 +330,18 @@ def main() -> int:
         "config": {"workload": args.workload, "points": n,
                    "quantizer": "to" if w["quantizer"] == 1 else "cube", "l2_bound": w["l2"],
                    "curve": "hilbert" if w["hilbert"] else "z-order", "rle": w["rle"],
-                   "chunks": 1, "frame": "zstd L3, host worker threads",
+                   "chunks": 1,
+                   "output": "assembled stream in pinned host memory, byte-identical to the "
+                             "reference's pre-zstd buffer; zstd frame rates in 'framed' and "
+                             "'exact_zstd_GBps'",
                    "tuning": {k: (round(v, 4) if isinstance(v, float) else v)
                               for k, v in tuning.items()}},
-        "ratio": round(12.0 * n / frame_len, 4),
-        "prezstd_bytes": st.prezstd_bytes,
-        "frame_bytes": frame_len,
+        "verified": verified,
+        "stream_bytes": stream_len,
+        "stream_ratio": round(12.0 * n / stream_len, 4),
+        "framed": framed,
         "roofline": roof,
+        "roofline_frame": roof_f,
         "cpu_baseline": cpu,
         "kernel_ms_per_step": dev_ms,
         **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in extra.items()},
@@ -311,6 +351,20 @@ def main() -> int:
     if args.copy_probe:
         copy_probe(args.copy_probe)
     return 0
+
+
+def verify_stream(workload: str, w: dict, n: int, stream: bytes) -> str:
+    """Checks a timed stream against the oracle's, recorded as a SHA-256 in
+    tests/golden/workloads.json (made by tests/golden/make_workloads.py)."""
+    import hashlib
+    path = os.path.join(ROOT, "tests", "golden", "workloads.json")
+    g = json.load(open(path)).get(workload) if os.path.exists(path) else None
+    if not g or g["points"] != n:
+        return "unchecked: no oracle stream recorded for this workload and size"
+    ok = len(stream) == g["stream_bytes"] and \
+        hashlib.sha256(stream).hexdigest() == g["stream_sha256"]
+    return ("sha256 equal to the oracle's stream" if ok
+            else "MISMATCH against the oracle's stream")
 
 
 def dev_frame_bytes(total: int, tuning: dict) -> int:
