@@ -324,6 +324,32 @@ int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
 int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
                     xnyz_buf* out_points);
 
+/* xnyz_decompress on the device: the host reads the header, section records
+ * and metas; the device decodes the Huffman / Exp-Golomb / Golomb-Rice
+ * sections in parallel (sub-sequences that synchronise on common code-word
+ * starts), rebuilds the block and LCP metadata with one segmented scan
+ * (recover_from_lcp_meta, source/quantizer/block_quantizer.hpp:153-207),
+ * recovers and unshifts the points.  Same inputs and the same points, in
+ * the same order, as xnyz_decompress / BlockDecompressor[RLE]<float>::
+ * decompress (source/composer/block_decompressor.hpp:31-192,
+ * block_decompressor_rle.hpp:32-207); a zstd frame is opened on the host
+ * first.  out_points: host floats, as xnyz_decompress. */
+int xnyz_decompress_device(xnyz_ctx* ctx, const uint8_t* in, uint64_t len,
+                           const xnyz_opts* opts, xnyz_buf* out_points);
+
+/* The same, leaving the points in device memory: d_points holds room for
+ * cap_points points (x, y, z floats); *n_points gets the count (also when
+ * the call returns XNYZ_ENOSPC because the room is too small). */
+int xnyz_decompress_to_device(xnyz_ctx* ctx, const uint8_t* in, uint64_t len,
+                              const xnyz_opts* opts, float* d_points, uint64_t cap_points,
+                              uint64_t* n_points);
+
+/* Replaces mve::hilbertToMorton3 (source/quantizer/hilbert-curves-cpp/
+ * hilbert.cpp:396-421) on device: out[i] = hilbertToMorton3(in[i], it),
+ * host arrays of n codes. */
+int xnyz_hilbert_to_morton(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
+                           uint32_t iteration, uint64_t* out);
+
 /* Stage-2 launch sequences (key generation, radix passes, RLE cells, block
  * runs, statistics: some 40 kernels) are captured into a hipGraph per launch
  * shape and replayed while the shape repeats (same cloud sizes and options

@@ -24,7 +24,8 @@ from ._lib import Context, XnyzError  # noqa: F401
 __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDecompressor",
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
            "compress", "compress_batch", "compress_batch_into", "compress_batch_noalloc", "host_buffer", "search_batch", "features_batch", "mlp_train",
-           "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune"]
+           "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune",
+           "decompress_device", "decompress_to_device", "hilbert_to_morton"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -432,6 +433,51 @@ def decompress(data: bytes, quantizer, l2_bound: float, rle: bool,
     _lib.check(_lib.lib().xnyz_decompress(C.c_void_p(src.ctypes.data), len(data), C.byref(o),
                                            C.byref(out)))
     return np.frombuffer(_lib.take(out), dtype=np.float32).reshape(-1, 3).copy()
+
+
+def decompress_device(data: bytes, quantizer, l2_bound: float, rle: bool,
+                      is_frame: bool = True, ctx: Context | None = None) -> np.ndarray:
+    """decompress with the decoding on the GPU (xnyz_decompress_device): the
+    same (N, 3) float32 points."""
+    o = _opts(quantizer, l2_bound, False, rle, 1024,
+              _lib.FRAME_ZSTD3 if is_frame else _lib.FRAME_NONE, 0)
+    out = _lib.Buf()
+    src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    ctx = ctx or default_context(0)
+    _lib.check(_lib.lib().xnyz_decompress_device(ctx.handle, C.c_void_p(src.ctypes.data),
+                                                  len(data), C.byref(o), C.byref(out)))
+    return np.frombuffer(_lib.take(out), dtype=np.float32).reshape(-1, 3).copy()
+
+
+def decompress_to_device(data: bytes, quantizer, l2_bound: float, rle: bool, out_device,
+                         is_frame: bool = True, ctx: Context | None = None) -> int:
+    """Decodes on the GPU into `out_device` (a contiguous float32 (M, 3) tensor
+    on the GPU, M >= the point count); returns the point count.  Raises
+    XnyzError (XNYZ_ENOSPC) when M is too small."""
+    import torch
+    t = out_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("out_device must be a contiguous float32 (M, 3) tensor on the GPU")
+    o = _opts(quantizer, l2_bound, False, rle, 1024,
+              _lib.FRAME_ZSTD3 if is_frame else _lib.FRAME_NONE, 0)
+    src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    n = C.c_uint64()
+    ctx = ctx or default_context(t.device.index or 0)
+    torch.cuda.synchronize(t.device)
+    _lib.check(_lib.lib().xnyz_decompress_to_device(
+        ctx.handle, C.c_void_p(src.ctypes.data), len(data), C.byref(o),
+        C.c_void_p(t.data_ptr()), t.shape[0], C.byref(n)))
+    return n.value
+
+
+def hilbert_to_morton(codes, iteration: int, ctx: Context | None = None) -> np.ndarray:
+    """hilbertToMorton3 of each code on the GPU."""
+    a = np.ascontiguousarray(codes, dtype=np.uint64)
+    out = np.empty_like(a)
+    ctx = ctx or default_context(0)
+    _lib.check(_lib.lib().xnyz_hilbert_to_morton(ctx.handle, C.c_void_p(a.ctypes.data), a.size,
+                                                  int(iteration), C.c_void_p(out.ctypes.data)))
+    return out
 
 
 class BlockDecompressor:

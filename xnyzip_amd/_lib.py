@@ -105,7 +105,8 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_compress_selected_batch_device_into",
            "xnyz_quantize_device",
-           "xnyz_morton_to_hilbert", "xnyz_decompress", "xnyz_graphs_enable",
+           "xnyz_morton_to_hilbert", "xnyz_hilbert_to_morton", "xnyz_decompress",
+           "xnyz_decompress_device", "xnyz_decompress_to_device", "xnyz_graphs_enable",
            "xnyz_tuning_set", "xnyz_tuning_get", "xnyz_tuning_save", "xnyz_tuning_load",
            "xnyz_autotune", "xnyz_profile_enable", "xnyz_profile_report", "xnyz_free",
            "xnyz_host_alloc", "xnyz_host_free"]
@@ -146,6 +147,10 @@ def lib() -> C.CDLL:
         L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]
+        L.xnyz_decompress_device.argtypes = [P, P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]
+        L.xnyz_decompress_to_device.argtypes = [P, P, C.c_uint64, C.POINTER(Opts), P, C.c_uint64,
+                                                C.POINTER(C.c_uint64)]
+        L.xnyz_hilbert_to_morton.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_profile_enable.argtypes = [P, C.c_int]
         L.xnyz_graphs_enable.argtypes = [P, C.c_int]
         L.xnyz_tuning_set.argtypes = [P, C.c_int, C.POINTER(Tuning)]

@@ -30,8 +30,8 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", "-Wall",
 DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["kernels.hip", "mlp.hip", "zframe.hip", "noalloc.hip", "hilbert_sm.cpp", "api.cpp",
-           "decode.cpp"]
+SOURCES = ["kernels.hip", "mlp.hip", "zframe.hip", "noalloc.hip", "decode_dev.hip",
+           "hilbert_sm.cpp", "api.cpp", "decode.cpp"]
 
 
 def _run(cmd):

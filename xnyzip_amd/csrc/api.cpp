@@ -20,6 +20,7 @@
 #include <zstd.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -37,6 +38,7 @@
 #include "../../include/xnyz.h"
 #include "codebook.hpp"
 #include "common.hpp"
+#include "decode_dev.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "modes.hpp"
@@ -344,6 +346,7 @@ struct xnyz_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   HilbertSM hs{};
+  HilbertInv hinv{};
   // device buffers
   DBuf tiles, sp, sc, sh, lut, bigv, bigc, large, nlarge;
   DBuf keys[2], lo[2], cells, cells_lo, pos, cnt, out, arena, upload, tmp;
@@ -351,6 +354,10 @@ struct xnyz_ctx {
   DBuf feat;             // selector statistics
   DBuf patch;            // stream header / meta bytes for k_patch
   DBuf ztab, zscr, zout, zpos;  // device-built frames
+  // device decoder: stream, symbols, quads, block flags, patches and their
+  // prefix sums, block arrays, RLE counts and offsets, points, tables, scratch
+  DBuf dstream, dsym, dquad, dhead, dpatch, dps, dblk, dcnt, dpts, dtab, darena;
+  HBuf h_dflag;
   // selector: parameters, FP16 weights, optimiser state, samples
   DBuf mP, mWh, mG, mM, mV, mX, mY, mIdx, mScratch, mLoss, mOut, mConf;
   // pinned host mirrors
@@ -374,10 +381,12 @@ struct xnyz_ctx {
     return {&tiles, &sp, &sc, &sh, &lut, &bigv, &bigc, &large, &nlarge, &keys[0], &keys[1],
             &lo[0], &lo[1], &cells, &cells_lo, &pos, &cnt, &out, &arena, &upload, &tmp,
             &idx[0], &idx[1], &lo64[0], &lo64[1], &feat, &patch, &ztab, &zscr, &zout, &zpos,
-            &mP, &mWh, &mG, &mM, &mV, &mX, &mY, &mIdx, &mScratch, &mLoss, &mOut, &mConf};
+            &mP, &mWh, &mG, &mM, &mV, &mX, &mY, &mIdx, &mScratch, &mLoss, &mOut, &mConf,
+            &dstream, &dsym, &dquad, &dhead, &dpatch, &dps, &dblk, &dcnt, &dpts, &dtab, &darena};
   }
   std::vector<HBuf*> hbufs() {
-    return {&h_tiles, &h_sp, &h_sc, &h_sh, &h_lut, &h_out, &h_feat, &h_zout, &h_ztab, &h_zpos};
+    return {&h_tiles, &h_sp, &h_sc, &h_sh, &h_lut, &h_out, &h_feat, &h_zout, &h_ztab, &h_zpos,
+            &h_dflag};
   }
 
   // Every buffer's member name, for allocation errors.
@@ -394,7 +403,9 @@ struct xnyz_ctx {
     XNYZ_NAME(mIdx); XNYZ_NAME(mScratch); XNYZ_NAME(mLoss); XNYZ_NAME(mOut); XNYZ_NAME(mConf);
     XNYZ_NAME(h_tiles); XNYZ_NAME(h_sp); XNYZ_NAME(h_sc); XNYZ_NAME(h_sh); XNYZ_NAME(h_lut);
     XNYZ_NAME(h_out); XNYZ_NAME(h_feat); XNYZ_NAME(h_zout); XNYZ_NAME(h_ztab); XNYZ_NAME(h_zpos);
-    XNYZ_NAME(na_host);
+    XNYZ_NAME(na_host); XNYZ_NAME(dstream); XNYZ_NAME(dsym); XNYZ_NAME(dquad);
+    XNYZ_NAME(dhead); XNYZ_NAME(dpatch); XNYZ_NAME(dps); XNYZ_NAME(dblk); XNYZ_NAME(dcnt);
+    XNYZ_NAME(dpts); XNYZ_NAME(dtab); XNYZ_NAME(darena); XNYZ_NAME(h_dflag);
 #undef XNYZ_NAME
   }
 
@@ -1653,6 +1664,397 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   return XNYZ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device decoder (BlockDecompressor[RLE]<float>::decompress,
+// source/composer/block_decompressor.hpp:31-192,
+// block_decompressor_rle.hpp:32-207): the host reads the header, the section
+// records and the metas; the device decodes every section and rebuilds the
+// points (decode_dev.hip).
+
+struct DevSec {  // one section of the assembled stream
+  u64 meta_at, meta_len, data_at, data_len;
+};
+
+// Host-side code tables of one entropy-coded section, in the layout CodeDesc
+// points into (uploaded into one table buffer).
+struct HostCode {
+  CodeDesc d{};
+  std::vector<u32> lut, tree;
+  std::vector<u64> values;
+};
+
+u32 rd_be32(const uint8_t* m) {
+  return uint32_t(m[0]) << 24 | uint32_t(m[1]) << 16 | uint32_t(m[2]) << 8 | uint32_t(m[3]);
+}
+
+// huffman_encoder.hpp:126-146 (meta), :172-201 (load_meta): Exp-Golomb
+// header, or u32 BE entry count and entries (symbol, u8 length, u32 BE
+// bits).  sym_bytes: 8 (u64, big-endian), 1 (u8) or 4 (int, raw LE).
+HostCode huffman_code(const uint8_t* m, u64 ml, u32 sym_bytes, bool is_signed, u64 max_symbol) {
+  HostCode h;
+  if (ml >= 4 && m[0] == 0xE1 && m[1] == 0xC1 && m[2] == 0x01) {
+    if (m[3] != 0x01) throw Error(XNYZ_ECORRUPT, "unknown Huffman meta mode");
+    h.d.kind = kCodeEG;
+    h.d.k = ml > 4 ? m[4] : 0;
+    if (h.d.k > 62) throw Error(XNYZ_ECORRUPT, "Exp-Golomb k too large");
+    h.d.zigzag = is_signed;
+    return h;
+  }
+  if (ml < 4) throw Error(XNYZ_ECORRUPT, "Huffman meta too short");
+  const u32 entries = rd_be32(m);
+  const u64 esz = sym_bytes + 5;
+  if (4 + (u64)entries * esz > ml) throw Error(XNYZ_ECORRUPT, "Huffman meta truncated");
+  if (!entries) throw Error(XNYZ_ECORRUPT, "Huffman meta without entries");
+  struct E {
+    u64 sym;
+    u32 len, bits;
+  };
+  std::vector<E> es(entries);
+  u32 maxlen = 0;
+  for (u32 i = 0; i < entries; ++i) {
+    const uint8_t* e = m + 4 + i * esz;
+    u64 v = 0;
+    if (sym_bytes == 8) {
+      for (int k = 0; k < 8; ++k) v = (v << 8) | e[k];
+    } else if (sym_bytes == 4) {
+      int32_t x;
+      std::memcpy(&x, e, 4);
+      v = (u64)(int64_t)x;
+    } else {
+      v = e[0];
+    }
+    const u32 len = e[sym_bytes];
+    const u32 bits = rd_be32(e + sym_bytes + 1);
+    if (!is_signed && v > max_symbol) throw Error(XNYZ_EUNSUPPORTED, "Huffman symbol out of range");
+    if (len == 0 || len > 32) throw Error(XNYZ_EUNSUPPORTED, "Huffman code length outside 1..32");
+    es[i] = E{v, len, bits};
+    maxlen = std::max(maxlen, len);
+  }
+  if (entries == 1) {  // huffman_encoder.hpp:211-215: every symbol is this one
+    h.d.kind = kCodeConst;
+    h.d.const_value = es[0].sym;
+    return h;
+  }
+  h.d.kind = kCodeHuf;
+  const u32 lb = std::min<u32>(kLutBits, maxlen);
+  h.d.lb = lb;
+  h.lut.assign(1u << lb, 0u);
+  h.values.resize(entries);
+  auto new_node = [&]() {
+    h.tree.push_back(kTreeNull);
+    h.tree.push_back(kTreeNull);
+    return (u32)(h.tree.size() / 2 - 1);
+  };
+  for (u32 i = 0; i < entries; ++i) {
+    const E& e = es[i];
+    h.values[i] = e.sym;
+    if (e.len <= lb) {
+      const u32 lo = e.bits << (lb - e.len), hi = (e.bits + 1) << (lb - e.len);
+      for (u32 x = lo; x < hi; ++x) {
+        if (h.lut[x]) throw Error(XNYZ_ECORRUPT, "Huffman codes are not prefix-free");
+        h.lut[x] = (e.len << 24) | i;
+      }
+    } else {
+      const u32 pre = e.bits >> (e.len - lb);
+      u32 node;
+      if (!h.lut[pre]) {
+        node = new_node();
+        h.lut[pre] = node;  // length 0: continue in the tree
+        if (node == 0) {    // node 0 would read as "no entry": keep it unused
+          node = new_node();
+          h.lut[pre] = node;
+        }
+      } else if ((h.lut[pre] >> 24) == 0) {
+        node = h.lut[pre] & 0xFFFFFFu;
+      } else {
+        throw Error(XNYZ_ECORRUPT, "Huffman codes are not prefix-free");
+      }
+      for (u32 b = e.len - lb; b-- > 0;) {
+        const u32 bit = (e.bits >> b) & 1;
+        u32& ch = h.tree[2 * node + bit];
+        if (b == 0) {
+          if (ch != kTreeNull) throw Error(XNYZ_ECORRUPT, "Huffman codes are not prefix-free");
+          ch = kTreeLeaf | i;
+        } else {
+          if (ch == kTreeNull) {
+            const u32 nn = new_node();
+            h.tree[2 * node + bit] = nn;
+            node = nn;
+          } else if (ch & kTreeLeaf) {
+            throw Error(XNYZ_ECORRUPT, "Huffman codes are not prefix-free");
+          } else {
+            node = ch;
+          }
+        }
+      }
+    }
+  }
+  if (h.tree.empty()) h.tree.assign(2, kTreeNull);
+  return h;
+}
+
+int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz_opts* o,
+                           float* d_out, u64 cap_points, uint64_t* n_points,
+                           xnyz_buf* host_out, double* ms_device) {
+  if (!in || !o) throw Error(XNYZ_EINVAL, "null argument");
+  if (o->quantizer != XNYZ_CUBE && o->quantizer != XNYZ_TRUNCATED_OCTAHEDRON)
+    throw Error(XNYZ_EINVAL, "bad quantizer");
+  std::vector<uint8_t> raw;
+  const uint8_t* buf = in;
+  u64 blen = len;
+  if (o->frame_mode != XNYZ_FRAME_NONE) {  // source/composer/utils.hpp:13-29
+    const unsigned long long fs = ZSTD_getFrameContentSize(in, len);
+    if (fs == ZSTD_CONTENTSIZE_ERROR) throw Error(XNYZ_ECORRUPT, "Not a zstd frame");
+    if (fs == ZSTD_CONTENTSIZE_UNKNOWN) throw Error(XNYZ_ECORRUPT, "Original size unknown");
+    raw.resize(fs);
+    const size_t d = ZSTD_decompress(raw.data(), fs, in, len);
+    if (ZSTD_isError(d)) throw Error(XNYZ_ECORRUPT, ZSTD_getErrorName(d));
+    if (d != fs) throw Error(XNYZ_ECORRUPT, "Decompressed size mismatch");
+    buf = raw.data();
+    blen = raw.size();
+  }
+  // ---- header and section records (block_decompressor.hpp:37-60) ------------
+  u64 at = 0;
+  auto get = [&](void* dst, u64 n) {
+    if (blen - at < n || at > blen) throw Error(XNYZ_ECORRUPT, "truncated stream");
+    std::memcpy(dst, buf + at, n);
+    at += n;
+  };
+  u64 num = 0, params = 0;
+  get(&num, 8);
+  get(&params, 8);
+  if (params > (blen - at) / 4) throw Error(XNYZ_ECORRUPT, "truncated stream");
+  at += params * 4;
+  int32_t rng[3];
+  get(rng, 12);
+  uint8_t hil = 0, direct = 0;
+  get(&hil, 1);
+  get(&direct, 1);
+  auto section = [&]() {
+    DevSec r;
+    get(&r.data_len, 8);
+    get(&r.meta_len, 8);
+    r.meta_at = at;
+    if (r.meta_len > blen - at) throw Error(XNYZ_ECORRUPT, "truncated section");
+    at += r.meta_len;
+    r.data_at = at;
+    if (r.data_len > blen - at) throw Error(XNYZ_ECORRUPT, "truncated section");
+    at += r.data_len;
+    return r;
+  };
+  const bool rle = o->rle != 0;
+  DevSec s_blk{}, s_cnt{}, s_gr{}, s_huf{}, s_qgr{}, s_quad{}, s_rle{};
+  if (!direct) {
+    s_blk = section();
+    s_cnt = section();
+  }
+  s_gr = section();
+  s_huf = section();
+  if (!direct) {
+    s_qgr = section();
+    s_quad = section();
+  }
+  if (rle) s_rle = section();
+  float off[3];
+  get(off, 12);
+  if (num == 0) throw Error(XNYZ_EUNSUPPORTED, "stream of zero points");
+  if (num >= 0xFFFFFFFFull) throw Error(XNYZ_EUNSUPPORTED, "more than 2^32-2 coded points");
+  if (!direct && s_qgr.meta_len + s_qgr.data_len)
+    throw Error(XNYZ_EUNSUPPORTED, "quads patches (a quad delta above 254)");
+  auto le32_at = [&](u64 p) {
+    uint32_t v;
+    std::memcpy(&v, buf + p, 4);
+    return v;
+  };
+  // BitPacker metas (bitpacking_encoder.hpp:27-32): u32 LE n, u8 width
+  u64 B = 0;
+  u32 w1 = 0, w2 = 0;
+  if (!direct) {
+    if (s_blk.meta_len != 5 || s_cnt.meta_len != 5) throw Error(XNYZ_ECORRUPT, "BitPacker meta");
+    B = le32_at(s_blk.meta_at);
+    w1 = buf[s_blk.meta_at + 4];
+    w2 = buf[s_cnt.meta_at + 4];
+    if (le32_at(s_cnt.meta_at) != B || w1 == 0 || w1 > 64 || w2 == 0 || w2 > 64 ||
+        B * w1 > s_blk.data_len * 8 || B * w2 > s_cnt.data_len * 8 || B == 0 || B > num)
+      throw Error(XNYZ_ECORRUPT, "BitPacker sections");
+  }
+  // Golomb-Rice meta (golomb_rice_encoder.hpp:126-133): u32 LE n, u8 k
+  u64 P = 0;
+  HostCode gr;
+  if (s_gr.meta_len || s_gr.data_len) {
+    if (s_gr.meta_len != 5) throw Error(XNYZ_ECORRUPT, "Golomb-Rice meta");
+    P = le32_at(s_gr.meta_at);
+    gr.d.kind = kCodeGR;
+    gr.d.k = buf[s_gr.meta_at + 4];
+    if (gr.d.k > 62) throw Error(XNYZ_ECORRUPT, "Golomb-Rice k too large");
+  }
+  HostCode hs = huffman_code(buf + s_huf.meta_at, s_huf.meta_len, 8, false, 255);
+  HostCode hq;
+  if (!direct) hq = huffman_code(buf + s_quad.meta_at, s_quad.meta_len, 1, false, 255);
+  HostCode hc;
+  if (rle) hc = huffman_code(buf + s_rle.meta_at, s_rle.meta_len, 4, true, 0);
+
+  // ---- device buffers -------------------------------------------------------
+  hipStream_t st = ctx->stream;
+  const double t0 = now_ms();
+  ctx->h_dflag.ensure(64);
+  ctx->dstream.ensure(blen + 64);
+  check_hip(hipMemcpyAsync(ctx->dstream.p, buf, blen, hipMemcpyHostToDevice, st), "stream H2D");
+  launch_zero_u8(ctx->dstream.as<uint8_t>() + blen, 64, st);
+  // code tables of the sections, one buffer
+  std::vector<uint8_t> tab;
+  auto put_tab = [&](HostCode& h) {
+    auto add = [&](const void* p, size_t n) {
+      const size_t a = (tab.size() + 255) & ~size_t(255);
+      tab.resize(a + n);
+      std::memcpy(tab.data() + a, p, n);
+      return a;
+    };
+    size_t lut = 0, tree = 0, vals = 0;
+    if (h.d.kind == kCodeHuf) {
+      lut = add(h.lut.data(), h.lut.size() * 4);
+      tree = add(h.tree.data(), h.tree.size() * 4);
+      vals = add(h.values.data(), h.values.size() * 8);
+    }
+    return std::array<size_t, 3>{lut, tree, vals};
+  };
+  const auto ts = put_tab(hs);
+  const auto tq = put_tab(hq);
+  const auto tc = put_tab(hc);
+  ctx->dtab.ensure(tab.size() + 256);
+  if (!tab.empty())
+    check_hip(hipMemcpyAsync(ctx->dtab.p, tab.data(), tab.size(), hipMemcpyHostToDevice, st),
+              "tables H2D");
+  auto bind = [&](HostCode& h, const std::array<size_t, 3>& t, const DevSec& sec) {
+    h.d.base_bit = sec.data_at * 8;
+    h.d.bits = sec.data_len * 8;
+    if (h.d.kind == kCodeHuf) {
+      const uint8_t* b0 = ctx->dtab.as<uint8_t>();
+      h.d.lut = reinterpret_cast<const u32*>(b0 + t[0]);
+      h.d.tree = reinterpret_cast<const u32*>(b0 + t[1]);
+      h.d.values = reinterpret_cast<const u64*>(b0 + t[2]);
+    }
+  };
+  bind(hs, ts, s_huf);
+  if (!direct) bind(hq, tq, s_quad);
+  if (rle) bind(hc, tc, s_rle);
+  bind(gr, {0, 0, 0}, s_gr);
+  // scratch: the largest section's sub-sequence tables, scans, tile states
+  u64 maxbits = std::max({s_huf.data_len, s_quad.data_len, s_rle.data_len, s_gr.data_len}) * 8;
+  const u64 nsub = maxbits / kSub + 2;
+  const size_t scratch = (size_t)nsub * 48 + lcp_scratch_bytes(num) + (num / 4096 + 64) * 16 +
+                         (8u << 20);
+  ctx->darena.ensure(scratch);
+  Arena ar{ctx->darena.as<char>(), ctx->darena.cap, 0};
+  const uint8_t* dS = ctx->dstream.as<uint8_t>();
+  DecScratch ws{ctx->h_dflag.as<u32>()};
+
+  // ---- sections ---------------------------------------------------------------
+  ctx->dsym.ensure(num + 16);
+  decode_section<uint8_t>(dS, hs.d, num, ctx->dsym.as<uint8_t>(), ws, ar, st);
+  ctx->dps.ensure((P + 1) * 8 + 16);
+  u64* dps = ctx->dps.as<u64>();
+  if (P) {
+    ctx->dpatch.ensure(P * 8 + 16);
+    decode_section<u64>(dS, gr.d, P, ctx->dpatch.as<u64>(), ws, ar, st);
+    scan_excl_u64(ctx->dpatch.as<u64>(), dps, P, dps + P, ar, st);
+  } else {
+    launch_zero_u8(reinterpret_cast<uint8_t*>(dps), 8, st);
+  }
+  const u64* blkst = nullptr;
+  if (!direct) {
+    ctx->dquad.ensure(num + 16);
+    decode_section<uint8_t>(dS, hq.d, num, ctx->dquad.as<uint8_t>(), ws, ar, st);
+    // block ids and counts: prefix sums of the BitPacker deltas
+    // (block_decompressor.hpp:163-165), then the first element of each block
+    ctx->dblk.ensure(B * 8 * 4 + 64);
+    u64* bid = ctx->dblk.as<u64>();
+    u64* bcnt = bid + B;
+    u64* tmp = bcnt + B;
+    u64* bstart = tmp + B;
+    launch_bitunpack(dS, s_blk.data_at * 8, w1, B, bid, st);
+    launch_bitunpack(dS, s_cnt.data_at * 8, w2, B, bcnt, st);
+    scan_excl_u64(bid, tmp, B, nullptr, ar, st);
+    launch_add_u64(bid, tmp, B, st);
+    scan_excl_u64(bcnt, tmp, B, nullptr, ar, st);
+    launch_add_u64(bcnt, tmp, B, st);
+    u64* total = ar.take<u64>(1);
+    scan_excl_u64(bcnt, bstart, B, total, ar, st);
+    check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    u64 tot;
+    std::memcpy(&tot, ws.host_flag, 8);
+    if (tot != num) throw Error(XNYZ_ECORRUPT, "block counts do not add up to the points");
+    ctx->dhead.ensure(num + 16);
+    launch_zero_u8(ctx->dhead.as<uint8_t>(), num, st);
+    launch_mark_heads(bstart, B, ctx->dhead.as<uint8_t>(), st);
+    blkst = bid;
+  }
+  u64 total_pts = num;
+  const u64* out_at = nullptr;
+  const u64* counts = nullptr;
+  if (rle) {  // counts, then each cell's first output point
+    ctx->dcnt.ensure(num * 16 + 64);
+    u64* c = ctx->dcnt.as<u64>();
+    u64* at2 = c + num;
+    decode_section<u64>(dS, hc.d, num, c, ws, ar, st);
+    u64* total = ar.take<u64>(1);
+    scan_excl_u64(c, at2, num, total, ar, st);
+    check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    std::memcpy(&total_pts, ws.host_flag, 8);
+    if (total_pts >= (1ull << 40)) throw Error(XNYZ_EUNSUPPORTED, "RLE counts beyond 2^40 points");
+    out_at = at2;
+    counts = c;
+  }
+  if (n_points) *n_points = total_pts;
+  float* dst = d_out;
+  if (d_out && total_pts > cap_points)
+    throw Error(XNYZ_ENOSPC, "output holds " + std::to_string(cap_points) + " points, " +
+                                 std::to_string(total_pts) + " needed");
+  if (!dst) {
+    ctx->dpts.ensure(total_pts * 12 + 64);
+    dst = ctx->dpts.as<float>();
+  }
+  LcpArgs a{};
+  a.n = num;
+  a.direct = direct;
+  a.hilbert = hil;
+  a.to = o->quantizer == XNYZ_TRUNCATED_OCTAHEDRON;
+  a.rle = rle;
+  const int mr = std::max({rng[0], rng[1], rng[2]});
+  u32 bits = 0;
+  for (int t = mr; t > 0; t >>= 1) ++bits;
+  a.bits = bits ? bits : 1;
+  if (direct && hil && a.bits > 21)
+    throw Error(XNYZ_EUNSUPPORTED, "direct-mode Hilbert code of more than 21 digits");
+  a.nx = (u64)(uint32_t)rng[0] / 128 + 1;
+  a.ny = (u64)(uint32_t)rng[1] / 128 + 1;
+  a.b = o->l2_bound;
+  a.rec_div = a.to ? float(std::sqrt(5.0)) : float(std::sqrt(3.0));
+  for (int k = 0; k < 3; ++k) a.off[k] = off[k];
+  a.sym = ctx->dsym.as<uint8_t>();
+  a.quad = direct ? nullptr : ctx->dquad.as<uint8_t>();
+  a.head = direct ? nullptr : ctx->dhead.as<uint8_t>();
+  a.ps = dps;
+  a.np = P;
+  a.total = total_pts;
+  a.blkst = blkst;
+  a.out_at = out_at;
+  a.counts = counts;
+  a.out = dst;
+  launch_lcp(a, ctx->hinv, ar.take<char>(lcp_scratch_bytes(num)), st);
+  if (host_out) {
+    uint8_t* p = xmalloc(total_pts * 12 + 16);
+    host_out->data = p;
+    host_out->len = total_pts * 12;
+    check_hip(hipMemcpyAsync(p, dst, total_pts * 12, hipMemcpyDeviceToHost, st), "points D2H");
+  }
+  check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  if (ms_device) *ms_device = now_ms() - t0;
+  return XNYZ_OK;
+}
+
 template <typename F>
 int guarded(F&& f) {
   try {
@@ -1752,6 +2154,7 @@ int xnyz_create(int device, xnyz_ctx** out) {
       use_device(c);
       check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
       c->hs = build_hilbert_sm();
+      c->hinv = build_hilbert_inv();
       c->na_dev = noalloc_scratch(device);
       c->na_host.ensure(kNoAllocHostBytes);
     } catch (...) {
@@ -2259,6 +2662,46 @@ int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n, uint32
 int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
                     xnyz_buf* out_points) {
   return guarded([&] { return decompress_impl(in, len, opts, out_points); });
+}
+
+int xnyz_decompress_device(xnyz_ctx* ctx, const uint8_t* in, uint64_t len,
+                           const xnyz_opts* opts, xnyz_buf* out_points) {
+  return guarded([&] {
+    if (!ctx || !out_points) throw Error(XNYZ_EINVAL, "null argument");
+    use_device(ctx);
+    ProfGuard pg(ctx);
+    return decompress_device_impl(ctx, in, len, opts, nullptr, 0, nullptr, out_points, nullptr);
+  });
+}
+
+int xnyz_decompress_to_device(xnyz_ctx* ctx, const uint8_t* in, uint64_t len,
+                              const xnyz_opts* opts, float* d_points, uint64_t cap_points,
+                              uint64_t* n_points) {
+  return guarded([&] {
+    if (!ctx || !d_points || !n_points) throw Error(XNYZ_EINVAL, "null argument");
+    use_device(ctx);
+    ProfGuard pg(ctx);
+    return decompress_device_impl(ctx, in, len, opts, d_points, cap_points, n_points, nullptr,
+                                  nullptr);
+  });
+}
+
+int xnyz_hilbert_to_morton(xnyz_ctx* ctx, const uint64_t* in, uint64_t n, uint32_t iteration,
+                           uint64_t* out) {
+  return guarded([&] {
+    if (!ctx || !in || !out) throw Error(XNYZ_EINVAL, "null argument");
+    if (iteration > 21) throw Error(XNYZ_EUNSUPPORTED, "iteration above 21");
+    if (n == 0) return XNYZ_OK;
+    use_device(ctx);
+    hipStream_t s = ctx->stream;
+    ctx->tmp.ensure((size_t)n * 16);
+    u64* d = ctx->tmp.as<u64>();
+    check_hip(hipMemcpyAsync(d, in, n * 8, hipMemcpyHostToDevice, s), "codes H2D");
+    launch_h2m(d, n, iteration, ctx->hinv, d + n, s);
+    check_hip(hipMemcpyAsync(out, d + n, n * 8, hipMemcpyDeviceToHost, s), "codes D2H");
+    ctx->sync();
+    return XNYZ_OK;
+  });
 }
 
 int xnyz_graphs_enable(xnyz_ctx* ctx, int on) {

@@ -1,0 +1,615 @@
+// decode_dev.hip -- the decoder on the device (gfx950): assembled stream ->
+// points, for BlockDecompressor[RLE]<float>::decompress
+// (source/composer/block_decompressor.hpp:31-192,
+// block_decompressor_rle.hpp:32-207, recover_from_lcp_meta
+// block_quantizer.hpp:153-207).
+//
+// Entropy-coded sections (Huffman / Exp-Golomb / Golomb-Rice) carry no
+// restart points.  They are decoded in parallel by self-synchronisation: the
+// section's bits are cut into sub-sequences of kSub bits, one per thread;
+// every thread decodes its sub-sequence from its first bit as if a code word
+// started there (k_pd_spec), recording where its last code word ends (the
+// first code-word start at or past its end, "exit") and how many code words
+// it saw.  The true entry of sub-sequence t is the exit of t - 1; rounds of
+// k_pd_sync re-decode each sub-sequence from its predecessor's exit until the
+// new path meets the old one at a common code-word start (prefix codes fall
+// into step within a few code words), after which the rest of the old path,
+// its count and its exit stand.  Rounds repeat until no exit changes (each
+// round fixes at least the next sub-sequence, so the loop ends).  An
+// exclusive scan of the counts gives every sub-sequence its first output
+// index, and k_pd_write decodes once more from the settled entries, writing
+// the values.  Code words decoded from the final byte's padding fall after
+// the section's last index and are not written.
+//
+// The LCP metadata is then rebuilt by one scan over the elements with a
+// composite operator (k_lcp_reduce / k_lcp_tiles / k_lcp_apply): the block
+// index (count of block starts), the patch index (count of PFD symbol 0),
+// the quad within the block (sum of quad deltas since the block's start) and
+// the sub-block key (sum of repos deltas since the last block start or quad
+// change), whose patch part comes from a prefix sum of the patch values.  The
+// apply pass turns every element into its lattice node, recovers and
+// unshifts it, and writes the point (RLE: count copies at its output offset).
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+#include "decode_dev.hpp"
+
+#pragma clang fp contract(off)
+
+namespace xnyz {
+namespace {
+
+#define LAUNCH_CHECK(name)                      \
+  do {                                          \
+    check_hip(hipGetLastError(), name);         \
+    if (g_profiler) g_profiler->mark(name, s);  \
+  } while (0)
+
+inline u32 blocks_for(u64 n, u32 per_block) {
+  const u64 g = (n + per_block - 1) / per_block;
+  return (u32)(g ? g : 1);
+}
+
+// ---- bit access --------------------------------------------------------------
+// The stream buffer is 256-byte aligned and padded with at least 16 zero bytes,
+// so two aligned 8-byte loads around any bit before the stream's end stay
+// inside it.
+
+__device__ __forceinline__ u64 ld64(const uint8_t* s, u64 w) {
+  return reinterpret_cast<const u64*>(s)[w];
+}
+
+// 64 bits from bit g on, the first one in bit 63 (MSB-first streams: Huffman,
+// Exp-Golomb; bit i of a byte stream is bit 7 - i%8 of byte i/8).
+__device__ __forceinline__ u64 peek_msb(const uint8_t* s, u64 g) {
+  const u64 w = g >> 6;
+  const u32 o = (u32)(g & 63);
+  const u64 a = __builtin_bswap64(ld64(s, w));
+  if (!o) return a;
+  return (a << o) | (__builtin_bswap64(ld64(s, w + 1)) >> (64 - o));
+}
+
+// 64 bits from bit g on, the first one in bit 0 (LSB-first streams:
+// Golomb-Rice; bit i is bit i%8 of byte i/8).
+__device__ __forceinline__ u64 peek_lsb(const uint8_t* s, u64 g) {
+  const u64 w = g >> 6;
+  const u32 o = (u32)(g & 63);
+  const u64 a = ld64(s, w);
+  if (!o) return a;
+  return (a >> o) | (ld64(s, w + 1) << (64 - o));
+}
+
+// ---- one code word -----------------------------------------------------------
+// Decodes the code word starting at section bit p (< c.bits) into *v and
+// returns the bit after it.  A code word that would run past the section's
+// end returns c.bits + 1 (no valid stream has one; the caller's walk stops).
+
+__device__ __forceinline__ u64 next_huf(const uint8_t* s, const CodeDesc& c, const u32* lut,
+                                        u64 p, u64& v) {
+  const u64 w = peek_msb(s, c.base_bit + p);
+  const u32 e = lut[w >> (64 - c.lb)];
+  const u32 len = e >> 24;
+  if (len) {
+    v = e & 0xFFFFFFu;
+    return p + len;
+  }
+  u32 node = e & 0xFFFFFFu;
+  for (u32 i = c.lb; i < 64; ++i) {  // codes are at most 32 bits long
+    const u32 ch = c.tree[2 * node + ((w >> (63 - i)) & 1)];
+    if (ch == kTreeNull) return c.bits + 1;
+    if (ch & kTreeLeaf) {
+      v = ch & ~kTreeLeaf;
+      return p + i + 1;
+    }
+    node = ch;
+  }
+  return c.bits + 1;
+}
+
+// huffman_encoder.hpp:365-396: z zeros, then n1 = (n >> k) + 1 in z + 1 bits
+// MSB-first, then the k low bits of n LSB-first.
+__device__ __forceinline__ u64 next_eg(const uint8_t* s, const CodeDesc& c, u64 p, u64& v) {
+  const u64 w = peek_msb(s, c.base_bit + p);
+  if (!w) return c.bits + 1;  // more than 63 zeros: no such code word here
+  const u32 z = (u32)__clzll((i64)w);
+  const u64 q = p + z + 1 + z + c.k;  // end of the code word
+  if (q > c.bits) return c.bits + 1;
+  u64 n1;
+  if (2 * z + 1 <= 64) {
+    n1 = (w << z) >> (63 - z);
+  } else {
+    n1 = peek_msb(s, c.base_bit + p + z) >> (63 - z);
+  }
+  u64 low = 0;
+  if (c.k) {
+    const u64 t = peek_msb(s, c.base_bit + p + 2 * z + 1) >> (64 - c.k);
+    low = __builtin_bitreverse64(t) >> (64 - c.k);
+  }
+  v = ((n1 - 1) << c.k) | low;
+  return q;
+}
+
+// golomb_rice_encoder.hpp:92-148: q one bits, a zero, then the k low bits
+// LSB-first.
+__device__ __forceinline__ u64 next_gr(const uint8_t* s, const CodeDesc& c, u64 p, u64& v) {
+  u64 q = 0;
+  u64 w = peek_lsb(s, c.base_bit + p);
+  while (w == ~0ull) {
+    q += 64;
+    if (p + q >= c.bits) return c.bits + 1;
+    w = peek_lsb(s, c.base_bit + p + q);
+  }
+  q += (u64)__builtin_ctzll(~w);
+  const u64 e = p + q + 1 + c.k;
+  if (e > c.bits) return c.bits + 1;
+  u64 r = 0;
+  if (c.k) {
+    r = peek_lsb(s, c.base_bit + p + q + 1);
+    if (c.k < 64) r &= (1ull << c.k) - 1;
+  }
+  v = (q << c.k) + r;
+  return e;
+}
+
+template <u32 kKind>
+__device__ __forceinline__ u64 next_code(const uint8_t* s, const CodeDesc& c, const u32* lut,
+                                         u64 p, u64& v) {
+  if constexpr (kKind == kCodeHuf) return next_huf(s, c, lut, p, v);
+  if constexpr (kKind == kCodeEG) return next_eg(s, c, p, v);
+  return next_gr(s, c, p, v);
+}
+
+// The Huffman lookup table goes to LDS (at most 2^kLutBits entries).
+template <u32 kKind>
+__device__ __forceinline__ const u32* stage_lut(const CodeDesc& c, u32* sh) {
+  if constexpr (kKind != kCodeHuf) return nullptr;
+  const u32 n = 1u << c.lb;
+  for (u32 i = threadIdx.x; i < n; i += blockDim.x) sh[i] = c.lut[i];
+  __syncthreads();
+  return sh;
+}
+
+// ---- self-synchronising decode ------------------------------------------------
+
+template <u32 kKind>
+__global__ __launch_bounds__(kBlock) void k_pd_spec(const uint8_t* __restrict__ s, CodeDesc c,
+                                                    u64 nsub, u64* __restrict__ start,
+                                                    u64* __restrict__ exitp,
+                                                    u32* __restrict__ cnt) {
+  __shared__ u32 lsh[1u << kLutBits];
+  const u32* lut = stage_lut<kKind>(c, lsh);
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsub) return;
+  const u64 b = t * kSub, end = b + kSub < c.bits ? b + kSub : c.bits;
+  u64 p = b, v;
+  u32 n = 0;
+  while (p < end) {
+    p = next_code<kKind>(s, c, lut, p, v);
+    ++n;
+  }
+  start[t] = b;
+  exitp[t] = p;
+  cnt[t] = n;
+}
+
+// One round: sub-sequence t takes the exit of t - 1 from the previous round
+// as its entry and walks the new path and its current one in step (always
+// advancing the one behind) until they meet at a code-word start inside its
+// range; the old path's remaining count and exit then stand.
+template <u32 kKind>
+__global__ __launch_bounds__(kBlock) void k_pd_sync(const uint8_t* __restrict__ s, CodeDesc c,
+                                                    u64 nsub, u64* __restrict__ start,
+                                                    const u64* __restrict__ exit_in,
+                                                    u64* __restrict__ exit_out,
+                                                    u32* __restrict__ cnt,
+                                                    u32* __restrict__ changed) {
+  __shared__ u32 lsh[1u << kLutBits];
+  const u32* lut = stage_lut<kKind>(c, lsh);
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsub) return;
+  const u64 old_exit = exit_in[t];
+  if (t == 0) {
+    exit_out[0] = old_exit;
+    return;
+  }
+  const u64 entry = exit_in[t - 1], cur = start[t];
+  if (entry == cur) {
+    exit_out[t] = old_exit;
+    return;
+  }
+  const u64 b = t * kSub, end = b + kSub < c.bits ? b + kSub : c.bits;
+  u64 p = entry, q = cur, v;
+  u32 np = 0, nq = 0;
+  while (p != q && p < end && q < end) {
+    if (p < q) {
+      p = next_code<kKind>(s, c, lut, p, v);
+      ++np;
+    } else {
+      q = next_code<kKind>(s, c, lut, q, v);
+      ++nq;
+    }
+  }
+  u64 ex;
+  u32 n;
+  if (p == q && p < end) {  // in step: the rest of the old path stands
+    ex = old_exit;
+    n = np + (cnt[t] - nq);
+  } else {
+    while (p < end) {
+      p = next_code<kKind>(s, c, lut, p, v);
+      ++np;
+    }
+    ex = p;
+    n = np;
+  }
+  start[t] = entry;
+  cnt[t] = n;
+  exit_out[t] = ex;
+  if (ex != old_exit) atomicOr(changed, 1u);
+}
+
+template <u32 kKind, typename T>
+__global__ __launch_bounds__(kBlock) void k_pd_write(const uint8_t* __restrict__ s, CodeDesc c,
+                                                     u64 nsub, const u64* __restrict__ start,
+                                                     const u32* __restrict__ base,
+                                                     const u32* __restrict__ cnt, u64 count,
+                                                     T* __restrict__ out) {
+  __shared__ u32 lsh[1u << kLutBits];
+  const u32* lut = stage_lut<kKind>(c, lsh);
+  const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsub) return;
+  u64 p = start[t], o = base[t];
+  const u32 n = cnt[t];
+  for (u32 i = 0; i < n && o < count; ++i, ++o) {
+    u64 v = 0;
+    p = next_code<kKind>(s, c, lut, p, v);
+    if constexpr (kKind == kCodeHuf) v = c.values[v];
+    if (c.zigzag) v = (u64)(i64)((v >> 1) ^ (~(v & 1) + 1));
+    out[o] = (T)v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_const(T* out, u64 n, T v) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    out[i] = v;
+}
+
+template <u32 kKind, typename T>
+void decode_section_t(const uint8_t* src, const CodeDesc& c, u64 count, T* out, DecScratch& w,
+                      Arena& ar, hipStream_t s) {
+  const u64 nsub = (c.bits + kSub - 1) / kSub;
+  if (nsub >= 0xFFFFFFFFull) throw Error(-5, "entropy-coded section too long");
+  const u32 g = blocks_for(nsub, kBlock);
+  u64* start = ar.take<u64>(nsub);
+  u64* ex[2] = {ar.take<u64>(nsub), ar.take<u64>(nsub)};
+  u32* cnt = ar.take<u32>(nsub);
+  u32* base = ar.take<u32>(nsub);
+  u32* flag = ar.take<u32>(2);
+  k_pd_spec<kKind><<<g, kBlock, 0, s>>>(src, c, nsub, start, ex[0], cnt);
+  LAUNCH_CHECK("k_pd_spec");
+  int cur = 0;
+  for (u64 round = 0;; ++round) {
+    if (round > nsub + 1) throw Error(-7, "decoder: sub-sequences did not settle");
+    k_fill_const<u32><<<1, 64, 0, s>>>(flag, 1, 0u);
+    LAUNCH_CHECK("k_fill_const");
+    k_pd_sync<kKind><<<g, kBlock, 0, s>>>(src, c, nsub, start, ex[cur], ex[cur ^ 1], cnt, flag);
+    LAUNCH_CHECK("k_pd_sync");
+    cur ^= 1;
+    check_hip(hipMemcpyAsync(w.host_flag, flag, 4, hipMemcpyDeviceToHost, s), "flag D2H");
+    check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+    if (!*w.host_flag) break;
+  }
+  scan_excl_u32(cnt, base, nsub, flag + 1, ar, s);
+  check_hip(hipMemcpyAsync(w.host_flag, flag + 1, 4, hipMemcpyDeviceToHost, s), "total D2H");
+  check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (*w.host_flag < count) throw Error(-6, "entropy-coded section holds fewer values than its count");
+  k_pd_write<kKind, T><<<g, kBlock, 0, s>>>(src, c, nsub, start, base, cnt, count, out);
+  LAUNCH_CHECK("k_pd_write");
+}
+
+// ---- fixed-width sections and block starts -----------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_bitunpack(const uint8_t* __restrict__ src, u64 base_bit,
+                                                      u32 w, u64 n, u64* __restrict__ out) {
+  const u64 mask = w >= 64 ? ~0ull : ((1ull << w) - 1);
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    out[i] = peek_lsb(src, base_bit + i * w) & mask;
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_u64(u64* a, const u64* b, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    a[i] += b[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_heads(const u64* start, u64 n, uint8_t* flags) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    flags[start[i]] = 1;
+}
+
+// ---- LCP / direct reconstruction ----------------------------------------------
+
+// Prefix state over the coded elements.  Plain counts: nb (block starts), nz
+// (patch symbols).  Segmented at block starts: q (sum of quad deltas).
+// Segmented at "repos heads" (a block start or a non-zero quad delta,
+// block_quantizer.hpp:131-136): a (sum of the non-patch repos deltas), zsr
+// (patch symbols before the last repos head).  Direct mode uses nz and a,
+// with no heads.
+struct LState {
+  u64 a;
+  u32 nb, nz, q, zsr;
+  u32 fb, fr;
+};
+
+__device__ __forceinline__ LState lcombine(const LState& x, const LState& y) {
+  LState r;
+  r.nb = x.nb + y.nb;
+  r.nz = x.nz + y.nz;
+  r.q = y.fb ? y.q : x.q + y.q;
+  r.fb = x.fb | y.fb;
+  r.a = y.fr ? y.a : x.a + y.a;
+  r.zsr = y.fr ? x.nz + y.zsr : x.zsr;
+  r.fr = x.fr | y.fr;
+  return r;
+}
+
+__device__ __forceinline__ LState lzero() {
+  LState r;
+  r.a = 0;
+  r.nb = r.nz = r.q = r.zsr = r.fb = r.fr = 0;
+  return r;
+}
+
+__device__ __forceinline__ LState lelem(const LcpArgs& A, u64 j) {
+  LState e = lzero();
+  const u32 sym = A.sym[j];
+  e.nz = sym == 0;
+  e.a = sym ? sym - 1 : 0;
+  if (!A.direct) {
+    const u32 bh = A.head[j];
+    const u32 qd = (u32)A.quad[j] - 1u;  // quads PFD symbol v + 1
+    e.nb = bh;
+    e.q = qd;
+    e.fb = bh;
+    e.fr = bh | (qd != 0);
+  }
+  return e;
+}
+
+// Ordered block reduction / exclusive scan of one LState per thread (LDS).
+__device__ LState block_excl(LState v, LState* sh, LState& total) {
+  const u32 t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (u32 o = 1; o < kBlock; o <<= 1) {
+    LState x = t >= o ? lcombine(sh[t - o], sh[t]) : sh[t];
+    __syncthreads();
+    sh[t] = x;
+    __syncthreads();
+  }
+  total = sh[kBlock - 1];
+  LState ex = t ? sh[t - 1] : lzero();
+  __syncthreads();
+  return ex;
+}
+
+constexpr u32 kLItems = 16;
+constexpr u32 kLTile = kBlock * kLItems;
+
+__global__ __launch_bounds__(kBlock) void k_lcp_reduce(LcpArgs A, LState* tiles) {
+  __shared__ LState sh[kBlock];
+  const u64 b0 = (u64)blockIdx.x * kLTile + (u64)threadIdx.x * kLItems;
+  LState acc = lzero();
+  for (u32 i = 0; i < kLItems; ++i)
+    if (b0 + i < A.n) acc = lcombine(acc, lelem(A, b0 + i));
+  LState tot;
+  (void)block_excl(acc, sh, tot);
+  if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of the tile totals, in place.
+__global__ __launch_bounds__(kBlock) void k_lcp_tiles(LState* tiles, u32 nt) {
+  __shared__ LState sh[kBlock];
+  LState carry = lzero();
+  for (u32 base = 0; base < nt; base += kBlock) {
+    const u32 i = base + threadIdx.x;
+    const LState v = i < nt ? tiles[i] : lzero();
+    LState tot;
+    const LState ex = block_excl(v, sh, tot);
+    if (i < nt) tiles[i] = lcombine(carry, ex);
+    carry = lcombine(carry, tot);
+  }
+}
+
+__device__ __forceinline__ u32 compact3(u64 c) {  // z_order_curve.hpp:32-40
+  u64 x = c & 0x1249249249249249ull;
+  x = (x | (x >> 2)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x >> 4)) & 0x100f00f00f00f00full;
+  x = (x | (x >> 8)) & 0x1f0000ff0000ffull;
+  x = (x | (x >> 16)) & 0x1f00000000ffffull;
+  x = (x | (x >> 32)) & 0x1fffff;
+  return (u32)x;
+}
+
+// hilbertToMorton3(h, L) (hilbert.cpp:396-421): digits L..0 from the top
+// through the machine; digits above L stay.  Digit 21 is bit 63 alone.
+__device__ __forceinline__ u64 h2m(u64 h, u32 L, const u8* apply, const u8* next) {
+  u32 st = 0;
+  u64 r = h;
+  for (int i = (int)L; i >= 0; --i) {
+    const u32 sh = 3u * (u32)i;
+    const u32 hd = (u32)((h >> sh) & 7);
+    const u32 md = apply[st * 8 + hd];
+    const u64 mask = sh < 63 ? (7ull << sh) : (1ull << 63);
+    r = (r & ~mask) | (((u64)md << sh) & mask);
+    st = next[st * 8 + hd];
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_lcp_apply(LcpArgs A, const LState* tiles,
+                                                      HilbertInv hi) {
+  __shared__ LState sh[kBlock];
+  __shared__ u8 ap[24 * 8], nx_[24 * 8];
+  for (u32 i = threadIdx.x; i < 24 * 8; i += kBlock) {
+    ap[i] = hi.apply[i];
+    nx_[i] = hi.next[i];
+  }
+  const u64 b0 = (u64)blockIdx.x * kLTile + (u64)threadIdx.x * kLItems;
+  LState acc = lzero();
+  for (u32 i = 0; i < kLItems; ++i)
+    if (b0 + i < A.n) acc = lcombine(acc, lelem(A, b0 + i));
+  LState tot;
+  LState run = lcombine(tiles[blockIdx.x], block_excl(acc, sh, tot));
+  for (u32 i = 0; i < kLItems; ++i) {
+    const u64 j = b0 + i;
+    if (j >= A.n) break;
+    run = lcombine(run, lelem(A, j));
+    u64 cx, cy, cz;
+    if (A.direct) {  // block_decompressor.hpp:82-101
+      const u64 sfc = run.a + A.ps[run.nz < A.np ? run.nz : A.np];
+      const u64 m = A.hilbert ? h2m(sfc, A.bits, ap, nx_) : sfc;
+      cx = compact3(m);
+      cy = compact3(m >> 1);
+      cz = compact3(m >> 2);
+    } else {  // block_quantizer.hpp:166-203
+      const u64 id = A.blkst[run.nb - 1];
+      const u64 pbx = (id % A.nx * 2) * 64, pby = (id / A.nx % A.ny * 2) * 64,
+                pbz = (id / A.nx / A.ny * 2) * 64;
+      const u64 rj = run.a + (A.ps[run.nz < A.np ? run.nz : A.np] -
+                              A.ps[run.zsr < A.np ? run.zsr : A.np]);
+      const u64 m = A.hilbert ? h2m(rj, 20, ap, nx_) : rj;
+      cx = pbx + (u64)(run.q & 1) * 64 + compact3(m);
+      cy = pby + (u64)((run.q >> 1) & 1) * 64 + compact3(m >> 1);
+      cz = pbz + (u64)((run.q >> 2) & 1) * 64 + compact3(m >> 2);
+    }
+    int n3[3] = {(int)cx, (int)cy, (int)cz};
+    float v[3];
+    if (A.to) {  // truncated_octahedron_quantizer.hpp:57-76
+      if (n3[2] & 1) {
+        n3[0] = (n3[0] << 1) - 1;
+        n3[1] = (n3[1] << 1) - 1;
+      } else {
+        n3[0] <<= 1;
+        n3[1] <<= 1;
+      }
+      for (int a = 0; a < 3; ++a) v[a] = ((float(n3[a]) * 2.0f) * A.b) / A.rec_div;
+    } else {  // cube_quantizer.hpp:29-38
+      for (int a = 0; a < 3; ++a) v[a] = (((float(n3[a]) + 0.5f) * 2.0f) * A.b) / A.rec_div;
+    }
+    for (int a = 0; a < 3; ++a) v[a] = v[a] + A.off[a];  // shifting.hpp:39-50
+    if (!A.rle) {
+      float* o = A.out + 3 * j;
+      o[0] = v[0];
+      o[1] = v[1];
+      o[2] = v[2];
+    } else {  // block_decompressor_rle.hpp:173-187: count copies, in cell order
+      const u64 at = A.out_at[j];
+      if (at >= A.total) continue;
+      float* o = A.out + 3 * at;
+      const u64 c = A.counts[j] < A.total - at ? A.counts[j] : A.total - at;
+      for (u64 k = 0; k < c; ++k) {
+        o[3 * k] = v[0];
+        o[3 * k + 1] = v[1];
+        o[3 * k + 2] = v[2];
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_h2m(const u64* in, u64 n, u32 L, HilbertInv hi,
+                                                u64* out) {
+  __shared__ u8 ap[24 * 8], nx_[24 * 8];
+  for (u32 i = threadIdx.x; i < 24 * 8; i += kBlock) {
+    ap[i] = hi.apply[i];
+    nx_[i] = hi.next[i];
+  }
+  __syncthreads();
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    out[i] = h2m(in[i], L, ap, nx_);
+}
+
+}  // namespace
+
+template <typename T>
+void decode_section(const uint8_t* src, const CodeDesc& c, u64 count, T* out, DecScratch& w,
+                    Arena& ar, hipStream_t s) {
+  if (!count) return;
+  const size_t mark = ar.off;
+  if (c.kind == kCodeConst) {
+    const u32 g = blocks_for(count, kBlock * 16);
+    k_fill_const<T><<<g < 16384 ? g : 16384, kBlock, 0, s>>>(out, count, (T)c.const_value);
+    LAUNCH_CHECK("k_fill_const");
+  } else if (c.kind == kCodeHuf) {
+    if (c.lb < 1 || c.lb > kLutBits) throw Error(-7, "decoder: bad lookup width");
+    decode_section_t<kCodeHuf>(src, c, count, out, w, ar, s);
+  } else if (c.kind == kCodeEG) {
+    decode_section_t<kCodeEG>(src, c, count, out, w, ar, s);
+  } else {
+    decode_section_t<kCodeGR>(src, c, count, out, w, ar, s);
+  }
+  ar.off = mark;
+}
+
+template void decode_section<uint8_t>(const uint8_t*, const CodeDesc&, u64, uint8_t*,
+                                      DecScratch&, Arena&, hipStream_t);
+template void decode_section<u64>(const uint8_t*, const CodeDesc&, u64, u64*, DecScratch&,
+                                  Arena&, hipStream_t);
+
+void launch_bitunpack(const uint8_t* stream, u64 base_bit, u32 w, u64 n, u64* out,
+                      hipStream_t s) {
+  if (!n) return;
+  const u32 g = blocks_for(n, kBlock * 4);
+  k_bitunpack<<<g < 16384 ? g : 16384, kBlock, 0, s>>>(stream, base_bit, w, n, out);
+  LAUNCH_CHECK("k_bitunpack");
+}
+
+void launch_add_u64(u64* a, const u64* b, u64 n, hipStream_t s) {
+  if (!n) return;
+  const u32 g = blocks_for(n, kBlock * 4);
+  k_add_u64<<<g < 16384 ? g : 16384, kBlock, 0, s>>>(a, b, n);
+  LAUNCH_CHECK("k_add_u64");
+}
+
+void launch_mark_heads(const u64* start, u64 n, uint8_t* flags, hipStream_t s) {
+  if (!n) return;
+  const u32 g = blocks_for(n, kBlock * 4);
+  k_mark_heads<<<g < 16384 ? g : 16384, kBlock, 0, s>>>(start, n, flags);
+  LAUNCH_CHECK("k_mark_heads");
+}
+
+void launch_zero_u8(uint8_t* p, u64 n, hipStream_t s) {
+  if (!n) return;
+  const u32 g = blocks_for(n, kBlock * 16);
+  k_fill_const<uint8_t><<<g < 16384 ? g : 16384, kBlock, 0, s>>>(p, n, (uint8_t)0);
+  LAUNCH_CHECK("k_fill_const");
+}
+
+void launch_h2m(const u64* in, u64 n, u32 iteration, const HilbertInv& hi, u64* out,
+                hipStream_t s) {
+  if (!n) return;
+  const u32 g = blocks_for(n, kBlock * 4);
+  k_h2m<<<g < 16384 ? g : 16384, kBlock, 0, s>>>(in, n, iteration, hi, out);
+  LAUNCH_CHECK("k_h2m");
+}
+
+size_t lcp_scratch_bytes(u64 n) { return (size_t)blocks_for(n, kLTile) * sizeof(LState) + 256; }
+
+void launch_lcp(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream_t s) {
+  if (!a.n) return;
+  const u32 nt = blocks_for(a.n, kLTile);
+  LState* tiles = static_cast<LState*>(scratch);
+  k_lcp_reduce<<<nt, kBlock, 0, s>>>(a, tiles);
+  LAUNCH_CHECK("k_lcp_reduce");
+  k_lcp_tiles<<<1, kBlock, 0, s>>>(tiles, nt);
+  LAUNCH_CHECK("k_lcp_tiles");
+  k_lcp_apply<<<nt, kBlock, 0, s>>>(a, tiles, hi);
+  LAUNCH_CHECK("k_lcp_apply");
+}
+
+}  // namespace xnyz

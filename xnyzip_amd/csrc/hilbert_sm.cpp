@@ -20,6 +20,7 @@
 #include <set>
 #include <vector>
 
+#include "decode_dev.hpp"
 #include "kernels.hpp"
 
 namespace xnyz {
@@ -129,6 +130,44 @@ uint64_t hilbert_to_morton3(uint64_t h, unsigned iteration) {
     }
   }
   return r;
+}
+
+// hilbertToMorton3 as a machine over the same rotation group, walked from the
+// top digit down.  The bottom-up loop of the reference maps Hilbert digit i
+// to Morton digit md_i = h2m[h_i] (digit i is not yet touched when its turn
+// comes) and then permutes every digit below i by h2m_perms[md_i]; so Morton
+// digit j ends as P_L(...P_{j+1}(h2m[h_j])), P_i = h2m_perms[h2m[h_i]].  Top
+// down: state Q (identity first), digit j -> Q(h2m[h_j]), then
+// Q <- Q o P_j.
+HilbertInv build_hilbert_inv() {
+  const Tables t;
+  std::vector<P8> states;
+  std::map<P8, int> index;
+  P8 id{};
+  for (int i = 0; i < 8; ++i) id[i] = static_cast<u8>(i);
+  states.push_back(id);
+  index[id] = 0;
+  HilbertInv inv{};
+  for (size_t s = 0; s < states.size(); ++s) {
+    for (int hd = 0; hd < 8; ++hd) {
+      const u8 md = t.h2m[hd];
+      inv.apply[s * 8 + hd] = states[s][md];
+      P8 nxt{};
+      for (int x = 0; x < 8; ++x) nxt[x] = states[s][t.h2m_perms[md][x]];
+      auto it = index.find(nxt);
+      int ni;
+      if (it == index.end()) {
+        ni = static_cast<int>(states.size());
+        if (ni >= 24) throw Error(-7, "hilbert: more than 24 inverse states");
+        index[nxt] = ni;
+        states.push_back(nxt);
+      } else {
+        ni = it->second;
+      }
+      inv.next[s * 8 + hd] = static_cast<u8>(ni);
+    }
+  }
+  return inv;
 }
 
 HilbertSM build_hilbert_sm() {
