@@ -350,6 +350,48 @@ int xnyz_decompress_to_device(xnyz_ctx* ctx, const uint8_t* in, uint64_t len,
 int xnyz_hilbert_to_morton(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
                            uint32_t iteration, uint64_t* out);
 
+/* ---- Multi-chunk container ------------------------------------------------
+ * A part file holding the chunks' outputs back to back and a manifest of
+ * one JSON line per chunk, in the shape of the MPI driver's per-rank part
+ * file and manifest (source/mpi_driver.cpp:709-714, :902-914: block_id,
+ * input_offset, input_size, rank, part_offset, compressed_size), with each
+ * chunk's coding mode added: quantizer, l2_bound, curve, rle, gr_k,
+ * pfd_width, direct_threshold, frame ("zstd" or "stream").  The stream does
+ * not record the RLE flag or the quantizer, and a learned batch codes every
+ * chunk its own way, so the container keeps them.  Manifests written by the
+ * reference's driver read too (mode fields absent: TO, zstd frames; the
+ * caller supplies bound, curve and RLE). */
+typedef struct xnyz_chunk {
+  uint64_t input_offset;     /* byte offset of the chunk in the input (12 per point) */
+  uint64_t input_points;
+  uint64_t part_offset;      /* set by xnyz_container_write */
+  uint64_t compressed_size;  /* set by xnyz_container_write */
+  int32_t quantizer;         /* enum xnyz_quantizer */
+  float l2_bound;
+  int32_t is_hilbert;
+  int32_t rle;
+  int32_t gr_k;              /* -1: not recorded */
+  int32_t pfd_width;
+  int32_t direct_threshold;
+  int32_t frame_mode;        /* XNYZ_FRAME_NONE: assembled stream; otherwise a zstd frame */
+} xnyz_chunk;
+
+/* Writes chunk s's output out[out_offsets[s], out_offsets[s + 1]) (the layout
+ * of the _into entry points) to the part file and its line to the manifest;
+ * fills each chunk's part_offset and compressed_size. */
+int xnyz_container_write(const char* part_path, const char* manifest_path, const uint8_t* out,
+                         const uint64_t* out_offsets, uint32_t count, xnyz_chunk* chunks);
+
+/* The manifest's chunks: chunks->data holds len / sizeof(xnyz_chunk)
+ * entries (release with xnyz_free). */
+int xnyz_container_read(const char* manifest_path, xnyz_buf* chunks);
+
+/* Decodes every chunk with its own mode, in manifest order, on the device of
+ * ctx (or on the host when ctx is NULL): out_points gets all points, chunk
+ * after chunk, each in its decoder's order (x, y, z floats). */
+int xnyz_container_decompress(xnyz_ctx* ctx, const char* part_path, const char* manifest_path,
+                              xnyz_buf* out_points);
+
 /* Stage-2 launch sequences (key generation, radix passes, RLE cells, block
  * runs, statistics: some 40 kernels) are captured into a hipGraph per launch
  * shape and replayed while the shape repeats (same cloud sizes and options

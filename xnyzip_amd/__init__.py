@@ -14,6 +14,7 @@ libxnyz.so's HIP kernels; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from enum import IntEnum
 
 import numpy as np
@@ -25,7 +26,8 @@ __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDeco
            "BlockDecompressorRLE", "decompress", "Context", "XnyzError",
            "compress", "compress_batch", "compress_batch_into", "compress_batch_noalloc", "host_buffer", "search_batch", "features_batch", "mlp_train",
            "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune",
-           "decompress_device", "decompress_to_device", "hilbert_to_morton"]
+           "decompress_device", "decompress_to_device", "hilbert_to_morton",
+           "container_write", "container_read", "container_decompress"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -478,6 +480,55 @@ def hilbert_to_morton(codes, iteration: int, ctx: Context | None = None) -> np.n
     _lib.check(_lib.lib().xnyz_hilbert_to_morton(ctx.handle, C.c_void_p(a.ctypes.data), a.size,
                                                   int(iteration), C.c_void_p(out.ctypes.data)))
     return out
+
+
+def container_write(part_path: str, manifest_path: str, out: np.ndarray, offsets, chunks):
+    """Writes a multi-chunk container (xnyz_container_write): chunk s is
+    out[offsets[s]:offsets[s + 1]], described by chunks[s] (dict with
+    input_offset, input_points, quantizer, l2_bound, is_hilbert, rle and
+    optionally gr_k, pfd_width, direct_threshold, frame_mode).  Returns the
+    chunk dicts with part_offset and compressed_size filled."""
+    count = len(offsets) - 1
+    if count != len(chunks):
+        raise ValueError("need one chunk entry per output")
+    buf = np.ascontiguousarray(out, dtype=np.uint8)
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    arr = (_lib.Chunk * count)()
+    for i, c in enumerate(chunks):
+        arr[i] = _lib.Chunk(int(c.get("input_offset", 0)), int(c.get("input_points", 0)), 0, 0,
+                            int(c["quantizer"]), float(c["l2_bound"]),
+                            int(bool(c.get("is_hilbert", False))), int(bool(c.get("rle", False))),
+                            int(c.get("gr_k", -1)), int(c.get("pfd_width", 8)),
+                            int(c.get("direct_threshold", 1024)),
+                            int(c.get("frame_mode", _lib.FRAME_ZSTD3)))
+    _lib.check(_lib.lib().xnyz_container_write(os.fsencode(part_path), os.fsencode(manifest_path),
+                                                C.c_void_p(buf.ctypes.data),
+                                                C.c_void_p(off.ctypes.data), count,
+                                                C.cast(arr, C.c_void_p)))
+    return [a.as_dict() for a in arr]
+
+
+def container_read(manifest_path: str):
+    """The chunk dicts of a container's manifest."""
+    b = _lib.Buf()
+    _lib.check(_lib.lib().xnyz_container_read(os.fsencode(manifest_path), C.byref(b)))
+    raw = _lib.take(b)
+    n = len(raw) // C.sizeof(_lib.Chunk)
+    arr = (_lib.Chunk * n).from_buffer_copy(raw) if n else []
+    return [a.as_dict() for a in arr]
+
+
+def container_decompress(part_path: str, manifest_path: str, gpu: bool = True,
+                         ctx: Context | None = None) -> np.ndarray:
+    """All points of a container, chunk after chunk (xnyz_container_decompress):
+    decoded on the GPU, or on the host with gpu=False."""
+    if gpu:
+        ctx = ctx or default_context(0)
+    b = _lib.Buf()
+    _lib.check(_lib.lib().xnyz_container_decompress(ctx.handle if gpu else None,
+                                                     os.fsencode(part_path),
+                                                     os.fsencode(manifest_path), C.byref(b)))
+    return np.frombuffer(_lib.take(b), dtype=np.float32).reshape(-1, 3).copy()
 
 
 class BlockDecompressor:

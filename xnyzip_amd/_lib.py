@@ -95,6 +95,17 @@ class Stats(C.Structure):
         return out
 
 
+class Chunk(C.Structure):
+    _fields_ = [("input_offset", C.c_uint64), ("input_points", C.c_uint64),
+                ("part_offset", C.c_uint64), ("compressed_size", C.c_uint64),
+                ("quantizer", C.c_int32), ("l2_bound", C.c_float), ("is_hilbert", C.c_int32),
+                ("rle", C.c_int32), ("gr_k", C.c_int32), ("pfd_width", C.c_int32),
+                ("direct_threshold", C.c_int32), ("frame_mode", C.c_int32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -106,7 +117,8 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress_selected_batch_device_into",
            "xnyz_quantize_device",
            "xnyz_morton_to_hilbert", "xnyz_hilbert_to_morton", "xnyz_decompress",
-           "xnyz_decompress_device", "xnyz_decompress_to_device", "xnyz_graphs_enable",
+           "xnyz_decompress_device", "xnyz_decompress_to_device", "xnyz_container_write",
+           "xnyz_container_read", "xnyz_container_decompress", "xnyz_graphs_enable",
            "xnyz_tuning_set", "xnyz_tuning_get", "xnyz_tuning_save", "xnyz_tuning_load",
            "xnyz_autotune", "xnyz_profile_enable", "xnyz_profile_report", "xnyz_free",
            "xnyz_host_alloc", "xnyz_host_free"]
@@ -151,6 +163,9 @@ def lib() -> C.CDLL:
         L.xnyz_decompress_to_device.argtypes = [P, P, C.c_uint64, C.POINTER(Opts), P, C.c_uint64,
                                                 C.POINTER(C.c_uint64)]
         L.xnyz_hilbert_to_morton.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
+        L.xnyz_container_write.argtypes = [C.c_char_p, C.c_char_p, P, P, C.c_uint32, P]
+        L.xnyz_container_read.argtypes = [C.c_char_p, C.POINTER(Buf)]
+        L.xnyz_container_decompress.argtypes = [P, C.c_char_p, C.c_char_p, C.POINTER(Buf)]
         L.xnyz_profile_enable.argtypes = [P, C.c_int]
         L.xnyz_graphs_enable.argtypes = [P, C.c_int]
         L.xnyz_tuning_set.argtypes = [P, C.c_int, C.POINTER(Tuning)]

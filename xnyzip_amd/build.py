@@ -31,7 +31,7 @@ DEVICE = [f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = ["kernels.hip", "mlp.hip", "zframe.hip", "noalloc.hip", "decode_dev.hip",
-           "hilbert_sm.cpp", "api.cpp", "decode.cpp"]
+           "hilbert_sm.cpp", "api.cpp", "decode.cpp", "container.cpp"]
 
 
 def _run(cmd):

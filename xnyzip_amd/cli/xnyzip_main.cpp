@@ -3,6 +3,17 @@
 //   XnYZip <input_file> <cube|to> <L2 bound> <-z|-h> <-rle|-normal>
 //          <decompression_file_name> [direct_threshold] [--offset=B] [--count=B]
 //
+// and, for many chunks of one file on one GPU (the MPI driver's per-block
+// loop, source/mpi_driver.cpp:819-943, as one batched call), a container of
+// part_rank0000.bin + manifest_rank0000.jsonl in the driver's shape:
+//
+//   XnYZip --batch <input_file> <cube|to> <L2 bound> <-z|-h|-auto> <-rle|-normal|-auto>
+//          <output_dir> [--chunk-points=N] [--stream]
+//   XnYZip --unbatch <output_dir> <decompression_file_name> [--cpu]
+//
+// -auto lets the learned selector (falling back to the exhaustive search)
+// pick each chunk's curve and RLE flag; the manifest records the choice.
+//
 // Same arguments, chunk flags (main.cpp:15-45, source/io/fileUtils.hpp:44-97)
 // and stdout lines as the reference: the lines its harness scripts parse
 // ("compression ratio:", "chunk input bytes:", "chunk compressed bytes:",
@@ -20,6 +31,9 @@
 #include <iostream>
 #include <string>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
+#include <unistd.h>
 
 #include "xnyz.h"
 
@@ -91,9 +105,172 @@ int die(const char* what) {
   return 1;
 }
 
+std::string flag_value(int argc, char** argv, const char* name) {
+  const std::string k = std::string(name) + "=";
+  for (int i = 1; i < argc; ++i)
+    if (std::string(argv[i]).rfind(k, 0) == 0) return std::string(argv[i]).substr(k.size());
+  return "";
+}
+
+bool has_flag(int argc, char** argv, const char* name) {
+  for (int i = 1; i < argc; ++i)
+    if (std::string(argv[i]) == name) return true;
+  return false;
+}
+
+// The packaged selector parameters (xnyzip_amd/data/selector_params.npy,
+// next to the binary's ../data): a version-1 .npy of float32, read as raw
+// little-endian floats after its header.
+std::vector<float> load_selector() {
+  char self[4096];
+  const ssize_t n = readlink("/proc/self/exe", self, sizeof(self) - 1);
+  if (n <= 0) throw std::runtime_error("cannot locate the executable");
+  self[n] = 0;
+  std::string dir(self);
+  dir = dir.substr(0, dir.rfind('/'));
+  const std::string path = dir + "/../data/selector_params.npy";
+  std::ifstream f(path, std::ios::binary);
+  if (!f.is_open()) throw std::runtime_error("cannot open " + path);
+  std::vector<char> b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  if (b.size() < 10 || std::memcmp(b.data(), "\x93NUMPY", 6) != 0)
+    throw std::runtime_error(path + ": not a .npy file");
+  const size_t hl = (unsigned char)b[8] | (size_t)(unsigned char)b[9] << 8;
+  const size_t at = 10 + hl;
+  if (b.size() != at + XNYZ_MLP_PARAMS * 4) throw std::runtime_error(path + ": wrong size");
+  std::vector<float> p(XNYZ_MLP_PARAMS);
+  std::memcpy(p.data(), b.data() + at, p.size() * 4);
+  return p;
+}
+
+int run_batch(int argc, char** argv, const std::vector<char*>& pos) {
+  if (pos.size() != 8) {
+    std::printf("Usage: %s --batch <input_file> <cube|to> <L2 bound> <-z|-h|-auto> "
+                "<-rle|-normal|-auto> <output_dir> [--chunk-points=N] [--stream]\n", argv[0]);
+    return 1;
+  }
+  const std::vector<float> pts = read_points(pos[2], ChunkSpec{});
+  const uint64_t n = pts.size() / 3;
+  if (!n) {
+    std::fprintf(stderr, "empty input\n");
+    return 1;
+  }
+  const std::string cs = flag_value(argc, argv, "--chunk-points");
+  const uint64_t chunk = cs.empty() ? (1u << 20) : std::stoull(cs);
+  if (!chunk) {
+    std::fprintf(stderr, "--chunk-points must be positive\n");
+    return 1;
+  }
+  const std::string curve = pos[5], rmode = pos[6];
+  const bool learned = curve == "-auto" || rmode == "-auto";
+  std::vector<uint64_t> offs;
+  for (uint64_t a = 0; a < n; a += chunk) offs.push_back(a);
+  offs.push_back(n);
+  const uint32_t count = uint32_t(offs.size() - 1);
+  xnyz_opts base{};
+  base.quantizer = std::string(pos[3]) == "to" ? XNYZ_TRUNCATED_OCTAHEDRON : XNYZ_CUBE;
+  base.l2_bound = std::stof(pos[4]);
+  base.is_hilbert = curve == "-h";
+  base.rle = rmode == "-rle";
+  base.direct_threshold = 1024;
+  base.frame_mode = has_flag(argc, argv, "--stream") ? XNYZ_FRAME_NONE : XNYZ_FRAME_ZSTD3;
+  std::vector<xnyz_opts> opts(count, base);
+  auto t0 = std::chrono::steady_clock::now();
+  xnyz_ctx* ctx = nullptr;
+  if (xnyz_create(0, &ctx) != XNYZ_OK) return die("xnyz_create");
+  float* d = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d), n * 12) != hipSuccess ||
+      hipMemcpy(d, pts.data(), n * 12, hipMemcpyHostToDevice) != hipSuccess) {
+    std::fprintf(stderr, "XnYZip: device upload failed\n");
+    return 1;
+  }
+  std::vector<uint64_t> oo(count + 1);
+  std::vector<xnyz_stats> st(count);
+  std::vector<xnyz_mode> modes(count);
+  std::vector<uint8_t> out(n * 13 + (1u << 20));
+  int rc;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (learned) {
+      const std::vector<float> params = load_selector();
+      rc = xnyz_compress_selected_batch_device_into(ctx, d, offs.data(), count, opts.data(),
+                                                    params.data(), 0.5f, out.data(), out.size(),
+                                                    oo.data(), st.data(), modes.data(), nullptr);
+    } else {
+      rc = xnyz_compress_batch_device_into(ctx, d, offs.data(), count, opts.data(), out.data(),
+                                           out.size(), oo.data(), st.data());
+      for (uint32_t s = 0; s < count; ++s)
+        modes[s] = xnyz_mode{base.is_hilbert, base.rle, -1, 8};
+    }
+    if (rc != XNYZ_ENOSPC) break;
+    out.resize(oo[count] + (1u << 20));
+  }
+  if (rc != XNYZ_OK) return die("batched compression");
+  const double tc = seconds_since(t0);
+  std::vector<xnyz_chunk> ch(count);
+  for (uint32_t s = 0; s < count; ++s) {
+    xnyz_chunk& c = ch[s];
+    c.input_offset = offs[s] * 12;
+    c.input_points = offs[s + 1] - offs[s];
+    c.quantizer = base.quantizer;
+    c.l2_bound = base.l2_bound;
+    c.is_hilbert = modes[s].is_hilbert;
+    c.rle = modes[s].rle;
+    c.gr_k = modes[s].gr_k;
+    c.pfd_width = modes[s].pfd_width;
+    c.direct_threshold = base.direct_threshold;
+    c.frame_mode = base.frame_mode;
+  }
+  const std::string dir = pos[7];
+  const std::string part = dir + "/part_rank0000.bin", man = dir + "/manifest_rank0000.jsonl";
+  if (xnyz_container_write(part.c_str(), man.c_str(), out.data(), oo.data(), count, ch.data()) !=
+      XNYZ_OK)
+    return die("xnyz_container_write");
+  const uint64_t bytes = oo[count] - oo[0];
+  std::printf("chunks: %u\n", count);
+  std::printf("chunk input bytes: %lu\n", (unsigned long)(n * 12));
+  std::printf("chunk compressed bytes: %lu\n", (unsigned long)bytes);
+  std::printf("compression ratio: %f\n", (double)n * 12 / (double)bytes);
+  std::cout << "XnYZip compression time = " << tc << "s" << std::endl;
+  (void)hipFree(d);
+  xnyz_destroy(ctx);
+  return 0;
+}
+
+int run_unbatch(int argc, char** argv, const std::vector<char*>& pos) {
+  if (pos.size() != 4) {
+    std::printf("Usage: %s --unbatch <output_dir> <decompression_file_name> [--cpu]\n", argv[0]);
+    return 1;
+  }
+  const std::string dir = pos[2];
+  const std::string part = dir + "/part_rank0000.bin", man = dir + "/manifest_rank0000.jsonl";
+  xnyz_ctx* ctx = nullptr;
+  if (!has_flag(argc, argv, "--cpu") && xnyz_create(0, &ctx) != XNYZ_OK) return die("xnyz_create");
+  auto t0 = std::chrono::steady_clock::now();
+  xnyz_buf b{};
+  if (xnyz_container_decompress(ctx, part.c_str(), man.c_str(), &b) != XNYZ_OK)
+    return die("xnyz_container_decompress");
+  std::cout << "XnYZip decompression time = " << seconds_since(t0) << "s" << std::endl;
+  std::printf("decompressed finished, size: %lu\n", (unsigned long)(b.len / 12));
+  write_floats(pos[3], reinterpret_cast<const float*>(b.data), b.len / 4);
+  xnyz_free(&b);
+  if (ctx) xnyz_destroy(ctx);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+  {
+    std::vector<char*> p;
+    for (int i = 0; i < argc; ++i)
+      if (std::string(argv[i]).rfind("--", 0) != 0 || i == 1) p.push_back(argv[i]);
+    try {
+      if (argc > 1 && std::string(argv[1]) == "--batch") return run_batch(argc, argv, p);
+      if (argc > 1 && std::string(argv[1]) == "--unbatch") return run_unbatch(argc, argv, p);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "%s\n", e.what());
+      return 1;
+    }
+  }
   ChunkSpec chunk = parse_chunk_flags(argc, argv);
   std::vector<char*> pos;
   for (int i = 0; i < argc; ++i) {

@@ -54,6 +54,11 @@ extern "C" void ZSTD_invalidateRepCodes(ZSTD_CCtx* cctx);
 namespace xnyz {
 HilbertSM build_hilbert_sm();
 int decompress_impl(const uint8_t* frame, uint64_t len, const xnyz_opts* o, xnyz_buf* out);
+// container.cpp
+std::vector<xnyz_chunk> read_manifest(const char* path);
+void write_container(const char* part_path, const char* manifest_path, const uint8_t* out,
+                     const uint64_t* out_offsets, uint32_t count, xnyz_chunk* chunks);
+std::vector<uint8_t> read_part(const char* part_path, const xnyz_chunk& c);
 }  // namespace xnyz
 
 using namespace xnyz;
@@ -1900,6 +1905,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   ctx->h_dflag.ensure(64);
   ctx->dstream.ensure(blen + 64);
   check_hip(hipMemcpyAsync(ctx->dstream.p, buf, blen, hipMemcpyHostToDevice, st), "stream H2D");
+  ctx->copied();
   launch_zero_u8(ctx->dstream.as<uint8_t>() + blen, 64, st);
   // code tables of the sections, one buffer
   std::vector<uint8_t> tab;
@@ -1981,7 +1987,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     u64* total = ar.take<u64>(1);
     scan_excl_u64(bcnt, bstart, B, total, ar, st);
     check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
-    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    ctx->sync();
     u64 tot;
     std::memcpy(&tot, ws.host_flag, 8);
     if (tot != num) throw Error(XNYZ_ECORRUPT, "block counts do not add up to the points");
@@ -2001,7 +2007,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     u64* total = ar.take<u64>(1);
     scan_excl_u64(c, at2, num, total, ar, st);
     check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
-    check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    ctx->sync();
     std::memcpy(&total_pts, ws.host_flag, 8);
     if (total_pts >= (1ull << 40)) throw Error(XNYZ_EUNSUPPORTED, "RLE counts beyond 2^40 points");
     out_at = at2;
@@ -2050,7 +2056,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     host_out->len = total_pts * 12;
     check_hip(hipMemcpyAsync(p, dst, total_pts * 12, hipMemcpyDeviceToHost, st), "points D2H");
   }
-  check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+  ctx->sync();
   if (ms_device) *ms_device = now_ms() - t0;
   return XNYZ_OK;
 }
@@ -2683,6 +2689,60 @@ int xnyz_decompress_to_device(xnyz_ctx* ctx, const uint8_t* in, uint64_t len,
     ProfGuard pg(ctx);
     return decompress_device_impl(ctx, in, len, opts, d_points, cap_points, n_points, nullptr,
                                   nullptr);
+  });
+}
+
+int xnyz_container_write(const char* part_path, const char* manifest_path, const uint8_t* out,
+                         const uint64_t* out_offsets, uint32_t count, xnyz_chunk* chunks) {
+  return guarded([&] {
+    if (!part_path || !manifest_path || !out_offsets || !chunks || (!out && count))
+      throw Error(XNYZ_EINVAL, "null argument");
+    write_container(part_path, manifest_path, out, out_offsets, count, chunks);
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_container_read(const char* manifest_path, xnyz_buf* chunks) {
+  return guarded([&] {
+    if (!manifest_path || !chunks) throw Error(XNYZ_EINVAL, "null argument");
+    const std::vector<xnyz_chunk> v = read_manifest(manifest_path);
+    const size_t bytes = v.size() * sizeof(xnyz_chunk);
+    chunks->data = xmalloc(bytes);
+    if (bytes) std::memcpy(chunks->data, v.data(), bytes);
+    chunks->len = bytes;
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_container_decompress(xnyz_ctx* ctx, const char* part_path, const char* manifest_path,
+                              xnyz_buf* out_points) {
+  return guarded([&] {
+    if (!part_path || !manifest_path || !out_points) throw Error(XNYZ_EINVAL, "null argument");
+    const std::vector<xnyz_chunk> v = read_manifest(manifest_path);
+    std::vector<uint8_t> all;
+    if (ctx) use_device(ctx);
+    for (const xnyz_chunk& c : v) {
+      const std::vector<uint8_t> b = read_part(part_path, c);
+      xnyz_opts o{};
+      o.quantizer = c.quantizer;
+      o.l2_bound = c.l2_bound;
+      o.rle = c.rle;
+      o.frame_mode = c.frame_mode == XNYZ_FRAME_NONE ? XNYZ_FRAME_NONE : XNYZ_FRAME_ZSTD3;
+      if (!(o.l2_bound > 0.f)) throw Error(XNYZ_EINVAL, "chunk without an L2 bound");
+      xnyz_buf pts{};
+      if (ctx) {
+        ProfGuard pg(ctx);
+        decompress_device_impl(ctx, b.data(), b.size(), &o, nullptr, 0, nullptr, &pts, nullptr);
+      } else {
+        decompress_impl(b.data(), b.size(), &o, &pts);
+      }
+      all.insert(all.end(), pts.data, pts.data + pts.len);
+      xnyz_free(&pts);
+    }
+    out_points->data = xmalloc(all.size());
+    if (!all.empty()) std::memcpy(out_points->data, all.data(), all.size());
+    out_points->len = all.size();
+    return XNYZ_OK;
   });
 }
 
