@@ -217,6 +217,31 @@ int xnyz_compress_batch_device_noalloc(xnyz_ctx* ctx, const float* d_xyz,
                                        const xnyz_opts* opts, uint8_t* out, uint64_t cap,
                                        uint64_t* out_offsets, xnyz_stats* stats);
 
+/* BlockCompressor<float>::fit_profile (source/composer/block_compressor.hpp:
+ * 241-246, compress_internal :278-431) on device input: one cloud coded in
+ * block mode (whatever its range) with repos and quads Huffman codebooks
+ * that cover every symbol 0..255 (HuffmanEncoder::build_complete,
+ * huffman_encoder.hpp:69-84).  `profile` receives the serialized profile
+ * (serialize_profile, block_encoding_profile.hpp:34-44: u32 LE size and
+ * bytes of the repos meta, then of the quads meta); out_frame and
+ * out_prezstd (nullable) the cloud's own output.  RLE is
+ * XNYZ_EUNSUPPORTED, as in the reference's driver. */
+int xnyz_fit_profile_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
+                            xnyz_buf* out_frame, xnyz_buf* out_prezstd, xnyz_buf* profile);
+
+/* BlockCompressor<float>::compress_with_profile (block_compressor.hpp:
+ * 248-256) over a batch (layout as xnyz_compress_batch_device): every cloud
+ * in block mode with the profile's codebooks (their metas embedded as the
+ * reference's load_meta + get_meta writes them), so all chunks share their
+ * codebook bytes -- the MPI driver's --shared-codebook mode
+ * (source/mpi_driver.cpp:722-779) on one GPU.  A cloud with a symbol the
+ * profile lacks is XNYZ_EINVAL. */
+int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
+                                            const uint64_t* offsets, uint32_t count,
+                                            const xnyz_opts* opts, const uint8_t* profile,
+                                            uint64_t profile_len, xnyz_buf* frames,
+                                            xnyz_buf* prezstd, xnyz_stats* stats);
+
 /* Exhaustive per-chunk mode search; no reference counterpart (the
  * reference codes every chunk with the options it is given).  For each cloud
  * of the batch (layout as xnyz_compress_batch_device; opts supply quantizer,

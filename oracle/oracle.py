@@ -65,6 +65,8 @@ def lib() -> C.CDLL:
         L.oracle_compress_mode.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int, C.c_int,
                                            C.c_int, C.c_int, C.c_int, _PP, _U64P,
                                            C.POINTER(OracleInfo)]
+        L.oracle_compress_shared.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int, _P,
+                                             C.c_uint64, _PP, _U64P, _PP, _U64P]
         L.oracle_compress_timed.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int,
                                             C.c_int, C.c_int, C.c_int, _U64P,
                                             C.POINTER(OracleInfo)]
@@ -175,6 +177,25 @@ def compress_mode(points, quantizer: int, l2: float, hilbert: bool, rle: bool,
     if rc != 0:
         raise RuntimeError(f"oracle_compress_mode: {_err()}")
     return _take(pre, pl.value, lib().oracle_free)
+
+
+def compress_shared(points, quantizer: int, l2: float, hilbert: bool, profile: bytes | None = None):
+    """fit_profile (profile None) or compress_with_profile (block_compressor.hpp:241-431):
+    (assembled stream, serialized profile or None)."""
+    a = _f32(points)
+    pre, pl = C.c_void_p(), C.c_uint64()
+    po, pol = C.c_void_p(), C.c_uint64()
+    pin = np.frombuffer(profile, np.uint8) if profile else None
+    rc = lib().oracle_compress_shared(a.ctypes.data, a.shape[0], quantizer, l2, int(hilbert),
+                                      pin.ctypes.data if pin is not None else None,
+                                      len(profile) if profile else 0, C.byref(pre), C.byref(pl),
+                                      None if profile else C.byref(po),
+                                      None if profile else C.byref(pol))
+    if rc != 0:
+        raise RuntimeError(f"oracle_compress_shared: {_err()}")
+    stream = _take(pre, pl.value, lib().oracle_free)
+    prof = None if profile else _take(po, pol.value, lib().oracle_free)
+    return stream, prof
 
 
 def compress_timed(points, quantizer: int, l2: float, hilbert: bool, rle: bool,

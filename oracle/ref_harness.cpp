@@ -38,12 +38,26 @@ void out(const std::vector<uint8_t>& m, const std::vector<uint8_t>& d, uint8_t**
 }
 
 // mode 0: one build; mode 1: the RLE counts double build
-// (block_compressor_rle.hpp:290-303); mode 2: exp-golomb preset.
+// (block_compressor_rle.hpp:290-303); mode 2: exp-golomb preset; mode 3:
+// build_complete over 0..255 (fit_profile, block_compressor.hpp:364-370);
+// mode 4: that meta loaded into a fresh encoder and written again
+// (compress_with_profile, :362-363).
 template <typename T>
 int huff(const T* v, uint64_t n, int mode, uint8_t** meta, uint64_t* ml, uint8_t** data,
          uint64_t* dl) {
   std::vector<T> d(v, v + n);
   XnYZip::HuffmanEncoder<T> enc;
+  if (mode == 3 || mode == 4) {  // build_complete(d, 255); 4: a fresh encoder loads its meta
+    enc.build_complete(d, T(255));
+    if (mode == 4) {
+      XnYZip::HuffmanEncoder<T> e2;
+      e2.load_meta(enc.get_meta());
+      out(e2.get_meta(), e2.encode(d), meta, ml, data, dl);
+      return 0;
+    }
+    out(enc.get_meta(), enc.encode(d), meta, ml, data, dl);
+    return 0;
+  }
   if (mode == 2) enc.set_use_exp_golomb(true);
   enc.build(d);
   auto m = enc.get_meta();

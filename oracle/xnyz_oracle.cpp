@@ -727,6 +727,22 @@ class Huffman {
     build_from_freq(freq);
   }
 
+  // :69-84: the data's frequencies, then every symbol 0..alphabet_max not
+  // seen with frequency 1 (inserted in ascending order after the data's
+  // symbols); never Exp-Golomb.
+  void build_complete(const std::vector<T>& data, T alphabet_max) {
+    code_table.clear();
+    use_eg = false;
+    eg_k = 0;
+    std::unordered_map<T, uint64_t> freq;
+    for (auto v : data) freq[v]++;
+    for (T s = 0; s <= alphabet_max; ++s) {
+      if (freq.find(s) == freq.end()) freq[s] = 1;
+      if (s == std::numeric_limits<T>::max()) break;
+    }
+    build_from_freq(freq);
+  }
+
   // :126-146
   std::vector<uint8_t> meta() const {
     std::vector<uint8_t> out;
@@ -1097,6 +1113,85 @@ LCPMeta encode_block(const std::vector<V3i>& q, int rx, int ry, int rz,
   return M;
 }
 
+// Shared-codebook compression (block_compressor.hpp:241-431,
+// compress_internal): block mode always, the repos and quads Huffman
+// codebooks either built to cover every symbol 0..255 (fit_profile) or
+// loaded from a profile (compress_with_profile); the embedded metas are the
+// encoders' get_meta() -- after load_meta, that is the loaded map's order.
+// Profile blob: serialize_profile (block_encoding_profile.hpp:34-44).
+std::vector<uint8_t> compress_shared(const std::vector<V3f>& points, int qtype, float b,
+                                     bool curve, const std::vector<uint8_t>* profile_in,
+                                     std::vector<uint8_t>* profile_out) {
+  V3f offset{};
+  auto shifted = shift_points(points, offset);
+  auto q = quantize(shifted, qtype, b);
+  Buf buf;
+  buf.put(static_cast<uint64_t>(points.size()));
+  buf.put(static_cast<size_t>(0));
+  int rx = 0, ry = 0, rz = 0;
+  for (const auto& p : q) {
+    rx = std::max(rx, p[0]);
+    ry = std::max(ry, p[1]);
+    rz = std::max(rz, p[2]);
+  }
+  buf.put(rx);
+  buf.put(ry);
+  buf.put(rz);
+  buf.put(curve);
+  buf.put(false);  // is_direct: the shared path always codes blocks
+  LCPMeta M = block_quantize(q, 64, 64, 64, rx, ry, rz, curve);
+  auto blk = M.blkst;
+  auto cnt = M.blkcnt;
+  for (size_t i = cnt.size() - 1; i > 0; i--) cnt[i] -= cnt[i - 1];
+  for (size_t i = blk.size() - 1; i > 0; i--) blk[i] -= blk[i - 1];
+  auto p1 = bitpack(blk);
+  buf.section(p1.meta, p1.data);
+  auto p2 = bitpack(cnt);
+  buf.section(p2.meta, p2.data);
+  auto pr = pfd_encode(M.repos);
+  auto g1 = gr_pack(pr.patches);
+  buf.section(g1.meta, g1.data);
+  std::vector<uint8_t> pm[2];
+  if (profile_in) {  // serialize_profile: u32 LE size + bytes, twice
+    const auto& blob = *profile_in;
+    size_t pos = 0;
+    for (int i = 0; i < 2; ++i) {
+      if (pos + 4 > blob.size()) throw std::runtime_error("short profile");
+      uint32_t n = uint32_t(blob[pos]) | uint32_t(blob[pos + 1]) << 8 |
+                   uint32_t(blob[pos + 2]) << 16 | uint32_t(blob[pos + 3]) << 24;
+      pos += 4;
+      if (pos + n > blob.size()) throw std::runtime_error("short profile");
+      pm[i].assign(blob.begin() + pos, blob.begin() + pos + n);
+      pos += n;
+    }
+  }
+  Huffman<uint64_t> h1;
+  if (profile_in) h1.load_meta(pm[0]);
+  else h1.build_complete(pr.deltas, uint64_t(255));
+  auto m1 = h1.meta();
+  buf.section(m1, h1.encode(pr.deltas));
+  auto pq = pfd_encode(M.quads);
+  auto g2 = gr_pack(pq.patches);
+  buf.section(g2.meta, g2.data);
+  Huffman<uint8_t> h2;
+  if (profile_in) h2.load_meta(pm[1]);
+  else h2.build_complete(pq.deltas, uint8_t(255));
+  auto m2 = h2.meta();
+  buf.section(m2, h2.encode(pq.deltas));
+  buf.put(offset[0]);
+  buf.put(offset[1]);
+  buf.put(offset[2]);
+  if (profile_out) {
+    profile_out->clear();
+    for (const auto* m : {&m1, &m2}) {
+      const uint32_t n = uint32_t(m->size());
+      for (int k = 0; k < 4; ++k) profile_out->push_back(uint8_t(n >> (8 * k)));
+      profile_out->insert(profile_out->end(), m->begin(), m->end());
+    }
+  }
+  return std::move(buf.b);
+}
+
 struct V3iHasher {  // block_compressor_rle.hpp:98-105
   std::size_t operator()(const V3i& v) const {
     uint64_t h = (uint64_t)v[0] * 73856093u ^ (uint64_t)v[1] * 19349663u ^
@@ -1401,6 +1496,27 @@ int oracle_compress_timed(const float* xyz, uint64_t n, int quantizer,
   });
 }
 
+int oracle_compress_shared(const float* xyz, uint64_t n, int quantizer, float l2_bound,
+                           int is_hilbert, const uint8_t* profile_in, uint64_t profile_len,
+                           uint8_t** pre, uint64_t* pre_len, uint8_t** profile_out,
+                           uint64_t* profile_out_len) {
+  return guarded([&] {
+    if (n == 0) return fail("empty input");
+    std::vector<uint8_t> pin;
+    if (profile_in) pin.assign(profile_in, profile_in + profile_len);
+    std::vector<uint8_t> pout;
+    auto b = compress_shared(to_points(xyz, n), quantizer, l2_bound, is_hilbert != 0,
+                             profile_in ? &pin : nullptr, profile_out ? &pout : nullptr);
+    *pre = dup_bytes(b);
+    *pre_len = b.size();
+    if (profile_out) {
+      *profile_out = dup_bytes(pout);
+      *profile_out_len = pout.size();
+    }
+    return 0;
+  });
+}
+
 int oracle_decompress_buffer(const uint8_t* buf, uint64_t len, int quantizer,
                              float l2_bound, int rle, float** xyz, uint64_t* n) {
   return guarded([&] {
@@ -1511,6 +1627,17 @@ static int huff_impl(const T* v, uint64_t n, int mode, uint8_t** meta, uint64_t*
                      uint8_t** data, uint64_t* dl) {
   std::vector<T> d(v, v + n);
   Huffman<T> h;
+  if (mode == 3 || mode == 4) {  // build_complete(d, 255); 4: reloaded by load_meta
+    h.build_complete(d, T(255));
+    if (mode == 4) {
+      Huffman<T> h2;
+      h2.load_meta(h.meta());
+      out_packed(Packed{h2.meta(), h2.encode(d)}, meta, ml, data, dl);
+      return 0;
+    }
+    out_packed(Packed{h.meta(), h.encode(d)}, meta, ml, data, dl);
+    return 0;
+  }
   if (mode == 2) h.use_eg = true;
   h.build(d);
   auto m = h.meta();

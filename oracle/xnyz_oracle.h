@@ -96,6 +96,14 @@ int oracle_gr_pack_u8(const uint8_t* v, uint64_t n, uint8_t** meta,
  * mode 0: one build() (repos / quads streams);
  * mode 1: the RLE counts double build (block_compressor_rle.hpp:290-303);
  * mode 2: build() with use_exp_golomb preset. */
+/* BlockCompressor<float>::fit_profile (profile_in NULL; profile_out gets the
+ * serialize_profile blob) and compress_with_profile (profile_in given)
+ * (block_compressor.hpp:241-431).  Assembled stream only. */
+int oracle_compress_shared(const float* xyz, uint64_t n, int quantizer, float l2_bound,
+                           int is_hilbert, const uint8_t* profile_in, uint64_t profile_len,
+                           uint8_t** pre, uint64_t* pre_len, uint8_t** profile_out,
+                           uint64_t* profile_out_len);
+
 int oracle_huffman(const void* v, uint64_t n, int sym_type, int mode,
                    uint8_t** meta, uint64_t* meta_len, uint8_t** data,
                    uint64_t* data_len);

@@ -56,6 +56,13 @@ def huffman_cases(rng):
     out.append((O.SYM_INT, 1, (rng.geometric(0.2, 4000) * 3).tolist()))
     out.append((O.SYM_INT, 2, rng.geometric(0.3, 700).tolist()))
     out.append((O.SYM_INT, 0, [-5, 3, 3, -5, 0, 2**31 - 1, -2**31]))
+    # shared codebooks (own generator, so the cases above keep their values):
+    # build_complete over 0..255 (mode 3) and its meta reloaded (mode 4)
+    r2 = np.random.default_rng(7)
+    for mode in (3, 4):
+        out.append((O.SYM_U64, mode, np.minimum(r2.geometric(0.05, 3000), 255).tolist()))
+        out.append((O.SYM_U8, mode, (1 + r2.binomial(7, 0.3, 2000)).tolist()))
+    out.append((O.SYM_U64, 3, [2, 2, 2]))
     return out
 
 

@@ -43,3 +43,17 @@ def test_hilbert_matches_reference(oracle):
             assert oracle.h2m(m, it) == O.ref_h2m(m, it)
     for m in range(0, 1 << 18, 97):
         assert oracle.m2h(m, 20) == O.ref_m2h(m, 20)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_shared_codebooks_match_reference(oracle, seed):
+    """build_complete over 0..255 (fit_profile) and its meta reloaded by a
+    fresh encoder (compress_with_profile): meta bytes and data bits."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(1, 5000))
+    sym = np.where(rng.random(n) < 0.2, 0, 1 + rng.geometric(0.1, n).clip(0, 254))
+    q = (1 + rng.binomial(7, rng.random(), n)).astype(np.uint8)
+    for mode in (3, 4):
+        assert oracle.huffman(sym.astype(np.uint64), O.SYM_U64, mode) == \
+            oracle.huffman(sym.astype(np.uint64), O.SYM_U64, mode, impl="ref")
+        assert oracle.huffman(q, O.SYM_U8, mode) == oracle.huffman(q, O.SYM_U8, mode, impl="ref")

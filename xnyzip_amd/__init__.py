@@ -27,7 +27,8 @@ __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDeco
            "compress", "compress_batch", "compress_batch_into", "compress_batch_noalloc", "host_buffer", "search_batch", "features_batch", "mlp_train",
            "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune",
            "decompress_device", "decompress_to_device", "hilbert_to_morton",
-           "container_write", "container_read", "container_decompress"]
+           "container_write", "container_read", "container_decompress", "fit_profile",
+           "compress_with_profile_batch"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -480,6 +481,43 @@ def hilbert_to_morton(codes, iteration: int, ctx: Context | None = None) -> np.n
     _lib.check(_lib.lib().xnyz_hilbert_to_morton(ctx.handle, C.c_void_p(a.ctypes.data), a.size,
                                                   int(iteration), C.c_void_p(out.ctypes.data)))
     return out
+
+
+def fit_profile(points_device, quantizer, l2_bound, is_hilbert=False,
+                frame_mode=_lib.FRAME_ZSTD3, want_prezstd=False, ctx: Context | None = None):
+    """BlockCompressor::fit_profile on a GPU tensor: (frame, prezstd or None,
+    serialized profile bytes)."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    ctx = ctx or default_context(t.device.index or 0)
+    o = _opts(quantizer, l2_bound, is_hilbert, False, 1024, frame_mode, 0)
+    fr, pre, prof = _lib.Buf(), _lib.Buf(), _lib.Buf()
+    torch.cuda.synchronize(t.device)
+    _lib.check(_lib.lib().xnyz_fit_profile_device(ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0],
+                                                   C.byref(o), C.byref(fr),
+                                                   C.byref(pre) if want_prezstd else None,
+                                                   C.byref(prof)))
+    return _lib.take(fr), (_lib.take(pre) if want_prezstd else None), _lib.take(prof)
+
+
+def compress_with_profile_batch(points_device, offsets, opts, profile: bytes,
+                                want_prezstd=False, ctx: Context | None = None):
+    """BlockCompressor::compress_with_profile over a batch: (frames, prezstd, stats)."""
+    t, count, off, arr = _batch_args(points_device, offsets, opts, _lib.FRAME_ZSTD3)
+    ctx = ctx or default_context(t.device.index or 0)
+    prof = np.frombuffer(profile, np.uint8)
+    frames = (_lib.Buf * count)()
+    pres = (_lib.Buf * count)() if want_prezstd else None
+    stats = (_lib.Stats * count)()
+    _lib.check(_lib.lib().xnyz_compress_with_profile_batch_device(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(prof.ctypes.data), len(profile),
+        C.cast(frames, C.c_void_p), C.cast(pres, C.c_void_p) if pres is not None else None,
+        C.cast(stats, C.c_void_p)))
+    return ([_lib.take(f) for f in frames],
+            [_lib.take(p) for p in pres] if pres is not None else None, list(stats))
 
 
 def container_write(part_path: str, manifest_path: str, out: np.ndarray, offsets, chunks):

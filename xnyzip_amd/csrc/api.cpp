@@ -824,6 +824,14 @@ struct CallOut {
   const char* refine = nullptr;
   xnyz_mode* modes_out = nullptr;
   char* corrected = nullptr;
+  // Shared codebooks (block_compressor.hpp:241-431, compress_internal): 1 =
+  // fit_profile (the repos and quads codebooks cover every symbol 0..255),
+  // 2 = compress_with_profile (both loaded from profile_metas[0], [1]).
+  // Every segment is coded in block mode; profile_out (fit) gets segment 0's
+  // serialized profile (block_encoding_profile.hpp:34-44).
+  int shared = 0;
+  const std::vector<uint8_t>* profile_metas = nullptr;
+  std::vector<uint8_t>* profile_out = nullptr;
   // No-allocation call: every buffer comes from the fixed regions (g_bump),
   // stage 2 runs without a graph, and each segment's stream is copied from
   // device memory straight into dst (XNYZ_FRAME_NONE only).
@@ -997,7 +1005,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       h.mse = c.sum_err2 / h.n;
     }
     const int max_range = std::max({h.rx, h.ry, h.rz});
-    h.direct = max_range < opts[s].direct_threshold;
+    h.direct = !co.shared && max_range < opts[s].direct_threshold;
     KeyParams& kp = sp[s].kp;
     kp = KeyParams{};
     kp.direct = h.direct;
@@ -1145,7 +1153,15 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     h.B = h.direct ? 0 : c.nb;
     h.P = c.n_patches;
     if (!h.direct) {
-      h.cb_quad.build(sym_counts(hh.quad, hh.first_quad, kSymBins), h.ncoded);
+      std::vector<SymCount> qs = sym_counts(hh.quad, hh.first_quad, kSymBins);
+      if (co.shared == 1) {
+        h.cb_quad.build_complete(qs, h.ncoded, 255);
+      } else if (co.shared == 2) {
+        if (!h.cb_quad.load_meta(co.profile_metas[1], qs) || !h.cb_quad.covers_data())
+          throw Error(XNYZ_EINVAL, "profile's quads codebook does not cover the cloud");
+      } else {
+        h.cb_quad.build(qs, h.ncoded);
+      }
       fill_lut(h.cb_quad, lut[s].quad, kSymBins);
       h.w1 = c.max_blkd ? bitlen(c.max_blkd) : 1;
       h.w2 = c.max_cntd ? bitlen(c.max_cntd) : 1;
@@ -1220,7 +1236,14 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       used[s].gr_k = k;
       used[s].pfd_width = w;
     }
-    h.cb_sym.build(syms, h.ncoded);
+    if (co.shared == 1) {
+      h.cb_sym.build_complete(syms, h.ncoded, 255);
+    } else if (co.shared == 2) {
+      if (!h.cb_sym.load_meta(co.profile_metas[0], syms) || !h.cb_sym.covers_data())
+        throw Error(XNYZ_EINVAL, "profile's repos codebook does not cover the cloud");
+    } else {
+      h.cb_sym.build(syms, h.ncoded);
+    }
     fill_lut(h.cb_sym, lut[s].sym, kSymBins);
     if (h.cb_sym.max_length() > 56 || h.cb_quad.max_length() > 56 ||
         h.cb_cnt.max_length() > 56)
@@ -1228,6 +1251,14 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   });
   if (co.modes_out)
     for (u32 s = 0; s < S; ++s) co.modes_out[s] = used[s];
+  if (co.profile_out) {  // serialize_profile: u32 LE size + repos meta, u32 LE size + quads meta
+    co.profile_out->clear();
+    for (const auto& m : {g[0].cb_sym.meta(), g[0].cb_quad.meta()}) {
+      const uint32_t n = (uint32_t)m.size();
+      for (int k = 0; k < 4; ++k) co.profile_out->push_back(uint8_t(n >> (8 * k)));
+      co.profile_out->insert(co.profile_out->end(), m.begin(), m.end());
+    }
+  }
   if (co.search) {  // size model per segment (modes.hpp)
     co.search->resize(S);
     for (u32 s = 0; s < S; ++s) {
@@ -2317,6 +2348,59 @@ int xnyz_compress_batch_device_noalloc(xnyz_ctx* ctx, const float* d_xyz,
     co.cap = cap;
     co.dst_off = out_offsets;
     co.noalloc = true;
+    return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+  });
+}
+
+int xnyz_fit_profile_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
+                            xnyz_buf* out_frame, xnyz_buf* out_prezstd, xnyz_buf* profile) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !profile) throw Error(XNYZ_EINVAL, "null argument");
+    validate(opts, n);
+    if (opts->rle) throw Error(XNYZ_EUNSUPPORTED, "shared codebooks are not defined for RLE");
+    use_device(ctx);
+    xnyz_buf scratch{};
+    CallOut co{out_frame ? out_frame : &scratch, out_prezstd, nullptr, nullptr};
+    std::vector<uint8_t> prof;
+    co.shared = 1;
+    co.profile_out = &prof;
+    compress_impl(ctx, d_xyz, {0, n}, opts, co, now_ms());
+    if (scratch.data) xnyz_free(&scratch);
+    profile->data = xmalloc(prof.size());
+    std::memcpy(profile->data, prof.data(), prof.size());
+    profile->len = prof.size();
+    return XNYZ_OK;
+  });
+}
+
+int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
+                                            const uint64_t* offsets, uint32_t count,
+                                            const xnyz_opts* opts, const uint8_t* profile,
+                                            uint64_t profile_len, xnyz_buf* frames,
+                                            xnyz_buf* prezstd, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !frames || !profile)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    for (u32 s = 0; s < count; ++s)
+      if (opts[s].rle) throw Error(XNYZ_EUNSUPPORTED, "shared codebooks are not defined for RLE");
+    // deserialize_profile (block_encoding_profile.hpp:48-70)
+    std::vector<uint8_t> metas[2];
+    u64 pos = 0;
+    for (auto& m : metas) {
+      if (pos + 4 > profile_len) throw Error(XNYZ_EINVAL, "truncated profile");
+      const uint32_t n = uint32_t(profile[pos]) | uint32_t(profile[pos + 1]) << 8 |
+                         uint32_t(profile[pos + 2]) << 16 | uint32_t(profile[pos + 3]) << 24;
+      pos += 4;
+      if (pos + n > profile_len) throw Error(XNYZ_EINVAL, "truncated profile");
+      m.assign(profile + pos, profile + pos + n);
+      pos += n;
+    }
+    if (metas[0].empty() || metas[1].empty()) throw Error(XNYZ_EINVAL, "empty profile");
+    use_device(ctx);
+    CallOut co{frames, prezstd, stats, nullptr};
+    co.shared = 2;
+    co.profile_metas = metas;
     return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
   });
 }

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <limits>
 #include <memory>
 #include <queue>
@@ -66,6 +67,64 @@ class Codebook {
     }
     Node* root = build_tree(freq);
     assign(root, 0u, 0u);
+  }
+
+  // huffman_encoder.hpp:69-84 (build_complete): the data's symbols, then
+  // every symbol 0..alphabet_max it lacks with frequency 1, inserted in
+  // ascending order after them; never Exp-Golomb.  The data bits still
+  // count the data's symbols only.
+  void build_complete(std::vector<SymCount> syms, uint64_t total, uint64_t alphabet_max) {
+    code_table.clear();
+    use_eg = false;
+    eg_k = 0;
+    counts_ = syms;
+    std::sort(syms.begin(), syms.end(),
+              [](const SymCount& a, const SymCount& b) { return a.first < b.first; });
+    std::unordered_map<T, uint64_t> freq;
+    for (const auto& s : syms) freq[static_cast<T>(s.sym)] += s.freq;
+    for (uint64_t s = 0; s <= alphabet_max; ++s)
+      if (freq.find(static_cast<T>(s)) == freq.end()) freq[static_cast<T>(s)] = 1;
+    (void)total;
+    Node* root = build_tree(freq);
+    assign(root, 0u, 0u);
+  }
+
+  // huffman_encoder.hpp:172-203 (load_meta) on a fresh encoder: the entries
+  // inserted into code_table in meta order, so meta() writes them in that
+  // map's iteration order, as the reference's get_meta() after load_meta().
+  // Returns false for an Exp-Golomb meta or a malformed one.
+  bool load_meta(const std::vector<uint8_t>& m, std::vector<SymCount> syms) {
+    code_table.clear();
+    use_eg = false;
+    eg_k = 0;
+    counts_ = std::move(syms);
+    if (m.size() >= 4 && m[0] == 0xE1 && m[1] == 0xC1 && m[2] == 0x01) return false;
+    if (m.size() < 4) return false;
+    const uint32_t n = uint32_t(m[0]) << 24 | uint32_t(m[1]) << 16 | uint32_t(m[2]) << 8 | m[3];
+    const size_t esz = (sizeof(T) == 8 ? 8 : sizeof(T)) + 5;
+    if (4 + (size_t)n * esz > m.size()) return false;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint8_t* e = m.data() + 4 + i * esz;
+      T sym;
+      if constexpr (sizeof(T) == 8) {
+        uint64_t v = 0;
+        for (int k = 0; k < 8; ++k) v = (v << 8) | e[k];
+        sym = static_cast<T>(v);
+      } else {
+        std::memcpy(&sym, e, sizeof(T));
+      }
+      const uint8_t* t = e + (sizeof(T) == 8 ? 8 : sizeof(T));
+      const uint32_t bits = uint32_t(t[1]) << 24 | uint32_t(t[2]) << 16 | uint32_t(t[3]) << 8 | t[4];
+      code_table[sym] = Code{bits, t[0]};
+    }
+    return true;
+  }
+
+  // Whether every symbol the data holds has a code.
+  bool covers_data() const {
+    for (const auto& s : counts_)
+      if (!code_table.count(static_cast<T>(s.sym))) return false;
+    return true;
   }
 
   // huffman_encoder.hpp:126-146
