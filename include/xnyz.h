@@ -242,6 +242,21 @@ int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
                                             uint64_t profile_len, xnyz_buf* frames,
                                             xnyz_buf* prezstd, xnyz_stats* stats);
 
+/* The MPI driver's mapreduce mode (source/mpi_driver.cpp:365-648: global
+ * bounding box, quantize every chunk against it, shuffle and reduce the cell
+ * counts across ranks, then BlockCompressorRLE::encode_cells_with_counts,
+ * block_compressor_rle.hpp:341-470) on one GPU: the cross-chunk dedup is
+ * the device's one sort of the whole input, so the shuffle and reduce
+ * vanish.  The stream is the one a single rank's driver writes: RLE cells
+ * and counts, block mode always, the largest quantized coordinate as the
+ * range of all three axes, the global minimum as the trailer.  opts supply
+ * quantizer, l2_bound, curve and frame_mode (rle is implied).  The minimum
+ * follows shift_points (shifting.hpp:11-36): it differs from the driver's
+ * FLT_MAX-seeded one only when the first point holds a NaN. */
+int xnyz_compress_global_rle_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
+                                    const xnyz_opts* opts, xnyz_buf* out_frame,
+                                    xnyz_buf* out_prezstd, xnyz_stats* stats);
+
 /* Exhaustive per-chunk mode search; no reference counterpart (the
  * reference codes every chunk with the options it is given).  For each cloud
  * of the batch (layout as xnyz_compress_batch_device; opts supply quantizer,

@@ -67,6 +67,8 @@ def lib() -> C.CDLL:
                                            C.POINTER(OracleInfo)]
         L.oracle_compress_shared.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int, _P,
                                              C.c_uint64, _PP, _U64P, _PP, _U64P]
+        L.oracle_compress_global_rle.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int,
+                                                 _PP, _U64P]
         L.oracle_compress_timed.argtypes = [_P, C.c_uint64, C.c_int, C.c_float, C.c_int,
                                             C.c_int, C.c_int, C.c_int, _U64P,
                                             C.POINTER(OracleInfo)]
@@ -196,6 +198,18 @@ def compress_shared(points, quantizer: int, l2: float, hilbert: bool, profile: b
     stream = _take(pre, pl.value, lib().oracle_free)
     prof = None if profile else _take(po, pol.value, lib().oracle_free)
     return stream, prof
+
+
+def compress_global_rle(points, quantizer: int, l2: float, hilbert: bool) -> bytes:
+    """The mapreduce mode's stream on one rank (mpi_driver.cpp:365-648,
+    block_compressor_rle.hpp:341-470)."""
+    a = _f32(points)
+    pre, pl = C.c_void_p(), C.c_uint64()
+    rc = lib().oracle_compress_global_rle(a.ctypes.data, a.shape[0], quantizer, l2, int(hilbert),
+                                          C.byref(pre), C.byref(pl))
+    if rc != 0:
+        raise RuntimeError(f"oracle_compress_global_rle: {_err()}")
+    return _take(pre, pl.value, lib().oracle_free)
 
 
 def compress_timed(points, quantizer: int, l2: float, hilbert: bool, rle: bool,

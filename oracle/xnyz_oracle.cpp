@@ -1297,6 +1297,60 @@ Result compress(const std::vector<V3f>& points, int qtype, float b, bool curve,
   return R;
 }
 
+// The MPI driver's mapreduce mode on one rank (source/mpi_driver.cpp:365-648)
+// followed by BlockCompressorRLE::encode_cells_with_counts
+// (block_compressor_rle.hpp:341-470): the global per-axis minimum (strict
+// '<' from FLT_MAX, :377-401), every point quantized against it and counted
+// per cell (:346-360, an int64 sum clamped to INT32_MAX, :599-603), the
+// largest quantized coordinate as the range of all three axes (:469-480,
+// :608-611), block mode always, counts in the sorted cells' order, the RLE
+// counts stream built twice as in compress(), the minimum as the trailer.
+std::vector<uint8_t> compress_global_rle(const std::vector<V3f>& points, int qtype, float b,
+                                         bool curve) {
+  V3f mn{{std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
+          std::numeric_limits<float>::max()}};
+  for (const auto& p : points)
+    for (int a = 0; a < 3; ++a)
+      if (p[a] < mn[a]) mn[a] = p[a];
+  std::vector<V3f> shifted(points.size());
+  for (size_t i = 0; i < points.size(); ++i)
+    for (int a = 0; a < 3; ++a) shifted[i][a] = points[i][a] - mn[a];
+  auto q = quantize(shifted, qtype, b);
+  std::unordered_map<V3i, int64_t, V3iHasher> table;
+  for (const auto& c : q) table[c] += 1;
+  int gmax = 0;
+  std::vector<V3i> cells;
+  std::vector<int> counts;
+  for (const auto& [c, n] : table) {
+    gmax = std::max({gmax, c[0], c[1], c[2]});
+    cells.push_back(c);
+    counts.push_back(static_cast<int>(std::min<int64_t>(n, INT32_MAX)));
+  }
+  Buf buf;
+  buf.put(static_cast<uint64_t>(cells.size()));
+  buf.put(static_cast<uint64_t>(0));
+  buf.put(gmax);
+  buf.put(gmax);
+  buf.put(gmax);
+  buf.put(curve);
+  buf.put(false);
+  oracle_info info{};
+  LCPMeta M = encode_block(cells, gmax, gmax, gmax, curve, buf, info);
+  std::vector<int> c2(counts.size());
+  for (size_t i = 0; i < counts.size(); i++) c2[i] = counts[M.ords[i]];
+  Huffman<int> h3;
+  h3.build(c2);
+  auto m5 = h3.meta();
+  auto d5 = h3.encode(c2);
+  if (static_cast<double>(m5.size()) > static_cast<double>(d5.size()) * 0.25) h3.use_eg = true;
+  h3.build(c2);
+  buf.section(h3.meta(), h3.encode(c2));
+  buf.put(mn[0]);
+  buf.put(mn[1]);
+  buf.put(mn[2]);
+  return std::move(buf.b);
+}
+
 // ---------------------------------------------------------------------------
 // Decompressors: block_decompressor.hpp:31-192, block_decompressor_rle.hpp:32-207
 std::vector<V3f> decompress_buffer(const std::vector<uint8_t>& buffer, int qtype,
@@ -1513,6 +1567,17 @@ int oracle_compress_shared(const float* xyz, uint64_t n, int quantizer, float l2
       *profile_out = dup_bytes(pout);
       *profile_out_len = pout.size();
     }
+    return 0;
+  });
+}
+
+int oracle_compress_global_rle(const float* xyz, uint64_t n, int quantizer, float l2_bound,
+                               int is_hilbert, uint8_t** pre, uint64_t* pre_len) {
+  return guarded([&] {
+    if (n == 0) return fail("empty input");
+    auto b = compress_global_rle(to_points(xyz, n), quantizer, l2_bound, is_hilbert != 0);
+    *pre = dup_bytes(b);
+    *pre_len = b.size();
     return 0;
   });
 }

@@ -104,6 +104,12 @@ int oracle_compress_shared(const float* xyz, uint64_t n, int quantizer, float l2
                            uint8_t** pre, uint64_t* pre_len, uint8_t** profile_out,
                            uint64_t* profile_out_len);
 
+/* The MPI driver's mapreduce mode on one rank (mpi_driver.cpp:365-648) and
+ * BlockCompressorRLE::encode_cells_with_counts (block_compressor_rle.hpp:
+ * 341-470): the assembled stream of the whole input's global RLE. */
+int oracle_compress_global_rle(const float* xyz, uint64_t n, int quantizer, float l2_bound,
+                               int is_hilbert, uint8_t** pre, uint64_t* pre_len);
+
 int oracle_huffman(const void* v, uint64_t n, int sym_type, int mode,
                    uint8_t** meta, uint64_t* meta_len, uint8_t** data,
                    uint64_t* data_len);

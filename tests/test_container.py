@@ -64,3 +64,15 @@ def test_reference_manifest_line(tmp_path):
     assert (c["input_offset"], c["input_points"], c["part_offset"], c["compressed_size"]) == \
         (1200, 50, 77, 55)
     assert c["quantizer"] == O.TO and c["frame_mode"] == 0 and c["rle"] == 0
+
+
+def test_mapreduce_manifest_line(tmp_path):
+    """The MPI driver's mapreduce line (mpi_driver.cpp:626-635): spaces after
+    the colons, no input_offset, "mode": "mapreduce" -> RLE."""
+    import xnyzip_amd as X
+    man = tmp_path / "m.jsonl"
+    man.write_text('{"rank": 0, "mode": "mapreduce", "input_bytes": 1200, '
+                   '"input_blocks_owned": 2, "owned_cells": 40, "part_offset": 0, '
+                   '"compressed_size": 99}\n')
+    (c,) = X.container_read(str(man))
+    assert (c["input_offset"], c["input_points"], c["compressed_size"], c["rle"]) == (0, 100, 99, 1)

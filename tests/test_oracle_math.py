@@ -338,3 +338,32 @@ def test_frame_decodes_to_buffer(oracle):
     out = ctypes.create_string_buffer(len(c.pre))
     r = lib.ZSTD_decompress(out, ctypes.c_size_t(len(c.pre)), c.frame, ctypes.c_size_t(len(c.frame)))
     assert r == len(c.pre) and out.raw == c.pre
+
+
+def test_global_rle_matches_rle_compress_on_a_cube(oracle):
+    """The mapreduce stream (mpi_driver.cpp:365-648 on one rank) is the RLE
+    stream in block mode with every axis given the largest range: on a cloud
+    whose three ranges agree, it is the plain RLE compress() stream."""
+    rng = np.random.default_rng(12)
+    pts = np.round(rng.random((20000, 3)) * 40, 1).astype(np.float32)
+    pts[0] = [0, 0, 0]
+    pts[1] = [40, 40, 40]
+    nodes, _ = oracle.quantize(pts, O.CUBE, 0.05)
+    assert len(set(nodes.max(axis=0).tolist())) == 1
+    assert oracle.compress_global_rle(pts, O.CUBE, 0.05, True) == \
+        oracle.compress(pts, O.CUBE, 0.05, True, True, 0).pre
+
+
+def test_global_rle_counts_every_point(oracle):
+    rng = np.random.default_rng(13)
+    pts = np.round(rng.random((30000, 3)) * [50, 20, 5], 0).astype(np.float32)
+    pre = oracle.compress_global_rle(pts, O.TO, 0.3, False)
+    import struct
+    n = struct.unpack_from("<Q", pre)[0]
+    rx, ry, rz = struct.unpack_from("<iii", pre, 16)
+    assert rx == ry == rz and pre[29] == 0  # one range, block mode
+    nodes, _ = oracle.quantize(pts, O.TO, 0.3)
+    assert n == len({tuple(x) for x in nodes.tolist()})
+    assert rx == max(0, int(nodes.max()))
+    back = oracle.decompress_buffer(pre, O.TO, 0.3, True)
+    assert len(back) == len(pts)

@@ -28,7 +28,7 @@ __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDeco
            "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune",
            "decompress_device", "decompress_to_device", "hilbert_to_morton",
            "container_write", "container_read", "container_decompress", "fit_profile",
-           "compress_with_profile_batch"]
+           "compress_with_profile_batch", "compress_global_rle"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -518,6 +518,24 @@ def compress_with_profile_batch(points_device, offsets, opts, profile: bytes,
         C.cast(stats, C.c_void_p)))
     return ([_lib.take(f) for f in frames],
             [_lib.take(p) for p in pres] if pres is not None else None, list(stats))
+
+
+def compress_global_rle(points_device, quantizer, l2_bound, is_hilbert=False,
+                        frame_mode=_lib.FRAME_ZSTD3, want_prezstd=False,
+                        ctx: Context | None = None):
+    """The MPI driver's mapreduce mode on one GPU: (frame, prezstd or None, stats)."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    ctx = ctx or default_context(t.device.index or 0)
+    o = _opts(quantizer, l2_bound, is_hilbert, True, 1024, frame_mode, 0)
+    fr, pre, st = _lib.Buf(), _lib.Buf(), _lib.Stats()
+    torch.cuda.synchronize(t.device)
+    _lib.check(_lib.lib().xnyz_compress_global_rle_device(
+        ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0], C.byref(o), C.byref(fr),
+        C.byref(pre) if want_prezstd else None, C.byref(st)))
+    return _lib.take(fr), (_lib.take(pre) if want_prezstd else None), st
 
 
 def container_write(part_path: str, manifest_path: str, out: np.ndarray, offsets, chunks):

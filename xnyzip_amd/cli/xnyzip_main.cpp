@@ -10,6 +10,11 @@
 //   XnYZip --batch <input_file> <cube|to> <L2 bound> <-z|-h|-auto> <-rle|-normal|-auto>
 //          <output_dir> [--chunk-points=N] [--stream]
 //   XnYZip --unbatch <output_dir> <decompression_file_name> [--cpu]
+//   XnYZip --mapreduce <input_file> <cube|to> <L2 bound> <-z|-h> <output_dir>
+//
+// --mapreduce is the MPI driver's global RLE (--mode mapreduce,
+// source/mpi_driver.cpp:365-648) on one GPU: one stream of the whole file's
+// cells and counts, its part file and one manifest line; --unbatch reads it.
 //
 // -auto lets the learned selector (falling back to the exhaustive search)
 // pick each chunk's curve and RLE flag; the manifest records the choice.
@@ -235,6 +240,65 @@ int run_batch(int argc, char** argv, const std::vector<char*>& pos) {
   return 0;
 }
 
+int run_mapreduce(int argc, char** argv, const std::vector<char*>& pos) {
+  (void)argc;
+  if (pos.size() != 7) {
+    std::printf("Usage: %s --mapreduce <input_file> <cube|to> <L2 bound> <-z|-h> <output_dir>\n",
+                argv[0]);
+    return 1;
+  }
+  const std::vector<float> pts = read_points(pos[2], ChunkSpec{});
+  const uint64_t n = pts.size() / 3;
+  if (!n) {
+    std::fprintf(stderr, "empty input\n");
+    return 1;
+  }
+  xnyz_opts o{};
+  o.quantizer = std::string(pos[3]) == "to" ? XNYZ_TRUNCATED_OCTAHEDRON : XNYZ_CUBE;
+  o.l2_bound = std::stof(pos[4]);
+  o.is_hilbert = std::string(pos[5]) == "-h";
+  o.rle = 1;
+  o.direct_threshold = 1024;
+  o.frame_mode = XNYZ_FRAME_ZSTD3;
+  auto t0 = std::chrono::steady_clock::now();
+  xnyz_ctx* ctx = nullptr;
+  if (xnyz_create(0, &ctx) != XNYZ_OK) return die("xnyz_create");
+  float* d = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d), n * 12) != hipSuccess ||
+      hipMemcpy(d, pts.data(), n * 12, hipMemcpyHostToDevice) != hipSuccess) {
+    std::fprintf(stderr, "XnYZip: device upload failed\n");
+    return 1;
+  }
+  xnyz_buf frame{};
+  xnyz_stats st{};
+  if (xnyz_compress_global_rle_device(ctx, d, n, &o, &frame, nullptr, &st) != XNYZ_OK)
+    return die("xnyz_compress_global_rle_device");
+  const double tc = seconds_since(t0);
+  const std::string dir = pos[6];
+  std::ofstream part(dir + "/part_rank0000.bin", std::ios::binary);
+  std::ofstream meta(dir + "/manifest_rank0000.jsonl");
+  if (!part.is_open() || !meta.is_open()) {
+    std::fprintf(stderr, "cannot open output files in %s\n", dir.c_str());
+    return 1;
+  }
+  part.write(reinterpret_cast<const char*>(frame.data), std::streamsize(frame.len));
+  // mpi_driver.cpp:626-635, with the mode fields the decoder needs
+  meta << "{\"rank\": 0, \"mode\": \"mapreduce\", \"input_bytes\": " << n * 12
+       << ", \"input_blocks_owned\": 1, \"owned_cells\": " << st.num_points
+       << ", \"part_offset\": 0, \"compressed_size\": " << frame.len
+       << ", \"quantizer\": \"" << pos[3] << "\", \"l2_bound\": " << pos[4]
+       << ", \"curve\": \"" << (o.is_hilbert ? "hilbert" : "z") << "\", \"rle\": 1}\n";
+  std::printf("[mapreduce] rank 0: input=%llu B, encoded %llu cells -> %llu B\n",
+              (unsigned long long)(n * 12), (unsigned long long)st.num_points,
+              (unsigned long long)frame.len);
+  std::printf("compression ratio: %f\n", (double)n * 12 / (double)frame.len);
+  std::cout << "XnYZip compression time = " << tc << "s" << std::endl;
+  xnyz_free(&frame);
+  (void)hipFree(d);
+  xnyz_destroy(ctx);
+  return 0;
+}
+
 int run_unbatch(int argc, char** argv, const std::vector<char*>& pos) {
   if (pos.size() != 4) {
     std::printf("Usage: %s --unbatch <output_dir> <decompression_file_name> [--cpu]\n", argv[0]);
@@ -266,6 +330,7 @@ int main(int argc, char** argv) {
     try {
       if (argc > 1 && std::string(argv[1]) == "--batch") return run_batch(argc, argv, p);
       if (argc > 1 && std::string(argv[1]) == "--unbatch") return run_unbatch(argc, argv, p);
+      if (argc > 1 && std::string(argv[1]) == "--mapreduce") return run_mapreduce(argc, argv, p);
     } catch (const std::exception& e) {
       std::fprintf(stderr, "%s\n", e.what());
       return 1;

@@ -830,6 +830,11 @@ struct CallOut {
   // Every segment is coded in block mode; profile_out (fit) gets segment 0's
   // serialized profile (block_encoding_profile.hpp:34-44).
   int shared = 0;
+  // The MPI driver's mapreduce mode on one GPU (mpi_driver.cpp:365-648 with
+  // encode_cells_with_counts, block_compressor_rle.hpp:341-470): RLE, block
+  // mode always, the largest quantized coordinate as the range of all three
+  // axes.
+  bool global_rle = false;
   const std::vector<uint8_t>* profile_metas = nullptr;
   std::vector<uint8_t>* profile_out = nullptr;
   // No-allocation call: every buffer comes from the fixed regions (g_bump),
@@ -1005,7 +1010,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       h.mse = c.sum_err2 / h.n;
     }
     const int max_range = std::max({h.rx, h.ry, h.rz});
-    h.direct = !co.shared && max_range < opts[s].direct_threshold;
+    h.direct = !co.shared && !co.global_rle && max_range < opts[s].direct_threshold;
+    if (co.global_rle) h.rx = h.ry = h.rz = max_range;
     KeyParams& kp = sp[s].kp;
     kp = KeyParams{};
     kp.direct = h.direct;
@@ -2402,6 +2408,22 @@ int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
     co.shared = 2;
     co.profile_metas = metas;
     return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+  });
+}
+
+int xnyz_compress_global_rle_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
+                                    const xnyz_opts* opts, xnyz_buf* out_frame,
+                                    xnyz_buf* out_prezstd, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !out_frame) throw Error(XNYZ_EINVAL, "null argument");
+    validate(opts, n);
+    xnyz_opts o = *opts;
+    o.rle = 1;
+    o.mode_flags = 0;
+    use_device(ctx);
+    CallOut co{out_frame, out_prezstd, stats, nullptr};
+    co.global_rle = true;
+    return compress_impl(ctx, d_xyz, {0, n}, &o, co, now_ms());
   });
 }
 

@@ -49,6 +49,7 @@ bool field(const std::string& l, const char* key, std::string& out) {
   const size_t p = l.find(k);
   if (p == std::string::npos) return false;
   size_t a = p + k.size();
+  while (a < l.size() && (l[a] == ' ' || l[a] == '\t')) ++a;
   if (a < l.size() && l[a] == '"') {
     const size_t e = l.find('"', a + 1);
     if (e == std::string::npos) return false;
@@ -56,7 +57,7 @@ bool field(const std::string& l, const char* key, std::string& out) {
     return true;
   }
   size_t e = a;
-  while (e < l.size() && l[e] != ',' && l[e] != '}') ++e;
+  while (e < l.size() && l[e] != ',' && l[e] != '}' && l[e] != ' ') ++e;
   out = l.substr(a, e - a);
   return !out.empty();
 }
@@ -69,11 +70,15 @@ xnyz_chunk parse_line(const std::string& l, size_t lineno) {
     return v;
   };
   xnyz_chunk c{};
-  c.input_offset = std::stoull(need("input_offset"));
+  std::string v;
+  // a mapreduce line (mpi_driver.cpp:626-635) has no input_offset: one part
+  // per rank holding the rank's owned cells
+  c.input_offset = field(l, "input_offset", v) ? std::stoull(v) : 0;
   c.part_offset = std::stoull(need("part_offset"));
   c.compressed_size = std::stoull(need("compressed_size"));
-  std::string v;
-  c.input_points = field(l, "points", v) ? std::stoull(v) : std::stoull(need("input_size")) / 12;
+  if (field(l, "points", v)) c.input_points = std::stoull(v);
+  else if (field(l, "input_size", v)) c.input_points = std::stoull(v) / 12;
+  else if (field(l, "input_bytes", v)) c.input_points = std::stoull(v) / 12;
   // The reference's own manifests carry no mode: the defaults are the MPI
   // driver's (TO quantizer, zstd frames); the bound, curve and RLE flag must
   // then be given by the caller.
@@ -82,6 +87,7 @@ xnyz_chunk parse_line(const std::string& l, size_t lineno) {
   c.l2_bound = field(l, "l2_bound", v) ? std::stof(v) : 0.f;
   c.is_hilbert = field(l, "curve", v) ? v == "hilbert" : 0;
   c.rle = field(l, "rle", v) ? std::stoi(v) != 0 : 0;
+  if (field(l, "mode", v) && v == "mapreduce") c.rle = 1;
   c.gr_k = field(l, "gr_k", v) ? std::stoi(v) : -1;
   c.pfd_width = field(l, "pfd_width", v) ? std::stoi(v) : 8;
   c.direct_threshold = field(l, "direct_threshold", v) ? std::stoi(v) : 1024;
