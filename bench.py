@@ -56,7 +56,7 @@ BATCHED = {
                                 l2=0.05, hilbert=False, rle=False),
     # independent clouds, log-uniform sizes in [1K, 4M], terrain (even i) or
     # shells (odd i) with seed i, L2 bound {0.001, 0.01, 0.1}[i mod 3]
-    "batched-uneven": dict(kind="uneven", clouds=512, quantizer=1, hilbert=False, rle=True),
+    "batched-uneven": dict(kind="uneven", clouds=4096, quantizer=1, hilbert=False, rle=True),
     # half terrain, half shells, chunks of 2^U(16, 20) points: exhaustive
     # mode search ("plain") vs the learned selector
     "selector-64M": dict(kind="selector", n=64_000_000, quantizer=1),
@@ -91,17 +91,20 @@ class OutBuffer:
         return np.zeros(nbytes, dtype=np.uint8)
 
     def compress(self, X, dev, offs, opts, ctx):
-        import numpy as np
+        return self.call(lambda buf: X.compress_batch_into(dev, offs, opts, buf, ctx=ctx))
+
+    def call(self, fn):
+        """fn(buf) with the buffer grown once if the call reports it too small."""
         from xnyzip_amd import _lib
         try:
-            return X.compress_batch_into(dev, offs, opts, self.buf, ctx=ctx)
+            return fn(self.buf)
         except _lib.XnyzError as e:
             if e.code != _lib.XNYZ_ENOSPC:
                 raise
             need = int(e.args[0].rsplit(":", 1)[1].split()[0])
             self.buf = None
             self.buf = self._alloc(need + (1 << 20))
-            return X.compress_batch_into(dev, offs, opts, self.buf, ctx=ctx)
+            return fn(self.buf)
 
 
 def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launches: int,
@@ -148,6 +151,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary frame modes")
+    ap.add_argument("--no-verify", action="store_true",
+                    help="batched workloads: skip the oracle check of sampled clouds")
     ap.add_argument("--tuning", default=None, metavar="FILE",
                     help="run-time tuning table (default: the packaged MI355X table)")
     ap.add_argument("--autotune", action="store_true",
@@ -367,6 +372,34 @@ def verify_stream(workload: str, w: dict, n: int, stream: bytes) -> str:
             else "MISMATCH against the oracle's stream")
 
 
+def _verify_clouds(buf, res, dev, offs, opts, sample: int = 8) -> str:
+    """Checks sampled clouds of a timed batch (the smallest ones, which reach
+    direct mode, and seeded picks of up to 400k points) against the oracle's
+    assembled streams, on host threads."""
+    import random
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+    count = len(offs) - 1
+    size = [offs[i + 1] - offs[i] for i in range(count)]
+    pick = sorted(range(count), key=lambda i: size[i])[:sample // 2]
+    rest = [i for i in range(count) if size[i] <= 400_000 and i not in pick]
+    pick += random.Random(0).sample(rest, min(len(rest), sample - len(pick)))
+    clouds = {i: dev[offs[i]:offs[i + 1]].cpu().numpy() for i in pick}
+
+    def one(i):
+        o = opts[i]
+        want = O.compress(clouds[i], o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
+                          o.get("rle", False), o.get("direct_threshold", 1024)).pre
+        got = bytes(buf[int(res[i]):int(res[i + 1])])
+        return i, got == want
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        bad = [i for i, ok in ex.map(one, pick) if not ok]
+    if bad:
+        return f"MISMATCH against the oracle in clouds {bad}"
+    return f"clouds {sorted(pick)} byte-identical to the oracle's streams"
+
+
 def dev_frame_bytes(total: int, tuning: dict) -> int:
     """Stream bytes of a single host-threaded frame whose blocks the device
     encodes under `tuning` (the split rule of the library's frame stage: the
@@ -414,12 +447,6 @@ def measured_traffic(kernel: str, workload: str, n: int):
     return None
 
 
-def _uneven_clouds(count):
-    from xnyzip_amd import datasets
-    return ([datasets.uneven_cloud(i) for i in range(count)],
-            [datasets.uneven_opts(i) for i in range(count)])
-
-
 def run_batched(args) -> int:
     import numpy as np
     import torch
@@ -436,12 +463,11 @@ def run_batched(args) -> int:
         opts = [dict(quantizer=w["quantizer"], l2_bound=w["l2"], is_hilbert=w["hilbert"],
                      rle=w["rle"]) for _ in range(len(offs) - 1)]
     elif w["kind"] == "uneven":
-        clouds, opts = _uneven_clouds(args.clouds or w["clouds"])
+        def progress(done, count):
+            print(f"[bench] {done}/{count} clouds generated", file=sys.stderr, flush=True)
+        pts, offs, opts = datasets.uneven_workload(args.clouds or w["clouds"], progress=progress)
         for o in opts:
             o.update(is_hilbert=w["hilbert"], rle=w["rle"])
-        pts = np.concatenate(clouds)
-        offs = np.concatenate([[0], np.cumsum([len(c) for c in clouds])]).tolist()
-        del clouds
     else:
         pts, offs, opts = datasets.selector_workload(args.points or w["n"])
     n = len(pts)
@@ -473,44 +499,74 @@ def run_batched(args) -> int:
            "data": "synthetic",
            "config": {"workload": args.workload, "points": n, "clouds": len(offs) - 1}}
     if w["kind"] != "selector":
-        for mode, key in ((_lib.FRAME_ZSTD3, "frames"), (_lib.FRAME_NONE, "prezstd"),
-                          (_lib.FRAME_ZSTD_GPU, "gpuframes")):
+        forms = [(_lib.FRAME_NONE, "prezstd"), (_lib.FRAME_ZSTD3, "frames")]
+        # The device-built frame needs ~18 bytes of scratch per stream byte
+        # (per 16 KiB block: candidates, parse slots, FSE records); beyond
+        # about 700M points that exceeds what is left of HBM.
+        if n <= 700_000_000:
+            forms.append((_lib.FRAME_ZSTD_GPU, "gpuframes"))
+        else:
+            out["gpuframes"] = "not run: device-frame scratch beyond HBM at this size"
+        for mode, key in forms:
             o2 = [dict(o, frame_mode=mode) for o in opts]
 
             def step():
                 res, st = sink.compress(X, dev, offs, o2, ctx)
-                return int(res[-1]), st
-            dt, (nbytes, _) = timed(step, args.steps, wu)
+                return int(res[-1] - res[0]), res
+            dt, (nbytes, res) = timed(step, args.steps, wu)
+            torch.cuda.synchronize()
             gbps = 12.0 * n * args.steps / dt / 1e9
             out[f"{key}_GBps"] = round(gbps, 3)
             out[f"{key}_bytes"] = nbytes
             out[f"{key}_ms_per_step"] = round(dt / args.steps * 1e3, 3)
-        out.update(metric="compress GB/s (fp32 xyz in), batched, zstd L3 frame per cloud",
-                   value=out["frames_GBps"], ms_per_step=out["frames_ms_per_step"],
-                   ratio=round(12.0 * n / out["frames_bytes"], 4))
+            print(f"[bench] {key}: {gbps:.2f} GB/s", file=sys.stderr, flush=True)
+            if mode == _lib.FRAME_NONE and not args.no_verify:
+                out["verified"] = _verify_clouds(sink.buf, res, dev, offs, opts)
+        out.update(metric="compress GB/s (fp32 xyz in), batched, assembled stream per cloud "
+                          "(byte-identical to the reference's pre-zstd buffer); zstd L3 frame "
+                          "per cloud in 'frames_GBps'",
+                   value=out["prezstd_GBps"], ms_per_step=out["prezstd_ms_per_step"],
+                   ratio=round(12.0 * n / out["frames_bytes"], 4),
+                   stream_ratio=round(12.0 * n / out["prezstd_bytes"], 4))
     else:
         # The plain method searches every chunk's mode exhaustively (curve x
         # RLE x Golomb-Rice k x PFD width) before compressing it; its GPU form
         # is one search and one compression call per chunk.  The batched form
         # runs all chunks in one launch set; the learned form picks modes
         # with the selector (falling back to the search below confidence
-        # 0.5).  Frames: zstd-format blocks built on the device (the GPU
-        # form end to end); zstd level 3 on host threads is reported beside.
+        # 0.5); the reference-mode form codes every chunk with the options it
+        # is given (Z-order, no RLE, the reference's own k rule and PFD
+        # limit), as the reference does.  Every form writes into one reused
+        # pinned buffer.  Frames: zstd-format blocks built on the device (the
+        # GPU form end to end); zstd level 3 on host threads is reported
+        # beside.
         from xnyzip_amd import selector
         params = selector.load_params()
         res = {}
-        for label, p in (("plain_search", None), ("learned", params)):
-            for mode, key in ((_lib.FRAME_ZSTD_GPU, "gpuframes"), (_lib.FRAME_ZSTD3, "l3frames")):
-                o2 = [dict(o, frame_mode=mode) for o in opts]
 
-                def step():
-                    fr, _, _, modes, searched = X.compress_selected_batch(
-                        dev, offs, o2, params=p, min_confidence=0.5, ctx=ctx)
-                    return sum(len(f) for f in fr), int(np.sum(searched & 1))
-                dt, (nbytes, fell) = timed(step, args.steps, wu)
+        def selected(p, o2):
+            r, _, _, searched = sink.call(lambda buf: X.compress_selected_batch_into(
+                dev, offs, o2, buf, params=p, min_confidence=0.5, ctx=ctx))
+            return int(r[-1] - r[0]), int(np.sum(searched & 1))
+
+        def reference_mode(o2):
+            r, _ = sink.compress(X, dev, offs, o2, ctx)
+            return int(r[-1] - r[0]), 0
+
+        for mode, key in ((_lib.FRAME_ZSTD_GPU, "gpuframes"), (_lib.FRAME_ZSTD3, "l3frames"),
+                          (_lib.FRAME_NONE, "prezstd")):
+            o2 = [dict(o, frame_mode=mode) for o in opts]
+            for label, fn in (("plain_search", lambda: selected(None, o2)),
+                              ("learned", lambda: selected(params, o2)),
+                              ("reference_mode", lambda: reference_mode(o2))):
+                dt, (nbytes, fell) = timed(fn, args.steps, wu)
+                torch.cuda.synchronize()
                 res[f"{label}_{key}_GBps"] = round(12.0 * n * args.steps / dt / 1e9, 3)
                 res[f"{label}_{key}_bytes"] = nbytes
-                res[f"{label}_fell_back"] = fell
+                if label == "learned":
+                    res["learned_fell_back"] = fell
+                print(f"[bench] {label} {key}: {res[f'{label}_{key}_GBps']} GB/s",
+                      file=sys.stderr, flush=True)
 
         def per_chunk():
             tot = 0
@@ -520,29 +576,22 @@ def run_batched(args) -> int:
                 m = X.search_batch(sub, [0, b - a], [opts[i]], ctx=ctx)[0]["best"]
                 o = dict(opts[i], is_hilbert=bool(m["is_hilbert"]), rle=bool(m["rle"]),
                          gr_k=m["gr_k"], pfd_width=m["pfd_width"], frame_mode=_lib.FRAME_ZSTD_GPU)
-                fr, _, _ = X.compress_batch(sub, [0, b - a], [o], ctx=ctx)
-                tot += len(fr[0])
+                r, _ = sink.compress(X, sub, [0, b - a], [o], ctx)
+                tot += int(r[1] - r[0])
             return tot
         k1 = max(1, min(2, args.steps))
         dt, nbytes = timed(per_chunk, k1, 1)
         res["plain_per_chunk_gpuframes_GBps"] = round(12.0 * n * k1 / dt / 1e9, 3)
         res["plain_per_chunk_gpuframes_bytes"] = nbytes
-        o2 = [dict(o, frame_mode=_lib.FRAME_NONE) for o in opts]
-        fr, _, _, _, _ = X.compress_selected_batch(dev, offs, o2, params=None, ctx=ctx)
-        res["plain_search_prezstd_bytes"] = sum(len(f) for f in fr)
-        fr, _, _, _, _ = X.compress_selected_batch(dev, offs, o2, params=params,
-                                                   min_confidence=0.5, ctx=ctx)
-        res["learned_prezstd_bytes"] = sum(len(f) for f in fr)
-        fr, _, _ = X.compress_batch(dev, offs, o2, ctx=ctx)
-        res["reference_mode_prezstd_bytes"] = sum(len(f) for f in fr)
         res["learned_vs_search_bytes"] = round(
             res["learned_prezstd_bytes"] / res["plain_search_prezstd_bytes"] - 1, 6)
+        res["reference_mode_vs_search_bytes"] = round(
+            res["reference_mode_prezstd_bytes"] / res["plain_search_prezstd_bytes"] - 1, 6)
+        for key in ("gpuframes", "l3frames", "prezstd"):
+            res[f"speedup_{key}_vs_plain_batched"] = round(
+                res[f"learned_{key}_GBps"] / res[f"plain_search_{key}_GBps"], 3)
         res["speedup_vs_plain_per_chunk"] = round(
             res["learned_gpuframes_GBps"] / res["plain_per_chunk_gpuframes_GBps"], 3)
-        res["speedup_vs_plain_batched"] = round(
-            res["learned_gpuframes_GBps"] / res["plain_search_gpuframes_GBps"], 3)
-        res["speedup_l3frames_vs_plain_batched"] = round(
-            res["learned_l3frames_GBps"] / res["plain_search_l3frames_GBps"], 3)
         out.update(metric="compress GB/s (fp32 xyz in), batched+learned vs plain (exhaustive "
                           "search per chunk), device-built zstd frames",
                    value=res["learned_gpuframes_GBps"], **res,

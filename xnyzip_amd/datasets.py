@@ -20,6 +20,7 @@ float32 arrays of interleaved x, y, z, and run on the host.
 from __future__ import annotations
 
 import os
+import threading
 
 import numpy as np
 
@@ -115,6 +116,33 @@ def uneven_cloud(i: int, threads: int = 4) -> np.ndarray:
     if i % 2 == 0:
         return terrain(n, seed=i, threads=threads)
     return shells(n, seed=i, count=max(1, min(2000, n // 1000)))
+
+
+def uneven_workload(count: int, threads: int = 0, progress=None):
+    """The first `count` uneven clouds back to back in one array (filled in
+    place by a thread pool over the clouds, so the whole set is never held
+    twice), their point offsets and options.  progress(done, count) is
+    called after every 256th finished cloud."""
+    from concurrent.futures import ThreadPoolExecutor
+    sizes = [uneven_size(i) for i in range(count)]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    out = np.empty((int(offs[-1]), 3), dtype=np.float32)
+
+    done = [0]
+    lock = threading.Lock()
+
+    def one(i: int) -> None:
+        out[offs[i]:offs[i + 1]] = uneven_cloud(i, threads=1)
+        with lock:
+            done[0] += 1
+            if progress is not None and done[0] % 256 == 0:
+                progress(done[0], count)
+
+    # largest clouds first, so the pool does not end on one big cloud
+    order = sorted(range(count), key=lambda i: -sizes[i])
+    with ThreadPoolExecutor(max_workers=threads or min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(one, order))
+    return out, offs.tolist(), [uneven_opts(i) for i in range(count)]
 
 
 def uneven_opts(i: int) -> dict:
