@@ -89,6 +89,38 @@ def test_wide_keys(X, rle):
     assert not st.is_direct
 
 
+def _grid(step, n, origin=0):
+    g = origin + np.arange(n) * step
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+
+
+# cube bound whose divisor is exactly 1.0f: integer coordinates are the nodes
+CUBE_B1 = float(np.float32(np.sqrt(3.0) / 2.0))
+
+
+@pytest.mark.parametrize("rle", [False, True])
+@pytest.mark.parametrize("direct", [False, True])
+def test_tile_spans_beyond_the_window(X, direct, rle):
+    """Tiles whose Golomb-Rice runs exceed the section writer's LDS window
+    (over 66 bits per element), which are written straight to memory.  Block
+    mode: a dense grid of Morton deltas 512 sets k = 9; the last 4096 cells
+    are pairs one sub-block diagonal apart in blocks of their own (quotients
+    of 512).  Direct mode: the last 4096 points lie 2^14 apart on each axis
+    (Morton deltas near 2^42, k near 35: codes longer than 32 bits)."""
+    dense = _grid(8, 160)
+    if direct:
+        sparse = _grid(16384, 16, origin=16384)
+    else:
+        bx, by, bz = np.meshgrid(np.arange(16), np.arange(16), np.arange(16, 24), indexing="ij")
+        base = np.stack([bx.ravel(), by.ravel(), bz.ravel()], 1) * 128
+        sparse = np.concatenate([base, base + 63])
+    pts = np.concatenate([dense, sparse]).astype(np.float32)
+    st, ref = _check(X, pts, O.CUBE, CUBE_B1, False, rle, thr=(1 << 20) if direct else 1024)
+    assert st.is_direct == direct
+    assert ref.info.patches > 3_000_000
+
+
 def test_large_counts(X):
     pts = _uniform(6, 3000)
     heavy = np.repeat(pts[:3], [9000, 5000, 300], axis=0)  # counts above 4096

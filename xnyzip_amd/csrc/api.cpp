@@ -1383,7 +1383,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->copied();
   launch_patch(ctx->patch.as<Patch>(), (u32)patches.size(), ctx->patch.as<uint8_t>() + prec,
                reinterpret_cast<uint8_t*>(ow), st);
-  write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, st);
+  write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, any_block, any_direct, st);
 
   // Fills co.stats (frame bytes from co.frames / co.dst_off) once the outputs
   // are in place.

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <type_traits>
 #include <utility>
 
 #include "common.hpp"
@@ -1072,6 +1073,57 @@ __device__ __forceinline__ void lcp_items(const KV& k, bool direct, u32 e0, u32 
   }
 }
 
+// lcp() (or the direct-mode delta) of the kN consecutive elements
+// first + b .. first + b + kN - 1 of a range of `count` elements whose
+// segment starts at e0: each key is loaded once, the one before the run
+// as well; items at or past `count` give v = qd = 0.
+template <u32 kN>
+__device__ __forceinline__ void lcp_run(const KV& k, bool direct, u32 e0, u32 first, u32 count,
+                                        u32 b, u64* v, u32* qd) {
+  const bool wide = k.wide && !direct;
+  const u32 last = first + count - 1;
+  u64 a[kN + 1];
+  u32 w[kN + 1];
+  {
+    const u32 j = first + b;  // b < count for a thread with elements
+    const u32 jp = j > e0 ? j - 1 : j;
+    a[0] = k.a[jp];
+    if (wide) w[0] = k.b[jp];
+  }
+#pragma unroll
+  for (u32 q = 0; q < kN; ++q) {
+    const u32 j = first + b + q;
+    const u32 jc = j < last ? j : last;
+    a[q + 1] = k.a[jc];
+    if (wide) w[q + 1] = k.b[jc];
+  }
+#pragma unroll
+  for (u32 q = 0; q < kN; ++q) {
+    const u32 j = first + b + q;
+    const u64 c0 = a[q + 1], c1 = a[q];
+    u64 r;
+    u32 d = 0;
+    if (direct) {
+      r = j == e0 ? c0 : c0 - c1;
+    } else {
+      const u64 hc = wide ? c0 : c0 >> 21, hp = wide ? c1 : c1 >> 21;
+      const u32 lc = wide ? w[q + 1] : (u32)(c0 & 0x1FFFFFu);
+      const u32 lp = wide ? w[q] : (u32)(c1 & 0x1FFFFFu);
+      const u32 quad = lc >> 18, sfc = lc & 0x3FFFFu;
+      if (j == e0 || hp != hc) {
+        r = sfc;
+        d = quad;
+      } else {
+        const u32 pq = lp >> 18;
+        d = (quad - pq) & 0xFFu;
+        r = pq == quad ? (u64)sfc - (u64)(lp & 0x3FFFFu) : (u64)sfc;
+      }
+    }
+    v[q] = b + q < count ? r : 0;
+    qd[q] = b + q < count ? d : 0;
+  }
+}
+
 enum FlagKind { kFlagCell = 0, kFlagBlock = 1 };
 
 // In a batch mixing RLE and plain segments, the plain segments' points all
@@ -1220,6 +1272,45 @@ void compact(const Batch& bt, const KV& k, u32* pos, int which, Arena& ar, hipSt
 // LDS histogram with first occurrences -> the segment's device histogram.
 // Every thread of the block calls this (bin d = threadIdx.x); clears the LDS
 // copy for the next segment.
+// LDS histograms replicated kRep times (copy c at c * stride words, the
+// stride odd modulo the 32 banks), so that the lanes of a wavefront adding to
+// one bin mostly hit distinct addresses and banks.
+constexpr u32 kRep = 8;
+constexpr u32 kRepStride256 = 257;
+
+// Adds 1 to bin s for every valid lane: a single atomic when all valid lanes
+// of the wavefront share the bin (skewed symbol distributions: RLE counts of
+// 1, quad deltas of 0), else one per lane on the lane's copy.
+template <u32 kStride>
+__device__ __forceinline__ void hist_add(u32* h, u32 s, bool valid) {
+  const u64 act = __ballot(valid);
+  if (!act) return;
+  const u32 first = (u32)__ffsll((unsigned long long)act) - 1;
+  const u32 s0 = (u32)__builtin_amdgcn_readlane((int)s, (int)first);
+  const u64 same = __ballot(valid && s == s0);
+  if (same == act) {
+    if (lane_id() == first) atomicAdd(&h[s0], (u32)__popcll(act));
+  } else if (valid) {
+    atomicAdd(&h[(lane_id() & (kRep - 1)) * kStride + s], 1u);
+  }
+}
+
+// Folds the copies of a replicated histogram into copy 0 and clears the rest
+// (synchronises first; the caller synchronises before reading copy 0).
+template <u32 kBins, u32 kStride>
+__device__ __forceinline__ void fold_hist(u32* h) {
+  __syncthreads();
+  for (u32 d = threadIdx.x; d < kBins; d += kBlock) {
+    u32 sum = h[d];
+#pragma unroll
+    for (u32 c = 1; c < kRep; ++c) {
+      sum += h[c * kStride + d];
+      h[c * kStride + d] = 0;
+    }
+    h[d] = sum;
+  }
+}
+
 __device__ __forceinline__ void flush_hist(u32* h, u32* f, u32* gh, u32* gf) {
   __syncthreads();
   const u32 d = threadIdx.x;
@@ -1236,18 +1327,19 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
                                                        u64* __restrict__ ua,
                                                        u32* __restrict__ ub,
                                                        u32* __restrict__ cnt) {
-  __shared__ u32 hs[kCntBinsSeg], fs[kCntBinsSeg];
+  __shared__ u32 hs[kRep * kRepStride256], fs[kCntBinsSeg];
   static_assert(kCntBinsSeg == kBlock, "one count bin per thread");
   u32 t0, t1;
   block_tiles(bt.ntiles, t0, t1);
   if (t0 >= t1) return;
-  hs[threadIdx.x] = 0;
+  for (u32 i = threadIdx.x; i < kRep * kRepStride256; i += kBlock) hs[i] = 0;
   fs[threadIdx.x] = 0xFFFFFFFFu;
   __syncthreads();
   u32 cur = bt.tiles[t0].seg;
   for (u32 ti = t0; ti < t1; ++ti) {
     const Tile t = bt.tiles[ti];
     if (t.seg != cur) {
+      fold_hist<kCntBinsSeg, kRepStride256>(hs);
       flush_hist(hs, fs, bt.sh[cur].cnt, bt.sh[cur].first_cnt);
       cur = t.seg;
     }
@@ -1275,15 +1367,15 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
 #pragma unroll
     for (u32 it = 0; it < kItems; ++it) {
       const u32 o = it * kBlock + threadIdx.x;
+      const u32 c = en[it] - st[it];
+      hist_add<kRepStride256>(hs, c, o < r.count && c < kCntBinsSeg);
       if (o < r.count) {
         const u32 u = r.first + o;
-        const u32 c = en[it] - st[it];
         ua[u] = ka[it];
         if (in.wide) ub[u] = kb[it];
         cnt[u] = c;
         const u32 rel = u - sc.e0;
         if (c < kCntBinsSeg) {
-          atomicAdd(&hs[c], 1u);
           if (rel < fs[c]) atomicMin(&fs[c], rel);
         } else {
           const u32 slot = atomicAdd(bt.n_large, 1u);
@@ -1296,6 +1388,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
       }
     }
   }
+  fold_hist<kCntBinsSeg, kRepStride256>(hs);
   flush_hist(hs, fs, bt.sh[cur].cnt, bt.sh[cur].first_cnt);
 }
 
@@ -1303,9 +1396,11 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
 // (block mode) or of the SFC deltas (direct mode, block_compressor.hpp:110-117),
 // quads symbols, patch count and wrapping sum.  First occurrences are
 // relative to the segment's first element.
+constexpr u32 kMagStride = kMagBins + 1;  // 69: odd modulo 32
 struct StatLds {
-  u32 hs[256], fs[256], hq[256], fq[256];
-  u32 mag[kMagBins];
+  u32 hs[kRep * kRepStride256], hq[kRep * kRepStride256];  // replicated
+  u32 fs[256], fq[256];
+  u32 mag[kRep * kMagStride];  // replicated
   u64 bitpop[64];
   u64 red[kWavesPerBlock];
 };
@@ -1322,6 +1417,10 @@ __device__ __forceinline__ void flush_stats(StatLds& L, SegCtl& sc, SegHist& h, 
   }
   np = nqp = 0;
   ps = 0;
+  fold_hist<kMagBins, kMagStride>(L.mag);
+  fold_hist<256, kRepStride256>(L.hq);
+  fold_hist<256, kRepStride256>(L.hs);
+  __syncthreads();
   const u32 d = threadIdx.x;
   if (d < kMagBins) {
     if (L.mag[d]) atomicAdd(&h.mag[d], L.mag[d]);
@@ -1341,9 +1440,9 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
   block_tiles(bt.ntiles, t0, t1);
   if (t0 >= t1) return;
   const u32 d = threadIdx.x, l = lane_id();
-  L.hs[d] = L.hq[d] = 0;
+  for (u32 i = d; i < kRep * kRepStride256; i += kBlock) L.hs[i] = L.hq[i] = 0;
+  for (u32 i = d; i < kRep * kMagStride; i += kBlock) L.mag[i] = 0;
   L.fs[d] = L.fq[d] = 0xFFFFFFFFu;
-  if (d < kMagBins) L.mag[d] = 0;
   if (d < 64) L.bitpop[d] = 0;
   __syncthreads();
   u32 np = 0, nqp = 0;
@@ -1361,41 +1460,44 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
     const bool direct = sp.kp.direct != 0;
     const u32 lim = sp.lim;
     u64 bp = 0;  // this lane's share of bitpop: lane b counts bit b
-    // Loads first (all items in flight), then the LDS updates.
+    // Loads first (all items in flight), then the LDS updates.  Each thread
+    // takes 16 consecutive elements (each key loaded once).
     u64 v[kItems];
-    u32 qd[kItems], oo[kItems];
-#pragma unroll
-    for (u32 it = 0; it < kItems; ++it) oo[it] = it * kBlock + threadIdx.x;
-    if (r.count) {
-      lcp_items<kItems>(k, direct, sc.e0, r.first, r.count, oo, v, qd);
+    u32 qd[kItems];
+    const u32 b = threadIdx.x * kItems;
+    if (b < r.count) {
+      lcp_run<kItems>(k, direct, sc.e0, r.first, r.count, b, v, qd);
     } else {
 #pragma unroll
       for (u32 it = 0; it < kItems; ++it) v[it] = qd[it] = 0;
     }
 #pragma unroll
     for (u32 it = 0; it < kItems; ++it) {
-      const u32 o = it * kBlock + threadIdx.x;
-      if (o < r.count) {
-        const u32 rel = r.first + o - sc.e0;
-        if (!direct) {
-          const u32 qs = qd[it] > 254 ? 0u : qd[it] + 1u;
+      const u32 o = b + it;
+      const bool in = o < r.count;
+      const u32 rel = r.first + o - sc.e0;
+      const u32 qs = qd[it] > 254 ? 0u : qd[it] + 1u;
+      const u32 sym = v[it] > lim ? 0u : (u32)v[it] + 1u;
+      if (!direct) {
+        hist_add<kRepStride256>(L.hq, qs, in);
+        if (in) {
           nqp += qd[it] > 254;
-          atomicAdd(&L.hq[qs], 1u);
           if (rel < L.fq[qs]) atomicMin(&L.fq[qs], rel);
         }
-        const u32 sym = v[it] > lim ? 0u : (u32)v[it] + 1u;
-        atomicAdd(&L.hs[sym], 1u);
+      }
+      hist_add<kRepStride256>(L.hs, sym, in);
+      hist_add<kMagStride>(L.mag, bitlen64(v[it]), in);
+      if (in) {
         if (rel < L.fs[sym]) atomicMin(&L.fs[sym], rel);
         if (v[it] > lim) {
           ++np;
           ps += v[it];
         }
-        atomicAdd(&L.mag[bitlen64(v[it])], 1u);
       }
     }
     if (sp.search) {  // bit populations of the values > 254 (mode search only)
       for (u32 it = 0; it < kItems; ++it) {
-        const bool big = it * kBlock + threadIdx.x < r.count && v[it] > 254;
+        const bool big = b + it < r.count && v[it] > 254;
         if (__ballot(big)) {
 #pragma unroll 8
           for (u32 b = 0; b < 64; ++b) {
@@ -1836,82 +1938,458 @@ __global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const 
   pack_tile<kKind>(sp, sc, tile_off[blockIdx.x], cd, win, sh, out);
 }
 
+// A thread's run of bits inside a section: its codes are consecutive in the
+// stream, so whole 32-bit words are assembled in a register and stored
+// plainly; only the run's first and last words, which neighbouring runs
+// share, are ORed.  Zero words are skipped: the target (the LDS window or the
+// zero-filled output) already holds zeros.  `words` is the LDS window
+// (kLds) or the output (big-endian byte order applied here for MSB-first
+// streams).
+template <bool kMsb, bool kLds>
+struct BitRun {
+  using Idx = typename std::conditional<kLds, u32, u64>::type;  // window words fit 32 bits
+  u32* words;
+  Idx cw, first, last;  // current word; the run's first and last words
+  u64 acc;              // bits not yet stored; MSB-first: word cw in acc's top half
+  u32 fill;             // bits of word cw already taken (by this run or before it)
+
+  __device__ __forceinline__ BitRun(u32* w, u64 p, u64 nbits)
+      : words(w), cw((Idx)(p >> 5)), first((Idx)(p >> 5)), last((Idx)((p + nbits - 1) >> 5)),
+        acc(0), fill((u32)(p & 31)) {}
+
+  __device__ __forceinline__ void store(u32 v) {
+    if (v) {
+      if (!kLds && kMsb) v = bswap32(v);
+      if (cw == first || cw == last) atomicOr(&words[cw], v);
+      else words[cw] = v;
+    }
+    ++cw;
+  }
+  // `n` <= 32 bits of `v` (its low bits) at the current position.
+  __device__ __forceinline__ void put(u32 v, u32 n) {
+    if (!n) return;
+    if (kMsb) {
+      acc |= ((u64)v << (64 - n)) >> fill;
+    } else {
+      acc |= (u64)(n == 32 ? v : v & ((1u << n) - 1u)) << fill;
+    }
+    fill += n;
+    if (fill >= 32) {
+      store(kMsb ? (u32)(acc >> 32) : (u32)acc);
+      acc = kMsb ? acc << 32 : acc >> 32;
+      fill -= 32;
+    }
+  }
+  // `n` zero bits (MSB-first Exp-Golomb prefixes).
+  __device__ __forceinline__ void zeros(u32 n) {
+    fill += n;
+    while (fill >= 32) {
+      store(kMsb ? (u32)(acc >> 32) : (u32)acc);
+      acc = kMsb ? acc << 32 : acc >> 32;
+      fill -= 32;
+    }
+  }
+  // `n` one bits (LSB-first Golomb-Rice quotients; any length).
+  __device__ __forceinline__ void ones(u32 n) {
+    while (n) {
+      const u32 t = n < 32 ? n : 32;
+      put(t == 32 ? 0xFFFFFFFFu : (1u << t) - 1u, t);
+      n -= t;
+    }
+  }
+  __device__ __forceinline__ void code(const Code& c) {
+    if (kMsb) {
+      zeros(c.lead);
+      if (c.len > 32) {
+        put((u32)(c.val >> 32), c.len - 32);
+        put((u32)c.val, 32);
+      } else {
+        put((u32)c.val, c.len);
+      }
+    } else {
+      ones(c.lead);
+      if (c.len > 32) {
+        put((u32)c.val, 32);
+        put((u32)(c.val >> 32), c.len - 32);
+      } else {
+        put((u32)c.val, c.len);
+      }
+    }
+  }
+  __device__ __forceinline__ void finish() {
+    if (fill) store(kMsb ? (u32)(acc >> 32) : (u32)acc);
+  }
+};
+
+// Packs one section of a tile whose threads own kItems consecutive elements
+// each ("blocked": thread t codes elements 16t .. 16t + 15, so its codes are
+// one run of the stream).  code(q) gives the thread's q-th code ({0,0,0}
+// past the range).  The runs are placed by a block scan of their lengths,
+// assembled in the LDS window (or straight into the output when the tile's
+// span exceeds it), and the window's words stored, its two edge words ORed.
+// Starts with a barrier (inside the scan), so calls for several sections may
+// follow one another on the same window.
+template <u32 kKind, typename CodeFn>
+__device__ __forceinline__ void pack_blocked(const SegParam& sp, const SegCtl& sc, u64 toff,
+                                             CodeFn code, u32* win, u64* sh, u32* out) {
+  constexpr bool kMsb = SecTraits<kKind>::kMsb;
+  u64 mine = 0;
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const Code c = code(q);
+    mine += (u64)c.lead + c.len;
+  }
+  u64 tot;
+  const u64 ex = block_excl_scan(mine, sh, tot);
+  if (tot == 0) return;
+  const u64 abs0 = sp.sec_byte[kKind] * 8 + (toff - sc.sec_bit0[kKind]);
+  const u64 w0 = abs0 >> 5;
+  const u32 o0 = (u32)(abs0 & 31);
+  const u64 nwords = (o0 + tot + 31) >> 5;
+  if (nwords <= kVlWin) {
+    for (u32 i = threadIdx.x; i < nwords; i += kBlock) win[i] = 0;
+    __syncthreads();
+    if (mine) {
+      BitRun<kMsb, true> run(win, o0 + ex, mine);
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) run.code(code(q));
+      run.finish();
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nwords; i += kBlock) {
+      u32 v = win[i];
+      if (kMsb) v = bswap32(v);
+      if (i == 0 || i == nwords - 1) {
+        if (v) atomicOr(&out[w0 + i], v);
+      } else {
+        out[w0 + i] = v;
+      }
+    }
+    __syncthreads();  // the window is free for the next section
+  } else if (mine) {
+    BitRun<kMsb, false> run(out, abs0 + ex, mine);
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) run.code(code(q));
+    run.finish();
+  }
+}
+
 // The four cell-space sections (Golomb-Rice patches, Huffman symbols,
-// quads, RLE counts) of one tile in one pass: the keys (and counts) are
-// loaded and the repos / quads derived once, then each active section's
-// codes are formed and packed in turn on the same LDS window.
+// quads, RLE counts) of one tile in one pass.  Each thread owns 16
+// consecutive elements, so its codes form one run of each section.  The
+// tile's keys (and counts) are loaded coalesced into LDS and read back per
+// thread; the repos / quads are derived once; every thread then emits its
+// runs of all four sections together, into the LDS window when the four
+// spans fit in it (stored to the output afterwards), otherwise straight into
+// the output.  kDirect: the direct-mode tiles (64-bit deltas, no quads);
+// each launch skips the other mode's tiles.  kWide: block-mode keys split
+// into (u64 hi, u32 lo), loaded straight from memory.
+//
+// LDS staging: element e of the tile (e = -1: the element before the tile)
+// sits at slot e + 1, halves of a 64-bit key in two 32-bit arrays, with one
+// pad word per 16 slots, so the 64 lanes reading 16 consecutive slots each
+// hit distinct banks.
+__device__ __forceinline__ u32 stage_slot(u32 s) { return s + (s >> 4); }
+constexpr u32 kStageWords = kTile + 1 + (kTile + 1) / 16 + 1;  // one 32-bit array
+constexpr u32 kCellLds = 2 * kStageWords > kVlWin ? 2 * kStageWords : kVlWin;
+
+// Emits every element's codes of the Huffman-coded sections (symbols,
+// quads, RLE counts) into three runs.
+template <typename R1, typename R2, typename R3, typename F1, typename F2, typename F3>
+__device__ __forceinline__ void emit_cells(R1& rh, R2& rq, R3& rc, bool h, bool qu, bool c,
+                                           F1 fh, F2 fq, F3 fc) {
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    if (h) rh.code(fh(q));
+    if (qu) rq.code(fq(q));
+    if (c) rc.code(fc(q));
+  }
+  if (h) rh.finish();
+  if (qu) rq.finish();
+  if (c) rc.finish();
+}
+
+// The thread's Golomb-Rice run: only patches have codes.
+template <typename R, typename F>
+__device__ __forceinline__ void emit_gr(R& run, F f) {
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const Code c = f(q);
+    if (c.len) run.code(c);
+  }
+  run.finish();
+}
+
+template <bool kDirect, bool kWide>
 __global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
                                                             const u64* tile_off, u32* out) {
   static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
-  __shared__ u32 win[kVlWin];
-  __shared__ u64 sh[kWavesPerBlock];
-  __shared__ HufEntry tsym[kBlock], tquad[kBlock], tcnt[kBlock];
+  using V = typename std::conditional<kDirect, u64, u32>::type;  // block-mode repos < 2^18
+  __shared__ u32 win[kCellLds];
+  __shared__ u64 sh[4 * kWavesPerBlock];
+  __shared__ HufEntry tsym[kBlock], tquad[kDirect ? 1 : kBlock], tcnt[kBlock];
   const Tile t = bt.tiles[blockIdx.x];
   const SegParam& sp = bt.sp[t.seg];
+  if ((sp.kp.direct != 0) != kDirect) return;
   const SegCtl& sc = bt.sc[t.seg];
   const Range r = tile_range(sc.e0, sc.ne, t.local);
   if (r.count == 0) return;
-  const bool gr = sec_active<kSecGR>(sp, sc), quad = sec_active<kSecQuad>(sp, sc);
+  const bool gr = sec_active<kSecGR>(sp, sc), quad = !kDirect;
   const bool rle = sec_active<kSecRle>(sp, sc);
   const SegLut& lut = bt.lut[t.seg];
   tsym[threadIdx.x] = lut.sym[threadIdx.x];
-  if (quad) tquad[threadIdx.x] = lut.quad[threadIdx.x];
+  if (!kDirect) tquad[threadIdx.x] = lut.quad[threadIdx.x];
   if (rle) tcnt[threadIdx.x] = lut.cnt[threadIdx.x];
-  __syncthreads();
-  const u32 w = threadIdx.x / kWave, l = lane_id(), wstart = w * kWaveSub;
+  const u32 b = threadIdx.x * kItems;  // this thread's first element in the tile
+  const bool has = b < r.count;
   const u32 lim = sp.lim, grk = sp.grk;
-  const u64 nt = bt.ntiles, b = blockIdx.x;
-  u32 o[kItems], qd[kItems], c[kItems];
-  u64 v[kItems];
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) o[q] = wstart + q * kWave + l;
-  if (rle) {
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) c[q] = x.cnt[r.first + (o[q] < r.count ? o[q] : r.count - 1)];
-  }
-  lcp_items<kItems>(x.k, sp.kp.direct != 0, sc.e0, r.first, r.count, o, v, qd);
-  Code cd[kItems];
-  if (gr) {  // Golomb-Rice of the patches, LSB-first
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q)
-      cd[q] = o[q] < r.count && v[q] > lim
-                  ? Code{(v[q] & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(v[q] >> grk)}
-                  : Code{0, 0, 0};
-    pack_tile<kSecGR>(sp, sc, tile_off[kSecGR * nt + b], cd, win, sh, out);
-  }
+  const u64 nt = bt.ntiles, blk = blockIdx.x;
+  const u32 last = r.first + r.count - 1;
+  V v[kItems];
+  u32 qd4[kItems / 4];  // quad deltas, four bytes per word
+  u32 c[kItems];
 #pragma unroll
   for (u32 q = 0; q < kItems; ++q) {
-    const HufEntry e = tsym[v[q] > lim ? 0u : (u32)v[q] + 1u];
-    cd[q] = o[q] < r.count ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
+    v[q] = 0;
+    c[q] = 0;
   }
-  pack_tile<kSecHuf>(sp, sc, tile_off[kSecHuf * nt + b], cd, win, sh, out);
-  if (quad) {
 #pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      const HufEntry e = tquad[qd[q] > 254 ? 0u : qd[q] + 1u];
-      cd[q] = o[q] < r.count ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
+  for (u32 q = 0; q < kItems / 4; ++q) qd4[q] = 0;
+  if (kWide) {
+    // (hi, lo) keys: strided loads straight from memory
+    __syncthreads();
+    if (has) {
+      u64 vv[kItems];
+      u32 dd[kItems];
+      lcp_run<kItems>(x.k, false, sc.e0, r.first, r.count, b, vv, dd);
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) {
+        v[q] = (V)vv[q];
+        qd4[q / 4] |= dd[q] << (8 * (q % 4));
+      }
     }
-    pack_tile<kSecQuad>(sp, sc, tile_off[kSecQuad * nt + b], cd, win, sh, out);
+  } else {
+    u32* lo32 = win;
+    u32* hi32 = win + kStageWords;
+    {
+      // slots 0 .. count: elements -1 .. count - 1, all loads in flight first
+      constexpr u32 kLd = (kTile + 1 + kBlock - 1) / kBlock;
+      u64 kk[kLd];
+#pragma unroll
+      for (u32 i = 0; i < kLd; ++i) {
+        const u32 s = i * kBlock + threadIdx.x;
+        const u32 j = s == 0 ? (r.first > sc.e0 ? r.first - 1 : r.first) : r.first + s - 1;
+        kk[i] = x.k.a[j < last ? j : last];
+      }
+#pragma unroll
+      for (u32 i = 0; i < kLd; ++i) {
+        const u32 s = i * kBlock + threadIdx.x;
+        if (s <= r.count) {
+          lo32[stage_slot(s)] = (u32)kk[i];
+          hi32[stage_slot(s)] = (u32)(kk[i] >> 32);
+        }
+      }
+    }
+    __syncthreads();
+    if (has) {
+      u64 a[kItems + 1];
+#pragma unroll
+      for (u32 q = 0; q <= kItems; ++q) {
+        const u32 s = stage_slot(b + q);
+        a[q] = ((u64)hi32[s] << 32) | lo32[s];
+      }
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) {
+        const u32 j = r.first + b + q;
+        const u64 c0 = a[q + 1], c1 = a[q];
+        V rr;
+        u32 d = 0;
+        if (kDirect) {
+          rr = (V)(j == sc.e0 ? c0 : c0 - c1);
+        } else {
+          const u32 lc = (u32)(c0 & 0x1FFFFFu), lp = (u32)(c1 & 0x1FFFFFu);
+          const u32 qc = lc >> 18, sfc = lc & 0x3FFFFu;
+          if (j == sc.e0 || (c0 >> 21) != (c1 >> 21)) {
+            rr = sfc;
+            d = qc;
+          } else {
+            const u32 pq = lp >> 18;
+            d = (qc - pq) & 0xFFu;
+            rr = pq == qc ? sfc - (lp & 0x3FFFFu) : sfc;
+          }
+        }
+        const bool in = b + q < r.count;
+        v[q] = in ? rr : 0;
+        qd4[q / 4] |= (in ? d : 0u) << (8 * (q % 4));
+      }
+    }
   }
   if (rle) {
-    const bool eg = sp.cnt_eg != 0;
-    const u32 kk = sp.egk;
+    // counts, staged the same way (element e at slot e of the low array)
+    u32 cc[kItems];
+    __syncthreads();  // the key slots have been read
 #pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      Code e;
-      if (eg) {  // Exp-Golomb of zigzag(int)
-        const u64 z = (u64)c[q] << 1;
-        const u64 n1 = (z >> kk) + 1;
-        const u32 nb = bitlen64(n1);
-        const u64 low = kk ? (__builtin_bitreverse64(z) >> (64 - kk)) : 0;
-        e = Code{(n1 << kk) | low, nb + kk, nb - 1};
-      } else {
-        const HufEntry h = c[q] < kCntBinsSeg ? tcnt[c[q]] : big_code(bt, sp, c[q]);
-        e = Code{h.bits, h.len, 0};
-      }
-      cd[q] = o[q] < r.count ? e : Code{0, 0, 0};
+    for (u32 i = 0; i < kItems; ++i) {
+      const u32 j = r.first + i * kBlock + threadIdx.x;
+      cc[i] = x.cnt[j < last ? j : last];
     }
-    pack_tile<kSecRle>(sp, sc, tile_off[kSecRle * nt + b], cd, win, sh, out);
+#pragma unroll
+    for (u32 i = 0; i < kItems; ++i) {
+      const u32 e = i * kBlock + threadIdx.x;
+      if (e < r.count) win[stage_slot(e)] = cc[i];
+    }
+    __syncthreads();
+    if (has) {
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q)
+        if (b + q < r.count) c[q] = win[stage_slot(b + q)];
+    }
+  }
+  __syncthreads();  // staging done: the area becomes the window
+
+  auto valid = [&](u32 q) { return b + q < r.count; };
+  auto code_gr = [&](u32 q) {  // Golomb-Rice of the patches, LSB-first
+    const u64 y = v[q];
+    return valid(q) && y > lim ? Code{(y & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(y >> grk)}
+                               : Code{0, 0, 0};
+  };
+  auto code_huf = [&](u32 q) {
+    const HufEntry e = tsym[v[q] > lim ? 0u : (u32)v[q] + 1u];
+    return valid(q) ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
+  };
+  auto code_quad = [&](u32 q) {
+    const u32 d = (qd4[q / 4] >> (8 * (q % 4))) & 0xFFu;
+    const HufEntry e = tquad[kDirect ? 0u : (d > 254 ? 0u : d + 1u)];
+    return valid(q) ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
+  };
+  const bool eg = sp.cnt_eg != 0;
+  const u32 egk = sp.egk;
+  auto code_rle = [&](u32 q) {
+    Code e;
+    if (eg) {  // Exp-Golomb of zigzag(int)
+      const u64 z = (u64)c[q] << 1;
+      const u64 n1 = (z >> egk) + 1;
+      const u32 nb = bitlen64(n1);
+      const u64 low = egk ? (__builtin_bitreverse64(z) >> (64 - egk)) : 0;
+      e = Code{(n1 << egk) | low, nb + egk, nb - 1};
+    } else {
+      const HufEntry h = c[q] < kCntBinsSeg ? tcnt[c[q]] : big_code(bt, sp, c[q]);
+      e = Code{h.bits, h.len, 0};
+    }
+    return valid(q) ? e : Code{0, 0, 0};
+  };
+
+  // run lengths of the four sections and their block scans, together
+  constexpr u32 kKinds[4] = {kSecGR, kSecHuf, kSecQuad, kSecRle};
+  u64 mine[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    if (gr && valid(q) && v[q] > lim) mine[0] += ((u64)v[q] >> grk) + grk + 1;
+    mine[1] += code_huf(q).len;
+    if (quad) mine[2] += code_quad(q).len;
+    if (rle) {
+      const Code e = code_rle(q);
+      mine[3] += (u64)e.lead + e.len;
+    }
+  }
+  u64 ex[4], tot[4];
+  {
+    const u32 l = lane_id(), w = threadIdx.x / kWave;
+    u64 inc[4];
+#pragma unroll
+    for (u32 s = 0; s < 4; ++s) inc[s] = mine[s];
+#pragma unroll
+    for (u32 o = 1; o < kWave; o <<= 1) {
+      u64 up[4];
+#pragma unroll
+      for (u32 s = 0; s < 4; ++s) up[s] = __shfl_up(inc[s], o, kWave);
+#pragma unroll
+      for (u32 s = 0; s < 4; ++s)
+        if (l >= o) inc[s] += up[s];
+    }
+    if (l == kWave - 1) {
+#pragma unroll
+      for (u32 s = 0; s < 4; ++s) sh[s * kWavesPerBlock + w] = inc[s];
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 s = 0; s < 4; ++s) {
+      u64 base = 0, all = 0;
+#pragma unroll
+      for (u32 i = 0; i < (u32)kWavesPerBlock; ++i) {
+        const u64 y = sh[s * kWavesPerBlock + i];
+        if (i < w) base += y;
+        all += y;
+      }
+      ex[s] = base + inc[s] - mine[s];
+      tot[s] = all;
+    }
+  }
+  // each section's span: bit position in the output, words, place in the window
+  u64 abs0[4];
+  u32 nw[4], woff[4];
+  u32 wtot = 0;
+#pragma unroll
+  for (u32 s = 0; s < 4; ++s) {
+    const u32 kind = kKinds[s];
+    abs0[s] = tot[s] ? sp.sec_byte[kind] * 8 + (tile_off[kind * nt + blk] - sc.sec_bit0[kind]) : 0;
+    const u64 n = tot[s] ? ((abs0[s] & 31) + tot[s] + 31) >> 5 : 0;
+    nw[s] = n > kVlWin ? kVlWin + 1 : (u32)n;
+    woff[s] = wtot;
+    wtot = wtot + nw[s] > kVlWin + 1 ? kVlWin + 1 : wtot + nw[s];
+  }
+  const bool g = gr && mine[0], h = mine[1] != 0, qu = quad && mine[2], cn = rle && mine[3];
+  if (wtot > kVlWin) {
+    // spans too long for the window: runs straight into the output
+    // (one section at a time)
+    if (g) {
+      BitRun<false, false> run(out, abs0[0] + ex[0], mine[0]);
+      emit_gr(run, code_gr);
+    }
+    if (h) {
+      BitRun<true, false> run(out, abs0[1] + ex[1], mine[1]);
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) run.code(code_huf(q));
+      run.finish();
+    }
+    if (qu) {
+      BitRun<true, false> run(out, abs0[2] + ex[2], mine[2]);
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) run.code(code_quad(q));
+      run.finish();
+    }
+    if (cn) {
+      BitRun<true, false> run(out, abs0[3] + ex[3], mine[3]);
+#pragma unroll
+      for (u32 q = 0; q < kItems; ++q) run.code(code_rle(q));
+      run.finish();
+    }
+    return;
+  }
+  for (u32 i = threadIdx.x; i < wtot; i += kBlock) win[i] = 0;
+  __syncthreads();
+  {
+    BitRun<true, true> rh(win + woff[1], (abs0[1] & 31) + ex[1], mine[1]);
+    BitRun<true, true> rq(win + woff[2], (abs0[2] & 31) + ex[2], mine[2]);
+    BitRun<true, true> rc(win + woff[3], (abs0[3] & 31) + ex[3], mine[3]);
+    emit_cells(rh, rq, rc, h, qu, cn, code_huf, code_quad, code_rle);
+  }
+  if (g) {
+    BitRun<false, true> rg(win + woff[0], (abs0[0] & 31) + ex[0], mine[0]);
+    emit_gr(rg, code_gr);
+  }
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < wtot; i += kBlock) {
+    const u32 s = i >= woff[3] && nw[3] ? 3u : i >= woff[2] && nw[2] ? 2u : i >= woff[1] && nw[1] ? 1u : 0u;
+    const u32 li = i - woff[s];
+    u32 y = win[i];
+    if (s != 0) y = bswap32(y);  // the MSB-first sections
+    u32* dst = out + (abs0[s] >> 5) + li;
+    if (li == 0 || li == nw[s] - 1) {
+      if (y) atomicOr(dst, y);
+    } else {
+      *dst = y;
+    }
   }
 }
 
@@ -2098,12 +2576,20 @@ void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32
 }
 
 void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
-                    const SecPlan& p, u32* out_words, hipStream_t s) {
+                    const SecPlan& p, u32* out_words, bool any_block, bool any_direct,
+                    hipStream_t s) {
   const SecCtx x{KV{kv.a, kv.b, bt.wide}, cnt, bpos};
-  write_one<kSecBlk>(bt, x, p, out_words, s);
-  write_one<kSecCnt>(bt, x, p, out_words, s);
-  k_sec_write_cells<<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
-  LAUNCH_CHECK("k_sec_write<cells>");
+  if (any_block) {
+    write_one<kSecBlk>(bt, x, p, out_words, s);
+    write_one<kSecCnt>(bt, x, p, out_words, s);
+    if (bt.wide) k_sec_write_cells<false, true><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
+    else k_sec_write_cells<false, false><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
+    LAUNCH_CHECK("k_sec_write<cells>");
+  }
+  if (any_direct) {
+    k_sec_write_cells<true, false><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
+    LAUNCH_CHECK("k_sec_write<cells_direct>");
+  }
 }
 
 }  // namespace xnyz

@@ -254,7 +254,8 @@ void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32
                    const SecPlan& p, Arena& ar, hipStream_t s);
 // Phase B: write every section at SegParam::sec_byte.
 void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
-                    const SecPlan& p, u32* out_words, hipStream_t s);
+                    const SecPlan& p, u32* out_words, bool any_block, bool any_direct,
+                    hipStream_t s);
 
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s);
 
