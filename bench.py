@@ -241,6 +241,36 @@ def main() -> int:
           f"({dt / args.steps * 1e3:.1f} ms/step)", file=sys.stderr, flush=True)
     verified = verify_stream(args.workload, w, n, bytes(sink.buf[s_at:s_at + stream_len]))
     print(f"[bench] last timed stream: {verified}", file=sys.stderr, flush=True)
+    single = {"GBps": round(value, 3), "ms_per_step": round(dt / args.steps * 1e3, 3)}
+
+    # The same steps pipelined (xnyz_compress_batch_device_into_async): call
+    # k + 1's kernels run while call k's stream crosses PCIe into its pinned
+    # host buffer (two alternate); the clock stops when every stream of the
+    # K calls is in host memory.  Both buffers are checked afterwards.
+    pipelined = None
+    if not args.pageable_out:
+        pb = [X.host_buffer(stream_len + (1 << 20)) for _ in range(2)]
+        o_s = [dict(o1[0], frame_mode=stream_mode)]
+
+        def pstep(k):
+            return X.compress_batch_into(dev, [0, n], o_s, pb[k % 2], ctx=ctx, wait=False)
+        for k in range(max(2, args.warmup)):
+            pstep(k)
+        ctx.wait()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            res, _ = pstep(k)
+        ctx.wait()
+        torch.cuda.synchronize()
+        dt_p = time.perf_counter() - t0
+        ok = [verify_stream(args.workload, w, n, bytes(b[:stream_len])) for b in pb]
+        pipelined = {"GBps": round(12.0 * n * args.steps / dt_p / 1e9, 3),
+                     "ms_per_step": round(dt_p / args.steps * 1e3, 3),
+                     "verified": ok[0] if ok[0] == ok[1] else f"{ok[0]} / {ok[1]}"}
+        print(f"[bench] pipelined calls: {pipelined['GBps']:.2f} GB/s "
+              f"({pipelined['ms_per_step']:.1f} ms/step)", file=sys.stderr, flush=True)
+        del pb
 
     # The zstd frame stage over the same number of steps: level 3 split over
     # the device (head blocks) and host worker threads (tail chunks) -- one
@@ -319,14 +349,19 @@ def main() -> int:
 
     dev_ms = {k: round(v["ms"] / args.steps, 3) for k, v in
               sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
+    # The headline is the pipelined rate when its streams check out (K whole
+    # compressions, each stream in host memory when the clock stops), else
+    # the rate of calls that each wait for their own copy.
+    use_pipe = pipelined is not None and pipelined["verified"] == verified
+    head = pipelined if use_pipe else single
     out = {
         "metric": "compress GB/s (fp32 xyz in) at fixed L2 bound",
-        "value": round(value, 3),
+        "value": head["GBps"],
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -339,9 +374,15 @@ def main() -> int:
                    "output": "assembled stream in pinned host memory, byte-identical to the "
                              "reference's pre-zstd buffer; zstd frame rates in 'framed' and "
                              "'exact_zstd_GBps'",
+                   "calls": ("pipelined: call k+1's kernels overlap call k's copy to the host "
+                             "(xnyz_compress_batch_device_into_async, two pinned outputs); "
+                             "calls that wait for their own copy in 'single_call'")
+                            if use_pipe else "each call waits for its own copy",
                    "tuning": {k: (round(v, 4) if isinstance(v, float) else v)
                               for k, v in tuning.items()}},
         "verified": verified,
+        "single_call": single,
+        "pipelined": pipelined,
         "stream_bytes": stream_len,
         "stream_ratio": round(12.0 * n / stream_len, 4),
         "framed": framed,

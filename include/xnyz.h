@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define XNYZ_ABI_VERSION 5
+#define XNYZ_ABI_VERSION 6
 
 enum xnyz_status {
   XNYZ_OK = 0,
@@ -192,6 +192,27 @@ int xnyz_compress_batch_device(xnyz_ctx* ctx, const float* d_xyz, const uint64_t
 int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets,
                                     uint32_t count, const xnyz_opts* opts, uint8_t* out,
                                     uint64_t cap, uint64_t* out_offsets, xnyz_stats* stats);
+
+/* xnyz_compress_batch_device_into returning as soon as the assembled
+ * streams are in device memory and their copy into `out` is queued on the
+ * context's copy stream: the next call's kernels run while this call's
+ * output crosses PCIe.  No reference counterpart beyond compress(); it is
+ * for ingest and serving loops that compress one cloud or batch after
+ * another.  opts[s].frame_mode must be XNYZ_FRAME_NONE and `out` pinned
+ * (xnyz_host_alloc), else XNYZ_EINVAL.  out_offsets and stats are final
+ * when the call returns; the bytes in `out` only after xnyz_wait(ctx) (or
+ * xnyz_destroy).  The context keeps two stream buffers: a call whose
+ * buffer's previous copy is still running waits for it first, so at most
+ * two copies are in flight.  Copies run in call order; the caller must not
+ * read, free or reuse `out` of a call before xnyz_wait. */
+int xnyz_compress_batch_device_into_async(xnyz_ctx* ctx, const float* d_xyz,
+                                          const uint64_t* offsets, uint32_t count,
+                                          const xnyz_opts* opts, uint8_t* out, uint64_t cap,
+                                          uint64_t* out_offsets, xnyz_stats* stats);
+
+/* Waits until every copy queued by xnyz_compress_batch_device_into_async on
+ * this context has landed in host memory. */
+int xnyz_wait(xnyz_ctx* ctx);
 
 /* xnyz_compress_batch_device_into without allocating: no hipMalloc,
  * hipFree, hipHostMalloc or graph instantiation during the call.  No

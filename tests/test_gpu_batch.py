@@ -122,6 +122,64 @@ def test_batch_into_caller_buffer(X):
         assert "%d bytes needed" % res[-1] in str(e.value)
 
 
+def test_pinned_streams_and_async_calls(X):
+    """Assembled streams into pinned memory: a batch is packed on the device
+    (segments at byte offsets of any alignment) and copied once.  The
+    asynchronous form returns before the copy lands and gives the same bytes
+    after ctx.wait(): calls on both of the context's stream buffers in
+    flight, a synchronous call between them, one output buffer reused (the
+    later call's bytes land last)."""
+    from xnyzip_amd import _lib
+    rng = np.random.default_rng(11)
+    small = [(_uniform(200 + i, int(rng.integers(1, 2000))),
+              dict(quantizer=int(i % 2), l2_bound=0.01, is_hilbert=i % 3 == 0, rle=i % 2 == 1))
+             for i in range(90)]
+    batches = [_clouds(), small, _clouds()[:1]]
+    ctx = X.Context(0)
+    ref = []
+    for b in batches:
+        pts = torch.from_numpy(np.concatenate([c for c, _ in b])).cuda()
+        offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in b])]).tolist()
+        opts = [dict(o, frame_mode=_lib.FRAME_NONE) for _, o in b]
+        want = [O.compress(c, o["quantizer"], o["l2_bound"], o["is_hilbert"], o["rle"],
+                           o.get("direct_threshold", 1024)).pre for c, o in b]
+        ref.append((pts, offs, opts, want))
+
+    def check(buf, res, want):
+        assert len(res) == len(want) + 1
+        for i, w in enumerate(want):
+            assert bytes(buf[int(res[i]):int(res[i + 1])]) == w, i
+
+    for pts, offs, opts, want in ref:  # synchronous
+        buf = X.host_buffer(sum(map(len, want)) + 64)
+        res, _ = X.compress_batch_into(pts, offs, opts, buf, ctx=ctx)
+        check(buf, res, want)
+    cap = max(sum(map(len, w)) for _, _, _, w in ref) + 64
+    bufs = [X.host_buffer(cap) for _ in range(3)]
+    got = []
+    for k, (bi, bb) in enumerate([(0, 0), (1, 1), (2, 2), (1, None), (2, 0)]):
+        pts, offs, opts, want = ref[bi]
+        if bb is None:
+            plain = np.zeros(cap, np.uint8)
+            res, _ = X.compress_batch_into(pts, offs, opts, plain, ctx=ctx)
+            check(plain, res, want)
+            continue
+        res, _ = X.compress_batch_into(pts, offs, opts, bufs[bb], ctx=ctx, wait=False)
+        got.append((bb, res.copy(), want))
+    ctx.wait()
+    for bb, res, want in got[1:]:  # buffer 0's first call was overwritten
+        check(bufs[bb], res, want)
+    pts, offs, opts, _ = ref[0]
+    with pytest.raises(_lib.XnyzError) as e:
+        X.compress_batch_into(pts, offs, opts, np.zeros(cap, np.uint8), ctx=ctx, wait=False)
+    assert e.value.code == _lib.XNYZ_EINVAL
+    with pytest.raises(_lib.XnyzError) as e:
+        X.compress_batch_into(pts, offs, [dict(o, frame_mode=_lib.FRAME_ZSTD3) for o in opts], bufs[0],
+                              ctx=ctx, wait=False)
+    assert e.value.code == _lib.XNYZ_EINVAL
+    ctx.close()
+
+
 def test_graph_replay_matches_eager(X):
     """Stage 2 replayed from a captured hipGraph gives the eager bytes on
     every replay, with other work (another launch shape, eager calls and

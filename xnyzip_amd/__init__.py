@@ -87,7 +87,7 @@ def compress(points, quantizer=QUANTIZER_TYPE.TRUNCATED_OCTAHEDRON, l2_bound=0.0
     if dev_ptr is not None:
         if want_perm and not rle:
             perm = np.empty(n, dtype=np.uint32)
-        torch.cuda.synchronize(keep.device)  # the library works on its own stream
+        _inputs_ready(keep.device)  # the library works on its own stream
         rc = L.xnyz_compress_device(ctx.handle, C.c_void_p(dev_ptr), n, C.byref(o), C.byref(fr),
                                     C.byref(pre) if want_prezstd else None,
                                     C.c_void_p(perm.ctypes.data) if perm is not None else None,
@@ -134,7 +134,7 @@ def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
     frames = (_lib.Buf * count)()
     pres = (_lib.Buf * count)() if want_prezstd else None
     stats = (_lib.Stats * count)()
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_compress_batch_device(
         ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
         C.cast(arr, C.c_void_p), C.cast(frames, C.c_void_p),
@@ -144,15 +144,27 @@ def compress_batch(points_device, offsets, opts, want_prezstd: bool = False,
     return out_frames, out_pres, list(stats)
 
 
+def _inputs_ready(device) -> None:
+    """Waits for torch's current stream, which produced the inputs: the
+    library reads them on its own stream.  Not a device-wide synchronise, which
+    would also wait for the library's copies still in flight
+    (compress_batch_into with wait=False)."""
+    import torch
+    torch.cuda.current_stream(device).synchronize()
+
+
 def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
-                        ctx: Context | None = None):
+                        ctx: Context | None = None, wait: bool = True):
     """compress_batch writing every cloud's output back to back into `out`
     (a uint8 numpy array, reused across calls).  Returns (offsets, stats):
     cloud s's output is out[offsets[s]:offsets[s + 1]]; offsets[0] is 0
     except for a split frame with room for its anchored layout (the host's
     blocks placed while compressed, the device's part in front of them).
     Raises XnyzError with code XNYZ_ENOSPC if `out` is too small
-    (offsets[-1] then holds the size needed)."""
+    (offsets[-1] then holds the size needed).  wait=False
+    (xnyz_compress_batch_device_into_async: assembled streams into a pinned
+    `out` from host_buffer) returns once the copy is queued; the bytes are
+    in `out` after ctx.wait()."""
     import torch
     t = points_device.reshape(-1, 3)
     if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
@@ -172,8 +184,10 @@ def compress_batch_into(points_device, offsets, opts, out: np.ndarray,
                        o.get("gr_k"), o.get("pfd_width"))
     res = np.zeros(count + 1, dtype=np.uint64)
     stats = (_lib.Stats * count)()
-    torch.cuda.synchronize(t.device)
-    rc = _lib.lib().xnyz_compress_batch_device_into(
+    _inputs_ready(t.device)
+    fn = (_lib.lib().xnyz_compress_batch_device_into if wait
+          else _lib.lib().xnyz_compress_batch_device_into_async)
+    rc = fn(
         ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
         C.cast(arr, C.c_void_p), C.c_void_p(out.ctypes.data), out.size,
         C.c_void_p(res.ctypes.data), C.cast(stats, C.c_void_p))
@@ -243,7 +257,7 @@ def autotune(points_device, opts: dict, reps: int = 2, ctx: Context | None = Non
               opts.get("gr_k"), opts.get("pfd_width"))
     best = _lib.Tuning()
     ms = C.c_double()
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_autotune(ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0],
                                         C.byref(o), int(reps), C.byref(best), C.byref(ms)))
     return best.as_dict(), ms.value
@@ -269,7 +283,7 @@ def search_batch(points_device, offsets, opts, ctx: Context | None = None):
         arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
                        o.get("rle", False), o.get("direct_threshold", 1024), _lib.FRAME_NONE, 0)
     res = (_lib.SearchResult * count)()
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_search_batch_device(
         ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
         C.cast(arr, C.c_void_p), C.cast(res, C.c_void_p)))
@@ -296,7 +310,7 @@ def _batch_args(points_device, offsets, opts, frame_default=_lib.FRAME_NONE):
         arr[i] = _opts(o["quantizer"], o["l2_bound"], o.get("is_hilbert", False),
                        o.get("rle", False), o.get("direct_threshold", 1024),
                        o.get("frame_mode", frame_default), o.get("zstd_workers", 0))
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     return t, count, off, arr
 
 
@@ -466,7 +480,7 @@ def decompress_to_device(data: bytes, quantizer, l2_bound: float, rle: bool, out
     src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
     n = C.c_uint64()
     ctx = ctx or default_context(t.device.index or 0)
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_decompress_to_device(
         ctx.handle, C.c_void_p(src.ctypes.data), len(data), C.byref(o),
         C.c_void_p(t.data_ptr()), t.shape[0], C.byref(n)))
@@ -494,7 +508,7 @@ def fit_profile(points_device, quantizer, l2_bound, is_hilbert=False,
     ctx = ctx or default_context(t.device.index or 0)
     o = _opts(quantizer, l2_bound, is_hilbert, False, 1024, frame_mode, 0)
     fr, pre, prof = _lib.Buf(), _lib.Buf(), _lib.Buf()
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_fit_profile_device(ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0],
                                                    C.byref(o), C.byref(fr),
                                                    C.byref(pre) if want_prezstd else None,
@@ -531,7 +545,7 @@ def compress_global_rle(points_device, quantizer, l2_bound, is_hilbert=False,
     ctx = ctx or default_context(t.device.index or 0)
     o = _opts(quantizer, l2_bound, is_hilbert, True, 1024, frame_mode, 0)
     fr, pre, st = _lib.Buf(), _lib.Buf(), _lib.Stats()
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_compress_global_rle_device(
         ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0], C.byref(o), C.byref(fr),
         C.byref(pre) if want_prezstd else None, C.byref(st)))
@@ -614,7 +628,7 @@ def quantize(points_device, quantizer, l2_bound, ctx: Context | None = None):
     off = np.empty(3, dtype=np.float32)
     ctx = ctx or default_context(t.device.index or 0)
     import torch
-    torch.cuda.synchronize(t.device)
+    _inputs_ready(t.device)
     _lib.check(_lib.lib().xnyz_quantize_device(ctx.handle, C.c_void_p(t.data_ptr()), n,
                                                 int(quantizer), float(l2_bound),
                                                 C.c_void_p(nodes.ctypes.data),

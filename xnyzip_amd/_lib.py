@@ -111,7 +111,8 @@ _lock = threading.Lock()
 
 EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_compress", "xnyz_compress_device", "xnyz_compress_batch_device",
-           "xnyz_compress_batch_device_into", "xnyz_compress_batch_device_noalloc",
+           "xnyz_compress_batch_device_into", "xnyz_compress_batch_device_into_async",
+           "xnyz_wait", "xnyz_compress_batch_device_noalloc",
            "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_compress_selected_batch_device_into",
@@ -146,6 +147,9 @@ def lib() -> C.CDLL:
         L.xnyz_compress_batch_device.argtypes = [P, P, P, C.c_uint32, P, P, P, P]
         L.xnyz_compress_batch_device_into.argtypes = [P, P, P, C.c_uint32, P, P, C.c_uint64, P,
                                                       P]
+        L.xnyz_compress_batch_device_into_async.argtypes = [P, P, P, C.c_uint32, P, P,
+                                                            C.c_uint64, P, P]
+        L.xnyz_wait.argtypes = [P]
         L.xnyz_compress_batch_device_noalloc.argtypes = [P, P, P, C.c_uint32, P, P, C.c_uint64,
                                                          P, P]
         L.xnyz_search_batch_device.argtypes = [P, P, P, C.c_uint32, P, P]
@@ -226,6 +230,10 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+    def wait(self) -> None:
+        """Waits for the copies queued by compress_batch_into(..., wait=False)."""
+        check(lib().xnyz_wait(self._h))
 
     def graphs(self, on: bool) -> None:
         """Replay the stage-2 launch sequence from a captured hipGraph (default on)."""

@@ -359,6 +359,14 @@ struct xnyz_ctx {
   DBuf feat;             // selector statistics
   DBuf patch;            // stream header / meta bytes for k_patch
   DBuf ztab, zscr, zout, zpos;  // device-built frames
+  // Packed outputs of a batch (k_gather) and its segment table; the
+  // asynchronous form keeps two stream buffers and two packed buffers, one
+  // of each possibly still being copied to the host.
+  DBuf pk, gtab, aout[2], apk[2], agtab[2];
+  HBuf h_gtab;
+  hipEvent_t aev[2] = {nullptr, nullptr};  // copy of slot i done
+  bool abusy[2] = {false, false};
+  int aslot = 0;
   // device decoder: stream, symbols, quads, block flags, patches and their
   // prefix sums, block arrays, RLE counts and offsets, points, tables, scratch
   DBuf dstream, dsym, dquad, dhead, dpatch, dps, dblk, dcnt, dpts, dtab, darena;
@@ -387,11 +395,12 @@ struct xnyz_ctx {
             &lo[0], &lo[1], &cells, &cells_lo, &pos, &cnt, &out, &arena, &upload, &tmp,
             &idx[0], &idx[1], &lo64[0], &lo64[1], &feat, &patch, &ztab, &zscr, &zout, &zpos,
             &mP, &mWh, &mG, &mM, &mV, &mX, &mY, &mIdx, &mScratch, &mLoss, &mOut, &mConf,
-            &dstream, &dsym, &dquad, &dhead, &dpatch, &dps, &dblk, &dcnt, &dpts, &dtab, &darena};
+            &dstream, &dsym, &dquad, &dhead, &dpatch, &dps, &dblk, &dcnt, &dpts, &dtab, &darena,
+            &pk, &gtab, &aout[0], &aout[1], &apk[0], &apk[1], &agtab[0], &agtab[1]};
   }
   std::vector<HBuf*> hbufs() {
     return {&h_tiles, &h_sp, &h_sc, &h_sh, &h_lut, &h_out, &h_feat, &h_zout, &h_ztab, &h_zpos,
-            &h_dflag};
+            &h_dflag, &h_gtab};
   }
 
   // Every buffer's member name, for allocation errors.
@@ -411,6 +420,8 @@ struct xnyz_ctx {
     XNYZ_NAME(na_host); XNYZ_NAME(dstream); XNYZ_NAME(dsym); XNYZ_NAME(dquad);
     XNYZ_NAME(dhead); XNYZ_NAME(dpatch); XNYZ_NAME(dps); XNYZ_NAME(dblk); XNYZ_NAME(dcnt);
     XNYZ_NAME(dpts); XNYZ_NAME(dtab); XNYZ_NAME(darena); XNYZ_NAME(h_dflag);
+    XNYZ_NAME(pk); XNYZ_NAME(gtab); XNYZ_NAME(aout[0]); XNYZ_NAME(aout[1]); XNYZ_NAME(apk[0]);
+    XNYZ_NAME(apk[1]); XNYZ_NAME(agtab[0]); XNYZ_NAME(agtab[1]); XNYZ_NAME(h_gtab);
 #undef XNYZ_NAME
   }
 
@@ -835,6 +846,10 @@ struct CallOut {
   // mode always, the largest quantized coordinate as the range of all three
   // axes.
   bool global_rle = false;
+  // Assembled streams into pinned caller memory, returning once the copy is
+  // queued on the copy stream (xnyz_wait ends it); the stream buffer
+  // alternates between two slots, a slot's previous copy waited for first.
+  bool async_out = false;
   const std::vector<uint8_t>* profile_metas = nullptr;
   std::vector<uint8_t>* profile_out = nullptr;
   // No-allocation call: every buffer comes from the fixed regions (g_bump),
@@ -1372,8 +1387,25 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     }
     put(at, h.off, 12);
   }
-  ctx->out.ensure(out_total + 64);
-  u32* ow = ctx->out.as<u32>();
+  // Assembled streams into caller memory that the DMA engine can write
+  // (pinned, or any memory for a no-allocation call) leave straight from
+  // device memory: one copy for one cloud; a batch is packed on the device
+  // first (k_gather), then copied at once.
+  bool all_none = true;
+  for (u32 s = 0; s < S; ++s) all_none &= opts[s].frame_mode == XNYZ_FRAME_NONE;
+  const bool direct_out =
+      co.noalloc || (all_none && co.dst && !co.pre && co.cap && pinned_range(co.dst, co.cap));
+  if (co.async_out && (!direct_out || co.noalloc))
+    throw Error(XNYZ_EINVAL, "asynchronous calls return assembled streams (XNYZ_FRAME_NONE) "
+                             "into pinned memory (xnyz_host_alloc)");
+  const int slot = ctx->aslot;
+  if (co.async_out && ctx->abusy[slot]) {  // this slot's previous copy must have ended
+    check_hip(hipEventSynchronize(ctx->aev[slot]), "hipEventSynchronize");
+    ctx->abusy[slot] = false;
+  }
+  DBuf& obuf = co.async_out ? ctx->aout[slot] : ctx->out;
+  obuf.ensure(out_total + 64);
+  u32* ow = obuf.as<u32>();
   launch_fill_u32(ow, (out_total + 64) / 4, 0, st);
   const size_t prec = patches.size() * sizeof(Patch);
   ctx->patch.ensure(prec + pbytes.size() + 16);
@@ -1419,24 +1451,61 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     }
   };
 
-  // Assembled streams into caller memory that the DMA engine can write
-  // (pinned, or any memory for a no-allocation call): each stream is copied
-  // straight from device memory to its place, with no staging buffer (one
-  // copy per cloud, so only for batches of a few clouds; larger batches take
-  // the staged path below).
-  bool all_none = true;
-  for (u32 s = 0; s < S; ++s) all_none &= opts[s].frame_mode == XNYZ_FRAME_NONE;
-  if (co.noalloc || (all_none && S <= 64 && co.dst && !co.pre && co.cap &&
-                     pinned_range(co.dst, co.cap))) {
+  if (direct_out) {
     co.dst_off[0] = 0;
     for (u32 s = 0; s < S; ++s) co.dst_off[s + 1] = co.dst_off[s] + g[s].total;
     if (co.dst_off[S] > co.cap)
       throw Error(XNYZ_ENOSPC, "output buffer too small: " + std::to_string(co.dst_off[S]) +
                                    " bytes needed");
-    for (u32 s = 0; s < S; ++s)
-      check_hip(hipMemcpyAsync(co.dst + co.dst_off[s],
-                               reinterpret_cast<const uint8_t*>(ow) + g[s].base, g[s].total,
-                               hipMemcpyDeviceToHost, st), "stream D2H");
+    if (co.noalloc) {  // one copy per cloud: no packed buffer to carve
+      for (u32 s = 0; s < S; ++s)
+        check_hip(hipMemcpyAsync(co.dst + co.dst_off[s],
+                                 reinterpret_cast<const uint8_t*>(ow) + g[s].base, g[s].total,
+                                 hipMemcpyDeviceToHost, st), "stream D2H");
+      ctx->copied();
+      ctx->sync();
+      const double t = now_ms();
+      fill_stats(t, t);
+      return XNYZ_OK;
+    }
+    const u64 bytes = co.dst_off[S];
+    const uint8_t* src = reinterpret_cast<const uint8_t*>(ow) + g[0].base;
+    if (S > 1) {
+      DBuf& pkb = co.async_out ? ctx->apk[slot] : ctx->pk;
+      DBuf& tb = co.async_out ? ctx->agtab[slot] : ctx->gtab;
+      const size_t tbytes = S * sizeof(GatherSeg);
+      ctx->h_gtab.ensure(tbytes);
+      GatherSeg* gt = ctx->h_gtab.as<GatherSeg>();
+      for (u32 s = 0; s < S; ++s) gt[s] = GatherSeg{g[s].base, co.dst_off[s], g[s].total};
+      tb.ensure(tbytes);
+      pkb.ensure(bytes + 64);
+      check_hip(hipMemcpyAsync(tb.p, gt, tbytes, hipMemcpyHostToDevice, st), "H2D");
+      ctx->copied();
+      launch_gather(reinterpret_cast<const uint8_t*>(ow), tb.as<GatherSeg>(), S, bytes,
+                    pkb.as<uint8_t>(), st);
+      src = pkb.as<uint8_t>();
+    }
+    if (co.async_out) {
+      if (!ctx->copy_stream)
+        check_hip(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking),
+                  "hipStreamCreate");
+      if (!ctx->ev_pipe)
+        check_hip(hipEventCreateWithFlags(&ctx->ev_pipe, hipEventDisableTiming), "hipEventCreate");
+      if (!ctx->aev[slot])
+        check_hip(hipEventCreateWithFlags(&ctx->aev[slot], hipEventDisableTiming),
+                  "hipEventCreate");
+      check_hip(hipEventRecord(ctx->ev_pipe, st), "hipEventRecord");
+      check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->ev_pipe, 0), "hipStreamWaitEvent");
+      check_hip(hipMemcpyAsync(co.dst, src, bytes, hipMemcpyDeviceToHost, ctx->copy_stream),
+                "stream D2H");
+      check_hip(hipEventRecord(ctx->aev[slot], ctx->copy_stream), "hipEventRecord");
+      ctx->abusy[slot] = true;
+      ctx->aslot = slot ^ 1;
+      const double t = now_ms();
+      fill_stats(t, t);
+      return XNYZ_OK;
+    }
+    check_hip(hipMemcpyAsync(co.dst, src, bytes, hipMemcpyDeviceToHost, st), "stream D2H");
     ctx->copied();
     ctx->sync();
     const double t = now_ms();
@@ -2214,6 +2283,7 @@ int xnyz_destroy(xnyz_ctx* ctx) {
   return guarded([&] {
     use_device(ctx);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);  // pending copies
     for (DBuf* b : ctx->dbufs()) b->release();
     for (HBuf* b : ctx->hbufs()) b->release();
     ctx->na_host.release();
@@ -2222,10 +2292,9 @@ int xnyz_destroy(xnyz_ctx* ctx) {
     for (hipEvent_t e : ctx->slice_ev) (void)hipEventDestroy(e);
     if (ctx->ev_pipe) (void)hipEventDestroy(ctx->ev_pipe);
     if (ctx->ev_sizes) (void)hipEventDestroy(ctx->ev_sizes);
-    if (ctx->copy_stream) {
-      (void)hipStreamSynchronize(ctx->copy_stream);
-      (void)hipStreamDestroy(ctx->copy_stream);
-    }
+    for (hipEvent_t e : ctx->aev)
+      if (e) (void)hipEventDestroy(e);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx->fs;
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2293,6 +2362,35 @@ int xnyz_compress_batch_device_into(xnyz_ctx* ctx, const float* d_xyz, const uin
     co.cap = cap;
     co.dst_off = out_offsets;
     return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+  });
+}
+
+int xnyz_compress_batch_device_into_async(xnyz_ctx* ctx, const float* d_xyz,
+                                          const uint64_t* offsets, uint32_t count,
+                                          const xnyz_opts* opts, uint8_t* out, uint64_t cap,
+                                          uint64_t* out_offsets, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !out_offsets || !out || !cap)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    use_device(ctx);
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    co.async_out = true;
+    return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+  });
+}
+
+int xnyz_wait(xnyz_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) throw Error(XNYZ_EINVAL, "null context");
+    use_device(ctx);
+    if (ctx->copy_stream)
+      check_hip(hipStreamSynchronize(ctx->copy_stream), "hipStreamSynchronize");
+    ctx->abusy[0] = ctx->abusy[1] = false;
+    return XNYZ_OK;
   });
 }
 

@@ -2413,6 +2413,42 @@ void write_one(const Batch& bt, const SecCtx& x, const SecPlan& p, u32* out, hip
 
 }  // namespace
 
+// Packs the segments' outputs back to back: segment s's `len` bytes at
+// src + src_off (256-byte aligned) go to dst + dst_off (any alignment).  A
+// workgroup copies one of `slices` parts of a segment's whole destination
+// words, reading each as a funnel shift of two aligned source words; slice 0
+// also writes the partial words at the segment's two ends byte by byte (the
+// neighbouring segments own their other bytes).
+__global__ __launch_bounds__(kBlock) void k_gather(const uint8_t* __restrict__ src,
+                                                   const GatherSeg* __restrict__ segs,
+                                                   u32 slices, uint8_t* __restrict__ dst) {
+  const GatherSeg g = segs[blockIdx.x / slices];
+  const u32 k = blockIdx.x % slices;
+  if (g.len == 0) return;
+  const u64 d0 = g.dst_off, d1 = g.dst_off + g.len;
+  const u64 w0 = (d0 + 3) >> 2, w1 = d1 >> 2;  // whole destination words [w0, w1)
+  if (k == 0) {
+    const u64 head_end = w0 < w1 ? (w0 << 2) : d1;
+    for (u64 b = d0 + threadIdx.x; b < head_end; b += kBlock) dst[b] = src[g.src_off + (b - d0)];
+    if (w0 < w1)
+      for (u64 b = (w1 << 2) + threadIdx.x; b < d1; b += kBlock) dst[b] = src[g.src_off + (b - d0)];
+  }
+  if (w0 >= w1) return;
+  const u64 nw = w1 - w0;
+  const u64 a = w0 + nw * k / slices, e = w0 + nw * (k + 1) / slices;
+  // destination word w holds source bytes from s = src_off + 4w - d0; with
+  // sh = s & 3 it is bytes [sh, sh + 4) of the aligned pair at s - sh
+  const u64 s0 = g.src_off + (w0 << 2) - d0;
+  const u32 sh = (u32)(s0 & 3);
+  const u32* sw = reinterpret_cast<const u32*>(src + (s0 - sh));
+  u32* dw = reinterpret_cast<u32*>(dst) + w0;
+  for (u64 w = a + threadIdx.x; w < e; w += kBlock) {
+    const u64 i = w - w0;
+    const u32 lo = sw[i];
+    dw[i] = sh ? __builtin_amdgcn_alignbyte(sw[i + 1], lo, sh) : lo;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 
@@ -2420,6 +2456,16 @@ void launch_init(const Batch& bt, hipStream_t s) {
   const u64 words = (u64)bt.S * (sizeof(SegHist) / 4);
   k_init<<<grid_for(words, kBlock * 4, 16384), kBlock, 0, s>>>(bt);
   LAUNCH_CHECK("k_init");
+}
+
+void launch_gather(const uint8_t* src, const GatherSeg* segs, u32 count, u64 bytes,
+                   uint8_t* dst, hipStream_t s) {
+  if (!count || !bytes) return;
+  // about 64 KiB per workgroup, at least one slice per segment
+  u64 slices = (bytes / count) >> 16;
+  slices = slices < 1 ? 1 : slices > 1024 ? 1024 : slices;
+  k_gather<<<(u32)(count * slices), kBlock, 0, s>>>(src, segs, (u32)slices, dst);
+  LAUNCH_CHECK("k_gather");
 }
 
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s) {

@@ -259,6 +259,15 @@ void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u3
 
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s);
 
+// Segment s of a gather: `len` bytes from src + src_off to dst + dst_off.
+struct GatherSeg {
+  u64 src_off, dst_off, len;
+};
+// Packs `count` segments (their source offsets 256-byte aligned, the source
+// readable 8 bytes past each segment's end); `bytes` is their total.
+void launch_gather(const uint8_t* src, const GatherSeg* segs, u32 count, u64 bytes,
+                   uint8_t* dst, hipStream_t s);
+
 // Byte runs copied into the stream buffer: out[dst, dst + len) =
 // bytes[src, src + len) per record.
 struct Patch {
