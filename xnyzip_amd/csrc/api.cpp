@@ -1060,6 +1060,11 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     hi_bits = std::max(hi_bits, h.hi_bits);
   }
   bt.wide = wide;
+  // Passes each segment's keys need: the batch sorts for its widest keys, and
+  // a segment with narrower ones keeps its keys in place after its own.
+  for (u32 s = 0; s < S; ++s)
+    sp[s].passes = wide ? radix_passes(0, g[s].hi_bits, true, rshape)
+                        : radix_passes(g[s].key_bits, 0, false, rshape);
   ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
 
   // ---- stage 2: keys, sort, cells, runs, statistics ---------------------------

@@ -796,18 +796,34 @@ __device__ __forceinline__ u64 count_index(const SegParam& sp, u32 local, u32 su
   return (u64)D * nsub * sp.tile0 + (u64)d * sp.ntiles * nsub + (u64)local * nsub + sub;
 }
 
+// Whether pass `pass` moves this segment's keys: a segment whose keys need
+// fewer passes than the batch's keeps them in place from then on (its digits
+// there are all zero); if the passes it skips are odd in number, the batch's
+// last pass copies them across so that they end in the same buffer as the
+// rest.
+__device__ __forceinline__ bool pass_active(const SegParam& sp, u32 pass) {
+  return sp.passes == 0 || pass < sp.passes;
+}
+
 template <typename K, u32 kBits, u32 kSub>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(Batch bt, const K* __restrict__ keys,
-                                                       u32 shift, u32* __restrict__ counts) {
+                                                       u32 shift, u32 pass,
+                                                       u32* __restrict__ counts) {
   using RS = RadixShapeT<kBits, kSub>;
   __shared__ u32 h[kWavesPerBlock][RS::D];
-  for (u32 i = threadIdx.x; i < kWavesPerBlock * RS::D; i += kBlock) (&h[0][0])[i] = 0;
-  __syncthreads();
   const u32 tix = blockIdx.x / kSub, sub = blockIdx.x % kSub;
   const Tile t = bt.tiles[tix];
   const SegParam& sp = bt.sp[t.seg];
   const Range r = tile_range(sp.p0, sp.n, t.local);
   const u32 base = sub * RS::kRT;
+  if (!pass_active(sp, pass)) {  // uniform: every key counts as digit 0
+    const u32 n = r.count > base ? (r.count - base < RS::kRT ? r.count - base : RS::kRT) : 0u;
+    if (threadIdx.x < RS::D)
+      counts[count_index(sp, t.local, sub, threadIdx.x, RS::D, kSub)] = threadIdx.x ? 0u : n;
+    return;
+  }
+  for (u32 i = threadIdx.x; i < kWavesPerBlock * RS::D; i += kBlock) (&h[0][0])[i] = 0;
+  __syncthreads();
   const u32 w = threadIdx.x / kWave;
   if (r.count > base) {  // uniform; an empty radix tile still writes its zero counts
     // every key in flight before the first count: loads clamped to the
@@ -839,6 +855,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
                                                           K* __restrict__ kout,
                                                           const V* __restrict__ vin,
                                                           V* __restrict__ vout, u32 shift,
+                                                          u32 pass, u32 npasses,
                                                           const u32* __restrict__ offs) {
   using RS = RadixShapeT<kBits, kSub>;
   constexpr u32 D = RS::D, kIt = RS::kIt;
@@ -863,6 +880,15 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
   const u32 wstart = w * RS::kWSub;
   // The radix tile's global digit offsets are strided in memory; start that
   // load now so its latency overlaps the key loads and the ranking.
+  if (!pass_active(sp, pass)) {  // uniform: keys stay, or cross over in the last pass
+    if (pass + 1 == npasses && ((npasses - sp.passes) & 1u)) {
+      for (u32 i = threadIdx.x; i < count; i += kBlock) {
+        kout[first + i] = kin[first + i];
+        if (kHasV) vout[first + i] = vin[first + i];
+      }
+    }
+    return;
+  }
   const u32 my_off = threadIdx.x < D ? offs[count_index(sp, t.local, sub, threadIdx.x, D, kSub)] : 0u;
   if (count == 0) return;  // uniform: an empty radix tile moves nothing
   K k[kIt];
@@ -958,15 +984,15 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
 
 template <typename K, typename V, bool kHasV, u32 kBits, u32 kSub>
 void radix_pass_t(const Batch& bt, const K* kin, K* kout, const V* vin, V* vout, u32 shift,
-                  Arena& ar, hipStream_t s) {
+                  u32 pass, u32 npasses, Arena& ar, hipStream_t s) {
   const size_t mark = ar.off;
   const u64 ncounts = (u64)(1u << kBits) * bt.ntiles * kSub;
   u32* counts = ar.take<u32>(ncounts);
-  k_radix_hist<K, kBits, kSub><<<bt.ntiles * kSub, kBlock, 0, s>>>(bt, kin, shift, counts);
+  k_radix_hist<K, kBits, kSub><<<bt.ntiles * kSub, kBlock, 0, s>>>(bt, kin, shift, pass, counts);
   LAUNCH_CHECK("k_radix_hist");
   scan_excl<u32>(counts, counts, ncounts, nullptr, ar, s);
-  k_radix_scatter<K, V, kHasV, kBits, kSub>
-      <<<bt.ntiles * kSub, kBlock, 0, s>>>(bt, kin, kout, vin, vout, shift, counts);
+  k_radix_scatter<K, V, kHasV, kBits, kSub><<<bt.ntiles * kSub, kBlock, 0, s>>>(
+      bt, kin, kout, vin, vout, shift, pass, npasses, counts);
   LAUNCH_CHECK("k_radix_scatter");
   ar.off = mark;
 }
@@ -975,13 +1001,15 @@ void radix_pass_t(const Batch& bt, const K* kin, K* kout, const V* vin, V* vout,
 // bit are zero).
 template <typename K, typename V, bool kHasV>
 void radix_pass(const Batch& bt, const K* kin, K* kout, const V* vin, V* vout, u32 shift,
-                RadixShape shape, Arena& ar, hipStream_t s) {
+                RadixShape shape, u32 pass, u32 npasses, Arena& ar, hipStream_t s) {
   const bool half = shape.sub == 2;
-#define XNYZ_RADIX_CASE(B)                                                                 \
-  if (shape.bits == B) {                                                                   \
-    if (half) radix_pass_t<K, V, kHasV, B, 2>(bt, kin, kout, vin, vout, shift, ar, s);     \
-    else radix_pass_t<K, V, kHasV, B, 1>(bt, kin, kout, vin, vout, shift, ar, s);          \
-    return;                                                                                \
+#define XNYZ_RADIX_CASE(B)                                                                    \
+  if (shape.bits == B) {                                                                      \
+    if (half)                                                                                 \
+      radix_pass_t<K, V, kHasV, B, 2>(bt, kin, kout, vin, vout, shift, pass, npasses, ar, s); \
+    else                                                                                      \
+      radix_pass_t<K, V, kHasV, B, 1>(bt, kin, kout, vin, vout, shift, pass, npasses, ar, s); \
+    return;                                                                                   \
   }
   XNYZ_RADIX_CASE(6)
   XNYZ_RADIX_CASE(7)
@@ -2533,12 +2561,19 @@ void scan_excl_u64(const u64* in, u64* out, u64 n, u64* total, Arena& ar, hipStr
   ar.off = mark;
 }
 
+u32 radix_passes(u32 bits, u32 hi_bits, bool wide, RadixShape shape) {
+  const u32 w = shape.bits;
+  return wide ? (21 + w - 1) / w + (hi_bits + w - 1) / w : (bits + w - 1) / w;
+}
+
 int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, RadixShape shape,
                Arena& ar, hipStream_t s) {
+  const u32 P = radix_passes(bits, 0, false, shape);
   int cur = 0;
-  for (u32 shift = 0; shift < bits; shift += shape.bits) {
-    if (v) radix_pass<u64, u32, true>(bt, a, a2, v, v2, shift, shape, ar, s);
-    else radix_pass<u64, u32, false>(bt, a, a2, nullptr, nullptr, shift, shape, ar, s);
+  u32 j = 0;
+  for (u32 shift = 0; shift < bits; shift += shape.bits, ++j) {
+    if (v) radix_pass<u64, u32, true>(bt, a, a2, v, v2, shift, shape, j, P, ar, s);
+    else radix_pass<u64, u32, false>(bt, a, a2, nullptr, nullptr, shift, shape, j, P, ar, s);
     std::swap(a, a2);
     std::swap(v, v2);
     cur ^= 1;
@@ -2547,15 +2582,17 @@ int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, Radi
 }
 int radix_sort_wide(const Batch& bt, u64* hi, u64* hi2, u32* lo, u32* lo2, u32 hi_bits,
                     RadixShape shape, Arena& ar, hipStream_t s) {
+  const u32 P = radix_passes(0, hi_bits, true, shape);
   int cur = 0;
-  for (u32 shift = 0; shift < 21; shift += shape.bits) {
-    radix_pass<u32, u64, true>(bt, lo, lo2, hi, hi2, shift, shape, ar, s);
+  u32 j = 0;
+  for (u32 shift = 0; shift < 21; shift += shape.bits, ++j) {
+    radix_pass<u32, u64, true>(bt, lo, lo2, hi, hi2, shift, shape, j, P, ar, s);
     std::swap(hi, hi2);
     std::swap(lo, lo2);
     cur ^= 1;
   }
-  for (u32 shift = 0; shift < hi_bits; shift += shape.bits) {
-    radix_pass<u64, u32, true>(bt, hi, hi2, lo, lo2, shift, shape, ar, s);
+  for (u32 shift = 0; shift < hi_bits; shift += shape.bits, ++j) {
+    radix_pass<u64, u32, true>(bt, hi, hi2, lo, lo2, shift, shape, j, P, ar, s);
     std::swap(hi, hi2);
     std::swap(lo, lo2);
     cur ^= 1;
@@ -2564,15 +2601,17 @@ int radix_sort_wide(const Batch& bt, u64* hi, u64* hi2, u32* lo, u32* lo2, u32 h
 }
 int radix_sort_wide_v(const Batch& bt, u64* hi, u64* hi2, u64* lo64, u64* lo64b, u32 hi_bits,
                       RadixShape shape, Arena& ar, hipStream_t s) {
+  const u32 P = radix_passes(0, hi_bits, true, shape);
   int cur = 0;
-  for (u32 shift = 0; shift < 21; shift += shape.bits) {
-    radix_pass<u64, u64, true>(bt, lo64, lo64b, hi, hi2, shift, shape, ar, s);
+  u32 j = 0;
+  for (u32 shift = 0; shift < 21; shift += shape.bits, ++j) {
+    radix_pass<u64, u64, true>(bt, lo64, lo64b, hi, hi2, shift, shape, j, P, ar, s);
     std::swap(hi, hi2);
     std::swap(lo64, lo64b);
     cur ^= 1;
   }
-  for (u32 shift = 0; shift < hi_bits; shift += shape.bits) {
-    radix_pass<u64, u64, true>(bt, hi, hi2, lo64, lo64b, shift, shape, ar, s);
+  for (u32 shift = 0; shift < hi_bits; shift += shape.bits, ++j) {
+    radix_pass<u64, u64, true>(bt, hi, hi2, lo64, lo64b, shift, shape, j, P, ar, s);
     std::swap(hi, hi2);
     std::swap(lo64, lo64b);
     cur ^= 1;

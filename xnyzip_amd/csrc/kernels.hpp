@@ -115,6 +115,8 @@ struct SegParam {
                       // patch (254 = p_for_delta_encoder.hpp:31; lower values
                       // only come from a mode override)
   u32 search;         // also collect SegHist::bitpop (mode search)
+  u32 passes;         // key sort: radix passes this segment's keys need (0: all);
+                      // later passes leave its keys in place
   u64 sec_byte[kNumSec];  // byte offset of each data section (after sizing)
 };
 
@@ -228,6 +230,9 @@ struct RadixShape {
   u32 sub = 1;
 };
 
+// Digit passes of a key sort: narrow keys of `bits` bits, or wide keys (a
+// 21-bit low word, then `hi_bits` of the high word).
+u32 radix_passes(u32 bits, u32 hi_bits, bool wide, RadixShape shape);
 int radix_sort(const Batch& bt, u64* a, u64* a2, u32* v, u32* v2, u32 bits, RadixShape shape,
                Arena& ar, hipStream_t s);
 int radix_sort_wide(const Batch& bt, u64* hi, u64* hi2, u32* lo, u32* lo2, u32 hi_bits,
