@@ -1852,6 +1852,11 @@ HostCode huffman_code(const uint8_t* m, u64 ml, u32 sym_bytes, bool is_signed, u
     return h;
   }
   h.d.kind = kCodeHuf;
+  // Symbols below 2^24 go into the table entries themselves (the decoder
+  // then needs no second lookup per value).
+  bool dv = true;
+  for (const E& e : es) dv &= e.sym < (1ull << 24);
+  h.d.direct_values = dv;
   const u32 lb = std::min<u32>(kLutBits, maxlen);
   h.d.lb = lb;
   h.lut.assign(1u << lb, 0u);
@@ -1868,7 +1873,7 @@ HostCode huffman_code(const uint8_t* m, u64 ml, u32 sym_bytes, bool is_signed, u
       const u32 lo = e.bits << (lb - e.len), hi = (e.bits + 1) << (lb - e.len);
       for (u32 x = lo; x < hi; ++x) {
         if (h.lut[x]) throw Error(XNYZ_ECORRUPT, "Huffman codes are not prefix-free");
-        h.lut[x] = (e.len << 24) | i;
+        h.lut[x] = (e.len << 24) | (dv ? (u32)e.sym : i);
       }
     } else {
       const u32 pre = e.bits >> (e.len - lb);
@@ -1890,7 +1895,7 @@ HostCode huffman_code(const uint8_t* m, u64 ml, u32 sym_bytes, bool is_signed, u
         u32& ch = h.tree[2 * node + bit];
         if (b == 0) {
           if (ch != kTreeNull) throw Error(XNYZ_ECORRUPT, "Huffman codes are not prefix-free");
-          ch = kTreeLeaf | i;
+          ch = kTreeLeaf | (dv ? (u32)e.sym : i);
         } else {
           if (ch == kTreeNull) {
             const u32 nn = new_node();

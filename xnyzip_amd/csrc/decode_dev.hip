@@ -51,9 +51,9 @@ inline u32 blocks_for(u64 n, u32 per_block) {
 }
 
 // ---- bit access --------------------------------------------------------------
-// The stream buffer is 256-byte aligned and padded with at least 16 zero bytes,
-// so two aligned 8-byte loads around any bit before the stream's end stay
-// inside it.
+// The stream buffer is 256-byte aligned and padded with 64 zero bytes, so
+// aligned 8-byte loads up to three words past any bit before the stream's
+// end stay inside it.
 
 __device__ __forceinline__ u64 ld64(const uint8_t* s, u64 w) {
   return reinterpret_cast<const u64*>(s)[w];
@@ -79,14 +79,65 @@ __device__ __forceinline__ u64 peek_lsb(const uint8_t* s, u64 g) {
   return (a >> o) | (ld64(s, w + 1) << (64 - o));
 }
 
+// A thread's forward-only view of the stream: three consecutive 64-bit words
+// in registers (byte-swapped for the MSB-first codes), so that a walk over a
+// sub-sequence reloads once per 64 bits instead of waiting on two loads per
+// code word.  Every peek of a walk is at or past the previous one.  The
+// buffer's 64 zero bytes of padding keep the two words read ahead inside it.
+template <bool kMsb>
+struct BitWin {
+  const u64* s;
+  u64 wi;  // word of w0
+  u64 w0, w1, w2;
+
+  __device__ __forceinline__ u64 ld(u64 k) const {
+    const u64 x = s[k];
+    return kMsb ? __builtin_bswap64(x) : x;
+  }
+  __device__ __forceinline__ void at(u64 g) {
+    wi = g >> 6;
+    w0 = ld(wi);
+    w1 = ld(wi + 1);
+    w2 = ld(wi + 2);
+  }
+  __device__ __forceinline__ BitWin(const uint8_t* base, u64 g)
+      : s(reinterpret_cast<const u64*>(base)) {
+    at(g);
+  }
+  // peek_msb / peek_lsb of bit g (g at or past the last peek)
+  __device__ __forceinline__ u64 peek(u64 g) {
+    u64 d = g - (wi << 6);
+    if (d >= 64) {
+      if (d < 128) {
+        w0 = w1;
+        w1 = w2;
+        w2 = ld(wi + 3);
+        ++wi;
+        d -= 64;
+      } else {
+        at(g);
+        d = g & 63;
+      }
+    }
+    const u32 o = (u32)d;
+    if (!o) return w0;
+    return kMsb ? (w0 << o) | (w1 >> (64 - o)) : (w0 >> o) | (w1 << (64 - o));
+  }
+};
+
+template <u32 kKind>
+using WinOf = BitWin<kKind != kCodeGR>;
+
 // ---- one code word -----------------------------------------------------------
-// Decodes the code word starting at section bit p (< c.bits) into *v and
+// Each decodes the code word starting at section bit p (< c.bits) into v and
 // returns the bit after it.  A code word that would run past the section's
 // end returns c.bits + 1 (no valid stream has one; the caller's walk stops).
 
-__device__ __forceinline__ u64 next_huf(const uint8_t* s, const CodeDesc& c, const u32* lut,
-                                        u64 p, u64& v) {
-  const u64 w = peek_msb(s, c.base_bit + p);
+// Huffman: a lookup of lb bits, then the tree for longer codes (at most 32
+// bits).
+__device__ __forceinline__ u64 next_huf_w(BitWin<true>& W, const CodeDesc& c, const u32* lut,
+                                          u64 p, u64& v) {
+  const u64 w = W.peek(c.base_bit + p);
   const u32 e = lut[w >> (64 - c.lb)];
   const u32 len = e >> 24;
   if (len) {
@@ -94,7 +145,7 @@ __device__ __forceinline__ u64 next_huf(const uint8_t* s, const CodeDesc& c, con
     return p + len;
   }
   u32 node = e & 0xFFFFFFu;
-  for (u32 i = c.lb; i < 64; ++i) {  // codes are at most 32 bits long
+  for (u32 i = c.lb; i < 64; ++i) {
     const u32 ch = c.tree[2 * node + ((w >> (63 - i)) & 1)];
     if (ch == kTreeNull) return c.bits + 1;
     if (ch & kTreeLeaf) {
@@ -108,21 +159,21 @@ __device__ __forceinline__ u64 next_huf(const uint8_t* s, const CodeDesc& c, con
 
 // huffman_encoder.hpp:365-396: z zeros, then n1 = (n >> k) + 1 in z + 1 bits
 // MSB-first, then the k low bits of n LSB-first.
-__device__ __forceinline__ u64 next_eg(const uint8_t* s, const CodeDesc& c, u64 p, u64& v) {
-  const u64 w = peek_msb(s, c.base_bit + p);
-  if (!w) return c.bits + 1;  // more than 63 zeros: no such code word here
+__device__ __forceinline__ u64 next_eg_w(BitWin<true>& W, const CodeDesc& c, u64 p, u64& v) {
+  const u64 w = W.peek(c.base_bit + p);
+  if (!w) return c.bits + 1;
   const u32 z = (u32)__clzll((i64)w);
-  const u64 q = p + z + 1 + z + c.k;  // end of the code word
+  const u64 q = p + z + 1 + z + c.k;
   if (q > c.bits) return c.bits + 1;
   u64 n1;
   if (2 * z + 1 <= 64) {
     n1 = (w << z) >> (63 - z);
   } else {
-    n1 = peek_msb(s, c.base_bit + p + z) >> (63 - z);
+    n1 = W.peek(c.base_bit + p + z) >> (63 - z);
   }
   u64 low = 0;
   if (c.k) {
-    const u64 t = peek_msb(s, c.base_bit + p + 2 * z + 1) >> (64 - c.k);
+    const u64 t = W.peek(c.base_bit + p + 2 * z + 1) >> (64 - c.k);
     low = __builtin_bitreverse64(t) >> (64 - c.k);
   }
   v = ((n1 - 1) << c.k) | low;
@@ -131,20 +182,20 @@ __device__ __forceinline__ u64 next_eg(const uint8_t* s, const CodeDesc& c, u64 
 
 // golomb_rice_encoder.hpp:92-148: q one bits, a zero, then the k low bits
 // LSB-first.
-__device__ __forceinline__ u64 next_gr(const uint8_t* s, const CodeDesc& c, u64 p, u64& v) {
+__device__ __forceinline__ u64 next_gr_w(BitWin<false>& W, const CodeDesc& c, u64 p, u64& v) {
   u64 q = 0;
-  u64 w = peek_lsb(s, c.base_bit + p);
+  u64 w = W.peek(c.base_bit + p);
   while (w == ~0ull) {
     q += 64;
     if (p + q >= c.bits) return c.bits + 1;
-    w = peek_lsb(s, c.base_bit + p + q);
+    w = W.peek(c.base_bit + p + q);
   }
   q += (u64)__builtin_ctzll(~w);
   const u64 e = p + q + 1 + c.k;
   if (e > c.bits) return c.bits + 1;
   u64 r = 0;
   if (c.k) {
-    r = peek_lsb(s, c.base_bit + p + q + 1);
+    r = W.peek(c.base_bit + p + q + 1);
     if (c.k < 64) r &= (1ull << c.k) - 1;
   }
   v = (q << c.k) + r;
@@ -152,11 +203,11 @@ __device__ __forceinline__ u64 next_gr(const uint8_t* s, const CodeDesc& c, u64 
 }
 
 template <u32 kKind>
-__device__ __forceinline__ u64 next_code(const uint8_t* s, const CodeDesc& c, const u32* lut,
-                                         u64 p, u64& v) {
-  if constexpr (kKind == kCodeHuf) return next_huf(s, c, lut, p, v);
-  if constexpr (kKind == kCodeEG) return next_eg(s, c, p, v);
-  return next_gr(s, c, p, v);
+__device__ __forceinline__ u64 next_code_w(WinOf<kKind>& W, const CodeDesc& c, const u32* lut,
+                                           u64 p, u64& v) {
+  if constexpr (kKind == kCodeHuf) return next_huf_w(W, c, lut, p, v);
+  if constexpr (kKind == kCodeEG) return next_eg_w(W, c, p, v);
+  if constexpr (kKind == kCodeGR) return next_gr_w(W, c, p, v);
 }
 
 // The Huffman lookup table goes to LDS (at most 2^kLutBits entries).
@@ -183,8 +234,9 @@ __global__ __launch_bounds__(kBlock) void k_pd_spec(const uint8_t* __restrict__ 
   const u64 b = t * kSub, end = b + kSub < c.bits ? b + kSub : c.bits;
   u64 p = b, v;
   u32 n = 0;
+  WinOf<kKind> W(s, c.base_bit + p);
   while (p < end) {
-    p = next_code<kKind>(s, c, lut, p, v);
+    p = next_code_w<kKind>(W, c, lut, p, v);
     ++n;
   }
   start[t] = b;
@@ -220,12 +272,13 @@ __global__ __launch_bounds__(kBlock) void k_pd_sync(const uint8_t* __restrict__ 
   const u64 b = t * kSub, end = b + kSub < c.bits ? b + kSub : c.bits;
   u64 p = entry, q = cur, v;
   u32 np = 0, nq = 0;
+  WinOf<kKind> WP(s, c.base_bit + (p < end ? p : 0)), WQ(s, c.base_bit + (q < end ? q : 0));
   while (p != q && p < end && q < end) {
     if (p < q) {
-      p = next_code<kKind>(s, c, lut, p, v);
+      p = next_code_w<kKind>(WP, c, lut, p, v);
       ++np;
     } else {
-      q = next_code<kKind>(s, c, lut, q, v);
+      q = next_code_w<kKind>(WQ, c, lut, q, v);
       ++nq;
     }
   }
@@ -236,7 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_pd_sync(const uint8_t* __restrict__ 
     n = np + (cnt[t] - nq);
   } else {
     while (p < end) {
-      p = next_code<kKind>(s, c, lut, p, v);
+      p = next_code_w<kKind>(WP, c, lut, p, v);
       ++np;
     }
     ex = p;
@@ -260,12 +313,36 @@ __global__ __launch_bounds__(kBlock) void k_pd_write(const uint8_t* __restrict__
   if (t >= nsub) return;
   u64 p = start[t], o = base[t];
   const u32 n = cnt[t];
+  WinOf<kKind> W(s, c.base_bit + (p < c.bits ? p : 0));
+  // Byte outputs go out four at a time on 4-byte boundaries (the thread's
+  // first and last partial words byte by byte): a thread's values are
+  // consecutive, and one store per value would touch a cache line per lane.
+  u32 pack = 0;
   for (u32 i = 0; i < n && o < count; ++i, ++o) {
     u64 v = 0;
-    p = next_code<kKind>(s, c, lut, p, v);
-    if constexpr (kKind == kCodeHuf) v = c.values[v];
+    p = next_code_w<kKind>(W, c, lut, p, v);
+    if constexpr (kKind == kCodeHuf)
+      if (!c.direct_values) v = c.values[v];
     if (c.zigzag) v = (u64)(i64)((v >> 1) ^ (~(v & 1) + 1));
-    out[o] = (T)v;
+    if constexpr (sizeof(T) == 1) {
+      const u32 sh = 8u * (u32)(o & 3);
+      pack |= ((u32)v & 0xFFu) << sh;
+      const u64 w0 = o & ~3ull;
+      const bool last = i + 1 == n || o + 1 == count;
+      if ((o & 3) == 3 || last) {
+        // whole word inside this thread's range: one store; else the bytes
+        const u64 first = base[t];
+        if (w0 >= first && (o & 3) == 3) {
+          *reinterpret_cast<u32*>(out + w0) = pack;
+        } else {
+          for (u64 b = (w0 > first ? w0 : first); b <= o; ++b)
+            out[b] = (T)(pack >> (8u * (u32)(b & 3)));
+        }
+        pack = 0;
+      }
+    } else {
+      out[o] = (T)v;
+    }
   }
 }
 
@@ -402,12 +479,79 @@ __device__ LState block_excl(LState v, LState* sh, LState& total) {
 constexpr u32 kLItems = 16;
 constexpr u32 kLTile = kBlock * kLItems;
 
+// A thread's kLItems consecutive elements' symbols, quads and block-start
+// flags, one 16-byte load per array (the arrays are padded by 16 bytes past
+// the last element; bytes past it are ignored).
+struct LBytes {
+  u32 sym[4], head[4], quad[4];
+};
+static_assert(kLItems == 16, "one 16-byte load per array");
+
+__device__ __forceinline__ LBytes load_bytes(const LcpArgs& A, u64 b0) {
+  LBytes b;
+  const uint4 s4 = *reinterpret_cast<const uint4*>(A.sym + b0);
+  b.sym[0] = s4.x, b.sym[1] = s4.y, b.sym[2] = s4.z, b.sym[3] = s4.w;
+  if (!A.direct) {
+    const uint4 h4 = *reinterpret_cast<const uint4*>(A.head + b0);
+    const uint4 q4 = *reinterpret_cast<const uint4*>(A.quad + b0);
+    b.head[0] = h4.x, b.head[1] = h4.y, b.head[2] = h4.z, b.head[3] = h4.w;
+    b.quad[0] = q4.x, b.quad[1] = q4.y, b.quad[2] = q4.z, b.quad[3] = q4.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b.head[k] = b.quad[k] = 0;
+  }
+  return b;
+}
+
+__device__ __forceinline__ u32 byte_at(const u32* w, u32 i) { return (w[i >> 2] >> (8 * (i & 3))) & 0xFFu; }
+
+// lelem() from an item's three bytes.
+__device__ __forceinline__ LState lelem_v(const LcpArgs& A, u32 sym, u32 bh, u32 qb) {
+  LState e = lzero();
+  e.nz = sym == 0;
+  e.a = sym ? sym - 1 : 0;
+  if (!A.direct) {
+    const u32 qd = qb - 1u;
+    e.nb = bh;
+    e.q = qd;
+    e.fb = bh;
+    e.fr = bh | (qd != 0);
+  }
+  return e;
+}
+
+// Word w of a four-word array without indexing it (registers, not memory).
+__device__ __forceinline__ u32 word_of(const u32* x, u32 w) {
+  return w == 0 ? x[0] : w == 1 ? x[1] : w == 2 ? x[2] : x[3];
+}
+
+// lelem() of item i from the loaded bytes.
+__device__ __forceinline__ LState lelem_b(const LcpArgs& A, const LBytes& b, u32 i) {
+  LState e = lzero();
+  const u32 sym = byte_at(b.sym, i);
+  e.nz = sym == 0;
+  e.a = sym ? sym - 1 : 0;
+  if (!A.direct) {
+    const u32 bh = byte_at(b.head, i);
+    const u32 qd = byte_at(b.quad, i) - 1u;
+    e.nb = bh;
+    e.q = qd;
+    e.fb = bh;
+    e.fr = bh | (qd != 0);
+  }
+  return e;
+}
+
 __global__ __launch_bounds__(kBlock) void k_lcp_reduce(LcpArgs A, LState* tiles) {
   __shared__ LState sh[kBlock];
   const u64 b0 = (u64)blockIdx.x * kLTile + (u64)threadIdx.x * kLItems;
   LState acc = lzero();
-  for (u32 i = 0; i < kLItems; ++i)
-    if (b0 + i < A.n) acc = lcombine(acc, lelem(A, b0 + i));
+  if (b0 < A.n) {
+    const LBytes lb = load_bytes(A, b0);
+#pragma unroll
+    for (u32 i = 0; i < kLItems; ++i)
+      if (b0 + i < A.n) acc = lcombine(acc, lelem_b(A, lb, i));
+  }
   LState tot;
   (void)block_excl(acc, sh, tot);
   if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
@@ -453,8 +597,52 @@ __device__ __forceinline__ u64 h2m(u64 h, u32 L, const u8* apply, const u8* next
   return r;
 }
 
+// Unsigned 64-bit division by a divisor fixed for the launch, as a multiply
+// and shifts (the branch-free round-up method: magic = 1 + floor(2^(65 + l) / d)
+// with l = floor(log2 d), q = ((n - hi) / 2 + hi) >> l, hi = mulhi(magic, n));
+// exact for every 64-bit n.  Powers of two take magic 0 and shift l - 1; d = 1
+// is flagged.
+struct DivU64 {
+  u64 magic;
+  u32 more, one;
+};
+
+DivU64 div_u64(u64 d) {
+  DivU64 r{0, 0, d == 1 ? 1u : 0u};
+  if (d <= 1) return r;
+  const u32 l = 63 - (u32)__builtin_clzll(d);
+  if ((d & (d - 1)) == 0) {
+    r.more = l - 1;
+    return r;
+  }
+  const unsigned __int128 num = (unsigned __int128)1 << (64 + l);
+  u64 pm = (u64)(num / d);
+  const u64 rem = (u64)(num % d);
+  pm += pm;
+  const u64 twice = rem + rem;
+  if (twice >= d || twice < rem) pm += 1;
+  r.magic = pm + 1;
+  r.more = l;
+  return r;
+}
+
+__device__ __forceinline__ u64 div_by(u64 n, const DivU64& m) {
+  if (m.one) return n;
+  const u64 t = __umul64hi(m.magic, n);
+  return (((n - t) >> 1) + t) >> m.more;
+}
+
+// Launch constants of k_lcp_apply: the block-grid divisions, and the state
+// of the inverse Hilbert machine after the 15 zero digits above a block-mode
+// code's 6 (with the Morton digits they produce).
+struct LcpConst {
+  DivU64 dx, dy;
+  u64 h_hi;
+  u32 h_st;
+};
+
 __global__ __launch_bounds__(kBlock) void k_lcp_apply(LcpArgs A, const LState* tiles,
-                                                      HilbertInv hi) {
+                                                      HilbertInv hi, LcpConst K) {
   __shared__ LState sh[kBlock];
   __shared__ u8 ap[24 * 8], nx_[24 * 8];
   for (u32 i = threadIdx.x; i < 24 * 8; i += kBlock) {
@@ -462,62 +650,115 @@ __global__ __launch_bounds__(kBlock) void k_lcp_apply(LcpArgs A, const LState* t
     nx_[i] = hi.next[i];
   }
   const u64 b0 = (u64)blockIdx.x * kLTile + (u64)threadIdx.x * kLItems;
+  LBytes lb;
   LState acc = lzero();
-  for (u32 i = 0; i < kLItems; ++i)
-    if (b0 + i < A.n) acc = lcombine(acc, lelem(A, b0 + i));
+  if (b0 < A.n) {
+    lb = load_bytes(A, b0);
+#pragma unroll
+    for (u32 i = 0; i < kLItems; ++i)
+      if (b0 + i < A.n) acc = lcombine(acc, lelem_b(A, lb, i));
+  }
   LState tot;
   LState run = lcombine(tiles[blockIdx.x], block_excl(acc, sh, tot));
-  for (u32 i = 0; i < kLItems; ++i) {
-    const u64 j = b0 + i;
-    if (j >= A.n) break;
-    run = lcombine(run, lelem(A, j));
-    u64 cx, cy, cz;
-    if (A.direct) {  // block_decompressor.hpp:82-101
-      const u64 sfc = run.a + A.ps[run.nz < A.np ? run.nz : A.np];
-      const u64 m = A.hilbert ? h2m(sfc, A.bits, ap, nx_) : sfc;
-      cx = compact3(m);
-      cy = compact3(m >> 1);
-      cz = compact3(m >> 2);
-    } else {  // block_quantizer.hpp:166-203
-      const u64 id = A.blkst[run.nb - 1];
-      const u64 pbx = (id % A.nx * 2) * 64, pby = (id / A.nx % A.ny * 2) * 64,
-                pbz = (id / A.nx / A.ny * 2) * 64;
-      const u64 rj = run.a + (A.ps[run.nz < A.np ? run.nz : A.np] -
-                              A.ps[run.zsr < A.np ? run.zsr : A.np]);
-      const u64 m = A.hilbert ? h2m(rj, 20, ap, nx_) : rj;
-      cx = pbx + (u64)(run.q & 1) * 64 + compact3(m);
-      cy = pby + (u64)((run.q >> 1) & 1) * 64 + compact3(m >> 1);
-      cz = pbz + (u64)((run.q >> 2) & 1) * 64 + compact3(m >> 2);
+  if (b0 >= A.n) return;
+  // Four items at a time: their prefix states, then every gathered load
+  // (block id, patch sums, RLE offset and count) in flight together, then
+  // the points.
+  constexpr u32 kG = 4;  // one word of each byte array
+  for (u32 g = 0; g < kLItems; g += kG) {
+    const u32 sw = word_of(lb.sym, g / 4), hw = word_of(lb.head, g / 4),
+              qw = word_of(lb.quad, g / 4);
+    LState rs[kG];
+#pragma unroll
+    for (u32 k = 0; k < kG; ++k) {
+      if (b0 + g + k < A.n)
+        run = lcombine(run, lelem_v(A, (sw >> (8 * k)) & 0xFFu, (hw >> (8 * k)) & 0xFFu,
+                                    (qw >> (8 * k)) & 0xFFu));
+      rs[k] = run;
     }
-    int n3[3] = {(int)cx, (int)cy, (int)cz};
-    float v[3];
-    if (A.to) {  // truncated_octahedron_quantizer.hpp:57-76
-      if (n3[2] & 1) {
-        n3[0] = (n3[0] << 1) - 1;
-        n3[1] = (n3[1] << 1) - 1;
-      } else {
-        n3[0] <<= 1;
-        n3[1] <<= 1;
+    u64 id[kG], p1[kG], p2[kG], at[kG], cn[kG];
+#pragma unroll
+    for (u32 k = 0; k < kG; ++k) {
+      const u64 j = b0 + g + k;
+      const bool in = j < A.n;
+      p1[k] = A.ps[rs[k].nz < A.np ? rs[k].nz : A.np];
+      if (!A.direct) {
+        id[k] = in && rs[k].nb ? A.blkst[rs[k].nb - 1] : 0;
+        p2[k] = A.ps[rs[k].zsr < A.np ? rs[k].zsr : A.np];
       }
-      for (int a = 0; a < 3; ++a) v[a] = ((float(n3[a]) * 2.0f) * A.b) / A.rec_div;
-    } else {  // cube_quantizer.hpp:29-38
-      for (int a = 0; a < 3; ++a) v[a] = (((float(n3[a]) + 0.5f) * 2.0f) * A.b) / A.rec_div;
+      if (A.rle) {
+        at[k] = in ? A.out_at[j] : 0;
+        cn[k] = in ? A.counts[j] : 0;
+      }
     }
-    for (int a = 0; a < 3; ++a) v[a] = v[a] + A.off[a];  // shifting.hpp:39-50
-    if (!A.rle) {
-      float* o = A.out + 3 * j;
-      o[0] = v[0];
-      o[1] = v[1];
-      o[2] = v[2];
-    } else {  // block_decompressor_rle.hpp:173-187: count copies, in cell order
-      const u64 at = A.out_at[j];
-      if (at >= A.total) continue;
-      float* o = A.out + 3 * at;
-      const u64 c = A.counts[j] < A.total - at ? A.counts[j] : A.total - at;
-      for (u64 k = 0; k < c; ++k) {
-        o[3 * k] = v[0];
-        o[3 * k + 1] = v[1];
-        o[3 * k + 2] = v[2];
+#pragma unroll
+    for (u32 k = 0; k < kG; ++k) {
+      const u64 j = b0 + g + k;
+      if (j >= A.n) continue;
+      const LState& r = rs[k];
+      u64 cx, cy, cz;
+      if (A.direct) {  // block_decompressor.hpp:82-101
+        const u64 sfc = r.a + p1[k];
+        const u64 m = A.hilbert ? h2m(sfc, A.bits, ap, nx_) : sfc;
+        cx = compact3(m);
+        cy = compact3(m >> 1);
+        cz = compact3(m >> 2);
+      } else {  // block_quantizer.hpp:166-203
+        const u64 q1 = div_by(id[k], K.dx), q2 = div_by(q1, K.dy);
+        const u64 pbx = ((id[k] - q1 * A.nx) * 2) * 64, pby = ((q1 - q2 * A.ny) * 2) * 64,
+                  pbz = (q2 * 2) * 64;
+        const u64 rj = r.a + (p1[k] - p2[k]);
+        u64 m = rj;
+        if (A.hilbert) {
+          if (rj < (1ull << 18)) {  // six live digits from the precomputed state
+            u32 st = K.h_st;
+            m = K.h_hi;
+#pragma unroll
+            for (int i = 5; i >= 0; --i) {
+              const u32 hd = (u32)(rj >> (3 * i)) & 7u;
+              m |= (u64)ap[st * 8 + hd] << (3 * i);
+              st = nx_[st * 8 + hd];
+            }
+          } else {
+            m = h2m(rj, 20, ap, nx_);
+          }
+        }
+        cx = pbx + (u64)(r.q & 1) * 64 + compact3(m);
+        cy = pby + (u64)((r.q >> 1) & 1) * 64 + compact3(m >> 1);
+        cz = pbz + (u64)((r.q >> 2) & 1) * 64 + compact3(m >> 2);
+      }
+      int n3[3] = {(int)cx, (int)cy, (int)cz};
+      float v[3];
+      if (A.to) {  // truncated_octahedron_quantizer.hpp:57-76
+        if (n3[2] & 1) {
+          n3[0] = (n3[0] << 1) - 1;
+          n3[1] = (n3[1] << 1) - 1;
+        } else {
+          n3[0] <<= 1;
+          n3[1] <<= 1;
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = ((float(n3[a]) * 2.0f) * A.b) / A.rec_div;
+      } else {  // cube_quantizer.hpp:29-38
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = (((float(n3[a]) + 0.5f) * 2.0f) * A.b) / A.rec_div;
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) v[a] = v[a] + A.off[a];  // shifting.hpp:39-50
+      if (!A.rle) {
+        float* o = A.out + 3 * j;
+        o[0] = v[0];
+        o[1] = v[1];
+        o[2] = v[2];
+      } else {  // block_decompressor_rle.hpp:173-187: count copies, in cell order
+        if (at[k] >= A.total) continue;
+        float* o = A.out + 3 * at[k];
+        const u64 c = cn[k] < A.total - at[k] ? cn[k] : A.total - at[k];
+        for (u64 t = 0; t < c; ++t) {
+          o[3 * t] = v[0];
+          o[3 * t + 1] = v[1];
+          o[3 * t + 2] = v[2];
+        }
       }
     }
   }
@@ -610,7 +851,12 @@ void launch_lcp(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream
   LAUNCH_CHECK("k_lcp_reduce");
   k_lcp_tiles<<<1, kBlock, 0, s>>>(tiles, nt);
   LAUNCH_CHECK("k_lcp_tiles");
-  k_lcp_apply<<<nt, kBlock, 0, s>>>(a, tiles, hi);
+  LcpConst K{div_u64(a.nx), div_u64(a.ny), 0, 0};
+  for (int i = 20; i >= 6; --i) {  // hilbertToMorton3's loop over zero digits
+    K.h_hi |= (u64)hi.apply[K.h_st * 8] << (3 * i);
+    K.h_st = hi.next[K.h_st * 8];
+  }
+  k_lcp_apply<<<nt, kBlock, 0, s>>>(a, tiles, hi, K);
   LAUNCH_CHECK("k_lcp_apply");
 }
 

@@ -28,6 +28,8 @@ struct CodeDesc {
   const u32* tree;    // Huffman: tree[2 * node + bit] = child node, or
                       // kTreeLeaf | value index, or kTreeNull
   const u64* values;  // Huffman: symbol of each value index
+  u32 direct_values;  // Huffman: the lookup table and tree carry the symbols
+                      // themselves (every symbol below 2^24), not indices
   u64 const_value;    // kCodeConst: a one-entry code table (every symbol equal)
 };
 
