@@ -151,6 +151,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary frame modes")
+    ap.add_argument("--no-decode", action="store_true",
+                    help="skip timing the device decoder on the headline stream")
     ap.add_argument("--no-verify", action="store_true",
                     help="batched workloads: skip the oracle check of sampled clouds")
     ap.add_argument("--tuning", default=None, metavar="FILE",
@@ -242,6 +244,8 @@ def main() -> int:
     verified = verify_stream(args.workload, w, n, bytes(sink.buf[s_at:s_at + stream_len]))
     print(f"[bench] last timed stream: {verified}", file=sys.stderr, flush=True)
     single = {"GBps": round(value, 3), "ms_per_step": round(dt / args.steps * 1e3, 3)}
+    decoded = None if args.no_decode else decode_stream(
+        X, ctx, args, w, n, sink.buf[s_at:s_at + stream_len])
 
     # The same steps pipelined (xnyz_compress_batch_device_into_async): call
     # k + 1's kernels run while call k's stream crosses PCIe into its pinned
@@ -383,6 +387,7 @@ def main() -> int:
         "verified": verified,
         "single_call": single,
         "pipelined": pipelined,
+        "decode": decoded,
         "stream_bytes": stream_len,
         "stream_ratio": round(12.0 * n / stream_len, 4),
         "framed": framed,
@@ -397,6 +402,58 @@ def main() -> int:
     if args.copy_probe:
         copy_probe(args.copy_probe)
     return 0
+
+
+def decode_stream(X, ctx, args, w: dict, n: int, stream) -> dict:
+    """The device decoder on the timed stream (xnyz_decompress_to_device: the
+    stream in pinned host memory, the points left in HBM).  The points are
+    checked against the SHA-256 of the oracle decoder's points for this
+    stream (tests/golden/workloads.json); the dominant kernel of one profiled
+    call is set against the HBM peak with its algorithmic bytes."""
+    import hashlib
+    import torch
+    q, l2, rle = w["quantizer"], w["l2"], w["rle"]
+    out = torch.empty((n, 3), dtype=torch.float32, device="cuda:0")
+    for _ in range(2):
+        m = X.decompress_to_device(stream, q, l2, rle, out, is_frame=False, ctx=ctx)
+    torch.cuda.synchronize()
+    k = max(3, min(10, args.steps))
+    t0 = time.perf_counter()
+    for _ in range(k):
+        m = X.decompress_to_device(stream, q, l2, rle, out, is_frame=False, ctx=ctx)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / k
+    path = os.path.join(ROOT, "tests", "golden", "workloads.json")
+    g = json.load(open(path)).get(args.workload, {}) if os.path.exists(path) else {}
+    verified = "unchecked: no oracle decode recorded for this workload"
+    if g.get("decoded_sha256") and g.get("points") == n:
+        got = hashlib.sha256(out[:m].cpu().numpy().astype("<f4").tobytes()).hexdigest()
+        verified = ("sha256 equal to the oracle decoder's points" if got == g["decoded_sha256"]
+                    and m == g.get("decoded_points") else "MISMATCH against the oracle decoder")
+    ctx.profile(True)
+    X.decompress_to_device(stream, q, l2, rle, out, is_frame=False, ctx=ctx)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    del out
+    kern = {k2: round(v["ms"], 3) for k2, v in sorted(rep.items(), key=lambda kv: -kv[1]["ms"])}
+    # k_lcp_apply per element: symbol, quad and block-start bytes (3), the
+    # block id and two patch sums it gathers (24), the RLE offset and count
+    # (16); per point written 12
+    cells = int.from_bytes(bytes(stream[:8]), "little")
+    ab = cells * (3 + 24 + (16 if rle else 0)) + 12 * m
+    ms = rep.get("k_lcp_apply", {}).get("ms")
+    roof = None
+    if ms:
+        ach = ab / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": "k_lcp_apply", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                "algo_bytes_per_launch": ab, "avg_launch_ms": round(ms, 4)}
+    print(f"[bench] decode: {12.0 * m / dt / 1e9:.2f} GB/s of points ({dt * 1e3:.1f} ms/call), "
+          f"{verified}", file=sys.stderr, flush=True)
+    return {"GBps": round(12.0 * m / dt / 1e9, 3), "unit": "GB/s of fp32 points out",
+            "ms_per_call": round(dt * 1e3, 3), "points": m, "verified": verified,
+            "input": "assembled stream in pinned host memory (its upload included)",
+            "roofline": roof, "kernel_ms": kern}
 
 
 def verify_stream(workload: str, w: dict, n: int, stream: bytes) -> str:

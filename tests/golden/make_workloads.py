@@ -1,7 +1,8 @@
 """Writes tests/golden/workloads.json: the oracle's assembled stream for the
 50M-point terrain (TO, L2 0.01, Hilbert, RLE) as a SHA-256, with the input's
-SHA-256, so a GPU test can check the HIP path at that size without running
-the oracle (about two minutes of one core) on the GPU host.
+SHA-256 and the SHA-256 of the points the oracle's decoder returns for that
+stream, so a GPU test can check the HIP compressor and decoder at that size
+without running the oracle (a few minutes of one core) on the GPU host.
 
     python tests/golden/make_workloads.py
 """
@@ -30,6 +31,9 @@ def main() -> int:
              "num_points": int.from_bytes(pre[:8], "little"),
              "stream_bytes": len(pre),
              "stream_sha256": hashlib.sha256(pre).hexdigest()}
+    back = O.decompress_buffer(pre, O.TO, 0.01, True)
+    entry["decoded_points"] = int(back.shape[0])
+    entry["decoded_sha256"] = hashlib.sha256(back.astype("<f4").tobytes()).hexdigest()
     path = os.path.join(HERE, "workloads.json")
     data = json.load(open(path)) if os.path.exists(path) else {}
     data["terrain-50M"] = entry

@@ -1,7 +1,8 @@
 """The measured workloads through the HIP path, against the oracle.
 
 * terrain, 50M points (TO, L2 0.01, Hilbert, RLE -- the throughput
-  configuration): the assembled stream's SHA-256 against the oracle's,
+  configuration): the assembled stream's SHA-256 against the oracle's, and
+  the device decoder's points on it against the oracle decoder's,
   committed in tests/golden/workloads.json by tests/golden/make_workloads.py
   (the oracle needs about two minutes of one core for this size); the input's
   SHA-256 is checked first so the generator is known to give the same bytes.
@@ -83,6 +84,13 @@ def test_terrain_50m_stream(X, terrain50m):
     assert st.num_points == g["num_points"]
     assert _sha(pre) == g["stream_sha256"], \
         f"50M terrain stream differs from the oracle's ({len(pre)} B)"
+    # the device decoder on that stream: the oracle decoder's points
+    del dev
+    out = torch.empty((g["points"], 3), dtype=torch.float32, device="cuda")
+    m = X.decompress_to_device(pre, O.TO, 0.01, True, out, is_frame=False)
+    assert m == g["decoded_points"]
+    assert _sha(out[:m].cpu().numpy().astype("<f4").tobytes()) == g["decoded_sha256"], \
+        "50M terrain decode differs from the oracle decoder's points"
 
 
 C2 = [(q, b) for q in (O.TO, O.CUBE) for b in (0.001, 0.01, 0.1)]

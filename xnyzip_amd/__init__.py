@@ -446,7 +446,7 @@ def decompress(data: bytes, quantizer, l2_bound: float, rle: bool,
     o = _opts(quantizer, l2_bound, False, rle, 1024,
               _lib.FRAME_ZSTD3 if is_frame else _lib.FRAME_NONE, 0)
     out = _lib.Buf()
-    src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    src = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
     _lib.check(_lib.lib().xnyz_decompress(C.c_void_p(src.ctypes.data), len(data), C.byref(o),
                                            C.byref(out)))
     return np.frombuffer(_lib.take(out), dtype=np.float32).reshape(-1, 3).copy()
@@ -459,7 +459,7 @@ def decompress_device(data: bytes, quantizer, l2_bound: float, rle: bool,
     o = _opts(quantizer, l2_bound, False, rle, 1024,
               _lib.FRAME_ZSTD3 if is_frame else _lib.FRAME_NONE, 0)
     out = _lib.Buf()
-    src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    src = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
     ctx = ctx or default_context(0)
     _lib.check(_lib.lib().xnyz_decompress_device(ctx.handle, C.c_void_p(src.ctypes.data),
                                                   len(data), C.byref(o), C.byref(out)))
@@ -477,7 +477,7 @@ def decompress_to_device(data: bytes, quantizer, l2_bound: float, rle: bool, out
         raise ValueError("out_device must be a contiguous float32 (M, 3) tensor on the GPU")
     o = _opts(quantizer, l2_bound, False, rle, 1024,
               _lib.FRAME_ZSTD3 if is_frame else _lib.FRAME_NONE, 0)
-    src = np.frombuffer(data, dtype=np.uint8) if data else np.zeros(1, np.uint8)
+    src = np.frombuffer(data, dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
     n = C.c_uint64()
     ctx = ctx or default_context(t.device.index or 0)
     _inputs_ready(t.device)
