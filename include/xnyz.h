@@ -380,8 +380,8 @@ int xnyz_morton_to_hilbert(xnyz_ctx* ctx, const uint64_t* in, uint64_t n,
  * the stream does not record: opts->quantizer, l2_bound and rle;
  * opts->frame_mode XNYZ_FRAME_NONE means `in` is an assembled (pre-zstd)
  * stream, anything else a zstd frame.  `out_points` receives n*3 floats
- * (len = 12*n bytes) in stream order.  Runs on the host: the entropy-coded
- * streams have no restart points.  No device or context needed. */
+ * (len = 12*n bytes) in stream order.  Runs on the host (no device or
+ * context needed); xnyz_decompress_device below is the device form. */
 int xnyz_decompress(const uint8_t* in, uint64_t len, const xnyz_opts* opts,
                     xnyz_buf* out_points);
 
