@@ -60,6 +60,13 @@ BATCHED = {
     # half terrain, half shells, chunks of 2^U(16, 20) points: exhaustive
     # mode search ("plain") vs the learned selector
     "selector-64M": dict(kind="selector", n=64_000_000, quantizer=1),
+    # the shell chunks with shared codebooks: fit_profile on the first chunk,
+    # compress_with_profile on all of them (the MPI driver's --shared-codebook)
+    "shells-100M-shared": dict(kind="shared", n=100_000_000, chunk=1 << 20, quantizer=1,
+                               l2=0.05, hilbert=False, rle=False),
+    # the terrain as one global RLE job (the MPI driver's mapreduce mode)
+    "terrain-50M-global-rle": dict(kind="global", n=50_000_000, quantizer=1, l2=0.01,
+                                   hilbert=True, rle=True),
 }
 
 
@@ -470,6 +477,69 @@ def verify_stream(workload: str, w: dict, n: int, stream: bytes) -> str:
             else "MISMATCH against the oracle's stream")
 
 
+def run_modes(args, w, X, _lib, dev, offs, opts, n, ctx, timed, wu) -> dict:
+    """Shared codebooks and global RLE: their device paths timed as streams
+    and as device-built frames, into one reused pinned host buffer, with
+    streams of sampled chunks (shared) or of a 2^22-point prefix (global RLE)
+    checked against the oracle after the timed steps."""
+    import hashlib
+    from oracle import oracle as O
+    res = {}
+    q, l2, h = w["quantizer"], w["l2"], w["hilbert"]
+    pick = [0, 1, 2, len(offs) - 2]  # shared: chunks checked against the oracle
+    sink = OutBuffer(13 * n, pinned=not args.pageable_out)
+    for mode, key in ((_lib.FRAME_NONE, "prezstd"), (_lib.FRAME_ZSTD_GPU, "gpuframes")):
+        if w["kind"] == "shared":
+            c0 = dev[offs[0]:offs[1]]
+            _, _, prof = X.fit_profile(c0, q, l2, h, ctx=ctx)
+            o2 = [dict(o, frame_mode=mode) for o in opts]
+
+            def step():
+                r, _ = sink.call(lambda buf: X.compress_with_profile_batch_into(
+                    dev, offs, o2, prof, buf, ctx=ctx))
+                return r
+        else:
+            def step():
+                r, _ = sink.call(lambda buf: X.compress_global_rle_into(
+                    dev, q, l2, buf, h, frame_mode=mode, ctx=ctx))
+                return r
+        dt, r = timed(step, args.steps, wu)
+        nbytes = int(r[-1] - r[0])
+        fr = [bytes(sink.buf[int(r[i]):int(r[i + 1])]) if i in pick else b""
+              for i in range(len(r) - 1)]
+        res[f"{key}_GBps"] = round(12.0 * n * args.steps / dt / 1e9, 3)
+        res[f"{key}_bytes"] = nbytes
+        res[f"{key}_ms_per_step"] = round(dt / args.steps * 1e3, 3)
+        print(f"[bench] {key}: {res[f'{key}_GBps']:.2f} GB/s", file=sys.stderr, flush=True)
+        if mode == _lib.FRAME_NONE and not args.no_verify:
+            if w["kind"] == "shared":
+                bad = []
+                for i in pick:
+                    c = dev[offs[i]:offs[i + 1]].cpu().numpy()
+                    want, _ = O.compress_shared(c, q, l2, h, profile=prof)
+                    if fr[i] != want:
+                        bad.append(i)
+                res["verified"] = (f"chunks {pick} byte-identical to the oracle's streams"
+                                   if not bad else f"MISMATCH against the oracle in chunks {bad}")
+            else:
+                m = min(n, 1 << 22)
+                sub = dev[:m].contiguous()
+                _, pre, _ = X.compress_global_rle(sub, q, l2, h, frame_mode=mode,
+                                                  want_prezstd=True, ctx=ctx)
+                want = O.compress_global_rle(sub.cpu().numpy(), q, l2, h)
+                res["verified"] = ("first 2^22 points' stream byte-identical to the oracle's"
+                                   if pre == want else "MISMATCH against the oracle")
+    res["value"] = res["prezstd_GBps"]
+    res["ms_per_step"] = res["prezstd_ms_per_step"]
+    res["stream_ratio"] = round(12.0 * n / res["prezstd_bytes"], 4)
+    res["gpuframe_ratio"] = round(12.0 * n / res["gpuframes_bytes"], 4)
+    res["metric"] = ("compress GB/s (fp32 xyz in), " +
+                     ("shared codebooks per chunk" if w["kind"] == "shared"
+                      else "global RLE of the whole cloud") +
+                     ", assembled streams; device-built frames in 'gpuframes_GBps'")
+    return res
+
+
 def _verify_clouds(buf, res, dev, offs, opts, sample: int = 8) -> str:
     """Checks sampled clouds of a timed batch (the smallest ones, which reach
     direct mode, and seeded picks of up to 400k points) against the oracle's
@@ -554,12 +624,18 @@ def run_batched(args) -> int:
 
     w = dict(BATCHED[args.workload])
     t_gen = time.perf_counter()
-    if w["kind"] == "chunked":
+    if w["kind"] in ("chunked", "shared"):
         pts = datasets.shells(args.points or w["n"], seed=0)
         n = len(pts)
         offs = list(range(0, n, w["chunk"])) + [n]
         opts = [dict(quantizer=w["quantizer"], l2_bound=w["l2"], is_hilbert=w["hilbert"],
                      rle=w["rle"]) for _ in range(len(offs) - 1)]
+    elif w["kind"] == "global":
+        pts = datasets.terrain(args.points or w["n"], seed=0)
+        n = len(pts)
+        offs = [0, n]
+        opts = [dict(quantizer=w["quantizer"], l2_bound=w["l2"], is_hilbert=w["hilbert"],
+                     rle=True)]
     elif w["kind"] == "uneven":
         def progress(done, count):
             print(f"[bench] {done}/{count} clouds generated", file=sys.stderr, flush=True)
@@ -596,7 +672,9 @@ def run_batched(args) -> int:
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
            "data": "synthetic",
            "config": {"workload": args.workload, "points": n, "clouds": len(offs) - 1}}
-    if w["kind"] != "selector":
+    if w["kind"] in ("shared", "global"):
+        out.update(run_modes(args, w, X, _lib, dev, offs, opts, n, ctx, timed, wu))
+    elif w["kind"] != "selector":
         forms = [(_lib.FRAME_NONE, "prezstd"), (_lib.FRAME_ZSTD3, "frames")]
         # The device-built frame needs ~18 bytes of scratch per stream byte
         # (per 16 KiB block: candidates, parse slots, FSE records); beyond

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define XNYZ_ABI_VERSION 6
+#define XNYZ_ABI_VERSION 7
 
 enum xnyz_status {
   XNYZ_OK = 0,
@@ -263,6 +263,17 @@ int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
                                             uint64_t profile_len, xnyz_buf* frames,
                                             xnyz_buf* prezstd, xnyz_stats* stats);
 
+/* xnyz_compress_with_profile_batch_device writing into caller memory, laid
+ * out as by xnyz_compress_batch_device_into (out_offsets: count + 1 entries;
+ * XNYZ_ENOSPC with the size needed in out_offsets[count]).  Same
+ * counterpart: compress_with_profile (block_compressor.hpp:248-256). */
+int xnyz_compress_with_profile_batch_device_into(xnyz_ctx* ctx, const float* d_xyz,
+                                                 const uint64_t* offsets, uint32_t count,
+                                                 const xnyz_opts* opts, const uint8_t* profile,
+                                                 uint64_t profile_len, uint8_t* out,
+                                                 uint64_t cap, uint64_t* out_offsets,
+                                                 xnyz_stats* stats);
+
 /* The MPI driver's mapreduce mode (source/mpi_driver.cpp:365-648: global
  * bounding box, quantize every chunk against it, shuffle and reduce the cell
  * counts across ranks, then BlockCompressorRLE::encode_cells_with_counts,
@@ -277,6 +288,13 @@ int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
 int xnyz_compress_global_rle_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
                                     const xnyz_opts* opts, xnyz_buf* out_frame,
                                     xnyz_buf* out_prezstd, xnyz_stats* stats);
+
+/* xnyz_compress_global_rle_device writing into caller memory (out_offsets:
+ * 2 entries, layout and XNYZ_ENOSPC as for xnyz_compress_batch_device_into).
+ * Same counterpart: the mapreduce mode (mpi_driver.cpp:365-648). */
+int xnyz_compress_global_rle_device_into(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
+                                         const xnyz_opts* opts, uint8_t* out, uint64_t cap,
+                                         uint64_t* out_offsets, xnyz_stats* stats);
 
 /* Exhaustive per-chunk mode search; no reference counterpart (the
  * reference codes every chunk with the options it is given).  For each cloud

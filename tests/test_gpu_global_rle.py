@@ -68,3 +68,16 @@ def test_cli_mapreduce(X, tmp_path):
     assert r.returncode == 0, r.stderr
     got = np.fromfile(out, np.float32).reshape(-1, 3)
     assert np.array_equal(got, O.decompress_buffer(want, O.TO, 0.01, True))
+
+
+def test_caller_memory_form(X):
+    """compress_global_rle_into writes the allocating form's bytes."""
+    from xnyzip_amd import _lib
+    from xnyzip_amd import datasets as D
+    pts = torch.from_numpy(D.terrain(500_000, seed=9)).cuda()
+    for mode in (_lib.FRAME_NONE, _lib.FRAME_ZSTD3):
+        fr, _, _ = X.compress_global_rle(pts, O.TO, 0.01, True, frame_mode=mode)
+        buf = X.host_buffer(len(fr) + 64)
+        res, st = X.compress_global_rle_into(pts, O.TO, 0.01, buf, True, frame_mode=mode)
+        assert bytes(buf[int(res[0]):int(res[1])]) == fr, mode
+        assert st.num_points > 0

@@ -92,3 +92,26 @@ def test_rle_refused(X):
         X.compress_with_profile_batch(pts.cuda(), [0, 1000],
                                       [dict(quantizer=O.TO, l2_bound=0.05, rle=True)], prof)
     assert e.value.code == _lib.XNYZ_EUNSUPPORTED
+
+
+def test_caller_memory_form(X):
+    """compress_with_profile_batch_into writes the allocating form's bytes
+    back to back (pinned and pageable memory), and reports the size needed
+    when the buffer is short."""
+    from xnyzip_amd import _lib
+    from xnyzip_amd import datasets as D
+    pts = D.shells(300_000, seed=8, count=150)
+    offs = [0, 100_000, 170_000, 300_000]
+    dev = torch.from_numpy(pts).cuda()
+    _, _, prof = X.fit_profile(dev[:100_000], O.TO, 0.05)
+    for mode in (_lib.FRAME_NONE, _lib.FRAME_ZSTD3):
+        opts = [dict(quantizer=O.TO, l2_bound=0.05, frame_mode=mode)] * 3
+        frames, _, _ = X.compress_with_profile_batch(dev, offs, opts, prof)
+        for buf in (X.host_buffer(sum(map(len, frames)) + 64),
+                    np.zeros(sum(map(len, frames)) + 64, np.uint8)):
+            res, _ = X.compress_with_profile_batch_into(dev, offs, opts, prof, buf)
+            for i, f in enumerate(frames):
+                assert bytes(buf[int(res[i]):int(res[i + 1])]) == f, (mode, i)
+        with pytest.raises(_lib.XnyzError) as e:
+            X.compress_with_profile_batch_into(dev, offs, opts, prof, np.zeros(8, np.uint8))
+        assert e.value.code == _lib.XNYZ_ENOSPC

@@ -28,7 +28,8 @@ __all__ = ["QUANTIZER_TYPE", "BlockCompressor", "BlockCompressorRLE", "BlockDeco
            "mlp_predict", "compress_selected_batch", "compress_selected_batch_into", "quantize", "morton_to_hilbert", "autotune",
            "decompress_device", "decompress_to_device", "hilbert_to_morton",
            "container_write", "container_read", "container_decompress", "fit_profile",
-           "compress_with_profile_batch", "compress_global_rle"]
+           "compress_with_profile_batch", "compress_with_profile_batch_into",
+           "compress_global_rle", "compress_global_rle_into"]
 
 
 class QUANTIZER_TYPE(IntEnum):  # source/composer/utils.hpp:55-59 (ADAPTIVE not offered)
@@ -532,6 +533,47 @@ def compress_with_profile_batch(points_device, offsets, opts, profile: bytes,
         C.cast(stats, C.c_void_p)))
     return ([_lib.take(f) for f in frames],
             [_lib.take(p) for p in pres] if pres is not None else None, list(stats))
+
+
+def compress_with_profile_batch_into(points_device, offsets, opts, profile: bytes,
+                                     out: np.ndarray, ctx: Context | None = None):
+    """compress_with_profile_batch writing into `out` (layout as
+    compress_batch_into).  Returns (offsets, stats)."""
+    t, count, off, arr = _batch_args(points_device, offsets, opts, _lib.FRAME_ZSTD3)
+    if out.dtype != np.uint8 or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint8 array")
+    ctx = ctx or default_context(t.device.index or 0)
+    prof = np.frombuffer(profile, np.uint8)
+    res = np.zeros(count + 1, dtype=np.uint64)
+    stats = (_lib.Stats * count)()
+    _lib.check(_lib.lib().xnyz_compress_with_profile_batch_device_into(
+        ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
+        C.cast(arr, C.c_void_p), C.c_void_p(prof.ctypes.data), len(profile),
+        C.c_void_p(out.ctypes.data), out.size, C.c_void_p(res.ctypes.data),
+        C.cast(stats, C.c_void_p)))
+    return res, list(stats)
+
+
+def compress_global_rle_into(points_device, quantizer, l2_bound, out: np.ndarray,
+                             is_hilbert=False, frame_mode=_lib.FRAME_ZSTD3,
+                             ctx: Context | None = None):
+    """compress_global_rle writing into `out`.  Returns (offsets, stats): the
+    output is out[offsets[0]:offsets[1]]."""
+    import torch
+    t = points_device.reshape(-1, 3)
+    if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+        raise ValueError("points must be a contiguous float32 (N, 3) tensor on the GPU")
+    if out.dtype != np.uint8 or not out.flags.c_contiguous:
+        raise ValueError("out must be a contiguous uint8 array")
+    ctx = ctx or default_context(t.device.index or 0)
+    o = _opts(quantizer, l2_bound, is_hilbert, True, 1024, frame_mode, 0)
+    res = np.zeros(2, dtype=np.uint64)
+    st = _lib.Stats()
+    _inputs_ready(t.device)
+    _lib.check(_lib.lib().xnyz_compress_global_rle_device_into(
+        ctx.handle, C.c_void_p(t.data_ptr()), t.shape[0], C.byref(o),
+        C.c_void_p(out.ctypes.data), out.size, C.c_void_p(res.ctypes.data), C.byref(st)))
+    return res, st
 
 
 def compress_global_rle(points_device, quantizer, l2_bound, is_hilbert=False,

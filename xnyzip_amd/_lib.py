@@ -120,7 +120,9 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_morton_to_hilbert", "xnyz_hilbert_to_morton", "xnyz_decompress",
            "xnyz_decompress_device", "xnyz_decompress_to_device", "xnyz_container_write",
            "xnyz_container_read", "xnyz_container_decompress", "xnyz_fit_profile_device",
-           "xnyz_compress_with_profile_batch_device", "xnyz_compress_global_rle_device",
+           "xnyz_compress_with_profile_batch_device",
+           "xnyz_compress_with_profile_batch_device_into", "xnyz_compress_global_rle_device",
+           "xnyz_compress_global_rle_device_into",
            "xnyz_graphs_enable",
            "xnyz_tuning_set", "xnyz_tuning_get", "xnyz_tuning_save", "xnyz_tuning_load",
            "xnyz_autotune", "xnyz_profile_enable", "xnyz_profile_report", "xnyz_free",
@@ -176,6 +178,10 @@ def lib() -> C.CDLL:
         L.xnyz_compress_global_rle_device.argtypes = [P, P, C.c_uint64, C.POINTER(Opts),
                                                       C.POINTER(Buf), C.POINTER(Buf),
                                                       C.POINTER(Stats)]
+        L.xnyz_compress_with_profile_batch_device_into.argtypes = [
+            P, P, P, C.c_uint32, P, P, C.c_uint64, P, C.c_uint64, P, P]
+        L.xnyz_compress_global_rle_device_into.argtypes = [
+            P, P, C.c_uint64, C.POINTER(Opts), P, C.c_uint64, P, C.POINTER(Stats)]
         L.xnyz_container_write.argtypes = [C.c_char_p, C.c_char_p, P, P, C.c_uint32, P]
         L.xnyz_container_read.argtypes = [C.c_char_p, C.POINTER(Buf)]
         L.xnyz_container_decompress.argtypes = [P, C.c_char_p, C.c_char_p, C.POINTER(Buf)]

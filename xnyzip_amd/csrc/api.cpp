@@ -2487,35 +2487,79 @@ int xnyz_fit_profile_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const
   });
 }
 
+namespace {
+
+// deserialize_profile (block_encoding_profile.hpp:48-70): the repos and the
+// quads metas.
+void read_profile(const uint8_t* profile, uint64_t profile_len, std::vector<uint8_t>* metas) {
+  u64 pos = 0;
+  for (int k = 0; k < 2; ++k) {
+    if (pos + 4 > profile_len) throw Error(XNYZ_EINVAL, "truncated profile");
+    const uint32_t n = uint32_t(profile[pos]) | uint32_t(profile[pos + 1]) << 8 |
+                       uint32_t(profile[pos + 2]) << 16 | uint32_t(profile[pos + 3]) << 24;
+    pos += 4;
+    if (pos + n > profile_len) throw Error(XNYZ_EINVAL, "truncated profile");
+    metas[k].assign(profile + pos, profile + pos + n);
+    pos += n;
+  }
+  if (metas[0].empty() || metas[1].empty()) throw Error(XNYZ_EINVAL, "empty profile");
+}
+
+int with_profile(xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets, uint32_t count,
+                 const xnyz_opts* opts, const uint8_t* profile, uint64_t profile_len,
+                 CallOut& co) {
+  if (!ctx || !d_xyz || !offsets || !opts || !count || !profile)
+    throw Error(XNYZ_EINVAL, "null argument or empty batch");
+  std::vector<u64> off = rebase(offsets, count, d_xyz);
+  for (u32 s = 0; s < count; ++s)
+    if (opts[s].rle) throw Error(XNYZ_EUNSUPPORTED, "shared codebooks are not defined for RLE");
+  std::vector<uint8_t> metas[2];
+  read_profile(profile, profile_len, metas);
+  use_device(ctx);
+  co.shared = 2;
+  co.profile_metas = metas;
+  return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+}
+
+int global_rle(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, const xnyz_opts* opts,
+               CallOut& co) {
+  if (!ctx || !d_xyz) throw Error(XNYZ_EINVAL, "null argument");
+  validate(opts, n);
+  xnyz_opts o = *opts;
+  o.rle = 1;
+  o.mode_flags = 0;
+  use_device(ctx);
+  co.global_rle = true;
+  return compress_impl(ctx, d_xyz, {0, n}, &o, co, now_ms());
+}
+
+}  // namespace
+
 int xnyz_compress_with_profile_batch_device(xnyz_ctx* ctx, const float* d_xyz,
                                             const uint64_t* offsets, uint32_t count,
                                             const xnyz_opts* opts, const uint8_t* profile,
                                             uint64_t profile_len, xnyz_buf* frames,
                                             xnyz_buf* prezstd, xnyz_stats* stats) {
   return guarded([&] {
-    if (!ctx || !d_xyz || !offsets || !opts || !count || !frames || !profile)
-      throw Error(XNYZ_EINVAL, "null argument or empty batch");
-    std::vector<u64> off = rebase(offsets, count, d_xyz);
-    for (u32 s = 0; s < count; ++s)
-      if (opts[s].rle) throw Error(XNYZ_EUNSUPPORTED, "shared codebooks are not defined for RLE");
-    // deserialize_profile (block_encoding_profile.hpp:48-70)
-    std::vector<uint8_t> metas[2];
-    u64 pos = 0;
-    for (auto& m : metas) {
-      if (pos + 4 > profile_len) throw Error(XNYZ_EINVAL, "truncated profile");
-      const uint32_t n = uint32_t(profile[pos]) | uint32_t(profile[pos + 1]) << 8 |
-                         uint32_t(profile[pos + 2]) << 16 | uint32_t(profile[pos + 3]) << 24;
-      pos += 4;
-      if (pos + n > profile_len) throw Error(XNYZ_EINVAL, "truncated profile");
-      m.assign(profile + pos, profile + pos + n);
-      pos += n;
-    }
-    if (metas[0].empty() || metas[1].empty()) throw Error(XNYZ_EINVAL, "empty profile");
-    use_device(ctx);
+    if (!frames) throw Error(XNYZ_EINVAL, "null frames");
     CallOut co{frames, prezstd, stats, nullptr};
-    co.shared = 2;
-    co.profile_metas = metas;
-    return compress_impl(ctx, d_xyz, off, opts, co, now_ms());
+    return with_profile(ctx, d_xyz, offsets, count, opts, profile, profile_len, co);
+  });
+}
+
+int xnyz_compress_with_profile_batch_device_into(xnyz_ctx* ctx, const float* d_xyz,
+                                                 const uint64_t* offsets, uint32_t count,
+                                                 const xnyz_opts* opts, const uint8_t* profile,
+                                                 uint64_t profile_len, uint8_t* out,
+                                                 uint64_t cap, uint64_t* out_offsets,
+                                                 xnyz_stats* stats) {
+  return guarded([&] {
+    if (!out_offsets || (!out && cap)) throw Error(XNYZ_EINVAL, "null output");
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    return with_profile(ctx, d_xyz, offsets, count, opts, profile, profile_len, co);
   });
 }
 
@@ -2523,15 +2567,22 @@ int xnyz_compress_global_rle_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t 
                                     const xnyz_opts* opts, xnyz_buf* out_frame,
                                     xnyz_buf* out_prezstd, xnyz_stats* stats) {
   return guarded([&] {
-    if (!ctx || !d_xyz || !out_frame) throw Error(XNYZ_EINVAL, "null argument");
-    validate(opts, n);
-    xnyz_opts o = *opts;
-    o.rle = 1;
-    o.mode_flags = 0;
-    use_device(ctx);
+    if (!out_frame) throw Error(XNYZ_EINVAL, "null argument");
     CallOut co{out_frame, out_prezstd, stats, nullptr};
-    co.global_rle = true;
-    return compress_impl(ctx, d_xyz, {0, n}, &o, co, now_ms());
+    return global_rle(ctx, d_xyz, n, opts, co);
+  });
+}
+
+int xnyz_compress_global_rle_device_into(xnyz_ctx* ctx, const float* d_xyz, uint64_t n,
+                                         const xnyz_opts* opts, uint8_t* out, uint64_t cap,
+                                         uint64_t* out_offsets, xnyz_stats* stats) {
+  return guarded([&] {
+    if (!out_offsets || (!out && cap)) throw Error(XNYZ_EINVAL, "null output");
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    return global_rle(ctx, d_xyz, n, opts, co);
   });
 }
 
