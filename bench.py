@@ -9,7 +9,11 @@ the reference's compress() hands to ZSTD_compress (source/composer/
 block_compressor_rle.hpp:38-321, source/composer/utils.hpp:35-53), byte for
 byte; the last timed stream is checked against the oracle's afterwards.
 `value` is GB/s of fp32 xyz input (12 bytes per point) over the K timed
-steps.
+steps, the steps pipelined (xnyz_compress_batch_device_into_async: step
+k + 1's kernels run while step k's stream crosses PCIe into one of two pinned
+buffers; every stream is in host memory when the clock stops, and both
+buffers are checked); "single_call" is the rate of steps that each wait for
+their own copy.  "decode" times the device decoder on the timed stream.
 
 The zstd frame stage is reported beside it over the same K steps
 ("framed": level 3 split between device-built head blocks and host worker
@@ -149,8 +153,8 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="terrain-50M",
                     choices=sorted(WORKLOADS) + sorted(BATCHED))
     ap.add_argument("--clouds", type=int, default=0, help="batched-uneven: cloud count")
@@ -357,6 +361,19 @@ def main() -> int:
                "sample": f"first {m} points of the same workload, one compress() up to the "
                          f"assembled stream (the headline's boundary; the RLE diagnostic "
                          f"included, as in the reference), {info.seconds_total:.2f} s"}
+        if decoded is not None:
+            # the oracle's decoder (BlockDecompressor[RLE]::decompress restated)
+            # on the sample's stream
+            q = O.TO if w["quantizer"] == 1 else O.CUBE
+            pre = O.compress(sample, q, w["l2"], w["hilbert"], w["rle"]).pre
+            t0 = time.perf_counter()
+            back = O.decompress_buffer(pre, q, w["l2"], w["rle"])
+            dtc = time.perf_counter() - t0
+            decoded["cpu_baseline"] = {
+                "value": round(12.0 * len(back) / dtc / 1e9, 5), "unit": "GB/s of fp32 points out",
+                "cores": 1, "kind": "port",
+                "sample": f"the oracle's decoder on the stream of the first {m} points, "
+                          f"{dtc:.2f} s"}
 
     dev_ms = {k: round(v["ms"] / args.steps, 3) for k, v in
               sorted(prof.items(), key=lambda kv: -kv[1]["ms"])}
