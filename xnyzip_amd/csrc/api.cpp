@@ -723,6 +723,9 @@ QuantParams quant_params(const xnyz_opts* o) {
   qp.div = static_cast<float>(static_cast<double>(two_b) / root);
   qp.b = o->l2_bound;
   qp.rec_div = static_cast<float>(root);
+  qp.inv = 1.0f / qp.div;
+  qp.rec_inv = 1.0f / qp.rec_div;
+  qp.fast = div_const_ok(qp.div) ? 1 : 0;
   return qp;
 }
 
@@ -2154,6 +2157,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   a.ny = (u64)(uint32_t)rng[1] / 128 + 1;
   a.b = o->l2_bound;
   a.rec_div = a.to ? float(std::sqrt(5.0)) : float(std::sqrt(3.0));
+  a.rec_inv = 1.0f / a.rec_div;
   for (int k = 0; k < 3; ++k) a.off[k] = off[k];
   a.sym = ctx->dsym.as<uint8_t>();
   a.quad = direct ? nullptr : ctx->dquad.as<uint8_t>();

@@ -33,6 +33,7 @@
 
 #include "common.hpp"
 #include "decode_dev.hpp"
+#include "exact_div.hpp"
 
 #pragma clang fp contract(off)
 
@@ -738,10 +739,10 @@ __global__ __launch_bounds__(kBlock) void k_lcp_apply(LcpArgs A, const LState* t
           n3[1] <<= 1;
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = ((float(n3[a]) * 2.0f) * A.b) / A.rec_div;
+        for (int a = 0; a < 3; ++a) v[a] = div_const((float(n3[a]) * 2.0f) * A.b, A.rec_div, A.rec_inv);
       } else {  // cube_quantizer.hpp:29-38
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = (((float(n3[a]) + 0.5f) * 2.0f) * A.b) / A.rec_div;
+        for (int a = 0; a < 3; ++a) v[a] = div_const(((float(n3[a]) + 0.5f) * 2.0f) * A.b, A.rec_div, A.rec_inv);
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) v[a] = v[a] + A.off[a];  // shifting.hpp:39-50

@@ -67,6 +67,7 @@ struct LcpArgs {
   u32 bits;                 // direct mode: Hilbert iteration
   u64 nx, ny;               // block mode: blocks per axis
   float b, rec_div;         // L2 bound, float(sqrt(5)) or float(sqrt(3))
+  float rec_inv;            // RN(1/rec_div) (div_const, exact_div.hpp)
   float off[3];             // shift offset
   const uint8_t* sym;       // PFD symbols of repos / sfc deltas
   const uint8_t* quad;      // PFD symbols of quads (block mode)

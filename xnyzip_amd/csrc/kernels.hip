@@ -290,7 +290,16 @@ __device__ __forceinline__ void quant_point(const F3& p, const float* off,
   sx = p.x - off[0];
   sy = p.y - off[1];
   sz = p.z - off[2];
-  const float qx = sx / qp.div, qy = sy / qp.div, qz = sz / qp.div;
+  float qx, qy, qz;
+  if (qp.fast) {  // uniform
+    qx = div_const(sx, qp.div, qp.inv);
+    qy = div_const(sy, qp.div, qp.inv);
+    qz = div_const(sz, qp.div, qp.inv);
+  } else {
+    qx = sx / qp.div;
+    qy = sy / qp.div;
+    qz = sz / qp.div;
+  }
   if (qp.to) {
     to_node(qx, qy, qz, X, Y, Z);
   } else {
@@ -312,13 +321,13 @@ __device__ __forceinline__ void recover_node(int X, int Y, int Z, const QuantPar
       X <<= 1;
       Y <<= 1;
     }
-    rx = ((float)X * 2.0f * qp.b) / qp.rec_div;
-    ry = ((float)Y * 2.0f * qp.b) / qp.rec_div;
-    rz = ((float)Z * 2.0f * qp.b) / qp.rec_div;
+    rx = div_const((float)X * 2.0f * qp.b, qp.rec_div, qp.rec_inv);
+    ry = div_const((float)Y * 2.0f * qp.b, qp.rec_div, qp.rec_inv);
+    rz = div_const((float)Z * 2.0f * qp.b, qp.rec_div, qp.rec_inv);
   } else {
-    rx = (((float)X + 0.5f) * 2.0f * qp.b) / qp.rec_div;
-    ry = (((float)Y + 0.5f) * 2.0f * qp.b) / qp.rec_div;
-    rz = (((float)Z + 0.5f) * 2.0f * qp.b) / qp.rec_div;
+    rx = div_const(((float)X + 0.5f) * 2.0f * qp.b, qp.rec_div, qp.rec_inv);
+    ry = div_const(((float)Y + 0.5f) * 2.0f * qp.b, qp.rec_div, qp.rec_inv);
+    rz = div_const(((float)Z + 0.5f) * 2.0f * qp.b, qp.rec_div, qp.rec_inv);
   }
 }
 
@@ -805,12 +814,19 @@ __device__ __forceinline__ bool pass_active(const SegParam& sp, u32 pass) {
   return sp.passes == 0 || pass < sp.passes;
 }
 
+// Digit counts of one radix tile.  The LDS histogram is replicated per lane
+// group (copy lane & 15, copies 2^kBits + 1 words apart, odd modulo the 32
+// banks): in the passes over the keys' top digits most keys of a tile share
+// a few digits, and one copy per wavefront would serialise their atomics.
+constexpr u32 kHistRep = 16;
+
 template <typename K, u32 kBits, u32 kSub>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(Batch bt, const K* __restrict__ keys,
                                                        u32 shift, u32 pass,
                                                        u32* __restrict__ counts) {
   using RS = RadixShapeT<kBits, kSub>;
-  __shared__ u32 h[kWavesPerBlock][RS::D];
+  constexpr u32 kStride = RS::D + 1;
+  __shared__ u32 h[kHistRep * kStride];
   const u32 tix = blockIdx.x / kSub, sub = blockIdx.x % kSub;
   const Tile t = bt.tiles[tix];
   const SegParam& sp = bt.sp[t.seg];
@@ -822,9 +838,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(Batch bt, const K* __rest
       counts[count_index(sp, t.local, sub, threadIdx.x, RS::D, kSub)] = threadIdx.x ? 0u : n;
     return;
   }
-  for (u32 i = threadIdx.x; i < kWavesPerBlock * RS::D; i += kBlock) (&h[0][0])[i] = 0;
+  for (u32 i = threadIdx.x; i < kHistRep * kStride; i += kBlock) h[i] = 0;
   __syncthreads();
-  const u32 w = threadIdx.x / kWave;
+  u32* mine = h + (lane_id() & (kHistRep - 1)) * kStride;
   if (r.count > base) {  // uniform; an empty radix tile still writes its zero counts
     // every key in flight before the first count: loads clamped to the
     // tile's last key, so none is conditional
@@ -838,14 +854,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(Batch bt, const K* __rest
 #pragma unroll
     for (u32 it = 0; it < RS::kIt; ++it)
       if (base + it * kBlock + threadIdx.x < r.count)
-        atomicAdd(&h[w][(u32)(kk[it] >> shift) & (RS::D - 1)], 1u);
+        atomicAdd(&mine[(u32)(kk[it] >> shift) & (RS::D - 1)], 1u);
   }
   __syncthreads();
   const u32 d = threadIdx.x;
   if (d < RS::D) {
     u32 c = 0;
 #pragma unroll
-    for (int k = 0; k < kWavesPerBlock; ++k) c += h[k][d];
+    for (u32 k = 0; k < kHistRep; ++k) c += h[k * kStride + d];
     counts[count_index(sp, t.local, sub, d, RS::D, kSub)] = c;
   }
 }
@@ -1424,11 +1440,16 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
 // (block mode) or of the SFC deltas (direct mode, block_compressor.hpp:110-117),
 // quads symbols, patch count and wrapping sum.  First occurrences are
 // relative to the segment's first element.
-constexpr u32 kMagStride = kMagBins + 1;  // 69: odd modulo 32
+//
+// A thread codes elements t, t + 256, ... of each tile and walks its tiles
+// in order, so the elements it sees come in increasing position: a thread
+// records a symbol's first occurrence once (an LDS atomicMin) and remembers
+// that in a register bitmap, instead of reading the LDS minimum for every
+// element.  The histograms take one atomic per element on the lane's copy
+// (kRep copies), with no validity test in full tiles.
 struct StatLds {
   u32 hs[kRep * kRepStride256], hq[kRep * kRepStride256];  // replicated
   u32 fs[256], fq[256];
-  u32 mag[kRep * kMagStride];  // replicated
   u64 bitpop[64];
   u64 red[kWavesPerBlock];
 };
@@ -1445,15 +1466,10 @@ __device__ __forceinline__ void flush_stats(StatLds& L, SegCtl& sc, SegHist& h, 
   }
   np = nqp = 0;
   ps = 0;
-  fold_hist<kMagBins, kMagStride>(L.mag);
   fold_hist<256, kRepStride256>(L.hq);
   fold_hist<256, kRepStride256>(L.hs);
   __syncthreads();
   const u32 d = threadIdx.x;
-  if (d < kMagBins) {
-    if (L.mag[d]) atomicAdd(&h.mag[d], L.mag[d]);
-    L.mag[d] = 0;
-  }
   if (d < 64) {
     if (L.bitpop[d]) atomicAdd(&h.bitpop[d], L.bitpop[d]);
     L.bitpop[d] = 0;
@@ -1462,75 +1478,111 @@ __device__ __forceinline__ void flush_stats(StatLds& L, SegCtl& sc, SegHist& h, 
   flush_hist(L.hs, L.fs, h.sym, h.first_sym);
 }
 
+// Symbols whose first occurrence a thread has recorded: 256 bits for the
+// repos / delta symbols (w0..w3), and for the quads symbols -- which are 0,
+// 1..8 or 250..255, quad deltas being differences of 3-bit values -- bit
+// (s + 8) & 15 of q, distinct for each of them.
+__device__ __forceinline__ bool seen_sym(u64 w0, u64 w1, u64 w2, u64 w3, u32 s) {
+  const u64 x = (s & 128) ? ((s & 64) ? w3 : w2) : ((s & 64) ? w1 : w0);
+  return (x >> (s & 63)) & 1ull;
+}
+
+constexpr u32 kStatGroup = 8;  // elements per thread loaded together
+
+template <bool kFull>
+__device__ __forceinline__ void stats_items(StatLds& L, u64& w0, u64& w1, u64& w2, u64& w3,
+                                            u32& sq, u32 rep, bool direct, u32 lim, u32 count,
+                                            u32 rel0, const u32* o, const u64* v, const u32* qd,
+                                            u32& np, u32& nqp, u64& ps) {
+#pragma unroll
+  for (u32 it = 0; it < kStatGroup; ++it) {
+    if (!kFull && o[it] >= count) break;
+    const u32 rel = rel0 + o[it];
+    const u32 qs = qd[it] > 254 ? 0u : qd[it] + 1u;
+    const bool big = v[it] > lim;
+    const u32 sym = big ? 0u : (u32)v[it] + 1u;
+    if (!direct) {
+      atomicAdd(&L.hq[rep + qs], 1u);
+      nqp += qd[it] > 254;
+      const u32 qb = 1u << ((qs + 8) & 15);
+      if (!(sq & qb)) {
+        sq |= qb;
+        atomicMin(&L.fq[qs], rel);
+      }
+    }
+    atomicAdd(&L.hs[rep + sym], 1u);
+    if (!seen_sym(w0, w1, w2, w3, sym)) {
+      const u64 m = 1ull << (sym & 63);
+      const u32 h = sym >> 6;
+      w0 |= h == 0 ? m : 0ull;
+      w1 |= h == 1 ? m : 0ull;
+      w2 |= h == 2 ? m : 0ull;
+      w3 |= h == 3 ? m : 0ull;
+      atomicMin(&L.fs[sym], rel);
+    }
+    np += big;
+    ps += big ? v[it] : 0ull;
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
   __shared__ StatLds L;
   u32 t0, t1;
   block_tiles(bt.ntiles, t0, t1);
   if (t0 >= t1) return;
   const u32 d = threadIdx.x, l = lane_id();
+  const u32 rep = (l & (kRep - 1)) * kRepStride256;  // this lane's histogram copy
   for (u32 i = d; i < kRep * kRepStride256; i += kBlock) L.hs[i] = L.hq[i] = 0;
-  for (u32 i = d; i < kRep * kMagStride; i += kBlock) L.mag[i] = 0;
   L.fs[d] = L.fq[d] = 0xFFFFFFFFu;
   if (d < 64) L.bitpop[d] = 0;
   __syncthreads();
   u32 np = 0, nqp = 0;
   u64 ps = 0;
+  u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // symbols recorded (seen_sym)
+  u32 sq = 0;
   u32 cur = bt.tiles[t0].seg;
   for (u32 ti = t0; ti < t1; ++ti) {
     const Tile t = bt.tiles[ti];
     if (t.seg != cur) {
       flush_stats(L, bt.sc[cur], bt.sh[cur], np, nqp, ps);
+      w0 = w1 = w2 = w3 = 0;
+      sq = 0;
       cur = t.seg;
     }
     const SegParam& sp = bt.sp[t.seg];
     const SegCtl& sc = bt.sc[t.seg];
     const Range r = tile_range(sc.e0, sc.ne, t.local);
+    if (!r.count) continue;
     const bool direct = sp.kp.direct != 0;
     const u32 lim = sp.lim;
-    u64 bp = 0;  // this lane's share of bitpop: lane b counts bit b
-    // Loads first (all items in flight), then the LDS updates.  Each thread
-    // takes 16 consecutive elements (each key loaded once).
-    u64 v[kItems];
-    u32 qd[kItems];
-    const u32 b = threadIdx.x * kItems;
-    if (b < r.count) {
-      lcp_run<kItems>(k, direct, sc.e0, r.first, r.count, b, v, qd);
-    } else {
+    // Element it * kBlock + thread of the tile (coalesced loads; a thread's
+    // elements still come in increasing position), a group's loads in
+    // flight before its LDS updates.
+    const u32 rel0 = r.first - sc.e0;
+    const bool full = r.count == kTile;
+    u64 bp = 0;  // mode search: this lane's share of bitpop (lane b counts bit b)
+    for (u32 g0 = 0; g0 < kItems; g0 += kStatGroup) {
+      u64 v[kStatGroup];
+      u32 qd[kStatGroup], o[kStatGroup];
 #pragma unroll
-      for (u32 it = 0; it < kItems; ++it) v[it] = qd[it] = 0;
-    }
+      for (u32 it = 0; it < kStatGroup; ++it) o[it] = (g0 + it) * kBlock + threadIdx.x;
+      lcp_items<kStatGroup>(k, direct, sc.e0, r.first, r.count, o, v, qd);
+      if (full)  // uniform
+        stats_items<true>(L, w0, w1, w2, w3, sq, rep, direct, lim, r.count, rel0, o, v, qd, np,
+                          nqp, ps);
+      else
+        stats_items<false>(L, w0, w1, w2, w3, sq, rep, direct, lim, r.count, rel0, o, v, qd, np,
+                           nqp, ps);
+      if (sp.search) {  // bit populations of the values > 254 (mode search only)
 #pragma unroll
-    for (u32 it = 0; it < kItems; ++it) {
-      const u32 o = b + it;
-      const bool in = o < r.count;
-      const u32 rel = r.first + o - sc.e0;
-      const u32 qs = qd[it] > 254 ? 0u : qd[it] + 1u;
-      const u32 sym = v[it] > lim ? 0u : (u32)v[it] + 1u;
-      if (!direct) {
-        hist_add<kRepStride256>(L.hq, qs, in);
-        if (in) {
-          nqp += qd[it] > 254;
-          if (rel < L.fq[qs]) atomicMin(&L.fq[qs], rel);
-        }
-      }
-      hist_add<kRepStride256>(L.hs, sym, in);
-      hist_add<kMagStride>(L.mag, bitlen64(v[it]), in);
-      if (in) {
-        if (rel < L.fs[sym]) atomicMin(&L.fs[sym], rel);
-        if (v[it] > lim) {
-          ++np;
-          ps += v[it];
-        }
-      }
-    }
-    if (sp.search) {  // bit populations of the values > 254 (mode search only)
-      for (u32 it = 0; it < kItems; ++it) {
-        const bool big = b + it < r.count && v[it] > 254;
-        if (__ballot(big)) {
+        for (u32 it = 0; it < kStatGroup; ++it) {
+          const bool big = o[it] < r.count && v[it] > 254;
+          if (__ballot(big)) {
 #pragma unroll 8
-          for (u32 b = 0; b < 64; ++b) {
-            const u32 c = (u32)__popcll(__ballot(big && ((v[it] >> b) & 1ull)));
-            if (l == b) bp += c;
+            for (u32 bb = 0; bb < 64; ++bb) {
+              const u32 c = (u32)__popcll(__ballot(big && ((v[it] >> bb) & 1ull)));
+              if (l == bb) bp += c;
+            }
           }
         }
       }
@@ -2049,6 +2101,78 @@ struct BitRun {
   }
 };
 
+// A thread's run of bits in the LDS window, assembled without branches: the
+// word being filled is ORed into the window after every piece (its bits only
+// grow, so ORing it again is harmless) and the position moves on by selects.
+// The window starts zeroed, and neighbouring runs share their edge words
+// through the OR; finish() ORs the run's last word.
+template <bool kMsb>
+struct OrRun {
+  u32* words;
+  u32 cw;    // word being filled
+  u64 acc;   // MSB-first: word cw in acc's top half; LSB-first: in its bottom half
+  u32 fill;  // bits of word cw taken, 0..31 between pieces
+
+  __device__ __forceinline__ OrRun(u32* w, u32 p) : words(w), cw(p >> 5), acc(0), fill(p & 31) {}
+
+  // OR word cw into the window; move to the next word if cw is full.
+  __device__ __forceinline__ void step() {
+    atomicOr(&words[cw], kMsb ? (u32)(acc >> 32) : (u32)acc);
+    const bool full = fill >= 32;
+    cw += full ? 1u : 0u;
+    acc = full ? (kMsb ? acc << 32 : acc >> 32) : acc;
+    fill -= full ? 32u : 0u;
+  }
+  // `n` <= 32 bits of `v` (its low bits).
+  __device__ __forceinline__ void put(u32 v, u32 n) {
+    if (kMsb) acc |= n ? ((u64)v << (64 - n)) >> fill : 0ull;
+    else acc |= (u64)(n >= 32 ? v : v & ((1u << n) - 1u)) << fill;
+    fill += n;
+    step();
+  }
+  // `n` <= 64 bits of `v`.
+  __device__ __forceinline__ void put64(u64 v, u32 n) {
+    const u32 nh = n > 32 ? n - 32 : 0u, nl = n > 32 ? 32u : n;
+    if (kMsb) {
+      put((u32)(v >> 32), nh);
+      put((u32)v, nl);
+    } else {
+      put((u32)v, nl);
+      put((u32)(v >> 32), nh);
+    }
+  }
+  // `n` <= 64 zero bits (MSB-first Exp-Golomb prefixes).
+  __device__ __forceinline__ void zeros(u32 n) {
+    fill += n;
+    step();
+    step();
+  }
+  // `n` one bits (LSB-first Golomb-Rice quotients; any length).
+  __device__ __forceinline__ void ones(u32 n) {
+    while (n) {
+      const u32 t = n < 32 ? n : 32u;
+      put(t == 32 ? 0xFFFFFFFFu : (1u << t) - 1u, t);
+      n -= t;
+    }
+  }
+  // The last word of the run (its bits past the last OR).
+  __device__ __forceinline__ void finish() {
+    atomicOr(&words[cw], kMsb ? (u32)(acc >> 32) : (u32)acc);
+  }
+  // A Huffman code: a 32-bit value; a code longer than 32 bits (a tree deeper
+  // than the reference's uint32_t prefix holds) is len - 32 zero bits, then
+  // the value.
+  __device__ __forceinline__ void huff(const Code& c) {
+    if (c.len > 32) put(0u, c.len - 32);
+    put((u32)c.val, c.len > 32 ? 32u : c.len);
+  }
+  __device__ __forceinline__ void code(const Code& c) {
+    if (kMsb) zeros(c.lead);
+    else ones(c.lead);
+    put64(c.val, c.len);
+  }
+};
+
 // Packs one section of a tile whose threads own kItems consecutive elements
 // each ("blocked": thread t codes elements 16t .. 16t + 15, so its codes are
 // one run of the stream).  code(q) gives the thread's q-th code ({0,0,0}
@@ -2397,14 +2521,28 @@ __global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
   for (u32 i = threadIdx.x; i < wtot; i += kBlock) win[i] = 0;
   __syncthreads();
   {
-    BitRun<true, true> rh(win + woff[1], (abs0[1] & 31) + ex[1], mine[1]);
-    BitRun<true, true> rq(win + woff[2], (abs0[2] & 31) + ex[2], mine[2]);
-    BitRun<true, true> rc(win + woff[3], (abs0[3] & 31) + ex[3], mine[3]);
-    emit_cells(rh, rq, rc, h, qu, cn, code_huf, code_quad, code_rle);
+    // Huffman-coded symbols and quads; RLE counts Huffman or Exp-Golomb coded
+    OrRun<true> rh(win + woff[1], (u32)((abs0[1] & 31) + ex[1]));
+    OrRun<true> rq(win + woff[2], (u32)((abs0[2] & 31) + ex[2]));
+    OrRun<true> rc(win + woff[3], (u32)((abs0[3] & 31) + ex[3]));
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      if (h) rh.huff(code_huf(q));
+      if (qu) rq.huff(code_quad(q));
+      if (cn) {
+        if (eg) rc.code(code_rle(q));
+        else rc.huff(code_rle(q));
+      }
+    }
+    if (h) rh.finish();
+    if (qu) rq.finish();
+    if (cn) rc.finish();
   }
   if (g) {
-    BitRun<false, true> rg(win + woff[0], (abs0[0] & 31) + ex[0], mine[0]);
-    emit_gr(rg, code_gr);
+    OrRun<false> rg(win + woff[0], (u32)((abs0[0] & 31) + ex[0]));
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) rg.code(code_gr(q));
+    rg.finish();
   }
   __syncthreads();
   for (u32 i = threadIdx.x; i < wtot; i += kBlock) {

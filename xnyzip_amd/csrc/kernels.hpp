@@ -15,6 +15,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "exact_div.hpp"
 
 namespace xnyz {
 
@@ -77,6 +78,9 @@ struct QuantParams {
   float b;        // L2 bound (recover, diagnostics)
   float rec_div;  // float(sqrt(5)) or float(sqrt(3)) (recover)
   int to;         // 1: truncated octahedron, 0: cube
+  float inv;      // RN(1/div)      (div_const)
+  float rec_inv;  // RN(1/rec_div)  (div_const)
+  int fast;       // div_const_ok(div): divide by multiplication (exact_div.hpp)
 };
 
 struct KeyParams {
@@ -140,7 +144,6 @@ struct SegCtl {
   u64 sec_bit0[kNumSec];   // scan offset of the segment's first tile
 };
 
-constexpr u32 kMagBins = 68;  // bitlen(v) in 0..64, padded
 
 struct SegHist {  // per-segment symbol counts and first occurrences
   u32 sym[kSymBins], first_sym[kSymBins];
@@ -150,7 +153,6 @@ struct SegHist {  // per-segment symbol counts and first occurrences
   // bit b set, so sum(v >> k) = sum_{b >= k} bitpop[b] << (b - k) exactly,
   // for every Golomb-Rice k.
   u64 bitpop[64];
-  u32 mag[kMagBins];  // histogram of bitlen(v) over all repos values
 };
 
 // Per-segment statistics for the learned mode selector, from one pass over
