@@ -744,25 +744,59 @@ __global__ __launch_bounds__(kBlock) void k_scan_partials(T* part, u32 np, T* to
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
-template <typename T>
+// Up to this many partials, each apply workgroup sums the partials before
+// its own (one load per thread) instead of a separate kernel scanning them.
+constexpr u32 kScanDirect = 2048;
+
+// Exclusive scan of one tile of kScanTile values plus the tile's carry.  The
+// tile is loaded and stored wave-striped (coalesced) and transposed through
+// LDS, so that each thread scans kScanItems consecutive values.  kDirect: the
+// carry is the sum of partials [0, blockIdx.x) (np <= kScanDirect), and the
+// last workgroup writes the total.
+template <typename T, bool kDirect>
 __global__ __launch_bounds__(kBlock) void k_scan_apply(const T* in, T* out, u64 n,
-                                                       const T* part) {
+                                                       const T* part, u32 np, T* total) {
   __shared__ T sh[kWavesPerBlock];
-  const u64 base = (u64)blockIdx.x * kScanTile + (u64)threadIdx.x * kScanItems;
+  __shared__ T st[kScanTile + kScanTile / 16];  // one pad word per 16 values
+  const u64 base = (u64)blockIdx.x * kScanTile;
+  auto slot = [](u32 i) { return i + (i >> 4); };
+  T carry;
+  if (kDirect) {
+    T c = 0;
+    for (u32 i = threadIdx.x; i < blockIdx.x; i += kBlock) c += part[i];
+    carry = block_sum(c, sh);
+  } else {
+    carry = part[blockIdx.x];
+  }
   T v[kScanItems];
+#pragma unroll
+  for (u32 k = 0; k < kScanItems; ++k) {
+    const u64 i = base + k * kBlock + threadIdx.x;
+    v[k] = i < n ? in[i] : 0;
+  }
+#pragma unroll
+  for (u32 k = 0; k < kScanItems; ++k) st[slot(k * kBlock + threadIdx.x)] = v[k];
+  __syncthreads();
   T s = 0;
 #pragma unroll
   for (u32 k = 0; k < kScanItems; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0;
+    v[k] = st[slot(threadIdx.x * kScanItems + k)];
     s += v[k];
   }
   T tot;
-  T run = block_excl_scan(s, sh, tot) + part[blockIdx.x];
+  T run = block_excl_scan(s, sh, tot) + carry;  // synchronises: the reads are done
 #pragma unroll
   for (u32 k = 0; k < kScanItems; ++k) {
-    if (base + k < n) out[base + k] = run;
+    st[slot(threadIdx.x * kScanItems + k)] = run;
     run += v[k];
   }
+  __syncthreads();
+#pragma unroll
+  for (u32 k = 0; k < kScanItems; ++k) {
+    const u64 i = base + k * kBlock + threadIdx.x;
+    if (i < n) out[i] = st[slot(k * kBlock + threadIdx.x)];
+  }
+  if (kDirect && total && blockIdx.x + 1 == np && threadIdx.x == 0) *total = carry + tot;
 }
 
 template <typename T>
@@ -771,9 +805,14 @@ void scan_excl(const T* in, T* out, u64 n, T* total, Arena& ar, hipStream_t s) {
   T* part = ar.take<T>(np);
   k_scan_reduce<T><<<np, kBlock, 0, s>>>(in, n, part);
   LAUNCH_CHECK("k_scan_reduce");
+  if (np <= kScanDirect) {
+    k_scan_apply<T, true><<<np, kBlock, 0, s>>>(in, out, n, part, np, total);
+    LAUNCH_CHECK("k_scan_apply");
+    return;
+  }
   k_scan_partials<T><<<1, kBlock, 0, s>>>(part, np, total);
   LAUNCH_CHECK("k_scan_partials");
-  k_scan_apply<T><<<np, kBlock, 0, s>>>(in, out, n, part);
+  k_scan_apply<T, false><<<np, kBlock, 0, s>>>(in, out, n, part, np, nullptr);
   LAUNCH_CHECK("k_scan_apply");
 }
 
@@ -925,12 +964,16 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(Batch bt, const K* __r
     const u32 o = wstart + s * kWave + l;
     const bool valid = o < count;
     const u32 d = (u32)(k[s] >> shift) & (D - 1);
+    // peers = the lanes whose digit equals this lane's: for every bit b, the
+    // lanes that agree on it are ballot(bit) where the bit is set and its
+    // complement elsewhere, i.e. ~(ballot ^ m) with m all ones where the bit
+    // is set (one three-input bit operation per 32-bit half).
     u64 peers = __ballot(valid);
 #pragma unroll
     for (u32 b = 0; b < kBits; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const u64 bb = __ballot(bit);
-      peers &= bit ? bb : ~bb;
+      const u32 m = (u32)((int)(d << (31 - b)) >> 31);
+      const u64 bb = __ballot(m != 0);
+      peers &= ~(bb ^ (((u64)m << 32) | m));
     }
     const u32 rr = (u32)__popcll(peers & lt);
     const u32 prev = wc[w][d];
