@@ -1104,6 +1104,12 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     }
   }
   u32* cnt = any_rle ? ctx->cnt.as<u32>() : nullptr;
+  // Block starts (each block's first cell): with RLE cells, found while the
+  // cells are and kept in the sort's spare key buffer (free once the keys
+  // are sorted), since the cell starts still occupy `pos`; otherwise in `pos`.
+  auto block_starts = [&](int sorted) {
+    return any_rle ? ctx->keys[sorted ^ 1].as<u32>() : ctx->pos.as<u32>();
+  };
   int cur = 0;
   const u32* perm_dev = nullptr;
   auto stage2 = [&] {
@@ -1120,10 +1126,10 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     }
     KeyView k{ctx->keys[cur].as<u64>(), wide ? ctx->lo[cur].as<u32>() : nullptr};
     if (any_rle)
-      rle_cells(bt, k, ctx->pos.as<u32>(), ctx->cells.as<u64>(),
+      rle_cells(bt, k, ctx->pos.as<u32>(), block_starts(cur), ctx->cells.as<u64>(),
                 wide ? ctx->cells_lo.as<u32>() : nullptr, cnt, ar, st);
     if (any_rle) k = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
-    block_runs_and_stats(bt, k, any_direct, any_block, ctx->pos.as<u32>(), ar, st);
+    block_runs_and_stats(bt, k, any_direct, any_block, any_rle, block_starts(cur), ar, st);
   };
   if (ctx->graphs && !g_profiler && !co.noalloc) {
     // The launch sequence is a function of this signature alone (kernel
@@ -1332,7 +1338,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
 
   // ---- stage 3: section sizes -------------------------------------------------
   const SecPlan plan = plan_sections(bt, ar);
-  size_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ar, st);
+  size_sections(bt, kv, cnt, block_starts(cur), plan, ar, st);
   ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
   ctx->sync();
 
@@ -1423,7 +1429,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->copied();
   launch_patch(ctx->patch.as<Patch>(), (u32)patches.size(), ctx->patch.as<uint8_t>() + prec,
                reinterpret_cast<uint8_t*>(ow), st);
-  write_sections(bt, kv, cnt, ctx->pos.as<u32>(), plan, ow, any_block, any_direct, st);
+  write_sections(bt, kv, cnt, block_starts(cur), plan, ow, any_block, any_direct, st);
 
   // Fills co.stats (frame bytes from co.frames / co.dst_off) once the outputs
   // are in place.

@@ -1352,6 +1352,135 @@ void compact(const Batch& bt, const KV& k, u32* pos, int which, Arena& ar, hipSt
   ar.off = mark;
 }
 
+// Cell and block starts of the sorted points in one pass (batches with RLE
+// segments).  A point starts a cell where its key differs from the one before
+// it (every point of a plain segment is a cell of its own), and a block where
+// its block id does (block-mode segments); a block start is a cell start, so
+// the block's first cell is the cell rank of that point.  Results:
+// upos[cell] = its first point, bpos[block] = its first cell -- what
+// compact<kFlagCell> over the points and then compact<kFlagBlock> over the
+// cells give, without the second pass over the keys.  part[0, nt) counts the
+// cells of each tile, part[nt, 2 nt) its blocks.
+__device__ __forceinline__ void cell_block_flags(const KV& k, const Range& r, u32 first,
+                                                 bool all, bool blocks, const u32* o, bool* fc,
+                                                 bool* fb) {
+  const bool wide = k.wide;
+  u64 a0[kItems], a1[kItems];
+  u32 w0[kItems], w1[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const u32 j = r.first + (o[q] < r.count ? o[q] : r.count - 1), jp = j > first ? j - 1 : j;
+    a0[q] = k.a[j];
+    a1[q] = k.a[jp];
+    if (wide) {
+      w0[q] = k.b[j];
+      w1[q] = k.b[jp];
+    }
+  }
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const u32 j = r.first + (o[q] < r.count ? o[q] : r.count - 1);
+    const bool in = o[q] < r.count, head = j == first;
+    const bool hi_diff = (wide ? a0[q] : a0[q] >> 21) != (wide ? a1[q] : a1[q] >> 21);
+    const bool key_diff = a0[q] != a1[q] || (wide && w0[q] != w1[q]);
+    fc[q] = in && (all || head || key_diff);
+    fb[q] = in && blocks && (head || hi_diff);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_flag_count2(Batch bt, KV k, u32* part) {
+  __shared__ u32 sh[kWavesPerBlock];
+  const Tile t = bt.tiles[blockIdx.x];
+  const SegParam& sp = bt.sp[t.seg];
+  const Range r = tile_range(sp.p0, sp.n, t.local);
+  u32 cc = 0, cb = 0;
+  if (r.count) {  // uniform
+    const u32 first = r.first - (u32)((u64)t.local * kTile);
+    u32 o[kItems];
+    bool fc[kItems], fb[kItems];
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) o[q] = q * kBlock + threadIdx.x;
+    cell_block_flags(k, r, first, !sp.rle, !sp.kp.direct, o, fc, fb);
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      cc += fc[q] ? 1u : 0u;
+      cb += fb[q] ? 1u : 0u;
+    }
+  }
+  const u32 a = block_sum(cc, sh), b = block_sum(cb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = a;
+    part[bt.ntiles + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_flag_scatter2(Batch bt, KV k, const u32* part,
+                                                          u32* upos, u32* bpos) {
+  __shared__ u32 shc[kWavesPerBlock], shb[kWavesPerBlock];
+  const Tile t = bt.tiles[blockIdx.x];
+  const SegParam& sp = bt.sp[t.seg];
+  const Range r = tile_range(sp.p0, sp.n, t.local);
+  if (!r.count) return;  // uniform
+  const u32 first = r.first - (u32)((u64)t.local * kTile);
+  // Wavefront w owns elements [w*kWaveSub, (w+1)*kWaveSub) of the tile,
+  // visited 64 at a time (coalesced); ballots give the in-order ranks.
+  const u32 w = threadIdx.x / kWave, l = lane_id();
+  const u64 lt = (1ull << l) - 1;
+  const u32 wstart = w * kWaveSub;
+  u64 mc[kItems], mb[kItems];
+  u32 o[kItems], nc = 0, nb = 0;
+  bool fc[kItems], fb[kItems];
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) o[q] = wstart + q * kWave + l;
+  cell_block_flags(k, r, first, !sp.rle, !sp.kp.direct, o, fc, fb);
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    mc[q] = __ballot(fc[q]);
+    mb[q] = __ballot(fb[q]);
+    nc += (u32)__popcll(mc[q]);
+    nb += (u32)__popcll(mb[q]);
+  }
+  if (l == 0) {
+    shc[w] = nc;
+    shb[w] = nb;
+  }
+  __syncthreads();
+  // block ranks: the scan over [cells | blocks] puts every block after all
+  // cells; part[nt] is the cell total
+  u32 rc = part[blockIdx.x], rb = part[bt.ntiles + blockIdx.x] - part[bt.ntiles];
+#pragma unroll
+  for (u32 i = 0; i < (u32)kWavesPerBlock; ++i)
+    if (i < w) {
+      rc += shc[i];
+      rb += shb[i];
+    }
+#pragma unroll
+  for (u32 q = 0; q < kItems; ++q) {
+    const u32 c = rc + (u32)__popcll(mc[q] & lt);
+    if (fc[q]) upos[c] = r.first + o[q];
+    if (fb[q]) bpos[rb + (u32)__popcll(mb[q] & lt)] = c;
+    rc += (u32)__popcll(mc[q]);
+    rb += (u32)__popcll(mb[q]);
+  }
+}
+
+// Per-segment starts and counts of cells (e0, ne) and blocks (b0, nb) from
+// the scan over [cells | blocks] (total: its sum).
+__global__ void k_seg_starts2(Batch bt, const u32* part, const u32* total) {
+  const u32 nt = bt.ntiles, cells = part[nt];
+  for (u32 s = blockIdx.x * blockDim.x + threadIdx.x; s < bt.S; s += gridDim.x * blockDim.x) {
+    const SegParam& sp = bt.sp[s];
+    const bool last = s + 1 == bt.S;
+    const u32 c0 = part[sp.tile0], c1 = last ? cells : part[bt.sp[s + 1].tile0];
+    const u32 b0 = part[nt + sp.tile0] - cells;
+    const u32 b1 = (last ? *total : part[nt + bt.sp[s + 1].tile0]) - cells;
+    bt.sc[s].e0 = c0;
+    bt.sc[s].ne = c1 - c0;
+    bt.sc[s].b0 = b0;
+    bt.sc[s].nb = b1 - b0;
+  }
+}
+
 // RLE cells (block_compressor_rle.hpp:107-120 deduplicates with a hash map;
 // the stream only depends on the sorted cells and their counts).  Cells of
 // RLE segments only; counts below kCntBinsSeg go to the segment's histogram
@@ -2144,34 +2273,30 @@ struct BitRun {
   }
 };
 
-// A thread's run of bits in the LDS window, assembled without branches: the
-// word being filled is ORed into the window after every piece (its bits only
-// grow, so ORing it again is harmless) and the position moves on by selects.
-// The window starts zeroed, and neighbouring runs share their edge words
-// through the OR; finish() ORs the run's last word.
+// A thread's run of bits in the LDS window: every piece of at most 32 bits
+// is shifted to its bit position and ORed into the (at most two) window
+// words it touches, with no state but the position and no branches.  The
+// window starts zeroed; neighbouring runs share edge words through the OR.
 template <bool kMsb>
 struct OrRun {
   u32* words;
-  u32 cw;    // word being filled
-  u64 acc;   // MSB-first: word cw in acc's top half; LSB-first: in its bottom half
-  u32 fill;  // bits of word cw taken, 0..31 between pieces
+  u32 pos;  // bit position in the window
 
-  __device__ __forceinline__ OrRun(u32* w, u32 p) : words(w), cw(p >> 5), acc(0), fill(p & 31) {}
+  __device__ __forceinline__ OrRun(u32* w, u32 p) : words(w), pos(p) {}
 
-  // OR word cw into the window; move to the next word if cw is full.
-  __device__ __forceinline__ void step() {
-    atomicOr(&words[cw], kMsb ? (u32)(acc >> 32) : (u32)acc);
-    const bool full = fill >= 32;
-    cw += full ? 1u : 0u;
-    acc = full ? (kMsb ? acc << 32 : acc >> 32) : acc;
-    fill -= full ? 32u : 0u;
-  }
   // `n` <= 32 bits of `v` (its low bits).
   __device__ __forceinline__ void put(u32 v, u32 n) {
-    if (kMsb) acc |= n ? ((u64)v << (64 - n)) >> fill : 0ull;
-    else acc |= (u64)(n >= 32 ? v : v & ((1u << n) - 1u)) << fill;
-    fill += n;
-    step();
+    const u32 w = pos >> 5, sh = pos & 31;
+    if (kMsb) {
+      const u64 x = n ? ((u64)v << (64 - n)) >> sh : 0ull;
+      atomicOr(&words[w], (u32)(x >> 32));
+      atomicOr(&words[w + 1], (u32)x);
+    } else {
+      const u64 x = (u64)(n >= 32 ? v : v & ((1u << n) - 1u)) << sh;
+      atomicOr(&words[w], (u32)x);
+      atomicOr(&words[w + 1], (u32)(x >> 32));
+    }
+    pos += n;
   }
   // `n` <= 64 bits of `v`.
   __device__ __forceinline__ void put64(u64 v, u32 n) {
@@ -2184,12 +2309,6 @@ struct OrRun {
       put((u32)(v >> 32), nh);
     }
   }
-  // `n` <= 64 zero bits (MSB-first Exp-Golomb prefixes).
-  __device__ __forceinline__ void zeros(u32 n) {
-    fill += n;
-    step();
-    step();
-  }
   // `n` one bits (LSB-first Golomb-Rice quotients; any length).
   __device__ __forceinline__ void ones(u32 n) {
     while (n) {
@@ -2198,22 +2317,20 @@ struct OrRun {
       n -= t;
     }
   }
-  // The last word of the run (its bits past the last OR).
-  __device__ __forceinline__ void finish() {
-    atomicOr(&words[cw], kMsb ? (u32)(acc >> 32) : (u32)acc);
-  }
   // A Huffman code: a 32-bit value; a code longer than 32 bits (a tree deeper
   // than the reference's uint32_t prefix holds) is len - 32 zero bits, then
   // the value.
   __device__ __forceinline__ void huff(const Code& c) {
-    if (c.len > 32) put(0u, c.len - 32);
+    pos += c.len > 32 ? c.len - 32 : 0u;
     put((u32)c.val, c.len > 32 ? 32u : c.len);
   }
+  // MSB-first: `lead` zero bits; LSB-first: `lead` one bits; then the value.
   __device__ __forceinline__ void code(const Code& c) {
-    if (kMsb) zeros(c.lead);
+    if (kMsb) pos += c.lead;
     else ones(c.lead);
     put64(c.val, c.len);
   }
+  __device__ __forceinline__ void finish() {}
 };
 
 // Packs one section of a tile whose threads own kItems consecutive elements
@@ -2800,19 +2917,29 @@ int radix_sort_wide_v(const Batch& bt, u64* hi, u64* hi2, u64* lo64, u64* lo64b,
   return cur;
 }
 
-void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u64* ua, u32* ub, u32* cnt,
-               Arena& ar, hipStream_t s) {
+void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u32* bpos, u64* ua, u32* ub,
+               u32* cnt, Arena& ar, hipStream_t s) {
   const KV k{in.a, in.b, bt.wide};
-  compact<kFlagCell>(bt, k, upos, 0, ar, s);
+  const size_t mark = ar.off;
+  u32* part = ar.take<u32>(2ull * bt.ntiles);
+  u32* total = ar.take<u32>(1);
+  k_flag_count2<<<bt.ntiles, kBlock, 0, s>>>(bt, k, part);
+  LAUNCH_CHECK("k_flag_count2");
+  scan_excl<u32>(part, part, 2ull * bt.ntiles, total, ar, s);
+  k_flag_scatter2<<<bt.ntiles, kBlock, 0, s>>>(bt, k, part, upos, bpos);
+  LAUNCH_CHECK("k_flag_scatter2");
+  k_seg_starts2<<<grid_for(bt.S, kBlock), kBlock, 0, s>>>(bt, part, total);
+  LAUNCH_CHECK("k_seg_starts2");
+  ar.off = mark;
   k_rle_finish<<<persist_grid(bt), kBlock, 0, s>>>(bt, k, upos, ua, ub, cnt);
   LAUNCH_CHECK("k_rle_finish");
 }
 
 void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, bool block_any,
-                          u32* bpos, Arena& ar, hipStream_t s) {
+                          bool blocks_done, u32* bpos, Arena& ar, hipStream_t s) {
   (void)direct_any;
   const KV k{kv.a, kv.b, bt.wide};
-  if (block_any) compact<kFlagBlock>(bt, k, bpos, 1, ar, s);
+  if (block_any && !blocks_done) compact<kFlagBlock>(bt, k, bpos, 1, ar, s);
   k_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, k);
   LAUNCH_CHECK("k_stats");
   if (block_any) {
