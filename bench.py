@@ -133,14 +133,16 @@ def algo_bytes(kernel: str, n_points: int, n_coded: int, key_bytes: int, launche
         "k_keygen": 12 * n_points + per_pass_keys,    # read xyz, write keys
         "k_radix_hist": per_pass_keys,                # read keys
         "k_radix_scatter": 2 * per_pass_keys,         # read + write keys
-        "k_flag_count": 8 * n_points,
-        "k_flag_scatter": 8 * n_points + 4 * n_coded,
+        # cell and block starts: read the sorted keys, write the cell starts
+        # (the block starts, at most one per cell, are not counted)
+        "k_flag_count2": 8 * n_points,
+        "k_flag_scatter2": 8 * n_points + 4 * n_coded,
         "k_rle_finish": 4 * n_coded + 8 * n_coded + 4 * n_coded,
         "k_stats": 8 * n_coded,                       # read sorted cells
         "k_sec_bits": 8 * n_coded,
         # read the sorted cells, write the repos / quads / counts sections
         # (at most the whole stream: a lower bound on the time's worth)
-        "k_sec_write_cells": 8 * n_coded + out_bytes,
+        "k_sec_write<cells>": 8 * n_coded + out_bytes,
         "k_block_stats": 8 * n_coded,
         "k_zcand": dev_frame_bytes,
         "k_zparse": dev_frame_bytes,

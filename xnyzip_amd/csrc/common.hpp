@@ -1,5 +1,5 @@
 // common.hpp -- types shared by the host orchestration and the gfx950
-// kernels, plus wave64 / 256-thread block primitives and bit writers.
+// kernels, plus wave64 / 256-thread block primitives.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -102,65 +102,6 @@ __device__ __forceinline__ T block_sum(T v, T* sh) {
 }
 
 __device__ __forceinline__ u32 bswap32(u32 v) { return __builtin_bswap32(v); }
-
-// OR `len` (<= 64) bits of `val` into a big-endian-bit-order word array at
-// bit `p` (word bit 31 is the first bit of the word).
-template <bool kShared>
-__device__ __forceinline__ void or_word(u32* w, u32 v, bool be) {
-  if (!v) return;
-  if (kShared) {
-    atomicOr(w, v);
-  } else {
-    atomicOr(w, be ? bswap32(v) : v);
-  }
-}
-
-template <bool kShared>
-__device__ __forceinline__ void put_msb(u32* words, u64 p, u64 val, u32 len) {
-  if (!len) return;
-  const u64 t = val << (64 - len);  // left-aligned
-  const u32 o = (u32)(p & 31);
-  u32* w = words + (p >> 5);
-  const u64 hi = t >> o;
-  or_word<kShared>(w, (u32)(hi >> 32), true);
-  or_word<kShared>(w + 1, (u32)hi, true);
-  if (o) or_word<kShared>(w + 2, (u32)((t << (64 - o)) >> 32), true);
-}
-
-template <bool kShared>
-__device__ __forceinline__ void put_lsb(u32* words, u64 p, u64 val, u32 len) {
-  if (!len) return;
-  if (len < 64) val &= (1ull << len) - 1;
-  const u32 o = (u32)(p & 31);
-  u32* w = words + (p >> 5);
-  const u64 lo = val << o;
-  or_word<kShared>(w, (u32)lo, false);
-  or_word<kShared>(w + 1, (u32)(lo >> 32), false);
-  if (o) or_word<kShared>(w + 2, (u32)(val >> (64 - o)), false);
-}
-
-// A run of `n` one bits at bit `p`, LSB-first.
-template <bool kShared>
-__device__ __forceinline__ void put_ones_lsb(u32* words, u64 p, u64 n) {
-  while (n) {
-    const u32 o = (u32)(p & 31);
-    const u32 take = (u32)((n < (u64)(32 - o)) ? n : (u64)(32 - o));
-    const u32 mask = (take == 32 ? 0xFFFFFFFFu : ((1u << take) - 1u)) << o;
-    or_word<kShared>(words + (p >> 5), mask, false);
-    p += take;
-    n -= take;
-  }
-}
-
-template <bool kShared, bool kMsb>
-__device__ __forceinline__ void put_code(u32* words, u64 p, const Code& c) {
-  if (kMsb) {
-    put_msb<kShared>(words, p + c.lead, c.val, c.len);
-  } else {
-    put_ones_lsb<kShared>(words, p, c.lead);
-    put_lsb<kShared>(words, p + c.lead, c.val, c.len);
-  }
-}
 
 __device__ __forceinline__ u32 bitlen64(u64 v) { return v ? 64u - (u32)__clzll((i64)v) : 0u; }
 

@@ -71,7 +71,6 @@ __device__ __forceinline__ Range tile_range(u32 first, u32 count, u32 local) {
   return Range{first + (u32)f, (u32)(c < kTile ? c : kTile)};
 }
 
-enum Space { kPoints = 0, kCells = 1, kBlocks = 2 };
 
 // Reduction kernels run persistent workgroups: workgroup b walks the
 // contiguous tile span [b*T/G, (b+1)*T/G), accumulates in registers / LDS
@@ -127,13 +126,6 @@ __device__ __forceinline__ void max_to(u64* a, u64 v) {
 }
 __device__ __forceinline__ void max_to(int* a, int v) {
   if (v > __atomic_load_n(a, __ATOMIC_RELAXED)) atomicMax(a, v);
-}
-
-__device__ __forceinline__ Range stage_range(const SegParam& sp, const SegCtl& sc, u32 local,
-                                             Space sp_kind) {
-  if (sp_kind == kPoints) return tile_range(sp.p0, sp.n, local);
-  if (sp_kind == kCells) return tile_range(sc.e0, sc.ne, local);
-  return tile_range(sc.b0, sc.nb, local);
 }
 
 // ---------------------------------------------------------------------------
@@ -1764,19 +1756,6 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
   flush_stats(L, bt.sc[cur], bt.sh[cur], np, nqp, ps);
 }
 
-// Block-id and block-count deltas (block_compressor.hpp:146-147): u64
-// wrapping differences, the first value absolute.
-__device__ __forceinline__ u64 block_delta(const KV& k, const u32* bpos, const SegCtl& sc,
-                                           const KeyParams& kp, u32 b, int which) {
-  if (which == 0) {
-    const u64 id = id_of(k.hi(bpos[b]), kp);
-    return b > sc.b0 ? id - id_of(k.hi(bpos[b - 1]), kp) : id;
-  }
-  const u32 st = bpos[b];
-  const u64 c = (u64)((b + 1 < sc.b0 + sc.nb ? bpos[b + 1] : sc.e0 + sc.ne) - st);
-  return b > sc.b0 ? c - (u64)(st - bpos[b - 1]) : c;
-}
-
 __device__ __forceinline__ void flush_block_stats(SegCtl& sc, u64& m0, u64& m1, u64* sh) {
   const u64 a = block_max(m0, sh), b = block_max(m1, sh);
   if (threadIdx.x == 0) {
@@ -1804,8 +1783,10 @@ __global__ __launch_bounds__(kBlock) void k_block_stats(Batch bt, KV k, const u3
     const SegCtl& sc = bt.sc[t.seg];
     const Range r = tile_range(sc.b0, sc.nb, t.local);
     if (!r.count) continue;
-    // block_delta() of kGroup items at a time: block positions first, then
-    // their keys, each step's loads issued together
+    // Block-id and block-count deltas (block_compressor.hpp:146-147: u64
+    // wrapping differences, the first value absolute) of kGroup items at a
+    // time: block positions first, then their keys, each step's loads issued
+    // together
     const u32 b0 = sc.b0, bend = sc.b0 + sc.nb, cend = sc.e0 + sc.ne;
     for (u32 g0 = 0; g0 < kItems; g0 += kGroup) {
       u32 st[kGroup], pv[kGroup], nx[kGroup];
@@ -1842,18 +1823,13 @@ __global__ __launch_bounds__(kBlock) void k_block_stats(Batch bt, KV k, const u3
 }
 
 // ---------------------------------------------------------------------------
-// Sections.  coder<kind>() yields the Code of element j of a segment.
+// Sections: sizes (k_sec_bits), then the writers (k_sec_write_fixed for the
+// BitPacker sections, k_sec_write_cells for the variable-length ones).
 
 struct SecCtx {
   KV k;
   const u32* cnt;
   const u32* bpos;
-};
-
-template <u32 kKind>
-struct SecTraits {
-  static constexpr bool kMsb = (kKind == kSecHuf || kKind == kSecQuad || kKind == kSecRle);
-  static constexpr Space kSpace = (kKind == kSecBlk || kKind == kSecCnt) ? kBlocks : kCells;
 };
 
 template <u32 kKind>
@@ -1871,46 +1847,6 @@ __device__ __forceinline__ HufEntry big_code(const Batch& bt, const SegParam& sp
     if (bt.big_vals[mid] < c) lo = mid + 1; else hi = mid;
   }
   return bt.big_codes[lo < sp.big0 + sp.nbig ? lo : sp.big0];
-}
-
-template <u32 kKind>
-__device__ __forceinline__ Code coder(const Batch& bt, const SecCtx& x, const SegParam& sp,
-                                      const SegCtl& sc, const SegLut& lut, u32 j) {
-  const KV& k = x.k;
-  if (kKind == kSecBlk || kKind == kSecCnt) {  // BitPacker, LSB-first
-    const u64 v = block_delta(k, x.bpos, sc, sp.kp, j, kKind == kSecBlk ? 0 : 1);
-    return Code{v, kKind == kSecBlk ? sp.w1 : sp.w2, 0};
-  }
-  if (kKind == kSecRle) {
-    const u32 c = x.cnt[j];
-    if (sp.cnt_eg) {  // Exp-Golomb of zigzag(int)
-      const u64 z = (u64)c << 1;
-      const u32 kk = sp.egk;
-      const u64 n1 = (z >> kk) + 1;
-      const u32 nb = bitlen64(n1);
-      const u64 low = kk ? (__builtin_bitreverse64(z) >> (64 - kk)) : 0;
-      return Code{(n1 << kk) | low, nb + kk, nb - 1};
-    }
-    const HufEntry e = c < kCntBinsSeg ? lut.cnt[c] : big_code(bt, sp, c);
-    return Code{e.bits, e.len, 0};
-  }
-  u64 v;
-  u32 qd = 0;
-  if (sp.kp.direct) {
-    v = (j == sc.e0) ? k.a[j] : k.a[j] - k.a[j - 1];
-  } else {
-    lcp(k, j, sc.e0, v, qd);
-  }
-  if (kKind == kSecGR) {  // Golomb-Rice of the patches, LSB-first
-    if (v <= sp.lim) return Code{0, 0, 0};
-    return Code{(v & ((1ull << sp.grk) - 1)) << 1, sp.grk + 1, (u32)(v >> sp.grk)};
-  }
-  if (kKind == kSecHuf) {
-    const HufEntry e = lut.sym[v > sp.lim ? 0u : (u32)v + 1u];
-    return Code{e.bits, e.len, 0};
-  }
-  const HufEntry e = lut.quad[qd > 254 ? 0u : qd + 1u];  // kSecQuad
-  return Code{e.bits, e.len, 0};
 }
 
 constexpr u32 kVlWin = 8448;  // LDS words: 4096 codes of 64 bits plus the edges
@@ -1991,203 +1927,6 @@ __global__ void k_seg_sections(Batch bt, const u64* tile_off, const u64* total) 
     bt.sc[s].sec_bit0[q] = st;
     bt.sc[s].sec_bits[q] = en - st;
   }
-}
-
-// The codes of a thread's kItems elements (element o = wstart + q * kWave +
-// lane of the tile), as coder<kKind> gives them, computed in two steps:
-// every load an element needs (keys, counts, block positions) is issued for
-// all items first, with indices clamped into the range, and only then are the
-// codes formed -- one memory latency per step, not one per element.  `tab`
-// is the section's Huffman table, staged in LDS.
-template <u32 kKind>
-__device__ __forceinline__ void tile_codes(const Batch& bt, const SecCtx& x, const SegParam& sp,
-                                           const SegCtl& sc, const Range& r, u32 wstart, u32 l,
-                                           const HufEntry* tab, Code* cd) {
-  const KV& k = x.k;
-  const bool wide = k.wide;
-  const u32 last = r.count - 1;  // r.count >= 1
-  auto elem = [&](u32 q) { return wstart + q * kWave + l; };
-  auto idx = [&](u32 q) { return r.first + (elem(q) < r.count ? elem(q) : last); };
-  if (kKind == kSecBlk || kKind == kSecCnt) {
-    const u32 b0 = sc.b0, bend = sc.b0 + sc.nb, cend = sc.e0 + sc.ne;
-    u32 st[kItems], pv[kItems], nx[kItems];
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      const u32 b = idx(q);
-      st[q] = x.bpos[b];
-      pv[q] = x.bpos[b > b0 ? b - 1 : b];
-      if (kKind == kSecCnt) {
-        const u32 after = x.bpos[b + 1 < bend ? b + 1 : b];
-        nx[q] = b + 1 < bend ? after : cend;
-      }
-    }
-    if (kKind == kSecCnt) {
-#pragma unroll
-      for (u32 q = 0; q < kItems; ++q) {
-        const u32 b = idx(q);
-        const u64 c = (u64)(nx[q] - st[q]);
-        const u64 v = b > b0 ? c - (u64)(st[q] - pv[q]) : c;
-        cd[q] = elem(q) < r.count ? Code{v, sp.w2, 0} : Code{0, 0, 0};
-      }
-      return;
-    }
-    u64 h0[kItems], h1[kItems];
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      h0[q] = k.a[st[q]];
-      h1[q] = k.a[pv[q]];
-    }
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      const u32 b = idx(q);
-      const u64 id = id_of(wide ? h0[q] : h0[q] >> 21, sp.kp);
-      const u64 v = b > b0 ? id - id_of(wide ? h1[q] : h1[q] >> 21, sp.kp) : id;
-      cd[q] = elem(q) < r.count ? Code{v, sp.w1, 0} : Code{0, 0, 0};
-    }
-    return;
-  }
-  if (kKind == kSecRle) {
-    u32 c[kItems];
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) c[q] = x.cnt[idx(q)];
-    const bool eg = sp.cnt_eg != 0;
-    const u32 kk = sp.egk;
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) {
-      Code v;
-      if (eg) {  // Exp-Golomb of zigzag(int)
-        const u64 z = (u64)c[q] << 1;
-        const u64 n1 = (z >> kk) + 1;
-        const u32 nb = bitlen64(n1);
-        const u64 low = kk ? (__builtin_bitreverse64(z) >> (64 - kk)) : 0;
-        v = Code{(n1 << kk) | low, nb + kk, nb - 1};
-      } else {
-        const HufEntry e = c[q] < kCntBinsSeg ? tab[c[q]] : big_code(bt, sp, c[q]);
-        v = Code{e.bits, e.len, 0};
-      }
-      cd[q] = elem(q) < r.count ? v : Code{0, 0, 0};
-    }
-    return;
-  }
-  // repos / sfc deltas and quads (lcp), from the keys at j and j - 1
-  const u32 lim = sp.lim, grk = sp.grk;
-  u32 o[kItems], qd[kItems];
-  u64 v[kItems];
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) o[q] = elem(q);
-  lcp_items<kItems>(k, sp.kp.direct != 0, sc.e0, r.first, r.count, o, v, qd);
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    Code c;
-    if (kKind == kSecGR) {  // Golomb-Rice of the patches, LSB-first
-      c = v[q] <= lim ? Code{0, 0, 0}
-                      : Code{(v[q] & ((1ull << grk) - 1)) << 1, grk + 1, (u32)(v[q] >> grk)};
-    } else if (kKind == kSecHuf) {
-      const HufEntry e = tab[v[q] > lim ? 0u : (u32)v[q] + 1u];
-      c = Code{e.bits, e.len, 0};
-    } else {  // kSecQuad
-      const HufEntry e = tab[qd[q] > 254 ? 0u : qd[q] + 1u];
-      c = Code{e.bits, e.len, 0};
-    }
-    cd[q] = elem(q) < r.count ? c : Code{0, 0, 0};
-  }
-}
-
-// Phase B: scan one tile's codes of section kKind (cd: the thread's kItems
-// codes in wave-striped order) inside the tile, assemble the tile's bits in
-// LDS (MSB-first streams in big-endian word order), then store interior
-// words and OR the two edge words, which neighbouring tiles share.  A tile
-// whose span exceeds the LDS window writes to global memory.  Starts with a
-// barrier, so calls for several sections may follow one another on the same
-// window.
-template <u32 kKind>
-__device__ __forceinline__ void pack_tile(const SegParam& sp, const SegCtl& sc, u64 toff,
-                                          const Code* cd, u32* win, u64* sh, u32* out) {
-  constexpr bool kMsb = SecTraits<kKind>::kMsb;
-  const u32 w = threadIdx.x / kWave, l = lane_id();
-  // rel[q] = bit offset of the lane's q-th code in its wave: the kItems lane
-  // scans advance level by level, all items together
-  u64 rel[kItems];
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) rel[q] = (u64)cd[q].lead + cd[q].len;
-#pragma unroll
-  for (u32 o = 1; o < kWave; o <<= 1) {
-    u64 up[kItems];
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) up[q] = __shfl_up(rel[q], o, kWave);
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q)
-      if (l >= o) rel[q] += up[q];
-  }
-  u64 carry = 0;
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {  // inclusive -> exclusive, items in order
-    const u64 inc = rel[q];
-    rel[q] = carry + inc - ((u64)cd[q].lead + cd[q].len);
-    carry += __shfl(inc, kWave - 1, kWave);
-  }
-  __syncthreads();  // the window and sh of a previous section are free
-  if (l == 0) sh[w] = carry;
-  __syncthreads();
-  u64 ex = 0, tot = 0;
-#pragma unroll
-  for (u32 i = 0; i < (u32)kWavesPerBlock; ++i) {
-    if (i < w) ex += sh[i];
-    tot += sh[i];
-  }
-  if (tot == 0) return;
-  const u64 abs0 = sp.sec_byte[kKind] * 8 + (toff - sc.sec_bit0[kKind]);
-  const u64 w0 = abs0 >> 5;
-  const u32 o0 = (u32)(abs0 & 31);
-  const u64 nwords = (o0 + tot + 31) >> 5;
-  if (nwords <= kVlWin) {
-    for (u32 i = threadIdx.x; i < nwords; i += kBlock) win[i] = 0;
-    __syncthreads();
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) put_code<true, kMsb>(win, o0 + ex + rel[q], cd[q]);
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < nwords; i += kBlock) {
-      u32 v = win[i];
-      if (kMsb) v = bswap32(v);
-      if (i == 0 || i == nwords - 1) {
-        if (v) atomicOr(&out[w0 + i], v);
-      } else {
-        out[w0 + i] = v;
-      }
-    }
-  } else {
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) put_code<false, kMsb>(out, abs0 + ex + rel[q], cd[q]);
-  }
-}
-
-// One section's tile: its codes (tile_codes), then pack_tile.  Element order
-// inside the tile: wavefront w owns [w*kWaveSub, (w+1)*kWaveSub), taken 64
-// consecutive elements per step, so every load is coalesced.
-template <u32 kKind>
-__global__ __launch_bounds__(kBlock) void k_sec_write(Batch bt, SecCtx x, const u64* tile_off,
-                                                      u32* out) {
-  constexpr bool kTab = kKind == kSecHuf || kKind == kSecQuad || kKind == kSecRle;
-  static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
-  __shared__ u32 win[kVlWin];
-  __shared__ u64 sh[kWavesPerBlock];
-  __shared__ HufEntry tab[kTab ? kBlock : 1];
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  const SegCtl& sc = bt.sc[t.seg];
-  if (!sec_active<kKind>(sp, sc)) return;
-  const Range r = stage_range(sp, sc, t.local, SecTraits<kKind>::kSpace);
-  if (r.count == 0) return;
-  if (kTab) {
-    const SegLut& lut = bt.lut[t.seg];
-    tab[threadIdx.x] = kKind == kSecHuf ? lut.sym[threadIdx.x]
-                       : kKind == kSecQuad ? lut.quad[threadIdx.x] : lut.cnt[threadIdx.x];
-    __syncthreads();
-  }
-  const u32 w = threadIdx.x / kWave, l = lane_id();
-  Code cd[kItems];
-  tile_codes<kKind>(bt, x, sp, sc, r, w * kWaveSub, l, tab, cd);
-  pack_tile<kKind>(sp, sc, tile_off[blockIdx.x], cd, win, sh, out);
 }
 
 // A thread's run of bits inside a section: its codes are consecutive in the
@@ -2333,59 +2072,6 @@ struct OrRun {
   __device__ __forceinline__ void finish() {}
 };
 
-// Packs one section of a tile whose threads own kItems consecutive elements
-// each ("blocked": thread t codes elements 16t .. 16t + 15, so its codes are
-// one run of the stream).  code(q) gives the thread's q-th code ({0,0,0}
-// past the range).  The runs are placed by a block scan of their lengths,
-// assembled in the LDS window (or straight into the output when the tile's
-// span exceeds it), and the window's words stored, its two edge words ORed.
-// Starts with a barrier (inside the scan), so calls for several sections may
-// follow one another on the same window.
-template <u32 kKind, typename CodeFn>
-__device__ __forceinline__ void pack_blocked(const SegParam& sp, const SegCtl& sc, u64 toff,
-                                             CodeFn code, u32* win, u64* sh, u32* out) {
-  constexpr bool kMsb = SecTraits<kKind>::kMsb;
-  u64 mine = 0;
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    const Code c = code(q);
-    mine += (u64)c.lead + c.len;
-  }
-  u64 tot;
-  const u64 ex = block_excl_scan(mine, sh, tot);
-  if (tot == 0) return;
-  const u64 abs0 = sp.sec_byte[kKind] * 8 + (toff - sc.sec_bit0[kKind]);
-  const u64 w0 = abs0 >> 5;
-  const u32 o0 = (u32)(abs0 & 31);
-  const u64 nwords = (o0 + tot + 31) >> 5;
-  if (nwords <= kVlWin) {
-    for (u32 i = threadIdx.x; i < nwords; i += kBlock) win[i] = 0;
-    __syncthreads();
-    if (mine) {
-      BitRun<kMsb, true> run(win, o0 + ex, mine);
-#pragma unroll
-      for (u32 q = 0; q < kItems; ++q) run.code(code(q));
-      run.finish();
-    }
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < nwords; i += kBlock) {
-      u32 v = win[i];
-      if (kMsb) v = bswap32(v);
-      if (i == 0 || i == nwords - 1) {
-        if (v) atomicOr(&out[w0 + i], v);
-      } else {
-        out[w0 + i] = v;
-      }
-    }
-    __syncthreads();  // the window is free for the next section
-  } else if (mine) {
-    BitRun<kMsb, false> run(out, abs0 + ex, mine);
-#pragma unroll
-    for (u32 q = 0; q < kItems; ++q) run.code(code(q));
-    run.finish();
-  }
-}
-
 // The four cell-space sections (Golomb-Rice patches, Huffman symbols,
 // quads, RLE counts) of one tile in one pass.  Each thread owns 16
 // consecutive elements, so its codes form one run of each section.  The
@@ -2404,22 +2090,6 @@ __device__ __forceinline__ void pack_blocked(const SegParam& sp, const SegCtl& s
 __device__ __forceinline__ u32 stage_slot(u32 s) { return s + (s >> 4); }
 constexpr u32 kStageWords = kTile + 1 + (kTile + 1) / 16 + 1;  // one 32-bit array
 constexpr u32 kCellLds = 2 * kStageWords > kVlWin ? 2 * kStageWords : kVlWin;
-
-// Emits every element's codes of the Huffman-coded sections (symbols,
-// quads, RLE counts) into three runs.
-template <typename R1, typename R2, typename R3, typename F1, typename F2, typename F3>
-__device__ __forceinline__ void emit_cells(R1& rh, R2& rq, R3& rc, bool h, bool qu, bool c,
-                                           F1 fh, F2 fq, F3 fc) {
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    if (h) rh.code(fh(q));
-    if (qu) rq.code(fq(q));
-    if (c) rc.code(fc(q));
-  }
-  if (h) rh.finish();
-  if (qu) rq.finish();
-  if (c) rc.finish();
-}
 
 // The thread's Golomb-Rice run: only patches have codes.
 template <typename R, typename F>
@@ -2719,22 +2389,89 @@ __global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
   }
 }
 
-// Profiler / rocprof-visible names of the section writers.
-template <u32 kKind>
-constexpr const char* sec_write_name() {
-  return kKind == kSecBlk    ? "k_sec_write<blk>"
-         : kKind == kSecCnt  ? "k_sec_write<cnt>"
-         : kKind == kSecGR   ? "k_sec_write<gr>"
-         : kKind == kSecHuf  ? "k_sec_write<huf>"
-         : kKind == kSecQuad ? "k_sec_write<quad>"
-                             : "k_sec_write<rle>";
-}
-
-template <u32 kKind>
-void write_one(const Batch& bt, const SecCtx& x, const SecPlan& p, u32* out, hipStream_t s) {
-  k_sec_write<kKind><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off + (u64)kKind * bt.ntiles,
-                                                  out);
-  LAUNCH_CHECK(sec_write_name<kKind>());
+// The two BitPacker sections of a tile of blocks (block-id deltas and
+// block-count deltas, block_compressor.hpp:146-147, bitpacking_encoder.hpp:
+// 17-51) in one pass.  Every delta of a section has the section's width w,
+// so element e of the tile lies at bits [o0 + e w, o0 + (e + 1) w) of the
+// tile's span, LSB-first: the deltas are staged in LDS and each output word
+// gathers the (at most 32 / w + 2) deltas that overlap it -- no scan and no
+// atomics but on the span's two edge words, which the neighbouring tiles or
+// the section's meta share.
+__global__ __launch_bounds__(kBlock) void k_sec_write_fixed(Batch bt, SecCtx x, u32* out) {
+  __shared__ u64 vals[kTile];
+  const Tile t = bt.tiles[blockIdx.x];
+  const SegParam& sp = bt.sp[t.seg];
+  const SegCtl& sc = bt.sc[t.seg];
+  if (sp.kp.direct) return;  // no blocks in direct mode
+  const Range r = tile_range(sc.b0, sc.nb, t.local);
+  if (r.count == 0) return;
+  const KV& k = x.k;
+  const u32* bpos = x.bpos;
+  // the deltas of every element (o = it * kBlock + thread), block
+  // positions first, then their keys, each group's loads issued together
+  const u32 b0 = sc.b0, bend = sc.b0 + sc.nb, cend = sc.e0 + sc.ne;
+  u64 db[kItems], dc[kItems];
+  for (u32 g0 = 0; g0 < kItems; g0 += kGroup) {
+    u32 st[kGroup], pv[kGroup], nx[kGroup];
+    u64 h0[kGroup], h1[kGroup];
+#pragma unroll
+    for (u32 it = 0; it < kGroup; ++it) {
+      const u32 o = (g0 + it) * kBlock + threadIdx.x;
+      const u32 b = r.first + (o < r.count ? o : r.count - 1);
+      st[it] = bpos[b];
+      pv[it] = bpos[b > b0 ? b - 1 : b];
+      const u32 after = bpos[b + 1 < bend ? b + 1 : b];
+      nx[it] = b + 1 < bend ? after : cend;
+    }
+#pragma unroll
+    for (u32 it = 0; it < kGroup; ++it) {
+      h0[it] = k.a[st[it]];
+      h1[it] = k.a[pv[it]];
+    }
+#pragma unroll
+    for (u32 it = 0; it < kGroup; ++it) {
+      const u32 b = r.first + (g0 + it) * kBlock + threadIdx.x;
+      const u64 id = id_of(k.wide ? h0[it] : h0[it] >> 21, sp.kp);
+      db[g0 + it] = b > b0 ? id - id_of(k.wide ? h1[it] : h1[it] >> 21, sp.kp) : id;
+      const u64 cc = (u64)(nx[it] - st[it]);
+      dc[g0 + it] = b > b0 ? cc - (u64)(st[it] - pv[it]) : cc;
+    }
+  }
+#pragma unroll
+  for (u32 sec = 0; sec < 2; ++sec) {
+    if (sec) __syncthreads();  // the first section's words are gathered
+#pragma unroll
+    for (u32 it = 0; it < kItems; ++it) {
+      const u32 o = it * kBlock + threadIdx.x;
+      if (o < r.count) vals[o] = sec ? dc[it] : db[it];
+    }
+    __syncthreads();
+    const u32 kind = sec ? kSecCnt : kSecBlk;
+    const u32 w = sec ? sp.w2 : sp.w1;
+    const u64 mask = w >= 64 ? ~0ull : (1ull << w) - 1;
+    const u64 abs0 = sp.sec_byte[kind] * 8 + (u64)(r.first - b0) * w;
+    const u32 o0 = (u32)(abs0 & 31);
+    u32* dst = out + (abs0 >> 5);
+    const u32 nwords = (u32)((o0 + (u64)r.count * w + 31) >> 5);
+    for (u32 kw = threadIdx.x; kw < nwords; kw += kBlock) {
+      // tile bits [32 kw, 32 kw + 32); element e starts at o0 + e w
+      const u32 lo = 32 * kw;
+      const u32 e0 = lo > o0 ? (lo - o0) / w : 0u;
+      u32 e1 = (lo + 31 - o0) / w;
+      e1 = e1 < r.count - 1 ? e1 : r.count - 1;
+      u32 word = 0;
+      for (u32 e = e0; e <= e1; ++e) {
+        const u64 v = vals[e] & mask;
+        const u32 p = o0 + e * w;
+        word |= p >= lo ? (u32)(v << (p - lo)) : (u32)(v >> (lo - p));
+      }
+      if (kw == 0 || kw + 1 == nwords) {
+        if (word) atomicOr(&dst[kw], word);
+      } else {
+        dst[kw] = word;
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -2973,8 +2710,8 @@ void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u3
                     hipStream_t s) {
   const SecCtx x{KV{kv.a, kv.b, bt.wide}, cnt, bpos};
   if (any_block) {
-    write_one<kSecBlk>(bt, x, p, out_words, s);
-    write_one<kSecCnt>(bt, x, p, out_words, s);
+    k_sec_write_fixed<<<bt.ntiles, kBlock, 0, s>>>(bt, x, out_words);
+    LAUNCH_CHECK("k_sec_write<blk,cnt>");
     if (bt.wide) k_sec_write_cells<false, true><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
     else k_sec_write_cells<false, false><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
     LAUNCH_CHECK("k_sec_write<cells>");
