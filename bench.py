@@ -473,6 +473,7 @@ def decode_stream(X, ctx, args, w: dict, n: int, stream) -> dict:
         ach = ab / (ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": "k_lcp_apply", "achieved": round(ach, 1),
                 "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                "traffic": measured_traffic("k_lcp_apply", args.workload, n),
                 "algo_bytes_per_launch": ab, "avg_launch_ms": round(ms, 4)}
     print(f"[bench] decode: {12.0 * m / dt / 1e9:.2f} GB/s of points ({dt * 1e3:.1f} ms/call), "
           f"{verified}", file=sys.stderr, flush=True)
