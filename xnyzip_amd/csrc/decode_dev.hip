@@ -642,15 +642,29 @@ struct LcpConst {
   u32 h_st;
 };
 
+// The tile's points are assembled in LDS and stored with coalesced writes
+// (each thread's 16 consecutive elements would otherwise store 192-byte runs
+// at a 192-byte lane stride, writing partial lines); RLE tiles stage their
+// cells' output offsets and counts the same way.  LDS is reused phase by
+// phase: the block scan, the offsets and counts, the points.
+constexpr u32 kLOut = kLTile;  // points staged per tile (the rest stored directly)
+constexpr u32 kLSlots = kLTile + kLTile / 16;  // staged offsets / counts, one pad per 16
+constexpr u32 kLWords = 3 * kLOut > 2 * kLSlots ? 3 * kLOut : 2 * kLSlots;
+static_assert(kLWords * 4 >= kBlock * sizeof(LState), "the scan fits the staging area");
+
+__device__ __forceinline__ u32 lslot(u32 i) { return i + (i >> 4); }
+
 __global__ __launch_bounds__(kBlock) void k_lcp_apply(LcpArgs A, const LState* tiles,
                                                       HilbertInv hi, LcpConst K) {
-  __shared__ LState sh[kBlock];
+  __shared__ u32 lds[kLWords];
   __shared__ u8 ap[24 * 8], nx_[24 * 8];
   for (u32 i = threadIdx.x; i < 24 * 8; i += kBlock) {
     ap[i] = hi.apply[i];
     nx_[i] = hi.next[i];
   }
-  const u64 b0 = (u64)blockIdx.x * kLTile + (u64)threadIdx.x * kLItems;
+  const u64 tf = (u64)blockIdx.x * kLTile;  // the tile's first element
+  const u32 tc = (u32)(A.n - tf < kLTile ? A.n - tf : kLTile);
+  const u64 b0 = tf + (u64)threadIdx.x * kLItems;
   LBytes lb;
   LState acc = lzero();
   if (b0 < A.n) {
@@ -660,109 +674,143 @@ __global__ __launch_bounds__(kBlock) void k_lcp_apply(LcpArgs A, const LState* t
       if (b0 + i < A.n) acc = lcombine(acc, lelem_b(A, lb, i));
   }
   LState tot;
-  LState run = lcombine(tiles[blockIdx.x], block_excl(acc, sh, tot));
-  if (b0 >= A.n) return;
-  // Four items at a time: their prefix states, then every gathered load
-  // (block id, patch sums, RLE offset and count) in flight together, then
-  // the points.
-  constexpr u32 kG = 4;  // one word of each byte array
-  for (u32 g = 0; g < kLItems; g += kG) {
-    const u32 sw = word_of(lb.sym, g / 4), hw = word_of(lb.head, g / 4),
-              qw = word_of(lb.quad, g / 4);
-    LState rs[kG];
-#pragma unroll
-    for (u32 k = 0; k < kG; ++k) {
-      if (b0 + g + k < A.n)
-        run = lcombine(run, lelem_v(A, (sw >> (8 * k)) & 0xFFu, (hw >> (8 * k)) & 0xFFu,
-                                    (qw >> (8 * k)) & 0xFFu));
-      rs[k] = run;
+  LState run = lcombine(tiles[blockIdx.x], block_excl(acc, reinterpret_cast<LState*>(lds), tot));
+  // Output points of the tile: [at0, at0 + span); RLE cells' offsets relative
+  // to at0 and their counts (clamped to the output) staged in LDS.
+  u64 at0 = tf, span = tc;
+  u32 rel[kLItems], cnt[kLItems];
+  if (A.rle) {
+    at0 = A.out_at[tf];
+    const u64 at1 = tf + tc < A.n ? A.out_at[tf + tc] : A.total;
+    span = (at1 < A.total ? at1 : A.total) - (at0 < A.total ? at0 : A.total);
+    u32* srel = lds;
+    u32* scnt = lds + kLSlots;
+    for (u32 i = threadIdx.x; i < tc; i += kBlock) {
+      const u64 a = A.out_at[tf + i], c = A.counts[tf + i];
+      const u64 room = a < A.total ? A.total - a : 0;
+      srel[lslot(i)] = (u32)(a - at0);
+      scnt[lslot(i)] = (u32)(c < room ? c : room);
     }
-    u64 id[kG], p1[kG], p2[kG], at[kG], cn[kG];
+    __syncthreads();
 #pragma unroll
-    for (u32 k = 0; k < kG; ++k) {
-      const u64 j = b0 + g + k;
-      const bool in = j < A.n;
-      p1[k] = A.ps[rs[k].nz < A.np ? rs[k].nz : A.np];
-      if (!A.direct) {
-        id[k] = in && rs[k].nb ? A.blkst[rs[k].nb - 1] : 0;
-        p2[k] = A.ps[rs[k].zsr < A.np ? rs[k].zsr : A.np];
-      }
-      if (A.rle) {
-        at[k] = in ? A.out_at[j] : 0;
-        cn[k] = in ? A.counts[j] : 0;
-      }
+    for (u32 i = 0; i < kLItems; ++i) {
+      const u32 e = threadIdx.x * kLItems + i;
+      rel[i] = e < tc ? srel[lslot(e)] : 0u;
+      cnt[i] = e < tc ? scnt[lslot(e)] : 0u;
     }
+  } else {
 #pragma unroll
-    for (u32 k = 0; k < kG; ++k) {
-      const u64 j = b0 + g + k;
-      if (j >= A.n) continue;
-      const LState& r = rs[k];
-      u64 cx, cy, cz;
-      if (A.direct) {  // block_decompressor.hpp:82-101
-        const u64 sfc = r.a + p1[k];
-        const u64 m = A.hilbert ? h2m(sfc, A.bits, ap, nx_) : sfc;
-        cx = compact3(m);
-        cy = compact3(m >> 1);
-        cz = compact3(m >> 2);
-      } else {  // block_quantizer.hpp:166-203
-        const u64 q1 = div_by(id[k], K.dx), q2 = div_by(q1, K.dy);
-        const u64 pbx = ((id[k] - q1 * A.nx) * 2) * 64, pby = ((q1 - q2 * A.ny) * 2) * 64,
-                  pbz = (q2 * 2) * 64;
-        const u64 rj = r.a + (p1[k] - p2[k]);
-        u64 m = rj;
-        if (A.hilbert) {
-          if (rj < (1ull << 18)) {  // six live digits from the precomputed state
-            u32 st = K.h_st;
-            m = K.h_hi;
+    for (u32 i = 0; i < kLItems; ++i) {
+      rel[i] = threadIdx.x * kLItems + i;
+      cnt[i] = 1;
+    }
+  }
+  __syncthreads();  // the staging area takes the points from here on
+  float* sout = reinterpret_cast<float*>(lds);
+  if (b0 < A.n) {
+    // Four items at a time: their prefix states, then every gathered load
+    // (block id, patch sums) in flight together, then the points.
+    constexpr u32 kG = 4;  // one word of each byte array
+    for (u32 g = 0; g < kLItems; g += kG) {
+      const u32 sw = word_of(lb.sym, g / 4), hw = word_of(lb.head, g / 4),
+                qw = word_of(lb.quad, g / 4);
+      LState rs[kG];
 #pragma unroll
-            for (int i = 5; i >= 0; --i) {
-              const u32 hd = (u32)(rj >> (3 * i)) & 7u;
-              m |= (u64)ap[st * 8 + hd] << (3 * i);
-              st = nx_[st * 8 + hd];
+      for (u32 k = 0; k < kG; ++k) {
+        if (b0 + g + k < A.n)
+          run = lcombine(run, lelem_v(A, (sw >> (8 * k)) & 0xFFu, (hw >> (8 * k)) & 0xFFu,
+                                      (qw >> (8 * k)) & 0xFFu));
+        rs[k] = run;
+      }
+      u64 id[kG], p1[kG], p2[kG];
+#pragma unroll
+      for (u32 k = 0; k < kG; ++k) {
+        const u64 j = b0 + g + k;
+        const bool in = j < A.n;
+        p1[k] = A.ps[rs[k].nz < A.np ? rs[k].nz : A.np];
+        if (!A.direct) {
+          id[k] = in && rs[k].nb ? A.blkst[rs[k].nb - 1] : 0;
+          p2[k] = A.ps[rs[k].zsr < A.np ? rs[k].zsr : A.np];
+        }
+      }
+#pragma unroll
+      for (u32 k = 0; k < kG; ++k) {
+        const u64 j = b0 + g + k;
+        if (j >= A.n) continue;
+        const LState& r = rs[k];
+        u64 cx, cy, cz;
+        if (A.direct) {  // block_decompressor.hpp:82-101
+          const u64 sfc = r.a + p1[k];
+          const u64 m = A.hilbert ? h2m(sfc, A.bits, ap, nx_) : sfc;
+          cx = compact3(m);
+          cy = compact3(m >> 1);
+          cz = compact3(m >> 2);
+        } else {  // block_quantizer.hpp:166-203
+          const u64 q1 = div_by(id[k], K.dx), q2 = div_by(q1, K.dy);
+          const u64 pbx = ((id[k] - q1 * A.nx) * 2) * 64, pby = ((q1 - q2 * A.ny) * 2) * 64,
+                    pbz = (q2 * 2) * 64;
+          const u64 rj = r.a + (p1[k] - p2[k]);
+          u64 m = rj;
+          if (A.hilbert) {
+            if (rj < (1ull << 18)) {  // six live digits from the precomputed state
+              u32 st = K.h_st;
+              m = K.h_hi;
+#pragma unroll
+              for (int i = 5; i >= 0; --i) {
+                const u32 hd = (u32)(rj >> (3 * i)) & 7u;
+                m |= (u64)ap[st * 8 + hd] << (3 * i);
+                st = nx_[st * 8 + hd];
+              }
+            } else {
+              m = h2m(rj, 20, ap, nx_);
             }
-          } else {
-            m = h2m(rj, 20, ap, nx_);
           }
+          cx = pbx + (u64)(r.q & 1) * 64 + compact3(m);
+          cy = pby + (u64)((r.q >> 1) & 1) * 64 + compact3(m >> 1);
+          cz = pbz + (u64)((r.q >> 2) & 1) * 64 + compact3(m >> 2);
         }
-        cx = pbx + (u64)(r.q & 1) * 64 + compact3(m);
-        cy = pby + (u64)((r.q >> 1) & 1) * 64 + compact3(m >> 1);
-        cz = pbz + (u64)((r.q >> 2) & 1) * 64 + compact3(m >> 2);
-      }
-      int n3[3] = {(int)cx, (int)cy, (int)cz};
-      float v[3];
-      if (A.to) {  // truncated_octahedron_quantizer.hpp:57-76
-        if (n3[2] & 1) {
-          n3[0] = (n3[0] << 1) - 1;
-          n3[1] = (n3[1] << 1) - 1;
-        } else {
-          n3[0] <<= 1;
-          n3[1] <<= 1;
+        int n3[3] = {(int)cx, (int)cy, (int)cz};
+        float v[3];
+        if (A.to) {  // truncated_octahedron_quantizer.hpp:57-76
+          if (n3[2] & 1) {
+            n3[0] = (n3[0] << 1) - 1;
+            n3[1] = (n3[1] << 1) - 1;
+          } else {
+            n3[0] <<= 1;
+            n3[1] <<= 1;
+          }
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            v[a] = div_const((float(n3[a]) * 2.0f) * A.b, A.rec_div, A.rec_inv);
+        } else {  // cube_quantizer.hpp:29-38
+#pragma unroll
+          for (int a = 0; a < 3; ++a)
+            v[a] = div_const(((float(n3[a]) + 0.5f) * 2.0f) * A.b, A.rec_div, A.rec_inv);
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = div_const((float(n3[a]) * 2.0f) * A.b, A.rec_div, A.rec_inv);
-      } else {  // cube_quantizer.hpp:29-38
-#pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = div_const(((float(n3[a]) + 0.5f) * 2.0f) * A.b, A.rec_div, A.rec_inv);
-      }
-#pragma unroll
-      for (int a = 0; a < 3; ++a) v[a] = v[a] + A.off[a];  // shifting.hpp:39-50
-      if (!A.rle) {
-        float* o = A.out + 3 * j;
-        o[0] = v[0];
-        o[1] = v[1];
-        o[2] = v[2];
-      } else {  // block_decompressor_rle.hpp:173-187: count copies, in cell order
-        if (at[k] >= A.total) continue;
-        float* o = A.out + 3 * at[k];
-        const u64 c = cn[k] < A.total - at[k] ? cn[k] : A.total - at[k];
-        for (u64 t = 0; t < c; ++t) {
-          o[3 * t] = v[0];
-          o[3 * t + 1] = v[1];
-          o[3 * t + 2] = v[2];
+        for (int a = 0; a < 3; ++a) v[a] = v[a] + A.off[a];  // shifting.hpp:39-50
+        // block_decompressor_rle.hpp:173-187: count copies, in cell order
+        // (one point without RLE)
+        for (u32 t = 0; t < cnt[g + k]; ++t) {
+          const u32 o = rel[g + k] + t;
+          if (o < kLOut) {
+            sout[3 * o] = v[0];
+            sout[3 * o + 1] = v[1];
+            sout[3 * o + 2] = v[2];
+          } else {
+            float* d = A.out + 3 * (at0 + o);
+            d[0] = v[0];
+            d[1] = v[1];
+            d[2] = v[2];
+          }
         }
       }
     }
   }
+  __syncthreads();
+  const u32 staged = 3 * (u32)(span < kLOut ? span : kLOut);
+  float* dst = A.out + 3 * at0;
+  for (u32 i = threadIdx.x; i < staged; i += kBlock) dst[i] = sout[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_h2m(const u64* in, u64 n, u32 L, HilbertInv hi,
