@@ -313,36 +313,60 @@ __global__ __launch_bounds__(kBlock) void k_pd_write(const uint8_t* __restrict__
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nsub) return;
   u64 p = start[t], o = base[t];
-  const u32 n = cnt[t];
+  const u32 n = o < count ? (u32)(cnt[t] < count - o ? cnt[t] : count - o) : 0u;
   WinOf<kKind> W(s, c.base_bit + (p < c.bits ? p : 0));
-  // Byte outputs go out four at a time on 4-byte boundaries (the thread's
-  // first and last partial words byte by byte): a thread's values are
-  // consecutive, and one store per value would touch a cache line per lane.
-  u32 pack = 0;
-  for (u32 i = 0; i < n && o < count; ++i, ++o) {
-    u64 v = 0;
+  // A thread's values are consecutive in the output: they are decoded kBatch
+  // at a time into registers and stored together, so the lines a thread
+  // writes are complete while still in L2 (one store per decoded value left
+  // every lane's line partly written for a long time, and with ~10^6 threads
+  // the partial lines were evicted and written back repeatedly).  Byte
+  // outputs: the values up to the first 4-byte boundary are stored byte by
+  // byte (that word is shared with the previous thread), then batches start
+  // on word boundaries and store whole words, the last partial word byte by
+  // byte.
+  constexpr u32 kBatch = 16;
+  auto decode_one = [&](u64& v) {
     p = next_code_w<kKind>(W, c, lut, p, v);
     if constexpr (kKind == kCodeHuf)
       if (!c.direct_values) v = c.values[v];
     if (c.zigzag) v = (u64)(i64)((v >> 1) ^ (~(v & 1) + 1));
+  };
+  u32 i0 = 0;
+  if constexpr (sizeof(T) == 1) {
+    const u32 head = (4u - (u32)(o & 3)) & 3u;
+    for (; i0 < head && i0 < n; ++i0) {
+      u64 v = 0;
+      decode_one(v);
+      out[o + i0] = (T)v;
+    }
+  }
+  for (; i0 < n; i0 += kBatch) {
+    const u32 m = n - i0 < kBatch ? n - i0 : kBatch;
+    u64 v[kBatch];
+#pragma unroll
+    for (u32 k = 0; k < kBatch; ++k) {
+      v[k] = 0;
+      if (k < m) decode_one(v[k]);
+    }
+    T* dst = out + o + i0;  // 4-byte aligned for byte outputs
     if constexpr (sizeof(T) == 1) {
-      const u32 sh = 8u * (u32)(o & 3);
-      pack |= ((u32)v & 0xFFu) << sh;
-      const u64 w0 = o & ~3ull;
-      const bool last = i + 1 == n || o + 1 == count;
-      if ((o & 3) == 3 || last) {
-        // whole word inside this thread's range: one store; else the bytes
-        const u64 first = base[t];
-        if (w0 >= first && (o & 3) == 3) {
-          *reinterpret_cast<u32*>(out + w0) = pack;
+#pragma unroll
+      for (u32 q = 0; q < kBatch / 4; ++q) {
+        if (4 * q + 3 < m) {
+          reinterpret_cast<u32*>(dst)[q] = ((u32)v[4 * q] & 0xFFu) |
+                                           (((u32)v[4 * q + 1] & 0xFFu) << 8) |
+                                           (((u32)v[4 * q + 2] & 0xFFu) << 16) |
+                                           (((u32)v[4 * q + 3] & 0xFFu) << 24);
         } else {
-          for (u64 b = (w0 > first ? w0 : first); b <= o; ++b)
-            out[b] = (T)(pack >> (8u * (u32)(b & 3)));
+#pragma unroll
+          for (u32 b = 0; b < 4; ++b)
+            if (4 * q + b < m) dst[4 * q + b] = (T)v[4 * q + b];
         }
-        pack = 0;
       }
     } else {
-      out[o] = (T)v;
+#pragma unroll
+      for (u32 k = 0; k < kBatch; ++k)
+        if (k < m) dst[k] = (T)v[k];
     }
   }
 }
@@ -368,13 +392,19 @@ void decode_section_t(const uint8_t* src, const CodeDesc& c, u64 count, T* out, 
   k_pd_spec<kKind><<<g, kBlock, 0, s>>>(src, c, nsub, start, ex[0], cnt);
   LAUNCH_CHECK("k_pd_spec");
   int cur = 0;
-  for (u64 round = 0;; ++round) {
-    if (round > nsub + 1) throw Error(-7, "decoder: sub-sequences did not settle");
-    k_fill_const<u32><<<1, 64, 0, s>>>(flag, 1, 0u);
-    LAUNCH_CHECK("k_fill_const");
-    k_pd_sync<kKind><<<g, kBlock, 0, s>>>(src, c, nsub, start, ex[cur], ex[cur ^ 1], cnt, flag);
-    LAUNCH_CHECK("k_pd_sync");
-    cur ^= 1;
+  // Rounds go in pairs between looks at the flag (the second round of a
+  // settled pair costs a launch: every sub-sequence's entry already matches),
+  // halving the host round trips; the flag reports the pair's second round.
+  for (u64 round = 0;; round += 2) {
+    if (round > nsub + 2) throw Error(-7, "decoder: sub-sequences did not settle");
+    for (int r = 0; r < 2; ++r) {
+      k_fill_const<u32><<<1, 64, 0, s>>>(flag, 1, 0u);
+      LAUNCH_CHECK("k_fill_const");
+      k_pd_sync<kKind><<<g, kBlock, 0, s>>>(src, c, nsub, start, ex[cur], ex[cur ^ 1], cnt,
+                                            flag);
+      LAUNCH_CHECK("k_pd_sync");
+      cur ^= 1;
+    }
     check_hip(hipMemcpyAsync(w.host_flag, flag, 4, hipMemcpyDeviceToHost, s), "flag D2H");
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (g_profiler) g_profiler->mark(nullptr, s);  // the next kernel starts a new interval
