@@ -382,6 +382,7 @@ struct xnyz_ctx {
   std::vector<hipEvent_t> slice_ev;  // copy-back slices (frame overlap)
   hipStream_t copy_stream = nullptr;  // copy-back beside device-built frame blocks
   hipEvent_t ev_pipe = nullptr;       // end of the stream-building work
+  hipEvent_t dec_ev[5] = {};          // decoding: upload pieces landed; [4] prior work done
   hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
@@ -2029,7 +2030,35 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   const double t0 = now_ms();
   ctx->h_dflag.ensure(64);
   ctx->dstream.ensure(blen + 64);
-  check_hip(hipMemcpyAsync(ctx->dstream.p, buf, blen, hipMemcpyHostToDevice, st), "stream H2D");
+  // The stream crosses PCIe in pieces, in stream order, on the copy stream:
+  // header and BitPacker sections, Golomb-Rice patches, Huffman symbols, the
+  // rest.  The sections are decoded in that order, each waiting only for the
+  // piece that holds it, so decoding overlaps the upload of what follows.
+  if (!ctx->copy_stream)
+    check_hip(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
+  for (hipEvent_t& e : ctx->dec_ev)
+    if (!e) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+  check_hip(hipEventRecord(ctx->dec_ev[4], st), "hipEventRecord");  // the stream buffer is free
+  check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->dec_ev[4], 0), "hipStreamWaitEvent");
+  const u64 cuts[4] = {direct ? s_gr.meta_at : s_cnt.data_at + s_cnt.data_len,
+                       s_gr.data_at + s_gr.data_len, s_huf.data_at + s_huf.data_len, blen};
+  for (int i = 0; i < 4; ++i) {
+    const u64 a0 = i ? cuts[i - 1] : 0, a1 = cuts[i] > a0 ? cuts[i] : a0;
+    if (a1 > a0)
+      check_hip(hipMemcpyAsync(ctx->dstream.as<uint8_t>() + a0, buf + a0, a1 - a0,
+                               hipMemcpyHostToDevice, ctx->copy_stream),
+                "stream H2D");
+    check_hip(hipEventRecord(ctx->dec_ev[i], ctx->copy_stream), "hipEventRecord");
+  }
+  auto arrived = [&](int piece) {
+    check_hip(hipStreamWaitEvent(st, ctx->dec_ev[piece], 0), "hipStreamWaitEvent");
+  };
+  // The host stream (the caller's, or the decompressed frame) must outlive
+  // the copies, also when decoding stops early with an error.
+  struct UploadDone {
+    hipStream_t s;
+    ~UploadDone() { (void)hipStreamSynchronize(s); }
+  } upload_done{ctx->copy_stream};
   ctx->copied();
   launch_zero_u8(ctx->dstream.as<uint8_t>() + blen, 64, st);
   // code tables of the sections, one buffer
@@ -2080,24 +2109,12 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   const uint8_t* dS = ctx->dstream.as<uint8_t>();
   DecScratch ws{ctx->h_dflag.as<u32>()};
 
-  // ---- sections ---------------------------------------------------------------
-  ctx->dsym.ensure(num + 16);
-  decode_section<uint8_t>(dS, hs.d, num, ctx->dsym.as<uint8_t>(), ws, ar, st);
-  ctx->dps.ensure((P + 1) * 8 + 16);
-  u64* dps = ctx->dps.as<u64>();
-  if (P) {
-    ctx->dpatch.ensure(P * 8 + 16);
-    decode_section<u64>(dS, gr.d, P, ctx->dpatch.as<u64>(), ws, ar, st);
-    scan_excl_u64(ctx->dpatch.as<u64>(), dps, P, dps + P, ar, st);
-  } else {
-    launch_zero_u8(reinterpret_cast<uint8_t*>(dps), 8, st);
-  }
+  // ---- sections, in stream order ---------------------------------------------------
   const u64* blkst = nullptr;
   if (!direct) {
-    ctx->dquad.ensure(num + 16);
-    decode_section<uint8_t>(dS, hq.d, num, ctx->dquad.as<uint8_t>(), ws, ar, st);
     // block ids and counts: prefix sums of the BitPacker deltas
     // (block_decompressor.hpp:163-165), then the first element of each block
+    arrived(0);
     ctx->dblk.ensure(B * 8 * 4 + 64);
     u64* bid = ctx->dblk.as<u64>();
     u64* bcnt = bid + B;
@@ -2120,6 +2137,24 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     launch_zero_u8(ctx->dhead.as<uint8_t>(), num, st);
     launch_mark_heads(bstart, B, ctx->dhead.as<uint8_t>(), st);
     blkst = bid;
+  }
+  arrived(1);
+  ctx->dps.ensure((P + 1) * 8 + 16);
+  u64* dps = ctx->dps.as<u64>();
+  if (P) {
+    ctx->dpatch.ensure(P * 8 + 16);
+    decode_section<u64>(dS, gr.d, P, ctx->dpatch.as<u64>(), ws, ar, st);
+    scan_excl_u64(ctx->dpatch.as<u64>(), dps, P, dps + P, ar, st);
+  } else {
+    launch_zero_u8(reinterpret_cast<uint8_t*>(dps), 8, st);
+  }
+  arrived(2);
+  ctx->dsym.ensure(num + 16);
+  decode_section<uint8_t>(dS, hs.d, num, ctx->dsym.as<uint8_t>(), ws, ar, st);
+  arrived(3);
+  if (!direct) {
+    ctx->dquad.ensure(num + 16);
+    decode_section<uint8_t>(dS, hq.d, num, ctx->dquad.as<uint8_t>(), ws, ar, st);
   }
   u64 total_pts = num;
   const u64* out_at = nullptr;
@@ -2313,6 +2348,8 @@ int xnyz_destroy(xnyz_ctx* ctx) {
     if (ctx->ev_pipe) (void)hipEventDestroy(ctx->ev_pipe);
     if (ctx->ev_sizes) (void)hipEventDestroy(ctx->ev_sizes);
     for (hipEvent_t e : ctx->aev)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->dec_ev)
       if (e) (void)hipEventDestroy(e);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx->fs;
