@@ -382,7 +382,7 @@ struct xnyz_ctx {
   std::vector<hipEvent_t> slice_ev;  // copy-back slices (frame overlap)
   hipStream_t copy_stream = nullptr;  // copy-back beside device-built frame blocks
   hipEvent_t ev_pipe = nullptr;       // end of the stream-building work
-  hipEvent_t dec_ev[5] = {};          // decoding: upload pieces landed; [4] prior work done
+  hipEvent_t dec_ev[4] = {};          // decoding: upload pieces landed; [3] prior work done
   hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
@@ -2030,20 +2030,21 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   const double t0 = now_ms();
   ctx->h_dflag.ensure(64);
   ctx->dstream.ensure(blen + 64);
-  // The stream crosses PCIe in pieces, in stream order, on the copy stream:
-  // header and BitPacker sections, Golomb-Rice patches, Huffman symbols, the
-  // rest.  The sections are decoded in that order, each waiting only for the
-  // piece that holds it, so decoding overlaps the upload of what follows.
+  // The stream crosses PCIe in pieces on the copy stream, the Golomb-Rice
+  // patches first (their decoding takes longest), then the sections after
+  // them (Huffman symbols, quads, RLE counts), then the header and the
+  // BitPacker sections; each section's decoding waits only for the piece that
+  // holds it, so decoding overlaps the upload of what follows.
   if (!ctx->copy_stream)
     check_hip(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
   for (hipEvent_t& e : ctx->dec_ev)
     if (!e) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-  check_hip(hipEventRecord(ctx->dec_ev[4], st), "hipEventRecord");  // the stream buffer is free
-  check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->dec_ev[4], 0), "hipStreamWaitEvent");
-  const u64 cuts[4] = {direct ? s_gr.meta_at : s_cnt.data_at + s_cnt.data_len,
-                       s_gr.data_at + s_gr.data_len, s_huf.data_at + s_huf.data_len, blen};
-  for (int i = 0; i < 4; ++i) {
-    const u64 a0 = i ? cuts[i - 1] : 0, a1 = cuts[i] > a0 ? cuts[i] : a0;
+  check_hip(hipEventRecord(ctx->dec_ev[3], st), "hipEventRecord");  // the stream buffer is free
+  check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->dec_ev[3], 0), "hipStreamWaitEvent");
+  const u64 gr0 = s_gr.data_at, gr1 = s_gr.data_at + s_gr.data_len;
+  const u64 pieces[3][2] = {{gr0, gr1}, {gr1, blen}, {0, gr0}};
+  for (int i = 0; i < 3; ++i) {
+    const u64 a0 = pieces[i][0], a1 = pieces[i][1];
     if (a1 > a0)
       check_hip(hipMemcpyAsync(ctx->dstream.as<uint8_t>() + a0, buf + a0, a1 - a0,
                                hipMemcpyHostToDevice, ctx->copy_stream),
@@ -2109,12 +2110,46 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   const uint8_t* dS = ctx->dstream.as<uint8_t>();
   DecScratch ws{ctx->h_dflag.as<u32>()};
 
-  // ---- sections, in stream order ---------------------------------------------------
+  // ---- sections, in upload order ---------------------------------------------------
+  arrived(0);
+  ctx->dps.ensure((P + 1) * 8 + 16);
+  u64* dps = ctx->dps.as<u64>();
+  if (P) {
+    ctx->dpatch.ensure(P * 8 + 16);
+    decode_section<u64>(dS, gr.d, P, ctx->dpatch.as<u64>(), ws, ar, st);
+    scan_excl_u64(ctx->dpatch.as<u64>(), dps, P, dps + P, ar, st);
+  } else {
+    launch_zero_u8(reinterpret_cast<uint8_t*>(dps), 8, st);
+  }
+  arrived(1);
+  ctx->dsym.ensure(num + 16);
+  decode_section<uint8_t>(dS, hs.d, num, ctx->dsym.as<uint8_t>(), ws, ar, st);
+  if (!direct) {
+    ctx->dquad.ensure(num + 16);
+    decode_section<uint8_t>(dS, hq.d, num, ctx->dquad.as<uint8_t>(), ws, ar, st);
+  }
+  u64 total_pts = num;
+  const u64* out_at = nullptr;
+  const u64* counts = nullptr;
+  if (rle) {  // counts, then each cell's first output point
+    ctx->dcnt.ensure(num * 16 + 64);
+    u64* c = ctx->dcnt.as<u64>();
+    u64* at2 = c + num;
+    decode_section<u64>(dS, hc.d, num, c, ws, ar, st);
+    u64* total = ar.take<u64>(1);
+    scan_excl_u64(c, at2, num, total, ar, st);
+    check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
+    ctx->sync();
+    std::memcpy(&total_pts, ws.host_flag, 8);
+    if (total_pts >= (1ull << 40)) throw Error(XNYZ_EUNSUPPORTED, "RLE counts beyond 2^40 points");
+    out_at = at2;
+    counts = c;
+  }
+  arrived(2);
   const u64* blkst = nullptr;
   if (!direct) {
     // block ids and counts: prefix sums of the BitPacker deltas
     // (block_decompressor.hpp:163-165), then the first element of each block
-    arrived(0);
     ctx->dblk.ensure(B * 8 * 4 + 64);
     u64* bid = ctx->dblk.as<u64>();
     u64* bcnt = bid + B;
@@ -2137,41 +2172,6 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     launch_zero_u8(ctx->dhead.as<uint8_t>(), num, st);
     launch_mark_heads(bstart, B, ctx->dhead.as<uint8_t>(), st);
     blkst = bid;
-  }
-  arrived(1);
-  ctx->dps.ensure((P + 1) * 8 + 16);
-  u64* dps = ctx->dps.as<u64>();
-  if (P) {
-    ctx->dpatch.ensure(P * 8 + 16);
-    decode_section<u64>(dS, gr.d, P, ctx->dpatch.as<u64>(), ws, ar, st);
-    scan_excl_u64(ctx->dpatch.as<u64>(), dps, P, dps + P, ar, st);
-  } else {
-    launch_zero_u8(reinterpret_cast<uint8_t*>(dps), 8, st);
-  }
-  arrived(2);
-  ctx->dsym.ensure(num + 16);
-  decode_section<uint8_t>(dS, hs.d, num, ctx->dsym.as<uint8_t>(), ws, ar, st);
-  arrived(3);
-  if (!direct) {
-    ctx->dquad.ensure(num + 16);
-    decode_section<uint8_t>(dS, hq.d, num, ctx->dquad.as<uint8_t>(), ws, ar, st);
-  }
-  u64 total_pts = num;
-  const u64* out_at = nullptr;
-  const u64* counts = nullptr;
-  if (rle) {  // counts, then each cell's first output point
-    ctx->dcnt.ensure(num * 16 + 64);
-    u64* c = ctx->dcnt.as<u64>();
-    u64* at2 = c + num;
-    decode_section<u64>(dS, hc.d, num, c, ws, ar, st);
-    u64* total = ar.take<u64>(1);
-    scan_excl_u64(c, at2, num, total, ar, st);
-    check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
-    ctx->sync();
-    std::memcpy(&total_pts, ws.host_flag, 8);
-    if (total_pts >= (1ull << 40)) throw Error(XNYZ_EUNSUPPORTED, "RLE counts beyond 2^40 points");
-    out_at = at2;
-    counts = c;
   }
   if (n_points) *n_points = total_pts;
   float* dst = d_out;
