@@ -382,7 +382,7 @@ struct xnyz_ctx {
   std::vector<hipEvent_t> slice_ev;  // copy-back slices (frame overlap)
   hipStream_t copy_stream = nullptr;  // copy-back beside device-built frame blocks
   hipEvent_t ev_pipe = nullptr;       // end of the stream-building work
-  hipEvent_t dec_ev[4] = {};          // decoding: upload pieces landed; [3] prior work done
+  hipEvent_t dec_ev[5] = {};          // decoding: upload pieces landed; [4] prior work done
   hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
   CachedGraph stage2;
@@ -2030,20 +2030,22 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   const double t0 = now_ms();
   ctx->h_dflag.ensure(64);
   ctx->dstream.ensure(blen + 64);
-  // The stream crosses PCIe in pieces on the copy stream, the Golomb-Rice
-  // patches first (their decoding takes longest), then the sections after
-  // them (Huffman symbols, quads, RLE counts), then the header and the
-  // BitPacker sections; each section's decoding waits only for the piece that
-  // holds it, so decoding overlaps the upload of what follows.
+  // The stream crosses PCIe in pieces on the copy stream: the Golomb-Rice
+  // patches first (their decoding takes longest), the sections after them
+  // (Huffman symbols, quads, RLE counts), the block counts, and last the
+  // header and block ids, which only the final pass needs.  Each section's
+  // decoding waits only for the piece that holds it, so decoding overlaps the
+  // upload of what follows.
   if (!ctx->copy_stream)
     check_hip(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking), "hipStreamCreate");
   for (hipEvent_t& e : ctx->dec_ev)
     if (!e) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-  check_hip(hipEventRecord(ctx->dec_ev[3], st), "hipEventRecord");  // the stream buffer is free
-  check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->dec_ev[3], 0), "hipStreamWaitEvent");
+  check_hip(hipEventRecord(ctx->dec_ev[4], st), "hipEventRecord");  // the stream buffer is free
+  check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->dec_ev[4], 0), "hipStreamWaitEvent");
   const u64 gr0 = s_gr.data_at, gr1 = s_gr.data_at + s_gr.data_len;
-  const u64 pieces[3][2] = {{gr0, gr1}, {gr1, blen}, {0, gr0}};
-  for (int i = 0; i < 3; ++i) {
+  const u64 cn0 = direct ? gr0 : s_cnt.data_at;
+  const u64 pieces[4][2] = {{gr0, gr1}, {gr1, blen}, {cn0, gr0}, {0, cn0}};
+  for (int i = 0; i < 4; ++i) {
     const u64 a0 = pieces[i][0], a1 = pieces[i][1];
     if (a1 > a0)
       check_hip(hipMemcpyAsync(ctx->dstream.as<uint8_t>() + a0, buf + a0, a1 - a0,
@@ -2145,20 +2147,17 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     out_at = at2;
     counts = c;
   }
-  arrived(2);
-  const u64* blkst = nullptr;
+  // block counts (block_decompressor.hpp:163-165: prefix sums of the
+  // BitPacker deltas), then the first element of each block
+  u64* bid = nullptr;
   if (!direct) {
-    // block ids and counts: prefix sums of the BitPacker deltas
-    // (block_decompressor.hpp:163-165), then the first element of each block
+    arrived(2);
     ctx->dblk.ensure(B * 8 * 4 + 64);
-    u64* bid = ctx->dblk.as<u64>();
+    bid = ctx->dblk.as<u64>();
     u64* bcnt = bid + B;
     u64* tmp = bcnt + B;
     u64* bstart = tmp + B;
-    launch_bitunpack(dS, s_blk.data_at * 8, w1, B, bid, st);
     launch_bitunpack(dS, s_cnt.data_at * 8, w2, B, bcnt, st);
-    scan_excl_u64(bid, tmp, B, nullptr, ar, st);
-    launch_add_u64(bid, tmp, B, st);
     scan_excl_u64(bcnt, tmp, B, nullptr, ar, st);
     launch_add_u64(bcnt, tmp, B, st);
     u64* total = ar.take<u64>(1);
@@ -2171,7 +2170,6 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     ctx->dhead.ensure(num + 16);
     launch_zero_u8(ctx->dhead.as<uint8_t>(), num, st);
     launch_mark_heads(bstart, B, ctx->dhead.as<uint8_t>(), st);
-    blkst = bid;
   }
   if (n_points) *n_points = total_pts;
   float* dst = d_out;
@@ -2206,11 +2204,21 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   a.ps = dps;
   a.np = P;
   a.total = total_pts;
-  a.blkst = blkst;
   a.out_at = out_at;
   a.counts = counts;
   a.out = dst;
-  launch_lcp(a, ctx->hinv, ar.take<char>(lcp_scratch_bytes(num)), st);
+  char* lcp_scratch = ar.take<char>(lcp_scratch_bytes(num));
+  launch_lcp_scan(a, lcp_scratch, st);
+  // block ids: prefix sums of their deltas (block_decompressor.hpp:163-165)
+  arrived(3);
+  if (!direct) {
+    u64* tmp = bid + 2 * B;
+    launch_bitunpack(dS, s_blk.data_at * 8, w1, B, bid, st);
+    scan_excl_u64(bid, tmp, B, nullptr, ar, st);
+    launch_add_u64(bid, tmp, B, st);
+    a.blkst = bid;
+  }
+  launch_lcp_apply(a, ctx->hinv, lcp_scratch, st);
   if (host_out) {
     uint8_t* p = xmalloc(total_pts * 12 + 16);
     host_out->data = p;

@@ -232,7 +232,8 @@ __global__ __launch_bounds__(kBlock) void k_pd_spec(const uint8_t* __restrict__ 
   const u32* lut = stage_lut<kKind>(c, lsh);
   const u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nsub) return;
-  const u64 b = t * kSub, end = b + kSub < c.bits ? b + kSub : c.bits;
+  constexpr u64 kS = kKind == kCodeGR ? kSubGR : kSub;
+  const u64 b = t * kS, end = b + kS < c.bits ? b + kS : c.bits;
   u64 p = b, v;
   u32 n = 0;
   WinOf<kKind> W(s, c.base_bit + p);
@@ -270,7 +271,8 @@ __global__ __launch_bounds__(kBlock) void k_pd_sync(const uint8_t* __restrict__ 
     exit_out[t] = old_exit;
     return;
   }
-  const u64 b = t * kSub, end = b + kSub < c.bits ? b + kSub : c.bits;
+  constexpr u64 kS = kKind == kCodeGR ? kSubGR : kSub;
+  const u64 b = t * kS, end = b + kS < c.bits ? b + kS : c.bits;
   u64 p = entry, q = cur, v;
   u32 np = 0, nq = 0;
   WinOf<kKind> WP(s, c.base_bit + (p < end ? p : 0)), WQ(s, c.base_bit + (q < end ? q : 0));
@@ -381,7 +383,8 @@ __global__ __launch_bounds__(kBlock) void k_fill_const(T* out, u64 n, T v) {
 template <u32 kKind, typename T>
 void decode_section_t(const uint8_t* src, const CodeDesc& c, u64 count, T* out, DecScratch& w,
                       Arena& ar, hipStream_t s) {
-  const u64 nsub = (c.bits + kSub - 1) / kSub;
+  constexpr u64 kS = kKind == kCodeGR ? kSubGR : kSub;
+  const u64 nsub = (c.bits + kS - 1) / kS;
   if (nsub >= 0xFFFFFFFFull) throw Error(-5, "entropy-coded section too long");
   const u32 g = blocks_for(nsub, kBlock);
   u64* start = ar.take<u64>(nsub);
@@ -922,7 +925,7 @@ void launch_h2m(const u64* in, u64 n, u32 iteration, const HilbertInv& hi, u64* 
 
 size_t lcp_scratch_bytes(u64 n) { return (size_t)blocks_for(n, kLTile) * sizeof(LState) + 256; }
 
-void launch_lcp(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream_t s) {
+void launch_lcp_scan(const LcpArgs& a, void* scratch, hipStream_t s) {
   if (!a.n) return;
   const u32 nt = blocks_for(a.n, kLTile);
   LState* tiles = static_cast<LState*>(scratch);
@@ -930,6 +933,12 @@ void launch_lcp(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream
   LAUNCH_CHECK("k_lcp_reduce");
   k_lcp_tiles<<<1, kBlock, 0, s>>>(tiles, nt);
   LAUNCH_CHECK("k_lcp_tiles");
+}
+
+void launch_lcp_apply(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream_t s) {
+  if (!a.n) return;
+  const u32 nt = blocks_for(a.n, kLTile);
+  LState* tiles = static_cast<LState*>(scratch);
   LcpConst K{div_u64(a.nx), div_u64(a.ny), 0, 0};
   for (int i = 20; i >= 6; --i) {  // hilbertToMorton3's loop over zero digits
     K.h_hi |= (u64)hi.apply[K.h_st * 8] << (3 * i);

@@ -10,7 +10,11 @@ namespace xnyz {
 
 enum CodeKind : u32 { kCodeHuf = 0, kCodeEG = 1, kCodeGR = 2, kCodeConst = 3 };
 
-constexpr u32 kSub = 512;      // bits per sub-sequence of the parallel decode
+constexpr u32 kSub = 512;      // bits per sub-sequence of the parallel decode (the shortest)
+// Golomb-Rice codes (unary quotient, then k remainder bits) fall into step
+// slowly after a wrong start -- their lengths hardly differ -- so their
+// sub-sequences are longer: fewer boundaries to settle, in fewer rounds.
+constexpr u32 kSubGR = 2048;
 constexpr u32 kLutBits = 11;   // Huffman first-level lookup: at most 2^11 entries
 constexpr u32 kTreeNull = 0xFFFFFFFFu;
 constexpr u32 kTreeLeaf = 0x80000000u;
@@ -80,9 +84,12 @@ struct LcpArgs {
   const u64* counts;        // RLE: copies of each cell
   float* out;               // points, x y z interleaved
 };
-// Scratch: lcp_scratch_bytes(n) bytes.
+// Scratch: lcp_scratch_bytes(n) bytes.  The scan (the per-tile prefix
+// states) needs the symbols, quads and block-start flags; the apply pass
+// also the block ids, patch sums and RLE offsets, and the scan's scratch.
 size_t lcp_scratch_bytes(u64 n);
-void launch_lcp(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream_t s);
+void launch_lcp_scan(const LcpArgs& a, void* scratch, hipStream_t s);
+void launch_lcp_apply(const LcpArgs& a, const HilbertInv& hi, void* scratch, hipStream_t s);
 // out[i] = hilbertToMorton3(in[i], iteration) for n codes (device arrays).
 void launch_h2m(const u64* in, u64 n, u32 iteration, const HilbertInv& hi, u64* out,
                 hipStream_t s);
