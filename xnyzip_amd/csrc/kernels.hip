@@ -425,6 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_quantize_nodes(Batch bt, const F3* _
   const SegCtl& sc = bt.sc[t.seg];
   const Range r = tile_range(sp.p0, sp.n, t.local);
   const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
+  const QuantParams qp = sp.qp;  // in registers: the stores may alias the table
 #pragma unroll 4
   for (u32 k = 0; k < kItems; ++k) {
     const u32 o = k * kBlock + threadIdx.x;
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(kBlock) void k_quantize_nodes(Batch bt, const F3* _
       const u32 i = r.first + o;
       float sx, sy, sz;
       int X, Y, Z;
-      quant_point(p[i], off, sp.qp, sx, sy, sz, X, Y, Z);
+      quant_point(p[i], off, qp, sx, sy, sz, X, Y, Z);
       nodes[3ull * i] = X;
       nodes[3ull * i + 1] = Y;
       nodes[3ull * i + 2] = Z;
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_keygen(Batch bt, const F3* __restric
   const SegCtl& sc = bt.sc[tl.seg];
   const Range r = tile_range(sp.p0, sp.n, tl.local);
   const KeyParams kp = sp.kp;
+  const QuantParams qp = sp.qp;  // in registers: the key stores may alias the table
   const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
   if (!r.count) return;
   for (u32 k0 = 0; k0 < kItems; k0 += kGroup) {
@@ -539,7 +541,7 @@ __global__ __launch_bounds__(kBlock) void k_keygen(Batch bt, const F3* __restric
         const u32 i = r.first + o;
         float sx, sy, sz;
         int X, Y, Z;
-        quant_point(pt[kk], off, sp.qp, sx, sy, sz, X, Y, Z);
+        quant_point(pt[kk], off, qp, sx, sy, sz, X, Y, Z);
         if (idx) idx[i] = i;  // point index payload (permutation output)
         u64 hi;
         u32 k21;
@@ -649,22 +651,24 @@ __global__ __launch_bounds__(kBlock) void k_features(Batch bt, const F3* __restr
     const SegCtl& sc = bt.sc[t.seg];
     const Range r = tile_range(sp.p0, sp.n, t.local);
     const float off[3] = {sc.offset[0], sc.offset[1], sc.offset[2]};
+    const QuantParams qp = sp.qp;  // in registers: the atomics may alias the table
+    const u32 p0 = sp.p0;
 #pragma unroll 4
     for (u32 k = 0; k < kItems; ++k) {
       const u32 o = k * kBlock + threadIdx.x;
       if (o < r.count) {
         const u32 i = r.first + o;
         int X, Y, Z;
-        const u64 m = node_morton(p[i], off, sp.qp, X, Y, Z);
+        const u64 m = node_morton(p[i], off, qp, X, Y, Z);
         const u64 cell = ((u64)(u32)X << 42) ^ ((u64)(u32)Y << 21) ^ (u64)(u32)Z;
         const u64 sub = ((u64)(u32)(X >> 6) << 42) ^ ((u64)(u32)(Y >> 6) << 21) ^ (u32)(Z >> 6);
         const u64 blk = ((u64)(u32)(X >> 7) << 42) ^ ((u64)(u32)(Y >> 7) << 21) ^ (u32)(Z >> 7);
         hll_add(L.cell, cell, 1);
         hll_add(L.sub, sub, 2);
         hll_add(L.blk, blk, 3);
-        if (i > sp.p0) {
+        if (i > p0) {
           int PX, PY, PZ;
-          const u64 pm = node_morton(p[i - 1], off, sp.qp, PX, PY, PZ);
+          const u64 pm = node_morton(p[i - 1], off, qp, PX, PY, PZ);
           const u64 x = m ^ pm;
           const u32 bl = bitlen64(x);
           atomicAdd(&L.xh[bl / 4 < 15 ? bl / 4 : 15], 1u);
@@ -1406,56 +1410,6 @@ __global__ __launch_bounds__(kBlock) void k_flag_count2(Batch bt, KV k, u32* par
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_flag_scatter2(Batch bt, KV k, const u32* part,
-                                                          u32* upos, u32* bpos) {
-  __shared__ u32 shc[kWavesPerBlock], shb[kWavesPerBlock];
-  const Tile t = bt.tiles[blockIdx.x];
-  const SegParam& sp = bt.sp[t.seg];
-  const Range r = tile_range(sp.p0, sp.n, t.local);
-  if (!r.count) return;  // uniform
-  const u32 first = r.first - (u32)((u64)t.local * kTile);
-  // Wavefront w owns elements [w*kWaveSub, (w+1)*kWaveSub) of the tile,
-  // visited 64 at a time (coalesced); ballots give the in-order ranks.
-  const u32 w = threadIdx.x / kWave, l = lane_id();
-  const u64 lt = (1ull << l) - 1;
-  const u32 wstart = w * kWaveSub;
-  u64 mc[kItems], mb[kItems];
-  u32 o[kItems], nc = 0, nb = 0;
-  bool fc[kItems], fb[kItems];
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) o[q] = wstart + q * kWave + l;
-  cell_block_flags(k, r, first, !sp.rle, !sp.kp.direct, o, fc, fb);
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    mc[q] = __ballot(fc[q]);
-    mb[q] = __ballot(fb[q]);
-    nc += (u32)__popcll(mc[q]);
-    nb += (u32)__popcll(mb[q]);
-  }
-  if (l == 0) {
-    shc[w] = nc;
-    shb[w] = nb;
-  }
-  __syncthreads();
-  // block ranks: the scan over [cells | blocks] puts every block after all
-  // cells; part[nt] is the cell total
-  u32 rc = part[blockIdx.x], rb = part[bt.ntiles + blockIdx.x] - part[bt.ntiles];
-#pragma unroll
-  for (u32 i = 0; i < (u32)kWavesPerBlock; ++i)
-    if (i < w) {
-      rc += shc[i];
-      rb += shb[i];
-    }
-#pragma unroll
-  for (u32 q = 0; q < kItems; ++q) {
-    const u32 c = rc + (u32)__popcll(mc[q] & lt);
-    if (fc[q]) upos[c] = r.first + o[q];
-    if (fb[q]) bpos[rb + (u32)__popcll(mb[q] & lt)] = c;
-    rc += (u32)__popcll(mc[q]);
-    rb += (u32)__popcll(mb[q]);
-  }
-}
-
 // Per-segment starts and counts of cells (e0, ne) and blocks (b0, nb) from
 // the scan over [cells | blocks] (total: its sum).
 __global__ void k_seg_starts2(Batch bt, const u32* part, const u32* total) {
@@ -1609,31 +1563,65 @@ __global__ __launch_bounds__(kBlock) void k_rle_finish(Batch bt, KV in, const u3
 // in order, so the elements it sees come in increasing position: a thread
 // records a symbol's first occurrence once (an LDS atomicMin) and remembers
 // that in a register bitmap, instead of reading the LDS minimum for every
-// element.  The histograms take one atomic per element on the lane's copy
-// (kRep copies), with no validity test in full tiles.
+// element.  The repos / delta histogram takes one atomic per element on the
+// lane's copy (kRep copies), except for the patch symbol 0, which is counted
+// in the patch count (np) and added once per flush: where most values are
+// patches (one point per block) every lane would hit the same bin.  The
+// quads symbols -- 0, 1..8 or 250..255, quad deltas being differences of
+// 3-bit values -- go to 16 private bins per thread, bin (s + 8) & 15, 17
+// words apart so that the lanes of a wave never share a bank on one bin.
+constexpr u32 kQBins = 16, kQStride = 17;
+
+__device__ __forceinline__ u32 quad_bin(u32 qs) { return (qs + 8) & 15; }
+// The quads symbol of bin b (b = 1 would be 249, which never occurs).
+__device__ __forceinline__ u32 quad_sym(u32 b) { return b == 0 ? 8u : b < 8 ? 248u + b : b - 8; }
+
 struct StatLds {
-  u32 hs[kRep * kRepStride256], hq[kRep * kRepStride256];  // replicated
+  u32 hs[kRep * kRepStride256];  // replicated
+  u32 hq[256];                   // quads histogram, filled at the flush
+  u32 hq16[kBlock * kQStride];   // per-thread quads bins
   u32 fs[256], fq[256];
   u64 bitpop[64];
   u64 red[kWavesPerBlock];
 };
+
+__device__ __forceinline__ void init_stats(StatLds& L) {
+  const u32 d = threadIdx.x;
+  for (u32 i = d; i < kRep * kRepStride256; i += kBlock) L.hs[i] = 0;
+  for (u32 i = d; i < kBlock * kQStride; i += kBlock) L.hq16[i] = 0;
+  L.hq[d] = 0;
+  L.fs[d] = L.fq[d] = 0xFFFFFFFFu;
+  if (d < 64) L.bitpop[d] = 0;
+  __syncthreads();
+}
 
 __device__ __forceinline__ void flush_stats(StatLds& L, SegCtl& sc, SegHist& h, u32& np,
                                             u32& nqp, u64& ps) {
   const u32 a = block_sum(np, reinterpret_cast<u32*>(L.red));
   const u32 b = block_sum(nqp, reinterpret_cast<u32*>(L.red));
   const u64 c = block_sum(ps, L.red);
-  if (threadIdx.x == 0) {
+  const u32 d = threadIdx.x;
+  if (d == 0) {
     if (a) atomicAdd(&sc.n_patches, a);
     if (b) atomicAdd(&sc.n_qpatches, b);
     if (c) atomicAdd(&sc.patch_sum, c);
+    L.hs[0] += a;  // the patch symbol
   }
   np = nqp = 0;
   ps = 0;
-  fold_hist<256, kRepStride256>(L.hq);
+  {  // thread d sums bin d & 15 over rows d / 16 + 16 i
+    const u32 bin = d & (kQBins - 1);
+    u32 sum = 0;
+#pragma unroll
+    for (u32 i = 0; i < kBlock / kQBins; ++i) {
+      u32& x = L.hq16[(d / kQBins + kQBins * i) * kQStride + bin];
+      sum += x;
+      x = 0;
+    }
+    if (sum) atomicAdd(&L.hq[quad_sym(bin)], sum);
+  }
   fold_hist<256, kRepStride256>(L.hs);
   __syncthreads();
-  const u32 d = threadIdx.x;
   if (d < 64) {
     if (L.bitpop[d]) atomicAdd(&h.bitpop[d], L.bitpop[d]);
     L.bitpop[d] = 0;
@@ -1655,7 +1643,8 @@ constexpr u32 kStatGroup = 8;  // elements per thread loaded together
 
 template <bool kFull>
 __device__ __forceinline__ void stats_items(StatLds& L, u64& w0, u64& w1, u64& w2, u64& w3,
-                                            u32& sq, u32 rep, bool direct, u32 lim, u32 count,
+                                            u32& sq, u32 rep, u32 qrow, bool direct, u32 lim,
+                                            u32 count,
                                             u32 rel0, const u32* o, const u64* v, const u32* qd,
                                             u32& np, u32& nqp, u64& ps) {
 #pragma unroll
@@ -1666,15 +1655,15 @@ __device__ __forceinline__ void stats_items(StatLds& L, u64& w0, u64& w1, u64& w
     const bool big = v[it] > lim;
     const u32 sym = big ? 0u : (u32)v[it] + 1u;
     if (!direct) {
-      atomicAdd(&L.hq[rep + qs], 1u);
+      const u32 bin = quad_bin(qs);
+      atomicAdd(&L.hq16[qrow + bin], 1u);
       nqp += qd[it] > 254;
-      const u32 qb = 1u << ((qs + 8) & 15);
-      if (!(sq & qb)) {
-        sq |= qb;
+      if (!(sq >> bin & 1u)) {
+        sq |= 1u << bin;
         atomicMin(&L.fq[qs], rel);
       }
     }
-    atomicAdd(&L.hs[rep + sym], 1u);
+    if (!big) atomicAdd(&L.hs[rep + sym], 1u);
     if (!seen_sym(w0, w1, w2, w3, sym)) {
       const u64 m = 1ull << (sym & 63);
       const u32 h = sym >> 6;
@@ -1696,10 +1685,8 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
   if (t0 >= t1) return;
   const u32 d = threadIdx.x, l = lane_id();
   const u32 rep = (l & (kRep - 1)) * kRepStride256;  // this lane's histogram copy
-  for (u32 i = d; i < kRep * kRepStride256; i += kBlock) L.hs[i] = L.hq[i] = 0;
-  L.fs[d] = L.fq[d] = 0xFFFFFFFFu;
-  if (d < 64) L.bitpop[d] = 0;
-  __syncthreads();
+  const u32 qrow = d * kQStride;                      // this thread's quads bins
+  init_stats(L);
   u32 np = 0, nqp = 0;
   u64 ps = 0;
   u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // symbols recorded (seen_sym)
@@ -1732,10 +1719,10 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
       for (u32 it = 0; it < kStatGroup; ++it) o[it] = (g0 + it) * kBlock + threadIdx.x;
       lcp_items<kStatGroup>(k, direct, sc.e0, r.first, r.count, o, v, qd);
       if (full)  // uniform
-        stats_items<true>(L, w0, w1, w2, w3, sq, rep, direct, lim, r.count, rel0, o, v, qd, np,
+        stats_items<true>(L, w0, w1, w2, w3, sq, rep, qrow, direct, lim, r.count, rel0, o, v, qd, np,
                           nqp, ps);
       else
-        stats_items<false>(L, w0, w1, w2, w3, sq, rep, direct, lim, r.count, rel0, o, v, qd, np,
+        stats_items<false>(L, w0, w1, w2, w3, sq, rep, qrow, direct, lim, r.count, rel0, o, v, qd, np,
                            nqp, ps);
       if (sp.search) {  // bit populations of the values > 254 (mode search only)
 #pragma unroll
@@ -1747,6 +1734,165 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
               const u32 c = (u32)__popcll(__ballot(big && ((v[it] >> bb) & 1ull)));
               if (l == bb) bp += c;
             }
+          }
+        }
+      }
+    }
+    if (bp) atomicAdd(&L.bitpop[l], bp);
+  }
+  flush_stats(L, bt.sc[cur], bt.sh[cur], np, nqp, ps);
+}
+
+// The scatter of the cell and block starts (k_flag_count2 counted them),
+// with the cells' symbol statistics -- what k_stats takes over the cell
+// array -- from the same keys: a cell start's previous point is the last
+// point of the previous cell, so the keys at j and j - 1 give the cell's
+// repos / quads (or direct-mode delta) as lcp() does over the cells, and the
+// cells are not read back.  Persistent workgroups over contiguous tile
+// spans; the statistics flush once per segment a workgroup touches.
+// Three waves per SIMD: the compiler keeps it within 168 registers.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_flag_scatter2(
+    Batch bt, KV k, const u32* part, u32* upos, u32* bpos) {
+  __shared__ u32 shc[kWavesPerBlock], shb[kWavesPerBlock];
+  __shared__ StatLds L;
+  u32 t0, t1;
+  block_tiles(bt.ntiles, t0, t1);
+  if (t0 >= t1) return;
+  const u32 d = threadIdx.x, w = d / kWave, l = lane_id();
+  const u32 rep = (l & (kRep - 1)) * kRepStride256;  // this lane's histogram copy
+  const u32 qrow = d * kQStride;                      // this thread's quads bins
+  init_stats(L);
+  const u64 lt = (1ull << l) - 1;
+  const u32 wstart = w * kWaveSub, nt = bt.ntiles, cells_total = part[nt];
+  const bool wide = k.wide;
+  u32 np = 0, nqp = 0;
+  u64 ps = 0;
+  u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // symbols recorded (seen_sym)
+  u32 sq = 0;
+  u32 cur = bt.tiles[t0].seg;
+  for (u32 ti = t0; ti < t1; ++ti) {
+    const Tile t = bt.tiles[ti];
+    if (t.seg != cur) {
+      flush_stats(L, bt.sc[cur], bt.sh[cur], np, nqp, ps);
+      w0 = w1 = w2 = w3 = 0;
+      sq = 0;
+      cur = t.seg;
+    }
+    const SegParam& sp = bt.sp[t.seg];
+    const Range r = tile_range(sp.p0, sp.n, t.local);
+    if (!r.count) continue;  // uniform
+    const u32 first = r.first - (u32)((u64)t.local * kTile);
+    // segment fields in registers: the stores below may alias them
+    const bool all = !sp.rle, direct = sp.kp.direct != 0, search = sp.search != 0;
+    const u32 lim = sp.lim, e0 = part[sp.tile0];  // the segment's first cell
+    // Wavefront w owns elements [w*kWaveSub, (w+1)*kWaveSub) of the tile,
+    // visited 64 at a time (coalesced); ballots give the in-order ranks.
+    // Per group of kStatGroup elements: the keys at j and j - 1, the flags,
+    // and the cell's lcp() against the previous cell (block_quantizer.hpp:
+    // 128-149) or the direct-mode delta -- a cell start's previous point is
+    // the last point of the previous cell.  Only v and dq cross the rank
+    // barrier.
+    u64 mc[kItems], mb[kItems], v[kItems];
+    u32 o[kItems], dq[kItems], nc = 0, nb = 0;
+#pragma unroll
+    for (u32 g0 = 0; g0 < kItems; g0 += kStatGroup) {
+      u64 a0[kStatGroup], a1[kStatGroup];
+      u32 b0[kStatGroup], b1[kStatGroup];
+#pragma unroll
+      for (u32 i = 0; i < kStatGroup; ++i) {
+        const u32 q = g0 + i;
+        o[q] = wstart + q * kWave + l;
+        const u32 j = r.first + (o[q] < r.count ? o[q] : r.count - 1),
+                  jp = j > first ? j - 1 : j;
+        a0[i] = k.a[j];
+        a1[i] = k.a[jp];
+        if (wide) {
+          b0[i] = k.b[j];
+          b1[i] = k.b[jp];
+        }
+      }
+#pragma unroll
+      for (u32 i = 0; i < kStatGroup; ++i) {
+        const u32 q = g0 + i;
+        const u32 j = r.first + o[q];
+        const bool in = o[q] < r.count, head = j == first;
+        const u64 hcur = wide ? a0[i] : a0[i] >> 21, hprev = wide ? a1[i] : a1[i] >> 21;
+        const bool key_diff = a0[i] != a1[i] || (wide && b0[i] != b1[i]);
+        mc[q] = __ballot(in && (all || head || key_diff));
+        mb[q] = __ballot(in && !direct && (head || hcur != hprev));
+        nc += (u32)__popcll(mc[q]);
+        nb += (u32)__popcll(mb[q]);
+        dq[q] = 0;
+        if (direct) {
+          v[q] = head ? a0[i] : a0[i] - a1[i];
+        } else {
+          const u32 lc = wide ? b0[i] : (u32)(a0[i] & 0x1FFFFFu);
+          const u32 lp = wide ? b1[i] : (u32)(a1[i] & 0x1FFFFFu);
+          const u32 quad = lc >> 18, sfc = lc & 0x3FFFFu, pq = lp >> 18;
+          const bool restart = head || hprev != hcur;
+          dq[q] = restart ? quad : (quad - pq) & 0xFFu;
+          v[q] = !restart && pq == quad ? (u64)sfc - (u64)(lp & 0x3FFFFu) : (u64)sfc;
+        }
+      }
+    }
+    if (l == 0) {
+      shc[w] = nc;
+      shb[w] = nb;
+    }
+    __syncthreads();
+    // block ranks: the scan over [cells | blocks] puts every block after all
+    // cells; part[nt] is the cell total
+    u32 rc = part[ti], rb = part[nt + ti] - cells_total;
+#pragma unroll
+    for (u32 i = 0; i < (u32)kWavesPerBlock; ++i)
+      if (i < w) {
+        rc += shc[i];
+        rb += shb[i];
+      }
+    __syncthreads();  // shc / shb are free for the next tile
+    u64 bp = 0;  // mode search: this lane's share of bitpop (lane b counts bit b)
+#pragma unroll
+    for (u32 q = 0; q < kItems; ++q) {
+      const bool fc = (mc[q] >> l) & 1ull, fb = (mb[q] >> l) & 1ull;
+      const u32 c = rc + (u32)__popcll(mc[q] & lt);
+      if (fc) upos[c] = r.first + o[q];
+      if (fb) bpos[rb + (u32)__popcll(mb[q] & lt)] = c;
+      rc += (u32)__popcll(mc[q]);
+      rb += (u32)__popcll(mb[q]);
+      if (fc) {
+        const u32 rel = c - e0;
+        const u32 qs = dq[q] > 254 ? 0u : dq[q] + 1u;
+        const bool big = v[q] > lim;
+        const u32 sym = big ? 0u : (u32)v[q] + 1u;
+        if (!direct) {
+          const u32 bin = quad_bin(qs);
+          atomicAdd(&L.hq16[qrow + bin], 1u);
+          nqp += dq[q] > 254;
+          if (!(sq >> bin & 1u)) {
+            sq |= 1u << bin;
+            atomicMin(&L.fq[qs], rel);
+          }
+        }
+        if (!big) atomicAdd(&L.hs[rep + sym], 1u);
+        if (!seen_sym(w0, w1, w2, w3, sym)) {
+          const u64 m = 1ull << (sym & 63);
+          const u32 h = sym >> 6;
+          w0 |= h == 0 ? m : 0ull;
+          w1 |= h == 1 ? m : 0ull;
+          w2 |= h == 2 ? m : 0ull;
+          w3 |= h == 3 ? m : 0ull;
+          atomicMin(&L.fs[sym], rel);
+        }
+        np += big;
+        ps += big ? v[q] : 0ull;
+      }
+      if (search) {  // bit populations of the values > 254 (mode search only)
+        const bool hi = fc && v[q] > 254;
+        if (__ballot(hi)) {
+#pragma unroll 8
+          for (u32 bb = 0; bb < 64; ++bb) {
+            const u32 cnt = (u32)__popcll(__ballot(hi && ((v[q] >> bb) & 1ull)));
+            if (l == bb) bp += cnt;
           }
         }
       }
@@ -2663,7 +2809,7 @@ void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u32* bpos, u64* ua
   k_flag_count2<<<bt.ntiles, kBlock, 0, s>>>(bt, k, part);
   LAUNCH_CHECK("k_flag_count2");
   scan_excl<u32>(part, part, 2ull * bt.ntiles, total, ar, s);
-  k_flag_scatter2<<<bt.ntiles, kBlock, 0, s>>>(bt, k, part, upos, bpos);
+  k_flag_scatter2<<<persist_grid(bt), kBlock, 0, s>>>(bt, k, part, upos, bpos);
   LAUNCH_CHECK("k_flag_scatter2");
   k_seg_starts2<<<grid_for(bt.S, kBlock), kBlock, 0, s>>>(bt, part, total);
   LAUNCH_CHECK("k_seg_starts2");
@@ -2673,12 +2819,14 @@ void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u32* bpos, u64* ua
 }
 
 void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, bool block_any,
-                          bool blocks_done, u32* bpos, Arena& ar, hipStream_t s) {
+                          bool rle_done, u32* bpos, Arena& ar, hipStream_t s) {
   (void)direct_any;
   const KV k{kv.a, kv.b, bt.wide};
-  if (block_any && !blocks_done) compact<kFlagBlock>(bt, k, bpos, 1, ar, s);
-  k_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, k);
-  LAUNCH_CHECK("k_stats");
+  if (!rle_done) {
+    if (block_any) compact<kFlagBlock>(bt, k, bpos, 1, ar, s);
+    k_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, k);
+    LAUNCH_CHECK("k_stats");
+  }
   if (block_any) {
     k_block_stats<<<persist_grid(bt), kBlock, 0, s>>>(bt, k, bpos);
     LAUNCH_CHECK("k_block_stats");

@@ -244,14 +244,14 @@ int radix_sort_wide_v(const Batch& bt, u64* hi, u64* hi2, u64* lo64, u64* lo64b,
 
 // RLE: first position of every run of equal keys per segment, then the cells
 // (keys) and their counts, segment by segment, with count histograms.
-// Also the block starts (bpos: each block's first cell), found in the same
-// pass over the sorted points.
+// Also the block starts (bpos: each block's first cell) and the cells' symbol
+// statistics, found in the same pass over the sorted points.
 void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u32* bpos, u64* ua, u32* ub,
                u32* cnt, Arena& ar, hipStream_t s);
-// Block runs over the coded elements (unless rle_cells found them:
-// blocks_done) and all symbol statistics.
+// Block runs over the coded elements and their symbol statistics (unless
+// rle_cells found both: rle_done), then the BitPacker widths.
 void block_runs_and_stats(const Batch& bt, const KeyView& kv, bool direct_any, bool block_any,
-                          bool blocks_done, u32* bpos, Arena& ar, hipStream_t s);
+                          bool rle_done, u32* bpos, Arena& ar, hipStream_t s);
 
 // Phase A of every section: per-tile bit counts -> tile offsets; per-segment
 // section bits and first-tile offsets into SegCtl.
