@@ -17,8 +17,8 @@
 //                                                 encoder/huffman_encoder.hpp:150-170,365-396
 //
 // Floating point: no contraction (this pragma and -ffp-contract=off), IEEE
-// division, roundf = round half away from zero -- the reference's x86-64
-// baseline arithmetic.
+// quotients (exact_div.hpp), roundf = round half away from zero (fround.hpp)
+// -- the reference's x86-64 baseline arithmetic.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -249,11 +249,12 @@ __global__ void k_offsets(Batch bt, const float* xyz) {
 // ties; then the x/y halving.  cube_quantizer.hpp:20-21: floor(s / div).
 
 __device__ __forceinline__ void to_node(float qx, float qy, float qz, int& X, int& Y, int& Z) {
-  const float ex = roundf(qx * 0.5f) * 2.0f, ey = roundf(qy * 0.5f) * 2.0f,
-              ez = roundf(qz * 0.5f) * 2.0f;
-  const float ox = roundf((qx - 1.0f) * 0.5f) * 2.0f + 1.0f;
-  const float oy = roundf((qy - 1.0f) * 0.5f) * 2.0f + 1.0f;
-  const float oz = roundf((qz - 1.0f) * 0.5f) * 2.0f + 1.0f;
+  // round_half_away: roundf in three operations (fround.hpp)
+  const float ex = round_half_away(qx * 0.5f) * 2.0f, ey = round_half_away(qy * 0.5f) * 2.0f,
+              ez = round_half_away(qz * 0.5f) * 2.0f;
+  const float ox = round_half_away((qx - 1.0f) * 0.5f) * 2.0f + 1.0f;
+  const float oy = round_half_away((qy - 1.0f) * 0.5f) * 2.0f + 1.0f;
+  const float oz = round_half_away((qz - 1.0f) * 0.5f) * 2.0f + 1.0f;
   const float a = qx - ex, b = qy - ey, c = qz - ez;
   const float d = qx - ox, e = qy - oy, f = qz - oz;
   const float dE = __fadd_rn(__fmul_rn(a, a), __fadd_rn(__fmul_rn(b, b), __fmul_rn(c, c)));
@@ -466,28 +467,40 @@ struct SMTables {
   u8 apply[24 * 8];
   u8 next[24 * 8];
   u8 out[8];
+  // Two digits at once: two[s * 64 + (d1 << 3 | d2)] = next state after both
+  // (low byte) | the two output digits (h1 << 3 | h2) << 8.
+  uint16_t two[24 * 64];
 };
 
+// Every thread of the block calls this; synchronises before returning.
 __device__ __forceinline__ void load_sm(SMTables& t, const HilbertSM& hs) {
   for (u32 i = threadIdx.x; i < 24 * 8; i += blockDim.x) {
     t.apply[i] = hs.apply[i];
     t.next[i] = hs.next[i];
   }
   if (threadIdx.x < 8) t.out[threadIdx.x] = hs.out[threadIdx.x];
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < 24 * 64; i += blockDim.x) {
+    const u32 s = i >> 6;
+    const u32 m1 = t.apply[s * 8 + ((i >> 3) & 7)], s1 = t.next[s * 8 + m1];
+    const u32 m2 = t.apply[s1 * 8 + (i & 7)], s2 = t.next[s1 * 8 + m2];
+    t.two[i] = (uint16_t)(s2 | (((u32)t.out[m1] << 3 | t.out[m2]) << 8));
+  }
+  __syncthreads();
 }
 
 // mortonToHilbert3(m, 20) for m < 2^18: digits 20..6 are zero, stay zero and
 // leave the state at `init` (composition of 15 copies of the digit-0
-// permutation); digits 5..0 go through the machine.
+// permutation); digits 5..0 go through the machine, two at a time.
 __device__ __forceinline__ u32 hilbert18(u32 m, const SMTables& t, u32 init) {
   u32 s = init, r = 0;
 #pragma unroll
-  for (int i = 5; i >= 1; --i) {
-    const u32 md = t.apply[s * 8 + ((m >> (3 * i)) & 7)];
-    r |= (u32)t.out[md] << (3 * i);
-    s = t.next[s * 8 + md];
+  for (int i = 2; i >= 0; --i) {
+    const u32 e = t.two[s * 64 + ((m >> (6 * i)) & 63)];
+    r |= (e >> 8) << (6 * i);
+    s = e & 0xFFu;
   }
-  return r | t.out[t.apply[s * 8 + (m & 7)]];
+  return r;
 }
 
 // mortonToHilbert3(m, L) for any 64-bit m and L <= 21: digits above L are
@@ -522,7 +535,6 @@ __global__ __launch_bounds__(kBlock) void k_keygen(Batch bt, const F3* __restric
                                                    u64* __restrict__ lo64) {
   __shared__ SMTables t;
   load_sm(t, hs);
-  __syncthreads();
   const Tile tl = bt.tiles[blockIdx.x];
   const SegParam& sp = bt.sp[tl.seg];
   const SegCtl& sc = bt.sc[tl.seg];
@@ -687,7 +699,6 @@ __global__ __launch_bounds__(kBlock) void k_m2h(const u64* in, u32 n, u32 L, Hil
                                                 u64* out) {
   __shared__ SMTables t;
   load_sm(t, hs);
-  __syncthreads();
   for (u32 i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
     out[i] = hilbert_general(in[i], L, t, hs.identity);
 }

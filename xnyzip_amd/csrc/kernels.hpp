@@ -16,6 +16,7 @@
 
 #include "common.hpp"
 #include "exact_div.hpp"
+#include "fround.hpp"
 
 namespace xnyz {
 
