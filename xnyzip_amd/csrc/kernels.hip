@@ -2221,12 +2221,37 @@ struct OrRun {
     put((u32)c.val, c.len > 32 ? 32u : c.len);
   }
   // MSB-first: `lead` zero bits; LSB-first: `lead` one bits; then the value.
+  // LSB-first codes of at most 64 bits in all (Golomb-Rice codes with short
+  // quotients) go in as one piece: up to three words.
   __device__ __forceinline__ void code(const Code& c) {
+    const u32 n = c.lead + c.len;
+    if (!kMsb && n <= 64) {
+      const u64 unary = c.lead >= 64 ? ~0ull : (1ull << c.lead) - 1ull;
+      const u64 x = unary | (c.len ? c.val << c.lead : 0ull);  // len > 0: lead < 64
+      const u32 w = pos >> 5, sh = pos & 31;
+      const u64 y = x << sh;
+      atomicOr(&words[w], (u32)y);
+      atomicOr(&words[w + 1], (u32)(y >> 32));
+      if (sh + n > 64) atomicOr(&words[w + 2], (u32)(x >> (64 - sh)));
+      pos += n;
+      return;
+    }
     if (kMsb) pos += c.lead;
     else ones(c.lead);
     put64(c.val, c.len);
   }
   __device__ __forceinline__ void finish() {}
+};
+
+// MSB-first codes of a run of at most 64 bits, shifted together: each code
+// is `lead` zero bits, then `len` bits of val (a Huffman code longer than 32
+// bits keeps its leading zeros implicit, val < 2^32).
+struct MsbAcc {
+  u64 a = 0;
+  __device__ __forceinline__ void add(const Code& c) {
+    const u32 n = c.lead + c.len;  // the whole run is <= 64 bits
+    a = (n >= 64 ? 0ull : a << n) | c.val;
+  }
 };
 
 // The four cell-space sections (Golomb-Rice patches, Huffman symbols,
@@ -2509,18 +2534,36 @@ __global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
   __syncthreads();
   {
     // Huffman-coded symbols and quads; RLE counts Huffman or Exp-Golomb coded
+    // A thread's run of one section that fits in 64 bits (short codes, the
+    // usual case) is shifted together in a register and ORed in once;
+    // longer runs OR each code at its position.
     OrRun<true> rh(win + woff[1], (u32)((abs0[1] & 31) + ex[1]));
     OrRun<true> rq(win + woff[2], (u32)((abs0[2] & 31) + ex[2]));
     OrRun<true> rc(win + woff[3], (u32)((abs0[3] & 31) + ex[3]));
+    const bool sh = mine[1] <= 64, sq = mine[2] <= 64, sc = mine[3] <= 64;
+    MsbAcc ah, aq, ac;
 #pragma unroll
     for (u32 q = 0; q < kItems; ++q) {
-      if (h) rh.huff(code_huf(q));
-      if (qu) rq.huff(code_quad(q));
+      if (h) {
+        const Code e = code_huf(q);
+        if (sh) ah.add(e);
+        else rh.huff(e);
+      }
+      if (qu) {
+        const Code e = code_quad(q);
+        if (sq) aq.add(e);
+        else rq.huff(e);
+      }
       if (cn) {
-        if (eg) rc.code(code_rle(q));
-        else rc.huff(code_rle(q));
+        const Code e = code_rle(q);
+        if (sc) ac.add(e);
+        else if (eg) rc.code(e);
+        else rc.huff(e);
       }
     }
+    if (h && sh) rh.put64(ah.a, (u32)mine[1]);
+    if (qu && sq) rq.put64(aq.a, (u32)mine[2]);
+    if (cn && sc) rc.put64(ac.a, (u32)mine[3]);
     if (h) rh.finish();
     if (qu) rq.finish();
     if (cn) rc.finish();
