@@ -2055,6 +2055,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   }
   auto arrived = [&](int piece) {
     check_hip(hipStreamWaitEvent(st, ctx->dec_ev[piece], 0), "hipStreamWaitEvent");
+    ctx->copied();  // per-kernel timing: the next interval starts after the wait
   };
   // The host stream (the caller's, or the decompressed frame) must outlive
   // the copies, also when decoding stops early with an error.
