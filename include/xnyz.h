@@ -377,6 +377,19 @@ int xnyz_compress_selected_batch_device_into(xnyz_ctx* ctx, const float* d_xyz,
                                              uint64_t* out_offsets, xnyz_stats* stats,
                                              xnyz_mode* chosen, int32_t* searched);
 
+/* xnyz_compress_selected_batch_device_into returning once the assembled
+ * streams are in device memory and their copy into `out` is queued on the
+ * context's copy stream, as xnyz_compress_batch_device_into_async does (same
+ * rules: XNYZ_FRAME_NONE, pinned `out`, two stream buffers, bytes in `out`
+ * after xnyz_wait).  The modes are chosen (features, predictions, fallback
+ * search) and out_offsets, stats, chosen and searched are final when the
+ * call returns; the next call's selection and kernels run while this call's
+ * output crosses PCIe.  No reference counterpart beyond compress(). */
+int xnyz_compress_selected_batch_device_into_async(
+    xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets, uint32_t count,
+    const xnyz_opts* opts, const float* params, float min_confidence, uint8_t* out, uint64_t cap,
+    uint64_t* out_offsets, xnyz_stats* stats, xnyz_mode* chosen, int32_t* searched);
+
 /* Replaces TruncatedOctahedronQuantizer/CubeQuantizer::quantize after
  * shift_points (source/quantizer/truncated_octahedron_quantizer.hpp:21-55,
  * cube_quantizer.hpp:15-27, source/preprocessor/shifting.hpp:11-36) on

@@ -215,3 +215,42 @@ def test_packaged_selector_streams(X):
         print(f"selected {total} B vs search optimum {best} B "
               f"({100.0 * (total / best - 1):+.3f}%), searched {list(searched)}")
     assert st[-1].is_direct and st[-1].wrapped_points == 1
+
+
+def test_selected_batch_async_matches_sync(X):
+    """The asynchronous caller-memory form (copy queued on the copy stream,
+    bytes in `out` after ctx.wait()): three calls back to back, alternating
+    between two batches so that both of the context's stream buffers carry
+    a copy in flight, give the synchronous form's bytes, offsets and modes;
+    the second batch is the smaller."""
+    from xnyzip_amd import selector
+    params = selector.load_params()
+    cases = _mixed_chunks()
+    pts = torch.from_numpy(np.concatenate([c for c, _ in cases])).cuda()
+    offs = np.concatenate([[0], np.cumsum([len(c) for c, _ in cases])]).tolist()
+    from xnyzip_amd import _lib
+    opts = [dict(o, frame_mode=_lib.FRAME_NONE) for _, o in cases]
+    sub = cases[1:4]
+    pts2 = torch.from_numpy(np.concatenate([c for c, _ in sub])).cuda()
+    offs2 = np.concatenate([[0], np.cumsum([len(c) for c, _ in sub])]).tolist()
+    opts2 = opts[1:4]
+    ctx = X.Context(0)
+    want = []
+    for p, o, op in ((pts, offs, opts), (pts2, offs2, opts2)):
+        buf = np.zeros(32 << 20, np.uint8)
+        ro, _, modes, searched = X.compress_selected_batch_into(
+            p, o, op, buf, params=params, min_confidence=0.5, ctx=ctx)
+        want.append((bytes(buf[:int(ro[-1])]), ro.tolist(), modes, list(searched)))
+    outs = [X.host_buffer(32 << 20) for _ in range(3)]
+    got = []
+    for k, (p, o, op) in enumerate(((pts, offs, opts), (pts2, offs2, opts2), (pts, offs, opts))):
+        outs[k][:] = 0xA5
+        ro, _, modes, searched = X.compress_selected_batch_into(
+            p, o, op, outs[k], params=params, min_confidence=0.5, ctx=ctx, wait=False)
+        got.append((ro.tolist(), modes, list(searched)))
+    ctx.wait()
+    for k in range(3):
+        w = want[k % 2]
+        ro, modes, searched = got[k]
+        assert ro == w[1] and modes == w[2] and searched == w[3], k
+        assert bytes(outs[k][:ro[-1]]) == w[0], k

@@ -391,10 +391,14 @@ def compress_selected_batch(points_device, offsets, opts, params=None, min_confi
 
 
 def compress_selected_batch_into(points_device, offsets, opts, out: np.ndarray, params=None,
-                                 min_confidence=0.0, ctx: Context | None = None):
+                                 min_confidence=0.0, ctx: Context | None = None,
+                                 wait: bool = True):
     """compress_selected_batch writing the outputs back to back into `out`
     (a contiguous uint8 array, reused across calls; layout as
-    compress_batch_into).  Returns (offsets, stats, modes, searched)."""
+    compress_batch_into).  Returns (offsets, stats, modes, searched).
+    wait=False (xnyz_compress_selected_batch_device_into_async: assembled
+    streams into a pinned `out` from host_buffer) returns once the copy is
+    queued; the bytes are in `out` after ctx.wait()."""
     t, count, off, arr = _batch_args(points_device, offsets, opts, _lib.FRAME_ZSTD3)
     if out.dtype != np.uint8 or not out.flags.c_contiguous:
         raise ValueError("out must be a contiguous uint8 array")
@@ -404,7 +408,9 @@ def compress_selected_batch_into(points_device, offsets, opts, out: np.ndarray, 
     stats = (_lib.Stats * count)()
     modes = (_lib.Mode * count)()
     searched = np.zeros(count, np.int32)
-    _lib.check(_lib.lib().xnyz_compress_selected_batch_device_into(
+    fn = (_lib.lib().xnyz_compress_selected_batch_device_into if wait
+          else _lib.lib().xnyz_compress_selected_batch_device_into_async)
+    _lib.check(fn(
         ctx.handle, C.c_void_p(t.data_ptr()), C.c_void_p(off.ctypes.data), count,
         C.cast(arr, C.c_void_p), C.c_void_p(p.ctypes.data) if p is not None else None,
         float(min_confidence), C.c_void_p(out.ctypes.data), out.size,

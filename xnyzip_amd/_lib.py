@@ -116,6 +116,7 @@ EXPORTS = ["xnyz_abi_version", "xnyz_last_error", "xnyz_create", "xnyz_destroy",
            "xnyz_search_batch_device", "xnyz_features_batch_device", "xnyz_mlp_train",
            "xnyz_mlp_predict", "xnyz_compress_selected_batch_device",
            "xnyz_compress_selected_batch_device_into",
+           "xnyz_compress_selected_batch_device_into_async",
            "xnyz_quantize_device",
            "xnyz_morton_to_hilbert", "xnyz_hilbert_to_morton", "xnyz_decompress",
            "xnyz_decompress_device", "xnyz_decompress_to_device", "xnyz_container_write",
@@ -164,6 +165,8 @@ def lib() -> C.CDLL:
         L.xnyz_compress_selected_batch_device_into.argtypes = [P, P, P, C.c_uint32, P, P,
                                                                C.c_float, P, C.c_uint64, P, P,
                                                                P, P]
+        L.xnyz_compress_selected_batch_device_into_async.argtypes = [
+            P, P, P, C.c_uint32, P, P, C.c_float, P, C.c_uint64, P, P, P, P]
         L.xnyz_quantize_device.argtypes = [P, P, C.c_uint64, C.c_int32, C.c_float, P, P]
         L.xnyz_morton_to_hilbert.argtypes = [P, P, C.c_uint64, C.c_uint32, P]
         L.xnyz_decompress.argtypes = [P, C.c_uint64, C.POINTER(Opts), C.POINTER(Buf)]

@@ -2917,6 +2917,25 @@ int xnyz_compress_selected_batch_device_into(xnyz_ctx* ctx, const float* d_xyz,
   });
 }
 
+int xnyz_compress_selected_batch_device_into_async(
+    xnyz_ctx* ctx, const float* d_xyz, const uint64_t* offsets, uint32_t count,
+    const xnyz_opts* opts, const float* params, float min_confidence, uint8_t* out, uint64_t cap,
+    uint64_t* out_offsets, xnyz_stats* stats, xnyz_mode* chosen, int32_t* searched) {
+  return guarded([&] {
+    if (!ctx || !d_xyz || !offsets || !opts || !count || !out_offsets || !out || !cap)
+      throw Error(XNYZ_EINVAL, "null argument or empty batch");
+    std::vector<u64> off = rebase(offsets, count, d_xyz);
+    use_device(ctx);
+    CallOut co{nullptr, nullptr, stats, nullptr};
+    co.dst = out;
+    co.cap = cap;
+    co.dst_off = out_offsets;
+    co.async_out = true;
+    selected_impl(ctx, d_xyz, off, opts, params, min_confidence, co, chosen, searched);
+    return XNYZ_OK;
+  });
+}
+
 int xnyz_quantize_device(xnyz_ctx* ctx, const float* d_xyz, uint64_t n, int32_t quantizer,
                          float l2_bound, int32_t* nodes, float* offset) {
   return guarded([&] {

@@ -1652,6 +1652,20 @@ __device__ __forceinline__ bool seen_sym(u64 w0, u64 w1, u64 w2, u64 w3, u32 s) 
 
 constexpr u32 kStatGroup = 8;  // elements per thread loaded together
 
+// SegHist::bitpop of one wavefront's values: lane b adds how many of the
+// lanes with `hi` (a value > 254) have bit b of v set.  Bits at or above
+// the wave's longest such value are all zero, so only those below it are
+// counted (18 for block-mode repos values instead of 64).
+__device__ __forceinline__ void bitpop_add(bool hi, u64 v, u32 l, u64& bp) {
+  if (!__ballot(hi)) return;  // uniform
+  const u32 bl = (u32)__builtin_amdgcn_readfirstlane(
+      (int)wave_max(hi ? 64u - (u32)__clzll((long long)v) : 0u));
+  for (u32 bb = 0; bb < bl; ++bb) {
+    const u32 c = (u32)__popcll(__ballot(hi && ((v >> bb) & 1ull)));
+    if (l == bb) bp += c;
+  }
+}
+
 template <bool kFull>
 __device__ __forceinline__ void stats_items(StatLds& L, u64& w0, u64& w1, u64& w2, u64& w3,
                                             u32& sq, u32 rep, u32 qrow, bool direct, u32 lim,
@@ -1689,7 +1703,9 @@ __device__ __forceinline__ void stats_items(StatLds& L, u64& w0, u64& w1, u64& w
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
+// Four waves per SIMD: the compiler keeps it within 128 registers.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_stats(
+    Batch bt, KV k) {
   __shared__ StatLds L;
   u32 t0, t1;
   block_tiles(bt.ntiles, t0, t1);
@@ -1737,16 +1753,8 @@ __global__ __launch_bounds__(kBlock) void k_stats(Batch bt, KV k) {
                            nqp, ps);
       if (sp.search) {  // bit populations of the values > 254 (mode search only)
 #pragma unroll
-        for (u32 it = 0; it < kStatGroup; ++it) {
-          const bool big = o[it] < r.count && v[it] > 254;
-          if (__ballot(big)) {
-#pragma unroll 8
-            for (u32 bb = 0; bb < 64; ++bb) {
-              const u32 c = (u32)__popcll(__ballot(big && ((v[it] >> bb) & 1ull)));
-              if (l == bb) bp += c;
-            }
-          }
-        }
+        for (u32 it = 0; it < kStatGroup; ++it)
+          bitpop_add(o[it] < r.count && v[it] > 254, v[it], l, bp);
       }
     }
     if (bp) atomicAdd(&L.bitpop[l], bp);
@@ -1897,16 +1905,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         np += big;
         ps += big ? v[q] : 0ull;
       }
-      if (search) {  // bit populations of the values > 254 (mode search only)
-        const bool hi = fc && v[q] > 254;
-        if (__ballot(hi)) {
-#pragma unroll 8
-          for (u32 bb = 0; bb < 64; ++bb) {
-            const u32 cnt = (u32)__popcll(__ballot(hi && ((v[q] >> bb) & 1ull)));
-            if (l == bb) bp += cnt;
-          }
-        }
-      }
+      if (search)  // bit populations of the values > 254 (mode search only)
+        bitpop_add(fc && v[q] > 254, v[q], l, bp);
     }
     if (bp) atomicAdd(&L.bitpop[l], bp);
   }
