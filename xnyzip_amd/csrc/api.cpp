@@ -375,6 +375,11 @@ struct xnyz_ctx {
   DBuf mP, mWh, mG, mM, mV, mX, mY, mIdx, mScratch, mLoss, mOut, mConf;
   // pinned host mirrors
   HBuf h_tiles, h_sp, h_sc, h_sh, h_lut, h_out, h_feat, h_zout, h_ztab, h_zpos;
+  // pinned staging of the small copies inside a call (counts, large-count
+  // lists, patches, big code tables, selector inputs and outputs): a copy
+  // from or to pageable memory can wait behind another stream's copy in
+  // flight (an earlier call's stream crossing PCIe)
+  HBuf h_small, h_patch, h_big, h_mlp;
   bool prof_on = false;
   EventProfiler prof;
   FrameState* fs = nullptr;  // frame stage (created on first use)
@@ -401,7 +406,7 @@ struct xnyz_ctx {
   }
   std::vector<HBuf*> hbufs() {
     return {&h_tiles, &h_sp, &h_sc, &h_sh, &h_lut, &h_out, &h_feat, &h_zout, &h_ztab, &h_zpos,
-            &h_dflag, &h_gtab};
+            &h_dflag, &h_gtab, &h_small, &h_patch, &h_big, &h_mlp};
   }
 
   // Every buffer's member name, for allocation errors.
@@ -421,6 +426,7 @@ struct xnyz_ctx {
     XNYZ_NAME(na_host); XNYZ_NAME(dstream); XNYZ_NAME(dsym); XNYZ_NAME(dquad);
     XNYZ_NAME(dhead); XNYZ_NAME(dpatch); XNYZ_NAME(dps); XNYZ_NAME(dblk); XNYZ_NAME(dcnt);
     XNYZ_NAME(dpts); XNYZ_NAME(dtab); XNYZ_NAME(darena); XNYZ_NAME(h_dflag);
+    XNYZ_NAME(h_small); XNYZ_NAME(h_patch); XNYZ_NAME(h_big); XNYZ_NAME(h_mlp);
     XNYZ_NAME(pk); XNYZ_NAME(gtab); XNYZ_NAME(aout[0]); XNYZ_NAME(aout[1]); XNYZ_NAME(apk[0]);
     XNYZ_NAME(apk[1]); XNYZ_NAME(agtab[0]); XNYZ_NAME(agtab[1]); XNYZ_NAME(h_gtab);
 #undef XNYZ_NAME
@@ -1153,15 +1159,17 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   if (any_rle) kv = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
   ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
   ctx->d2h(ctx->h_sh, ctx->sh, sizeof(SegHist) * S);
-  u32 n_large = 0;
-  check_hip(hipMemcpyAsync(&n_large, ctx->nlarge.p, 4, hipMemcpyDeviceToHost, st), "D2H");
+  ctx->h_small.ensure(64);
+  check_hip(hipMemcpyAsync(ctx->h_small.p, ctx->nlarge.p, 4, hipMemcpyDeviceToHost, st), "D2H");
   ctx->sync();
+  const u32 n_large = *ctx->h_small.as<u32>();
   if (n_large > large_cap) throw Error(XNYZ_EINTERNAL, "large-count list overflow");
   if (n_large) {
-    std::vector<u32> raw((size_t)n_large * 3);
-    check_hip(hipMemcpyAsync(raw.data(), ctx->large.p, raw.size() * 4, hipMemcpyDeviceToHost, st),
-              "large D2H");
+    ctx->h_small.ensure((size_t)n_large * 12);
+    check_hip(hipMemcpyAsync(ctx->h_small.p, ctx->large.p, (size_t)n_large * 12,
+                             hipMemcpyDeviceToHost, st), "large D2H");
     ctx->sync();
+    const u32* raw = ctx->h_small.as<u32>();
     for (u32 i = 0; i < n_large; ++i) g[raw[3 * i]].large.emplace_back(raw[3 * i + 1], raw[3 * i + 2]);
   }
 
@@ -1328,11 +1336,14 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->bigv.ensure(big_vals.size() * 4 + 16);
   ctx->bigc.ensure(big_codes.size() * sizeof(HufEntry) + 16);
   if (!big_vals.empty()) {
-    check_hip(hipMemcpyAsync(ctx->bigv.p, big_vals.data(), big_vals.size() * 4,
-                             hipMemcpyHostToDevice, st), "big H2D");
-    check_hip(hipMemcpyAsync(ctx->bigc.p, big_codes.data(), big_codes.size() * sizeof(HufEntry),
-                             hipMemcpyHostToDevice, st), "big H2D");
-    ctx->sync();  // pageable sources
+    const size_t vb = big_vals.size() * 4, cb = big_codes.size() * sizeof(HufEntry);
+    const size_t cat = (vb + 15) & ~size_t(15);
+    ctx->h_big.ensure(cat + cb);
+    std::memcpy(ctx->h_big.p, big_vals.data(), vb);
+    std::memcpy(ctx->h_big.as<char>() + cat, big_codes.data(), cb);
+    check_hip(hipMemcpyAsync(ctx->bigv.p, ctx->h_big.p, vb, hipMemcpyHostToDevice, st), "big H2D");
+    check_hip(hipMemcpyAsync(ctx->bigc.p, ctx->h_big.as<char>() + cat, cb, hipMemcpyHostToDevice,
+                             st), "big H2D");
   }
   bt.big_vals = ctx->bigv.as<u32>();
   bt.big_codes = ctx->bigc.as<HufEntry>();
@@ -1424,8 +1435,10 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   launch_fill_u32(ow, (out_total + 64) / 4, 0, st);
   const size_t prec = patches.size() * sizeof(Patch);
   ctx->patch.ensure(prec + pbytes.size() + 16);
-  check_hip(hipMemcpyAsync(ctx->patch.p, patches.data(), prec, hipMemcpyHostToDevice, st), "H2D");
-  check_hip(hipMemcpyAsync(ctx->patch.as<uint8_t>() + prec, pbytes.data(), pbytes.size(),
+  ctx->h_patch.ensure(prec + pbytes.size() + 16);
+  std::memcpy(ctx->h_patch.p, patches.data(), prec);
+  if (!pbytes.empty()) std::memcpy(ctx->h_patch.as<uint8_t>() + prec, pbytes.data(), pbytes.size());
+  check_hip(hipMemcpyAsync(ctx->patch.p, ctx->h_patch.p, prec + pbytes.size(),
                            hipMemcpyHostToDevice, st), "H2D");
   ctx->copied();
   launch_patch(ctx->patch.as<Patch>(), (u32)patches.size(), ctx->patch.as<uint8_t>() + prec,
@@ -2691,25 +2704,41 @@ void upload(DBuf& d, const void* src, size_t bytes, hipStream_t st) {
 void predict_impl(xnyz_ctx* ctx, const float* params, const float* features, u64 n,
                   std::vector<MlpLabel>& out, std::vector<float>& conf) {
   hipStream_t st = ctx->stream;
-  upload(ctx->mP, params, kPTotal * sizeof(float), st);
+  constexpr u64 kChunk = 1 << 16;
+  const u64 m0 = std::min(n, kChunk);
+  // pinned staging (h_mlp): parameters, then one chunk's features, labels
+  // and confidences
+  const size_t pb = kPTotal * sizeof(float), xb = (size_t)m0 * kFeat * sizeof(float);
+  const size_t ob = (size_t)m0 * sizeof(MlpLabel), cb = (size_t)m0 * sizeof(float);
+  const size_t xo = (pb + 255) & ~size_t(255), oo = (xo + xb + 255) & ~size_t(255);
+  const size_t co = (oo + ob + 255) & ~size_t(255);
+  ctx->h_mlp.ensure(co + cb);
+  char* h = ctx->h_mlp.as<char>();
+  std::memcpy(h, params, pb);
+  ctx->mP.ensure(pb + 16);
+  check_hip(hipMemcpyAsync(ctx->mP.p, h, pb, hipMemcpyHostToDevice, st), "H2D");
   ctx->mWh.ensure(kHalfWeights * sizeof(__half) + 16);
   mlp_refresh_half(ctx->mP.as<float>(), ctx->mWh.as<__half>(), st);
   out.resize(n);
   conf.resize(n);
-  constexpr u64 kChunk = 1 << 16;
-  ctx->mScratch.ensure(mlp_scratch_floats((u32)std::min(n, kChunk)) * sizeof(float));
-  ctx->mOut.ensure(std::min(n, kChunk) * sizeof(MlpLabel));
-  ctx->mConf.ensure(std::min(n, kChunk) * sizeof(float));
+  ctx->mScratch.ensure(mlp_scratch_floats((u32)m0) * sizeof(float));
+  ctx->mOut.ensure(ob);
+  ctx->mConf.ensure(cb);
+  ctx->mX.ensure(xb + 16);
   for (u64 a = 0; a < n; a += kChunk) {
     const u32 m = (u32)std::min(kChunk, n - a);
-    upload(ctx->mX, features + a * kFeat, (size_t)m * kFeat * sizeof(float), st);
+    std::memcpy(h + xo, features + a * kFeat, (size_t)m * kFeat * sizeof(float));
+    check_hip(hipMemcpyAsync(ctx->mX.p, h + xo, (size_t)m * kFeat * sizeof(float),
+                             hipMemcpyHostToDevice, st), "H2D");
     mlp_predict(ctx->mX.as<float>(), m, ctx->mP.as<float>(), ctx->mWh.as<__half>(),
                 ctx->mOut.as<MlpLabel>(), ctx->mConf.as<float>(), ctx->mScratch.as<float>(), st);
-    check_hip(hipMemcpyAsync(out.data() + a, ctx->mOut.p, m * sizeof(MlpLabel),
-                             hipMemcpyDeviceToHost, st), "D2H");
-    check_hip(hipMemcpyAsync(conf.data() + a, ctx->mConf.p, m * sizeof(float),
-                             hipMemcpyDeviceToHost, st), "D2H");
+    check_hip(hipMemcpyAsync(h + oo, ctx->mOut.p, m * sizeof(MlpLabel), hipMemcpyDeviceToHost, st),
+              "D2H");
+    check_hip(hipMemcpyAsync(h + co, ctx->mConf.p, m * sizeof(float), hipMemcpyDeviceToHost, st),
+              "D2H");
     check_hip(hipStreamSynchronize(st), "sync");
+    std::memcpy(out.data() + a, h + oo, m * sizeof(MlpLabel));
+    std::memcpy(conf.data() + a, h + co, m * sizeof(float));
   }
 }
 
