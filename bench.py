@@ -84,6 +84,23 @@ def generate(w: dict, n: int | None = None):
     return datasets.uniform(n, seed=0)
 
 
+class _no_gc:
+    """Timed region without Python's cyclic garbage collector: a collection
+    that lands in it can free a pinned host buffer of an earlier phase
+    (hipHostFree of hundreds of MB takes tens of milliseconds).  Collects
+    first, re-enables after."""
+
+    def __enter__(self):
+        import gc
+        gc.collect()
+        gc.disable()
+
+    def __exit__(self, *exc):
+        import gc
+        gc.enable()
+        return False
+
+
 class OutBuffer:
     """Host buffer the outputs are written into, reused across steps (grown
     once if a call reports it too small): pinned memory from the library
@@ -220,12 +237,13 @@ def main() -> int:
         for _ in range(warmup):
             step(mode, out)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        last = None
-        for _ in range(steps):
-            last = step(mode, out)
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0, last
+        with _no_gc():
+            t0 = time.perf_counter()
+            last = None
+            for _ in range(steps):
+                last = step(mode, out)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, last
 
     stream_mode, frame_mode = _lib.FRAME_NONE, _lib.FRAME_ZSTD3_MT
     # Run-time parameters (persistent grid, radix shape, host chunk size, the
@@ -275,12 +293,13 @@ def main() -> int:
             pstep(k)
         ctx.wait()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            res, _ = pstep(k)
-        ctx.wait()
-        torch.cuda.synchronize()
-        dt_p = time.perf_counter() - t0
+        with _no_gc():
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                res, _ = pstep(k)
+            ctx.wait()
+            torch.cuda.synchronize()
+            dt_p = time.perf_counter() - t0
         ok = [verify_stream(args.workload, w, n, bytes(b[:stream_len])) for b in pb]
         pipelined = {"GBps": round(12.0 * n * args.steps / dt_p / 1e9, 3),
                      "ms_per_step": round(dt_p / args.steps * 1e3, 3),
@@ -444,11 +463,12 @@ def decode_stream(X, ctx, args, w: dict, n: int, stream) -> dict:
         m = X.decompress_to_device(stream, q, l2, rle, out, is_frame=False, ctx=ctx)
     torch.cuda.synchronize()
     k = max(3, min(10, args.steps))
-    t0 = time.perf_counter()
-    for _ in range(k):
-        m = X.decompress_to_device(stream, q, l2, rle, out, is_frame=False, ctx=ctx)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / k
+    with _no_gc():
+        t0 = time.perf_counter()
+        for _ in range(k):
+            m = X.decompress_to_device(stream, q, l2, rle, out, is_frame=False, ctx=ctx)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / k
     path = os.path.join(ROOT, "tests", "golden", "workloads.json")
     g = json.load(open(path)).get(args.workload, {}) if os.path.exists(path) else {}
     verified = "unchecked: no oracle decode recorded for this workload"
@@ -690,11 +710,12 @@ def run_batched(args) -> int:
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        out = None
-        for _ in range(steps):
-            out = fn()
-        return time.perf_counter() - t0, out
+        with _no_gc():
+            t0 = time.perf_counter()
+            out = None
+            for _ in range(steps):
+                out = fn()
+            return time.perf_counter() - t0, out
 
     sink = OutBuffer(13 * n, pinned=not args.pageable_out)
     out = {"unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": wu,
@@ -740,11 +761,12 @@ def run_batched(args) -> int:
                     pstep(k)
                 ctx.wait()
                 torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for k in range(args.steps):
-                    pstep(k)
-                ctx.wait()
-                torch.cuda.synchronize()
+                with _no_gc():
+                    t0 = time.perf_counter()
+                    for k in range(args.steps):
+                        pstep(k)
+                    ctx.wait()
+                    torch.cuda.synchronize()
                 dtp = time.perf_counter() - t0
                 out["prezstd_pipelined_GBps"] = round(12.0 * n * args.steps / dtp / 1e9, 3)
                 out["prezstd_pipelined_ms_per_step"] = round(dtp / args.steps * 1e3, 3)
@@ -833,11 +855,12 @@ def run_batched(args) -> int:
                     pstep(k)
                 ctx.wait()
                 torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for k in range(args.steps):
-                    pstep(k)
-                ctx.wait()
-                torch.cuda.synchronize()
+                with _no_gc():
+                    t0 = time.perf_counter()
+                    for k in range(args.steps):
+                        pstep(k)
+                    ctx.wait()
+                    torch.cuda.synchronize()
                 dtp = time.perf_counter() - t0
                 res[f"{label}_prezstd_pipelined_GBps"] = round(12.0 * n * args.steps / dtp / 1e9, 3)
                 same = all(bytes(b[:nbytes]) == want for b in bufs)
