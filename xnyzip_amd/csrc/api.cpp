@@ -442,14 +442,10 @@ struct xnyz_ctx {
   void copied() {  // after enqueueing a copy: the next kernel starts a new interval
     if (g_profiler) prof.reset(stream);
   }
-  void h2d(DBuf& d, const HBuf& h, size_t bytes) {
-    check_hip(hipMemcpyAsync(d.p, h.p, bytes, hipMemcpyHostToDevice, stream), "H2D");
-    copied();
-  }
-  void d2h(HBuf& h, const DBuf& d, size_t bytes) {
-    check_hip(hipMemcpyAsync(h.p, d.p, bytes, hipMemcpyDeviceToHost, stream), "D2H");
-    copied();
-  }
+  // Tables between pinned host memory and the device: a copy kernel (see
+  // launch_copy_words), never queued behind another stream's copy.
+  void h2d(DBuf& d, const HBuf& h, size_t bytes) { launch_copy_words(h.p, d.p, bytes, stream); }
+  void d2h(HBuf& h, const DBuf& d, size_t bytes) { launch_copy_words(d.p, h.p, bytes, stream); }
 };
 
 namespace {
@@ -1160,14 +1156,13 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
   ctx->d2h(ctx->h_sh, ctx->sh, sizeof(SegHist) * S);
   ctx->h_small.ensure(64);
-  check_hip(hipMemcpyAsync(ctx->h_small.p, ctx->nlarge.p, 4, hipMemcpyDeviceToHost, st), "D2H");
+  launch_copy_words(ctx->nlarge.p, ctx->h_small.p, 4, st);
   ctx->sync();
   const u32 n_large = *ctx->h_small.as<u32>();
   if (n_large > large_cap) throw Error(XNYZ_EINTERNAL, "large-count list overflow");
   if (n_large) {
     ctx->h_small.ensure((size_t)n_large * 12);
-    check_hip(hipMemcpyAsync(ctx->h_small.p, ctx->large.p, (size_t)n_large * 12,
-                             hipMemcpyDeviceToHost, st), "large D2H");
+    launch_copy_words(ctx->large.p, ctx->h_small.p, (size_t)n_large * 12, st);
     ctx->sync();
     const u32* raw = ctx->h_small.as<u32>();
     for (u32 i = 0; i < n_large; ++i) g[raw[3 * i]].large.emplace_back(raw[3 * i + 1], raw[3 * i + 2]);
@@ -1341,9 +1336,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
     ctx->h_big.ensure(cat + cb);
     std::memcpy(ctx->h_big.p, big_vals.data(), vb);
     std::memcpy(ctx->h_big.as<char>() + cat, big_codes.data(), cb);
-    check_hip(hipMemcpyAsync(ctx->bigv.p, ctx->h_big.p, vb, hipMemcpyHostToDevice, st), "big H2D");
-    check_hip(hipMemcpyAsync(ctx->bigc.p, ctx->h_big.as<char>() + cat, cb, hipMemcpyHostToDevice,
-                             st), "big H2D");
+    launch_copy_words(ctx->h_big.p, ctx->bigv.p, vb, st);
+    launch_copy_words(ctx->h_big.as<char>() + cat, ctx->bigc.p, cb, st);
   }
   bt.big_vals = ctx->bigv.as<u32>();
   bt.big_codes = ctx->bigc.as<HufEntry>();
@@ -1438,9 +1432,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   ctx->h_patch.ensure(prec + pbytes.size() + 16);
   std::memcpy(ctx->h_patch.p, patches.data(), prec);
   if (!pbytes.empty()) std::memcpy(ctx->h_patch.as<uint8_t>() + prec, pbytes.data(), pbytes.size());
-  check_hip(hipMemcpyAsync(ctx->patch.p, ctx->h_patch.p, prec + pbytes.size(),
-                           hipMemcpyHostToDevice, st), "H2D");
-  ctx->copied();
+  launch_copy_words(ctx->h_patch.p, ctx->patch.p, prec + pbytes.size(), st);
   launch_patch(ctx->patch.as<Patch>(), (u32)patches.size(), ctx->patch.as<uint8_t>() + prec,
                reinterpret_cast<uint8_t*>(ow), st);
   write_sections(bt, kv, cnt, block_starts(cur), plan, ow, any_block, any_direct, st);
@@ -1507,8 +1499,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       for (u32 s = 0; s < S; ++s) gt[s] = GatherSeg{g[s].base, co.dst_off[s], g[s].total};
       tb.ensure(tbytes);
       pkb.ensure(bytes + 64);
-      check_hip(hipMemcpyAsync(tb.p, gt, tbytes, hipMemcpyHostToDevice, st), "H2D");
-      ctx->copied();
+      launch_copy_words(gt, tb.p, tbytes, st);
       launch_gather(reinterpret_cast<const uint8_t*>(ow), tb.as<GatherSeg>(), S, bytes,
                     pkb.as<uint8_t>(), st);
       src = pkb.as<uint8_t>();
@@ -2716,7 +2707,7 @@ void predict_impl(xnyz_ctx* ctx, const float* params, const float* features, u64
   char* h = ctx->h_mlp.as<char>();
   std::memcpy(h, params, pb);
   ctx->mP.ensure(pb + 16);
-  check_hip(hipMemcpyAsync(ctx->mP.p, h, pb, hipMemcpyHostToDevice, st), "H2D");
+  launch_copy_words(h, ctx->mP.p, pb, st);
   ctx->mWh.ensure(kHalfWeights * sizeof(__half) + 16);
   mlp_refresh_half(ctx->mP.as<float>(), ctx->mWh.as<__half>(), st);
   out.resize(n);
@@ -2728,14 +2719,11 @@ void predict_impl(xnyz_ctx* ctx, const float* params, const float* features, u64
   for (u64 a = 0; a < n; a += kChunk) {
     const u32 m = (u32)std::min(kChunk, n - a);
     std::memcpy(h + xo, features + a * kFeat, (size_t)m * kFeat * sizeof(float));
-    check_hip(hipMemcpyAsync(ctx->mX.p, h + xo, (size_t)m * kFeat * sizeof(float),
-                             hipMemcpyHostToDevice, st), "H2D");
+    launch_copy_words(h + xo, ctx->mX.p, (size_t)m * kFeat * sizeof(float), st);
     mlp_predict(ctx->mX.as<float>(), m, ctx->mP.as<float>(), ctx->mWh.as<__half>(),
                 ctx->mOut.as<MlpLabel>(), ctx->mConf.as<float>(), ctx->mScratch.as<float>(), st);
-    check_hip(hipMemcpyAsync(h + oo, ctx->mOut.p, m * sizeof(MlpLabel), hipMemcpyDeviceToHost, st),
-              "D2H");
-    check_hip(hipMemcpyAsync(h + co, ctx->mConf.p, m * sizeof(float), hipMemcpyDeviceToHost, st),
-              "D2H");
+    launch_copy_words(ctx->mOut.p, h + oo, m * sizeof(MlpLabel), st);
+    launch_copy_words(ctx->mConf.p, h + co, m * sizeof(float), st);
     check_hip(hipStreamSynchronize(st), "sync");
     std::memcpy(out.data() + a, h + oo, m * sizeof(MlpLabel));
     std::memcpy(conf.data() + a, h + co, m * sizeof(float));

@@ -156,6 +156,16 @@ __global__ void k_init(Batch bt) {
   if (tid == 0) *bt.n_large = 0;
 }
 
+// Word copy between device memory and pinned host memory (either side):
+// the small tables a call exchanges with the host move through the shader,
+// not the copy engine, so they never queue behind an earlier call's stream
+// copy to the host.
+__global__ void k_copy_words(const u32* __restrict__ src, u32* __restrict__ dst, u64 n) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(u32* p, u64 count, u32 value) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (u64)gridDim.x * blockDim.x)
@@ -2729,6 +2739,14 @@ void launch_gather(const uint8_t* src, const GatherSeg* segs, u32 count, u64 byt
   slices = slices < 1 ? 1 : slices > 1024 ? 1024 : slices;
   k_gather<<<(u32)(count * slices), kBlock, 0, s>>>(src, segs, (u32)slices, dst);
   LAUNCH_CHECK("k_gather");
+}
+
+void launch_copy_words(const void* src, void* dst, u64 bytes, hipStream_t s) {
+  const u64 n = (bytes + 3) / 4;
+  if (!n) return;
+  k_copy_words<<<grid_for(n, kBlock * 4, 2048), kBlock, 0, s>>>(static_cast<const u32*>(src),
+                                                                 static_cast<u32*>(dst), n);
+  LAUNCH_CHECK("k_copy_words");
 }
 
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s) {

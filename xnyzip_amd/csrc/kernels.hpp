@@ -268,6 +268,10 @@ void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u3
                     const SecPlan& p, u32* out_words, bool any_block, bool any_direct,
                     hipStream_t s);
 
+// Copies ceil(bytes / 4) words between device memory and pinned host memory
+// with a kernel (both pointers 4-byte aligned; the last word may read and
+// write up to 3 bytes past `bytes`, inside the buffers' slack).
+void launch_copy_words(const void* src, void* dst, u64 bytes, hipStream_t s);
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s);
 
 // Segment s of a gather: `len` bytes from src + src_off to dst + dst_off.
