@@ -2089,10 +2089,14 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
   const auto ts = put_tab(hs);
   const auto tq = put_tab(hq);
   const auto tc = put_tab(hc);
+  // through pinned memory and the copy kernel: the copy engine is busy with
+  // the stream's upload
   ctx->dtab.ensure(tab.size() + 256);
-  if (!tab.empty())
-    check_hip(hipMemcpyAsync(ctx->dtab.p, tab.data(), tab.size(), hipMemcpyHostToDevice, st),
-              "tables H2D");
+  if (!tab.empty()) {
+    ctx->h_big.ensure(tab.size() + 16);
+    std::memcpy(ctx->h_big.p, tab.data(), tab.size());
+    launch_copy_words(ctx->h_big.p, ctx->dtab.p, tab.size(), st);
+  }
   auto bind = [&](HostCode& h, const std::array<size_t, 3>& t, const DevSec& sec) {
     h.d.base_bit = sec.data_at * 8;
     h.d.bits = sec.data_len * 8;
@@ -2145,7 +2149,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     decode_section<u64>(dS, hc.d, num, c, ws, ar, st);
     u64* total = ar.take<u64>(1);
     scan_excl_u64(c, at2, num, total, ar, st);
-    check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
+    launch_copy_words(total, ws.host_flag, 8, st);
     ctx->sync();
     std::memcpy(&total_pts, ws.host_flag, 8);
     if (total_pts >= (1ull << 40)) throw Error(XNYZ_EUNSUPPORTED, "RLE counts beyond 2^40 points");
@@ -2167,7 +2171,7 @@ int decompress_device_impl(xnyz_ctx* ctx, const uint8_t* in, u64 len, const xnyz
     launch_add_u64(bcnt, tmp, B, st);
     u64* total = ar.take<u64>(1);
     scan_excl_u64(bcnt, bstart, B, total, ar, st);
-    check_hip(hipMemcpyAsync(ws.host_flag, total, 8, hipMemcpyDeviceToHost, st), "D2H");
+    launch_copy_words(total, ws.host_flag, 8, st);
     ctx->sync();
     u64 tot;
     std::memcpy(&tot, ws.host_flag, 8);

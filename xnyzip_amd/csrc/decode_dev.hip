@@ -408,13 +408,13 @@ void decode_section_t(const uint8_t* src, const CodeDesc& c, u64 count, T* out, 
       LAUNCH_CHECK("k_pd_sync");
       cur ^= 1;
     }
-    check_hip(hipMemcpyAsync(w.host_flag, flag, 4, hipMemcpyDeviceToHost, s), "flag D2H");
+    launch_copy_words(flag, w.host_flag, 4, s);  // not the copy engine: uploads may hold it
     check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
     if (g_profiler) g_profiler->mark(nullptr, s);  // the next kernel starts a new interval
     if (!*w.host_flag) break;
   }
   scan_excl_u32(cnt, base, nsub, flag + 1, ar, s);
-  check_hip(hipMemcpyAsync(w.host_flag, flag + 1, 4, hipMemcpyDeviceToHost, s), "total D2H");
+  launch_copy_words(flag + 1, w.host_flag, 4, s);
   check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
   if (g_profiler) g_profiler->mark(nullptr, s);
   if (*w.host_flag < count) throw Error(-6, "entropy-coded section holds fewer values than its count");
