@@ -1780,6 +1780,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // cells are not read back.  Persistent workgroups over contiguous tile
 // spans; the statistics flush once per segment a workgroup touches.
 // Three waves per SIMD: the compiler keeps it within 168 registers.
+// kWide: (u64 hi, u32 lo) keys; the narrow form carries no low words.
+template <bool kWide>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_flag_scatter2(
     Batch bt, KV k, const u32* part, u32* upos, u32* bpos) {
   __shared__ u32 shc[kWavesPerBlock], shb[kWavesPerBlock];
@@ -1793,7 +1795,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   init_stats(L);
   const u64 lt = (1ull << l) - 1;
   const u32 wstart = w * kWaveSub, nt = bt.ntiles, cells_total = part[nt];
-  const bool wide = k.wide;
+  constexpr bool wide = kWide;
   u32 np = 0, nqp = 0;
   u64 ps = 0;
   u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // symbols recorded (seen_sym)
@@ -2881,7 +2883,10 @@ void rle_cells(const Batch& bt, const KeyView& in, u32* upos, u32* bpos, u64* ua
   k_flag_count2<<<bt.ntiles, kBlock, 0, s>>>(bt, k, part);
   LAUNCH_CHECK("k_flag_count2");
   scan_excl<u32>(part, part, 2ull * bt.ntiles, total, ar, s);
-  k_flag_scatter2<<<persist_grid(bt), kBlock, 0, s>>>(bt, k, part, upos, bpos);
+  if (bt.wide)
+    k_flag_scatter2<true><<<persist_grid(bt), kBlock, 0, s>>>(bt, k, part, upos, bpos);
+  else
+    k_flag_scatter2<false><<<persist_grid(bt), kBlock, 0, s>>>(bt, k, part, upos, bpos);
   LAUNCH_CHECK("k_flag_scatter2");
   k_seg_starts2<<<grid_for(bt.S, kBlock), kBlock, 0, s>>>(bt, part, total);
   LAUNCH_CHECK("k_seg_starts2");
