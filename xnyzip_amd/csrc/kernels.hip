@@ -1669,7 +1669,7 @@ constexpr u32 kStatGroup = 8;  // elements per thread loaded together
 __device__ __forceinline__ void bitpop_add(bool hi, u64 v, u32 l, u64& bp) {
   if (!__ballot(hi)) return;  // uniform
   const u32 bl = (u32)__builtin_amdgcn_readfirstlane(
-      (int)wave_max(hi ? 64u - (u32)__clzll((long long)v) : 0u));
+      (int)wave_max(hi ? 64u - (u32)__clzll((long long)(v | 1ull)) : 0u));  // hi: v > 254
   for (u32 bb = 0; bb < bl; ++bb) {
     const u32 c = (u32)__popcll(__ballot(hi && ((v >> bb) & 1ull)));
     if (l == bb) bp += c;
