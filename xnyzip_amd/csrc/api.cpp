@@ -975,8 +975,8 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   // sizing, scan partials
   ctx->arena.ensure((size_t)ntiles * (512 * 4 + kNumSec * 16 + 64) + (4u << 20));
   Arena ar{ctx->arena.as<char>(), ctx->arena.cap, 0};
-  ctx->h2d(ctx->tiles, ctx->h_tiles, sizeof(Tile) * ntiles);
-  ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
+  launch_copy_ranges(CopyRanges{{ctx->h_tiles.p, ctx->h_sp.p}, {ctx->tiles.p, ctx->sp.p},
+                                {sizeof(Tile) * ntiles, sizeof(SegParam) * S}, 2}, st);
 
   Batch bt;
   std::memset(&bt, 0, sizeof(bt));  // padding included: bt is part of a graph key
@@ -1153,10 +1153,10 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   }
   KeyView kv{ctx->keys[cur].as<u64>(), wide ? ctx->lo[cur].as<u32>() : nullptr};
   if (any_rle) kv = KeyView{ctx->cells.as<u64>(), wide ? ctx->cells_lo.as<u32>() : nullptr};
-  ctx->d2h(ctx->h_sc, ctx->sc, sizeof(SegCtl) * S);
-  ctx->d2h(ctx->h_sh, ctx->sh, sizeof(SegHist) * S);
   ctx->h_small.ensure(64);
-  launch_copy_words(ctx->nlarge.p, ctx->h_small.p, 4, st);
+  launch_copy_ranges(CopyRanges{{ctx->sc.p, ctx->sh.p, ctx->nlarge.p},
+                                {ctx->h_sc.p, ctx->h_sh.p, ctx->h_small.p},
+                                {sizeof(SegCtl) * S, sizeof(SegHist) * S, 4}, 3}, st);
   ctx->sync();
   const u32 n_large = *ctx->h_small.as<u32>();
   if (n_large > large_cap) throw Error(XNYZ_EINTERNAL, "large-count list overflow");
@@ -1326,19 +1326,25 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       big_codes.insert(big_codes.end(), h.big_codes.begin(), h.big_codes.end());
     }
   }
-  ctx->h2d(ctx->sp, ctx->h_sp, sizeof(SegParam) * S);
-  ctx->h2d(ctx->lut, ctx->h_lut, sizeof(SegLut) * S);
   ctx->bigv.ensure(big_vals.size() * 4 + 16);
   ctx->bigc.ensure(big_codes.size() * sizeof(HufEntry) + 16);
+  CopyRanges cr{{ctx->h_sp.p, ctx->h_lut.p}, {ctx->sp.p, ctx->lut.p},
+                {sizeof(SegParam) * S, sizeof(SegLut) * S}, 2};
   if (!big_vals.empty()) {
     const size_t vb = big_vals.size() * 4, cb = big_codes.size() * sizeof(HufEntry);
     const size_t cat = (vb + 15) & ~size_t(15);
     ctx->h_big.ensure(cat + cb);
     std::memcpy(ctx->h_big.p, big_vals.data(), vb);
     std::memcpy(ctx->h_big.as<char>() + cat, big_codes.data(), cb);
-    launch_copy_words(ctx->h_big.p, ctx->bigv.p, vb, st);
-    launch_copy_words(ctx->h_big.as<char>() + cat, ctx->bigc.p, cb, st);
+    cr.src[2] = ctx->h_big.p;
+    cr.dst[2] = ctx->bigv.p;
+    cr.bytes[2] = vb;
+    cr.src[3] = ctx->h_big.as<char>() + cat;
+    cr.dst[3] = ctx->bigc.p;
+    cr.bytes[3] = cb;
+    cr.n = 4;
   }
+  launch_copy_ranges(cr, st);
   bt.big_vals = ctx->bigv.as<u32>();
   bt.big_codes = ctx->bigc.as<HufEntry>();
 

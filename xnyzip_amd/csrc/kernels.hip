@@ -2743,6 +2743,27 @@ void launch_gather(const uint8_t* src, const GatherSeg* segs, u32 count, u64 byt
   LAUNCH_CHECK("k_gather");
 }
 
+// Up to four ranges in one launch (blockIdx.y picks the range).
+__global__ void k_copy_ranges(CopyRanges r) {
+  const u32 y = blockIdx.y;
+  const u32* __restrict__ src = static_cast<const u32*>(r.src[y]);
+  u32* __restrict__ dst = static_cast<u32*>(r.dst[y]);
+  const u64 n = (r.bytes[y] + 3) / 4;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (u64)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy_ranges(const CopyRanges& r, hipStream_t s) {
+  if (r.n <= 0) return;
+  u64 most = 0;
+  for (int i = 0; i < r.n; ++i) most = r.bytes[i] > most ? r.bytes[i] : most;
+  const u64 n = (most + 3) / 4;
+  if (!n) return;
+  k_copy_ranges<<<dim3(grid_for(n, kBlock * 4, 2048), (u32)r.n), kBlock, 0, s>>>(r);
+  LAUNCH_CHECK("k_copy_words");
+}
+
 void launch_copy_words(const void* src, void* dst, u64 bytes, hipStream_t s) {
   const u64 n = (bytes + 3) / 4;
   if (!n) return;

@@ -272,6 +272,14 @@ void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u3
 // with a kernel (both pointers 4-byte aligned; the last word may read and
 // write up to 3 bytes past `bytes`, inside the buffers' slack).
 void launch_copy_words(const void* src, void* dst, u64 bytes, hipStream_t s);
+// The same for up to four (src, dst, bytes) ranges in one launch.
+struct CopyRanges {
+  const void* src[4];
+  void* dst[4];
+  u64 bytes[4];
+  int n;
+};
+void launch_copy_ranges(const CopyRanges& r, hipStream_t s);
 void launch_fill_u32(u32* p, u64 count, u32 value, hipStream_t s);
 
 // Segment s of a gather: `len` bytes from src + src_off to dst + dst_off.
