@@ -387,6 +387,7 @@ struct xnyz_ctx {
   std::vector<hipEvent_t> slice_ev;  // copy-back slices (frame overlap)
   hipStream_t copy_stream = nullptr;  // copy-back beside device-built frame blocks
   hipEvent_t ev_pipe = nullptr;       // end of the stream-building work
+  hipEvent_t ev_fixed = nullptr;      // BitPacker sections written (early copy-back)
   hipEvent_t dec_ev[5] = {};          // decoding: upload pieces landed; [4] prior work done
   hipEvent_t ev_sizes = nullptr;      // device-built block sizes in host memory
   xnyz_tuning tuning[XNYZ_SIZE_CLASSES];
@@ -769,6 +770,7 @@ struct SegHost {
   u32 w1 = 0, w2 = 0;
   std::vector<std::vector<uint8_t>> metas;  // per section in stream order
   u64 base = 0, total = 0;                  // stream bytes in the output buffer
+  u64 cells_at = ~0ull;  // first byte (from base) of the cell-space sections' data
 };
 
 // Layout of one stream: section records in stream order.
@@ -1370,6 +1372,7 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       std::vector<uint8_t> m = meta_of(h, r.kind);
       const u64 data_at = at + 16 + m.size();
       if (r.kind >= 0) sp[s].sec_byte[r.kind] = data_at;
+      if (r.kind >= kSecGR && r.data_bytes && data_at - h.base < h.cells_at) h.cells_at = data_at - h.base;
       at = data_at + r.data_bytes;
       h.metas.push_back(std::move(m));
     }
@@ -1441,7 +1444,23 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
   launch_copy_words(ctx->h_patch.p, ctx->patch.p, prec + pbytes.size(), st);
   launch_patch(ctx->patch.as<Patch>(), (u32)patches.size(), ctx->patch.as<uint8_t>() + prec,
                reinterpret_cast<uint8_t*>(ow), st);
-  write_sections(bt, kv, cnt, block_starts(cur), plan, ow, any_block, any_direct, st);
+  // One cloud copied out as it is written: the bytes before the first word
+  // the cell-space writer touches are final once the BitPacker sections are,
+  // so their copy to the host starts on the copy stream while the cell-space
+  // sections are written.
+  u64 early = 0;
+  if (S == 1 && direct_out && !co.noalloc && !co.async_out && any_block && !any_direct &&
+      g[0].cells_at != ~0ull)
+    early = g[0].cells_at & ~u64(3);
+  if (early) {
+    if (!ctx->copy_stream)
+      check_hip(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking),
+                "hipStreamCreate");
+    if (!ctx->ev_fixed)
+      check_hip(hipEventCreateWithFlags(&ctx->ev_fixed, hipEventDisableTiming), "hipEventCreate");
+  }
+  write_sections(bt, kv, cnt, block_starts(cur), plan, ow, any_block, any_direct, st,
+                 early ? ctx->ev_fixed : nullptr);
 
   // Fills co.stats (frame bytes from co.frames / co.dst_off) once the outputs
   // are in place.
@@ -1526,6 +1545,20 @@ int compress_impl(xnyz_ctx* ctx, const float* d_xyz, const std::vector<u64>& off
       check_hip(hipEventRecord(ctx->aev[slot], ctx->copy_stream), "hipEventRecord");
       ctx->abusy[slot] = true;
       ctx->aslot = slot ^ 1;
+      const double t = now_ms();
+      fill_stats(t, t);
+      return XNYZ_OK;
+    }
+    if (early && early <= bytes) {
+      check_hip(hipStreamWaitEvent(ctx->copy_stream, ctx->ev_fixed, 0), "hipStreamWaitEvent");
+      check_hip(hipMemcpyAsync(co.dst, src, early, hipMemcpyDeviceToHost, ctx->copy_stream),
+                "stream D2H");
+      if (bytes > early)
+        check_hip(hipMemcpyAsync(co.dst + early, src + early, bytes - early,
+                                 hipMemcpyDeviceToHost, st), "stream D2H");
+      ctx->copied();
+      ctx->sync();
+      check_hip(hipStreamSynchronize(ctx->copy_stream), "hipStreamSynchronize");
       const double t = now_ms();
       fill_stats(t, t);
       return XNYZ_OK;
@@ -2369,6 +2402,7 @@ int xnyz_destroy(xnyz_ctx* ctx) {
     ctx->stage2.clear();
     for (hipEvent_t e : ctx->slice_ev) (void)hipEventDestroy(e);
     if (ctx->ev_pipe) (void)hipEventDestroy(ctx->ev_pipe);
+    if (ctx->ev_fixed) (void)hipEventDestroy(ctx->ev_fixed);
     if (ctx->ev_sizes) (void)hipEventDestroy(ctx->ev_sizes);
     for (hipEvent_t e : ctx->aev)
       if (e) (void)hipEventDestroy(e);

@@ -2953,11 +2953,12 @@ void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32
 
 void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
                     const SecPlan& p, u32* out_words, bool any_block, bool any_direct,
-                    hipStream_t s) {
+                    hipStream_t s, hipEvent_t fixed_done) {
   const SecCtx x{KV{kv.a, kv.b, bt.wide}, cnt, bpos};
   if (any_block) {
     k_sec_write_fixed<<<bt.ntiles, kBlock, 0, s>>>(bt, x, out_words);
     LAUNCH_CHECK("k_sec_write<blk,cnt>");
+    if (fixed_done) check_hip(hipEventRecord(fixed_done, s), "hipEventRecord");
     if (bt.wide) k_sec_write_cells<false, true><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
     else k_sec_write_cells<false, false><<<bt.ntiles, kBlock, 0, s>>>(bt, x, p.tile_off, out_words);
     LAUNCH_CHECK("k_sec_write<cells>");

@@ -263,10 +263,12 @@ struct SecPlan {
 SecPlan plan_sections(const Batch& bt, Arena& ar);
 void size_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
                    const SecPlan& p, Arena& ar, hipStream_t s);
-// Phase B: write every section at SegParam::sec_byte.
+// Phase B: write every section at SegParam::sec_byte.  fixed_done (nullable)
+// is recorded on s once the BitPacker sections are written, before the
+// cell-space sections.
 void write_sections(const Batch& bt, const KeyView& kv, const u32* cnt, const u32* bpos,
                     const SecPlan& p, u32* out_words, bool any_block, bool any_direct,
-                    hipStream_t s);
+                    hipStream_t s, hipEvent_t fixed_done = nullptr);
 
 // Copies ceil(bytes / 4) words between device memory and pinned host memory
 // with a kernel (both pointers 4-byte aligned; the last word may read and
