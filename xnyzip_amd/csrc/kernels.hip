@@ -2297,13 +2297,14 @@ __device__ __forceinline__ void emit_gr(R& run, F f) {
 }
 
 template <bool kDirect, bool kWide>
-__global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_sec_write_cells(Batch bt, SecCtx x,
                                                             const u64* tile_off, u32* out) {
   static_assert(kSymBins == kBlock && kCntBinsSeg == kBlock, "one table entry per thread");
   using V = typename std::conditional<kDirect, u64, u32>::type;  // block-mode repos < 2^18
   __shared__ u32 win[kCellLds];
   __shared__ u64 sh[4 * kWavesPerBlock];
-  __shared__ HufEntry tsym[kBlock], tquad[kDirect ? 1 : kBlock], tcnt[kBlock];
+  // quads codes by quad_bin: the quads symbols are 0, 1..8 or 250..255
+  __shared__ HufEntry tsym[kBlock], tquad[kDirect ? 1 : kQBins], tcnt[kBlock];
   const Tile t = bt.tiles[blockIdx.x];
   const SegParam& sp = bt.sp[t.seg];
   if ((sp.kp.direct != 0) != kDirect) return;
@@ -2314,7 +2315,7 @@ __global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
   const bool rle = sec_active<kSecRle>(sp, sc);
   const SegLut& lut = bt.lut[t.seg];
   tsym[threadIdx.x] = lut.sym[threadIdx.x];
-  if (!kDirect) tquad[threadIdx.x] = lut.quad[threadIdx.x];
+  if (!kDirect && threadIdx.x < kQBins) tquad[threadIdx.x] = lut.quad[quad_sym(threadIdx.x)];
   if (rle) tcnt[threadIdx.x] = lut.cnt[threadIdx.x];
   const u32 b = threadIdx.x * kItems;  // this thread's first element in the tile
   const bool has = b < r.count;
@@ -2435,7 +2436,7 @@ __global__ __launch_bounds__(kBlock) void k_sec_write_cells(Batch bt, SecCtx x,
   };
   auto code_quad = [&](u32 q) {
     const u32 d = (qd4[q / 4] >> (8 * (q % 4))) & 0xFFu;
-    const HufEntry e = tquad[kDirect ? 0u : (d > 254 ? 0u : d + 1u)];
+    const HufEntry e = tquad[kDirect ? 0u : quad_bin(d > 254 ? 0u : d + 1u)];
     return valid(q) ? Code{e.bits, e.len, 0} : Code{0, 0, 0};
   };
   const bool eg = sp.cnt_eg != 0;
